@@ -1,0 +1,60 @@
+"""In-tree build of the native libraries (gfx950).
+
+    python -m cicy_geom_amd.build          # both libraries
+    libcicy_dea.so  <- csrc/dea_solver.hip csrc/dea_mixed.hip ... (hipcc, gfx950)
+    libcicy_net.so  <- csrc/netgen.cpp (g++, host only)
+
+Every HIP source is compiled with packed-FP32 instruction generation disabled
+(-target-feature -packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_SOURCES = ["dea_solver.hip", "dea_mixed.hip", "dea_batched.hip", "dea_mpnn.hip"]
+HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", f"-I{INCLUDE}", f"-I{CSRC}"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_net(out_dir: str = HERE) -> str:
+    out = os.path.join(out_dir, "libcicy_net.so")
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+          os.path.join(CSRC, "netgen.cpp"), "-o", out])
+    return out
+
+
+def build_dea(out_dir: str = HERE) -> str:
+    out = os.path.join(out_dir, "libcicy_dea.so")
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, *srcs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    build_net()
+    build_dea()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    if "all" in what or "net" in what:
+        build_net()
+    if "all" in what or "dea" in what:
+        build_dea()

@@ -1,0 +1,866 @@
+// dea_solver.hip -- MI355X (gfx950) Direct Electrifying solve: blocked-SSS SpMV, fused
+// Jacobi-PCG, boundary current, and the C-ABI of include/cicy_dea.h.
+//
+// Reference path replaced: Direct_Electrifying::Solve_DEA_equations_CG_SSS and helpers
+// (Direct_Electrifying.cpp:1502-1735) and the boundary current sum of
+// Electrical_analysis.cpp:547-605.  Kernel layout and iteration structure are described
+// in dea_kernels.h.
+#include "dea_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+using namespace cicy;
+
+// ------------------------------------------------------------------------------------
+// kernels
+// ------------------------------------------------------------------------------------
+
+// x += alpha p; r -= alpha q; partial r.z, r.r; the last work-group closes the iteration:
+// convergence test sqrt(r.r) <= tol*||r_0|| (Direct_Electrifying.cpp:1619-1620), then
+// beta = r.z / rz_old (:1627-1632) and the iteration cap (:1559).
+template <int RB>
+__global__ __launch_bounds__(kThreads) void k_update_f64(int n, const double *__restrict__ diag,
+                                                         double *__restrict__ x,
+                                                         double *__restrict__ r,
+                                                         const double *__restrict__ p,
+                                                         const double *__restrict__ q,
+                                                         PcgState *__restrict__ st,
+                                                         double *__restrict__ partials) {
+    if (st->done) return;
+    __shared__ double red[8];
+    __shared__ int flag;
+    const double alpha = st->alpha;
+    const int r0 = blockIdx.x * RB;
+    const int nrows = min(RB, n - r0);
+    double prz = 0.0, prr = 0.0;
+    for (int t = threadIdx.x; t < nrows; t += kThreads) {
+        const int i = r0 + t;
+        x[i] = fma(alpha, p[i], x[i]);
+        const double rn = fma(-alpha, q[i], r[i]);
+        r[i] = rn;
+        const double z = (1.0 / diag[i]) * rn;
+        prz += z * rn;
+        prr += rn * rn;
+    }
+    block_sum2(prz, prr, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = prz;
+        partials[2 * blockIdx.x + 1] = prr;
+    }
+    if (arrive_last(&st->counter, &flag)) {
+        double rz = reduce_partials(partials, gridDim.x, 2, red);
+        double rr = reduce_partials(partials + 1, gridDim.x, 2, red);
+        if (threadIdx.x == 0) {
+            st->counter = 0;
+            st->iters += 1;
+            st->rr = rr;
+            if (rr <= st->stop2) {
+                st->status = CICY_OK;
+                st->done = 1;
+            } else if (!isfinite(rr)) {
+                st->status = CICY_BREAKDOWN;
+                st->done = 1;
+            } else {
+                st->beta = rz / st->rz;
+                st->rz = rz;
+                if (st->iters >= st->max_iter) {
+                    st->status = CICY_MAXITER;
+                    st->done = 1;
+                }
+            }
+        }
+    }
+}
+
+// Start of a solve: rz = r.(M^-1 r), rr = r.r, stopping threshold.  stop_ref2 < 0 means
+// "relative to ||r_0||" (the reference), otherwise it is ||b||^2.
+template <int RB>
+__global__ __launch_bounds__(kThreads) void k_init_f64(int n, const double *__restrict__ diag,
+                                                       const double *__restrict__ r,
+                                                       PcgState *__restrict__ st,
+                                                       double *__restrict__ partials,
+                                                       double tol, double stop_ref2,
+                                                       long long max_iter) {
+    __shared__ double red[8];
+    __shared__ int flag;
+    const int r0 = blockIdx.x * RB;
+    const int nrows = min(RB, n - r0);
+    double prz = 0.0, prr = 0.0;
+    for (int t = threadIdx.x; t < nrows; t += kThreads) {
+        const int i = r0 + t;
+        const double ri = r[i];
+        const double z = (1.0 / diag[i]) * ri;
+        prz += z * ri;
+        prr += ri * ri;
+    }
+    block_sum2(prz, prr, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = prz;
+        partials[2 * blockIdx.x + 1] = prr;
+    }
+    if (arrive_last(&st->counter, &flag)) {
+        double rz = reduce_partials(partials, gridDim.x, 2, red);
+        double rr = reduce_partials(partials + 1, gridDim.x, 2, red);
+        if (threadIdx.x == 0) {
+            st->counter = 0;
+            st->rz = rz;
+            st->rr = rr;
+            st->rr_init = rr;
+            double ref2 = stop_ref2 < 0.0 ? rr : stop_ref2;
+            double thr = tol * sqrt(ref2);
+            st->stop2 = thr * thr;
+            st->alpha = 0.0;
+            st->beta = 0.0;
+            st->pq = 0.0;
+            st->iters = 0;
+            st->max_iter = max_iter;
+            st->status = CICY_MAXITER;
+            st->done = 0;
+            if (rr == 0.0) {  // exact initial guess
+                st->status = CICY_OK;
+                st->done = 1;
+            }
+        }
+    }
+}
+
+// out = sum over rows of (a[i] - b[i])^2 and b^2 (true-residual norms); single pass.
+__global__ __launch_bounds__(kThreads) void k_resid_norms(int n, const double *__restrict__ bvec,
+                                                          const double *__restrict__ ax,
+                                                          double *__restrict__ out2,
+                                                          unsigned int *__restrict__ counter,
+                                                          double *__restrict__ partials) {
+    __shared__ double red[8];
+    __shared__ int flag;
+    double a = 0.0, bb = 0.0;
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        const double d = bvec[i] - ax[i];
+        a += d * d;
+        bb += bvec[i] * bvec[i];
+    }
+    block_sum2(a, bb, red);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = a;
+        partials[2 * blockIdx.x + 1] = bb;
+    }
+    if (arrive_last(counter, &flag)) {
+        double s = reduce_partials(partials, gridDim.x, 2, red);
+        double t = reduce_partials(partials + 1, gridDim.x, 2, red);
+        if (threadIdx.x == 0) {
+            out2[0] = s;
+            out2[1] = t;
+            *counter = 0;
+        }
+    }
+}
+
+// r <- r - ax (Update_residual_vector's final loop, Direct_Electrifying.cpp:1530-1532)
+__global__ __launch_bounds__(kThreads) void k_sub(int n, double *__restrict__ r,
+                                                  const double *__restrict__ ax) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) r[i] = r[i] - ax[i];
+}
+
+// FP64 boundary current: one work-group, fixed strided order, tree reduction.
+__global__ __launch_bounds__(kThreads) void k_boundary_current(
+    long long ne, const long long *__restrict__ nb, const long long *__restrict__ ni,
+    const double *__restrict__ re, const double *__restrict__ v, double *__restrict__ out) {
+    __shared__ double red[8];
+    double s = 0.0;
+    for (long long e = threadIdx.x; e < ne; e += kThreads) s += (v[ni[e]] - v[nb[e]]) / re[e];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// ------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------
+
+namespace cicy {
+
+void set_error(cicy_ctx *ctx, const char *fmt, ...) {
+    if (!ctx) return;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(ctx->err, sizeof(ctx->err), fmt, ap);
+    va_end(ap);
+}
+
+bool hip_ok(cicy_ctx *ctx, hipError_t e, const char *what) {
+    if (e == hipSuccess) return true;
+    set_error(ctx, "%s failed: %s", what, hipGetErrorString(e));
+    return false;
+}
+
+// Validates the reference's SSS arrays and builds the blocked layout.
+bool build_blocked(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                   const int64_t *col_ind, const double *values, const double *diag, int rb,
+                   BlockedHost &h) {
+    if (n < 0 || nnz < 0) {
+        set_error(ctx, "negative size (n=%lld nnz=%lld)", (long long)n, (long long)nnz);
+        return false;
+    }
+    if (n >= (1LL << 31) - 1 || nnz >= (1LL << 31) - 1) {
+        set_error(ctx, "system too large for 32-bit device indices (n=%lld nnz=%lld)",
+                  (long long)n, (long long)nnz);
+        return false;
+    }
+    if (n > 0 && (!row_ptr || !diag || (nnz > 0 && (!col_ind || !values)))) {
+        set_error(ctx, "null array");
+        return false;
+    }
+    if (n > 0 && (row_ptr[0] != 0 || row_ptr[n] != nnz)) {
+        set_error(ctx, "row_ptr must start at 0 and end at nnz (row_ptr[n]=%lld, nnz=%lld)",
+                  (long long)row_ptr[n], (long long)nnz);
+        return false;
+    }
+    h.n = (int)n;
+    h.nnz = (int)nnz;
+    h.rb = rb;
+    h.nblocks = n > 0 ? (int)((n + rb - 1) / rb) : 0;
+    h.rp.assign((size_t)n + 1, 0);
+    h.col.resize((size_t)nnz);
+    h.val.assign(values, values + nnz);
+    h.diag.assign(diag, diag + n);
+    std::vector<int> ucount((size_t)n + 1, 0);
+    for (int64_t r = 0; r < n; r++) {
+        if (row_ptr[r + 1] < row_ptr[r]) {
+            set_error(ctx, "row_ptr decreases at row %lld", (long long)r);
+            return false;
+        }
+        if (!(diag[r] != 0.0) || !std::isfinite(diag[r])) {
+            set_error(ctx, "zero or non-finite diagonal at row %lld (floating node)",
+                      (long long)r);
+            return false;
+        }
+        h.rp[r + 1] = (int)row_ptr[r + 1];
+        for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; j++) {
+            int64_t c = col_ind[j];
+            if (c < 0 || c >= r) {
+                set_error(ctx, "col_ind[%lld]=%lld is not strictly below row %lld", (long long)j,
+                          (long long)c, (long long)r);
+                return false;
+            }
+            h.col[j] = (int)c;
+            if (c / rb != r / rb) ucount[c + 1]++;
+        }
+    }
+    h.urp.assign((size_t)n + 1, 0);
+    for (int64_t i = 0; i < n; i++) h.urp[i + 1] = h.urp[i] + ucount[i + 1];
+    h.nremote = n > 0 ? h.urp[n] : 0;
+    h.ucol.resize((size_t)h.nremote);
+    h.uval.resize((size_t)h.nremote);
+    std::vector<int> fill(h.urp.begin(), h.urp.end() - (n > 0 ? 1 : 0));
+    for (int64_t r = 0; r < n; r++) {
+        for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; j++) {
+            int64_t c = col_ind[j];
+            if (c / rb != r / rb) {
+                int at = fill[c]++;
+                h.ucol[at] = (int)r;
+                h.uval[at] = values[j];
+            }
+        }
+    }
+    return true;
+}
+
+}  // namespace cicy
+
+namespace {
+
+template <typename T>
+bool dmalloc(cicy_ctx *ctx, T **p, size_t count, size_t *total) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (!hip_ok(ctx, hipMalloc((void **)p, bytes), "hipMalloc")) return false;
+    if (total) *total += bytes;
+    return true;
+}
+
+template <typename T>
+bool h2d(cicy_ctx *ctx, T *dst, const T *src, size_t count) {
+    if (count == 0) return true;
+    return hip_ok(ctx, hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice,
+                                      ctx->stream),
+                  "hipMemcpyAsync H2D");
+}
+
+template <typename T>
+bool d2h(cicy_ctx *ctx, T *dst, const T *src, size_t count) {
+    if (count == 0) return true;
+    return hip_ok(ctx, hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost,
+                                      ctx->stream),
+                  "hipMemcpyAsync D2H");
+}
+
+SssDev<double> dev_view64(const cicy_sys *s) {
+    SssDev<double> d;
+    d.n = s->n;
+    d.nblocks = s->nblocks;
+    d.rp = s->rp;
+    d.col = s->col;
+    d.val = s->val;
+    d.diag = s->diag;
+    d.urp = s->urp;
+    d.ucol = s->ucol;
+    d.uval = s->uval;
+    return d;
+}
+
+// Dispatch on the rows-per-block template parameter.
+template <template <int> class F, typename... A>
+bool dispatch_rb(int rb, A &&...args) {
+    switch (rb) {
+        case 128: F<128>::run(args...); return true;
+        case 256: F<256>::run(args...); return true;
+        case 512: F<512>::run(args...); return true;
+        case 1024: F<1024>::run(args...); return true;
+        default: return false;
+    }
+}
+
+template <int RB>
+struct LaunchSpmv64 {
+    static void run(cicy_sys *s, const double *v, double *out) {
+        hipLaunchKernelGGL((k_apply<double, RB, false, false>), dim3(s->nblocks), dim3(kThreads),
+                           0, s->ctx->stream, dev_view64(s), v, nullptr, nullptr, out, nullptr,
+                           nullptr, nullptr);
+    }
+};
+
+template <int RB>
+struct LaunchInit64 {
+    static void run(cicy_sys *s, double tol, double stop_ref2, long long max_iter) {
+        hipLaunchKernelGGL((k_init_f64<RB>), dim3(s->nblocks), dim3(kThreads), 0, s->ctx->stream,
+                           s->n, s->diag, s->r, s->st, s->partials, tol, stop_ref2, max_iter);
+    }
+};
+
+template <int RB>
+struct LaunchIter64 {
+    // one PCG iteration; `parity` selects which p buffer is old/new
+    static void run(cicy_sys *s, int parity) {
+        double *pold = parity ? s->p1 : s->p0;
+        double *pnew = parity ? s->p0 : s->p1;
+        hipLaunchKernelGGL((k_apply<double, RB, true, false>), dim3(s->nblocks), dim3(kThreads), 0,
+                           s->ctx->stream, dev_view64(s), pold, s->r, pnew, s->q, s->st,
+                           s->partials, nullptr);
+        hipLaunchKernelGGL((k_update_f64<RB>), dim3(s->nblocks), dim3(kThreads), 0,
+                           s->ctx->stream, s->n, s->diag, s->x, s->r, pnew, s->q, s->st,
+                           s->partials);
+    }
+};
+
+bool ensure_graph64(cicy_sys *s) {
+    if (s->graph_exec64) return true;
+    cicy_ctx *ctx = s->ctx;
+    hipGraph_t g;
+    if (!hip_ok(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
+                "hipStreamBeginCapture"))
+        return false;
+    for (int it = 0; it < kChunk; it++) dispatch_rb<LaunchIter64>(s->rb, s, it & 1);
+    if (!hip_ok(ctx, hipStreamEndCapture(ctx->stream, &g), "hipStreamEndCapture")) return false;
+    hipError_t e = hipGraphInstantiate(&s->graph_exec64, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    return hip_ok(ctx, e, "hipGraphInstantiate");
+}
+
+}  // namespace
+
+namespace cicy {
+
+// Runs chunks of kChunk iterations until the device flag says done.  The host polls the
+// state copied after each chunk while the next chunk is already queued.
+bool run_chunks(cicy_sys *s, hipGraphExec_t exec) {
+    cicy_ctx *ctx = s->ctx;
+    PcgState *hst = ctx->host_state;  // pinned, two slots
+    const int depth = 2;
+    hipEvent_t ev[2] = {ctx->ev_poll[0], ctx->ev_poll[1]};
+    int slot = 0;
+    for (int i = 0; i < depth; i++) {
+        if (!hip_ok(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch")) return false;
+        if (!hip_ok(ctx, hipMemcpyAsync(&hst[i], s->st, sizeof(PcgState), hipMemcpyDeviceToHost,
+                                        ctx->stream), "poll copy"))
+            return false;
+        if (!hip_ok(ctx, hipEventRecord(ev[i], ctx->stream), "hipEventRecord")) return false;
+    }
+    // host-side cap on the wall time of one solve; every kernel is finite, so stopping
+    // the launches ends the solve (status stays CICY_MAXITER)
+    double max_s = 600.0;
+    if (const char *e = getenv("CICY_MAX_SOLVE_SECONDS")) max_s = atof(e);
+    const auto t_start = std::chrono::steady_clock::now();
+    for (;;) {
+        if (!hip_ok(ctx, hipEventSynchronize(ev[slot]), "hipEventSynchronize")) return false;
+        if (hst[slot].done) break;
+        if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() >
+            max_s) {
+            set_error(ctx, "solve stopped after %.0f s (CICY_MAX_SOLVE_SECONDS)", max_s);
+            break;
+        }
+        if (!hip_ok(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch")) return false;
+        if (!hip_ok(ctx, hipMemcpyAsync(&hst[slot], s->st, sizeof(PcgState),
+                                        hipMemcpyDeviceToHost, ctx->stream), "poll copy"))
+            return false;
+        if (!hip_ok(ctx, hipEventRecord(ev[slot], ctx->stream), "hipEventRecord")) return false;
+        slot ^= 1;
+    }
+    return hip_ok(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel) {
+    cicy_ctx *ctx = s->ctx;
+    if (!s->has_b) {
+        *rel = -1.0;
+        return true;
+    }
+    dispatch_rb<LaunchSpmv64>(s->rb, s, xdev, s->q);
+    int nb = std::min(s->nblocks > 0 ? s->nblocks : 1, 1024);
+    hipLaunchKernelGGL(k_resid_norms, dim3(nb), dim3(kThreads), 0, ctx->stream, s->n, s->b, s->q,
+                       s->scal, s->counter2, s->partials);
+    double h[2];
+    if (!hip_ok(ctx, hipMemcpyAsync(h, s->scal, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream), "D2H norms") ||
+        !hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync"))
+        return false;
+    *rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+    return true;
+}
+
+bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
+                 cicy_solve_stats *stats) {
+    cicy_ctx *ctx = s->ctx;
+    if (s->n == 0) {
+        if (stats) {
+            memset(stats, 0, sizeof(*stats));
+            stats->status = CICY_OK;
+        }
+        return true;
+    }
+    if (rel_to == CICY_REL_B && !s->has_b) {
+        set_error(ctx, "CICY_REL_B needs b (cicy_sys_set_rhs)");
+        return false;
+    }
+    if (!ensure_graph64(s)) return false;
+    size_t vb = (size_t)s->n * sizeof(double);
+    hipStream_t st = ctx->stream;
+    if (!hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "hipEventRecord")) return false;
+    // fresh start from the stored r0, x0; both p buffers zero so that the first
+    // direction is p = M^-1 r (beta = 0)
+    if (!hip_ok(ctx, hipMemcpyAsync(s->r, s->r0, vb, hipMemcpyDeviceToDevice, st), "copy r0") ||
+        !hip_ok(ctx, hipMemcpyAsync(s->x, s->x0, vb, hipMemcpyDeviceToDevice, st), "copy x0") ||
+        !hip_ok(ctx, hipMemsetAsync(s->p0, 0, vb, st), "memset p0") ||
+        !hip_ok(ctx, hipMemsetAsync(s->p1, 0, vb, st), "memset p1") ||
+        !hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), st), "memset state"))
+        return false;
+    double stop_ref2 = -1.0;
+    if (rel_to == CICY_REL_B) stop_ref2 = s->bnorm2;
+    dispatch_rb<LaunchInit64>(s->rb, s, tol, stop_ref2, max_iter);
+    if (!run_chunks(s, s->graph_exec64)) return false;
+    if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "hipEventRecord") ||
+        !hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "hipEventSynchronize"))
+        return false;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    PcgState h;
+    if (!hip_ok(ctx, hipMemcpy(&h, s->st, sizeof(PcgState), hipMemcpyDeviceToHost), "D2H state"))
+        return false;
+    s->last_parity_x_valid = true;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->iters = h.iters;
+        stats->status = h.status;
+        stats->rel_res = h.rr_init > 0 ? std::sqrt(h.rr / h.rr_init) : 0.0;
+        stats->solve_ms = ms;
+        double tr = -1.0;
+        if (!sys_true_residual(s, s->x, &tr)) return false;
+        stats->true_rel_res = tr;
+    }
+    return true;
+}
+
+}  // namespace cicy
+
+// ------------------------------------------------------------------------------------
+// C-ABI
+// ------------------------------------------------------------------------------------
+
+extern "C" {
+
+int cicy_create(cicy_ctx **out, int device) {
+    if (!out) return 0;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return 0;
+    cicy_ctx *ctx = new cicy_ctx();
+    ctx->device = device;
+    ctx->rows_per_block = 256;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_poll[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_poll[1], hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->host_state, 2 * sizeof(PcgState), hipHostMallocDefault) !=
+            hipSuccess) {
+        delete ctx;
+        return 0;
+    }
+    *out = ctx;
+    return 1;
+}
+
+void cicy_destroy(cicy_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->host_state) (void)hipHostFree(ctx->host_state);
+    (void)hipEventDestroy(ctx->ev_start);
+    (void)hipEventDestroy(ctx->ev_stop);
+    (void)hipEventDestroy(ctx->ev_poll[0]);
+    (void)hipEventDestroy(ctx->ev_poll[1]);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *cicy_last_error(const cicy_ctx *ctx) { return ctx ? ctx->err : "null context"; }
+
+int cicy_set_rows_per_block(cicy_ctx *ctx, int rb) {
+    if (!ctx) return 0;
+    if (rb == 0) rb = 256;
+    if (rb != 128 && rb != 256 && rb != 512 && rb != 1024) {
+        set_error(ctx, "rows_per_block must be 128, 256, 512 or 1024");
+        return 0;
+    }
+    ctx->rows_per_block = rb;
+    return 1;
+}
+
+int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                    const int64_t *col_ind, const double *values, const double *diagonal,
+                    cicy_sys **out) {
+    if (!ctx || !out) return 0;
+    *out = nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return 0;
+    BlockedHost h;
+    if (!build_blocked(ctx, n, nnz, row_ptr, col_ind, values, diagonal, ctx->rows_per_block, h))
+        return 0;
+    cicy_sys *s = new cicy_sys();
+    s->ctx = ctx;
+    s->n = h.n;
+    s->nnz = h.nnz;
+    s->rb = h.rb;
+    s->nblocks = h.nblocks;
+    s->nremote = h.nremote;
+    size_t tot = 0;
+    const size_t nv = (size_t)h.n;
+    const size_t nparts = 2 * (size_t)std::max(h.nblocks, 1024);
+    bool ok = dmalloc(ctx, &s->rp, nv + 1, &tot) && dmalloc(ctx, &s->col, (size_t)h.nnz, &tot) &&
+              dmalloc(ctx, &s->val, (size_t)h.nnz, &tot) && dmalloc(ctx, &s->diag, nv, &tot) &&
+              dmalloc(ctx, &s->urp, nv + 1, &tot) &&
+              dmalloc(ctx, &s->ucol, (size_t)h.nremote, &tot) &&
+              dmalloc(ctx, &s->uval, (size_t)h.nremote, &tot) && dmalloc(ctx, &s->x, nv, &tot) &&
+              dmalloc(ctx, &s->r, nv, &tot) && dmalloc(ctx, &s->p0, nv, &tot) &&
+              dmalloc(ctx, &s->p1, nv, &tot) && dmalloc(ctx, &s->q, nv, &tot) &&
+              dmalloc(ctx, &s->x0, nv, &tot) && dmalloc(ctx, &s->r0, nv, &tot) &&
+              dmalloc(ctx, &s->b, nv, &tot) && dmalloc(ctx, &s->partials, nparts, &tot) &&
+              dmalloc(ctx, &s->st, 1, &tot) && dmalloc(ctx, &s->scal, 4, &tot) &&
+              dmalloc(ctx, &s->counter2, 1, &tot);
+    ok = ok && h2d(ctx, s->rp, h.rp.data(), nv + 1) && h2d(ctx, s->col, h.col.data(), h.nnz) &&
+         h2d(ctx, s->val, h.val.data(), h.nnz) && h2d(ctx, s->diag, h.diag.data(), nv) &&
+         h2d(ctx, s->urp, h.urp.data(), nv + 1) && h2d(ctx, s->ucol, h.ucol.data(), h.nremote) &&
+         h2d(ctx, s->uval, h.uval.data(), h.nremote) &&
+         hip_ok(ctx, hipMemsetAsync(s->x0, 0, nv * sizeof(double), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(s->p0, 0, nv * sizeof(double), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(s->r0, 0, nv * sizeof(double), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(s->counter2, 0, sizeof(unsigned), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), ctx->stream), "memset") &&
+         hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    s->device_bytes = tot;
+    if (!ok) {
+        cicy_sys_destroy(s);
+        return 0;
+    }
+    if (!cicy::mixed_setup(s, h)) {
+        cicy_sys_destroy(s);
+        return 0;
+    }
+    *out = s;
+    return 1;
+}
+
+void cicy_sys_destroy(cicy_sys *s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->graph_exec64) (void)hipGraphExecDestroy(s->graph_exec64);
+    cicy::mixed_free(s);
+    void *ptrs[] = {s->rp, s->col, s->val, s->diag, s->urp, s->ucol, s->uval, s->x, s->r,
+                    s->p0, s->p1, s->q, s->x0, s->r0, s->b, s->partials, s->st, s->scal,
+                    s->counter2};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+int64_t cicy_sys_size(const cicy_sys *s, int what) {
+    if (!s) return -1;
+    switch (what) {
+        case 0: return s->n;
+        case 1: return s->nnz;
+        case 2: return s->nremote;
+        case 3: return (int64_t)s->device_bytes;
+        default: return -1;
+    }
+}
+
+int cicy_sys_set_rhs(cicy_sys *s, const double *r0, const double *x0, const double *b) {
+    if (!s || !r0) return 0;
+    cicy_ctx *ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    size_t nv = (size_t)s->n;
+    bool ok = h2d(ctx, s->r0, r0, nv);
+    if (x0) ok = ok && h2d(ctx, s->x0, x0, nv);
+    else ok = ok && hip_ok(ctx, hipMemsetAsync(s->x0, 0, nv * sizeof(double), ctx->stream), "memset");
+    s->has_b = b != nullptr;
+    if (b) {
+        ok = ok && h2d(ctx, s->b, b, nv);
+        double nb2 = 0.0;
+        for (size_t i = 0; i < nv; i++) nb2 += b[i] * b[i];
+        s->bnorm2 = nb2;
+    }
+    ok = ok && hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return ok ? 1 : 0;
+}
+
+int cicy_sys_solve(cicy_sys *s, int method, double tol, int64_t max_iter, int rel_to,
+                   cicy_solve_stats *stats) {
+    if (!s) return 0;
+    (void)hipSetDevice(s->ctx->device);
+    if (max_iter <= 0) max_iter = 300LL * ((long long)s->n + 2);
+    if (!(tol >= 0.0)) {
+        set_error(s->ctx, "tolerance must be >= 0");
+        return 0;
+    }
+    bool ok;
+    if (method == CICY_METHOD_PCG_F64)
+        ok = cicy::solve_pcg64(s, tol, max_iter, rel_to, stats);
+    else if (method == CICY_METHOD_MIXED)
+        ok = cicy::solve_mixed(s, tol, max_iter, rel_to, stats);
+    else {
+        set_error(s->ctx, "unknown method %d", method);
+        ok = false;
+    }
+    return ok ? 1 : 0;
+}
+
+int cicy_sys_get(cicy_sys *s, double *x, double *r) {
+    if (!s) return 0;
+    cicy_ctx *ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    size_t nv = (size_t)s->n;
+    bool ok = true;
+    if (x) ok = ok && d2h(ctx, x, s->x, nv);
+    if (r) ok = ok && d2h(ctx, r, s->r, nv);
+    ok = ok && hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return ok ? 1 : 0;
+}
+
+int cicy_sys_spmv_bench(cicy_sys *s, int precision, int reps, double *avg_ms) {
+    if (!s || reps <= 0 || !avg_ms) return 0;
+    cicy_ctx *ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    if (s->n == 0) {
+        *avg_ms = 0.0;
+        return 1;
+    }
+    if (precision != 64 && precision != 32) {
+        set_error(ctx, "precision must be 64 or 32");
+        return 0;
+    }
+    // warm the caches once, then time `reps` launches between two events
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1 && !hip_ok(ctx, hipEventRecord(ctx->ev_start, ctx->stream), "event")) return 0;
+        int nrep = pass == 0 ? 1 : reps;
+        for (int i = 0; i < nrep; i++) {
+            if (precision == 64) dispatch_rb<LaunchSpmv64>(s->rb, s, s->p0, s->q);
+            else cicy::mixed_spmv32(s);
+        }
+    }
+    if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, ctx->stream), "event") ||
+        !hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "sync"))
+        return 0;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    *avg_ms = ms / reps;
+    return hip_ok(ctx, hipGetLastError(), "kernel launch") ? 1 : 0;
+}
+
+int cicy_spmv_sss_f64(cicy_ctx *ctx, int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                      const double *values, const double *diagonal, int64_t nnz, const double *v,
+                      double *out) {
+    if (!ctx || (n > 0 && (!v || !out))) return 0;
+    cicy_sys *s = nullptr;
+    if (!cicy_sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, &s)) return 0;
+    bool ok = true;
+    if (n > 0) {
+        ok = h2d(ctx, s->p0, v, (size_t)n);
+        dispatch_rb<LaunchSpmv64>(s->rb, s, s->p0, s->q);
+        ok = ok && hip_ok(ctx, hipGetLastError(), "k_apply launch") &&
+             d2h(ctx, out, s->q, (size_t)n) &&
+             hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    }
+    cicy_sys_destroy(s);
+    return ok ? 1 : 0;
+}
+
+int cicy_update_residual(cicy_ctx *ctx, int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                         const double *values, const double *diagonal, int64_t nnz,
+                         const double *x0, double *R) {
+    if (!ctx || (n > 0 && (!x0 || !R))) return 0;
+    cicy_sys *s = nullptr;
+    if (!cicy_sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, &s)) return 0;
+    bool ok = true;
+    if (n > 0) {
+        ok = h2d(ctx, s->p0, x0, (size_t)n) && h2d(ctx, s->r, R, (size_t)n);
+        dispatch_rb<LaunchSpmv64>(s->rb, s, s->p0, s->q);
+        hipLaunchKernelGGL(k_sub, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                           ctx->stream, (int)n, s->r, s->q);
+        ok = ok && hip_ok(ctx, hipGetLastError(), "launch") && d2h(ctx, R, s->r, (size_t)n) &&
+             hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    }
+    cicy_sys_destroy(s);
+    return ok ? 1 : 0;
+}
+
+int cicy_solve_dea_cg_sss(cicy_ctx *ctx, int64_t nodes, int64_t reserved_nodes, double tolerance,
+                          const int64_t *col_ind, const int64_t *row_ptr, const double *values,
+                          const double *diagonal, int64_t nnz, double *R, const double *VEF,
+                          double *voltages_sol, double *voltages, int64_t max_iter,
+                          cicy_solve_stats *stats) {
+    if (!ctx) return 0;
+    auto t0 = std::chrono::steady_clock::now();
+    const int64_t n = nodes - reserved_nodes;
+    if (n < 0 || reserved_nodes < 0) {
+        set_error(ctx, "invalid node counts (nodes=%lld reserved=%lld)", (long long)nodes,
+                  (long long)reserved_nodes);
+        if (stats) stats->status = CICY_BAD_INPUT;
+        return 0;
+    }
+    if ((n > 0 && (!R || !voltages_sol)) || (reserved_nodes > 0 && !VEF)) {
+        set_error(ctx, "null array");
+        if (stats) stats->status = CICY_BAD_INPUT;
+        return 0;
+    }
+    if (max_iter <= 0) max_iter = 300 * nodes;  // Direct_Electrifying.cpp:1559
+    cicy_solve_stats local;
+    memset(&local, 0, sizeof(local));
+    local.true_rel_res = -1.0;
+    if (n > 0) {
+        cicy_sys *s = nullptr;
+        if (!cicy_sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, &s)) {
+            if (stats) stats->status = CICY_BAD_INPUT;
+            return 0;
+        }
+        bool ok = cicy_sys_set_rhs(s, R, voltages_sol, nullptr) &&
+                  cicy::solve_pcg64(s, tolerance, max_iter, CICY_REL_R0, &local) &&
+                  cicy_sys_get(s, voltages_sol, R);
+        cicy_sys_destroy(s);
+        if (!ok) {
+            if (stats) {
+                *stats = local;
+                stats->status = CICY_DEVICE_ERROR;
+            }
+            return 0;
+        }
+    } else {
+        local.status = CICY_OK;
+    }
+    if (voltages) {
+        for (int64_t i = 0; i < reserved_nodes; i++) voltages[i] = VEF[i];
+        for (int64_t i = reserved_nodes; i < nodes; i++) voltages[i] = voltages_sol[i - reserved_nodes];
+    }
+    local.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = local;
+    return 1;
+}
+
+int cicy_boundary_current_f64(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
+                              const int64_t *n_int, const double *re, const double *voltages,
+                              int64_t nv, double *current) {
+    if (!ctx || !current || ne < 0 || nv < 0) return 0;
+    if (ne > 0 && (!n_bnd || !n_int || !re || !voltages)) {
+        set_error(ctx, "null array");
+        return 0;
+    }
+    for (int64_t e = 0; e < ne; e++) {
+        if (n_bnd[e] < 0 || n_bnd[e] >= nv || n_int[e] < 0 || n_int[e] >= nv) {
+            set_error(ctx, "boundary element %lld has a node outside [0, %lld)", (long long)e,
+                      (long long)nv);
+            return 0;
+        }
+    }
+    if (ne == 0) {
+        *current = 0.0;
+        return 1;
+    }
+    (void)hipSetDevice(ctx->device);
+    long long *dnb = nullptr, *dni = nullptr;
+    double *dre = nullptr, *dv = nullptr, *dout = nullptr;
+    bool ok = dmalloc(ctx, &dnb, (size_t)ne, nullptr) && dmalloc(ctx, &dni, (size_t)ne, nullptr) &&
+              dmalloc(ctx, &dre, (size_t)ne, nullptr) && dmalloc(ctx, &dv, (size_t)nv, nullptr) &&
+              dmalloc(ctx, &dout, 1, nullptr);
+    ok = ok && h2d(ctx, dnb, (const long long *)n_bnd, (size_t)ne) &&
+         h2d(ctx, dni, (const long long *)n_int, (size_t)ne) && h2d(ctx, dre, re, (size_t)ne) &&
+         h2d(ctx, dv, voltages, (size_t)nv);
+    if (ok) {
+        hipLaunchKernelGGL(k_boundary_current, dim3(1), dim3(kThreads), 0, ctx->stream,
+                           (long long)ne, dnb, dni, dre, dv, dout);
+        ok = hip_ok(ctx, hipGetLastError(), "launch") && d2h(ctx, current, dout, 1) &&
+             hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    }
+    void *ptrs[] = {dnb, dni, dre, dv, dout};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    return ok ? 1 : 0;
+}
+
+int cicy_host_blocked_spmv(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                           const double *values, const double *diagonal, int64_t nnz, int rb,
+                           const double *v, double *out) {
+    BlockedHost h;
+    cicy_ctx dummy;
+    if (rb != 128 && rb != 256 && rb != 512 && rb != 1024) return 0;
+    if (!build_blocked(&dummy, n, nnz, row_ptr, col_ind, values, diagonal, rb, h)) return 0;
+    // the device algorithm, one row block at a time
+    std::vector<double> sy;
+    for (int b = 0; b < h.nblocks; b++) {
+        int r0 = b * rb, nr = std::min(rb, h.n - r0);
+        sy.assign((size_t)nr, 0.0);
+        for (int t = 0; t < nr; t++) sy[t] = h.diag[r0 + t] * v[r0 + t];
+        for (int t = 0; t < nr; t++) {
+            int row = r0 + t;
+            double acc = 0.0;
+            for (int j = h.rp[row]; j < h.rp[row + 1]; j++) {
+                int c = h.col[j];
+                if (c >= r0) {
+                    acc += h.val[j] * v[c];
+                    sy[c - r0] += h.val[j] * v[row];
+                } else {
+                    acc += h.val[j] * v[c];
+                }
+            }
+            for (int j = h.urp[row]; j < h.urp[row + 1]; j++) acc += h.uval[j] * v[h.ucol[j]];
+            sy[t] += acc;
+        }
+        for (int t = 0; t < nr; t++) out[r0 + t] = sy[t];
+    }
+    return 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// netgen.h -- synthetic CNT networks and the DEA system assembly that feeds the solver.
+//
+// Host-side C++ (no GPU code).  It produces the same inputs the reference's
+// Direct_Electrifying::Compute_voltage_field builds from Hoshen_Kopelman and Cutoff_Wins
+// output (Direct_Electrifying.cpp:11-113): the SSS lower triangle, the diagonal, the
+// right-hand side b = -K_EF^T VEF, the prescribed voltages VEF, the point->node map
+// (LMM_cnts) and the ramp warm start, plus the boundary element lists that
+// Electrical_analysis::Calculate_parallel_resistor sums currents over.
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cicy_net_params {
+    double box;            // cube side of the RVE, micrometres; the RVE is [0,box]^3
+    int64_t n_cnt;         // number of straight tubes
+    double radius;         // tube radius (um); 0.005 for 10 nm diameter
+    double len_min;        // tube length ~ U[len_min, len_max] (um)
+    double len_max;
+    double seg_len;        // polyline segment length (um), 0.02
+    double d_tunnel;       // tunnelling cutoff (um), 0.0014
+    double d_vdw;          // van der Waals clamp (um), 0.00034
+    uint64_t seed;         // mt19937_64 seed
+    int per_point_nodes;   // 1: every polyline point is an element point (node);
+                           // 0: element points are tube ends + junction points (reference rule)
+} cicy_net_params;
+
+typedef struct cicy_elec_params {
+    double applied_voltage;    // V_app
+    double resistivity_cnt;    // Ohm.um
+    int junction_exponential;  // 1: C1*d*exp(C2*d)/A; 0: constant junction_resistance
+    double junction_resistance;
+    double c1, c2;             // tunnelling constants
+} cicy_elec_params;
+
+typedef struct cicy_net cicy_net;
+
+// Assembled DEA system for one (cluster, R_flag, family).  All arrays are owned by the
+// view and freed with cicy_sss_free.
+typedef struct cicy_sss_view {
+    int64_t nodes;          // global nodes (reserved + free)
+    int64_t reserved;       // R_d
+    int64_t n;              // free nodes = nodes - reserved
+    int64_t nnz;            // stored strictly-lower entries L
+    int64_t *row_ptr;       // [n+1]
+    int64_t *col_ind;       // [nnz]
+    double *values;         // [nnz]  (-G)
+    double *diag;           // [n]
+    double *b;              // [n]    -K_EF^T VEF (zero-guess residual)
+    double *vef;            // [reserved]
+    double *x0;             // [n]    ramp guess (zeros when the reference uses none)
+    double *r0;             // [n]    b - A x0 (equals b when x0 == 0)
+    int has_guess;          // 1 when R_flag==1 && R_d==2 (Direct_Electrifying.cpp:80)
+    int64_t n_points;
+    int64_t *lmm;           // [n_points] point -> global node or -1
+    int64_t *node_point;    // [nodes]   global node -> representative point (-1 reserved)
+    int64_t ne[2];          // boundary element counts for b1, b2 of `direction`
+    int64_t *e_bnd[2];      // boundary-side global node of each element
+    int64_t *e_int[2];      // other-end global node
+    double *e_re[2];        // element resistance (Ohm)
+    int direction;          // current direction used for the element lists (-1: none)
+    void *priv;             // owner of the arrays (internal)
+} cicy_sss_view;
+
+cicy_net *cicy_net_generate(const cicy_net_params *p);
+void cicy_net_free(cicy_net *net);
+
+// what: 0 points, 1 tubes, 2 junctions, 3 clusters
+int64_t cicy_net_count(const cicy_net *net, int what);
+const double *cicy_net_xyz(const cicy_net *net);          // [points*3]
+const int64_t *cicy_net_cnt_ptr(const cicy_net *net);     // [tubes+1]
+// Cluster k: number of tubes and percolation family (-1 when it does not percolate).
+int cicy_net_cluster(const cicy_net *net, int64_t k, int64_t *n_tubes, int *family);
+// Junction j: P1, P2 (points), N1, N2 (tubes), gap distance (um)
+int cicy_net_junction(const cicy_net *net, int64_t j, int64_t *p1, int64_t *p2,
+                      int64_t *n1, int64_t *n2, double *dist);
+int64_t cicy_net_boundary_points(const cicy_net *net, int b, const int64_t **pts);
+// Tubes touching face b (the reference's boundary_cnt[b], Cutoff_Wins.h:21-35).
+int64_t cicy_net_boundary_tubes(const cicy_net *net, int b, const int64_t **tubes);
+// Tubes of cluster k (ascending) and its junction indices (ascending).
+int64_t cicy_net_cluster_tubes(const cicy_net *net, int64_t k, const int64_t **tubes);
+int64_t cicy_net_cluster_junctions(const cicy_net *net, int64_t k, const int64_t **junc);
+// Element points of tube t (the reference's elements_cnt[t], ascending).
+int64_t cicy_net_elements(const cicy_net *net, int64_t t, const int64_t **pts);
+
+// Assemble the reduced system of cluster `k` with resistor flag `r_flag` (0 unit, 1 real)
+// for `family` (0..6).  `direction` (0..2 or -1) selects which boundary element lists to
+// build.  Returns NULL on a reference error condition (zero diagonal, invalid family,
+// element with < 2 points) and writes a message into err (may be NULL).
+cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int family,
+                                 const cicy_elec_params *ep, int direction, char *err,
+                                 int64_t err_len);
+void cicy_sss_free(cicy_sss_view *v);
+
+// Tunnelling constants from the mantissas the reference's input file gives
+// (Input_Reader.cpp:1450-1456).
+void cicy_tunnel_constants(double h, double e, double m, double lambda, double *c1,
+                           double *c2);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,245 @@
+"""ctypes binding of libcicy_dea.so (include/cicy_dea.h).
+
+Every call goes to the HIP library; there is no host fallback.  If the library or a GPU
+is missing the constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._libs import dea_lib
+
+CICY_OK, CICY_MAXITER, CICY_BREAKDOWN, CICY_BAD_INPUT, CICY_DEVICE_ERROR = 0, 1, 2, 3, 4
+METHOD_PCG_F64, METHOD_MIXED = 0, 1
+REL_R0, REL_B = 0, 1
+STATUS_NAMES = {0: "ok", 1: "maxiter", 2: "breakdown", 3: "bad_input", 4: "device_error"}
+
+_I64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_F64 = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_VP = ctypes.c_void_p
+
+# Every symbol the header declares; tests check that the library exports all of them.
+EXPORTED_SYMBOLS = [
+    "cicy_create", "cicy_destroy", "cicy_last_error", "cicy_set_rows_per_block",
+    "cicy_solve_dea_cg_sss", "cicy_update_residual", "cicy_spmv_sss_f64",
+    "cicy_boundary_current_f64", "cicy_sys_create", "cicy_sys_destroy", "cicy_sys_size",
+    "cicy_sys_set_rhs", "cicy_sys_solve", "cicy_sys_get", "cicy_sys_spmv_bench",
+    "cicy_host_blocked_spmv",
+]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int64), ("status", ctypes.c_int),
+                ("refinements", ctypes.c_int), ("rel_res", ctypes.c_double),
+                ("true_rel_res", ctypes.c_double), ("solve_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_} | {
+            "status_name": STATUS_NAMES.get(self.status, str(self.status))}
+
+
+_bound = None
+
+
+def lib():
+    global _bound
+    if _bound is not None:
+        return _bound
+    L = dea_lib()
+    P = ctypes.POINTER
+    L.cicy_create.argtypes = [P(_VP), ctypes.c_int]
+    L.cicy_destroy.argtypes = [_VP]
+    L.cicy_last_error.restype = ctypes.c_char_p
+    L.cicy_last_error.argtypes = [_VP]
+    L.cicy_set_rows_per_block.argtypes = [_VP, ctypes.c_int]
+    L.cicy_solve_dea_cg_sss.argtypes = [_VP, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+                                        _I64, _I64, _F64, _F64, ctypes.c_int64, _F64, _F64, _F64,
+                                        _F64, ctypes.c_int64, P(SolveStats)]
+    L.cicy_update_residual.argtypes = [_VP, ctypes.c_int64, _I64, _I64, _F64, _F64,
+                                       ctypes.c_int64, _F64, _F64]
+    L.cicy_spmv_sss_f64.argtypes = [_VP, ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
+                                    _F64, _F64]
+    L.cicy_boundary_current_f64.argtypes = [_VP, ctypes.c_int64, _I64, _I64, _F64, _F64,
+                                            ctypes.c_int64, P(ctypes.c_double)]
+    L.cicy_sys_create.argtypes = [_VP, ctypes.c_int64, ctypes.c_int64, _I64, _I64, _F64, _F64,
+                                  P(_VP)]
+    L.cicy_sys_destroy.argtypes = [_VP]
+    L.cicy_sys_size.restype = ctypes.c_int64
+    L.cicy_sys_size.argtypes = [_VP, ctypes.c_int]
+    L.cicy_sys_set_rhs.argtypes = [_VP, _F64, _VP, _VP]
+    L.cicy_sys_solve.argtypes = [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_int64,
+                                 ctypes.c_int, P(SolveStats)]
+    L.cicy_sys_get.argtypes = [_VP, _VP, _VP]
+    L.cicy_sys_spmv_bench.argtypes = [_VP, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
+    L.cicy_host_blocked_spmv.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
+                                         ctypes.c_int, _F64, _F64]
+    _bound = L
+    return L
+
+
+def _i64(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a if a.size else np.zeros(1, np.int64)
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if a.size else np.zeros(1, np.float64)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DEAError(RuntimeError):
+    pass
+
+
+class Context:
+    """One device context (stream, events); not re-entrant."""
+
+    def __init__(self, device: int = 0, rows_per_block: int = 0):
+        L = lib()
+        h = _VP()
+        if not L.cicy_create(ctypes.byref(h), device):
+            raise DEAError(f"cicy_create failed on device {device} (no GPU?)")
+        self._h = h
+        self._L = L
+        if rows_per_block:
+            self.set_rows_per_block(rows_per_block)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.cicy_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def last_error(self) -> str:
+        return (self._L.cicy_last_error(self._h) or b"").decode()
+
+    def _check(self, ok, what):
+        if not ok:
+            raise DEAError(f"{what}: {self.last_error}")
+
+    def set_rows_per_block(self, rb: int):
+        self._check(self._L.cicy_set_rows_per_block(self._h, rb), "cicy_set_rows_per_block")
+
+    def solve_dea_cg_sss(self, nodes, reserved_nodes, tolerance, col_ind, row_ptr, values,
+                         diagonal, R, VEF, voltages_sol, max_iter=0):
+        """Drop-in for Solve_DEA_equations_CG_SSS (Direct_Electrifying.cpp:1538-1659).
+
+        Returns (ok, voltages, voltages_sol, R, stats) where R is the final recursive
+        residual and voltages = [VEF | x]."""
+        n = nodes - reserved_nodes
+        R = _f64(np.array(R, dtype=np.float64, copy=True))
+        x = _f64(np.array(voltages_sol, dtype=np.float64, copy=True))
+        volt = np.zeros(max(nodes, 1))
+        st = SolveStats()
+        col_ind = _i64(col_ind)
+        ok = self._L.cicy_solve_dea_cg_sss(
+            self._h, nodes, reserved_nodes, tolerance, col_ind, _i64(row_ptr), _f64(values),
+            _f64(diagonal), len(np.asarray(values)), R, _f64(VEF), x, volt, max_iter,
+            ctypes.byref(st))
+        return bool(ok), volt[:nodes], x[:n], R[:n], st
+
+    def spmv(self, row_ptr, col_ind, values, diagonal, v):
+        n = len(diagonal)
+        out = np.zeros(max(n, 1))
+        self._check(self._L.cicy_spmv_sss_f64(self._h, n, _i64(row_ptr), _i64(col_ind),
+                                              _f64(values), _f64(diagonal), len(values),
+                                              _f64(v), out), "cicy_spmv_sss_f64")
+        return out[:n]
+
+    def update_residual(self, row_ptr, col_ind, values, diagonal, x0, R):
+        n = len(diagonal)
+        R = _f64(np.array(R, dtype=np.float64, copy=True))
+        self._check(self._L.cicy_update_residual(self._h, n, _i64(row_ptr), _i64(col_ind),
+                                                 _f64(values), _f64(diagonal), len(values),
+                                                 _f64(x0), R), "cicy_update_residual")
+        return R[:n]
+
+    def boundary_current(self, n_bnd, n_int, re, voltages) -> float:
+        out = ctypes.c_double()
+        self._check(self._L.cicy_boundary_current_f64(
+            self._h, len(n_bnd), _i64(n_bnd), _i64(n_int), _f64(re), _f64(voltages),
+            len(voltages), ctypes.byref(out)), "cicy_boundary_current_f64")
+        return out.value
+
+    def system(self, row_ptr, col_ind, values, diagonal) -> "System":
+        return System(self, row_ptr, col_ind, values, diagonal)
+
+
+class System:
+    """A device-resident SSS system (cicy_sys)."""
+
+    def __init__(self, ctx: Context, row_ptr, col_ind, values, diagonal):
+        self.ctx = ctx
+        h = _VP()
+        self.n = len(diagonal)
+        ctx._check(ctx._L.cicy_sys_create(ctx._h, self.n, len(values), _i64(row_ptr),
+                                          _i64(col_ind), _f64(values), _f64(diagonal),
+                                          ctypes.byref(h)), "cicy_sys_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx._L.cicy_sys_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def size(self, what: str) -> int:
+        return int(self.ctx._L.cicy_sys_size(
+            self._h, {"n": 0, "nnz": 1, "remote": 2, "bytes": 3}[what]))
+
+    def set_rhs(self, r0, x0=None, b=None):
+        r0 = _f64(r0)
+        x0 = None if x0 is None else _f64(x0)
+        b = None if b is None else _f64(b)
+        self.ctx._check(self.ctx._L.cicy_sys_set_rhs(self._h, r0, _ptr(x0), _ptr(b)),
+                        "cicy_sys_set_rhs")
+        self._keep = (r0, x0, b)
+
+    def solve(self, method=METHOD_PCG_F64, tol=1e-10, max_iter=0, rel_to=REL_R0) -> SolveStats:
+        st = SolveStats()
+        self.ctx._check(self.ctx._L.cicy_sys_solve(self._h, method, tol, max_iter, rel_to,
+                                                   ctypes.byref(st)), "cicy_sys_solve")
+        return st
+
+    def get(self):
+        x = np.zeros(max(self.n, 1))
+        r = np.zeros(max(self.n, 1))
+        self.ctx._check(self.ctx._L.cicy_sys_get(self._h, _ptr(x), _ptr(r)), "cicy_sys_get")
+        return x[:self.n], r[:self.n]
+
+    def spmv_bench(self, precision=64, reps=50) -> float:
+        ms = ctypes.c_double()
+        self.ctx._check(self.ctx._L.cicy_sys_spmv_bench(self._h, precision, reps,
+                                                        ctypes.byref(ms)),
+                        "cicy_sys_spmv_bench")
+        return ms.value
+
+
+def host_blocked_spmv(row_ptr, col_ind, values, diagonal, v, rows_per_block=256):
+    """The device layout applied by host loops (layout check without a GPU)."""
+    n = len(diagonal)
+    out = np.zeros(max(n, 1))
+    ok = lib().cicy_host_blocked_spmv(n, _i64(row_ptr), _i64(col_ind), _f64(values),
+                                      _f64(diagonal), len(values), rows_per_block, _f64(v), out)
+    if not ok:
+        raise DEAError("cicy_host_blocked_spmv rejected the input")
+    return out[:n]

@@ -1,0 +1,133 @@
+/*
+ * cicy_dea.h -- C-ABI of the MI355X (gfx950) Direct Electrifying conductance solver.
+ *
+ * Replaces the CPU solve of rnaimehaom/CICY_geom's Direct_Electrifying class:
+ *   Solve_DEA_equations_CG_SSS   Direct_Electrifying.h:63, Direct_Electrifying.cpp:1538-1659
+ *   spM_V_SSS                    Direct_Electrifying.h:67, Direct_Electrifying.cpp:1678-1695
+ *   Update_residual_vector       Direct_Electrifying.h:62, Direct_Electrifying.cpp:1502-1535
+ * and the boundary current sum of Electrical_analysis::Calculate_parallel_resistor
+ * (Electrical_analysis.cpp:355-473, :547-605).
+ *
+ * Conventions (mirroring the reference):
+ *   - every entry point returns 1 on success and 0 on failure, as the reference's int
+ *     functions do; cicy_last_error() holds the message the reference would write to
+ *     hout, and the stats struct carries a status code so that the max-iteration case
+ *     (a reference "success") and a breakdown are distinguishable;
+ *   - matrices are the reference's SSS arrays: row_ptr[n+1] (row_ptr[0] == 0), col_ind
+ *     and values of the strictly-lower triangle (col < row), diagonal[n]; indices are
+ *     int64 (`long` on LP64), values double;
+ *   - the caller owns every host array; the library borrows them for the duration of
+ *     the call.  Device copies live in the context or a cicy_sys handle;
+ *   - a context is not re-entrant: calls on one context are serialised by the caller.
+ *     Work inside a call is ordered on the context's stream and complete at return.
+ */
+#ifndef CICY_DEA_H
+#define CICY_DEA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cicy_ctx cicy_ctx;
+typedef struct cicy_sys cicy_sys;
+
+/* status codes (cicy_solve_stats.status) */
+#define CICY_OK 0            /* converged */
+#define CICY_MAXITER 1       /* iteration cap reached; the reference returns success here */
+#define CICY_BREAKDOWN 2     /* p.Ap not finite or not positive (singular/indefinite) */
+#define CICY_BAD_INPUT 3     /* malformed arrays (checked on the host before any launch) */
+#define CICY_DEVICE_ERROR 4  /* a HIP call failed */
+
+/* solve methods (cicy_sys_solve) */
+#define CICY_METHOD_PCG_F64 0    /* plain Jacobi-PCG in FP64, the reference's algorithm */
+#define CICY_METHOD_MIXED 1      /* FP32 Jacobi-PCG with FP64 reliable updates/refinement */
+
+/* stopping rule (cicy_sys_solve rel_to) */
+#define CICY_REL_R0 0  /* ||r_k|| <= tol*||r_0||, r_0 = b - A x0: the reference (:1571,:1619) */
+#define CICY_REL_B 1   /* ||b - A x|| <= tol*||b||, b the zero-guess right-hand side */
+
+typedef struct cicy_solve_stats {
+    int64_t iters;          /* CG iterations (the reference's k, Direct_Electrifying.cpp:1642) */
+    int status;             /* CICY_OK, CICY_MAXITER, CICY_BREAKDOWN, ... */
+    int refinements;        /* FP64 true-residual recomputations (mixed method) */
+    double rel_res;         /* final ||r_k|| / ||r_0|| of the recursive residual */
+    double true_rel_res;    /* ||b - A x|| / ||b|| when b was given, else -1 */
+    double solve_ms;        /* device time of the solve (HIP events), inputs resident */
+    double total_ms;        /* wall time of the call including host<->device copies */
+} cicy_solve_stats;
+
+/* ---- context ---------------------------------------------------------------------- */
+int cicy_create(cicy_ctx **out, int device);
+void cicy_destroy(cicy_ctx *ctx);
+const char *cicy_last_error(const cicy_ctx *ctx);
+/* rows per work-group of the SSS kernels (0 = autotuned/default); see cicy_autotune */
+int cicy_set_rows_per_block(cicy_ctx *ctx, int rows_per_block);
+
+/* ---- drop-in seam: Solve_DEA_equations_CG_SSS (Direct_Electrifying.cpp:1538-1659) -----
+ * nodes          global node count (reserved + free), as the reference passes it
+ * reserved_nodes number of Dirichlet nodes R_d (2, 4 or 6)
+ * tolerance      relative reduction of the initial recursive residual
+ * col_ind, row_ptr, values, diagonal   SSS arrays of the free block (n = nodes-R_d)
+ * R              [n] in: b - A x0; out: final recursive residual
+ * VEF            [R_d] prescribed voltages
+ * voltages_sol   [n] in: x0; out: solution
+ * voltages       [nodes] out: [VEF | x] (the class member `voltages`, :1646-1656)
+ * nnz            length of col_ind/values (the reference's vector sizes)
+ * max_iter <= 0 selects the reference's cap 300*nodes (:1559). */
+int cicy_solve_dea_cg_sss(cicy_ctx *ctx, int64_t nodes, int64_t reserved_nodes,
+                          double tolerance, const int64_t *col_ind, const int64_t *row_ptr,
+                          const double *values, const double *diagonal, int64_t nnz,
+                          double *R, const double *VEF, double *voltages_sol,
+                          double *voltages, int64_t max_iter, cicy_solve_stats *stats);
+
+/* ---- Update_residual_vector (Direct_Electrifying.cpp:1502-1535): R <- R - A x0 ---------- */
+int cicy_update_residual(cicy_ctx *ctx, int64_t n, const int64_t *row_ptr,
+                         const int64_t *col_ind, const double *values,
+                         const double *diagonal, int64_t nnz, const double *x0, double *R);
+
+/* ---- spM_V_SSS (Direct_Electrifying.cpp:1678-1695): out = A v, host vectors ------------- */
+int cicy_spmv_sss_f64(cicy_ctx *ctx, int64_t n, const int64_t *row_ptr,
+                      const int64_t *col_ind, const double *values, const double *diagonal,
+                      int64_t nnz, const double *v, double *out);
+
+/* ---- boundary current (Electrical_analysis.cpp:547-605) ------------------------------
+ * I = sum_e (voltages[n_int[e]] - voltages[n_bnd[e]]) / re[e] in FP64 on the device.
+ * voltages has nv entries (the reference's DEA->voltages, reserved nodes first). */
+int cicy_boundary_current_f64(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
+                              const int64_t *n_int, const double *re,
+                              const double *voltages, int64_t nv, double *current);
+
+/* ---- device-resident systems (repeated solves, benchmarking) --------------------------
+ * cicy_sys_create uploads one SSS system (validated on the host) and keeps it in HBM.
+ * The reference instead rebuilds and re-solves a fresh object per direction/cluster
+ * (Electrical_analysis.cpp:42, :252); here the device copy is reused across solves. */
+int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                    const int64_t *col_ind, const double *values, const double *diagonal,
+                    cicy_sys **out);
+void cicy_sys_destroy(cicy_sys *sys);
+int64_t cicy_sys_size(const cicy_sys *sys, int what); /* 0 n, 1 nnz, 2 remote entries,
+                                                           3 device bytes */
+/* r0 = b - A x0 (the reference's R), x0 (may be NULL = zeros), b (may be NULL; needed for
+ * the true residual and for CICY_REL_B). */
+int cicy_sys_set_rhs(cicy_sys *sys, const double *r0, const double *x0, const double *b);
+/* Solve on device-resident data.  max_iter <= 0: 300*(n+2). */
+int cicy_sys_solve(cicy_sys *sys, int method, double tol, int64_t max_iter, int rel_to,
+                   cicy_solve_stats *stats);
+int cicy_sys_get(cicy_sys *sys, double *x, double *r);
+/* Time `reps` back-to-back SSS SpMV launches on resident data; average ms per launch. */
+int cicy_sys_spmv_bench(cicy_sys *sys, int precision /*64 or 32*/, int reps, double *avg_ms);
+
+/* ---- test hook: the blocked SSS layout applied on the host (no GPU) -------------------
+ * Builds the device layout for rows_per_block and applies it with host loops, so the
+ * layout can be checked where no GPU is present. */
+int cicy_host_blocked_spmv(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                           const double *values, const double *diagonal, int64_t nnz,
+                           int rows_per_block, const double *v, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CICY_DEA_H */

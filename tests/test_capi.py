@@ -1,0 +1,69 @@
+"""The C-ABI libraries load and export every declared symbol; host-side logic of the
+device layout.  No GPU calls."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from cicy_geom_amd import dea, network
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cicy_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_dea_header_symbols_exported():
+    names = declared("cicy_dea.h")
+    assert len(names) >= 16
+    lib = ctypes.CDLL(os.path.join(ROOT, "cicy_geom_amd", "libcicy_dea.so"))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(dea.EXPORTED_SYMBOLS) == names
+
+
+def test_all_headers_exported():
+    libs = {"cicy_dea.h": "libcicy_dea.so"}
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h") and h in libs:
+            lib = ctypes.CDLL(os.path.join(ROOT, "cicy_geom_amd", libs[h]))
+            for n in declared(h):
+                assert hasattr(lib, n), (h, n)
+
+
+@pytest.mark.parametrize("rb", [128, 256, 512, 1024])
+def test_blocked_layout_on_host(rb):
+    s = network.largest_percolating_system(network.generate(1), 0)
+    v = np.random.default_rng(rb).standard_normal(s.n)
+    got = dea.host_blocked_spmv(s.row_ptr, s.col_ind, s.values, s.diag, v, rb)
+    want = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, v)
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-15 * np.abs(want).max())
+
+
+def test_blocked_layout_rejects_bad_input():
+    rp = np.array([0, 0, 1, 2], np.int64)
+    col = np.array([0, 2], np.int64)       # row 2 refers to column 2: not strictly lower
+    with pytest.raises(dea.DEAError):
+        dea.host_blocked_spmv(rp, col, np.array([-1.0, -1.0]), np.ones(3), np.ones(3))
+    with pytest.raises(dea.DEAError):
+        dea.host_blocked_spmv(np.array([0, 0, 1], np.int64), np.array([0], np.int64),
+                              np.array([-1.0]), np.array([1.0, 0.0]), np.ones(2))
+
+
+def test_context_needs_gpu_or_raises():
+    """Without a device the context refuses to start: there is no CPU fallback."""
+    try:
+        import torch  # noqa: F401  (only to learn whether a GPU is visible)
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    with pytest.raises(dea.DEAError):
+        dea.Context(0)

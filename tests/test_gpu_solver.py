@@ -1,0 +1,190 @@
+"""GPU parity of the HIP DEA solver against the CPU oracle (called through the C-ABI).
+
+Bars: SpMV and residual update within a few ulp of the oracle's SSS loop; voltages of the
+drop-in solve within 1e-8 relative of the reference-faithful CPU CG and within 1e-8 of
+the FP64 truth; boundary current / resistance within 1e-9 relative (see the tolerance
+notes next to each assertion).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cicy_geom_amd import dea, network
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dea.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    net = network.generate(1)
+    return network.largest_percolating_system(net, 0)
+
+
+def random_laplacian(n, seed, extra=2.0, spread=7.0):
+    """Connected chain + random junction graph, conductances over `spread` decades,
+    two Dirichlet nodes folded into b (as From_2d_to_1d_vectors does)."""
+    rng = np.random.default_rng(seed)
+    rows, cols, g = [], [], []
+    for i in range(1, n):
+        rows.append(i); cols.append(i - 1); g.append(10 ** rng.uniform(-spread / 2, 0))
+    m = int(extra * n)
+    a = rng.integers(0, n, m); b = rng.integers(0, n, m)
+    for i, j in zip(a, b):
+        if i != j:
+            rows.append(max(i, j)); cols.append(min(i, j)); g.append(10 ** rng.uniform(-spread, 0))
+    # dedupe (keep last, the reference's overwrite rule) and sort per row
+    d = {}
+    for r, c, v in zip(rows, cols, g):
+        d[(r, c)] = v
+    keys = sorted(d)
+    row_ptr = np.zeros(n + 1, np.int64)
+    col = np.array([k[1] for k in keys], np.int64)
+    val = np.array([-d[k] for k in keys])
+    for k in keys:
+        row_ptr[k[0] + 1] += 1
+    row_ptr = np.cumsum(row_ptr)
+    diag = np.zeros(n)
+    for (r, c), v in d.items():
+        diag[r] += v; diag[c] += v
+    gb = 10 ** rng.uniform(-3, 0, 2)
+    diag[0] += gb[0]; diag[n - 1] += gb[1]
+    bvec = np.zeros(n); bvec[0] = gb[0] * 1.0   # node 0 tied to V=1, node n-1 to V=0
+    return row_ptr, col, val, diag, bvec
+
+
+@pytest.mark.parametrize("rb", [128, 256, 512, 1024])
+def test_spmv_matches_oracle_c1(ctx, c1, rb):
+    ctx.set_rows_per_block(rb)
+    v = np.random.default_rng(rb).standard_normal(c1.n)
+    out = ctx.spmv(c1.row_ptr, c1.col_ind, c1.values, c1.diag, v)
+    ref = oracle.spmv_sss(c1.row_ptr, c1.col_ind, c1.values, c1.diag, v)
+    ctx.set_rows_per_block(256)
+    # different summation order (LDS atomics): a few ulp of the row's magnitude
+    scale = np.abs(c1.diag * v) + np.abs(ref)
+    assert np.all(np.abs(out - ref) <= 1e-14 * scale + 1e-300)
+
+
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 1000, 40_000])
+def test_spmv_ragged_sizes(ctx, n):
+    row_ptr, col, val, diag, _ = random_laplacian(n, seed=n)
+    v = np.random.default_rng(1).standard_normal(n)
+    out = ctx.spmv(row_ptr, col, val, diag, v)
+    ref = oracle.spmv_sss(row_ptr, col, val, diag, v)
+    assert np.allclose(out, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+
+
+def test_update_residual_matches_oracle(ctx, c1):
+    R = ctx.update_residual(c1.row_ptr, c1.col_ind, c1.values, c1.diag, c1.x0, c1.b)
+    assert np.allclose(R, c1.r0, rtol=1e-12, atol=1e-14 * np.abs(c1.b).max())
+
+
+def test_pcg_dropin_matches_reference_cpu(ctx, c1):
+    ref = oracle.pcg_sss(c1.nodes, c1.reserved, 1e-10, c1.row_ptr, c1.col_ind, c1.values,
+                         c1.diag, c1.r0, c1.vef, c1.x0)
+    ok, volt, x, r, st = ctx.solve_dea_cg_sss(c1.nodes, c1.reserved, 1e-10, c1.col_ind,
+                                              c1.row_ptr, c1.values, c1.diag, c1.r0, c1.vef,
+                                              c1.x0)
+    assert ok and st.status == dea.CICY_OK
+    truth, _, _ = oracle.cg_truth(c1.row_ptr, c1.col_ind, c1.values, c1.diag, c1.b, tol=1e-14)
+    e_ref = np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"])
+    e_truth = np.linalg.norm(x - truth) / np.linalg.norm(truth)
+    print(f"iters gpu={st.iters} cpu={ref['iters']} |x-x_cpu|={e_ref:.2e} |x-x_true|={e_truth:.2e}")
+    assert e_ref < 1e-8 and e_truth < 1e-8
+    assert abs(st.iters - ref["iters"]) <= max(5, ref["iters"] // 20)
+    assert np.array_equal(volt[:c1.reserved], c1.vef)
+    # the recursive residual handed back satisfies the reference's stopping rule
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(c1.r0)
+    # boundary current through b1 and b2 (current conservation, the reference's check)
+    I1 = ctx.boundary_current(*c1.boundary_elements[0], volt)
+    I2 = ctx.boundary_current(*c1.boundary_elements[1], volt)
+    I1_ref = oracle.boundary_current(*c1.boundary_elements[0], ref["voltages"])
+    print(f"I gpu={I1:.15e} cpu={I1_ref:.15e} check={I2:.15e}")
+    assert abs(I1 - I1_ref) / abs(I1_ref) < 1e-9
+    assert abs(I1 + I2) / abs(I1) < 1e-6 or abs(I1 - I2) / abs(I1) < 1e-6
+
+
+def test_boundary_current_matches_oracle(ctx, c1):
+    v = np.random.default_rng(3).uniform(0, 1, c1.nodes)
+    for nb, ni, re in c1.boundary_elements:
+        got = ctx.boundary_current(nb, ni, re, v)
+        want = oracle.boundary_current(nb, ni, re, v)
+        assert abs(got - want) <= 1e-13 * np.sum(np.abs(v[ni] - v[nb]) / re)
+
+
+def test_maxiter_is_success_with_status(ctx, c1):
+    ok, volt, x, r, st = ctx.solve_dea_cg_sss(c1.nodes, c1.reserved, 1e-10, c1.col_ind,
+                                              c1.row_ptr, c1.values, c1.diag, c1.r0, c1.vef,
+                                              c1.x0, max_iter=40)
+    ref = oracle.pcg_sss(c1.nodes, c1.reserved, 1e-10, c1.row_ptr, c1.col_ind, c1.values,
+                         c1.diag, c1.r0, c1.vef, c1.x0, max_iter=40)
+    assert ok and st.status == dea.CICY_MAXITER and st.iters == 40
+    assert ref["status"] == oracle.ORC_MAXITER
+    assert np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"]) < 1e-10
+
+
+def test_zero_rhs_converges_at_once(ctx, c1):
+    ok, volt, x, r, st = ctx.solve_dea_cg_sss(c1.nodes, c1.reserved, 1e-10, c1.col_ind,
+                                              c1.row_ptr, c1.values, c1.diag,
+                                              np.zeros(c1.n), c1.vef, np.zeros(c1.n))
+    assert ok and st.status == dea.CICY_OK and st.iters == 0 and not np.any(x)
+
+
+def test_bad_input_rejected_before_launch(ctx, c1):
+    col = c1.col_ind.copy()
+    col[5] = c1.n + 3       # outside the strictly-lower triangle
+    ok, *_, st = ctx.solve_dea_cg_sss(c1.nodes, c1.reserved, 1e-10, col, c1.row_ptr,
+                                      c1.values, c1.diag, c1.r0, c1.vef, c1.x0)
+    assert not ok and st.status == dea.CICY_BAD_INPUT
+    assert "strictly below" in ctx.last_error
+    diag = c1.diag.copy()
+    diag[7] = 0.0           # floating node
+    ok, *_, st = ctx.solve_dea_cg_sss(c1.nodes, c1.reserved, 1e-10, c1.col_ind, c1.row_ptr,
+                                      c1.values, diag, c1.r0, c1.vef, c1.x0)
+    assert not ok and "diagonal" in ctx.last_error
+
+
+def test_empty_system(ctx):
+    ok, volt, x, r, st = ctx.solve_dea_cg_sss(2, 2, 1e-10, np.zeros(0, np.int64),
+                                              np.zeros(1, np.int64), np.zeros(0), np.zeros(0),
+                                              np.zeros(0), np.array([0.0, 1.0]), np.zeros(0))
+    assert ok and list(volt) == [0.0, 1.0] and st.status == dea.CICY_OK
+
+
+@pytest.mark.parametrize("method", [dea.METHOD_PCG_F64, dea.METHOD_MIXED])
+def test_resident_solve_true_residual(ctx, c1, method):
+    sysd = ctx.system(c1.row_ptr, c1.col_ind, c1.values, c1.diag)
+    sysd.set_rhs(c1.b, None, c1.b)
+    st = sysd.solve(method, 1e-10, 0, dea.REL_B)
+    x, _ = sysd.get()
+    truth, _, _ = oracle.cg_truth(c1.row_ptr, c1.col_ind, c1.values, c1.diag, c1.b, tol=1e-14)
+    e = np.linalg.norm(x - truth) / np.linalg.norm(truth)
+    ax = oracle.spmv_sss(c1.row_ptr, c1.col_ind, c1.values, c1.diag, x)
+    tr = np.linalg.norm(c1.b - ax) / np.linalg.norm(c1.b)
+    print(st.as_dict(), f"err_truth={e:.2e} true_res={tr:.2e}")
+    assert st.status == dea.CICY_OK
+    assert tr <= 1.05e-10 and abs(tr - st.true_rel_res) <= 1e-3 * tr + 1e-16
+    assert e < 1e-8
+    sysd.close()
+
+
+@pytest.mark.parametrize("method", [dea.METHOD_PCG_F64, dea.METHOD_MIXED])
+def test_random_laplacian_solve(ctx, method):
+    row_ptr, col, val, diag, b = random_laplacian(20_000, seed=11, spread=5.0)
+    sysd = ctx.system(row_ptr, col, val, diag)
+    sysd.set_rhs(b, None, b)
+    st = sysd.solve(method, 1e-10, 0, dea.REL_B)
+    x, _ = sysd.get()
+    ax = oracle.spmv_sss(row_ptr, col, val, diag, x)
+    tr = np.linalg.norm(b - ax) / np.linalg.norm(b)
+    print(method, st.as_dict(), tr)
+    assert st.status == dea.CICY_OK and tr <= 1.05e-10
+    # discrete maximum principle: node voltages between the two boundary voltages
+    assert x.min() >= -1e-8 and x.max() <= 1 + 1e-8
+    sysd.close()
