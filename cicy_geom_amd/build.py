@@ -47,9 +47,21 @@ def build_dea(out_dir: str = HERE) -> str:
     return out
 
 
+def build_cpp_dropin_test() -> str:
+    """The C++ drop-in check program (tests/cpp/dropin_chain.cpp) linked against
+    libcicy_dea.so with an rpath to the package directory."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_chain.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "dropin_chain")
+    _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", src, f"-L{HERE}", "-lcicy_dea",
+          "-Wl,-rpath,$ORIGIN/../../cicy_geom_amd", "-L/opt/rocm/lib", "-lamdhip64",
+          "-o", out])
+    return out
+
+
 def build_all() -> None:
     build_net()
     build_dea()
+    build_cpp_dropin_test()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,530 @@
+// dea_batched.hip -- many DEA systems of uneven size solved together.
+//
+// The reference solves the clusters and directions of a window one after another, each
+// with a fresh Direct_Electrifying object (Electrical_analysis.cpp:34-105, :249-283).  Here
+// a batch stores all systems back to back (block-diagonal, blocked-SSS layout of
+// dea_kernels.h) with one PcgState/MixState per system.  Each launch covers the row blocks
+// of every unfinished system, so one hipGraph chunk advances all systems by kChunk
+// iterations; every system keeps its own alpha/beta, stopping test and iteration count.
+// After each chunk the host reads the per-system flags; when the blocks of finished
+// systems make up a large part of the grid, the active block list is compacted (double
+// buffered, so the chunk already queued keeps its list) and the chunk graph is captured
+// again for the smaller grid.
+#include "dea_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+using namespace cicy;
+
+struct cicy_batch {
+    cicy_ctx *ctx = nullptr;
+    int nsys = 0, rb = 256;
+    int n = 0, nnz = 0, nblocks = 0, nremote = 0;
+    std::vector<int> h_sys_row0, h_sys_blk0, h_blk_sys, h_blk_row0;
+    std::vector<double> bnorm2;
+    // device: matrix (FP64 + Jacobi-scaled FP32 values), tables, vectors, state
+    int *rp = nullptr, *col = nullptr, *urp = nullptr, *ucol = nullptr;
+    double *val = nullptr, *diag = nullptr, *uval = nullptr;
+    float *val32 = nullptr, *uval32 = nullptr;
+    int *blk_sys = nullptr, *blk_row0 = nullptr, *sys_row0 = nullptr, *sys_blk0 = nullptr;
+    int *active[2] = {nullptr, nullptr};
+    int *h_active[2] = {nullptr, nullptr};  // pinned staging
+    double *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
+    double *x0 = nullptr, *r0 = nullptr, *b = nullptr, *stop_ref2 = nullptr;
+    float *rs = nullptr, *xs = nullptr, *ps0 = nullptr, *ps1 = nullptr, *qs = nullptr;
+    double *partials = nullptr, *norms = nullptr;
+    PcgState *st = nullptr;
+    MixState *mix = nullptr;
+    unsigned int *ncount = nullptr;
+    PcgState *h_st[2] = {nullptr, nullptr};  // pinned poll slots
+    hipGraphExec_t exec = nullptr;
+    int exec_method = -1, exec_grid = -1, exec_buf = -1;
+    size_t device_bytes = 0;
+};
+
+namespace {
+
+// per-system ||b - A x||^2 and ||b||^2 (ax = A x formed before)
+template <int RB>
+__global__ __launch_bounds__(kThreads) void k_batch_norms(int n, BatchDev bd,
+                                                          const double *__restrict__ bvec,
+                                                          const double *__restrict__ ax,
+                                                          double *__restrict__ out,
+                                                          unsigned int *__restrict__ counters,
+                                                          double *__restrict__ partials) {
+    const Blk k = block_info<RB, true>(bd, n);
+    __shared__ double red[8];
+    __shared__ int flag;
+    double a = 0.0, t = 0.0;
+    for (int j = threadIdx.x; j < k.nrows; j += kThreads) {
+        const int i = k.r0 + j;
+        const double d = bvec[i] - ax[i];
+        a += d * d;
+        t += bvec[i] * bvec[i];
+    }
+    block_sum2(a, t, red);
+    if (threadIdx.x == 0) {
+        partials[2 * k.b] = a;
+        partials[2 * k.b + 1] = t;
+    }
+    if (arrive_last(&counters[k.sys], k.count, &flag)) {
+        double s = reduce_partials(partials + 2 * k.first, k.count, 2, red);
+        double u = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
+        if (threadIdx.x == 0) {
+            out[2 * k.sys] = s;
+            out[2 * k.sys + 1] = u;
+            counters[k.sys] = 0;
+        }
+    }
+}
+
+BatchDev bdev(const cicy_batch *B, const int *active) {
+    return BatchDev{active, B->blk_sys, B->blk_row0, B->sys_row0, B->sys_blk0};
+}
+
+SssDev<double> v64(const cicy_batch *B) {
+    return SssDev<double>{B->n, B->nblocks, B->rp, B->col, B->val, B->diag, B->urp, B->ucol,
+                          B->uval};
+}
+
+SssDev<float> v32(const cicy_batch *B) {
+    return SssDev<float>{B->n, B->nblocks, B->rp, B->col, B->val32, nullptr, B->urp, B->ucol,
+                         B->uval32};
+}
+
+template <int RB>
+void chunk_f64(cicy_batch *B, const int *act, int grid) {
+    hipStream_t st = B->ctx->stream;
+    const BatchDev bd = bdev(B, act);
+    for (int it = 0; it < kChunk; it++) {
+        double *pold = (it & 1) ? B->p1 : B->p0;
+        double *pnew = (it & 1) ? B->p0 : B->p1;
+        hipLaunchKernelGGL((k_apply<double, RB, true, false, true>), dim3(grid), dim3(kThreads),
+                           0, st, v64(B), bd, pold, B->r, pnew, B->q, B->st, B->partials,
+                           nullptr);
+        hipLaunchKernelGGL((k_update_f64<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
+                           B->diag, B->x, B->r, pnew, B->q, B->st, B->partials);
+    }
+}
+
+template <int RB>
+void chunk_mixed(cicy_batch *B, const int *act, int grid) {
+    hipStream_t st = B->ctx->stream;
+    const BatchDev bd = bdev(B, act);
+    for (int it = 0; it < kChunk; it++) {
+        float *pold = (it & 1) ? B->ps1 : B->ps0;
+        float *pnew = (it & 1) ? B->ps0 : B->ps1;
+        hipLaunchKernelGGL((k_apply<float, RB, true, true, true>), dim3(grid), dim3(kThreads), 0,
+                           st, v32(B), bd, pold, B->rs, pnew, B->qs, B->st, B->partials, nullptr);
+        hipLaunchKernelGGL((k_update_mixed<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n,
+                           bd, B->xs, B->rs, pnew, B->qs, B->st, B->mix, B->partials);
+    }
+    hipLaunchKernelGGL((k_fold<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd, B->diag,
+                       B->x, B->xs, B->mix, 0);
+    hipLaunchKernelGGL((k_apply<double, RB, false, false, true>), dim3(grid), dim3(kThreads), 0,
+                       st, v64(B), bd, B->x, nullptr, nullptr, B->q, nullptr, nullptr, B->mix);
+    hipLaunchKernelGGL((k_reliable_resid<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
+                       B->diag, B->b, B->q, B->r, B->rs, B->st, B->mix, B->partials,
+                       kStallLimit);
+}
+
+template <int RB>
+void init_all(cicy_batch *B, int method, double tol, long long max_iter) {
+    hipStream_t st = B->ctx->stream;
+    const BatchDev bd = bdev(B, nullptr);
+    if (method == CICY_METHOD_PCG_F64)
+        hipLaunchKernelGGL((k_init_f64<RB, true>), dim3(B->nblocks), dim3(kThreads), 0, st, B->n,
+                           bd, B->diag, B->r, B->p0, B->p1, B->st, B->partials, tol,
+                           B->stop_ref2, -1.0, max_iter);
+    else
+        hipLaunchKernelGGL((k_mixed_init<RB, true>), dim3(B->nblocks), dim3(kThreads), 0, st,
+                           B->n, bd, B->diag, B->r0, B->rs, B->xs, B->ps0, B->ps1, B->st, B->mix,
+                           B->partials, tol, B->stop_ref2, -1.0, max_iter, kDelta * kDelta);
+}
+
+template <int RB>
+void final_norms(cicy_batch *B) {
+    hipStream_t st = B->ctx->stream;
+    const BatchDev bd = bdev(B, nullptr);
+    hipLaunchKernelGGL((k_fold<RB, true>), dim3(B->nblocks), dim3(kThreads), 0, st, B->n, bd,
+                       B->diag, B->x, B->xs, B->mix, 1);
+    hipLaunchKernelGGL((k_apply<double, RB, false, false, true>), dim3(B->nblocks),
+                       dim3(kThreads), 0, st, v64(B), bd, B->x, nullptr, nullptr, B->q, nullptr,
+                       nullptr, nullptr);
+    hipLaunchKernelGGL((k_batch_norms<RB>), dim3(B->nblocks), dim3(kThreads), 0, st, B->n, bd,
+                       B->b, B->q, B->norms, B->ncount, B->partials);
+}
+
+bool capture(cicy_batch *B, int method, int buf, int grid) {
+    cicy_ctx *ctx = B->ctx;
+    if (B->exec && B->exec_method == method && B->exec_grid == grid && B->exec_buf == buf)
+        return true;
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    B->exec = nullptr;
+    hipGraph_t g;
+    if (!hip_ok(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
+                "hipStreamBeginCapture"))
+        return false;
+    const int *act = B->active[buf];
+    if (method == CICY_METHOD_PCG_F64) {
+        CICY_RB_SWITCH(B->rb, chunk_f64<RB>(B, act, grid));
+    } else {
+        CICY_RB_SWITCH(B->rb, chunk_mixed<RB>(B, act, grid));
+    }
+    if (!hip_ok(ctx, hipStreamEndCapture(ctx->stream, &g), "hipStreamEndCapture")) return false;
+    hipError_t e = hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (!hip_ok(ctx, e, "hipGraphInstantiate")) return false;
+    B->exec_method = method;
+    B->exec_grid = grid;
+    B->exec_buf = buf;
+    return true;
+}
+
+// Active block list of the unfinished systems, into pinned slot `buf`; returns its length.
+int build_active(cicy_batch *B, const PcgState *hs, int buf) {
+    int m = 0;
+    for (int s = 0; s < B->nsys; s++) {
+        if (hs[s].done) continue;
+        for (int b = B->h_sys_blk0[s]; b < B->h_sys_blk0[s + 1]; b++) B->h_active[buf][m++] = b;
+    }
+    return m;
+}
+
+template <typename T>
+bool balloc(cicy_batch *B, T **p, size_t count) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (!hip_ok(B->ctx, hipMalloc((void **)p, bytes), "hipMalloc")) return false;
+    B->device_bytes += bytes;
+    return true;
+}
+
+template <typename T>
+bool bh2d(cicy_batch *B, T *dst, const T *src, size_t count) {
+    if (!count) return true;
+    return hip_ok(B->ctx, hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice,
+                                         B->ctx->stream), "H2D");
+}
+
+}  // namespace
+
+extern "C" {
+
+int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *nnz,
+                      const int64_t *const *row_ptr, const int64_t *const *col_ind,
+                      const double *const *values, const double *const *diag,
+                      cicy_batch **out) {
+    if (!ctx || !out || nsys <= 0 || !n || !nnz || !row_ptr || !col_ind || !values || !diag) {
+        if (ctx) set_error(ctx, "cicy_batch_create: invalid arguments");
+        return 0;
+    }
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    const int rb = ctx->rows_per_block;
+    std::vector<BlockedHost> hs((size_t)nsys);
+    long long tn = 0, tnnz = 0, tb = 0, trem = 0;
+    for (int s = 0; s < nsys; s++) {
+        if (!build_blocked(ctx, n[s], nnz[s], row_ptr[s], col_ind[s], values[s], diag[s], rb,
+                           hs[s])) {
+            char msg[600];
+            snprintf(msg, sizeof(msg), "system %d: %s", s, ctx->err);
+            set_error(ctx, "%s", msg);
+            return 0;
+        }
+        tn += hs[s].n;
+        tnnz += hs[s].nnz;
+        tb += hs[s].nblocks;
+        trem += hs[s].nremote;
+    }
+    if (tn >= (1LL << 31) - 1 || tnnz >= (1LL << 31) - 1 || trem >= (1LL << 31) - 1) {
+        set_error(ctx, "batch too large for 32-bit device indices");
+        return 0;
+    }
+    cicy_batch *B = new cicy_batch();
+    B->ctx = ctx;
+    B->nsys = nsys;
+    B->rb = rb;
+    B->n = (int)tn;
+    B->nnz = (int)tnnz;
+    B->nblocks = (int)tb;
+    B->nremote = (int)trem;
+    B->bnorm2.assign((size_t)nsys, -1.0);
+    std::vector<int> rp((size_t)tn + 1), col((size_t)tnnz), urp((size_t)tn + 1),
+        ucol((size_t)trem);
+    std::vector<double> val((size_t)tnnz), dg((size_t)tn), uval((size_t)trem);
+    std::vector<float> val32((size_t)tnnz), uval32((size_t)trem);
+    B->h_sys_row0.assign((size_t)nsys + 1, 0);
+    B->h_sys_blk0.assign((size_t)nsys + 1, 0);
+    int ro = 0, jo = 0, uo = 0, bo = 0;
+    for (int s = 0; s < nsys; s++) {
+        const BlockedHost &h = hs[s];
+        std::vector<float> a32, u32;
+        scaled_values(h, a32, u32);
+        for (int i = 0; i < h.n; i++) {
+            rp[ro + i] = jo + h.rp[i];
+            urp[ro + i] = uo + h.urp[i];
+            dg[ro + i] = h.diag[i];
+        }
+        for (int j = 0; j < h.nnz; j++) {
+            col[jo + j] = ro + h.col[j];
+            val[jo + j] = h.val[j];
+            val32[jo + j] = a32[j];
+        }
+        for (int j = 0; j < h.nremote; j++) {
+            ucol[uo + j] = ro + h.ucol[j];
+            uval[uo + j] = h.uval[j];
+            uval32[uo + j] = u32[j];
+        }
+        for (int k = 0; k < h.nblocks; k++) {
+            B->h_blk_sys.push_back(s);
+            B->h_blk_row0.push_back(ro + k * rb);
+        }
+        ro += h.n;
+        jo += h.nnz;
+        uo += h.nremote;
+        bo += h.nblocks;
+        B->h_sys_row0[s + 1] = ro;
+        B->h_sys_blk0[s + 1] = bo;
+    }
+    rp[tn] = jo;
+    urp[tn] = uo;
+    const size_t N = (size_t)tn, NB = (size_t)tb;
+    bool ok = balloc(B, &B->rp, N + 1) && balloc(B, &B->col, (size_t)tnnz) &&
+              balloc(B, &B->val, (size_t)tnnz) && balloc(B, &B->diag, N) &&
+              balloc(B, &B->urp, N + 1) && balloc(B, &B->ucol, (size_t)trem) &&
+              balloc(B, &B->uval, (size_t)trem) && balloc(B, &B->val32, (size_t)tnnz) &&
+              balloc(B, &B->uval32, (size_t)trem) && balloc(B, &B->blk_sys, NB) &&
+              balloc(B, &B->blk_row0, NB) && balloc(B, &B->sys_row0, (size_t)nsys + 1) &&
+              balloc(B, &B->sys_blk0, (size_t)nsys + 1) && balloc(B, &B->active[0], NB) &&
+              balloc(B, &B->active[1], NB) && balloc(B, &B->x, N) && balloc(B, &B->r, N) &&
+              balloc(B, &B->p0, N) && balloc(B, &B->p1, N) && balloc(B, &B->q, N) &&
+              balloc(B, &B->x0, N) && balloc(B, &B->r0, N) && balloc(B, &B->b, N) &&
+              balloc(B, &B->stop_ref2, (size_t)nsys) && balloc(B, &B->rs, N) &&
+              balloc(B, &B->xs, N) && balloc(B, &B->ps0, N) && balloc(B, &B->ps1, N) &&
+              balloc(B, &B->qs, N) && balloc(B, &B->partials, 2 * NB) &&
+              balloc(B, &B->norms, 2 * (size_t)nsys) && balloc(B, &B->st, (size_t)nsys) &&
+              balloc(B, &B->mix, (size_t)nsys) && balloc(B, &B->ncount, (size_t)nsys);
+    ok = ok && hip_ok(ctx, hipHostMalloc((void **)&B->h_active[0], NB * sizeof(int),
+                                         hipHostMallocDefault), "hipHostMalloc") &&
+         hip_ok(ctx, hipHostMalloc((void **)&B->h_active[1], NB * sizeof(int),
+                                   hipHostMallocDefault), "hipHostMalloc") &&
+         hip_ok(ctx, hipHostMalloc((void **)&B->h_st[0], nsys * sizeof(PcgState),
+                                   hipHostMallocDefault), "hipHostMalloc") &&
+         hip_ok(ctx, hipHostMalloc((void **)&B->h_st[1], nsys * sizeof(PcgState),
+                                   hipHostMallocDefault), "hipHostMalloc");
+    ok = ok && bh2d(B, B->rp, rp.data(), N + 1) && bh2d(B, B->col, col.data(), (size_t)tnnz) &&
+         bh2d(B, B->val, val.data(), (size_t)tnnz) && bh2d(B, B->diag, dg.data(), N) &&
+         bh2d(B, B->urp, urp.data(), N + 1) && bh2d(B, B->ucol, ucol.data(), (size_t)trem) &&
+         bh2d(B, B->uval, uval.data(), (size_t)trem) &&
+         bh2d(B, B->val32, val32.data(), (size_t)tnnz) &&
+         bh2d(B, B->uval32, uval32.data(), (size_t)trem) &&
+         bh2d(B, B->blk_sys, B->h_blk_sys.data(), NB) &&
+         bh2d(B, B->blk_row0, B->h_blk_row0.data(), NB) &&
+         bh2d(B, B->sys_row0, B->h_sys_row0.data(), (size_t)nsys + 1) &&
+         bh2d(B, B->sys_blk0, B->h_sys_blk0.data(), (size_t)nsys + 1) &&
+         hip_ok(ctx, hipMemsetAsync(B->x0, 0, N * sizeof(double), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(B->r0, 0, N * sizeof(double), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(B->b, 0, N * sizeof(double), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(B->xs, 0, N * sizeof(float), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(B->ncount, 0, nsys * sizeof(unsigned), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(B->mix, 0, nsys * sizeof(MixState), ctx->stream), "memset") &&
+         hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (!ok) {
+        cicy_batch_destroy(B);
+        return 0;
+    }
+    *out = B;
+    return 1;
+}
+
+void cicy_batch_destroy(cicy_batch *B) {
+    if (!B) return;
+    (void)hipSetDevice(B->ctx->device);
+    (void)hipStreamSynchronize(B->ctx->stream);
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    void *dev[] = {B->rp, B->col, B->urp, B->ucol, B->val, B->diag, B->uval, B->val32, B->uval32,
+                   B->blk_sys, B->blk_row0, B->sys_row0, B->sys_blk0, B->active[0],
+                   B->active[1], B->x, B->r, B->p0, B->p1, B->q, B->x0, B->r0, B->b,
+                   B->stop_ref2, B->rs, B->xs, B->ps0, B->ps1, B->qs, B->partials, B->norms,
+                   B->st, B->mix, B->ncount};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    void *host[] = {B->h_active[0], B->h_active[1], B->h_st[0], B->h_st[1]};
+    for (void *p : host)
+        if (p) (void)hipHostFree(p);
+    delete B;
+}
+
+int64_t cicy_batch_size(const cicy_batch *B, int what) {
+    if (!B) return -1;
+    switch (what) {
+        case 0: return B->n;
+        case 1: return B->nnz;
+        case 2: return B->nremote;
+        case 3: return (int64_t)B->device_bytes;
+        case 4: return B->nsys;
+        case 5: return B->nblocks;
+        default: return -1;
+    }
+}
+
+int cicy_batch_set_rhs(cicy_batch *B, int sys, const double *r0, const double *x0,
+                       const double *b) {
+    if (!B || sys < 0 || sys >= B->nsys || !r0) return 0;
+    cicy_ctx *ctx = B->ctx;
+    (void)hipSetDevice(ctx->device);
+    const size_t off = (size_t)B->h_sys_row0[sys];
+    const size_t ns = (size_t)(B->h_sys_row0[sys + 1] - B->h_sys_row0[sys]);
+    bool ok = bh2d(B, B->r0 + off, r0, ns);
+    if (x0) ok = ok && bh2d(B, B->x0 + off, x0, ns);
+    else ok = ok && hip_ok(ctx, hipMemsetAsync(B->x0 + off, 0, ns * sizeof(double), ctx->stream), "memset");
+    // b is needed for the true residual; with x0 == 0 it equals r0
+    const double *bb = b ? b : (x0 ? nullptr : r0);
+    if (!bb) {
+        set_error(ctx, "cicy_batch_set_rhs: b is required when x0 is given");
+        return 0;
+    }
+    ok = ok && bh2d(B, B->b + off, bb, ns);
+    double nb2 = 0.0;
+    for (size_t i = 0; i < ns; i++) nb2 += bb[i] * bb[i];
+    B->bnorm2[sys] = nb2;
+    ok = ok && hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return ok ? 1 : 0;
+}
+
+int cicy_batch_solve(cicy_batch *B, int method, double tol, int64_t max_iter, int rel_to,
+                     cicy_solve_stats *per_sys, double *total_ms) {
+    if (!B) return 0;
+    cicy_ctx *ctx = B->ctx;
+    (void)hipSetDevice(ctx->device);
+    if (method != CICY_METHOD_PCG_F64 && method != CICY_METHOD_MIXED) {
+        set_error(ctx, "unknown method %d", method);
+        return 0;
+    }
+    if (max_iter <= 0) max_iter = 300LL * ((long long)B->n + 2);
+    hipStream_t st = ctx->stream;
+    const size_t N = (size_t)B->n;
+    std::vector<double> ref2((size_t)B->nsys, -1.0);
+    if (rel_to == CICY_REL_B) ref2 = B->bnorm2;
+    if (!hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "event") ||
+        !hip_ok(ctx, hipMemcpyAsync(B->stop_ref2, ref2.data(), ref2.size() * sizeof(double),
+                                    hipMemcpyHostToDevice, st), "H2D") ||
+        !hip_ok(ctx, hipMemcpyAsync(B->r, B->r0, N * sizeof(double), hipMemcpyDeviceToDevice, st),
+                "copy r0") ||
+        !hip_ok(ctx, hipMemcpyAsync(B->x, B->x0, N * sizeof(double), hipMemcpyDeviceToDevice, st),
+                "copy x0") ||
+        !hip_ok(ctx, hipMemsetAsync(B->st, 0, B->nsys * sizeof(PcgState), st), "memset"))
+        return 0;
+    CICY_RB_SWITCH(B->rb, init_all<RB>(B, method, tol, max_iter));
+    if (!hip_ok(ctx, hipStreamSynchronize(st), "sync")) return 0;  // ref2 is pageable
+    // initial active list: every block of every system not already converged
+    if (!hip_ok(ctx, hipMemcpy(B->h_st[0], B->st, B->nsys * sizeof(PcgState),
+                               hipMemcpyDeviceToHost), "D2H"))
+        return 0;
+    int buf = 0;
+    int grid = build_active(B, B->h_st[0], buf);
+    double max_s = 600.0;
+    if (const char *e = getenv("CICY_MAX_SOLVE_SECONDS")) max_s = atof(e);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (grid > 0) {
+        if (!bh2d(B, B->active[buf], B->h_active[buf], (size_t)grid) ||
+            !capture(B, method, buf, grid))
+            return 0;
+        // keep one chunk queued ahead of the poll
+        int slot = 0;
+        if (!hip_ok(ctx, hipGraphLaunch(B->exec, st), "hipGraphLaunch") ||
+            !hip_ok(ctx, hipMemcpyAsync(B->h_st[slot], B->st, B->nsys * sizeof(PcgState),
+                                        hipMemcpyDeviceToHost, st), "poll") ||
+            !hip_ok(ctx, hipEventRecord(ctx->ev_poll[slot], st), "event"))
+            return 0;
+        for (;;) {
+            // queue the next chunk before reading the previous poll
+            const int nslot = slot ^ 1;
+            if (!hip_ok(ctx, hipGraphLaunch(B->exec, st), "hipGraphLaunch") ||
+                !hip_ok(ctx, hipMemcpyAsync(B->h_st[nslot], B->st, B->nsys * sizeof(PcgState),
+                                            hipMemcpyDeviceToHost, st), "poll") ||
+                !hip_ok(ctx, hipEventRecord(ctx->ev_poll[nslot], st), "event"))
+                return 0;
+            if (!hip_ok(ctx, hipEventSynchronize(ctx->ev_poll[slot]), "event sync")) return 0;
+            const PcgState *hs = B->h_st[slot];
+            int live_blocks = 0;
+            for (int s = 0; s < B->nsys; s++)
+                if (!hs[s].done) live_blocks += B->h_sys_blk0[s + 1] - B->h_sys_blk0[s];
+            if (live_blocks == 0) break;
+            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >
+                max_s) {
+                set_error(ctx, "batch solve stopped after %.0f s (CICY_MAX_SOLVE_SECONDS)", max_s);
+                break;
+            }
+            if (live_blocks < grid * 7 / 10) {
+                // compact into the other list; the queued chunk still reads the current one
+                const int nb = buf ^ 1;
+                grid = build_active(B, hs, nb);
+                if (!bh2d(B, B->active[nb], B->h_active[nb], (size_t)grid) ||
+                    !capture(B, method, nb, grid))
+                    return 0;
+                buf = nb;
+            }
+            slot = nslot;
+        }
+    }
+    if (!hip_ok(ctx, hipStreamSynchronize(st), "sync")) return 0;
+    CICY_RB_SWITCH(B->rb, final_norms<RB>(B));
+    if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "event") ||
+        !hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "sync"))
+        return 0;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    if (total_ms) *total_ms = ms;
+    std::vector<PcgState> hs((size_t)B->nsys);
+    std::vector<MixState> hm((size_t)B->nsys);
+    std::vector<double> norms(2 * (size_t)B->nsys);
+    if (!hip_ok(ctx, hipMemcpy(hs.data(), B->st, hs.size() * sizeof(PcgState),
+                               hipMemcpyDeviceToHost), "D2H") ||
+        !hip_ok(ctx, hipMemcpy(hm.data(), B->mix, hm.size() * sizeof(MixState),
+                               hipMemcpyDeviceToHost), "D2H") ||
+        !hip_ok(ctx, hipMemcpy(norms.data(), B->norms, norms.size() * sizeof(double),
+                               hipMemcpyDeviceToHost), "D2H"))
+        return 0;
+    if (per_sys) {
+        for (int s = 0; s < B->nsys; s++) {
+            cicy_solve_stats &o = per_sys[s];
+            memset(&o, 0, sizeof(o));
+            o.iters = hs[s].iters;
+            o.status = hs[s].status;
+            o.refinements = method == CICY_METHOD_MIXED ? hm[s].refinements : 0;
+            o.rel_res = hs[s].rr_init > 0 ? std::sqrt((method == CICY_METHOD_MIXED ? hm[s].rtrue2
+                                                                               : hs[s].rr) /
+                                                      hs[s].rr_init)
+                                          : 0.0;
+            o.true_rel_res = norms[2 * s + 1] > 0 ? std::sqrt(norms[2 * s] / norms[2 * s + 1])
+                                                  : std::sqrt(norms[2 * s]);
+            o.solve_ms = ms;
+        }
+    }
+    return 1;
+}
+
+int cicy_batch_get(cicy_batch *B, int sys, double *x, double *r) {
+    if (!B || sys < 0 || sys >= B->nsys) return 0;
+    cicy_ctx *ctx = B->ctx;
+    (void)hipSetDevice(ctx->device);
+    const size_t off = (size_t)B->h_sys_row0[sys];
+    const size_t ns = (size_t)(B->h_sys_row0[sys + 1] - B->h_sys_row0[sys]);
+    bool ok = true;
+    if (x && ns)
+        ok = ok && hip_ok(ctx, hipMemcpyAsync(x, B->x + off, ns * sizeof(double),
+                                              hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    if (r && ns)
+        ok = ok && hip_ok(ctx, hipMemcpyAsync(r, B->r + off, ns * sizeof(double),
+                                              hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    ok = ok && hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return ok ? 1 : 0;
+}
+
+}  // extern "C"

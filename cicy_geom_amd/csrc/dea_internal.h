@@ -11,19 +11,18 @@ namespace cicy {
 
 // iterations per captured hipGraph chunk (even: the p double buffer returns to parity 0)
 constexpr int kChunk = 32;
+// mixed precision: reliable-update trigger r^.r^ <= kDelta^2 * (r^.r^ at the last update),
+// and the number of updates in a row without halving ||b - A x|| that ends a solve
+constexpr double kDelta = 0.1;
+constexpr int kStallLimit = 8;
 
-// Mixed-precision state beyond PcgState (device, one per system).
-struct MixState {
-    double rr_prev;     // scaled recursive r.r before the last update
-    double m2;          // scaled r.r right after the last reliable update
-    double tgt2;        // scaled r.r at which the stopping rule is expected to hold
-    double rtrue2;      // ||b - A x||^2 at the last reliable update
-    double delta2;      // update when r.r <= delta2 * m2
-    int need_update;
-    int refinements;
-    unsigned int counter;
-    int pad;
-};
+#define CICY_RB_SWITCH(rb, CALL)                              \
+    switch (rb) {                                             \
+        case 128: { constexpr int RB = 128; CALL; } break;    \
+        case 256: { constexpr int RB = 256; CALL; } break;    \
+        case 512: { constexpr int RB = 512; CALL; } break;    \
+        default: { constexpr int RB = 1024; CALL; } break;    \
+    }
 
 struct BlockedHost {
     int n = 0, nnz = 0, rb = 256, nblocks = 0, nremote = 0;
@@ -85,5 +84,6 @@ void mixed_free(cicy_sys *s);
 bool solve_mixed(cicy_sys *s, double tol, long long max_iter, int rel_to,
                  cicy_solve_stats *stats);
 void mixed_spmv32(cicy_sys *s);
+void scaled_values(const BlockedHost &h, std::vector<float> &v32, std::vector<float> &u32);
 
 }  // namespace cicy

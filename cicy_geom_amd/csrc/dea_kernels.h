@@ -18,12 +18,18 @@
 // The PCG iteration is two launches:
 //   k_apply<PCG>:  p = M^-1 r + beta p_old (own rows into LDS and HBM; gathered columns
 //                  recompute it from r, diag, p_old), q = A p, partial p.q; the last
-//                  work-group to finish reduces the partials in a fixed order and sets
-//                  alpha = rz / p.q.
+//                  work-group of the system to finish reduces the partials in a fixed
+//                  order and sets alpha = rz / p.q.
 //   k_update:      x += alpha p, r -= alpha q, z = M^-1 r, partial r.z and r.r; the last
 //                  work-group sets beta, the iteration count and the convergence flag.
 // Both return at once when the flag is set, so chunks of iterations are captured in a
 // hipGraph and launched without a host round trip per iteration.
+//
+// Batches (BATCH=true): many independent systems are stored one after another (block-
+// diagonal), every row block belongs to one system, and a block table maps a work-group
+// to (system, first row).  Each system has its own PcgState (alpha, beta, flags) and
+// reduction counter, so systems of different size converge independently inside one
+// launch; an optional `active` list restricts a launch to the blocks of unfinished systems.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -44,6 +50,19 @@ struct PcgState {
     int pad1;
 };
 
+// Mixed-precision state (one per system).
+struct MixState {
+    double rr_prev;     // scaled recursive r.r before the last update
+    double m2;          // scaled r.r right after the last reliable update
+    double tgt2;        // scaled r.r at which the stopping rule is expected to hold
+    double rtrue2;      // ||b - A x||^2 at the last reliable update
+    double delta2;      // update when r.r <= delta2 * m2
+    int need_update;
+    int refinements;
+    unsigned int counter;
+    int stall;
+};
+
 template <typename T>
 struct SssDev {
     int n, nblocks;
@@ -55,6 +74,39 @@ struct SssDev {
     const int *__restrict__ ucol;
     const T *__restrict__ uval;
 };
+
+struct BatchDev {
+    const int *active;    // [grid] block ids of this launch (nullptr: blockIdx.x)
+    const int *blk_sys;   // [blocks] system of each block
+    const int *blk_row0;  // [blocks] first (global) row of each block
+    const int *sys_row0;  // [nsys+1] row offsets
+    const int *sys_blk0;  // [nsys+1] block offsets
+};
+
+struct Blk {
+    int b, sys, r0, nrows, first, count;
+};
+
+template <int RB, bool BATCH>
+__device__ __forceinline__ Blk block_info(const BatchDev &bd, int n) {
+    Blk k;
+    if (!BATCH) {
+        k.b = blockIdx.x;
+        k.sys = 0;
+        k.r0 = k.b * RB;
+        k.nrows = min(RB, n - k.r0);
+        k.first = 0;
+        k.count = gridDim.x;
+    } else {
+        k.b = bd.active ? bd.active[blockIdx.x] : (int)blockIdx.x;
+        k.sys = bd.blk_sys[k.b];
+        k.r0 = bd.blk_row0[k.b];
+        k.nrows = min(RB, bd.sys_row0[k.sys + 1] - k.r0);
+        k.first = bd.sys_blk0[k.sys];
+        k.count = bd.sys_blk0[k.sys + 1] - k.first;
+    }
+    return k;
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -91,14 +143,15 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *lds8) {
     }
 }
 
-// Publishes this work-group's partial(s) and returns true in every thread of the last
-// work-group to arrive, after which all partials are visible to it.
-__device__ __forceinline__ bool arrive_last(unsigned int *counter, int *lds_flag) {
+// Publishes this work-group's partial(s) and returns true in every thread of the last of
+// `expected` work-groups to arrive, after which all their partials are visible to it.
+__device__ __forceinline__ bool arrive_last(unsigned int *counter, unsigned int expected,
+                                            int *lds_flag) {
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
         unsigned int v = atomicAdd(counter, 1u);
-        *lds_flag = (v == gridDim.x - 1) ? 1 : 0;
+        *lds_flag = (v == expected - 1) ? 1 : 0;
     }
     __syncthreads();
     bool last = *lds_flag != 0;
@@ -128,25 +181,30 @@ __device__ __forceinline__ T pdir_unit(T r, T beta, T pold) {
 }
 
 #ifdef __HIPCC__
-// q = A p over one row block; the guard, when given, skips the launch unless *guard != 0.
+
+// q = A p over one row block.
 // PCG: p = M^-1 r + beta p_old is formed on the fly (own rows are also written to pnew)
-// and the block's p.q partial is reduced into alpha by the last work-group.
+// and the block's p.q partial is reduced into alpha by the system's last work-group.
 // SPMV: p = src.  UNIT: unit diagonal (Jacobi-scaled system), M^-1 = I.
-template <typename T, int RB, bool PCG, bool UNIT>
-__global__ __launch_bounds__(kThreads) void k_apply(SssDev<T> s, const T *__restrict__ src,
+// guard: when given, the launch does nothing for systems whose need_update flag is 0.
+template <typename T, int RB, bool PCG, bool UNIT, bool BATCH>
+__global__ __launch_bounds__(kThreads) void k_apply(SssDev<T> s, BatchDev bd,
+                                                    const T *__restrict__ src,
                                                     const T *__restrict__ rvec,
                                                     T *__restrict__ pnew, T *__restrict__ q,
-                                                    PcgState *__restrict__ st,
+                                                    PcgState *__restrict__ st_base,
                                                     double *__restrict__ partials,
-                                                    const int *__restrict__ guard) {
+                                                    const MixState *__restrict__ guard) {
+    const Blk k = block_info<RB, BATCH>(bd, s.n);
+    PcgState *st = st_base ? st_base + k.sys : nullptr;
     if (PCG && st->done) return;
-    if (guard && !*guard) return;
+    if (guard && !guard[k.sys].need_update) return;
     __shared__ T sp[RB];
     __shared__ T sy[RB];
     __shared__ double red[8];
     __shared__ int flag;
-    const int r0 = blockIdx.x * RB;
-    const int nrows = min(RB, s.n - r0);
+    const int r0 = k.r0;
+    const int nrows = k.nrows;
     const T beta = PCG ? (T)st->beta : (T)0;
     auto pval = [&](int i) -> T {
         if (!PCG) return src[i];
@@ -189,22 +247,349 @@ __global__ __launch_bounds__(kThreads) void k_apply(SssDev<T> s, const T *__rest
     }
     if (!PCG) return;
     part = block_sum(part, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = part;
-    if (arrive_last(&st->counter, &flag)) {
-        double pq = reduce_partials(partials, gridDim.x, 1, red);
+    if (threadIdx.x == 0) partials[k.b] = part;
+    if (arrive_last(&st->counter, k.count, &flag)) {
+        double pq = reduce_partials(partials + k.first, k.count, 1, red);
         if (threadIdx.x == 0) {
             double alpha = st->rz / pq;
             st->pq = pq;
             st->alpha = alpha;
             st->counter = 0;
             if (!(pq > 0.0) || !isfinite(alpha)) {
-                st->status = CICY_BREAKDOWN;
+                st->status = 2;  // CICY_BREAKDOWN
                 st->done = 1;
             }
         }
     }
 }
 
-#endif
+// x += alpha p; r -= alpha q; partial r.z, r.r; the last work-group closes the iteration:
+// convergence test sqrt(r.r) <= tol*||r_0|| (Direct_Electrifying.cpp:1619-1620), then
+// beta = r.z / rz_old (:1627-1632) and the iteration cap (:1559).
+template <int RB, bool BATCH>
+__global__ __launch_bounds__(kThreads) void k_update_f64(int n, BatchDev bd,
+                                                         const double *__restrict__ diag,
+                                                         double *__restrict__ x,
+                                                         double *__restrict__ r,
+                                                         const double *__restrict__ p,
+                                                         const double *__restrict__ q,
+                                                         PcgState *__restrict__ st_base,
+                                                         double *__restrict__ partials) {
+    const Blk k = block_info<RB, BATCH>(bd, n);
+    PcgState *st = st_base + k.sys;
+    if (st->done) return;
+    __shared__ double red[8];
+    __shared__ int flag;
+    const double alpha = st->alpha;
+    double prz = 0.0, prr = 0.0;
+    for (int t = threadIdx.x; t < k.nrows; t += kThreads) {
+        const int i = k.r0 + t;
+        x[i] = fma(alpha, p[i], x[i]);
+        const double rn = fma(-alpha, q[i], r[i]);
+        r[i] = rn;
+        const double z = (1.0 / diag[i]) * rn;
+        prz += z * rn;
+        prr += rn * rn;
+    }
+    block_sum2(prz, prr, red);
+    if (threadIdx.x == 0) {
+        partials[2 * k.b] = prz;
+        partials[2 * k.b + 1] = prr;
+    }
+    if (arrive_last(&st->counter, k.count, &flag)) {
+        double rz = reduce_partials(partials + 2 * k.first, k.count, 2, red);
+        double rr = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
+        if (threadIdx.x == 0) {
+            st->counter = 0;
+            st->iters += 1;
+            st->rr = rr;
+            if (rr <= st->stop2) {
+                st->status = 0;  // CICY_OK
+                st->done = 1;
+            } else if (!isfinite(rr)) {
+                st->status = 2;
+                st->done = 1;
+            } else {
+                st->beta = rz / st->rz;
+                st->rz = rz;
+                if (st->iters >= st->max_iter) {
+                    st->status = 1;  // CICY_MAXITER
+                    st->done = 1;
+                }
+            }
+        }
+    }
+}
+
+// Start of a solve: rz = r.(M^-1 r), rr = r.r, stopping threshold.  The reference norm
+// is stop_ref2[sys] (or stop_ref2_all when stop_ref2 is null); a negative value means
+// "relative to ||r_0||" (the reference's rule), otherwise it is ||b||^2.  Also zeroes both p buffers so that the first direction is p = M^-1 r.
+template <int RB, bool BATCH>
+__global__ __launch_bounds__(kThreads) void k_init_f64(int n, BatchDev bd,
+                                                       const double *__restrict__ diag,
+                                                       const double *__restrict__ r,
+                                                       double *__restrict__ p0,
+                                                       double *__restrict__ p1,
+                                                       PcgState *__restrict__ st_base,
+                                                       double *__restrict__ partials,
+                                                       double tol,
+                                                       const double *__restrict__ stop_ref2,
+                                                       double stop_ref2_all,
+                                                       long long max_iter) {
+    const Blk k = block_info<RB, BATCH>(bd, n);
+    PcgState *st = st_base + k.sys;
+    __shared__ double red[8];
+    __shared__ int flag;
+    double prz = 0.0, prr = 0.0;
+    for (int t = threadIdx.x; t < k.nrows; t += kThreads) {
+        const int i = k.r0 + t;
+        const double ri = r[i];
+        const double z = (1.0 / diag[i]) * ri;
+        p0[i] = 0.0;
+        p1[i] = 0.0;
+        prz += z * ri;
+        prr += ri * ri;
+    }
+    block_sum2(prz, prr, red);
+    if (threadIdx.x == 0) {
+        partials[2 * k.b] = prz;
+        partials[2 * k.b + 1] = prr;
+    }
+    if (arrive_last(&st->counter, k.count, &flag)) {
+        double rz = reduce_partials(partials + 2 * k.first, k.count, 2, red);
+        double rr = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
+        if (threadIdx.x == 0) {
+            st->counter = 0;
+            st->rz = rz;
+            st->rr = rr;
+            st->rr_init = rr;
+            const double sr = stop_ref2 ? stop_ref2[k.sys] : stop_ref2_all;
+            double ref2 = sr >= 0.0 ? sr : rr;
+            double thr = tol * sqrt(ref2);
+            st->stop2 = thr * thr;
+            st->alpha = 0.0;
+            st->beta = 0.0;
+            st->pq = 0.0;
+            st->iters = 0;
+            st->max_iter = max_iter;
+            st->status = 1;
+            st->done = 0;
+            if (rr == 0.0) {  // exact initial guess
+                st->status = 0;
+                st->done = 1;
+            }
+        }
+    }
+}
+
+// ---- mixed precision (FP32 inner CG on the Jacobi-scaled system, FP64 reliable updates)
+
+template <int RB, bool BATCH>
+__global__ __launch_bounds__(kThreads) void k_mixed_init(int n, BatchDev bd,
+                                                         const double *__restrict__ diag,
+                                                         const double *__restrict__ r0v,
+                                                         float *__restrict__ rs,
+                                                         float *__restrict__ xs,
+                                                         float *__restrict__ ps0,
+                                                         float *__restrict__ ps1,
+                                                         PcgState *__restrict__ st_base,
+                                                         MixState *__restrict__ mx_base,
+                                                         double *__restrict__ partials,
+                                                         double tol,
+                                                         const double *__restrict__ stop_ref2,
+                                                         double stop_ref2_all,
+                                                         long long max_iter, double delta2) {
+    const Blk k = block_info<RB, BATCH>(bd, n);
+    PcgState *st = st_base + k.sys;
+    MixState *mx = mx_base + k.sys;
+    __shared__ double red[8];
+    __shared__ int flag;
+    double a = 0.0, t = 0.0;
+    for (int j = threadIdx.x; j < k.nrows; j += kThreads) {
+        const int i = k.r0 + j;
+        const double ri = r0v[i];
+        const float v = (float)(ri * (1.0 / sqrt(diag[i])));
+        rs[i] = v;
+        xs[i] = 0.f;
+        ps0[i] = 0.f;
+        ps1[i] = 0.f;
+        a += (double)v * (double)v;
+        t += ri * ri;
+    }
+    block_sum2(a, t, red);
+    if (threadIdx.x == 0) {
+        partials[2 * k.b] = a;
+        partials[2 * k.b + 1] = t;
+    }
+    if (arrive_last(&st->counter, k.count, &flag)) {
+        double rs2 = reduce_partials(partials + 2 * k.first, k.count, 2, red);
+        double r02 = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
+        if (threadIdx.x == 0) {
+            st->counter = 0;
+            st->rz = rs2;
+            st->rr = rs2;
+            st->rr_init = r02;
+            const double sr = stop_ref2 ? stop_ref2[k.sys] : stop_ref2_all;
+            double ref2 = sr >= 0.0 ? sr : r02;
+            double thr = tol * sqrt(ref2);
+            st->stop2 = thr * thr;
+            st->alpha = st->beta = st->pq = 0.0;
+            st->iters = 0;
+            st->max_iter = max_iter;
+            st->status = 1;
+            st->done = 0;
+            mx->rr_prev = rs2;
+            mx->m2 = rs2;
+            mx->rtrue2 = r02;
+            mx->tgt2 = r02 > 0.0 ? rs2 * (st->stop2 / r02) : 0.0;
+            mx->delta2 = delta2;
+            mx->need_update = 0;
+            mx->refinements = 0;
+            mx->counter = 0;
+            mx->stall = 0;
+            if (r02 <= st->stop2) {
+                st->status = 0;
+                st->done = 1;
+            }
+        }
+    }
+}
+
+// x^ += alpha p; r^ -= alpha q (FP32); partial r^.r^ (FP64).  The last work-group sets
+// beta, counts the iteration and raises the reliable-update flag.
+template <int RB, bool BATCH>
+__global__ __launch_bounds__(kThreads) void k_update_mixed(int n, BatchDev bd,
+                                                           float *__restrict__ xs,
+                                                           float *__restrict__ rs,
+                                                           const float *__restrict__ p,
+                                                           const float *__restrict__ q,
+                                                           PcgState *__restrict__ st_base,
+                                                           MixState *__restrict__ mx_base,
+                                                           double *__restrict__ partials) {
+    const Blk k = block_info<RB, BATCH>(bd, n);
+    PcgState *st = st_base + k.sys;
+    if (st->done) return;
+    MixState *mx = mx_base + k.sys;
+    __shared__ double red[8];
+    __shared__ int flag;
+    const float alpha = (float)st->alpha;
+    double prr = 0.0;
+    for (int j = threadIdx.x; j < k.nrows; j += kThreads) {
+        const int i = k.r0 + j;
+        xs[i] = fmaf(alpha, p[i], xs[i]);
+        const float rn = fmaf(-alpha, q[i], rs[i]);
+        rs[i] = rn;
+        prr += (double)rn * (double)rn;
+    }
+    prr = block_sum(prr, red);
+    if (threadIdx.x == 0) partials[k.b] = prr;
+    if (arrive_last(&st->counter, k.count, &flag)) {
+        double rr = reduce_partials(partials + k.first, k.count, 1, red);
+        if (threadIdx.x == 0) {
+            st->counter = 0;
+            st->iters += 1;
+            if (!isfinite(rr)) {
+                st->status = 2;
+                st->done = 1;
+            } else {
+                st->beta = rr / st->rz;
+                mx->rr_prev = st->rz;
+                st->rz = rr;
+                st->rr = rr;
+                if (rr <= mx->delta2 * mx->m2 || rr <= mx->tgt2) mx->need_update = 1;
+                if (st->iters >= st->max_iter) {
+                    // fold what has been accumulated and stop
+                    mx->need_update = 1;
+                    st->done = 1;
+                    st->status = 1;
+                }
+            }
+        }
+    }
+}
+
+// x += D^-1/2 x^; x^ = 0 (only for systems with the update flag unless `always`).
+template <int RB, bool BATCH>
+__global__ __launch_bounds__(kThreads) void k_fold(int n, BatchDev bd,
+                                                   const double *__restrict__ diag,
+                                                   double *__restrict__ x, float *__restrict__ xs,
+                                                   const MixState *__restrict__ mx_base,
+                                                   int always) {
+    const Blk k = block_info<RB, BATCH>(bd, n);
+    if (!always && !mx_base[k.sys].need_update) return;
+    for (int j = threadIdx.x; j < k.nrows; j += kThreads) {
+        const int i = k.r0 + j;
+        const float c = xs[i];
+        if (c != 0.f) x[i] = fma((double)c, 1.0 / sqrt(diag[i]), x[i]);
+        xs[i] = 0.f;
+    }
+}
+
+// r = b - A x (FP64, ax = A x already formed), r^ = D^-1/2 r in FP32; the last work-group
+// tests the stopping rule on ||r|| and restarts the recursion from r^ (same direction p).
+template <int RB, bool BATCH>
+__global__ __launch_bounds__(kThreads) void k_reliable_resid(int n, BatchDev bd,
+                                                             const double *__restrict__ diag,
+                                                             const double *__restrict__ b,
+                                                             const double *__restrict__ ax,
+                                                             double *__restrict__ r,
+                                                             float *__restrict__ rs,
+                                                             PcgState *__restrict__ st_base,
+                                                             MixState *__restrict__ mx_base,
+                                                             double *__restrict__ partials,
+                                                             int stall_limit) {
+    const Blk k = block_info<RB, BATCH>(bd, n);
+    MixState *mx = mx_base + k.sys;
+    if (!mx->need_update) return;
+    PcgState *st = st_base + k.sys;
+    __shared__ double red[8];
+    __shared__ int flag;
+    double a = 0.0, t = 0.0;
+    for (int j = threadIdx.x; j < k.nrows; j += kThreads) {
+        const int i = k.r0 + j;
+        const double ri = b[i] - ax[i];
+        r[i] = ri;
+        const float v = (float)(ri * (1.0 / sqrt(diag[i])));
+        rs[i] = v;
+        a += (double)v * (double)v;
+        t += ri * ri;
+    }
+    block_sum2(a, t, red);
+    if (threadIdx.x == 0) {
+        partials[2 * k.b] = a;
+        partials[2 * k.b + 1] = t;
+    }
+    if (arrive_last(&mx->counter, k.count, &flag)) {
+        double rs2 = reduce_partials(partials + 2 * k.first, k.count, 2, red);
+        double rt2 = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
+        if (threadIdx.x == 0) {
+            mx->counter = 0;
+            mx->refinements += 1;
+            mx->need_update = 0;
+            if (rt2 > 0.25 * mx->rtrue2) mx->stall += 1;
+            else mx->stall = 0;
+            mx->rtrue2 = rt2;
+            st->rr = rt2;
+            if (rt2 <= st->stop2) {
+                st->status = 0;
+                st->done = 1;
+            } else if (!isfinite(rt2)) {
+                st->status = 2;
+                st->done = 1;
+            } else if (!st->done) {
+                st->beta = mx->rr_prev > 0.0 ? rs2 / mx->rr_prev : 0.0;
+                st->rz = rs2;
+                mx->m2 = rs2;
+                mx->tgt2 = rs2 * (st->stop2 / rt2);
+                if (mx->stall >= stall_limit) {
+                    st->status = 1;
+                    st->done = 1;
+                }
+            }
+        }
+    }
+}
+
+#endif  // __HIPCC__
 
 }  // namespace cicy

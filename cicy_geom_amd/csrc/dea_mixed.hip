@@ -19,210 +19,13 @@
 #include "dea_internal.h"
 
 #include <cmath>
-#include <cstddef>
 #include <cstring>
 
 using namespace cicy;
 
 namespace {
 
-constexpr double kDelta = 0.1;     // reliable-update trigger: r^.r^ <= delta^2 * m2
-constexpr int kStallLimit = 8;     // updates in a row without halving ||b - A x||
-
-template <int RB>
-__global__ __launch_bounds__(kThreads) void k_mixed_init(int n, const double *__restrict__ diag,
-                                                         const double *__restrict__ r0,
-                                                         float *__restrict__ rs,
-                                                         float *__restrict__ xs,
-                                                         float *__restrict__ ps0,
-                                                         float *__restrict__ ps1,
-                                                         PcgState *__restrict__ st,
-                                                         MixState *__restrict__ mx,
-                                                         double *__restrict__ partials,
-                                                         double tol, double stop_ref2,
-                                                         long long max_iter) {
-    __shared__ double red[8];
-    __shared__ int flag;
-    const int b0 = blockIdx.x * RB;
-    const int nrows = min(RB, n - b0);
-    double a = 0.0, t = 0.0;
-    for (int k = threadIdx.x; k < nrows; k += kThreads) {
-        const int i = b0 + k;
-        const double ri = r0[i];
-        const float v = (float)(ri * (1.0 / sqrt(diag[i])));
-        rs[i] = v;
-        xs[i] = 0.f;
-        ps0[i] = 0.f;
-        ps1[i] = 0.f;
-        a += (double)v * (double)v;
-        t += ri * ri;
-    }
-    block_sum2(a, t, red);
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = a;
-        partials[2 * blockIdx.x + 1] = t;
-    }
-    if (arrive_last(&st->counter, &flag)) {
-        double rs2 = reduce_partials(partials, gridDim.x, 2, red);
-        double r02 = reduce_partials(partials + 1, gridDim.x, 2, red);
-        if (threadIdx.x == 0) {
-            st->counter = 0;
-            st->rz = rs2;
-            st->rr = rs2;
-            st->rr_init = r02;
-            double ref2 = stop_ref2 < 0.0 ? r02 : stop_ref2;
-            double thr = tol * sqrt(ref2);
-            st->stop2 = thr * thr;
-            st->alpha = st->beta = st->pq = 0.0;
-            st->iters = 0;
-            st->max_iter = max_iter;
-            st->status = CICY_MAXITER;
-            st->done = 0;
-            mx->rr_prev = rs2;
-            mx->m2 = rs2;
-            mx->rtrue2 = r02;
-            mx->tgt2 = r02 > 0.0 ? rs2 * (st->stop2 / r02) : 0.0;
-            mx->delta2 = kDelta * kDelta;
-            mx->need_update = 0;
-            mx->refinements = 0;
-            mx->counter = 0;
-            mx->pad = 0;
-            if (r02 <= st->stop2) {
-                st->status = CICY_OK;
-                st->done = 1;
-            }
-        }
-    }
-}
-
-// x^ += alpha p; r^ -= alpha q (FP32); partial r^.r^ (FP64).  The last work-group sets
-// beta, counts the iteration and raises the reliable-update flag.
-template <int RB>
-__global__ __launch_bounds__(kThreads) void k_update_mixed(int n, float *__restrict__ xs,
-                                                           float *__restrict__ rs,
-                                                           const float *__restrict__ p,
-                                                           const float *__restrict__ q,
-                                                           PcgState *__restrict__ st,
-                                                           MixState *__restrict__ mx,
-                                                           double *__restrict__ partials) {
-    if (st->done) return;
-    __shared__ double red[8];
-    __shared__ int flag;
-    const float alpha = (float)st->alpha;
-    const int b0 = blockIdx.x * RB;
-    const int nrows = min(RB, n - b0);
-    double prr = 0.0;
-    for (int k = threadIdx.x; k < nrows; k += kThreads) {
-        const int i = b0 + k;
-        xs[i] = fmaf(alpha, p[i], xs[i]);
-        const float rn = fmaf(-alpha, q[i], rs[i]);
-        rs[i] = rn;
-        prr += (double)rn * (double)rn;
-    }
-    prr = block_sum(prr, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = prr;
-    if (arrive_last(&st->counter, &flag)) {
-        double rr = reduce_partials(partials, gridDim.x, 1, red);
-        if (threadIdx.x == 0) {
-            st->counter = 0;
-            st->iters += 1;
-            if (!isfinite(rr)) {
-                st->status = CICY_BREAKDOWN;
-                st->done = 1;
-            } else {
-                st->beta = rr / st->rz;
-                mx->rr_prev = st->rz;
-                st->rz = rr;
-                st->rr = rr;
-                if (rr <= mx->delta2 * mx->m2 || rr <= mx->tgt2) mx->need_update = 1;
-                if (st->iters >= st->max_iter) {
-                    // fold what has been accumulated and stop
-                    mx->need_update = 1;
-                    st->done = 1;
-                    st->status = CICY_MAXITER;
-                }
-            }
-        }
-    }
-}
-
-// x += D^-1/2 x^; x^ = 0 (guarded by the update flag unless `always`).
-template <int RB>
-__global__ __launch_bounds__(kThreads) void k_fold(int n, const double *__restrict__ diag,
-                                                   double *__restrict__ x, float *__restrict__ xs,
-                                                   const MixState *__restrict__ mx, int always) {
-    if (!always && !mx->need_update) return;
-    const int b0 = blockIdx.x * RB;
-    const int nrows = min(RB, n - b0);
-    for (int k = threadIdx.x; k < nrows; k += kThreads) {
-        const int i = b0 + k;
-        const float c = xs[i];
-        if (c != 0.f) x[i] = fma((double)c, 1.0 / sqrt(diag[i]), x[i]);
-        xs[i] = 0.f;
-    }
-}
-
-// r = b - A x (FP64, ax = A x already formed), r^ = D^-1/2 r in FP32; the last work-group
-// tests the stopping rule on ||r|| and restarts the recursion from r^ (same direction p).
-template <int RB>
-__global__ __launch_bounds__(kThreads) void k_reliable_resid(int n, const double *__restrict__ diag,
-                                                             const double *__restrict__ b,
-                                                             const double *__restrict__ ax,
-                                                             double *__restrict__ r,
-                                                             float *__restrict__ rs,
-                                                             PcgState *__restrict__ st,
-                                                             MixState *__restrict__ mx,
-                                                             double *__restrict__ partials) {
-    if (!mx->need_update) return;
-    __shared__ double red[8];
-    __shared__ int flag;
-    const int b0 = blockIdx.x * RB;
-    const int nrows = min(RB, n - b0);
-    double a = 0.0, t = 0.0;
-    for (int k = threadIdx.x; k < nrows; k += kThreads) {
-        const int i = b0 + k;
-        const double ri = b[i] - ax[i];
-        r[i] = ri;
-        const float v = (float)(ri * (1.0 / sqrt(diag[i])));
-        rs[i] = v;
-        a += (double)v * (double)v;
-        t += ri * ri;
-    }
-    block_sum2(a, t, red);
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = a;
-        partials[2 * blockIdx.x + 1] = t;
-    }
-    if (arrive_last(&mx->counter, &flag)) {
-        double rs2 = reduce_partials(partials, gridDim.x, 2, red);
-        double rt2 = reduce_partials(partials + 1, gridDim.x, 2, red);
-        if (threadIdx.x == 0) {
-            mx->counter = 0;
-            mx->refinements += 1;
-            mx->need_update = 0;
-            if (rt2 > 0.25 * mx->rtrue2) mx->pad += 1;  // stall counter
-            else mx->pad = 0;
-            mx->rtrue2 = rt2;
-            st->rr = rt2;
-            if (rt2 <= st->stop2) {
-                st->status = CICY_OK;
-                st->done = 1;
-            } else if (!isfinite(rt2)) {
-                st->status = CICY_BREAKDOWN;
-                st->done = 1;
-            } else if (!st->done) {
-                st->beta = mx->rr_prev > 0.0 ? rs2 / mx->rr_prev : 0.0;
-                st->rz = rs2;
-                mx->m2 = rs2;
-                mx->tgt2 = rs2 * (st->stop2 / rt2);
-                if (mx->pad >= kStallLimit) {
-                    st->status = CICY_MAXITER;
-                    st->done = 1;
-                }
-            }
-        }
-    }
-}
+constexpr BatchDev kNoBatch = {nullptr, nullptr, nullptr, nullptr, nullptr};
 
 // r = b - ax
 __global__ __launch_bounds__(kThreads) void k_sub_from(int n, double *__restrict__ r,
@@ -257,53 +60,46 @@ void launch_chunk(cicy_sys *s) {
     for (int it = 0; it < kChunk; it++) {
         float *pold = (it & 1) ? s->ps1 : s->ps0;
         float *pnew = (it & 1) ? s->ps0 : s->ps1;
-        hipLaunchKernelGGL((k_apply<float, RB, true, true>), g, b, 0, st, view32(s), pold, s->rs,
-                           pnew, s->qs, s->st, s->partials, nullptr);
-        hipLaunchKernelGGL((k_update_mixed<RB>), g, b, 0, st, s->n, s->xs, s->rs, pnew, s->qs,
-                           s->st, s->mix, s->partials);
+        hipLaunchKernelGGL((k_apply<float, RB, true, true, false>), g, b, 0, st, view32(s),
+                           kNoBatch, pold, s->rs, pnew, s->qs, s->st, s->partials, nullptr);
+        hipLaunchKernelGGL((k_update_mixed<RB, false>), g, b, 0, st, s->n, kNoBatch, s->xs,
+                           s->rs, pnew, s->qs, s->st, s->mix, s->partials);
     }
-    hipLaunchKernelGGL((k_fold<RB>), g, b, 0, st, s->n, s->diag, s->x, s->xs, s->mix, 0);
-    hipLaunchKernelGGL((k_apply<double, RB, false, false>), g, b, 0, st, view64(s), s->x, nullptr,
-                       nullptr, s->q, nullptr, nullptr,
-                       (const int *)((char *)s->mix + offsetof(MixState, need_update)));
-    hipLaunchKernelGGL((k_reliable_resid<RB>), g, b, 0, st, s->n, s->diag, s->b, s->q, s->r,
-                       s->rs, s->st, s->mix, s->partials);
+    hipLaunchKernelGGL((k_fold<RB, false>), g, b, 0, st, s->n, kNoBatch, s->diag, s->x, s->xs,
+                       s->mix, 0);
+    hipLaunchKernelGGL((k_apply<double, RB, false, false, false>), g, b, 0, st, view64(s),
+                       kNoBatch, s->x, nullptr, nullptr, s->q, nullptr, nullptr, s->mix);
+    hipLaunchKernelGGL((k_reliable_resid<RB, false>), g, b, 0, st, s->n, kNoBatch, s->diag,
+                       s->b, s->q, s->r, s->rs, s->st, s->mix, s->partials, kStallLimit);
 }
 
 template <int RB>
 void launch_init(cicy_sys *s, double tol, double stop_ref2, long long max_iter) {
-    hipLaunchKernelGGL((k_mixed_init<RB>), dim3(s->nblocks), dim3(kThreads), 0, s->ctx->stream,
-                       s->n, s->diag, s->r0, s->rs, s->xs, s->ps0, s->ps1, s->st, s->mix,
-                       s->partials, tol, stop_ref2, max_iter);
+    hipLaunchKernelGGL((k_mixed_init<RB, false>), dim3(s->nblocks), dim3(kThreads), 0,
+                       s->ctx->stream, s->n, kNoBatch, s->diag, s->r0, s->rs, s->xs, s->ps0,
+                       s->ps1, s->st, s->mix, s->partials, tol, nullptr, stop_ref2, max_iter,
+                       kDelta * kDelta);
 }
 
 template <int RB>
 void launch_fold_always(cicy_sys *s) {
-    hipLaunchKernelGGL((k_fold<RB>), dim3(s->nblocks), dim3(kThreads), 0, s->ctx->stream, s->n,
-                       s->diag, s->x, s->xs, s->mix, 1);
+    hipLaunchKernelGGL((k_fold<RB, false>), dim3(s->nblocks), dim3(kThreads), 0, s->ctx->stream,
+                       s->n, kNoBatch, s->diag, s->x, s->xs, s->mix, 1);
 }
 
 template <int RB>
 void launch_spmv64(cicy_sys *s, const double *v, double *out) {
-    hipLaunchKernelGGL((k_apply<double, RB, false, false>), dim3(s->nblocks), dim3(kThreads), 0,
-                       s->ctx->stream, view64(s), v, nullptr, nullptr, out, nullptr, nullptr,
-                       nullptr);
+    hipLaunchKernelGGL((k_apply<double, RB, false, false, false>), dim3(s->nblocks),
+                       dim3(kThreads), 0, s->ctx->stream, view64(s), kNoBatch, v, nullptr,
+                       nullptr, out, nullptr, nullptr, nullptr);
 }
 
 template <int RB>
 void launch_spmv32(cicy_sys *s) {
-    hipLaunchKernelGGL((k_apply<float, RB, false, true>), dim3(s->nblocks), dim3(kThreads), 0,
-                       s->ctx->stream, view32(s), s->ps0, nullptr, nullptr, s->qs, nullptr,
-                       nullptr, nullptr);
+    hipLaunchKernelGGL((k_apply<float, RB, false, true, false>), dim3(s->nblocks),
+                       dim3(kThreads), 0, s->ctx->stream, view32(s), kNoBatch, s->ps0, nullptr,
+                       nullptr, s->qs, nullptr, nullptr, nullptr);
 }
-
-#define CICY_RB_SWITCH(rb, CALL)            \
-    switch (rb) {                           \
-        case 128: { constexpr int RB = 128; CALL; } break;   \
-        case 256: { constexpr int RB = 256; CALL; } break;   \
-        case 512: { constexpr int RB = 512; CALL; } break;   \
-        default: { constexpr int RB = 1024; CALL; } break;   \
-    }
 
 bool ensure_graph_mixed(cicy_sys *s) {
     if (s->graph_exec_mixed) return true;
@@ -326,16 +122,8 @@ namespace cicy {
 bool mixed_setup(cicy_sys *s, const BlockedHost &h) {
     cicy_ctx *ctx = s->ctx;
     const size_t nv = (size_t)h.n;
-    std::vector<double> sc(nv);
-    for (size_t i = 0; i < nv; i++) sc[i] = 1.0 / std::sqrt(h.diag[i]);
-    // a^ = a * s_hi * s_lo with hi = the larger index, so both stored copies of an entry
-    // round to the same FP32 value
-    std::vector<float> v32((size_t)h.nnz), u32((size_t)h.nremote);
-    for (int r = 0; r < h.n; r++) {
-        for (int j = h.rp[r]; j < h.rp[r + 1]; j++) v32[j] = (float)(h.val[j] * sc[r] * sc[h.col[j]]);
-        for (int j = h.urp[r]; j < h.urp[r + 1]; j++)
-            u32[j] = (float)(h.uval[j] * sc[h.ucol[j]] * sc[r]);
-    }
+    std::vector<float> v32, u32;
+    scaled_values(h, v32, u32);
     auto alloc = [&](void **p, size_t bytes) {
         if (!hip_ok(ctx, hipMalloc(p, std::max<size_t>(bytes, 4)), "hipMalloc")) return false;
         s->device_bytes += std::max<size_t>(bytes, 4);
@@ -357,6 +145,20 @@ bool mixed_setup(cicy_sys *s, const BlockedHost &h) {
          hip_ok(ctx, hipMemsetAsync(s->ps0, 0, 4 * std::max<size_t>(nv, 1), ctx->stream), "memset") &&
          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return ok;
+}
+
+// a^ = a * s_hi * s_lo, s = 1/sqrt(d), hi = the larger index, so both stored copies of an
+// entry round to the same FP32 value
+void scaled_values(const BlockedHost &h, std::vector<float> &v32, std::vector<float> &u32) {
+    std::vector<double> sc((size_t)h.n);
+    for (int i = 0; i < h.n; i++) sc[i] = 1.0 / std::sqrt(h.diag[i]);
+    v32.resize((size_t)h.nnz);
+    u32.resize((size_t)h.nremote);
+    for (int r = 0; r < h.n; r++) {
+        for (int j = h.rp[r]; j < h.rp[r + 1]; j++) v32[j] = (float)(h.val[j] * sc[r] * sc[h.col[j]]);
+        for (int j = h.urp[r]; j < h.urp[r + 1]; j++)
+            u32[j] = (float)(h.uval[j] * sc[h.ucol[j]] * sc[r]);
+    }
 }
 
 void mixed_free(cicy_sys *s) {

@@ -22,115 +22,6 @@ using namespace cicy;
 // kernels
 // ------------------------------------------------------------------------------------
 
-// x += alpha p; r -= alpha q; partial r.z, r.r; the last work-group closes the iteration:
-// convergence test sqrt(r.r) <= tol*||r_0|| (Direct_Electrifying.cpp:1619-1620), then
-// beta = r.z / rz_old (:1627-1632) and the iteration cap (:1559).
-template <int RB>
-__global__ __launch_bounds__(kThreads) void k_update_f64(int n, const double *__restrict__ diag,
-                                                         double *__restrict__ x,
-                                                         double *__restrict__ r,
-                                                         const double *__restrict__ p,
-                                                         const double *__restrict__ q,
-                                                         PcgState *__restrict__ st,
-                                                         double *__restrict__ partials) {
-    if (st->done) return;
-    __shared__ double red[8];
-    __shared__ int flag;
-    const double alpha = st->alpha;
-    const int r0 = blockIdx.x * RB;
-    const int nrows = min(RB, n - r0);
-    double prz = 0.0, prr = 0.0;
-    for (int t = threadIdx.x; t < nrows; t += kThreads) {
-        const int i = r0 + t;
-        x[i] = fma(alpha, p[i], x[i]);
-        const double rn = fma(-alpha, q[i], r[i]);
-        r[i] = rn;
-        const double z = (1.0 / diag[i]) * rn;
-        prz += z * rn;
-        prr += rn * rn;
-    }
-    block_sum2(prz, prr, red);
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = prz;
-        partials[2 * blockIdx.x + 1] = prr;
-    }
-    if (arrive_last(&st->counter, &flag)) {
-        double rz = reduce_partials(partials, gridDim.x, 2, red);
-        double rr = reduce_partials(partials + 1, gridDim.x, 2, red);
-        if (threadIdx.x == 0) {
-            st->counter = 0;
-            st->iters += 1;
-            st->rr = rr;
-            if (rr <= st->stop2) {
-                st->status = CICY_OK;
-                st->done = 1;
-            } else if (!isfinite(rr)) {
-                st->status = CICY_BREAKDOWN;
-                st->done = 1;
-            } else {
-                st->beta = rz / st->rz;
-                st->rz = rz;
-                if (st->iters >= st->max_iter) {
-                    st->status = CICY_MAXITER;
-                    st->done = 1;
-                }
-            }
-        }
-    }
-}
-
-// Start of a solve: rz = r.(M^-1 r), rr = r.r, stopping threshold.  stop_ref2 < 0 means
-// "relative to ||r_0||" (the reference), otherwise it is ||b||^2.
-template <int RB>
-__global__ __launch_bounds__(kThreads) void k_init_f64(int n, const double *__restrict__ diag,
-                                                       const double *__restrict__ r,
-                                                       PcgState *__restrict__ st,
-                                                       double *__restrict__ partials,
-                                                       double tol, double stop_ref2,
-                                                       long long max_iter) {
-    __shared__ double red[8];
-    __shared__ int flag;
-    const int r0 = blockIdx.x * RB;
-    const int nrows = min(RB, n - r0);
-    double prz = 0.0, prr = 0.0;
-    for (int t = threadIdx.x; t < nrows; t += kThreads) {
-        const int i = r0 + t;
-        const double ri = r[i];
-        const double z = (1.0 / diag[i]) * ri;
-        prz += z * ri;
-        prr += ri * ri;
-    }
-    block_sum2(prz, prr, red);
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = prz;
-        partials[2 * blockIdx.x + 1] = prr;
-    }
-    if (arrive_last(&st->counter, &flag)) {
-        double rz = reduce_partials(partials, gridDim.x, 2, red);
-        double rr = reduce_partials(partials + 1, gridDim.x, 2, red);
-        if (threadIdx.x == 0) {
-            st->counter = 0;
-            st->rz = rz;
-            st->rr = rr;
-            st->rr_init = rr;
-            double ref2 = stop_ref2 < 0.0 ? rr : stop_ref2;
-            double thr = tol * sqrt(ref2);
-            st->stop2 = thr * thr;
-            st->alpha = 0.0;
-            st->beta = 0.0;
-            st->pq = 0.0;
-            st->iters = 0;
-            st->max_iter = max_iter;
-            st->status = CICY_MAXITER;
-            st->done = 0;
-            if (rr == 0.0) {  // exact initial guess
-                st->status = CICY_OK;
-                st->done = 1;
-            }
-        }
-    }
-}
-
 // out = sum over rows of (a[i] - b[i])^2 and b^2 (true-residual norms); single pass.
 __global__ __launch_bounds__(kThreads) void k_resid_norms(int n, const double *__restrict__ bvec,
                                                           const double *__restrict__ ax,
@@ -150,7 +41,7 @@ __global__ __launch_bounds__(kThreads) void k_resid_norms(int n, const double *_
         partials[2 * blockIdx.x] = a;
         partials[2 * blockIdx.x + 1] = bb;
     }
-    if (arrive_last(counter, &flag)) {
+    if (arrive_last(counter, gridDim.x, &flag)) {
         double s = reduce_partials(partials, gridDim.x, 2, red);
         double t = reduce_partials(partials + 1, gridDim.x, 2, red);
         if (threadIdx.x == 0) {
@@ -325,20 +216,23 @@ bool dispatch_rb(int rb, A &&...args) {
     }
 }
 
+constexpr BatchDev kNoBatch = {nullptr, nullptr, nullptr, nullptr, nullptr};
+
 template <int RB>
 struct LaunchSpmv64 {
     static void run(cicy_sys *s, const double *v, double *out) {
-        hipLaunchKernelGGL((k_apply<double, RB, false, false>), dim3(s->nblocks), dim3(kThreads),
-                           0, s->ctx->stream, dev_view64(s), v, nullptr, nullptr, out, nullptr,
-                           nullptr, nullptr);
+        hipLaunchKernelGGL((k_apply<double, RB, false, false, false>), dim3(s->nblocks),
+                           dim3(kThreads), 0, s->ctx->stream, dev_view64(s), kNoBatch, v,
+                           nullptr, nullptr, out, nullptr, nullptr, nullptr);
     }
 };
 
 template <int RB>
 struct LaunchInit64 {
     static void run(cicy_sys *s, double tol, double stop_ref2, long long max_iter) {
-        hipLaunchKernelGGL((k_init_f64<RB>), dim3(s->nblocks), dim3(kThreads), 0, s->ctx->stream,
-                           s->n, s->diag, s->r, s->st, s->partials, tol, stop_ref2, max_iter);
+        hipLaunchKernelGGL((k_init_f64<RB, false>), dim3(s->nblocks), dim3(kThreads), 0,
+                           s->ctx->stream, s->n, kNoBatch, s->diag, s->r, s->p0, s->p1, s->st,
+                           s->partials, tol, nullptr, stop_ref2, max_iter);
     }
 };
 
@@ -348,12 +242,12 @@ struct LaunchIter64 {
     static void run(cicy_sys *s, int parity) {
         double *pold = parity ? s->p1 : s->p0;
         double *pnew = parity ? s->p0 : s->p1;
-        hipLaunchKernelGGL((k_apply<double, RB, true, false>), dim3(s->nblocks), dim3(kThreads), 0,
-                           s->ctx->stream, dev_view64(s), pold, s->r, pnew, s->q, s->st,
-                           s->partials, nullptr);
-        hipLaunchKernelGGL((k_update_f64<RB>), dim3(s->nblocks), dim3(kThreads), 0,
-                           s->ctx->stream, s->n, s->diag, s->x, s->r, pnew, s->q, s->st,
-                           s->partials);
+        hipLaunchKernelGGL((k_apply<double, RB, true, false, false>), dim3(s->nblocks),
+                           dim3(kThreads), 0, s->ctx->stream, dev_view64(s), kNoBatch, pold,
+                           s->r, pnew, s->q, s->st, s->partials, nullptr);
+        hipLaunchKernelGGL((k_update_f64<RB, false>), dim3(s->nblocks), dim3(kThreads), 0,
+                           s->ctx->stream, s->n, kNoBatch, s->diag, s->x, s->r, pnew, s->q,
+                           s->st, s->partials);
     }
 };
 
@@ -452,14 +346,11 @@ bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
     if (!hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "hipEventRecord")) return false;
     // fresh start from the stored r0, x0; both p buffers zero so that the first
     // direction is p = M^-1 r (beta = 0)
+    double stop_ref2 = rel_to == CICY_REL_B ? s->bnorm2 : -1.0;
     if (!hip_ok(ctx, hipMemcpyAsync(s->r, s->r0, vb, hipMemcpyDeviceToDevice, st), "copy r0") ||
         !hip_ok(ctx, hipMemcpyAsync(s->x, s->x0, vb, hipMemcpyDeviceToDevice, st), "copy x0") ||
-        !hip_ok(ctx, hipMemsetAsync(s->p0, 0, vb, st), "memset p0") ||
-        !hip_ok(ctx, hipMemsetAsync(s->p1, 0, vb, st), "memset p1") ||
         !hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), st), "memset state"))
         return false;
-    double stop_ref2 = -1.0;
-    if (rel_to == CICY_REL_B) stop_ref2 = s->bnorm2;
     dispatch_rb<LaunchInit64>(s->rb, s, tol, stop_ref2, max_iter);
     if (!run_chunks(s, s->graph_exec64)) return false;
     if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "hipEventRecord") ||

@@ -27,7 +27,8 @@ EXPORTED_SYMBOLS = [
     "cicy_solve_dea_cg_sss", "cicy_update_residual", "cicy_spmv_sss_f64",
     "cicy_boundary_current_f64", "cicy_sys_create", "cicy_sys_destroy", "cicy_sys_size",
     "cicy_sys_set_rhs", "cicy_sys_solve", "cicy_sys_get", "cicy_sys_spmv_bench",
-    "cicy_host_blocked_spmv",
+    "cicy_batch_create", "cicy_batch_destroy", "cicy_batch_size", "cicy_batch_set_rhs",
+    "cicy_batch_solve", "cicy_batch_get", "cicy_host_blocked_spmv",
 ]
 
 
@@ -77,6 +78,15 @@ def lib():
     L.cicy_sys_spmv_bench.argtypes = [_VP, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
     L.cicy_host_blocked_spmv.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
                                          ctypes.c_int, _F64, _F64]
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    L.cicy_batch_create.argtypes = [_VP, ctypes.c_int, _I64, _I64, PP, PP, PP, PP, P(_VP)]
+    L.cicy_batch_destroy.argtypes = [_VP]
+    L.cicy_batch_size.restype = ctypes.c_int64
+    L.cicy_batch_size.argtypes = [_VP, ctypes.c_int]
+    L.cicy_batch_set_rhs.argtypes = [_VP, ctypes.c_int, _F64, _VP, _VP]
+    L.cicy_batch_solve.argtypes = [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_int64,
+                                   ctypes.c_int, P(SolveStats), P(ctypes.c_double)]
+    L.cicy_batch_get.argtypes = [_VP, ctypes.c_int, _VP, _VP]
     _bound = L
     return L
 
@@ -181,6 +191,10 @@ class Context:
     def system(self, row_ptr, col_ind, values, diagonal) -> "System":
         return System(self, row_ptr, col_ind, values, diagonal)
 
+    def batch(self, systems) -> "Batch":
+        """systems: sequence of (row_ptr, col_ind, values, diagonal)."""
+        return Batch(self, systems)
+
 
 class System:
     """A device-resident SSS system (cicy_sys)."""
@@ -232,6 +246,59 @@ class System:
                                                         ctypes.byref(ms)),
                         "cicy_sys_spmv_bench")
         return ms.value
+
+
+class Batch:
+    """Many device-resident systems advanced together (cicy_batch)."""
+
+    def __init__(self, ctx: Context, systems):
+        self.ctx = ctx
+        arrs = [(_i64(rp), _i64(ci), _f64(v), _f64(d)) for rp, ci, v, d in systems]
+        self._keep = arrs
+        self.sizes = [len(d) for _, _, _, d in systems]
+        ns = len(arrs)
+        n = np.array(self.sizes, np.int64)
+        nnz = np.array([len(v) for _, _, v, _ in systems], np.int64)
+        ptrs = [(ctypes.c_void_p * ns)(*[a[k].ctypes.data for a in arrs]) for k in range(4)]
+        h = _VP()
+        ctx._check(ctx._L.cicy_batch_create(ctx._h, ns, n, nnz, *ptrs, ctypes.byref(h)),
+                   "cicy_batch_create")
+        self._h = h
+        self.nsys = ns
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx._L.cicy_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def size(self, what: str) -> int:
+        return int(self.ctx._L.cicy_batch_size(self._h, {
+            "n": 0, "nnz": 1, "remote": 2, "bytes": 3, "systems": 4, "blocks": 5}[what]))
+
+    def set_rhs(self, sys_id, r0, x0=None, b=None):
+        r0 = _f64(r0)
+        x0 = None if x0 is None else _f64(x0)
+        b = None if b is None else _f64(b)
+        self.ctx._check(self.ctx._L.cicy_batch_set_rhs(self._h, sys_id, r0, _ptr(x0), _ptr(b)),
+                        "cicy_batch_set_rhs")
+
+    def solve(self, method=METHOD_PCG_F64, tol=1e-10, max_iter=0, rel_to=REL_B):
+        st = (SolveStats * self.nsys)()
+        ms = ctypes.c_double()
+        self.ctx._check(self.ctx._L.cicy_batch_solve(self._h, method, tol, max_iter, rel_to, st,
+                                                     ctypes.byref(ms)), "cicy_batch_solve")
+        return list(st), ms.value
+
+    def get(self, sys_id):
+        n = self.sizes[sys_id]
+        x = np.zeros(max(n, 1))
+        r = np.zeros(max(n, 1))
+        self.ctx._check(self.ctx._L.cicy_batch_get(self._h, sys_id, _ptr(x), _ptr(r)),
+                        "cicy_batch_get")
+        return x[:n], r[:n]
 
 
 def host_blocked_spmv(row_ptr, col_ind, values, diagonal, v, rows_per_block=256):

@@ -119,6 +119,27 @@ int cicy_sys_get(cicy_sys *sys, double *x, double *r);
 /* Time `reps` back-to-back SSS SpMV launches on resident data; average ms per launch. */
 int cicy_sys_spmv_bench(cicy_sys *sys, int precision /*64 or 32*/, int reps, double *avg_ms);
 
+/* ---- batches of independent systems (clusters x directions x realisations) -----------
+ * The reference solves each (cluster, direction) system in turn with a fresh object
+ * (Electrical_analysis.cpp:34-105, :249-283).  A batch holds many systems of any sizes in
+ * HBM and advances all of them in the same launches; each keeps its own scalars, stopping
+ * test and iteration count.  Arrays are per system, as in cicy_sys_create. */
+typedef struct cicy_batch cicy_batch;
+int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *nnz,
+                      const int64_t *const *row_ptr, const int64_t *const *col_ind,
+                      const double *const *values, const double *const *diag,
+                      cicy_batch **out);
+void cicy_batch_destroy(cicy_batch *batch);
+/* 0 total rows, 1 total nnz, 2 remote entries, 3 device bytes, 4 systems, 5 row blocks */
+int64_t cicy_batch_size(const cicy_batch *batch, int what);
+/* r0 = b - A x0; x0 may be NULL (zeros, then b defaults to r0); b is required otherwise */
+int cicy_batch_set_rhs(cicy_batch *batch, int sys, const double *r0, const double *x0,
+                       const double *b);
+/* per_sys[nsys] receives each system's stats; total_ms the device time of the batch */
+int cicy_batch_solve(cicy_batch *batch, int method, double tol, int64_t max_iter, int rel_to,
+                     cicy_solve_stats *per_sys, double *total_ms);
+int cicy_batch_get(cicy_batch *batch, int sys, double *x, double *r);
+
 /* ---- test hook: the blocked SSS layout applied on the host (no GPU) -------------------
  * Builds the device layout for rows_per_block and applies it with host loops, so the
  * layout can be checked where no GPU is present. */

@@ -1,0 +1,77 @@
+"""Batched solves (many uneven systems in one launch sequence) against the CPU oracle and
+against the same systems solved one at a time."""
+import numpy as np
+import pytest
+
+import oracle
+from cicy_geom_amd import dea, network
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dea.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def systems():
+    out = []
+    for k, ncnt in enumerate([300, 900, 2000, 350, 1200]):
+        net = network.generate(4, realisation=k, n_cnt=ncnt)
+        for d in range(3):
+            ks = net.percolating_clusters(d)
+            if ks:
+                out.append(net.assemble(max(ks, key=lambda c: net.cluster(c)[0]), 1, d,
+                                        direction=d))
+    assert len(out) >= 6
+    return out
+
+
+@pytest.mark.parametrize("method", [dea.METHOD_PCG_F64, dea.METHOD_MIXED])
+def test_batch_matches_truth_and_single_solves(ctx, systems, method):
+    B = ctx.batch([(s.row_ptr, s.col_ind, s.values, s.diag) for s in systems])
+    for i, s in enumerate(systems):
+        B.set_rhs(i, s.b, None, s.b)
+    stats, ms = B.solve(method, 1e-10, 200_000, dea.REL_B)
+    for i, s in enumerate(systems):
+        x, _ = B.get(i)
+        st = stats[i]
+        ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
+        tr = np.linalg.norm(s.b - ax) / np.linalg.norm(s.b)
+        truth, _, _ = oracle.cg_truth(s.row_ptr, s.col_ind, s.values, s.diag, s.b, tol=1e-13)
+        err = np.linalg.norm(x - truth) / np.linalg.norm(truth)
+        print(i, s.n, st.as_dict()["status_name"], st.iters, st.refinements, f"{tr:.2e} {err:.2e}")
+        assert st.status == dea.CICY_OK
+        assert tr <= 1.05e-10 and abs(st.true_rel_res - tr) <= 1e-3 * tr + 1e-16
+        assert err < 1e-7
+    # each system's iteration count matches its stand-alone solve closely
+    if method == dea.METHOD_PCG_F64:
+        for i in (0, len(systems) - 1):
+            s = systems[i]
+            sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+            sysd.set_rhs(s.b, None, s.b)
+            st1 = sysd.solve(method, 1e-10, 200_000, dea.REL_B)
+            sysd.close()
+            assert abs(st1.iters - stats[i].iters) <= max(3, st1.iters // 50)
+    B.close()
+
+
+def test_batch_reference_stopping_rule_with_warm_start(ctx, systems):
+    """REL_R0 with the ramp guess: each system stops as Solve_DEA_equations_CG_SSS would."""
+    sub = systems[:4]
+    B = ctx.batch([(s.row_ptr, s.col_ind, s.values, s.diag) for s in sub])
+    for i, s in enumerate(sub):
+        B.set_rhs(i, s.r0, s.x0, s.b)
+    stats, _ = B.solve(dea.METHOD_PCG_F64, 1e-10, 0, dea.REL_R0)
+    for i, s in enumerate(sub):
+        ref = oracle.pcg_sss(s.nodes, s.reserved, 1e-10, s.row_ptr, s.col_ind, s.values, s.diag,
+                             s.r0, s.vef, s.x0)
+        x, r = B.get(i)
+        assert stats[i].status == dea.CICY_OK
+        assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(s.r0)
+        assert np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"]) < 1e-8
+        assert abs(stats[i].iters - ref["iters"]) <= max(5, ref["iters"] // 20)
+    B.close()

@@ -188,3 +188,14 @@ def test_random_laplacian_solve(ctx, method):
     # discrete maximum principle: node voltages between the two boundary voltages
     assert x.min() >= -1e-8 and x.max() <= 1 + 1e-8
     sysd.close()
+
+
+def test_cpp_dropin_program():
+    """The header-only C++ drop-in driven as Compute_voltage_field would drive it."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "dropin_chain")
+    assert os.path.exists(exe), "build() compiles tests/cpp/dropin_chain"
+    out = subprocess.run([exe, "3000"], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0 and out.stdout.startswith("OK")
