@@ -1,0 +1,734 @@
+// dea_mpnn.hip -- learned warm start for the DEA solve: a 3-layer message-passing network
+// over the conductance graph, trained and evaluated by hand-written gfx950 kernels.
+//
+// Not a reference component: the reference's warm start is the linear ramp of
+// Initial_guess_for_CG (Direct_Electrifying.cpp:1396-1470).  The network predicts a
+// correction of that ramp; zero-guess CG stays the fallback and the judge of its error.
+//
+// Network (H = 32 channels, F = 4 node features, 3 layers), weights stored in BF16,
+// activations and accumulation in FP32:
+//   z0 = W_in f + b_in, h = relu(z0)
+//   per layer: gate_ij = sigmoid(ga * e_ij + gc)  (per channel, e_ij = normalised log10 g)
+//              agg_i   = sum_j w_ij gate_ij h_j   (w_ij = g_ij / d_i)
+//              z = Ws h + Wn agg + bias,  h += relu(z)
+//   y = w_out . h + b_out,  V^ = V_app (1 - u + y)
+//   loss = mean ((V^ - V*) / V_app)^2
+// Thread mapping: 32 lanes per node (lane = channel), 8 nodes per 256-thread work-group;
+// the dense 32x64 products read the weights transposed from LDS (conflict-free) and the
+// node rows as LDS broadcasts.
+#include "dea_internal.h"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace cicy;
+
+namespace {
+
+constexpr int H = 32;
+constexpr int F = 4;
+constexpr int LAYERS = 3;
+constexpr int NODES_PER_PASS = kThreads / H;   // 8
+constexpr double E_MU = -5.0, E_SIGMA = 2.5;
+
+// flat parameter layout (same order as the float64 checker)
+constexpr int OFF_WIN = 0;
+constexpr int OFF_BIN = OFF_WIN + H * F;
+constexpr int OFF_L0 = OFF_BIN + H;
+constexpr int L_WS = 0, L_WN = H * H, L_BIAS = 2 * H * H, L_GA = 2 * H * H + H,
+              L_GC = 2 * H * H + 2 * H, L_SIZE = 2 * H * H + 3 * H;
+constexpr int OFF_WOUT = OFF_L0 + LAYERS * L_SIZE;
+constexpr int OFF_BOUT = OFF_WOUT + H;
+constexpr int NPARAM = OFF_BOUT + 1;
+
+__device__ __forceinline__ float bf(const __hip_bfloat16 *p, int i) {
+    return __bfloat162float(p[i]);
+}
+
+__device__ __forceinline__ float sigmoidf_(float x) { return 1.0f / (1.0f + __expf(-x)); }
+
+// ---- forward ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kThreads) void k_encode(int n, const float *__restrict__ feat,
+                                                     const __hip_bfloat16 *__restrict__ th,
+                                                     float *__restrict__ z0,
+                                                     float *__restrict__ h0) {
+    const int c = threadIdx.x & (H - 1);
+    float w[F];
+#pragma unroll
+    for (int f = 0; f < F; f++) w[f] = bf(th, OFF_WIN + c * F + f);
+    const float bias = bf(th, OFF_BIN + c);
+    for (int i = blockIdx.x * NODES_PER_PASS + threadIdx.x / H; i < n;
+         i += gridDim.x * NODES_PER_PASS) {
+        float z = bias;
+#pragma unroll
+        for (int f = 0; f < F; f++) z = fmaf(w[f], feat[(size_t)i * F + f], z);
+        if (z0) z0[(size_t)i * H + c] = z;
+        h0[(size_t)i * H + c] = fmaxf(z, 0.f);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_aggregate(int n, const int *__restrict__ nptr,
+                                                        const int *__restrict__ nbr,
+                                                        const float *__restrict__ w,
+                                                        const float *__restrict__ e,
+                                                        const __hip_bfloat16 *__restrict__ th,
+                                                        int layer, const float *__restrict__ h,
+                                                        float *__restrict__ agg) {
+    const int c = threadIdx.x & (H - 1);
+    const int off = OFF_L0 + layer * L_SIZE;
+    const float ga = bf(th, off + L_GA + c), gc = bf(th, off + L_GC + c);
+    for (int i = blockIdx.x * NODES_PER_PASS + threadIdx.x / H; i < n;
+         i += gridDim.x * NODES_PER_PASS) {
+        float acc = 0.f;
+        const int k1 = nptr[i + 1];
+        for (int k = nptr[i]; k < k1; k++) {
+            const float g = sigmoidf_(fmaf(ga, e[k], gc));
+            acc = fmaf(w[k] * g, h[(size_t)nbr[k] * H + c], acc);
+        }
+        agg[(size_t)i * H + c] = acc;
+    }
+}
+
+// z = Ws h + Wn agg + bias; hout = h + relu(z).  Weights staged transposed in LDS.
+__global__ __launch_bounds__(kThreads) void k_layer(int n, const __hip_bfloat16 *__restrict__ th,
+                                                    int layer, const float *__restrict__ h,
+                                                    const float *__restrict__ agg,
+                                                    float *__restrict__ z,
+                                                    float *__restrict__ hout) {
+    __shared__ float wt[2 * H][H];              // [k][c]: k < H from Ws, k >= H from Wn
+    __shared__ float rows[NODES_PER_PASS][2 * H];
+    const int off = OFF_L0 + layer * L_SIZE;
+    for (int t = threadIdx.x; t < 2 * H * H; t += kThreads) {
+        const int m = t / (H * H), rem = t % (H * H), c = rem / H, k = rem % H;
+        wt[m * H + k][c] = bf(th, off + (m ? L_WN : L_WS) + c * H + k);
+    }
+    const int c = threadIdx.x & (H - 1), slot = threadIdx.x / H;
+    const float bias = bf(th, off + L_BIAS + c);
+    for (int base = blockIdx.x * NODES_PER_PASS; base < n; base += gridDim.x * NODES_PER_PASS) {
+        const int i = base + slot;
+        __syncthreads();
+        if (i < n) {
+            rows[slot][c] = h[(size_t)i * H + c];
+            rows[slot][H + c] = agg[(size_t)i * H + c];
+        }
+        __syncthreads();
+        if (i < n) {
+            float acc = bias;
+#pragma unroll 16
+            for (int k = 0; k < 2 * H; k++) acc = fmaf(wt[k][c], rows[slot][k], acc);
+            if (z) z[(size_t)i * H + c] = acc;
+            hout[(size_t)i * H + c] = rows[slot][c] + fmaxf(acc, 0.f);
+        }
+    }
+}
+
+// y = w_out . h + b_out; V^ = V_app (1 - u + y).  Training: loss partials, dy and dh.
+__global__ __launch_bounds__(kThreads) void k_readout(int n, const __hip_bfloat16 *__restrict__ th,
+                                                      const float *__restrict__ h,
+                                                      const float *__restrict__ u, double v_app,
+                                                      double *__restrict__ vout,
+                                                      const float *__restrict__ target,
+                                                      float *__restrict__ dh,
+                                                      double *__restrict__ loss_part,
+                                                      float *__restrict__ grad) {
+    __shared__ double red[8];
+    __shared__ float gw[H];
+    __shared__ float gb;
+    const int c = threadIdx.x & (H - 1);
+    const float wo = bf(th, OFF_WOUT + c), bo = bf(th, OFF_BOUT);
+    if (threadIdx.x < H) gw[threadIdx.x] = 0.f;
+    if (threadIdx.x == 0) gb = 0.f;
+    __syncthreads();
+    double lsum = 0.0;
+    float gwl = 0.f, gbl = 0.f;
+    for (int base = blockIdx.x * NODES_PER_PASS; base < n; base += gridDim.x * NODES_PER_PASS) {
+        const int i = base + threadIdx.x / H;
+        float hv = 0.f;
+        if (i < n) hv = h[(size_t)i * H + c];
+        float s = wo * hv;
+#pragma unroll
+        for (int o = H / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, H);
+        if (i < n) {
+            const float y = s + bo;
+            const float frac = 1.f - u[i] + y;
+            if (vout && c == 0) vout[i] = v_app * (double)frac;
+            if (target) {
+                const float d = frac - target[i];
+                const float dy = 2.f * d / (float)n;
+                if (c == 0) {
+                    lsum += (double)d * (double)d;
+                    gbl += dy;
+                }
+                dh[(size_t)i * H + c] = dy * wo;
+                gwl = fmaf(dy, hv, gwl);
+            }
+        }
+    }
+    if (!target) return;
+    atomicAdd(&gw[c], gwl);
+    if (c == 0) atomicAdd(&gb, gbl);
+    lsum = block_sum(lsum, red);
+    __syncthreads();
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = lsum;
+    if (threadIdx.x < H) atomicAdd(&grad[OFF_WOUT + threadIdx.x], gw[threadIdx.x]);
+    if (threadIdx.x == 0) atomicAdd(&grad[OFF_BOUT], gb);
+}
+
+// ---- backward --------------------------------------------------------------------------
+
+// dz = dh * (z > 0); dh_in = dh + Ws^T dz; dagg = Wn^T dz; grads of Ws, Wn, bias.
+__global__ __launch_bounds__(kThreads) void k_layer_bwd(int n,
+                                                        const __hip_bfloat16 *__restrict__ th,
+                                                        int layer, const float *__restrict__ dh,
+                                                        const float *__restrict__ z,
+                                                        const float *__restrict__ h,
+                                                        const float *__restrict__ agg,
+                                                        float *__restrict__ dh_in,
+                                                        float *__restrict__ dagg,
+                                                        float *__restrict__ grad) {
+    __shared__ float wr[2 * H][H];   // [m*H + c][k]: row-major weights (k contiguous)
+    __shared__ float dzs[NODES_PER_PASS][H];
+    __shared__ float ins[NODES_PER_PASS][2 * H];
+    const int off = OFF_L0 + layer * L_SIZE;
+    for (int t = threadIdx.x; t < 2 * H * H; t += kThreads) {
+        const int m = t / (H * H), rem = t % (H * H);
+        wr[m * H + rem / H][rem % H] = bf(th, off + (m ? L_WN : L_WS) + rem);
+    }
+    const int lane = threadIdx.x & (H - 1), slot = threadIdx.x / H;
+    // each thread owns 8 of the 2*H*H weight-gradient entries
+    float gacc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) gacc[q] = 0.f;
+    float gbias = 0.f;
+    for (int base = blockIdx.x * NODES_PER_PASS; base < n; base += gridDim.x * NODES_PER_PASS) {
+        const int i = base + slot;
+        __syncthreads();
+        if (i < n) {
+            const float zv = z[(size_t)i * H + lane];
+            dzs[slot][lane] = zv > 0.f ? dh[(size_t)i * H + lane] : 0.f;
+            ins[slot][lane] = h[(size_t)i * H + lane];
+            ins[slot][H + lane] = agg[(size_t)i * H + lane];
+        } else {
+            dzs[slot][lane] = 0.f;
+            ins[slot][lane] = 0.f;
+            ins[slot][H + lane] = 0.f;
+        }
+        __syncthreads();
+        if (i < n) {
+            float a = dh[(size_t)i * H + lane], b = 0.f;   // lane = input channel k
+#pragma unroll 8
+            for (int c = 0; c < H; c++) {
+                const float d = dzs[slot][c];
+                a = fmaf(wr[c][lane], d, a);
+                b = fmaf(wr[H + c][lane], d, b);
+            }
+            dh_in[(size_t)i * H + lane] = a;
+            dagg[(size_t)i * H + lane] = b;
+        }
+        // weight gradients: entry e = threadIdx.x + q*256 of the [m][c][k] block
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int e = threadIdx.x + q * kThreads;
+            const int m = e / (H * H), rem = e % (H * H), c = rem / H, k = rem % H;
+            float s = 0.f;
+#pragma unroll
+            for (int sl = 0; sl < NODES_PER_PASS; sl++) s = fmaf(dzs[sl][c], ins[sl][m * H + k], s);
+            gacc[q] += s;
+        }
+        if (threadIdx.x < H) {
+            float s = 0.f;
+#pragma unroll
+            for (int sl = 0; sl < NODES_PER_PASS; sl++) s += dzs[sl][threadIdx.x];
+            gbias += s;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int e = threadIdx.x + q * kThreads;
+        const int m = e / (H * H), rem = e % (H * H);
+        atomicAdd(&grad[off + (m ? L_WN : L_WS) + rem], gacc[q]);
+    }
+    if (threadIdx.x < H) atomicAdd(&grad[off + L_BIAS + threadIdx.x], gbias);
+}
+
+// dh_in[j] += sum_i w_ij gate_ij dagg[i] over the neighbours i of j (the transpose of the
+// aggregation; wrev holds w_ij on j's row), plus the gate-parameter gradients.
+__global__ __launch_bounds__(kThreads) void k_aggregate_bwd(
+    int n, const int *__restrict__ nptr, const int *__restrict__ nbr,
+    const float *__restrict__ wrev, const float *__restrict__ e,
+    const __hip_bfloat16 *__restrict__ th, int layer, const float *__restrict__ dagg,
+    const float *__restrict__ h, float *__restrict__ dh_in, float *__restrict__ grad) {
+    __shared__ float sga[H], sgc[H];
+    const int c = threadIdx.x & (H - 1);
+    const int off = OFF_L0 + layer * L_SIZE;
+    const float ga = bf(th, off + L_GA + c), gc = bf(th, off + L_GC + c);
+    if (threadIdx.x < H) {
+        sga[threadIdx.x] = 0.f;
+        sgc[threadIdx.x] = 0.f;
+    }
+    __syncthreads();
+    float aa = 0.f, ac = 0.f;
+    for (int j = blockIdx.x * NODES_PER_PASS + threadIdx.x / H; j < n;
+         j += gridDim.x * NODES_PER_PASS) {
+        const float hj = h[(size_t)j * H + c];
+        float acc = 0.f;
+        const int k1 = nptr[j + 1];
+        for (int k = nptr[j]; k < k1; k++) {
+            const float ek = e[k];
+            const float g = sigmoidf_(fmaf(ga, ek, gc));
+            const float dg = dagg[(size_t)nbr[k] * H + c] * wrev[k];
+            acc = fmaf(dg, g, acc);
+            const float pre = dg * hj * g * (1.f - g);
+            aa = fmaf(pre, ek, aa);
+            ac += pre;
+        }
+        dh_in[(size_t)j * H + c] += acc;
+    }
+    atomicAdd(&sga[c], aa);
+    atomicAdd(&sgc[c], ac);
+    __syncthreads();
+    if (threadIdx.x < H) {
+        atomicAdd(&grad[off + L_GA + threadIdx.x], sga[threadIdx.x]);
+        atomicAdd(&grad[off + L_GC + threadIdx.x], sgc[threadIdx.x]);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_encode_bwd(int n, const float *__restrict__ feat,
+                                                         const float *__restrict__ z0,
+                                                         const float *__restrict__ dh,
+                                                         float *__restrict__ grad) {
+    __shared__ float sg[H * F + H];
+    const int c = threadIdx.x & (H - 1);
+    for (int t = threadIdx.x; t < H * F + H; t += kThreads) sg[t] = 0.f;
+    __syncthreads();
+    float gw[F] = {0.f, 0.f, 0.f, 0.f};
+    float gbias = 0.f;
+    for (int i = blockIdx.x * NODES_PER_PASS + threadIdx.x / H; i < n;
+         i += gridDim.x * NODES_PER_PASS) {
+        const float d = z0[(size_t)i * H + c] > 0.f ? dh[(size_t)i * H + c] : 0.f;
+#pragma unroll
+        for (int f = 0; f < F; f++) gw[f] = fmaf(d, feat[(size_t)i * F + f], gw[f]);
+        gbias += d;
+    }
+#pragma unroll
+    for (int f = 0; f < F; f++) atomicAdd(&sg[c * F + f], gw[f]);
+    atomicAdd(&sg[H * F + c], gbias);
+    __syncthreads();
+    for (int t = threadIdx.x; t < H * F + H; t += kThreads) atomicAdd(&grad[OFF_WIN + t], sg[t]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_adam(int np, float *__restrict__ theta,
+                                                   float *__restrict__ grad, float *__restrict__ m,
+                                                   float *__restrict__ v,
+                                                   __hip_bfloat16 *__restrict__ thb, float lr,
+                                                   float b1, float b2, float eps, float bc1,
+                                                   float bc2, float gscale) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= np) return;
+    const float g = grad[i] * gscale;
+    const float mi = b1 * m[i] + (1.f - b1) * g;
+    const float vi = b2 * v[i] + (1.f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float t = theta[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    theta[i] = t;
+    thb[i] = __float2bfloat16(t);
+    grad[i] = 0.f;
+}
+
+__global__ __launch_bounds__(kThreads) void k_to_bf16(int np, const float *__restrict__ theta,
+                                                      __hip_bfloat16 *__restrict__ thb) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < np) thb[i] = __float2bfloat16(theta[i]);
+}
+
+__global__ __launch_bounds__(kThreads) void k_residual_from(int n, const double *__restrict__ b,
+                                                            const double *__restrict__ ax,
+                                                            double *__restrict__ r) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) r[i] = b[i] - ax[i];
+}
+
+template <int RB>
+void spmv_x0(cicy_sys *s) {
+    const SssDev<double> d{s->n, s->nblocks, s->rp, s->col, s->val, s->diag, s->urp, s->ucol,
+                           s->uval};
+    const BatchDev nb{nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL((k_apply<double, RB, false, false, false>), dim3(s->nblocks),
+                       dim3(kThreads), 0, s->ctx->stream, d, nb, (const double *)s->x0,
+                       (const double *)nullptr, (double *)nullptr, s->q, (PcgState *)nullptr,
+                       (double *)nullptr, (const MixState *)nullptr);
+}
+
+int grid_for(int n) { return std::max(1, std::min((n + NODES_PER_PASS - 1) / NODES_PER_PASS, 4096)); }
+
+}  // namespace
+
+struct cicy_mpnn {
+    cicy_ctx *ctx = nullptr;
+    float *theta = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+    __hip_bfloat16 *thb = nullptr;
+    long long step = 0;
+};
+
+struct cicy_graph {
+    cicy_ctx *ctx = nullptr;
+    int n = 0, m = 0;
+    bool training = false;
+    double v_app = 1.0;
+    int *nptr = nullptr, *nbr = nullptr;
+    float *w = nullptr, *wrev = nullptr, *e = nullptr, *feat = nullptr, *u = nullptr;
+    float *target = nullptr;
+    // activations: h[0..3], agg[0..2], z0, z[0..2], gradient scratch dh, dh2, dagg
+    float *h[LAYERS + 1] = {nullptr}, *agg[LAYERS] = {nullptr}, *z0 = nullptr,
+          *z[LAYERS] = {nullptr};
+    float *dh = nullptr, *dh2 = nullptr, *dagg = nullptr;
+    double *vout = nullptr, *loss_part = nullptr;
+    bool have_target = false;
+};
+
+namespace {
+
+template <typename T>
+bool galloc(cicy_ctx *ctx, T **p, size_t count) {
+    return hip_ok(ctx, hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+}
+
+template <typename T>
+bool gh2d(cicy_ctx *ctx, T *dst, const T *src, size_t count) {
+    if (!count) return true;
+    return hip_ok(ctx, hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice,
+                                      ctx->stream), "H2D");
+}
+
+// forward; v_dev (may be null) receives V^ in FP64
+bool forward(cicy_mpnn *M, cicy_graph *G, double *v_dev, bool train) {
+    hipStream_t st = M->ctx->stream;
+    const int gd = grid_for(G->n);
+    hipLaunchKernelGGL(k_encode, dim3(gd), dim3(kThreads), 0, st, G->n, G->feat, M->thb,
+                       train ? G->z0 : nullptr, G->h[0]);
+    for (int l = 0; l < LAYERS; l++) {
+        // inference reuses two h buffers (ping-pong) and one agg buffer
+        float *hin = train ? G->h[l] : G->h[l & 1];
+        float *hout = train ? G->h[l + 1] : G->h[(l + 1) & 1];
+        float *ag = train ? G->agg[l] : G->agg[0];
+        hipLaunchKernelGGL(k_aggregate, dim3(gd), dim3(kThreads), 0, st, G->n, G->nptr, G->nbr,
+                           G->w, G->e, M->thb, l, hin, ag);
+        hipLaunchKernelGGL(k_layer, dim3(gd), dim3(kThreads), 0, st, G->n, M->thb, l, hin, ag,
+                           train ? G->z[l] : nullptr, hout);
+    }
+    float *hl = train ? G->h[LAYERS] : G->h[LAYERS & 1];
+    hipLaunchKernelGGL(k_readout, dim3(gd), dim3(kThreads), 0, st, G->n, M->thb, hl, G->u,
+                       G->v_app, v_dev, train ? G->target : nullptr, G->dh, G->loss_part,
+                       M->grad);
+    return hip_ok(M->ctx, hipGetLastError(), "mpnn forward launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t cicy_mpnn_num_params(void) { return NPARAM; }
+
+int cicy_mpnn_create(cicy_ctx *ctx, const float *theta, cicy_mpnn **out) {
+    if (!ctx || !out) return 0;
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    cicy_mpnn *M = new cicy_mpnn();
+    M->ctx = ctx;
+    bool ok = galloc(ctx, &M->theta, NPARAM) && galloc(ctx, &M->grad, NPARAM) &&
+              galloc(ctx, &M->m, NPARAM) && galloc(ctx, &M->v, NPARAM) &&
+              galloc(ctx, &M->thb, NPARAM) &&
+              hip_ok(ctx, hipMemsetAsync(M->theta, 0, NPARAM * 4, ctx->stream), "memset") &&
+              hip_ok(ctx, hipMemsetAsync(M->grad, 0, NPARAM * 4, ctx->stream), "memset") &&
+              hip_ok(ctx, hipMemsetAsync(M->m, 0, NPARAM * 4, ctx->stream), "memset") &&
+              hip_ok(ctx, hipMemsetAsync(M->v, 0, NPARAM * 4, ctx->stream), "memset");
+    if (ok && theta) ok = gh2d(ctx, M->theta, theta, NPARAM);
+    if (ok) {
+        hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                           ctx->stream, NPARAM, M->theta, M->thb);
+        ok = hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    }
+    if (!ok) {
+        cicy_mpnn_destroy(M);
+        return 0;
+    }
+    *out = M;
+    return 1;
+}
+
+void cicy_mpnn_destroy(cicy_mpnn *M) {
+    if (!M) return;
+    (void)hipSetDevice(M->ctx->device);
+    (void)hipStreamSynchronize(M->ctx->stream);
+    void *p[] = {M->theta, M->grad, M->m, M->v, M->thb};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    delete M;
+}
+
+int cicy_mpnn_set_params(cicy_mpnn *M, const float *theta) {
+    if (!M || !theta) return 0;
+    cicy_ctx *ctx = M->ctx;
+    (void)hipSetDevice(ctx->device);
+    bool ok = gh2d(ctx, M->theta, theta, NPARAM);
+    hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       ctx->stream, NPARAM, M->theta, M->thb);
+    ok = ok && hip_ok(ctx, hipMemsetAsync(M->m, 0, NPARAM * 4, ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(M->v, 0, NPARAM * 4, ctx->stream), "memset") &&
+         hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    M->step = 0;
+    return ok ? 1 : 0;
+}
+
+int cicy_mpnn_get_params(cicy_mpnn *M, float *theta) {
+    if (!M || !theta) return 0;
+    return hip_ok(M->ctx, hipMemcpy(theta, M->theta, NPARAM * 4, hipMemcpyDeviceToHost), "D2H")
+               ? 1 : 0;
+}
+
+int cicy_mpnn_get_grads(cicy_mpnn *M, float *grad) {
+    if (!M || !grad) return 0;
+    return hip_ok(M->ctx, hipMemcpy(grad, M->grad, NPARAM * 4, hipMemcpyDeviceToHost), "D2H")
+               ? 1 : 0;
+}
+
+int cicy_mpnn_zero_grads(cicy_mpnn *M) {
+    if (!M) return 0;
+    return hip_ok(M->ctx, hipMemsetAsync(M->grad, 0, NPARAM * 4, M->ctx->stream), "memset") &&
+                   hip_ok(M->ctx, hipStreamSynchronize(M->ctx->stream), "sync")
+               ? 1 : 0;
+}
+
+int cicy_graph_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                      const int64_t *col_ind, const double *values, const double *diag,
+                      const double *b, const double *node_xyz, double box, int direction,
+                      double v_app, int training, cicy_graph **out) {
+    if (!ctx || !out || n <= 0 || nnz < 0 || !row_ptr || !diag || !b || !node_xyz ||
+        (nnz > 0 && (!col_ind || !values)) || direction < 0 || direction > 2 || !(box > 0) ||
+        !(v_app > 0)) {
+        if (ctx) set_error(ctx, "cicy_graph_create: invalid arguments");
+        return 0;
+    }
+    if (n >= (1LL << 30) || 2 * nnz >= (1LL << 31) - 1) {
+        set_error(ctx, "cicy_graph_create: graph too large for 32-bit indices");
+        return 0;
+    }
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    // symmetric adjacency, each row sorted by neighbour
+    std::vector<int> deg((size_t)n + 1, 0);
+    for (int64_t r = 0; r < n; r++) {
+        for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; j++) {
+            const int64_t c = col_ind[j];
+            if (c < 0 || c >= r || !(values[j] < 0.0)) {
+                set_error(ctx, "cicy_graph_create: entry %lld is not a strictly-lower negative "
+                          "conductance", (long long)j);
+                return 0;
+            }
+            deg[r + 1]++;
+            deg[c + 1]++;
+        }
+        if (!(diag[r] > 0.0)) {
+            set_error(ctx, "cicy_graph_create: non-positive diagonal at %lld", (long long)r);
+            return 0;
+        }
+    }
+    for (int64_t i = 0; i < n; i++) deg[i + 1] += deg[i];
+    const int m = deg[n];
+    std::vector<int> nbr((size_t)m), fill(deg.begin(), deg.end() - 1);
+    std::vector<double> gval((size_t)m);
+    for (int64_t r = 0; r < n; r++)
+        for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; j++) {
+            const int64_t c = col_ind[j];
+            nbr[fill[r]] = (int)c;
+            gval[fill[r]++] = -values[j];
+            nbr[fill[c]] = (int)r;
+            gval[fill[c]++] = -values[j];
+        }
+    std::vector<float> w((size_t)m), wrev((size_t)m), e((size_t)m), feat((size_t)n * F),
+        u((size_t)n);
+    std::vector<int> idx;
+    for (int64_t i = 0; i < n; i++) {
+        // sort the row by neighbour (stable pairs)
+        const int a = deg[i], z = deg[i + 1];
+        idx.resize((size_t)(z - a));
+        for (int k = a; k < z; k++) idx[k - a] = k;
+        std::sort(idx.begin(), idx.end(), [&](int p, int q) { return nbr[p] < nbr[q]; });
+        std::vector<int> nb2(idx.size());
+        std::vector<double> g2(idx.size());
+        for (size_t k = 0; k < idx.size(); k++) {
+            nb2[k] = nbr[idx[k]];
+            g2[k] = gval[idx[k]];
+        }
+        for (int k = a; k < z; k++) {
+            nbr[k] = nb2[k - a];
+            gval[k] = g2[k - a];
+            const double g = gval[k];
+            w[k] = (float)(g / diag[i]);
+            wrev[k] = (float)(g / diag[nbr[k]]);
+            e[k] = (float)((std::log10(g) - E_MU) / E_SIGMA);
+        }
+        const int perm[3] = {direction, direction == 0 ? 1 : 0, direction == 2 ? 1 : 2};
+        for (int a2 = 0; a2 < 3; a2++) feat[(size_t)i * F + a2] = (float)(node_xyz[3 * i + perm[a2]] / box);
+        feat[(size_t)i * F + 3] = (float)(b[i] / (diag[i] * v_app));
+        u[i] = feat[(size_t)i * F];
+    }
+    cicy_graph *G = new cicy_graph();
+    G->ctx = ctx;
+    G->n = (int)n;
+    G->m = m;
+    G->training = training != 0;
+    G->v_app = v_app;
+    const size_t NH = (size_t)n * H;
+    bool ok = galloc(ctx, &G->nptr, (size_t)n + 1) && galloc(ctx, &G->nbr, (size_t)m) &&
+              galloc(ctx, &G->w, (size_t)m) && galloc(ctx, &G->wrev, (size_t)m) &&
+              galloc(ctx, &G->e, (size_t)m) && galloc(ctx, &G->feat, (size_t)n * F) &&
+              galloc(ctx, &G->u, (size_t)n) && galloc(ctx, &G->target, (size_t)n) &&
+              galloc(ctx, &G->vout, (size_t)n) && galloc(ctx, &G->loss_part, 4096);
+    const int nh = G->training ? LAYERS + 1 : 2, na = G->training ? LAYERS : 1;
+    for (int l = 0; ok && l < nh; l++) ok = galloc(ctx, &G->h[l], NH);
+    for (int l = 0; ok && l < na; l++) ok = galloc(ctx, &G->agg[l], NH);
+    if (G->training) {
+        ok = ok && galloc(ctx, &G->z0, NH) && galloc(ctx, &G->dh, NH) &&
+             galloc(ctx, &G->dh2, NH) && galloc(ctx, &G->dagg, NH);
+        for (int l = 0; ok && l < LAYERS; l++) ok = galloc(ctx, &G->z[l], NH);
+    }
+    ok = ok && gh2d(ctx, G->nptr, deg.data(), (size_t)n + 1) && gh2d(ctx, G->nbr, nbr.data(), (size_t)m) &&
+         gh2d(ctx, G->w, w.data(), (size_t)m) && gh2d(ctx, G->wrev, wrev.data(), (size_t)m) &&
+         gh2d(ctx, G->e, e.data(), (size_t)m) && gh2d(ctx, G->feat, feat.data(), (size_t)n * F) &&
+         gh2d(ctx, G->u, u.data(), (size_t)n) &&
+         hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (!ok) {
+        cicy_graph_destroy(G);
+        return 0;
+    }
+    *out = G;
+    return 1;
+}
+
+void cicy_graph_destroy(cicy_graph *G) {
+    if (!G) return;
+    (void)hipSetDevice(G->ctx->device);
+    (void)hipStreamSynchronize(G->ctx->stream);
+    void *p[] = {G->nptr, G->nbr, G->w, G->wrev, G->e, G->feat, G->u, G->target, G->vout,
+                 G->loss_part, G->z0, G->dh, G->dh2, G->dagg};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+    for (int l = 0; l <= LAYERS; l++)
+        if (G->h[l]) (void)hipFree(G->h[l]);
+    for (int l = 0; l < LAYERS; l++) {
+        if (G->agg[l]) (void)hipFree(G->agg[l]);
+        if (G->z[l]) (void)hipFree(G->z[l]);
+    }
+    delete G;
+}
+
+int cicy_graph_set_target(cicy_graph *G, const double *v_true) {
+    if (!G || !v_true || !G->training) return 0;
+    std::vector<float> t((size_t)G->n);
+    for (int i = 0; i < G->n; i++) t[i] = (float)(v_true[i] / G->v_app);
+    bool ok = gh2d(G->ctx, G->target, t.data(), (size_t)G->n) &&
+              hip_ok(G->ctx, hipStreamSynchronize(G->ctx->stream), "sync");
+    G->have_target = ok;
+    return ok ? 1 : 0;
+}
+
+int cicy_mpnn_forward(cicy_mpnn *M, cicy_graph *G, double *v_out, double *loss) {
+    if (!M || !G || M->ctx != G->ctx) return 0;
+    cicy_ctx *ctx = M->ctx;
+    (void)hipSetDevice(ctx->device);
+    const bool train = G->training && G->have_target;
+    if (loss && !train) {
+        set_error(ctx, "loss needs a training graph with a target");
+        return 0;
+    }
+    if (!forward(M, G, G->vout, train)) return 0;
+    bool ok = true;
+    if (v_out)
+        ok = ok && hip_ok(ctx, hipMemcpyAsync(v_out, G->vout, (size_t)G->n * sizeof(double),
+                                              hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    std::vector<double> lp;
+    const int gd = grid_for(G->n);
+    if (loss) {
+        lp.resize((size_t)gd);
+        ok = ok && hip_ok(ctx, hipMemcpyAsync(lp.data(), G->loss_part, gd * sizeof(double),
+                                              hipMemcpyDeviceToHost, ctx->stream), "D2H");
+    }
+    ok = ok && hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (ok && loss) {
+        double s = 0.0;
+        for (double x : lp) s += x;
+        *loss = s / G->n;
+    }
+    return ok ? 1 : 0;
+}
+
+int cicy_mpnn_backward(cicy_mpnn *M, cicy_graph *G) {
+    if (!M || !G || !G->training || !G->have_target) return 0;
+    cicy_ctx *ctx = M->ctx;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const int gd = grid_for(G->n);
+    // G->dh holds dL/dh_L from the readout of the last forward
+    float *dh = G->dh, *dh_in = G->dh2;
+    for (int l = LAYERS - 1; l >= 0; l--) {
+        hipLaunchKernelGGL(k_layer_bwd, dim3(gd), dim3(kThreads), 0, st, G->n, M->thb, l, dh,
+                           G->z[l], G->h[l], G->agg[l], dh_in, G->dagg, M->grad);
+        hipLaunchKernelGGL(k_aggregate_bwd, dim3(gd), dim3(kThreads), 0, st, G->n, G->nptr,
+                           G->nbr, G->wrev, G->e, M->thb, l, G->dagg, G->h[l], dh_in, M->grad);
+        std::swap(dh, dh_in);
+    }
+    hipLaunchKernelGGL(k_encode_bwd, dim3(gd), dim3(kThreads), 0, st, G->n, G->feat, G->z0, dh,
+                       M->grad);
+    return hip_ok(ctx, hipGetLastError(), "mpnn backward launch") &&
+                   hip_ok(ctx, hipStreamSynchronize(st), "sync")
+               ? 1 : 0;
+}
+
+int cicy_mpnn_adam_step(cicy_mpnn *M, double lr, double beta1, double beta2, double eps,
+                        double grad_scale) {
+    if (!M) return 0;
+    cicy_ctx *ctx = M->ctx;
+    (void)hipSetDevice(ctx->device);
+    M->step += 1;
+    const double bc1 = 1.0 - std::pow(beta1, (double)M->step);
+    const double bc2 = 1.0 - std::pow(beta2, (double)M->step);
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       ctx->stream, NPARAM, M->theta, M->grad, M->m, M->v, M->thb, (float)lr,
+                       (float)beta1, (float)beta2, (float)eps, (float)bc1, (float)bc2,
+                       (float)grad_scale);
+    return hip_ok(ctx, hipGetLastError(), "adam launch") &&
+                   hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync")
+               ? 1 : 0;
+}
+
+int cicy_mpnn_warm_start(cicy_mpnn *M, cicy_graph *G, cicy_sys *s, double *ms_out) {
+    if (!M || !G || !s || s->n != G->n || !s->has_b) {
+        if (M) set_error(M->ctx, "cicy_mpnn_warm_start: graph/system mismatch or no b");
+        return 0;
+    }
+    cicy_ctx *ctx = M->ctx;
+    (void)hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    if (!hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "event")) return 0;
+    // x0 = V^ straight into the system, then r0 = b - A x0 with the FP64 SSS kernel
+    if (!forward(M, G, s->x0, false)) return 0;
+    CICY_RB_SWITCH(s->rb, spmv_x0<RB>(s));
+    hipLaunchKernelGGL(k_residual_from, dim3((s->n + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       st, s->n, s->b, s->q, s->r0);
+    if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "event") ||
+        !hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "sync"))
+        return 0;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    if (ms_out) *ms_out = ms;
+    return 1;
+}
+
+}  // extern "C"

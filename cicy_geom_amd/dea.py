@@ -28,7 +28,11 @@ EXPORTED_SYMBOLS = [
     "cicy_boundary_current_f64", "cicy_sys_create", "cicy_sys_destroy", "cicy_sys_size",
     "cicy_sys_set_rhs", "cicy_sys_solve", "cicy_sys_get", "cicy_sys_spmv_bench",
     "cicy_batch_create", "cicy_batch_destroy", "cicy_batch_size", "cicy_batch_set_rhs",
-    "cicy_batch_solve", "cicy_batch_get", "cicy_host_blocked_spmv",
+    "cicy_batch_solve", "cicy_batch_get", "cicy_mpnn_num_params", "cicy_mpnn_create",
+    "cicy_mpnn_destroy", "cicy_mpnn_set_params", "cicy_mpnn_get_params", "cicy_mpnn_get_grads",
+    "cicy_mpnn_zero_grads", "cicy_graph_create", "cicy_graph_destroy", "cicy_graph_set_target",
+    "cicy_mpnn_forward", "cicy_mpnn_backward", "cicy_mpnn_adam_step", "cicy_mpnn_warm_start",
+    "cicy_host_blocked_spmv",
 ]
 
 
@@ -87,6 +91,24 @@ def lib():
     L.cicy_batch_solve.argtypes = [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_int64,
                                    ctypes.c_int, P(SolveStats), P(ctypes.c_double)]
     L.cicy_batch_get.argtypes = [_VP, ctypes.c_int, _VP, _VP]
+    _F32 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.cicy_mpnn_num_params.restype = ctypes.c_int64
+    L.cicy_mpnn_num_params.argtypes = []
+    L.cicy_mpnn_create.argtypes = [_VP, _VP, P(_VP)]
+    L.cicy_mpnn_destroy.argtypes = [_VP]
+    L.cicy_mpnn_set_params.argtypes = [_VP, _F32]
+    L.cicy_mpnn_get_params.argtypes = [_VP, _F32]
+    L.cicy_mpnn_get_grads.argtypes = [_VP, _F32]
+    L.cicy_mpnn_zero_grads.argtypes = [_VP]
+    L.cicy_graph_create.argtypes = [_VP, ctypes.c_int64, ctypes.c_int64, _I64, _I64, _F64, _F64,
+                                    _F64, _F64, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_int, P(_VP)]
+    L.cicy_graph_destroy.argtypes = [_VP]
+    L.cicy_graph_set_target.argtypes = [_VP, _F64]
+    L.cicy_mpnn_forward.argtypes = [_VP, _VP, _VP, P(ctypes.c_double)]
+    L.cicy_mpnn_backward.argtypes = [_VP, _VP]
+    L.cicy_mpnn_adam_step.argtypes = [_VP] + [ctypes.c_double] * 5
+    L.cicy_mpnn_warm_start.argtypes = [_VP, _VP, _VP, P(ctypes.c_double)]
     _bound = L
     return L
 

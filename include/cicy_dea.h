@@ -140,6 +140,41 @@ int cicy_batch_solve(cicy_batch *batch, int method, double tol, int64_t max_iter
                      cicy_solve_stats *per_sys, double *total_ms);
 int cicy_batch_get(cicy_batch *batch, int sys, double *x, double *r);
 
+/* ---- learned warm start (no counterpart in the reference; its warm start is the ramp of
+ * Initial_guess_for_CG, Direct_Electrifying.cpp:1396-1470) ------------------------------
+ * A 3-layer message-passing network (32 channels, BF16 weights, FP32 accumulation) over
+ * the conductance graph predicts the voltage field.  Parameters are a flat FP32 vector of
+ * cicy_mpnn_num_params() entries (layout in dea_mpnn.hip); training runs on the device:
+ * forward (with loss) -> backward (accumulates gradients) -> Adam step. */
+typedef struct cicy_mpnn cicy_mpnn;
+typedef struct cicy_graph cicy_graph;
+int64_t cicy_mpnn_num_params(void);
+int cicy_mpnn_create(cicy_ctx *ctx, const float *theta /* may be NULL: zeros */,
+                     cicy_mpnn **out);
+void cicy_mpnn_destroy(cicy_mpnn *mpnn);
+int cicy_mpnn_set_params(cicy_mpnn *mpnn, const float *theta);
+int cicy_mpnn_get_params(cicy_mpnn *mpnn, float *theta);
+int cicy_mpnn_get_grads(cicy_mpnn *mpnn, float *grad);
+int cicy_mpnn_zero_grads(cicy_mpnn *mpnn);
+/* Graph of one SSS system: node_xyz[n*3] are the free nodes' coordinates, box the RVE
+ * side, direction the current direction (0 x, 1 y, 2 z), b the zero-guess right-hand
+ * side.  training != 0 keeps the activations needed by backward. */
+int cicy_graph_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                      const int64_t *col_ind, const double *values, const double *diag,
+                      const double *b, const double *node_xyz, double box, int direction,
+                      double v_app, int training, cicy_graph **out);
+void cicy_graph_destroy(cicy_graph *graph);
+int cicy_graph_set_target(cicy_graph *graph, const double *v_true);
+/* v_out[n] (may be NULL) receives the predicted voltages; loss (may be NULL) the MSE of
+ * V^/V_app against the target (training graphs). */
+int cicy_mpnn_forward(cicy_mpnn *mpnn, cicy_graph *graph, double *v_out, double *loss);
+int cicy_mpnn_backward(cicy_mpnn *mpnn, cicy_graph *graph);
+int cicy_mpnn_adam_step(cicy_mpnn *mpnn, double lr, double beta1, double beta2, double eps,
+                        double grad_scale);
+/* Predict straight into a device-resident system: x0 := V^, r0 := b - A x0 (the system
+ * must have b).  ms (may be NULL) receives the device time. */
+int cicy_mpnn_warm_start(cicy_mpnn *mpnn, cicy_graph *graph, cicy_sys *sys, double *ms);
+
 /* ---- test hook: the blocked SSS layout applied on the host (no GPU) -------------------
  * Builds the device layout for rows_per_block and applies it with host loops, so the
  * layout can be checked where no GPU is present. */

@@ -1,0 +1,104 @@
+"""Warm-start network on the GPU: forward/backward against the float64 oracle
+(oracle/mpnn.py), training on solver-generated data, and the warm start judged against
+zero-guess CG."""
+import numpy as np
+import pytest
+
+from oracle import mpnn as ompnn
+from cicy_geom_amd import dea, network, warmstart
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dea.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def c1sys():
+    net = network.generate(1)
+    s = network.largest_percolating_system(net, 0, with_xyz=True)
+    return s, net.params.box
+
+
+def oracle_graph(s, box):
+    return ompnn.graph_from_sss(s.row_ptr, s.col_ind, s.values, s.diag, s.b, s.node_xyz, box, 0,
+                                1.0)
+
+
+def test_layouts_agree():
+    assert [n for n, _ in warmstart.param_layout()] == [n for n, _ in ompnn.param_layout()]
+    assert warmstart.init_params().size == ompnn.n_params() == int(dea.lib().cicy_mpnn_num_params())
+
+
+def test_forward_and_gradients_match_oracle(ctx, c1sys):
+    s, box = c1sys
+    rng = np.random.default_rng(0)
+    theta = warmstart.init_params(7)
+    theta[-33:-1] = rng.standard_normal(32).astype(np.float32) * 0.2   # w_out
+    th_bf16 = ompnn.bf16_round(theta)            # the device computes with BF16 weights
+    net = warmstart.WarmStartNet(ctx, theta)
+    g = warmstart.Graph(ctx, s, box, 0, 1.0, training=True)
+    vt = rng.uniform(0, 1, s.n)
+    g.set_target(vt)
+    net.zero_grads()
+    v_gpu, loss_gpu = net.forward(g, want_loss=True)
+    gr = oracle_graph(s, box)
+    v_ref = ompnn.forward(th_bf16, gr)
+    assert np.max(np.abs(v_gpu - v_ref)) < 1e-4 * max(1.0, np.abs(v_ref).max())
+    # the device keeps the target in FP32
+    loss_ref, grad_ref = ompnn.loss_and_grad(th_bf16, gr, vt.astype(np.float32).astype(float))
+    assert abs(loss_gpu - loss_ref) <= 1e-5 * loss_ref
+    net.zero_grads()
+    net.loss_and_grad(g)
+    grad_gpu = net.grads().astype(np.float64)
+    rel = np.linalg.norm(grad_gpu - grad_ref) / np.linalg.norm(grad_ref)
+    print(f"loss gpu={loss_gpu:.6e} ref={loss_ref:.6e} grad rel err={rel:.2e}")
+    assert rel < 1e-3
+    # every parameter block agrees, not only the large ones
+    o = 0
+    for name, shape in ompnn.param_layout():
+        k = int(np.prod(shape))
+        a, b = grad_gpu[o:o + k], grad_ref[o:o + k]
+        assert np.linalg.norm(a - b) <= 2e-3 * np.linalg.norm(b) + 1e-9, name
+        o += k
+    g.close()
+    net.close()
+
+
+def test_training_reduces_loss_and_warm_start_is_judged(ctx):
+    train = []
+    for k, ncnt in enumerate([400, 600, 800, 500, 700, 900]):
+        net = network.generate(4, realisation=100 + k, n_cnt=ncnt)
+        ks = net.percolating_clusters(0)
+        if not ks:
+            continue
+        s = net.assemble(max(ks, key=lambda c: net.cluster(c)[0]), 1, 0, direction=0,
+                         with_xyz=True)
+        train.append((s, net.params.box, 0, 1.0))
+    graphs = warmstart.training_set(ctx, train)
+    assert len(graphs) >= 4
+    model = warmstart.WarmStartNet(ctx, seed=1)
+    hist = model.train(graphs, epochs=40, lr=3e-3)
+    print("loss", hist[0], "->", hist[-1])
+    assert hist[-1] < 0.5 * hist[0]
+    # held-out: iterations from the learned guess vs zero guess, same stopping rule
+    net = network.generate(4, realisation=999, n_cnt=800)
+    ks = net.percolating_clusters(0)
+    s = net.assemble(max(ks, key=lambda c: net.cluster(c)[0]), 1, 0, direction=0, with_xyz=True)
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    sysd.set_rhs(s.b, None, s.b)
+    st0 = sysd.solve(dea.METHOD_PCG_F64, 1e-10, 0, dea.REL_B)
+    x0sol, _ = sysd.get()
+    g = warmstart.Graph(ctx, s, net.params.box, 0, 1.0)
+    ms = model.warm_start(g, sysd)
+    st1 = sysd.solve(dea.METHOD_PCG_F64, 1e-10, 0, dea.REL_B)
+    x1sol, _ = sysd.get()
+    print(f"zero-guess iters {st0.iters}, learned-guess iters {st1.iters}, inference {ms:.3f} ms")
+    assert st0.status == dea.CICY_OK and st1.status == dea.CICY_OK
+    assert st1.true_rel_res <= 1.05e-10
+    assert np.linalg.norm(x1sol - x0sol) / np.linalg.norm(x0sol) < 1e-8
+    sysd.close()
