@@ -31,6 +31,10 @@ struct cicy_batch {
     int *rp = nullptr, *col = nullptr, *urp = nullptr, *ucol = nullptr;
     double *val = nullptr, *diag = nullptr, *uval = nullptr;
     float *val32 = nullptr, *uval32 = nullptr;
+    int format = kFormatSss, nx = 0;
+    int *xblk = nullptr, *xrow = nullptr, *xnbr = nullptr;
+    double *band = nullptr, *xval = nullptr;
+    float *band32 = nullptr, *xval32 = nullptr;
     int *blk_sys = nullptr, *blk_row0 = nullptr, *sys_row0 = nullptr, *sys_blk0 = nullptr;
     int *active[2] = {nullptr, nullptr};
     int *h_active[2] = {nullptr, nullptr};  // pinned staging
@@ -87,14 +91,24 @@ BatchDev bdev(const cicy_batch *B, const int *active) {
     return BatchDev{active, B->blk_sys, B->blk_row0, B->sys_row0, B->sys_blk0};
 }
 
-SssDev<double> v64(const cicy_batch *B) {
-    return SssDev<double>{B->n, B->nblocks, B->rp, B->col, B->val, B->diag, B->urp, B->ucol,
-                          B->uval};
+DevMat<double> m64(const cicy_batch *B) {
+    DevMat<double> m;
+    m.format = B->format;
+    m.sss = SssDev<double>{B->n, B->nblocks, B->rp, B->col, B->val, B->diag, B->urp, B->ucol,
+                           B->uval};
+    m.band = BandDev<double>{B->n, B->nblocks, B->band, B->diag, B->xblk, B->xrow, B->xnbr,
+                             B->xval};
+    return m;
 }
 
-SssDev<float> v32(const cicy_batch *B) {
-    return SssDev<float>{B->n, B->nblocks, B->rp, B->col, B->val32, nullptr, B->urp, B->ucol,
-                         B->uval32};
+DevMat<float> m32(const cicy_batch *B) {
+    DevMat<float> m;
+    m.format = B->format;
+    m.sss = SssDev<float>{B->n, B->nblocks, B->rp, B->col, B->val32, nullptr, B->urp, B->ucol,
+                          B->uval32};
+    m.band = BandDev<float>{B->n, B->nblocks, B->band32, nullptr, B->xblk, B->xrow, B->xnbr,
+                            B->xval32};
+    return m;
 }
 
 template <int RB>
@@ -104,9 +118,8 @@ void chunk_f64(cicy_batch *B, const int *act, int grid) {
     for (int it = 0; it < kChunk; it++) {
         double *pold = (it & 1) ? B->p1 : B->p0;
         double *pnew = (it & 1) ? B->p0 : B->p1;
-        hipLaunchKernelGGL((k_apply<double, RB, true, false, true>), dim3(grid), dim3(kThreads),
-                           0, st, v64(B), bd, pold, B->r, pnew, B->q, B->st, B->partials,
-                           nullptr);
+        launch_apply<double, RB, true, false, true>(m64(B), grid, st, bd, pold, B->r, pnew, B->q,
+                                                    B->st, B->partials, nullptr);
         hipLaunchKernelGGL((k_update_f64<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
                            B->diag, B->x, B->r, pnew, B->q, B->st, B->partials);
     }
@@ -119,15 +132,15 @@ void chunk_mixed(cicy_batch *B, const int *act, int grid) {
     for (int it = 0; it < kChunk; it++) {
         float *pold = (it & 1) ? B->ps1 : B->ps0;
         float *pnew = (it & 1) ? B->ps0 : B->ps1;
-        hipLaunchKernelGGL((k_apply<float, RB, true, true, true>), dim3(grid), dim3(kThreads), 0,
-                           st, v32(B), bd, pold, B->rs, pnew, B->qs, B->st, B->partials, nullptr);
+        launch_apply<float, RB, true, true, true>(m32(B), grid, st, bd, pold, B->rs, pnew, B->qs,
+                                                  B->st, B->partials, nullptr);
         hipLaunchKernelGGL((k_update_mixed<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n,
                            bd, B->xs, B->rs, pnew, B->qs, B->st, B->mix, B->partials);
     }
     hipLaunchKernelGGL((k_fold<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd, B->diag,
                        B->x, B->xs, B->mix, 0);
-    hipLaunchKernelGGL((k_apply<double, RB, false, false, true>), dim3(grid), dim3(kThreads), 0,
-                       st, v64(B), bd, B->x, nullptr, nullptr, B->q, nullptr, nullptr, B->mix);
+    launch_apply<double, RB, false, false, true>(m64(B), grid, st, bd, B->x, nullptr, nullptr,
+                                                 B->q, nullptr, nullptr, B->mix);
     hipLaunchKernelGGL((k_reliable_resid<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
                        B->diag, B->b, B->q, B->r, B->rs, B->st, B->mix, B->partials,
                        kStallLimit);
@@ -153,9 +166,8 @@ void final_norms(cicy_batch *B) {
     const BatchDev bd = bdev(B, nullptr);
     hipLaunchKernelGGL((k_fold<RB, true>), dim3(B->nblocks), dim3(kThreads), 0, st, B->n, bd,
                        B->diag, B->x, B->xs, B->mix, 1);
-    hipLaunchKernelGGL((k_apply<double, RB, false, false, true>), dim3(B->nblocks),
-                       dim3(kThreads), 0, st, v64(B), bd, B->x, nullptr, nullptr, B->q, nullptr,
-                       nullptr, nullptr);
+    launch_apply<double, RB, false, false, true>(m64(B), B->nblocks, st, bd, B->x, nullptr,
+                                                 nullptr, B->q, nullptr, nullptr, nullptr);
     hipLaunchKernelGGL((k_batch_norms<RB>), dim3(B->nblocks), dim3(kThreads), 0, st, B->n, bd,
                        B->b, B->q, B->norms, B->ncount, B->partials);
 }
@@ -254,31 +266,78 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
     B->nblocks = (int)tb;
     B->nremote = (int)trem;
     B->bnorm2.assign((size_t)nsys, -1.0);
-    std::vector<int> rp((size_t)tn + 1), col((size_t)tnnz), urp((size_t)tn + 1),
-        ucol((size_t)trem);
-    std::vector<double> val((size_t)tnnz), dg((size_t)tn), uval((size_t)trem);
-    std::vector<float> val32((size_t)tnnz), uval32((size_t)trem);
+    // one layout for the whole batch: band when most entries of the batch are sub-diagonal
+    if (ctx->format == kFormatSss || ctx->format == kFormatBand) {
+        B->format = ctx->format;
+    } else {
+        double on = 0.0;
+        for (int s = 0; s < nsys; s++) on += band_fraction(hs[s]) * hs[s].nnz;
+        B->format = (tnnz == 0 || on >= 0.5 * (double)tnnz) ? kFormatBand : kFormatSss;
+    }
+    const bool band_fmt = B->format == kFormatBand;
+    std::vector<BandHost> bhs(band_fmt ? (size_t)nsys : 0);
+    long long tx = 0;
+    if (band_fmt) {
+        for (int s = 0; s < nsys; s++) {
+            build_band(hs[s], bhs[s]);
+            tx += bhs[s].m;
+        }
+        if (tx >= (1LL << 31) - 1) {
+            set_error(ctx, "batch too large for 32-bit device indices");
+            delete B;
+            return 0;
+        }
+    }
+    B->nx = (int)tx;
+    const size_t nsss = band_fmt ? 0 : 1;  // SSS arrays only for the SSS layout
+    std::vector<int> rp(nsss * ((size_t)tn + 1)), col(nsss * (size_t)tnnz),
+        urp(nsss * ((size_t)tn + 1)), ucol(nsss * (size_t)trem);
+    std::vector<double> val(nsss * (size_t)tnnz), dg((size_t)tn), uval(nsss * (size_t)trem);
+    std::vector<float> val32(nsss * (size_t)tnnz), uval32(nsss * (size_t)trem);
+    const size_t nb1 = band_fmt ? 1 : 0;
+    std::vector<double> band(nb1 * ((size_t)tn + 1), 0.0), xval(nb1 * (size_t)tx);
+    std::vector<float> band32(nb1 * ((size_t)tn + 1), 0.f), xval32(nb1 * (size_t)tx);
+    std::vector<int> xblk(nb1 * ((size_t)tb + 1)), xrow(nb1 * (size_t)tx), xnbr(nb1 * (size_t)tx);
+    int xo = 0;
     B->h_sys_row0.assign((size_t)nsys + 1, 0);
     B->h_sys_blk0.assign((size_t)nsys + 1, 0);
     int ro = 0, jo = 0, uo = 0, bo = 0;
     for (int s = 0; s < nsys; s++) {
         const BlockedHost &h = hs[s];
-        std::vector<float> a32, u32;
-        scaled_values(h, a32, u32);
-        for (int i = 0; i < h.n; i++) {
-            rp[ro + i] = jo + h.rp[i];
-            urp[ro + i] = uo + h.urp[i];
-            dg[ro + i] = h.diag[i];
-        }
-        for (int j = 0; j < h.nnz; j++) {
-            col[jo + j] = ro + h.col[j];
-            val[jo + j] = h.val[j];
-            val32[jo + j] = a32[j];
-        }
-        for (int j = 0; j < h.nremote; j++) {
-            ucol[uo + j] = ro + h.ucol[j];
-            uval[uo + j] = h.uval[j];
-            uval32[uo + j] = u32[j];
+        for (int i = 0; i < h.n; i++) dg[ro + i] = h.diag[i];
+        if (!band_fmt) {
+            std::vector<float> a32, u32;
+            scaled_values(h, a32, u32);
+            for (int i = 0; i < h.n; i++) {
+                rp[ro + i] = jo + h.rp[i];
+                urp[ro + i] = uo + h.urp[i];
+            }
+            for (int j = 0; j < h.nnz; j++) {
+                col[jo + j] = ro + h.col[j];
+                val[jo + j] = h.val[j];
+                val32[jo + j] = a32[j];
+            }
+            for (int j = 0; j < h.nremote; j++) {
+                ucol[uo + j] = ro + h.ucol[j];
+                uval[uo + j] = h.uval[j];
+                uval32[uo + j] = u32[j];
+            }
+        } else {
+            const BandHost &bh = bhs[s];
+            std::vector<float> a32, x32;
+            scaled_band(h, bh, a32, x32);
+            for (int i = 0; i < h.n; i++) {
+                band[ro + i] = bh.band[i];
+                band32[ro + i] = a32[i];
+            }
+            for (int k = 0; k < bh.m; k++) {
+                xrow[xo + k] = ro + bh.xrow[k];
+                xnbr[xo + k] = ro + bh.xnbr[k];
+                xval[xo + k] = bh.xval[k];
+                xval32[xo + k] = x32[k];
+            }
+            for (int k = 0; k < h.nblocks; k++) xblk[bo + k] = xo + bh.xblk[k];
+            xo += bh.m;
         }
         for (int k = 0; k < h.nblocks; k++) {
             B->h_blk_sys.push_back(s);
@@ -291,14 +350,33 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
         B->h_sys_row0[s + 1] = ro;
         B->h_sys_blk0[s + 1] = bo;
     }
-    rp[tn] = jo;
-    urp[tn] = uo;
     const size_t N = (size_t)tn, NB = (size_t)tb;
-    bool ok = balloc(B, &B->rp, N + 1) && balloc(B, &B->col, (size_t)tnnz) &&
-              balloc(B, &B->val, (size_t)tnnz) && balloc(B, &B->diag, N) &&
-              balloc(B, &B->urp, N + 1) && balloc(B, &B->ucol, (size_t)trem) &&
-              balloc(B, &B->uval, (size_t)trem) && balloc(B, &B->val32, (size_t)tnnz) &&
-              balloc(B, &B->uval32, (size_t)trem) && balloc(B, &B->blk_sys, NB) &&
+    bool ok = balloc(B, &B->diag, N);
+    if (!band_fmt) {
+        rp[tn] = jo;
+        urp[tn] = uo;
+        ok = ok && balloc(B, &B->rp, N + 1) && balloc(B, &B->col, (size_t)tnnz) &&
+             balloc(B, &B->val, (size_t)tnnz) && balloc(B, &B->urp, N + 1) &&
+             balloc(B, &B->ucol, (size_t)trem) && balloc(B, &B->uval, (size_t)trem) &&
+             balloc(B, &B->val32, (size_t)tnnz) && balloc(B, &B->uval32, (size_t)trem) &&
+             bh2d(B, B->rp, rp.data(), N + 1) && bh2d(B, B->col, col.data(), (size_t)tnnz) &&
+             bh2d(B, B->val, val.data(), (size_t)tnnz) && bh2d(B, B->urp, urp.data(), N + 1) &&
+             bh2d(B, B->ucol, ucol.data(), (size_t)trem) &&
+             bh2d(B, B->uval, uval.data(), (size_t)trem) &&
+             bh2d(B, B->val32, val32.data(), (size_t)tnnz) &&
+             bh2d(B, B->uval32, uval32.data(), (size_t)trem);
+    } else {
+        xblk[tb] = xo;
+        ok = ok && balloc(B, &B->band, N + 1) && balloc(B, &B->band32, N + 1) &&
+             balloc(B, &B->xblk, NB + 1) && balloc(B, &B->xrow, (size_t)tx) &&
+             balloc(B, &B->xnbr, (size_t)tx) && balloc(B, &B->xval, (size_t)tx) &&
+             balloc(B, &B->xval32, (size_t)tx) && bh2d(B, B->band, band.data(), N + 1) &&
+             bh2d(B, B->band32, band32.data(), N + 1) && bh2d(B, B->xblk, xblk.data(), NB + 1) &&
+             bh2d(B, B->xrow, xrow.data(), (size_t)tx) && bh2d(B, B->xnbr, xnbr.data(), (size_t)tx) &&
+             bh2d(B, B->xval, xval.data(), (size_t)tx) &&
+             bh2d(B, B->xval32, xval32.data(), (size_t)tx);
+    }
+    ok = ok && balloc(B, &B->blk_sys, NB) &&
               balloc(B, &B->blk_row0, NB) && balloc(B, &B->sys_row0, (size_t)nsys + 1) &&
               balloc(B, &B->sys_blk0, (size_t)nsys + 1) && balloc(B, &B->active[0], NB) &&
               balloc(B, &B->active[1], NB) && balloc(B, &B->x, N) && balloc(B, &B->r, N) &&
@@ -317,12 +395,7 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
                                    hipHostMallocDefault), "hipHostMalloc") &&
          hip_ok(ctx, hipHostMalloc((void **)&B->h_st[1], nsys * sizeof(PcgState),
                                    hipHostMallocDefault), "hipHostMalloc");
-    ok = ok && bh2d(B, B->rp, rp.data(), N + 1) && bh2d(B, B->col, col.data(), (size_t)tnnz) &&
-         bh2d(B, B->val, val.data(), (size_t)tnnz) && bh2d(B, B->diag, dg.data(), N) &&
-         bh2d(B, B->urp, urp.data(), N + 1) && bh2d(B, B->ucol, ucol.data(), (size_t)trem) &&
-         bh2d(B, B->uval, uval.data(), (size_t)trem) &&
-         bh2d(B, B->val32, val32.data(), (size_t)tnnz) &&
-         bh2d(B, B->uval32, uval32.data(), (size_t)trem) &&
+    ok = ok && bh2d(B, B->diag, dg.data(), N) &&
          bh2d(B, B->blk_sys, B->h_blk_sys.data(), NB) &&
          bh2d(B, B->blk_row0, B->h_blk_row0.data(), NB) &&
          bh2d(B, B->sys_row0, B->h_sys_row0.data(), (size_t)nsys + 1) &&
@@ -348,6 +421,7 @@ void cicy_batch_destroy(cicy_batch *B) {
     (void)hipStreamSynchronize(B->ctx->stream);
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     void *dev[] = {B->rp, B->col, B->urp, B->ucol, B->val, B->diag, B->uval, B->val32, B->uval32,
+                   B->band, B->band32, B->xblk, B->xrow, B->xnbr, B->xval, B->xval32,
                    B->blk_sys, B->blk_row0, B->sys_row0, B->sys_blk0, B->active[0],
                    B->active[1], B->x, B->r, B->p0, B->p1, B->q, B->x0, B->r0, B->b,
                    B->stop_ref2, B->rs, B->xs, B->ps0, B->ps1, B->qs, B->partials, B->norms,
@@ -369,6 +443,7 @@ int64_t cicy_batch_size(const cicy_batch *B, int what) {
         case 3: return (int64_t)B->device_bytes;
         case 4: return B->nsys;
         case 5: return B->nblocks;
+        case 6: return B->format;
         default: return -1;
     }
 }

@@ -30,11 +30,45 @@ struct BlockedHost {
     std::vector<double> val, diag, uval;
 };
 
+// Band + remainder layout (BandDev): band[r] = a(r, r-1), the other entries as symmetric
+// half-entries sorted by row, xblk[b] = first half-entry of row block b.
+struct BandHost {
+    int m = 0;
+    std::vector<double> band, xval;
+    std::vector<int> xblk, xrow, xnbr;
+};
+
+constexpr int kFormatSss = 1;   // blocked SSS with remote transpose list
+constexpr int kFormatBand = 2;  // band + remainder
+
+// Device view of one matrix in either layout.
+template <typename T>
+struct DevMat {
+    int format;
+    SssDev<T> sss;
+    BandDev<T> band;
+};
+
+#ifdef __HIPCC__
+template <typename T, int RB, bool PCG, bool UNIT, bool BATCH>
+inline void launch_apply(const DevMat<T> &m, int grid, hipStream_t st, const BatchDev &bd,
+                         const T *src, const T *rvec, T *pnew, T *q, PcgState *stp,
+                         double *partials, const MixState *guard) {
+    if (m.format == kFormatBand)
+        hipLaunchKernelGGL((k_apply_band<T, RB, PCG, UNIT, BATCH>), dim3(grid), dim3(kThreads), 0,
+                           st, m.band, bd, src, rvec, pnew, q, stp, partials, guard);
+    else
+        hipLaunchKernelGGL((k_apply<T, RB, PCG, UNIT, BATCH>), dim3(grid), dim3(kThreads), 0, st,
+                           m.sss, bd, src, rvec, pnew, q, stp, partials, guard);
+}
+#endif
+
 }  // namespace cicy
 
 struct cicy_ctx {
     int device = 0;
     int rows_per_block = 256;
+    int format = 0;  // 0 auto, kFormatSss, kFormatBand
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
@@ -45,9 +79,15 @@ struct cicy_ctx {
 struct cicy_sys {
     cicy_ctx *ctx = nullptr;
     int n = 0, nnz = 0, rb = 256, nblocks = 0, nremote = 0;
-    // blocked SSS, FP64
+    int format = cicy::kFormatSss;
+    // blocked SSS, FP64 (format kFormatSss)
     int *rp = nullptr, *col = nullptr, *urp = nullptr, *ucol = nullptr;
     double *val = nullptr, *diag = nullptr, *uval = nullptr;
+    // band + remainder (format kFormatBand)
+    int nx = 0;
+    int *xblk = nullptr, *xrow = nullptr, *xnbr = nullptr;
+    double *band = nullptr, *xval = nullptr;
+    float *band32 = nullptr, *xval32 = nullptr;
     // FP64 vectors
     double *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
     double *x0 = nullptr, *r0 = nullptr, *b = nullptr;
@@ -79,11 +119,20 @@ bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
                  cicy_solve_stats *stats);
 
 // dea_mixed.hip
-bool mixed_setup(cicy_sys *s, const BlockedHost &h);
+bool mixed_setup(cicy_sys *s, const BlockedHost &h, const BandHost &bh);
 void mixed_free(cicy_sys *s);
 bool solve_mixed(cicy_sys *s, double tol, long long max_iter, int rel_to,
                  cicy_solve_stats *stats);
 void mixed_spmv32(cicy_sys *s);
 void scaled_values(const BlockedHost &h, std::vector<float> &v32, std::vector<float> &u32);
+// band layout of a validated blocked system (same row blocks); fraction of lower entries
+// on the sub-diagonal
+void build_band(const BlockedHost &h, BandHost &b);
+double band_fraction(const BlockedHost &h);
+void scaled_band(const BlockedHost &h, const BandHost &b, std::vector<float> &band32,
+                 std::vector<float> &xval32);
+int choose_format(const cicy_ctx *ctx, const BlockedHost &h);
+DevMat<double> mat64(const cicy_sys *s);
+DevMat<float> mat32(const cicy_sys *s);
 
 }  // namespace cicy

@@ -75,6 +75,20 @@ struct SssDev {
     const T *__restrict__ uval;
 };
 
+// "Band + remainder" layout of the same SSS matrix: band[r] = a(r, r-1) (0 when row r
+// has no entry in column r-1; band[0] = band[n] = 0) plus every other stored entry as two
+// half-entries (row, nbr, val), sorted by row and indexed per row block by xblk.
+template <typename T>
+struct BandDev {
+    int n, nblocks;
+    const T *__restrict__ band;   // [n+1]
+    const T *__restrict__ diag;   // nullptr: unit diagonal
+    const int *__restrict__ xblk; // [nblocks+1]
+    const int *__restrict__ xrow; // [m]
+    const int *__restrict__ xnbr; // [m]
+    const T *__restrict__ xval;   // [m]
+};
+
 struct BatchDev {
     const int *active;    // [grid] block ids of this launch (nullptr: blockIdx.x)
     const int *blk_sys;   // [blocks] system of each block
@@ -162,9 +176,17 @@ __device__ __forceinline__ bool arrive_last(unsigned int *counter, unsigned int 
 // Fixed-order sum of nb partials by one work-group (deterministic); valid in thread 0.
 __device__ __forceinline__ double reduce_partials(const double *part, int nb, int stride,
                                                   double *lds4) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += kThreads) s += part[(size_t)i * stride];
-    return block_sum(s, lds4);
+    // four independent accumulators keep several loads in flight
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3 * kThreads < nb; i += 4 * kThreads) {
+        s0 += part[(size_t)i * stride];
+        s1 += part[(size_t)(i + kThreads) * stride];
+        s2 += part[(size_t)(i + 2 * kThreads) * stride];
+        s3 += part[(size_t)(i + 3 * kThreads) * stride];
+    }
+    for (; i < nb; i += kThreads) s0 += part[(size_t)i * stride];
+    return block_sum((s0 + s1) + (s2 + s3), lds4);
 }
 
 // Jacobi-preconditioned search direction p = (1/d) r + beta p_old.  One definition so
@@ -257,6 +279,86 @@ __global__ __launch_bounds__(kThreads) void k_apply(SssDev<T> s, BatchDev bd,
             st->counter = 0;
             if (!(pq > 0.0) || !isfinite(alpha)) {
                 st->status = 2;  // CICY_BREAKDOWN
+                st->done = 1;
+            }
+        }
+    }
+}
+
+// q = A p over one row block in the band + remainder layout.  The band part is a
+// three-point stencil on LDS (p[r-1], p[r], p[r+1] with a halo row on each side, band[r],
+// band[r+1]); the remainder half-entries of the block are added with LDS atomics.  Same
+// PCG/SPMV/UNIT/BATCH/guard semantics as k_apply.
+template <typename T, int RB, bool PCG, bool UNIT, bool BATCH>
+__global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev bd,
+                                                         const T *__restrict__ src,
+                                                         const T *__restrict__ rvec,
+                                                         T *__restrict__ pnew,
+                                                         T *__restrict__ q,
+                                                         PcgState *__restrict__ st_base,
+                                                         double *__restrict__ partials,
+                                                         const MixState *__restrict__ guard) {
+    const Blk k = block_info<RB, BATCH>(bd, s.n);
+    PcgState *st = st_base ? st_base + k.sys : nullptr;
+    if (PCG && st->done) return;
+    if (guard && !guard[k.sys].need_update) return;
+    __shared__ T sp[RB + 2];
+    __shared__ T sb[RB + 1];
+    __shared__ T sy[RB];
+    __shared__ double red[8];
+    __shared__ int flag;
+    const int r0 = k.r0;
+    const int nrows = k.nrows;
+    const int lo = BATCH ? bd.sys_row0[k.sys] : 0;
+    const int hi = BATCH ? bd.sys_row0[k.sys + 1] : s.n;
+    const T beta = PCG ? (T)st->beta : (T)0;
+    auto pval = [&](int i) -> T {
+        if (!PCG) return src[i];
+        if (UNIT) return pdir_unit<T>(rvec[i], beta, src[i]);
+        return pdir<T>(rvec[i], s.diag[i], beta, src[i]);
+    };
+    for (int t = threadIdx.x; t < nrows + 2; t += kThreads) {
+        const int i = r0 - 1 + t;
+        T p = (T)0;
+        if (i >= lo && i < hi) p = pval(i);
+        sp[t] = p;
+        if (PCG && t >= 1 && t <= nrows) pnew[i] = p;
+    }
+    for (int t = threadIdx.x; t < nrows + 1; t += kThreads) sb[t] = s.band[r0 + t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < nrows; t += kThreads) {
+        const T pr = sp[t + 1];
+        T y = UNIT ? pr : s.diag[r0 + t] * pr;
+        y = fma(sb[t], sp[t], y);
+        y = fma(sb[t + 1], sp[t + 2], y);
+        sy[t] = y;
+    }
+    __syncthreads();
+    const int e1 = s.xblk[k.b + 1];
+    for (int e = s.xblk[k.b] + threadIdx.x; e < e1; e += kThreads) {
+        const int c = s.xnbr[e];
+        const T pc = (c >= r0 && c < r0 + nrows) ? sp[c - r0 + 1] : pval(c);
+        atomicAdd(&sy[s.xrow[e] - r0], s.xval[e] * pc);
+    }
+    __syncthreads();
+    double part = 0.0;
+    for (int t = threadIdx.x; t < nrows; t += kThreads) {
+        const T qv = sy[t];
+        q[r0 + t] = qv;
+        if (PCG) part += (double)sp[t + 1] * (double)qv;
+    }
+    if (!PCG) return;
+    part = block_sum(part, red);
+    if (threadIdx.x == 0) partials[k.b] = part;
+    if (arrive_last(&st->counter, k.count, &flag)) {
+        double pq = reduce_partials(partials + k.first, k.count, 1, red);
+        if (threadIdx.x == 0) {
+            double alpha = st->rz / pq;
+            st->pq = pq;
+            st->alpha = alpha;
+            st->counter = 0;
+            if (!(pq > 0.0) || !isfinite(alpha)) {
+                st->status = 2;
                 st->done = 1;
             }
         }

@@ -42,17 +42,6 @@ __global__ __launch_bounds__(kThreads) void k_add(int n, double *__restrict__ ou
     if (i < n) out[i] = a[i] + b[i];
 }
 
-SssDev<double> view64(const cicy_sys *s) {
-    SssDev<double> d{s->n, s->nblocks, s->rp, s->col, s->val, s->diag, s->urp, s->ucol, s->uval};
-    return d;
-}
-
-SssDev<float> view32(const cicy_sys *s) {
-    SssDev<float> d{s->n, s->nblocks, s->rp, s->col, s->val32, nullptr, s->urp, s->ucol,
-                    s->uval32};
-    return d;
-}
-
 template <int RB>
 void launch_chunk(cicy_sys *s) {
     hipStream_t st = s->ctx->stream;
@@ -60,15 +49,17 @@ void launch_chunk(cicy_sys *s) {
     for (int it = 0; it < kChunk; it++) {
         float *pold = (it & 1) ? s->ps1 : s->ps0;
         float *pnew = (it & 1) ? s->ps0 : s->ps1;
-        hipLaunchKernelGGL((k_apply<float, RB, true, true, false>), g, b, 0, st, view32(s),
-                           kNoBatch, pold, s->rs, pnew, s->qs, s->st, s->partials, nullptr);
+        launch_apply<float, RB, true, true, false>(mat32(s), s->nblocks, st, kNoBatch, pold,
+                                                   s->rs, pnew, s->qs, s->st, s->partials,
+                                                   nullptr);
         hipLaunchKernelGGL((k_update_mixed<RB, false>), g, b, 0, st, s->n, kNoBatch, s->xs,
                            s->rs, pnew, s->qs, s->st, s->mix, s->partials);
     }
     hipLaunchKernelGGL((k_fold<RB, false>), g, b, 0, st, s->n, kNoBatch, s->diag, s->x, s->xs,
                        s->mix, 0);
-    hipLaunchKernelGGL((k_apply<double, RB, false, false, false>), g, b, 0, st, view64(s),
-                       kNoBatch, s->x, nullptr, nullptr, s->q, nullptr, nullptr, s->mix);
+    launch_apply<double, RB, false, false, false>(mat64(s), s->nblocks, st, kNoBatch, s->x,
+                                                  nullptr, nullptr, s->q, nullptr, nullptr,
+                                                  s->mix);
     hipLaunchKernelGGL((k_reliable_resid<RB, false>), g, b, 0, st, s->n, kNoBatch, s->diag,
                        s->b, s->q, s->r, s->rs, s->st, s->mix, s->partials, kStallLimit);
 }
@@ -89,16 +80,16 @@ void launch_fold_always(cicy_sys *s) {
 
 template <int RB>
 void launch_spmv64(cicy_sys *s, const double *v, double *out) {
-    hipLaunchKernelGGL((k_apply<double, RB, false, false, false>), dim3(s->nblocks),
-                       dim3(kThreads), 0, s->ctx->stream, view64(s), kNoBatch, v, nullptr,
-                       nullptr, out, nullptr, nullptr, nullptr);
+    launch_apply<double, RB, false, false, false>(mat64(s), s->nblocks, s->ctx->stream,
+                                                  kNoBatch, v, nullptr, nullptr, out, nullptr,
+                                                  nullptr, nullptr);
 }
 
 template <int RB>
 void launch_spmv32(cicy_sys *s) {
-    hipLaunchKernelGGL((k_apply<float, RB, false, true, false>), dim3(s->nblocks),
-                       dim3(kThreads), 0, s->ctx->stream, view32(s), kNoBatch, s->ps0, nullptr,
-                       nullptr, s->qs, nullptr, nullptr, nullptr);
+    launch_apply<float, RB, false, true, false>(mat32(s), s->nblocks, s->ctx->stream, kNoBatch,
+                                                s->ps0, nullptr, nullptr, s->qs, nullptr, nullptr,
+                                                nullptr);
 }
 
 bool ensure_graph_mixed(cicy_sys *s) {
@@ -119,29 +110,36 @@ bool ensure_graph_mixed(cicy_sys *s) {
 
 namespace cicy {
 
-bool mixed_setup(cicy_sys *s, const BlockedHost &h) {
+bool mixed_setup(cicy_sys *s, const BlockedHost &h, const BandHost &bh) {
     cicy_ctx *ctx = s->ctx;
     const size_t nv = (size_t)h.n;
-    std::vector<float> v32, u32;
-    scaled_values(h, v32, u32);
     auto alloc = [&](void **p, size_t bytes) {
         if (!hip_ok(ctx, hipMalloc(p, std::max<size_t>(bytes, 4)), "hipMalloc")) return false;
         s->device_bytes += std::max<size_t>(bytes, 4);
         return true;
     };
-    bool ok = alloc((void **)&s->val32, 4 * (size_t)h.nnz) &&
-              alloc((void **)&s->uval32, 4 * (size_t)h.nremote) &&
-              alloc((void **)&s->rs, 4 * nv) && alloc((void **)&s->xs, 4 * nv) &&
-              alloc((void **)&s->ps0, 4 * nv) && alloc((void **)&s->ps1, 4 * nv) &&
-              alloc((void **)&s->qs, 4 * nv) && alloc((void **)&s->mix, sizeof(MixState));
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return !bytes || hip_ok(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                                                    ctx->stream), "H2D");
+    };
+    std::vector<float> a32, b32;
+    bool ok = true;
+    if (s->format == kFormatSss) {
+        scaled_values(h, a32, b32);
+        ok = alloc((void **)&s->val32, 4 * (size_t)h.nnz) &&
+             alloc((void **)&s->uval32, 4 * (size_t)h.nremote) &&
+             up(s->val32, a32.data(), 4 * a32.size()) && up(s->uval32, b32.data(), 4 * b32.size());
+    } else {
+        scaled_band(h, bh, a32, b32);
+        ok = alloc((void **)&s->band32, 4 * (nv + 1)) &&
+             alloc((void **)&s->xval32, 4 * (size_t)bh.m) &&
+             up(s->band32, a32.data(), 4 * a32.size()) && up(s->xval32, b32.data(), 4 * b32.size());
+    }
+    ok = ok && alloc((void **)&s->rs, 4 * nv) && alloc((void **)&s->xs, 4 * nv) &&
+         alloc((void **)&s->ps0, 4 * nv) && alloc((void **)&s->ps1, 4 * nv) &&
+         alloc((void **)&s->qs, 4 * nv) && alloc((void **)&s->mix, sizeof(MixState));
     if (!ok) return false;
-    if (h.nnz)
-        ok = ok && hip_ok(ctx, hipMemcpyAsync(s->val32, v32.data(), 4 * (size_t)h.nnz,
-                                              hipMemcpyHostToDevice, ctx->stream), "H2D");
-    if (h.nremote)
-        ok = ok && hip_ok(ctx, hipMemcpyAsync(s->uval32, u32.data(), 4 * (size_t)h.nremote,
-                                              hipMemcpyHostToDevice, ctx->stream), "H2D");
-    ok = ok && hip_ok(ctx, hipMemsetAsync(s->mix, 0, sizeof(MixState), ctx->stream), "memset") &&
+    ok = hip_ok(ctx, hipMemsetAsync(s->mix, 0, sizeof(MixState), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->ps0, 0, 4 * std::max<size_t>(nv, 1), ctx->stream), "memset") &&
          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return ok;
@@ -163,7 +161,8 @@ void scaled_values(const BlockedHost &h, std::vector<float> &v32, std::vector<fl
 
 void mixed_free(cicy_sys *s) {
     if (s->graph_exec_mixed) (void)hipGraphExecDestroy(s->graph_exec_mixed);
-    void *ptrs[] = {s->val32, s->uval32, s->rs, s->xs, s->ps0, s->ps1, s->qs, s->mix};
+    void *ptrs[] = {s->val32, s->uval32, s->band32, s->xval32, s->rs, s->xs, s->ps0, s->ps1,
+                    s->qs, s->mix};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
 }

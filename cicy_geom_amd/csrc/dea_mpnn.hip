@@ -356,13 +356,10 @@ __global__ __launch_bounds__(kThreads) void k_residual_from(int n, const double 
 
 template <int RB>
 void spmv_x0(cicy_sys *s) {
-    const SssDev<double> d{s->n, s->nblocks, s->rp, s->col, s->val, s->diag, s->urp, s->ucol,
-                           s->uval};
     const BatchDev nb{nullptr, nullptr, nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL((k_apply<double, RB, false, false, false>), dim3(s->nblocks),
-                       dim3(kThreads), 0, s->ctx->stream, d, nb, (const double *)s->x0,
-                       (const double *)nullptr, (double *)nullptr, s->q, (PcgState *)nullptr,
-                       (double *)nullptr, (const MixState *)nullptr);
+    launch_apply<double, RB, false, false, false>(mat64(s), s->nblocks, s->ctx->stream, nb,
+                                                  s->x0, nullptr, nullptr, s->q, nullptr,
+                                                  nullptr, nullptr);
 }
 
 int grid_for(int n) { return std::max(1, std::min((n + NODES_PER_PASS - 1) / NODES_PER_PASS, 4096)); }

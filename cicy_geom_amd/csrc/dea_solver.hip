@@ -162,6 +162,90 @@ bool build_blocked(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr
     return true;
 }
 
+double band_fraction(const BlockedHost &h) {
+    if (h.nnz == 0) return 1.0;
+    long long on = 0;
+    for (int r = 0; r < h.n; r++)
+        for (int j = h.rp[r]; j < h.rp[r + 1]; j++)
+            if (h.col[j] == r - 1) on++;
+    return (double)on / (double)h.nnz;
+}
+
+void build_band(const BlockedHost &h, BandHost &b) {
+    b.band.assign((size_t)h.n + 1, 0.0);
+    std::vector<int> cnt((size_t)h.n + 1, 0);
+    for (int r = 0; r < h.n; r++)
+        for (int j = h.rp[r]; j < h.rp[r + 1]; j++) {
+            const int c = h.col[j];
+            if (c == r - 1) {
+                b.band[r] = h.val[j];
+            } else {
+                cnt[r + 1]++;
+                cnt[c + 1]++;
+            }
+        }
+    for (int i = 0; i < h.n; i++) cnt[i + 1] += cnt[i];
+    b.m = h.n > 0 ? cnt[h.n] : 0;
+    b.xrow.resize((size_t)b.m);
+    b.xnbr.resize((size_t)b.m);
+    b.xval.resize((size_t)b.m);
+    std::vector<int> fill(cnt.begin(), cnt.end() - (h.n > 0 ? 1 : 0));
+    for (int r = 0; r < h.n; r++)
+        for (int j = h.rp[r]; j < h.rp[r + 1]; j++) {
+            const int c = h.col[j];
+            if (c == r - 1) continue;
+            int a = fill[r]++;
+            b.xrow[a] = r;
+            b.xnbr[a] = c;
+            b.xval[a] = h.val[j];
+            a = fill[c]++;
+            b.xrow[a] = c;
+            b.xnbr[a] = r;
+            b.xval[a] = h.val[j];
+        }
+    b.xblk.assign((size_t)h.nblocks + 1, 0);
+    for (int k = 0; k <= h.nblocks; k++) b.xblk[k] = cnt[std::min(h.n, k * h.rb)];
+}
+
+// Jacobi-scaled FP32 values of the band layout (a * s_hi * s_lo, s = 1/sqrt(d))
+void scaled_band(const BlockedHost &h, const BandHost &b, std::vector<float> &band32,
+                 std::vector<float> &xval32) {
+    std::vector<double> sc((size_t)h.n);
+    for (int i = 0; i < h.n; i++) sc[i] = 1.0 / std::sqrt(h.diag[i]);
+    band32.assign((size_t)h.n + 1, 0.f);
+    for (int r = 1; r < h.n; r++) band32[r] = (float)(b.band[r] * sc[r] * sc[r - 1]);
+    xval32.resize((size_t)b.m);
+    for (int k = 0; k < b.m; k++) {
+        const int hi = std::max(b.xrow[k], b.xnbr[k]), lo = std::min(b.xrow[k], b.xnbr[k]);
+        xval32[k] = (float)(b.xval[k] * sc[hi] * sc[lo]);
+    }
+}
+
+int choose_format(const cicy_ctx *ctx, const BlockedHost &h) {
+    if (ctx->format == kFormatSss || ctx->format == kFormatBand) return ctx->format;
+    return band_fraction(h) >= 0.5 ? kFormatBand : kFormatSss;
+}
+
+DevMat<double> mat64(const cicy_sys *s) {
+    DevMat<double> m;
+    m.format = s->format;
+    m.sss = SssDev<double>{s->n, s->nblocks, s->rp, s->col, s->val, s->diag, s->urp, s->ucol,
+                           s->uval};
+    m.band = BandDev<double>{s->n, s->nblocks, s->band, s->diag, s->xblk, s->xrow, s->xnbr,
+                             s->xval};
+    return m;
+}
+
+DevMat<float> mat32(const cicy_sys *s) {
+    DevMat<float> m;
+    m.format = s->format;
+    m.sss = SssDev<float>{s->n, s->nblocks, s->rp, s->col, s->val32, nullptr, s->urp, s->ucol,
+                          s->uval32};
+    m.band = BandDev<float>{s->n, s->nblocks, s->band32, nullptr, s->xblk, s->xrow, s->xnbr,
+                            s->xval32};
+    return m;
+}
+
 }  // namespace cicy
 
 namespace {
@@ -190,20 +274,6 @@ bool d2h(cicy_ctx *ctx, T *dst, const T *src, size_t count) {
                   "hipMemcpyAsync D2H");
 }
 
-SssDev<double> dev_view64(const cicy_sys *s) {
-    SssDev<double> d;
-    d.n = s->n;
-    d.nblocks = s->nblocks;
-    d.rp = s->rp;
-    d.col = s->col;
-    d.val = s->val;
-    d.diag = s->diag;
-    d.urp = s->urp;
-    d.ucol = s->ucol;
-    d.uval = s->uval;
-    return d;
-}
-
 // Dispatch on the rows-per-block template parameter.
 template <template <int> class F, typename... A>
 bool dispatch_rb(int rb, A &&...args) {
@@ -221,9 +291,9 @@ constexpr BatchDev kNoBatch = {nullptr, nullptr, nullptr, nullptr, nullptr};
 template <int RB>
 struct LaunchSpmv64 {
     static void run(cicy_sys *s, const double *v, double *out) {
-        hipLaunchKernelGGL((k_apply<double, RB, false, false, false>), dim3(s->nblocks),
-                           dim3(kThreads), 0, s->ctx->stream, dev_view64(s), kNoBatch, v,
-                           nullptr, nullptr, out, nullptr, nullptr, nullptr);
+        launch_apply<double, RB, false, false, false>(mat64(s), s->nblocks, s->ctx->stream,
+                                                      kNoBatch, v, nullptr, nullptr, out,
+                                                      nullptr, nullptr, nullptr);
     }
 };
 
@@ -242,9 +312,9 @@ struct LaunchIter64 {
     static void run(cicy_sys *s, int parity) {
         double *pold = parity ? s->p1 : s->p0;
         double *pnew = parity ? s->p0 : s->p1;
-        hipLaunchKernelGGL((k_apply<double, RB, true, false, false>), dim3(s->nblocks),
-                           dim3(kThreads), 0, s->ctx->stream, dev_view64(s), kNoBatch, pold,
-                           s->r, pnew, s->q, s->st, s->partials, nullptr);
+        launch_apply<double, RB, true, false, false>(mat64(s), s->nblocks, s->ctx->stream,
+                                                     kNoBatch, pold, s->r, pnew, s->q, s->st,
+                                                     s->partials, nullptr);
         hipLaunchKernelGGL((k_update_f64<RB, false>), dim3(s->nblocks), dim3(kThreads), 0,
                            s->ctx->stream, s->n, kNoBatch, s->diag, s->x, s->r, pnew, s->q,
                            s->st, s->partials);
@@ -431,6 +501,16 @@ int cicy_set_rows_per_block(cicy_ctx *ctx, int rb) {
     return 1;
 }
 
+int cicy_set_format(cicy_ctx *ctx, int format) {
+    if (!ctx) return 0;
+    if (format != 0 && format != kFormatSss && format != kFormatBand) {
+        set_error(ctx, "format must be 0 (auto), 1 (blocked SSS) or 2 (band + remainder)");
+        return 0;
+    }
+    ctx->format = format;
+    return 1;
+}
+
 int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
                     const int64_t *col_ind, const double *values, const double *diagonal,
                     cicy_sys **out) {
@@ -450,21 +530,40 @@ int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_pt
     size_t tot = 0;
     const size_t nv = (size_t)h.n;
     const size_t nparts = 2 * (size_t)std::max(h.nblocks, 1024);
-    bool ok = dmalloc(ctx, &s->rp, nv + 1, &tot) && dmalloc(ctx, &s->col, (size_t)h.nnz, &tot) &&
-              dmalloc(ctx, &s->val, (size_t)h.nnz, &tot) && dmalloc(ctx, &s->diag, nv, &tot) &&
-              dmalloc(ctx, &s->urp, nv + 1, &tot) &&
-              dmalloc(ctx, &s->ucol, (size_t)h.nremote, &tot) &&
-              dmalloc(ctx, &s->uval, (size_t)h.nremote, &tot) && dmalloc(ctx, &s->x, nv, &tot) &&
+    s->format = choose_format(ctx, h);
+    BandHost bh;
+    bool ok = dmalloc(ctx, &s->diag, nv, &tot);
+    if (s->format == kFormatSss) {
+        ok = ok && dmalloc(ctx, &s->rp, nv + 1, &tot) &&
+             dmalloc(ctx, &s->col, (size_t)h.nnz, &tot) &&
+             dmalloc(ctx, &s->val, (size_t)h.nnz, &tot) && dmalloc(ctx, &s->urp, nv + 1, &tot) &&
+             dmalloc(ctx, &s->ucol, (size_t)h.nremote, &tot) &&
+             dmalloc(ctx, &s->uval, (size_t)h.nremote, &tot) &&
+             h2d(ctx, s->rp, h.rp.data(), nv + 1) && h2d(ctx, s->col, h.col.data(), h.nnz) &&
+             h2d(ctx, s->val, h.val.data(), h.nnz) && h2d(ctx, s->urp, h.urp.data(), nv + 1) &&
+             h2d(ctx, s->ucol, h.ucol.data(), h.nremote) &&
+             h2d(ctx, s->uval, h.uval.data(), h.nremote);
+    } else {
+        build_band(h, bh);
+        s->nx = bh.m;
+        ok = ok && dmalloc(ctx, &s->band, nv + 1, &tot) &&
+             dmalloc(ctx, &s->xblk, (size_t)h.nblocks + 1, &tot) &&
+             dmalloc(ctx, &s->xrow, (size_t)bh.m, &tot) && dmalloc(ctx, &s->xnbr, (size_t)bh.m, &tot) &&
+             dmalloc(ctx, &s->xval, (size_t)bh.m, &tot) &&
+             h2d(ctx, s->band, bh.band.data(), nv + 1) &&
+             h2d(ctx, s->xblk, bh.xblk.data(), (size_t)h.nblocks + 1) &&
+             h2d(ctx, s->xrow, bh.xrow.data(), (size_t)bh.m) &&
+             h2d(ctx, s->xnbr, bh.xnbr.data(), (size_t)bh.m) &&
+             h2d(ctx, s->xval, bh.xval.data(), (size_t)bh.m);
+    }
+    ok = ok && dmalloc(ctx, &s->x, nv, &tot) &&
               dmalloc(ctx, &s->r, nv, &tot) && dmalloc(ctx, &s->p0, nv, &tot) &&
               dmalloc(ctx, &s->p1, nv, &tot) && dmalloc(ctx, &s->q, nv, &tot) &&
               dmalloc(ctx, &s->x0, nv, &tot) && dmalloc(ctx, &s->r0, nv, &tot) &&
               dmalloc(ctx, &s->b, nv, &tot) && dmalloc(ctx, &s->partials, nparts, &tot) &&
               dmalloc(ctx, &s->st, 1, &tot) && dmalloc(ctx, &s->scal, 4, &tot) &&
               dmalloc(ctx, &s->counter2, 1, &tot);
-    ok = ok && h2d(ctx, s->rp, h.rp.data(), nv + 1) && h2d(ctx, s->col, h.col.data(), h.nnz) &&
-         h2d(ctx, s->val, h.val.data(), h.nnz) && h2d(ctx, s->diag, h.diag.data(), nv) &&
-         h2d(ctx, s->urp, h.urp.data(), nv + 1) && h2d(ctx, s->ucol, h.ucol.data(), h.nremote) &&
-         h2d(ctx, s->uval, h.uval.data(), h.nremote) &&
+    ok = ok && h2d(ctx, s->diag, h.diag.data(), nv) &&
          hip_ok(ctx, hipMemsetAsync(s->x0, 0, nv * sizeof(double), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->p0, 0, nv * sizeof(double), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->r0, 0, nv * sizeof(double), ctx->stream), "memset") &&
@@ -476,7 +575,7 @@ int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_pt
         cicy_sys_destroy(s);
         return 0;
     }
-    if (!cicy::mixed_setup(s, h)) {
+    if (!cicy::mixed_setup(s, h, bh)) {
         cicy_sys_destroy(s);
         return 0;
     }
@@ -490,9 +589,9 @@ void cicy_sys_destroy(cicy_sys *s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     if (s->graph_exec64) (void)hipGraphExecDestroy(s->graph_exec64);
     cicy::mixed_free(s);
-    void *ptrs[] = {s->rp, s->col, s->val, s->diag, s->urp, s->ucol, s->uval, s->x, s->r,
-                    s->p0, s->p1, s->q, s->x0, s->r0, s->b, s->partials, s->st, s->scal,
-                    s->counter2};
+    void *ptrs[] = {s->rp, s->col, s->val, s->diag, s->urp, s->ucol, s->uval, s->band,
+                    s->xblk, s->xrow, s->xnbr, s->xval, s->x, s->r, s->p0, s->p1, s->q,
+                    s->x0, s->r0, s->b, s->partials, s->st, s->scal, s->counter2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -505,6 +604,8 @@ int64_t cicy_sys_size(const cicy_sys *s, int what) {
         case 1: return s->nnz;
         case 2: return s->nremote;
         case 3: return (int64_t)s->device_bytes;
+        case 4: return s->format;
+        case 5: return s->format == kFormatBand ? s->nx : s->nremote;
         default: return -1;
     }
 }
@@ -724,12 +825,39 @@ int cicy_boundary_current_f64(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
 int cicy_host_blocked_spmv(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
                            const double *values, const double *diagonal, int64_t nnz, int rb,
                            const double *v, double *out) {
+    return cicy_host_layout_spmv(kFormatSss, n, row_ptr, col_ind, values, diagonal, nnz, rb, v,
+                                 out);
+}
+
+int cicy_host_layout_spmv(int format, int64_t n, const int64_t *row_ptr,
+                          const int64_t *col_ind, const double *values, const double *diagonal,
+                          int64_t nnz, int rb, const double *v, double *out) {
     BlockedHost h;
     cicy_ctx dummy;
     if (rb != 128 && rb != 256 && rb != 512 && rb != 1024) return 0;
+    if (format != kFormatSss && format != kFormatBand) return 0;
     if (!build_blocked(&dummy, n, nnz, row_ptr, col_ind, values, diagonal, rb, h)) return 0;
-    // the device algorithm, one row block at a time
     std::vector<double> sy;
+    if (format == kFormatBand) {
+        // the band kernel, one row block at a time
+        BandHost b;
+        build_band(h, b);
+        for (int k = 0; k < h.nblocks; k++) {
+            const int r0 = k * rb, nr = std::min(rb, h.n - r0);
+            sy.assign((size_t)nr, 0.0);
+            for (int t = 0; t < nr; t++) {
+                const int r = r0 + t;
+                double y = h.diag[r] * v[r];
+                if (r > 0) y += b.band[r] * v[r - 1];
+                if (r + 1 < h.n) y += b.band[r + 1] * v[r + 1];
+                sy[t] = y;
+            }
+            for (int e = b.xblk[k]; e < b.xblk[k + 1]; e++) sy[b.xrow[e] - r0] += b.xval[e] * v[b.xnbr[e]];
+            for (int t = 0; t < nr; t++) out[r0 + t] = sy[t];
+        }
+        return 1;
+    }
+    // the blocked SSS kernel, one row block at a time
     for (int b = 0; b < h.nblocks; b++) {
         int r0 = b * rb, nr = std::min(rb, h.n - r0);
         sy.assign((size_t)nr, 0.0);

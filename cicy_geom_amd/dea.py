@@ -14,6 +14,7 @@ from ._libs import dea_lib
 
 CICY_OK, CICY_MAXITER, CICY_BREAKDOWN, CICY_BAD_INPUT, CICY_DEVICE_ERROR = 0, 1, 2, 3, 4
 METHOD_PCG_F64, METHOD_MIXED = 0, 1
+FORMAT_AUTO, FORMAT_SSS, FORMAT_BAND = 0, 1, 2
 REL_R0, REL_B = 0, 1
 STATUS_NAMES = {0: "ok", 1: "maxiter", 2: "breakdown", 3: "bad_input", 4: "device_error"}
 
@@ -24,6 +25,7 @@ _VP = ctypes.c_void_p
 # Every symbol the header declares; tests check that the library exports all of them.
 EXPORTED_SYMBOLS = [
     "cicy_create", "cicy_destroy", "cicy_last_error", "cicy_set_rows_per_block",
+    "cicy_set_format", "cicy_host_layout_spmv",
     "cicy_solve_dea_cg_sss", "cicy_update_residual", "cicy_spmv_sss_f64",
     "cicy_boundary_current_f64", "cicy_sys_create", "cicy_sys_destroy", "cicy_sys_size",
     "cicy_sys_set_rhs", "cicy_sys_solve", "cicy_sys_get", "cicy_sys_spmv_bench",
@@ -61,6 +63,9 @@ def lib():
     L.cicy_last_error.restype = ctypes.c_char_p
     L.cicy_last_error.argtypes = [_VP]
     L.cicy_set_rows_per_block.argtypes = [_VP, ctypes.c_int]
+    L.cicy_set_format.argtypes = [_VP, ctypes.c_int]
+    L.cicy_host_layout_spmv.argtypes = [ctypes.c_int, ctypes.c_int64, _I64, _I64, _F64, _F64,
+                                        ctypes.c_int64, ctypes.c_int, _F64, _F64]
     L.cicy_solve_dea_cg_sss.argtypes = [_VP, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
                                         _I64, _I64, _F64, _F64, ctypes.c_int64, _F64, _F64, _F64,
                                         _F64, ctypes.c_int64, P(SolveStats)]
@@ -134,7 +139,7 @@ class DEAError(RuntimeError):
 class Context:
     """One device context (stream, events); not re-entrant."""
 
-    def __init__(self, device: int = 0, rows_per_block: int = 0):
+    def __init__(self, device: int = 0, rows_per_block: int = 0, fmt: int = FORMAT_AUTO):
         L = lib()
         h = _VP()
         if not L.cicy_create(ctypes.byref(h), device):
@@ -143,6 +148,11 @@ class Context:
         self._L = L
         if rows_per_block:
             self.set_rows_per_block(rows_per_block)
+        if fmt:
+            self.set_format(fmt)
+
+    def set_format(self, fmt: int):
+        self._check(self._L.cicy_set_format(self._h, fmt), "cicy_set_format")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -240,7 +250,8 @@ class System:
 
     def size(self, what: str) -> int:
         return int(self.ctx._L.cicy_sys_size(
-            self._h, {"n": 0, "nnz": 1, "remote": 2, "bytes": 3}[what]))
+            self._h, {"n": 0, "nnz": 1, "remote": 2, "bytes": 3, "format": 4,
+                      "off_layout": 5}[what]))
 
     def set_rhs(self, r0, x0=None, b=None):
         r0 = _f64(r0)
@@ -298,7 +309,8 @@ class Batch:
 
     def size(self, what: str) -> int:
         return int(self.ctx._L.cicy_batch_size(self._h, {
-            "n": 0, "nnz": 1, "remote": 2, "bytes": 3, "systems": 4, "blocks": 5}[what]))
+            "n": 0, "nnz": 1, "remote": 2, "bytes": 3, "systems": 4, "blocks": 5,
+            "format": 6}[what]))
 
     def set_rhs(self, sys_id, r0, x0=None, b=None):
         r0 = _f64(r0)
@@ -323,12 +335,13 @@ class Batch:
         return x[:n], r[:n]
 
 
-def host_blocked_spmv(row_ptr, col_ind, values, diagonal, v, rows_per_block=256):
-    """The device layout applied by host loops (layout check without a GPU)."""
+def host_blocked_spmv(row_ptr, col_ind, values, diagonal, v, rows_per_block=256,
+                      fmt=FORMAT_SSS):
+    """A device layout applied by host loops (layout check without a GPU)."""
     n = len(diagonal)
     out = np.zeros(max(n, 1))
-    ok = lib().cicy_host_blocked_spmv(n, _i64(row_ptr), _i64(col_ind), _f64(values),
-                                      _f64(diagonal), len(values), rows_per_block, _f64(v), out)
+    ok = lib().cicy_host_layout_spmv(fmt, n, _i64(row_ptr), _i64(col_ind), _f64(values),
+                                     _f64(diagonal), len(values), rows_per_block, _f64(v), out)
     if not ok:
         raise DEAError("cicy_host_blocked_spmv rejected the input")
     return out[:n]

@@ -64,6 +64,10 @@ void cicy_destroy(cicy_ctx *ctx);
 const char *cicy_last_error(const cicy_ctx *ctx);
 /* rows per work-group of the SSS kernels (0 = autotuned/default); see cicy_autotune */
 int cicy_set_rows_per_block(cicy_ctx *ctx, int rows_per_block);
+/* device layout of systems created afterwards: 0 auto (band + remainder when at least half
+ * of the stored entries sit on the sub-diagonal, as for tube-by-tube numbering), 1 blocked
+ * SSS with a remote transpose list, 2 band + remainder */
+int cicy_set_format(cicy_ctx *ctx, int format);
 
 /* ---- drop-in seam: Solve_DEA_equations_CG_SSS (Direct_Electrifying.cpp:1538-1659) -----
  * nodes          global node count (reserved + free), as the reference passes it
@@ -108,7 +112,8 @@ int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_pt
                     cicy_sys **out);
 void cicy_sys_destroy(cicy_sys *sys);
 int64_t cicy_sys_size(const cicy_sys *sys, int what); /* 0 n, 1 nnz, 2 remote entries,
-                                                           3 device bytes */
+                                                           3 device bytes, 4 format,
+                                                           5 off-layout entries */
 /* r0 = b - A x0 (the reference's R), x0 (may be NULL = zeros), b (may be NULL; needed for
  * the true residual and for CICY_REL_B). */
 int cicy_sys_set_rhs(cicy_sys *sys, const double *r0, const double *x0, const double *b);
@@ -181,6 +186,10 @@ int cicy_mpnn_warm_start(cicy_mpnn *mpnn, cicy_graph *graph, cicy_sys *sys, doub
 int cicy_host_blocked_spmv(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
                            const double *values, const double *diagonal, int64_t nnz,
                            int rows_per_block, const double *v, double *out);
+/* same for either layout (format 1 blocked SSS, 2 band + remainder) */
+int cicy_host_layout_spmv(int format, int64_t n, const int64_t *row_ptr,
+                          const int64_t *col_ind, const double *values, const double *diagonal,
+                          int64_t nnz, int rows_per_block, const double *v, double *out);
 
 #ifdef __cplusplus
 }

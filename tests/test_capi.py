@@ -37,11 +37,12 @@ def test_all_headers_exported():
                 assert hasattr(lib, n), (h, n)
 
 
+@pytest.mark.parametrize("fmt", [dea.FORMAT_SSS, dea.FORMAT_BAND])
 @pytest.mark.parametrize("rb", [128, 256, 512, 1024])
-def test_blocked_layout_on_host(rb):
+def test_device_layouts_on_host(rb, fmt):
     s = network.largest_percolating_system(network.generate(1), 0)
     v = np.random.default_rng(rb).standard_normal(s.n)
-    got = dea.host_blocked_spmv(s.row_ptr, s.col_ind, s.values, s.diag, v, rb)
+    got = dea.host_blocked_spmv(s.row_ptr, s.col_ind, s.values, s.diag, v, rb, fmt)
     want = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, v)
     assert np.allclose(got, want, rtol=1e-13, atol=1e-15 * np.abs(want).max())
 
@@ -67,3 +68,26 @@ def test_context_needs_gpu_or_raises():
         pytest.skip("a GPU is present")
     with pytest.raises(dea.DEAError):
         dea.Context(0)
+
+
+@pytest.mark.parametrize("fmt", [dea.FORMAT_SSS, dea.FORMAT_BAND])
+def test_layouts_on_unstructured_matrix(fmt):
+    """A matrix with no sub-diagonal structure (random junction graph) in either layout."""
+    rng = np.random.default_rng(4)
+    n = 3000
+    pairs = {}
+    for a, b in rng.integers(0, n, (6000, 2)):
+        if a != b:
+            pairs[(max(a, b), min(a, b))] = -rng.uniform(0.1, 1)
+    keys = sorted(pairs)
+    rp = np.zeros(n + 1, np.int64)
+    for r, _ in keys:
+        rp[r + 1] += 1
+    rp = np.cumsum(rp)
+    col = np.array([c for _, c in keys], np.int64)
+    val = np.array([pairs[k] for k in keys])
+    diag = np.full(n, 10.0)
+    v = rng.standard_normal(n)
+    got = dea.host_blocked_spmv(rp, col, val, diag, v, 256, fmt)
+    want = oracle.spmv_sss(rp, col, val, diag, v)
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-13)

@@ -30,9 +30,13 @@ def systems():
     return out
 
 
+@pytest.mark.parametrize("fmt", [dea.FORMAT_SSS, dea.FORMAT_BAND])
 @pytest.mark.parametrize("method", [dea.METHOD_PCG_F64, dea.METHOD_MIXED])
-def test_batch_matches_truth_and_single_solves(ctx, systems, method):
+def test_batch_matches_truth_and_single_solves(ctx, systems, method, fmt):
+    ctx.set_format(fmt)
     B = ctx.batch([(s.row_ptr, s.col_ind, s.values, s.diag) for s in systems])
+    ctx.set_format(dea.FORMAT_AUTO)
+    assert B.size("format") == fmt
     for i, s in enumerate(systems):
         B.set_rhs(i, s.b, None, s.b)
     stats, ms = B.solve(method, 1e-10, 200_000, dea.REL_B)

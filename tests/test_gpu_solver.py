@@ -59,23 +59,29 @@ def random_laplacian(n, seed, extra=2.0, spread=7.0):
     return row_ptr, col, val, diag, bvec
 
 
+@pytest.mark.parametrize("fmt", [dea.FORMAT_SSS, dea.FORMAT_BAND])
 @pytest.mark.parametrize("rb", [128, 256, 512, 1024])
-def test_spmv_matches_oracle_c1(ctx, c1, rb):
+def test_spmv_matches_oracle_c1(ctx, c1, rb, fmt):
     ctx.set_rows_per_block(rb)
+    ctx.set_format(fmt)
     v = np.random.default_rng(rb).standard_normal(c1.n)
     out = ctx.spmv(c1.row_ptr, c1.col_ind, c1.values, c1.diag, v)
     ref = oracle.spmv_sss(c1.row_ptr, c1.col_ind, c1.values, c1.diag, v)
     ctx.set_rows_per_block(256)
+    ctx.set_format(dea.FORMAT_AUTO)
     # different summation order (LDS atomics): a few ulp of the row's magnitude
     scale = np.abs(c1.diag * v) + np.abs(ref)
     assert np.all(np.abs(out - ref) <= 1e-14 * scale + 1e-300)
 
 
+@pytest.mark.parametrize("fmt", [dea.FORMAT_SSS, dea.FORMAT_BAND])
 @pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 1000, 40_000])
-def test_spmv_ragged_sizes(ctx, n):
+def test_spmv_ragged_sizes(ctx, n, fmt):
     row_ptr, col, val, diag, _ = random_laplacian(n, seed=n)
     v = np.random.default_rng(1).standard_normal(n)
+    ctx.set_format(fmt)
     out = ctx.spmv(row_ptr, col, val, diag, v)
+    ctx.set_format(dea.FORMAT_AUTO)
     ref = oracle.spmv_sss(row_ptr, col, val, diag, v)
     assert np.allclose(out, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
 
@@ -157,9 +163,13 @@ def test_empty_system(ctx):
     assert ok and list(volt) == [0.0, 1.0] and st.status == dea.CICY_OK
 
 
+@pytest.mark.parametrize("fmt", [dea.FORMAT_SSS, dea.FORMAT_BAND])
 @pytest.mark.parametrize("method", [dea.METHOD_PCG_F64, dea.METHOD_MIXED])
-def test_resident_solve_true_residual(ctx, c1, method):
+def test_resident_solve_true_residual(ctx, c1, method, fmt):
+    ctx.set_format(fmt)
     sysd = ctx.system(c1.row_ptr, c1.col_ind, c1.values, c1.diag)
+    ctx.set_format(dea.FORMAT_AUTO)
+    assert sysd.size("format") == fmt
     sysd.set_rhs(c1.b, None, c1.b)
     st = sysd.solve(method, 1e-10, 0, dea.REL_B)
     x, _ = sysd.get()
@@ -174,10 +184,13 @@ def test_resident_solve_true_residual(ctx, c1, method):
     sysd.close()
 
 
+@pytest.mark.parametrize("fmt", [dea.FORMAT_SSS, dea.FORMAT_BAND])
 @pytest.mark.parametrize("method", [dea.METHOD_PCG_F64, dea.METHOD_MIXED])
-def test_random_laplacian_solve(ctx, method):
+def test_random_laplacian_solve(ctx, method, fmt):
     row_ptr, col, val, diag, b = random_laplacian(20_000, seed=11, spread=5.0)
+    ctx.set_format(fmt)
     sysd = ctx.system(row_ptr, col, val, diag)
+    ctx.set_format(dea.FORMAT_AUTO)
     sysd.set_rhs(b, None, b)
     st = sysd.solve(method, 1e-10, 0, dea.REL_B)
     x, _ = sysd.get()
