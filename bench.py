@@ -6,9 +6,11 @@ percolating along x, R_flag=1 resistance solve (family := x), zero initial guess
 system is assembled on the host and uploaded once; the timed region covers only solves
 on HBM-resident data.  A "step" is one solve.
 
-Methods: plain FP64 Jacobi-PCG on the GPU (the reference's algorithm) and the
-mixed-precision FP32/FP64-reliable-update solver; `value` is the headline method's
-solves/s.  Prints one JSON line on stdout; progress goes to stderr.
+Methods: plain FP64 Jacobi-PCG on the GPU (the reference's algorithm), the
+mixed-precision FP32/FP64-reliable-update solver, and the learned warm start followed by
+the mixed solver (network inference is inside the timed solve).  Each method runs on the
+device layout and row-block size the autotuner picks for it on this system.  `value` is
+the headline method's solves/s.  Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -43,14 +45,27 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def time_solves(sysd, method, steps, warmup, tol):
+WEIGHTS = os.path.join(ROOT, "cicy_geom_amd", "weights", "warmstart.npz")
+
+
+def time_solves(sysd, method, steps, warmup, tol, s=None, model=None, graph=None):
+    """Wall time of `steps` solves on resident data (after `warmup` untimed ones).  With a
+    model, every solve starts by writing the network's prediction into x0/r0."""
     from cicy_geom_amd import dea
     stats = []
+
+    def one():
+        # the zero-guess right-hand side was uploaded once; the learned method writes its
+        # prediction into x0 and r0 = b - A x0 on the device as part of the solve
+        if model is not None:
+            model.warm_start(graph, sysd)
+        return sysd.solve(method, tol, MAX_ITER, dea.REL_B)
+
     for _ in range(warmup):
-        sysd.solve(method, tol, MAX_ITER, dea.REL_B)
+        one()
     t0 = time.perf_counter()
     for _ in range(steps):
-        stats.append(sysd.solve(method, tol, MAX_ITER, dea.REL_B))
+        stats.append(one())
     wall = time.perf_counter() - t0
     return wall, stats
 
@@ -86,7 +101,9 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--rows-per-block", type=int, default=0)
+    ap.add_argument("--rows-per-block", type=int, default=0,
+                    help="skip the autotuner and use this row-block size")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); multi-GPU "
@@ -97,42 +114,75 @@ def main():
 
     t0 = time.perf_counter()
     net = network.generate(args.config)
-    s = network.largest_percolating_system(net, 0)
+    box = net.params.box
+    s = network.largest_percolating_system(net, 0, with_xyz=True)
     del net
     log(f"[bench] config {args.config}: N={s.n} L={s.nnz} built in "
         f"{time.perf_counter() - t0:.1f}s")
 
-    ctx = dea.Context(0, rows_per_block=args.rows_per_block)
-    t0 = time.perf_counter()
-    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
-    sysd.set_rhs(s.b, None, s.b)
-    upload_s = time.perf_counter() - t0
-    log(f"[bench] uploaded ({sysd.size('bytes') / 1e9:.2f} GB device, remote entries "
-        f"{sysd.size('remote')}) in {upload_s:.2f}s")
-
-    results = {}
+    from cicy_geom_amd import autotune
+    ctx = dea.Context(0)
+    tuned = {}
     for name, method in (("pcg_f64", dea.METHOD_PCG_F64), ("mixed", dea.METHOD_MIXED)):
-        wall, stats = time_solves(sysd, method, args.steps, args.warmup, args.tol)
+        if args.rows_per_block:
+            tuned[name] = {"format": dea.FORMAT_AUTO, "rows_per_block": args.rows_per_block}
+        else:
+            t0 = time.perf_counter()
+            tuned[name] = autotune.tune(ctx, s, method, iters=96, cache_path=None,
+                                        rows=(256, 512, 1024), log=log)
+            log(f"[bench] autotune {name}: {autotune.LAYOUTS.get(tuned[name]['format'])} "
+                f"rb={tuned[name]['rows_per_block']} ({time.perf_counter() - t0:.1f}s)")
+
+    results, systems = {}, {}
+    model = graph = None
+    if os.path.exists(WEIGHTS) and not args.no_learned:
+        from cicy_geom_amd import warmstart
+        theta = np.load(WEIGHTS, allow_pickle=False)["theta"]
+        model = warmstart.WarmStartNet(ctx, theta)
+    plan = [("pcg_f64", dea.METHOD_PCG_F64, "pcg_f64", False),
+            ("mixed", dea.METHOD_MIXED, "mixed", False)]
+    if model is not None:
+        plan.append(("learned_mixed", dea.METHOD_MIXED, "mixed", True))
+    upload_s = 0.0
+    for name, method, tkey, learned in plan:
+        autotune.apply(ctx, tuned[tkey])
+        t0 = time.perf_counter()
+        sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+        sysd.set_rhs(s.b, None, s.b)
+        upload_s = time.perf_counter() - t0
+        if learned:
+            graph = warmstart.Graph(ctx, s, box, 0, 1.0)
+        wall, stats = time_solves(sysd, method, args.steps, args.warmup, args.tol, s,
+                                  model if learned else None, graph if learned else None)
         st = stats[-1]
         results[name] = dict(solves_per_s=args.steps / wall, ms_per_solve=1e3 * wall / args.steps,
                              iters=st.iters, refinements=st.refinements,
                              true_rel_res=st.true_rel_res, status=dea.STATUS_NAMES[st.status],
-                             device_ms=st.solve_ms)
+                             device_ms=st.solve_ms,
+                             layout=autotune.LAYOUTS.get(sysd.size("format")),
+                             rows_per_block=tuned[tkey]["rows_per_block"])
         log(f"[bench] {name}: {results[name]}")
         if st.status != dea.CICY_OK:
             log(f"[bench] WARNING {name} did not converge")
+        systems[name] = sysd
+        if learned:
+            graph.close()
 
     # SpMV roofline (HIP events on the library's stream, back-to-back launches)
     n, L = s.n, s.nnz
     spmv = {}
-    for prec, sv in ((64, 8), (32, 4)):
-        ms = sysd.spmv_bench(prec, 50)
+    for prec, sv, key in ((64, 8, "pcg_f64"), (32, 4, "mixed")):
+        ms = systems[key].spmv_bench(prec, 50)
         algo = network.spmv_bytes(n, L, sv=sv, si=4)
         spmv[prec] = dict(ms=ms, bytes=algo, gbs=algo / (ms * 1e-3) / 1e9)
         log(f"[bench] SpMV fp{prec}: {ms * 1e3:.1f} us, {algo / 1e6:.1f} MB algorithmic, "
             f"{spmv[prec]['gbs']:.0f} GB/s")
 
-    head = "mixed" if results["mixed"]["status"] == "ok" else "pcg_f64"
+    head = "pcg_f64"
+    for cand in ("mixed", "learned_mixed"):
+        if cand in results and results[cand]["status"] == "ok" and \
+                results[cand]["solves_per_s"] > results[head]["solves_per_s"]:
+            head = cand
     value = results[head]["solves_per_s"]
     out = {
         "metric": "CG solves/s to 1e-10 residual at 4M nodes; learned+mixed vs plain FP64 GPU CG",
@@ -145,19 +195,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "dtype": "f64" if head == "pcg_f64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"config {args.config}: {s.n}-node DEA system (8 um RVE, 160k "
                                "straight CNTs, largest x-percolating cluster, R_flag=1)",
                    "N": s.n, "L": s.nnz, "tol": args.tol, "residual": "||b-Ax||/||b||",
-                   "method": head, "rows_per_block": args.rows_per_block or 256},
+                   "method": head},
         "methods": results,
         "speedup_vs_plain_f64": value / results["pcg_f64"]["solves_per_s"],
-        "roofline": {"bound": "hbm", "achieved": spmv[32 if head == "mixed" else 64]["gbs"],
+        "roofline": {"bound": "hbm", "achieved": spmv[64 if head == "pcg_f64" else 32]["gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv[32 if head == "mixed" else 64]["gbs"] / HBM_PEAK_GBS,
+                     "frac": spmv[64 if head == "pcg_f64" else 32]["gbs"] / HBM_PEAK_GBS,
                      "traffic": None,
-                     "kernel": "k_apply SSS SpMV " + ("fp32" if head == "mixed" else "fp64")},
+                     "kernel": "SSS SpMV (k_apply/k_apply_band) " +
+                               ("fp64" if head == "pcg_f64" else "fp32 Jacobi-scaled"),
+                     "algorithmic_bytes": spmv[64 if head == "pcg_f64" else 32]["bytes"],
+                     "us_per_launch": spmv[64 if head == "pcg_f64" else 32]["ms"] * 1e3},
         "roofline_spmv_f64": {"achieved": spmv[64]["gbs"], "frac": spmv[64]["gbs"] / HBM_PEAK_GBS,
                               "us": spmv[64]["ms"] * 1e3},
         "upload_s": upload_s,
@@ -165,7 +218,13 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(s, results["pcg_f64"]["iters"])
         log(f"[bench] cpu baseline: {out['cpu_baseline']}")
-    sysd.close()
+    for sysd in systems.values():
+        sysd.close()
+    if model is not None:
+        out["learned"] = {"weights": os.path.relpath(WEIGHTS, ROOT),
+                          "iters_saved_vs_mixed_zero_guess":
+                              1.0 - results["learned_mixed"]["iters"] / results["mixed"]["iters"]}
+        model.close()
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -147,17 +147,44 @@ void chunk_mixed(cicy_batch *B, const int *act, int grid) {
 }
 
 template <int RB>
-void init_all(cicy_batch *B, int method, double tol, long long max_iter) {
+void init_blocks(cicy_batch *B, int method, double tol, long long max_iter, const int *act,
+                 int grid) {
     hipStream_t st = B->ctx->stream;
-    const BatchDev bd = bdev(B, nullptr);
+    const BatchDev bd = bdev(B, act);
     if (method == CICY_METHOD_PCG_F64)
-        hipLaunchKernelGGL((k_init_f64<RB, true>), dim3(B->nblocks), dim3(kThreads), 0, st, B->n,
-                           bd, B->diag, B->r, B->p0, B->p1, B->st, B->partials, tol,
-                           B->stop_ref2, -1.0, max_iter);
+        hipLaunchKernelGGL((k_init_f64<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
+                           B->diag, B->r, B->p0, B->p1, B->st, B->partials, tol, B->stop_ref2,
+                           -1.0, max_iter);
     else
-        hipLaunchKernelGGL((k_mixed_init<RB, true>), dim3(B->nblocks), dim3(kThreads), 0, st,
-                           B->n, bd, B->diag, B->r0, B->rs, B->xs, B->ps0, B->ps1, B->st, B->mix,
+        hipLaunchKernelGGL((k_mixed_init<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
+                           B->diag, B->r0, B->rs, B->xs, B->ps0, B->ps1, B->st, B->mix,
                            B->partials, tol, B->stop_ref2, -1.0, max_iter, kDelta * kDelta);
+}
+
+// r = b - A x over the blocks of `act` (q = A x formed by the caller)
+template <int RB>
+__global__ __launch_bounds__(kThreads) void k_batch_b_minus(int n, BatchDev bd,
+                                                            const double *__restrict__ b,
+                                                            const double *__restrict__ ax,
+                                                            double *__restrict__ r) {
+    const Blk k = block_info<RB, true>(bd, n);
+    for (int t = threadIdx.x; t < k.nrows; t += kThreads) r[k.r0 + t] = b[k.r0 + t] - ax[k.r0 + t];
+}
+
+template <int RB>
+void fold_all(cicy_batch *B) {
+    hipLaunchKernelGGL((k_fold<RB, true>), dim3(B->nblocks), dim3(kThreads), 0, B->ctx->stream,
+                       B->n, bdev(B, nullptr), B->diag, B->x, B->xs, B->mix, 1);
+}
+
+template <int RB>
+void resid_blocks(cicy_batch *B, const int *act, int grid) {
+    hipStream_t st = B->ctx->stream;
+    const BatchDev bd = bdev(B, act);
+    launch_apply<double, RB, false, false, true>(m64(B), grid, st, bd, B->x, nullptr, nullptr,
+                                                 B->q, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_batch_b_minus<RB>), dim3(grid), dim3(kThreads), 0, st, B->n, bd, B->b,
+                       B->q, B->r);
 }
 
 template <int RB>
@@ -472,6 +499,68 @@ int cicy_batch_set_rhs(cicy_batch *B, int sys, const double *r0, const double *x
     return ok ? 1 : 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Runs chunks of `method` over the blocks in pinned list `buf` (length grid) until every
+// system in it is done; compacts the list as systems finish.
+bool batch_loop(cicy_batch *B, int method, int buf, int grid) {
+    cicy_ctx *ctx = B->ctx;
+    hipStream_t st = ctx->stream;
+    if (grid <= 0) return true;
+    double max_s = 600.0;
+    if (const char *e = getenv("CICY_MAX_SOLVE_SECONDS")) max_s = atof(e);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!bh2d(B, B->active[buf], B->h_active[buf], (size_t)grid) || !capture(B, method, buf, grid))
+        return false;
+    // keep one chunk queued ahead of the poll
+    int slot = 0;
+    if (!hip_ok(ctx, hipGraphLaunch(B->exec, st), "hipGraphLaunch") ||
+        !hip_ok(ctx, hipMemcpyAsync(B->h_st[slot], B->st, B->nsys * sizeof(PcgState),
+                                    hipMemcpyDeviceToHost, st), "poll") ||
+        !hip_ok(ctx, hipEventRecord(ctx->ev_poll[slot], st), "event"))
+        return false;
+    for (;;) {
+        const int nslot = slot ^ 1;
+        if (!hip_ok(ctx, hipGraphLaunch(B->exec, st), "hipGraphLaunch") ||
+            !hip_ok(ctx, hipMemcpyAsync(B->h_st[nslot], B->st, B->nsys * sizeof(PcgState),
+                                        hipMemcpyDeviceToHost, st), "poll") ||
+            !hip_ok(ctx, hipEventRecord(ctx->ev_poll[nslot], st), "event"))
+            return false;
+        if (!hip_ok(ctx, hipEventSynchronize(ctx->ev_poll[slot]), "event sync")) return false;
+        const PcgState *hs = B->h_st[slot];
+        int live_blocks = 0;
+        for (int k = 0; k < grid; k++)
+            if (!hs[B->h_blk_sys[B->h_active[buf][k]]].done) live_blocks++;
+        if (live_blocks == 0) break;
+        if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > max_s) {
+            set_error(ctx, "batch solve stopped after %.0f s (CICY_MAX_SOLVE_SECONDS)", max_s);
+            break;
+        }
+        if (live_blocks < grid * 7 / 10) {
+            // compact into the other list; the queued chunk still reads the current one
+            const int nb = buf ^ 1;
+            int m = 0;
+            for (int k = 0; k < grid; k++) {
+                const int b = B->h_active[buf][k];
+                if (!hs[B->h_blk_sys[b]].done) B->h_active[nb][m++] = b;
+            }
+            grid = m;
+            if (!bh2d(B, B->active[nb], B->h_active[nb], (size_t)grid) ||
+                !capture(B, method, nb, grid))
+                return false;
+            buf = nb;
+        }
+        slot = nslot;
+    }
+    return hip_ok(ctx, hipStreamSynchronize(st), "sync");
+}
+
+}  // namespace
+
+extern "C" {
+
 int cicy_batch_solve(cicy_batch *B, int method, double tol, int64_t max_iter, int rel_to,
                      cicy_solve_stats *per_sys, double *total_ms) {
     if (!B) return 0;
@@ -495,90 +584,81 @@ int cicy_batch_solve(cicy_batch *B, int method, double tol, int64_t max_iter, in
                 "copy x0") ||
         !hip_ok(ctx, hipMemsetAsync(B->st, 0, B->nsys * sizeof(PcgState), st), "memset"))
         return 0;
-    CICY_RB_SWITCH(B->rb, init_all<RB>(B, method, tol, max_iter));
+    CICY_RB_SWITCH(B->rb, init_blocks<RB>(B, method, tol, max_iter, nullptr, B->nblocks));
     if (!hip_ok(ctx, hipStreamSynchronize(st), "sync")) return 0;  // ref2 is pageable
-    // initial active list: every block of every system not already converged
     if (!hip_ok(ctx, hipMemcpy(B->h_st[0], B->st, B->nsys * sizeof(PcgState),
                                hipMemcpyDeviceToHost), "D2H"))
         return 0;
     int buf = 0;
     int grid = build_active(B, B->h_st[0], buf);
-    double max_s = 600.0;
-    if (const char *e = getenv("CICY_MAX_SOLVE_SECONDS")) max_s = atof(e);
-    const auto t0 = std::chrono::steady_clock::now();
-    if (grid > 0) {
-        if (!bh2d(B, B->active[buf], B->h_active[buf], (size_t)grid) ||
-            !capture(B, method, buf, grid))
-            return 0;
-        // keep one chunk queued ahead of the poll
-        int slot = 0;
-        if (!hip_ok(ctx, hipGraphLaunch(B->exec, st), "hipGraphLaunch") ||
-            !hip_ok(ctx, hipMemcpyAsync(B->h_st[slot], B->st, B->nsys * sizeof(PcgState),
-                                        hipMemcpyDeviceToHost, st), "poll") ||
-            !hip_ok(ctx, hipEventRecord(ctx->ev_poll[slot], st), "event"))
-            return 0;
-        for (;;) {
-            // queue the next chunk before reading the previous poll
-            const int nslot = slot ^ 1;
-            if (!hip_ok(ctx, hipGraphLaunch(B->exec, st), "hipGraphLaunch") ||
-                !hip_ok(ctx, hipMemcpyAsync(B->h_st[nslot], B->st, B->nsys * sizeof(PcgState),
-                                            hipMemcpyDeviceToHost, st), "poll") ||
-                !hip_ok(ctx, hipEventRecord(ctx->ev_poll[nslot], st), "event"))
-                return 0;
-            if (!hip_ok(ctx, hipEventSynchronize(ctx->ev_poll[slot]), "event sync")) return 0;
-            const PcgState *hs = B->h_st[slot];
-            int live_blocks = 0;
-            for (int s = 0; s < B->nsys; s++)
-                if (!hs[s].done) live_blocks += B->h_sys_blk0[s + 1] - B->h_sys_blk0[s];
-            if (live_blocks == 0) break;
-            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >
-                max_s) {
-                set_error(ctx, "batch solve stopped after %.0f s (CICY_MAX_SOLVE_SECONDS)", max_s);
-                break;
-            }
-            if (live_blocks < grid * 7 / 10) {
-                // compact into the other list; the queued chunk still reads the current one
-                const int nb = buf ^ 1;
-                grid = build_active(B, hs, nb);
-                if (!bh2d(B, B->active[nb], B->h_active[nb], (size_t)grid) ||
-                    !capture(B, method, nb, grid))
-                    return 0;
-                buf = nb;
-            }
-            slot = nslot;
+    if (!batch_loop(B, method, buf, grid)) return 0;
+    std::vector<PcgState> hs((size_t)B->nsys);
+    std::vector<MixState> hm((size_t)B->nsys);
+    std::vector<long long> fp64_iters((size_t)B->nsys, 0), iters((size_t)B->nsys, 0);
+    std::vector<double> norms(2 * (size_t)B->nsys);
+    auto read_state = [&]() {
+        return hip_ok(ctx, hipMemcpy(hs.data(), B->st, hs.size() * sizeof(PcgState),
+                                     hipMemcpyDeviceToHost), "D2H") &&
+               hip_ok(ctx, hipMemcpy(hm.data(), B->mix, hm.size() * sizeof(MixState),
+                                     hipMemcpyDeviceToHost), "D2H");
+    };
+    auto read_norms = [&]() {
+        CICY_RB_SWITCH(B->rb, final_norms<RB>(B));
+        return hip_ok(ctx, hipMemcpy(norms.data(), B->norms, norms.size() * sizeof(double),
+                                     hipMemcpyDeviceToHost), "D2H");
+    };
+    auto true_rel = [&](int s) {
+        return norms[2 * s + 1] > 0 ? std::sqrt(norms[2 * s] / norms[2 * s + 1])
+                                    : std::sqrt(norms[2 * s]);
+    };
+    // final_norms folds the FP32 corrections (mixed) and forms every ||b - A x||
+    if (!read_state() || !read_norms()) return 0;
+    for (int s = 0; s < B->nsys; s++) iters[s] = hs[s].iters;
+    // Continue in FP64, from x with r = b - A x, every system whose FP32 recursion stalled
+    // (status CICY_MAXITER below the cap) or whose true residual misses tol*||b||.
+    for (int pass = 0; pass < 3; pass++) {
+        int m = 0;
+        std::vector<char> redo((size_t)B->nsys, 0);
+        for (int s = 0; s < B->nsys; s++) {
+            const bool stalled = method == CICY_METHOD_MIXED && hs[s].status == CICY_MAXITER &&
+                                 iters[s] < max_iter;
+            const bool short_of_b = rel_to == CICY_REL_B && hs[s].status == CICY_OK &&
+                                    true_rel(s) > tol;
+            if (!stalled && !short_of_b) continue;
+            redo[s] = 1;
+            for (int b = B->h_sys_blk0[s]; b < B->h_sys_blk0[s + 1]; b++) B->h_active[0][m++] = b;
         }
+        if (m == 0) break;
+        if (!bh2d(B, B->active[0], B->h_active[0], (size_t)m)) return 0;
+        CICY_RB_SWITCH(B->rb, resid_blocks<RB>(B, B->active[0], m));
+        CICY_RB_SWITCH(B->rb, init_blocks<RB>(B, CICY_METHOD_PCG_F64, tol, max_iter,
+                                              B->active[0], m));
+        if (!batch_loop(B, CICY_METHOD_PCG_F64, 0, m)) return 0;
+        if (!read_state() || !read_norms()) return 0;
+        for (int s = 0; s < B->nsys; s++)
+            if (redo[s]) {
+                iters[s] += hs[s].iters;
+                fp64_iters[s] += hs[s].iters;
+            }
     }
-    if (!hip_ok(ctx, hipStreamSynchronize(st), "sync")) return 0;
-    CICY_RB_SWITCH(B->rb, final_norms<RB>(B));
     if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "event") ||
         !hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "sync"))
         return 0;
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
     if (total_ms) *total_ms = ms;
-    std::vector<PcgState> hs((size_t)B->nsys);
-    std::vector<MixState> hm((size_t)B->nsys);
-    std::vector<double> norms(2 * (size_t)B->nsys);
-    if (!hip_ok(ctx, hipMemcpy(hs.data(), B->st, hs.size() * sizeof(PcgState),
-                               hipMemcpyDeviceToHost), "D2H") ||
-        !hip_ok(ctx, hipMemcpy(hm.data(), B->mix, hm.size() * sizeof(MixState),
-                               hipMemcpyDeviceToHost), "D2H") ||
-        !hip_ok(ctx, hipMemcpy(norms.data(), B->norms, norms.size() * sizeof(double),
-                               hipMemcpyDeviceToHost), "D2H"))
-        return 0;
     if (per_sys) {
         for (int s = 0; s < B->nsys; s++) {
             cicy_solve_stats &o = per_sys[s];
             memset(&o, 0, sizeof(o));
-            o.iters = hs[s].iters;
+            const bool mixed = method == CICY_METHOD_MIXED;
+            o.iters = iters[s];
             o.status = hs[s].status;
-            o.refinements = method == CICY_METHOD_MIXED ? hm[s].refinements : 0;
-            o.rel_res = hs[s].rr_init > 0 ? std::sqrt((method == CICY_METHOD_MIXED ? hm[s].rtrue2
-                                                                               : hs[s].rr) /
-                                                      hs[s].rr_init)
-                                          : 0.0;
-            o.true_rel_res = norms[2 * s + 1] > 0 ? std::sqrt(norms[2 * s] / norms[2 * s + 1])
-                                                  : std::sqrt(norms[2 * s]);
+            o.refinements = mixed ? hm[s].refinements : 0;
+            o.fp64_iters = fp64_iters[s];
+            o.true_rel_res = true_rel(s);
+            o.rel_res = (!mixed && fp64_iters[s] == 0 && hs[s].rr_init > 0)
+                            ? std::sqrt(hs[s].rr / hs[s].rr_init) : o.true_rel_res;
             o.solve_ms = ms;
         }
     }

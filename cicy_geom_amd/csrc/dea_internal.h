@@ -12,9 +12,10 @@ namespace cicy {
 // iterations per captured hipGraph chunk (even: the p double buffer returns to parity 0)
 constexpr int kChunk = 32;
 // mixed precision: reliable-update trigger r^.r^ <= kDelta^2 * (r^.r^ at the last update),
-// and the number of updates in a row without halving ||b - A x|| that ends a solve
+// and the number of updates in a row without halving ||b - A x|| after which the FP32
+// recursion is abandoned (the single-system solver then finishes in FP64)
 constexpr double kDelta = 0.1;
-constexpr int kStallLimit = 8;
+constexpr int kStallLimit = 3;
 
 #define CICY_RB_SWITCH(rb, CALL)                              \
     switch (rb) {                                             \
@@ -114,9 +115,9 @@ bool build_blocked(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr
                    const int64_t *col_ind, const double *values, const double *diag, int rb,
                    BlockedHost &h);
 bool run_chunks(cicy_sys *s, hipGraphExec_t exec);
-bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel);
+bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel, double *b2 = nullptr);
 bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
-                 cicy_solve_stats *stats);
+                 cicy_solve_stats *stats, bool keep_x = false);
 
 // dea_mixed.hip
 bool mixed_setup(cicy_sys *s, const BlockedHost &h, const BandHost &bh);

@@ -61,6 +61,8 @@ struct MixState {
     int refinements;
     unsigned int counter;
     int stall;
+    int restarts;
+    int pad;
 };
 
 template <typename T>
@@ -549,6 +551,7 @@ __global__ __launch_bounds__(kThreads) void k_mixed_init(int n, BatchDev bd,
             mx->refinements = 0;
             mx->counter = 0;
             mx->stall = 0;
+            mx->restarts = 0;
             if (r02 <= st->stop2) {
                 st->status = 0;
                 st->done = 1;
@@ -679,7 +682,14 @@ __global__ __launch_bounds__(kThreads) void k_reliable_resid(int n, BatchDev bd,
                 st->status = 2;
                 st->done = 1;
             } else if (!st->done) {
-                st->beta = mx->rr_prev > 0.0 ? rs2 / mx->rr_prev : 0.0;
+                // when the FP32 recursion has drifted from the true residual (r.r off by
+                // more than 2x) its search direction is stale: restart from r^ (beta = 0)
+                if (rs2 > 2.0 * st->rz) {
+                    st->beta = 0.0;
+                    mx->restarts += 1;
+                } else {
+                    st->beta = mx->rr_prev > 0.0 ? rs2 / mx->rr_prev : 0.0;
+                }
                 st->rz = rs2;
                 mx->m2 = rs2;
                 mx->tgt2 = rs2 * (st->stop2 / rt2);

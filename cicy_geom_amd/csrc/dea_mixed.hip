@@ -200,11 +200,6 @@ bool solve_mixed(cicy_sys *s, double tol, long long max_iter, int rel_to,
     CICY_RB_SWITCH(s->rb, launch_init<RB>(s, tol, stop_ref2, max_iter));
     if (!run_chunks(s, s->graph_exec_mixed)) return false;
     CICY_RB_SWITCH(s->rb, launch_fold_always<RB>(s));
-    if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "hipEventRecord") ||
-        !hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "hipEventSynchronize"))
-        return false;
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
     PcgState h;
     MixState m;
     if (!hip_ok(ctx, hipMemcpy(&h, s->st, sizeof(h), hipMemcpyDeviceToHost), "D2H") ||
@@ -214,20 +209,43 @@ bool solve_mixed(cicy_sys *s, double tol, long long max_iter, int rel_to,
     CICY_RB_SWITCH(s->rb, launch_spmv64<RB>(s, s->x, s->q));
     hipLaunchKernelGGL(k_sub_from, dim3((s->n + kThreads - 1) / kThreads), dim3(kThreads), 0, st,
                        s->n, s->r, s->b, s->q);
+    long long iters = h.iters;
+    int status = h.status;
+    int fallback_iters = 0;
+    const bool had_b = s->has_b;
+    s->has_b = true;  // b is valid here (given or formed above)
+    if (status != CICY_OK && status != CICY_BREAKDOWN && iters < max_iter) {
+        // the FP32 recursion stopped making progress: finish in FP64 from x, r = b - A x
+        cicy_solve_stats fs;
+        const bool ok = solve_pcg64(s, tol, max_iter - iters, rel_to, &fs, true);
+        if (!ok) {
+            s->has_b = had_b;
+            return false;
+        }
+        iters += fs.iters;
+        fallback_iters = (int)fs.iters;
+        status = fs.status;
+    }
+    if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "hipEventRecord") ||
+        !hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "hipEventSynchronize")) {
+        s->has_b = had_b;
+        return false;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    double tr = -1.0, b2 = 0.0;
+    const bool ok = sys_true_residual(s, s->x, &tr, &b2);
+    s->has_b = had_b;
+    if (!ok) return false;
     if (stats) {
         memset(stats, 0, sizeof(*stats));
-        stats->iters = h.iters;
-        stats->status = h.status;
+        stats->iters = iters;
+        stats->status = status;
         stats->refinements = m.refinements;
-        stats->rel_res = h.rr_init > 0 ? std::sqrt(m.rtrue2 / h.rr_init) : 0.0;
+        stats->rel_res = h.rr_init > 0 ? tr * std::sqrt(b2 / h.rr_init) : 0.0;
         stats->solve_ms = ms;
-        double tr = -1.0;
-        bool had_b = s->has_b;
-        s->has_b = true;  // b is valid here (given or formed above)
-        bool ok = sys_true_residual(s, s->x, &tr);
-        s->has_b = had_b;
-        if (!ok) return false;
         stats->true_rel_res = tr;
+        stats->fp64_iters = fallback_iters;
     }
     return hip_ok(ctx, hipStreamSynchronize(st), "sync");
 }

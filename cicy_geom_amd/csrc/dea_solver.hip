@@ -52,6 +52,14 @@ __global__ __launch_bounds__(kThreads) void k_resid_norms(int n, const double *_
     }
 }
 
+// r <- b - ax
+__global__ __launch_bounds__(kThreads) void k_b_minus(int n, double *__restrict__ r,
+                                                     const double *__restrict__ b,
+                                                     const double *__restrict__ ax) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) r[i] = b[i] - ax[i];
+}
+
 // r <- r - ax (Update_residual_vector's final loop, Direct_Electrifying.cpp:1530-1532)
 __global__ __launch_bounds__(kThreads) void k_sub(int n, double *__restrict__ r,
                                                   const double *__restrict__ ax) {
@@ -377,7 +385,7 @@ bool run_chunks(cicy_sys *s, hipGraphExec_t exec) {
     return hip_ok(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
-bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel) {
+bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel, double *b2) {
     cicy_ctx *ctx = s->ctx;
     if (!s->has_b) {
         *rel = -1.0;
@@ -393,11 +401,16 @@ bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel) {
         !hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync"))
         return false;
     *rel = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+    if (b2) *b2 = h[1];
     return true;
 }
 
+// Plain FP64 Jacobi-PCG.  With CICY_REL_B the stop is guaranteed on the true residual:
+// after the recursive residual meets tol*||b||, b - A x is recomputed in FP64 and, if it
+// does not, the iteration continues from x with r = b - A x (at most 4 times).
+// keep_x: continue from the current device x/r instead of the stored x0/r0.
 bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
-                 cicy_solve_stats *stats) {
+                 cicy_solve_stats *stats, bool keep_x) {
     cicy_ctx *ctx = s->ctx;
     if (s->n == 0) {
         if (stats) {
@@ -413,33 +426,51 @@ bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
     if (!ensure_graph64(s)) return false;
     size_t vb = (size_t)s->n * sizeof(double);
     hipStream_t st = ctx->stream;
-    if (!hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "hipEventRecord")) return false;
-    // fresh start from the stored r0, x0; both p buffers zero so that the first
-    // direction is p = M^-1 r (beta = 0)
-    double stop_ref2 = rel_to == CICY_REL_B ? s->bnorm2 : -1.0;
-    if (!hip_ok(ctx, hipMemcpyAsync(s->r, s->r0, vb, hipMemcpyDeviceToDevice, st), "copy r0") ||
-        !hip_ok(ctx, hipMemcpyAsync(s->x, s->x0, vb, hipMemcpyDeviceToDevice, st), "copy x0") ||
-        !hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), st), "memset state"))
+    if (!keep_x && !hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "hipEventRecord")) return false;
+    const double stop_ref2 = rel_to == CICY_REL_B ? s->bnorm2 : -1.0;
+    if (!keep_x &&
+        (!hip_ok(ctx, hipMemcpyAsync(s->r, s->r0, vb, hipMemcpyDeviceToDevice, st), "copy r0") ||
+         !hip_ok(ctx, hipMemcpyAsync(s->x, s->x0, vb, hipMemcpyDeviceToDevice, st), "copy x0")))
         return false;
-    dispatch_rb<LaunchInit64>(s->rb, s, tol, stop_ref2, max_iter);
-    if (!run_chunks(s, s->graph_exec64)) return false;
-    if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "hipEventRecord") ||
-        !hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "hipEventSynchronize"))
-        return false;
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    long long total_iters = 0;
+    int restarts = 0;
+    double rr_init = -1.0;
     PcgState h;
-    if (!hip_ok(ctx, hipMemcpy(&h, s->st, sizeof(PcgState), hipMemcpyDeviceToHost), "D2H state"))
-        return false;
-    s->last_parity_x_valid = true;
+    double tr = -1.0;
+    for (;;) {
+        if (!hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), st), "memset state"))
+            return false;
+        dispatch_rb<LaunchInit64>(s->rb, s, tol, stop_ref2, max_iter - total_iters);
+        if (!run_chunks(s, s->graph_exec64)) return false;
+        if (!hip_ok(ctx, hipMemcpy(&h, s->st, sizeof(PcgState), hipMemcpyDeviceToHost),
+                    "D2H state"))
+            return false;
+        if (rr_init < 0.0) rr_init = h.rr_init;
+        total_iters += h.iters;
+        tr = -1.0;
+        if (s->has_b && !sys_true_residual(s, s->x, &tr)) return false;
+        if (rel_to != CICY_REL_B || h.status != CICY_OK || tr <= tol || restarts >= 4 ||
+            total_iters >= max_iter)
+            break;
+        // q holds A x from the true-residual check: continue from r = b - A x
+        hipLaunchKernelGGL(k_b_minus, dim3((s->n + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                           st, s->n, s->r, s->b, s->q);
+        restarts++;
+    }
+    if (!keep_x) {
+        if (!hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "hipEventRecord") ||
+            !hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "hipEventSynchronize"))
+            return false;
+    }
+    float ms = 0.f;
+    if (!keep_x) (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
     if (stats) {
         memset(stats, 0, sizeof(*stats));
-        stats->iters = h.iters;
+        stats->iters = total_iters;
         stats->status = h.status;
-        stats->rel_res = h.rr_init > 0 ? std::sqrt(h.rr / h.rr_init) : 0.0;
+        stats->refinements = restarts;
+        stats->rel_res = rr_init > 0 ? std::sqrt(h.rr / rr_init) : 0.0;
         stats->solve_ms = ms;
-        double tr = -1.0;
-        if (!sys_true_residual(s, s->x, &tr)) return false;
         stats->true_rel_res = tr;
     }
     return true;

@@ -6,6 +6,7 @@ is missing the constructor raises.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -42,7 +43,7 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int64), ("status", ctypes.c_int),
                 ("refinements", ctypes.c_int), ("rel_res", ctypes.c_double),
                 ("true_rel_res", ctypes.c_double), ("solve_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double)]
+                ("total_ms", ctypes.c_double), ("fp64_iters", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_} | {
@@ -146,16 +147,23 @@ class Context:
             raise DEAError(f"cicy_create failed on device {device} (no GPU?)")
         self._h = h
         self._L = L
+        self._children = weakref.WeakSet()   # systems, batches, graphs, networks
         if rows_per_block:
             self.set_rows_per_block(rows_per_block)
         if fmt:
             self.set_format(fmt)
+
+    def _adopt(self, child):
+        """Objects holding device memory of this context are closed before it."""
+        self._children.add(child)
 
     def set_format(self, fmt: int):
         self._check(self._L.cicy_set_format(self._h, fmt), "cicy_set_format")
 
     def close(self):
         if getattr(self, "_h", None):
+            for child in list(getattr(self, "_children", ())):
+                child.close()
             self._L.cicy_destroy(self._h)
             self._h = None
 
@@ -239,6 +247,7 @@ class System:
                                           _i64(col_ind), _f64(values), _f64(diagonal),
                                           ctypes.byref(h)), "cicy_sys_create")
         self._h = h
+        ctx._adopt(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -297,6 +306,7 @@ class Batch:
         ctx._check(ctx._L.cicy_batch_create(ctx._h, ns, n, nnz, *ptrs, ctypes.byref(h)),
                    "cicy_batch_create")
         self._h = h
+        ctx._adopt(self)
         self.nsys = ns
 
     def close(self):

@@ -64,6 +64,7 @@ class Graph:
                                        v_app, 1 if training else 0, ctypes.byref(h)),
                    "cicy_graph_create")
         self._h = h
+        ctx._adopt(self)
 
     def set_target(self, v_true):
         self.ctx._check(dea.lib().cicy_graph_set_target(self._h, dea._f64(v_true)),
@@ -91,6 +92,7 @@ class WarmStartNet:
         ctx._check(L.cicy_mpnn_create(ctx._h, th.ctypes.data_as(ctypes.c_void_p),
                                       ctypes.byref(h)), "cicy_mpnn_create")
         self._h = h
+        ctx._adopt(self)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -56,6 +56,8 @@ typedef struct cicy_solve_stats {
     double true_rel_res;    /* ||b - A x|| / ||b|| when b was given, else -1 */
     double solve_ms;        /* device time of the solve (HIP events), inputs resident */
     double total_ms;        /* wall time of the call including host<->device copies */
+    int64_t fp64_iters;     /* mixed method: iterations finished in FP64 after the FP32
+                               recursion stopped making progress (0: none) */
 } cicy_solve_stats;
 
 /* ---- context ---------------------------------------------------------------------- */

@@ -47,10 +47,12 @@ def test_batch_matches_truth_and_single_solves(ctx, systems, method, fmt):
         tr = np.linalg.norm(s.b - ax) / np.linalg.norm(s.b)
         truth, _, _ = oracle.cg_truth(s.row_ptr, s.col_ind, s.values, s.diag, s.b, tol=1e-13)
         err = np.linalg.norm(x - truth) / np.linalg.norm(truth)
-        print(i, s.n, st.as_dict()["status_name"], st.iters, st.refinements, f"{tr:.2e} {err:.2e}")
+        print(i, s.n, st.as_dict()["status_name"], st.iters, st.refinements, st.fp64_iters,
+              f"{tr:.2e} {err:.2e}")
         assert st.status == dea.CICY_OK
-        assert tr <= 1.05e-10 and abs(st.true_rel_res - tr) <= 1e-3 * tr + 1e-16
-        assert err < 1e-7
+        assert tr <= 1e-10 * (1 + 1e-6) and abs(st.true_rel_res - tr) <= 1e-3 * tr + 1e-16
+        # error against the FP64 truth at the same residual (FP32 recursion: larger)
+        assert err < (1e-7 if method == dea.METHOD_PCG_F64 else 1e-4)
     # each system's iteration count matches its stand-alone solve closely
     if method == dea.METHOD_PCG_F64:
         for i in (0, len(systems) - 1):
