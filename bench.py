@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--rows-per-block", type=int, default=0,
                     help="skip the autotuner and use this row-block size")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--profile-spmv", action="store_true",
+                    help="only run the SpMV kernels (both precisions) and a 1 GiB reference "
+                         "copy, for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); multi-GPU "
@@ -122,6 +125,22 @@ def main():
 
     from cicy_geom_amd import autotune
     ctx = dea.Context(0)
+    if args.profile_spmv:
+        reps = 20
+        copy_ms = ctx.copy_bench(1 << 30, reps)
+        log(f"[bench] copy 1 GiB: {copy_ms:.3f} ms -> {2 * (1 << 30) / copy_ms / 1e6:.0f} GB/s")
+        sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+        sysd.set_rhs(s.b, None, s.b)
+        for prec in (64, 32):
+            ms = sysd.spmv_bench(prec, reps)
+            sv = 8 if prec == 64 else 4
+            algo = network.spmv_bytes(s.n, s.nnz, sv=sv, si=4)
+            log(f"[bench] SpMV fp{prec}: {ms * 1e3:.1f} us/launch, algorithmic {algo / 1e6:.1f} MB "
+                f"-> {algo / ms / 1e6:.0f} GB/s ({sysd.size('format')=})")
+        print(json.dumps({"copy_ms": copy_ms, "reps": reps, "n": s.n, "nnz": s.nnz}))
+        sysd.close()
+        ctx.close()
+        return
     tuned = {}
     for name, method in (("pcg_f64", dea.METHOD_PCG_F64), ("mixed", dea.METHOD_MIXED)):
         if args.rows_per_block:

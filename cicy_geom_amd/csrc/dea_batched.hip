@@ -41,7 +41,8 @@ struct cicy_batch {
     double *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
     double *x0 = nullptr, *r0 = nullptr, *b = nullptr, *stop_ref2 = nullptr;
     float *rs = nullptr, *xs = nullptr, *ps0 = nullptr, *ps1 = nullptr, *qs = nullptr;
-    double *partials = nullptr, *norms = nullptr;
+    double *partials = nullptr, *norms = nullptr, *gpart = nullptr;
+    unsigned int *gcount = nullptr;
     PcgState *st = nullptr;
     MixState *mix = nullptr;
     unsigned int *ncount = nullptr;
@@ -87,6 +88,8 @@ __global__ __launch_bounds__(kThreads) void k_batch_norms(int n, BatchDev bd,
     }
 }
 
+RedBuf rbuf(const cicy_batch *B) { return RedBuf{B->partials, B->gpart, B->gcount}; }
+
 BatchDev bdev(const cicy_batch *B, const int *active) {
     return BatchDev{active, B->blk_sys, B->blk_row0, B->sys_row0, B->sys_blk0};
 }
@@ -119,9 +122,9 @@ void chunk_f64(cicy_batch *B, const int *act, int grid) {
         double *pold = (it & 1) ? B->p1 : B->p0;
         double *pnew = (it & 1) ? B->p0 : B->p1;
         launch_apply<double, RB, true, false, true>(m64(B), grid, st, bd, pold, B->r, pnew, B->q,
-                                                    B->st, B->partials, nullptr);
+                                                    B->st, rbuf(B), nullptr);
         hipLaunchKernelGGL((k_update_f64<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
-                           B->diag, B->x, B->r, pnew, B->q, B->st, B->partials);
+                           B->diag, B->x, B->r, pnew, B->q, B->st, rbuf(B));
     }
 }
 
@@ -133,16 +136,16 @@ void chunk_mixed(cicy_batch *B, const int *act, int grid) {
         float *pold = (it & 1) ? B->ps1 : B->ps0;
         float *pnew = (it & 1) ? B->ps0 : B->ps1;
         launch_apply<float, RB, true, true, true>(m32(B), grid, st, bd, pold, B->rs, pnew, B->qs,
-                                                  B->st, B->partials, nullptr);
+                                                  B->st, rbuf(B), nullptr);
         hipLaunchKernelGGL((k_update_mixed<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n,
-                           bd, B->xs, B->rs, pnew, B->qs, B->st, B->mix, B->partials);
+                           bd, B->xs, B->rs, pnew, B->qs, B->st, B->mix, rbuf(B));
     }
     hipLaunchKernelGGL((k_fold<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd, B->diag,
                        B->x, B->xs, B->mix, 0);
     launch_apply<double, RB, false, false, true>(m64(B), grid, st, bd, B->x, nullptr, nullptr,
-                                                 B->q, nullptr, nullptr, B->mix);
+                                                 B->q, nullptr, RedBuf{}, B->mix);
     hipLaunchKernelGGL((k_reliable_resid<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
-                       B->diag, B->b, B->q, B->r, B->rs, B->st, B->mix, B->partials,
+                       B->diag, B->b, B->q, B->r, B->rs, B->st, B->mix, rbuf(B),
                        kStallLimit);
 }
 
@@ -153,12 +156,12 @@ void init_blocks(cicy_batch *B, int method, double tol, long long max_iter, cons
     const BatchDev bd = bdev(B, act);
     if (method == CICY_METHOD_PCG_F64)
         hipLaunchKernelGGL((k_init_f64<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
-                           B->diag, B->r, B->p0, B->p1, B->st, B->partials, tol, B->stop_ref2,
+                           B->diag, B->r, B->p0, B->p1, B->st, rbuf(B), tol, B->stop_ref2,
                            -1.0, max_iter);
     else
         hipLaunchKernelGGL((k_mixed_init<RB, true>), dim3(grid), dim3(kThreads), 0, st, B->n, bd,
                            B->diag, B->r0, B->rs, B->xs, B->ps0, B->ps1, B->st, B->mix,
-                           B->partials, tol, B->stop_ref2, -1.0, max_iter, kDelta * kDelta);
+                           rbuf(B), tol, B->stop_ref2, -1.0, max_iter, kDelta * kDelta);
 }
 
 // r = b - A x over the blocks of `act` (q = A x formed by the caller)
@@ -182,7 +185,7 @@ void resid_blocks(cicy_batch *B, const int *act, int grid) {
     hipStream_t st = B->ctx->stream;
     const BatchDev bd = bdev(B, act);
     launch_apply<double, RB, false, false, true>(m64(B), grid, st, bd, B->x, nullptr, nullptr,
-                                                 B->q, nullptr, nullptr, nullptr);
+                                                 B->q, nullptr, RedBuf{}, nullptr);
     hipLaunchKernelGGL((k_batch_b_minus<RB>), dim3(grid), dim3(kThreads), 0, st, B->n, bd, B->b,
                        B->q, B->r);
 }
@@ -194,7 +197,7 @@ void final_norms(cicy_batch *B) {
     hipLaunchKernelGGL((k_fold<RB, true>), dim3(B->nblocks), dim3(kThreads), 0, st, B->n, bd,
                        B->diag, B->x, B->xs, B->mix, 1);
     launch_apply<double, RB, false, false, true>(m64(B), B->nblocks, st, bd, B->x, nullptr,
-                                                 nullptr, B->q, nullptr, nullptr, nullptr);
+                                                 nullptr, B->q, nullptr, RedBuf{}, nullptr);
     hipLaunchKernelGGL((k_batch_norms<RB>), dim3(B->nblocks), dim3(kThreads), 0, st, B->n, bd,
                        B->b, B->q, B->norms, B->ncount, B->partials);
 }
@@ -412,6 +415,7 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
               balloc(B, &B->stop_ref2, (size_t)nsys) && balloc(B, &B->rs, N) &&
               balloc(B, &B->xs, N) && balloc(B, &B->ps0, N) && balloc(B, &B->ps1, N) &&
               balloc(B, &B->qs, N) && balloc(B, &B->partials, 2 * NB) &&
+              balloc(B, &B->gpart, 2 * NB) && balloc(B, &B->gcount, NB) &&
               balloc(B, &B->norms, 2 * (size_t)nsys) && balloc(B, &B->st, (size_t)nsys) &&
               balloc(B, &B->mix, (size_t)nsys) && balloc(B, &B->ncount, (size_t)nsys);
     ok = ok && hip_ok(ctx, hipHostMalloc((void **)&B->h_active[0], NB * sizeof(int),
@@ -432,6 +436,7 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
          hip_ok(ctx, hipMemsetAsync(B->b, 0, N * sizeof(double), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(B->xs, 0, N * sizeof(float), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(B->ncount, 0, nsys * sizeof(unsigned), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(B->gcount, 0, NB * sizeof(unsigned), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(B->mix, 0, nsys * sizeof(MixState), ctx->stream), "memset") &&
          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (!ok) {
@@ -452,6 +457,7 @@ void cicy_batch_destroy(cicy_batch *B) {
                    B->blk_sys, B->blk_row0, B->sys_row0, B->sys_blk0, B->active[0],
                    B->active[1], B->x, B->r, B->p0, B->p1, B->q, B->x0, B->r0, B->b,
                    B->stop_ref2, B->rs, B->xs, B->ps0, B->ps1, B->qs, B->partials, B->norms,
+                   B->gpart, B->gcount,
                    B->st, B->mix, B->ncount};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -586,8 +592,7 @@ int cicy_batch_solve(cicy_batch *B, int method, double tol, int64_t max_iter, in
         return 0;
     CICY_RB_SWITCH(B->rb, init_blocks<RB>(B, method, tol, max_iter, nullptr, B->nblocks));
     if (!hip_ok(ctx, hipStreamSynchronize(st), "sync")) return 0;  // ref2 is pageable
-    if (!hip_ok(ctx, hipMemcpy(B->h_st[0], B->st, B->nsys * sizeof(PcgState),
-                               hipMemcpyDeviceToHost), "D2H"))
+    if (!d2h_sync(ctx, B->h_st[0], B->st, B->nsys * sizeof(PcgState)))
         return 0;
     int buf = 0;
     int grid = build_active(B, B->h_st[0], buf);
@@ -597,15 +602,12 @@ int cicy_batch_solve(cicy_batch *B, int method, double tol, int64_t max_iter, in
     std::vector<long long> fp64_iters((size_t)B->nsys, 0), iters((size_t)B->nsys, 0);
     std::vector<double> norms(2 * (size_t)B->nsys);
     auto read_state = [&]() {
-        return hip_ok(ctx, hipMemcpy(hs.data(), B->st, hs.size() * sizeof(PcgState),
-                                     hipMemcpyDeviceToHost), "D2H") &&
-               hip_ok(ctx, hipMemcpy(hm.data(), B->mix, hm.size() * sizeof(MixState),
-                                     hipMemcpyDeviceToHost), "D2H");
+        return d2h_sync(ctx, hs.data(), B->st, hs.size() * sizeof(PcgState)) &&
+               d2h_sync(ctx, hm.data(), B->mix, hm.size() * sizeof(MixState));
     };
     auto read_norms = [&]() {
         CICY_RB_SWITCH(B->rb, final_norms<RB>(B));
-        return hip_ok(ctx, hipMemcpy(norms.data(), B->norms, norms.size() * sizeof(double),
-                                     hipMemcpyDeviceToHost), "D2H");
+        return d2h_sync(ctx, norms.data(), B->norms, norms.size() * sizeof(double));
     };
     auto true_rel = [&](int s) {
         return norms[2 * s + 1] > 0 ? std::sqrt(norms[2 * s] / norms[2 * s + 1])

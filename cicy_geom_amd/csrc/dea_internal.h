@@ -54,13 +54,13 @@ struct DevMat {
 template <typename T, int RB, bool PCG, bool UNIT, bool BATCH>
 inline void launch_apply(const DevMat<T> &m, int grid, hipStream_t st, const BatchDev &bd,
                          const T *src, const T *rvec, T *pnew, T *q, PcgState *stp,
-                         double *partials, const MixState *guard) {
+                         const RedBuf &rb, const MixState *guard) {
     if (m.format == kFormatBand)
         hipLaunchKernelGGL((k_apply_band<T, RB, PCG, UNIT, BATCH>), dim3(grid), dim3(kThreads), 0,
-                           st, m.band, bd, src, rvec, pnew, q, stp, partials, guard);
+                           st, m.band, bd, src, rvec, pnew, q, stp, rb, guard);
     else
         hipLaunchKernelGGL((k_apply<T, RB, PCG, UNIT, BATCH>), dim3(grid), dim3(kThreads), 0, st,
-                           m.sss, bd, src, rvec, pnew, q, stp, partials, guard);
+                           m.sss, bd, src, rvec, pnew, q, stp, rb, guard);
 }
 #endif
 
@@ -92,7 +92,8 @@ struct cicy_sys {
     // FP64 vectors
     double *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
     double *x0 = nullptr, *r0 = nullptr, *b = nullptr;
-    double *partials = nullptr, *scal = nullptr;
+    double *partials = nullptr, *scal = nullptr, *gpart = nullptr;
+    unsigned int *gcount = nullptr;
     cicy::PcgState *st = nullptr;
     unsigned int *counter2 = nullptr;
     bool has_b = false;
@@ -115,6 +116,9 @@ bool build_blocked(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr
                    const int64_t *col_ind, const double *values, const double *diag, int rb,
                    BlockedHost &h);
 bool run_chunks(cicy_sys *s, hipGraphExec_t exec);
+// device -> host copy ordered after all work queued on the context's stream, then waited
+// for (the stream is non-blocking, so a plain hipMemcpy would not wait for it)
+bool d2h_sync(cicy_ctx *ctx, void *dst, const void *src, size_t bytes);
 bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel, double *b2 = nullptr);
 bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
                  cicy_solve_stats *stats, bool keep_x = false);
@@ -134,6 +138,7 @@ void scaled_band(const BlockedHost &h, const BandHost &b, std::vector<float> &ba
                  std::vector<float> &xval32);
 int choose_format(const cicy_ctx *ctx, const BlockedHost &h);
 DevMat<double> mat64(const cicy_sys *s);
+inline RedBuf redbuf(const cicy_sys *s) { return RedBuf{s->partials, s->gpart, s->gcount}; }
 DevMat<float> mat32(const cicy_sys *s);
 
 }  // namespace cicy

@@ -191,6 +191,52 @@ __device__ __forceinline__ double reduce_partials(const double *part, int nb, in
     return block_sum((s0 + s1) + (s2 + s3), lds4);
 }
 
+// Reduction scratch: per work-group partials, per group-of-kGroup partials and counters.
+constexpr int kGroup = 64;
+struct RedBuf {
+    double *part;            // [2 * blocks]
+    double *gpart;           // [2 * blocks] (slot of the group's first block)
+    unsigned int *gcount;    // [blocks]     (slot of the group's first block)
+};
+
+// Two-level, fixed-order reduction of NV values over the work-groups of one system.  Each
+// work-group publishes its sums; the last of each group of kGroup work-groups adds the
+// group's partials in block order and arrives on `top`; the last group leader adds the group
+// sums in group order.  Returns true in that final work-group, with out[] valid in thread
+// 0.  Spreading the arrivals over one counter per group keeps atomic contention on any one
+// address at <= kGroup (a single counter serialises every work-group of the grid).
+template <int NV>
+__device__ __forceinline__ bool reduce_hier(double (&v)[NV], const Blk &k, const RedBuf &rb,
+                                            unsigned int *top, double *red, int *flag,
+                                            double (&out)[NV]) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        const double t = block_sum(v[i], red);
+        if (threadIdx.x == 0) rb.part[(size_t)k.b * NV + i] = t;
+    }
+    const int g0 = k.first + ((k.b - k.first) / kGroup) * kGroup;
+    const int gsz = min(kGroup, k.first + k.count - g0);
+    if (!arrive_last(&rb.gcount[g0], (unsigned)gsz, flag)) return false;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double t = (int)threadIdx.x < gsz ? rb.part[(size_t)(g0 + threadIdx.x) * NV + i] : 0.0;
+        t = block_sum(t, red);
+        if (threadIdx.x == 0) rb.gpart[(size_t)g0 * NV + i] = t;
+    }
+    if (threadIdx.x == 0) rb.gcount[g0] = 0;
+    const int ng = (k.count + kGroup - 1) / kGroup;
+    if (!arrive_last(top, (unsigned)ng, flag)) return false;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        double t = 0.0;
+        for (int gi = threadIdx.x; gi < ng; gi += kThreads)
+            t += rb.gpart[(size_t)(k.first + gi * kGroup) * NV + i];
+        out[i] = block_sum(t, red);
+    }
+    if (threadIdx.x == 0) *top = 0;
+    return true;
+}
+
 // Jacobi-preconditioned search direction p = (1/d) r + beta p_old.  One definition so
 // that the owner of a row and every gather of it compute the same bits.
 template <typename T>
@@ -217,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void k_apply(SssDev<T> s, BatchDev bd,
                                                     const T *__restrict__ rvec,
                                                     T *__restrict__ pnew, T *__restrict__ q,
                                                     PcgState *__restrict__ st_base,
-                                                    double *__restrict__ partials,
+                                                    RedBuf rbuf,
                                                     const MixState *__restrict__ guard) {
     const Blk k = block_info<RB, BATCH>(bd, s.n);
     PcgState *st = st_base ? st_base + k.sys : nullptr;
@@ -270,15 +316,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(SssDev<T> s, BatchDev bd,
         if (PCG) part += (double)sp[t] * (double)qv;
     }
     if (!PCG) return;
-    part = block_sum(part, red);
-    if (threadIdx.x == 0) partials[k.b] = part;
-    if (arrive_last(&st->counter, k.count, &flag)) {
-        double pq = reduce_partials(partials + k.first, k.count, 1, red);
+    double v[1] = {part}, out[1];
+    if (reduce_hier<1>(v, k, rbuf, &st->counter, red, &flag, out)) {
+        const double pq = out[0];
         if (threadIdx.x == 0) {
             double alpha = st->rz / pq;
             st->pq = pq;
             st->alpha = alpha;
-            st->counter = 0;
             if (!(pq > 0.0) || !isfinite(alpha)) {
                 st->status = 2;  // CICY_BREAKDOWN
                 st->done = 1;
@@ -298,7 +342,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev 
                                                          T *__restrict__ pnew,
                                                          T *__restrict__ q,
                                                          PcgState *__restrict__ st_base,
-                                                         double *__restrict__ partials,
+                                                         RedBuf rbuf,
                                                          const MixState *__restrict__ guard) {
     const Blk k = block_info<RB, BATCH>(bd, s.n);
     PcgState *st = st_base ? st_base + k.sys : nullptr;
@@ -350,15 +394,13 @@ __global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev 
         if (PCG) part += (double)sp[t + 1] * (double)qv;
     }
     if (!PCG) return;
-    part = block_sum(part, red);
-    if (threadIdx.x == 0) partials[k.b] = part;
-    if (arrive_last(&st->counter, k.count, &flag)) {
-        double pq = reduce_partials(partials + k.first, k.count, 1, red);
+    double v[1] = {part}, out[1];
+    if (reduce_hier<1>(v, k, rbuf, &st->counter, red, &flag, out)) {
+        const double pq = out[0];
         if (threadIdx.x == 0) {
             double alpha = st->rz / pq;
             st->pq = pq;
             st->alpha = alpha;
-            st->counter = 0;
             if (!(pq > 0.0) || !isfinite(alpha)) {
                 st->status = 2;
                 st->done = 1;
@@ -378,7 +420,7 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(int n, BatchDev bd,
                                                          const double *__restrict__ p,
                                                          const double *__restrict__ q,
                                                          PcgState *__restrict__ st_base,
-                                                         double *__restrict__ partials) {
+                                                         RedBuf rbuf) {
     const Blk k = block_info<RB, BATCH>(bd, n);
     PcgState *st = st_base + k.sys;
     if (st->done) return;
@@ -395,16 +437,10 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(int n, BatchDev bd,
         prz += z * rn;
         prr += rn * rn;
     }
-    block_sum2(prz, prr, red);
-    if (threadIdx.x == 0) {
-        partials[2 * k.b] = prz;
-        partials[2 * k.b + 1] = prr;
-    }
-    if (arrive_last(&st->counter, k.count, &flag)) {
-        double rz = reduce_partials(partials + 2 * k.first, k.count, 2, red);
-        double rr = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
+    double v[2] = {prz, prr}, out[2];
+    if (reduce_hier<2>(v, k, rbuf, &st->counter, red, &flag, out)) {
+        const double rz = out[0], rr = out[1];
         if (threadIdx.x == 0) {
-            st->counter = 0;
             st->iters += 1;
             st->rr = rr;
             if (rr <= st->stop2) {
@@ -435,7 +471,7 @@ __global__ __launch_bounds__(kThreads) void k_init_f64(int n, BatchDev bd,
                                                        double *__restrict__ p0,
                                                        double *__restrict__ p1,
                                                        PcgState *__restrict__ st_base,
-                                                       double *__restrict__ partials,
+                                                       RedBuf rbuf,
                                                        double tol,
                                                        const double *__restrict__ stop_ref2,
                                                        double stop_ref2_all,
@@ -454,16 +490,10 @@ __global__ __launch_bounds__(kThreads) void k_init_f64(int n, BatchDev bd,
         prz += z * ri;
         prr += ri * ri;
     }
-    block_sum2(prz, prr, red);
-    if (threadIdx.x == 0) {
-        partials[2 * k.b] = prz;
-        partials[2 * k.b + 1] = prr;
-    }
-    if (arrive_last(&st->counter, k.count, &flag)) {
-        double rz = reduce_partials(partials + 2 * k.first, k.count, 2, red);
-        double rr = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
+    double v[2] = {prz, prr}, out[2];
+    if (reduce_hier<2>(v, k, rbuf, &st->counter, red, &flag, out)) {
+        const double rz = out[0], rr = out[1];
         if (threadIdx.x == 0) {
-            st->counter = 0;
             st->rz = rz;
             st->rr = rr;
             st->rr_init = rr;
@@ -498,7 +528,7 @@ __global__ __launch_bounds__(kThreads) void k_mixed_init(int n, BatchDev bd,
                                                          float *__restrict__ ps1,
                                                          PcgState *__restrict__ st_base,
                                                          MixState *__restrict__ mx_base,
-                                                         double *__restrict__ partials,
+                                                         RedBuf rbuf,
                                                          double tol,
                                                          const double *__restrict__ stop_ref2,
                                                          double stop_ref2_all,
@@ -520,16 +550,10 @@ __global__ __launch_bounds__(kThreads) void k_mixed_init(int n, BatchDev bd,
         a += (double)v * (double)v;
         t += ri * ri;
     }
-    block_sum2(a, t, red);
-    if (threadIdx.x == 0) {
-        partials[2 * k.b] = a;
-        partials[2 * k.b + 1] = t;
-    }
-    if (arrive_last(&st->counter, k.count, &flag)) {
-        double rs2 = reduce_partials(partials + 2 * k.first, k.count, 2, red);
-        double r02 = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
+    double v[2] = {a, t}, out[2];
+    if (reduce_hier<2>(v, k, rbuf, &st->counter, red, &flag, out)) {
+        const double rs2 = out[0], r02 = out[1];
         if (threadIdx.x == 0) {
-            st->counter = 0;
             st->rz = rs2;
             st->rr = rs2;
             st->rr_init = r02;
@@ -570,7 +594,7 @@ __global__ __launch_bounds__(kThreads) void k_update_mixed(int n, BatchDev bd,
                                                            const float *__restrict__ q,
                                                            PcgState *__restrict__ st_base,
                                                            MixState *__restrict__ mx_base,
-                                                           double *__restrict__ partials) {
+                                                           RedBuf rbuf) {
     const Blk k = block_info<RB, BATCH>(bd, n);
     PcgState *st = st_base + k.sys;
     if (st->done) return;
@@ -586,12 +610,10 @@ __global__ __launch_bounds__(kThreads) void k_update_mixed(int n, BatchDev bd,
         rs[i] = rn;
         prr += (double)rn * (double)rn;
     }
-    prr = block_sum(prr, red);
-    if (threadIdx.x == 0) partials[k.b] = prr;
-    if (arrive_last(&st->counter, k.count, &flag)) {
-        double rr = reduce_partials(partials + k.first, k.count, 1, red);
+    double v[1] = {prr}, out[1];
+    if (reduce_hier<1>(v, k, rbuf, &st->counter, red, &flag, out)) {
+        const double rr = out[0];
         if (threadIdx.x == 0) {
-            st->counter = 0;
             st->iters += 1;
             if (!isfinite(rr)) {
                 st->status = 2;
@@ -641,7 +663,7 @@ __global__ __launch_bounds__(kThreads) void k_reliable_resid(int n, BatchDev bd,
                                                              float *__restrict__ rs,
                                                              PcgState *__restrict__ st_base,
                                                              MixState *__restrict__ mx_base,
-                                                             double *__restrict__ partials,
+                                                             RedBuf rbuf,
                                                              int stall_limit) {
     const Blk k = block_info<RB, BATCH>(bd, n);
     MixState *mx = mx_base + k.sys;
@@ -659,16 +681,10 @@ __global__ __launch_bounds__(kThreads) void k_reliable_resid(int n, BatchDev bd,
         a += (double)v * (double)v;
         t += ri * ri;
     }
-    block_sum2(a, t, red);
-    if (threadIdx.x == 0) {
-        partials[2 * k.b] = a;
-        partials[2 * k.b + 1] = t;
-    }
-    if (arrive_last(&mx->counter, k.count, &flag)) {
-        double rs2 = reduce_partials(partials + 2 * k.first, k.count, 2, red);
-        double rt2 = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
+    double v[2] = {a, t}, out[2];
+    if (reduce_hier<2>(v, k, rbuf, &mx->counter, red, &flag, out)) {
+        const double rs2 = out[0], rt2 = out[1];
         if (threadIdx.x == 0) {
-            mx->counter = 0;
             mx->refinements += 1;
             mx->need_update = 0;
             if (rt2 > 0.25 * mx->rtrue2) mx->stall += 1;

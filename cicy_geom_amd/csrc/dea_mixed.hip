@@ -50,25 +50,25 @@ void launch_chunk(cicy_sys *s) {
         float *pold = (it & 1) ? s->ps1 : s->ps0;
         float *pnew = (it & 1) ? s->ps0 : s->ps1;
         launch_apply<float, RB, true, true, false>(mat32(s), s->nblocks, st, kNoBatch, pold,
-                                                   s->rs, pnew, s->qs, s->st, s->partials,
+                                                   s->rs, pnew, s->qs, s->st, redbuf(s),
                                                    nullptr);
         hipLaunchKernelGGL((k_update_mixed<RB, false>), g, b, 0, st, s->n, kNoBatch, s->xs,
-                           s->rs, pnew, s->qs, s->st, s->mix, s->partials);
+                           s->rs, pnew, s->qs, s->st, s->mix, redbuf(s));
     }
     hipLaunchKernelGGL((k_fold<RB, false>), g, b, 0, st, s->n, kNoBatch, s->diag, s->x, s->xs,
                        s->mix, 0);
     launch_apply<double, RB, false, false, false>(mat64(s), s->nblocks, st, kNoBatch, s->x,
-                                                  nullptr, nullptr, s->q, nullptr, nullptr,
+                                                  nullptr, nullptr, s->q, nullptr, RedBuf{},
                                                   s->mix);
     hipLaunchKernelGGL((k_reliable_resid<RB, false>), g, b, 0, st, s->n, kNoBatch, s->diag,
-                       s->b, s->q, s->r, s->rs, s->st, s->mix, s->partials, kStallLimit);
+                       s->b, s->q, s->r, s->rs, s->st, s->mix, redbuf(s), kStallLimit);
 }
 
 template <int RB>
 void launch_init(cicy_sys *s, double tol, double stop_ref2, long long max_iter) {
     hipLaunchKernelGGL((k_mixed_init<RB, false>), dim3(s->nblocks), dim3(kThreads), 0,
                        s->ctx->stream, s->n, kNoBatch, s->diag, s->r0, s->rs, s->xs, s->ps0,
-                       s->ps1, s->st, s->mix, s->partials, tol, nullptr, stop_ref2, max_iter,
+                       s->ps1, s->st, s->mix, redbuf(s), tol, nullptr, stop_ref2, max_iter,
                        kDelta * kDelta);
 }
 
@@ -82,13 +82,13 @@ template <int RB>
 void launch_spmv64(cicy_sys *s, const double *v, double *out) {
     launch_apply<double, RB, false, false, false>(mat64(s), s->nblocks, s->ctx->stream,
                                                   kNoBatch, v, nullptr, nullptr, out, nullptr,
-                                                  nullptr, nullptr);
+                                                  RedBuf{}, nullptr);
 }
 
 template <int RB>
 void launch_spmv32(cicy_sys *s) {
     launch_apply<float, RB, false, true, false>(mat32(s), s->nblocks, s->ctx->stream, kNoBatch,
-                                                s->ps0, nullptr, nullptr, s->qs, nullptr, nullptr,
+                                                s->ps0, nullptr, nullptr, s->qs, nullptr, RedBuf{},
                                                 nullptr);
 }
 
@@ -202,8 +202,8 @@ bool solve_mixed(cicy_sys *s, double tol, long long max_iter, int rel_to,
     CICY_RB_SWITCH(s->rb, launch_fold_always<RB>(s));
     PcgState h;
     MixState m;
-    if (!hip_ok(ctx, hipMemcpy(&h, s->st, sizeof(h), hipMemcpyDeviceToHost), "D2H") ||
-        !hip_ok(ctx, hipMemcpy(&m, s->mix, sizeof(m), hipMemcpyDeviceToHost), "D2H"))
+    if (!d2h_sync(ctx, &h, s->st, sizeof(h)) ||
+        !d2h_sync(ctx, &m, s->mix, sizeof(m)))
         return false;
     // final residual r = b - A x (the R the caller gets back)
     CICY_RB_SWITCH(s->rb, launch_spmv64<RB>(s, s->x, s->q));

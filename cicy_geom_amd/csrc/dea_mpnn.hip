@@ -359,7 +359,7 @@ void spmv_x0(cicy_sys *s) {
     const BatchDev nb{nullptr, nullptr, nullptr, nullptr, nullptr};
     launch_apply<double, RB, false, false, false>(mat64(s), s->nblocks, s->ctx->stream, nb,
                                                   s->x0, nullptr, nullptr, s->q, nullptr,
-                                                  nullptr, nullptr);
+                                                  RedBuf{}, nullptr);
 }
 
 int grid_for(int n) { return std::max(1, std::min((n + NODES_PER_PASS - 1) / NODES_PER_PASS, 4096)); }
@@ -485,13 +485,13 @@ int cicy_mpnn_set_params(cicy_mpnn *M, const float *theta) {
 
 int cicy_mpnn_get_params(cicy_mpnn *M, float *theta) {
     if (!M || !theta) return 0;
-    return hip_ok(M->ctx, hipMemcpy(theta, M->theta, NPARAM * 4, hipMemcpyDeviceToHost), "D2H")
+    return d2h_sync(M->ctx, theta, M->theta, NPARAM * 4)
                ? 1 : 0;
 }
 
 int cicy_mpnn_get_grads(cicy_mpnn *M, float *grad) {
     if (!M || !grad) return 0;
-    return hip_ok(M->ctx, hipMemcpy(grad, M->grad, NPARAM * 4, hipMemcpyDeviceToHost), "D2H")
+    return d2h_sync(M->ctx, grad, M->grad, NPARAM * 4)
                ? 1 : 0;
 }
 

@@ -67,6 +67,15 @@ __global__ __launch_bounds__(kThreads) void k_sub(int n, double *__restrict__ r,
     if (i < n) r[i] = r[i] - ax[i];
 }
 
+// Plain copy (16-byte vectors, grid-stride): the known-size reference for traffic counters
+// and the achievable-bandwidth yardstick.
+__global__ __launch_bounds__(kThreads) void k_copy(size_t n16, const uint4 *__restrict__ src,
+                                                   uint4 *__restrict__ dst) {
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n16;
+         i += (size_t)gridDim.x * kThreads)
+        dst[i] = src[i];
+}
+
 // FP64 boundary current: one work-group, fixed strided order, tree reduction.
 __global__ __launch_bounds__(kThreads) void k_boundary_current(
     long long ne, const long long *__restrict__ nb, const long long *__restrict__ ni,
@@ -96,6 +105,12 @@ bool hip_ok(cicy_ctx *ctx, hipError_t e, const char *what) {
     if (e == hipSuccess) return true;
     set_error(ctx, "%s failed: %s", what, hipGetErrorString(e));
     return false;
+}
+
+bool d2h_sync(cicy_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return hip_ok(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream),
+                  "D2H") &&
+           hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
 }
 
 // Validates the reference's SSS arrays and builds the blocked layout.
@@ -301,7 +316,7 @@ struct LaunchSpmv64 {
     static void run(cicy_sys *s, const double *v, double *out) {
         launch_apply<double, RB, false, false, false>(mat64(s), s->nblocks, s->ctx->stream,
                                                       kNoBatch, v, nullptr, nullptr, out,
-                                                      nullptr, nullptr, nullptr);
+                                                      nullptr, RedBuf{}, nullptr);
     }
 };
 
@@ -310,7 +325,7 @@ struct LaunchInit64 {
     static void run(cicy_sys *s, double tol, double stop_ref2, long long max_iter) {
         hipLaunchKernelGGL((k_init_f64<RB, false>), dim3(s->nblocks), dim3(kThreads), 0,
                            s->ctx->stream, s->n, kNoBatch, s->diag, s->r, s->p0, s->p1, s->st,
-                           s->partials, tol, nullptr, stop_ref2, max_iter);
+                           redbuf(s), tol, nullptr, stop_ref2, max_iter);
     }
 };
 
@@ -322,10 +337,10 @@ struct LaunchIter64 {
         double *pnew = parity ? s->p0 : s->p1;
         launch_apply<double, RB, true, false, false>(mat64(s), s->nblocks, s->ctx->stream,
                                                      kNoBatch, pold, s->r, pnew, s->q, s->st,
-                                                     s->partials, nullptr);
+                                                     redbuf(s), nullptr);
         hipLaunchKernelGGL((k_update_f64<RB, false>), dim3(s->nblocks), dim3(kThreads), 0,
                            s->ctx->stream, s->n, kNoBatch, s->diag, s->x, s->r, pnew, s->q,
-                           s->st, s->partials);
+                           s->st, redbuf(s));
     }
 };
 
@@ -442,8 +457,7 @@ bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
             return false;
         dispatch_rb<LaunchInit64>(s->rb, s, tol, stop_ref2, max_iter - total_iters);
         if (!run_chunks(s, s->graph_exec64)) return false;
-        if (!hip_ok(ctx, hipMemcpy(&h, s->st, sizeof(PcgState), hipMemcpyDeviceToHost),
-                    "D2H state"))
+        if (!d2h_sync(ctx, &h, s->st, sizeof(PcgState)))
             return false;
         if (rr_init < 0.0) rr_init = h.rr_init;
         total_iters += h.iters;
@@ -592,6 +606,8 @@ int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_pt
               dmalloc(ctx, &s->p1, nv, &tot) && dmalloc(ctx, &s->q, nv, &tot) &&
               dmalloc(ctx, &s->x0, nv, &tot) && dmalloc(ctx, &s->r0, nv, &tot) &&
               dmalloc(ctx, &s->b, nv, &tot) && dmalloc(ctx, &s->partials, nparts, &tot) &&
+              dmalloc(ctx, &s->gpart, nparts, &tot) &&
+              dmalloc(ctx, &s->gcount, std::max<size_t>((size_t)h.nblocks, 1), &tot) &&
               dmalloc(ctx, &s->st, 1, &tot) && dmalloc(ctx, &s->scal, 4, &tot) &&
               dmalloc(ctx, &s->counter2, 1, &tot);
     ok = ok && h2d(ctx, s->diag, h.diag.data(), nv) &&
@@ -599,6 +615,9 @@ int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_pt
          hip_ok(ctx, hipMemsetAsync(s->p0, 0, nv * sizeof(double), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->r0, 0, nv * sizeof(double), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->counter2, 0, sizeof(unsigned), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(s->gcount, 0,
+                                    std::max<size_t>((size_t)h.nblocks, 1) * sizeof(unsigned),
+                                    ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), ctx->stream), "memset") &&
          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     s->device_bytes = tot;
@@ -622,7 +641,8 @@ void cicy_sys_destroy(cicy_sys *s) {
     cicy::mixed_free(s);
     void *ptrs[] = {s->rp, s->col, s->val, s->diag, s->urp, s->ucol, s->uval, s->band,
                     s->xblk, s->xrow, s->xnbr, s->xval, s->x, s->r, s->p0, s->p1, s->q,
-                    s->x0, s->r0, s->b, s->partials, s->st, s->scal, s->counter2};
+                    s->x0, s->r0, s->b, s->partials, s->gpart, s->gcount, s->st, s->scal,
+                    s->counter2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete s;
@@ -721,6 +741,32 @@ int cicy_sys_spmv_bench(cicy_sys *s, int precision, int reps, double *avg_ms) {
     (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
     *avg_ms = ms / reps;
     return hip_ok(ctx, hipGetLastError(), "kernel launch") ? 1 : 0;
+}
+
+int cicy_copy_bench(cicy_ctx *ctx, int64_t bytes, int reps, double *avg_ms) {
+    if (!ctx || bytes < 16 || reps <= 0 || !avg_ms) return 0;
+    (void)hipSetDevice(ctx->device);
+    const size_t n16 = (size_t)bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    if (!hip_ok(ctx, hipMalloc((void **)&a, n16 * 16), "hipMalloc") ||
+        !hip_ok(ctx, hipMalloc((void **)&b, n16 * 16), "hipMalloc")) {
+        if (a) (void)hipFree(a);
+        return 0;
+    }
+    bool ok = hip_ok(ctx, hipMemsetAsync(a, 0, n16 * 16, ctx->stream), "memset");
+    const int grid = (int)std::min<size_t>((n16 + kThreads - 1) / kThreads, 256 * 64);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kThreads), 0, ctx->stream, n16, a, b);
+    ok = ok && hip_ok(ctx, hipEventRecord(ctx->ev_start, ctx->stream), "event");
+    for (int i = 0; i < reps; i++)
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kThreads), 0, ctx->stream, n16, a, b);
+    ok = ok && hip_ok(ctx, hipEventRecord(ctx->ev_stop, ctx->stream), "event") &&
+         hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "sync");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    *avg_ms = ms / reps;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return ok ? 1 : 0;
 }
 
 int cicy_spmv_sss_f64(cicy_ctx *ctx, int64_t n, const int64_t *row_ptr, const int64_t *col_ind,

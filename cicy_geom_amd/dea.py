@@ -26,7 +26,7 @@ _VP = ctypes.c_void_p
 # Every symbol the header declares; tests check that the library exports all of them.
 EXPORTED_SYMBOLS = [
     "cicy_create", "cicy_destroy", "cicy_last_error", "cicy_set_rows_per_block",
-    "cicy_set_format", "cicy_host_layout_spmv",
+    "cicy_set_format", "cicy_host_layout_spmv", "cicy_copy_bench",
     "cicy_solve_dea_cg_sss", "cicy_update_residual", "cicy_spmv_sss_f64",
     "cicy_boundary_current_f64", "cicy_sys_create", "cicy_sys_destroy", "cicy_sys_size",
     "cicy_sys_set_rhs", "cicy_sys_solve", "cicy_sys_get", "cicy_sys_spmv_bench",
@@ -65,6 +65,7 @@ def lib():
     L.cicy_last_error.argtypes = [_VP]
     L.cicy_set_rows_per_block.argtypes = [_VP, ctypes.c_int]
     L.cicy_set_format.argtypes = [_VP, ctypes.c_int]
+    L.cicy_copy_bench.argtypes = [_VP, ctypes.c_int64, ctypes.c_int, P(ctypes.c_double)]
     L.cicy_host_layout_spmv.argtypes = [ctypes.c_int, ctypes.c_int64, _I64, _I64, _F64, _F64,
                                         ctypes.c_int64, ctypes.c_int, _F64, _F64]
     L.cicy_solve_dea_cg_sss.argtypes = [_VP, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
@@ -156,6 +157,13 @@ class Context:
     def _adopt(self, child):
         """Objects holding device memory of this context are closed before it."""
         self._children.add(child)
+
+    def copy_bench(self, nbytes: int, reps: int = 20) -> float:
+        """Average ms of a device copy of nbytes (reads nbytes, writes nbytes)."""
+        ms = ctypes.c_double()
+        self._check(self._L.cicy_copy_bench(self._h, nbytes, reps, ctypes.byref(ms)),
+                    "cicy_copy_bench")
+        return ms.value
 
     def set_format(self, fmt: int):
         self._check(self._L.cicy_set_format(self._h, fmt), "cicy_set_format")

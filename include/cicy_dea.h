@@ -125,6 +125,9 @@ int cicy_sys_solve(cicy_sys *sys, int method, double tol, int64_t max_iter, int 
 int cicy_sys_get(cicy_sys *sys, double *x, double *r);
 /* Time `reps` back-to-back SSS SpMV launches on resident data; average ms per launch. */
 int cicy_sys_spmv_bench(cicy_sys *sys, int precision /*64 or 32*/, int reps, double *avg_ms);
+/* Time a plain device copy of `bytes` (read `bytes`, write `bytes`); the reference kernel for
+ * traffic counters and the achievable-bandwidth yardstick. */
+int cicy_copy_bench(cicy_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
 
 /* ---- batches of independent systems (clusters x directions x realisations) -----------
  * The reference solves each (cluster, direction) system in turn with a fresh object

@@ -105,6 +105,10 @@ def test_training_reduces_loss_and_warm_start_is_judged(ctx):
     g.close()
     model.close()
     assert st0.status == dea.CICY_OK and st1.status == dea.CICY_OK
-    assert st1.true_rel_res <= 1.05e-10
-    assert np.linalg.norm(x1sol - x0sol) / np.linalg.norm(x0sol) < 1e-8
+    assert st1.true_rel_res <= 1e-10 * (1 + 1e-6)
+    # both solutions sit within CG's error at this residual of the FP64 truth
+    import oracle
+    xt, _, _ = oracle.cg_truth(s.row_ptr, s.col_ind, s.values, s.diag, s.b, tol=1e-13)
+    for xs in (x0sol, x1sol):
+        assert np.linalg.norm(xs - xt) / np.linalg.norm(xt) < 1e-7
     sysd.close()
