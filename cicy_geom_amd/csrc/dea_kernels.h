@@ -191,50 +191,65 @@ __device__ __forceinline__ double reduce_partials(const double *part, int nb, in
     return block_sum((s0 + s1) + (s2 + s3), lds4);
 }
 
-// Reduction scratch: per work-group partials, per group-of-kGroup partials and counters.
+// Reduction scratch: accumulators and arrival counters at the slot of each group's first
+// work-group (groups of kGroup work-groups), and the system's total at its first slot.
 constexpr int kGroup = 64;
 struct RedBuf {
-    double *part;            // [2 * blocks]
-    double *gpart;           // [2 * blocks] (slot of the group's first block)
-    unsigned int *gcount;    // [blocks]     (slot of the group's first block)
+    double *part;            // [2 * blocks] system totals (slot of the system's first block)
+    double *gpart;           // [2 * blocks] group accumulators (slot of the group's first block)
+    unsigned int *gcount;    // [blocks]     group arrival counters
 };
 
-// Two-level, fixed-order reduction of NV values over the work-groups of one system.  Each
-// work-group publishes its sums; the last of each group of kGroup work-groups adds the
-// group's partials in block order and arrives on `top`; the last group leader adds the group
-// sums in group order.  Returns true in that final work-group, with out[] valid in thread
-// 0.  Spreading the arrivals over one counter per group keeps atomic contention on any one
-// address at <= kGroup (a single counter serialises every work-group of the grid).
+// Makes `v` a register operand of an empty asm statement with a memory clobber: the value
+// must have returned (s_waitcnt) before any later memory operation is issued.
+__device__ __forceinline__ void depend_on(double v) { asm volatile("" ::"v"(v) : "memory"); }
+
+// Reduction of NV values over the work-groups of one system without cache-wide fences.
+// Thread 0 adds the work-group's sums into its group's accumulators with returning
+// device-scope FP64 atomics; once they have returned it arrives on the group counter.  The
+// last arriver of a group takes the group sums (atomicExch, which also resets them), adds
+// them into the system totals and arrives on `top`; the last group leader takes the totals.
+// Every step is a device-scope atomic, performed at the point of coherence, so no L2
+// write-back/invalidate (which an agent-scope __threadfence costs on a multi-XCD part) is
+// needed.  Returns true in the final work-group with out[] valid in thread 0.  The FP64
+// atomic adds make the last bits of the sums depend on arrival order.
 template <int NV>
-__device__ __forceinline__ bool reduce_hier(double (&v)[NV], const Blk &k, const RedBuf &rb,
-                                            unsigned int *top, double *red, int *flag,
-                                            double (&out)[NV]) {
+__device__ __forceinline__ bool reduce_sys(double (&v)[NV], const Blk &k, const RedBuf &rb,
+                                           unsigned int *top, double *red, int *flag,
+                                           double (&out)[NV]) {
+    double s[NV];
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
-        const double t = block_sum(v[i], red);
-        if (threadIdx.x == 0) rb.part[(size_t)k.b * NV + i] = t;
-    }
-    const int g0 = k.first + ((k.b - k.first) / kGroup) * kGroup;
-    const int gsz = min(kGroup, k.first + k.count - g0);
-    if (!arrive_last(&rb.gcount[g0], (unsigned)gsz, flag)) return false;
+    for (int i = 0; i < NV; i++) s[i] = block_sum(v[i], red);
+    if (threadIdx.x == 0) {
+        int last = 0;
+        const int g0 = k.first + ((k.b - k.first) / kGroup) * kGroup;
+        const int gsz = min(kGroup, k.first + k.count - g0);
+        double dep = 0.0;
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
-        double t = (int)threadIdx.x < gsz ? rb.part[(size_t)(g0 + threadIdx.x) * NV + i] : 0.0;
-        t = block_sum(t, red);
-        if (threadIdx.x == 0) rb.gpart[(size_t)g0 * NV + i] = t;
-    }
-    if (threadIdx.x == 0) rb.gcount[g0] = 0;
-    const int ng = (k.count + kGroup - 1) / kGroup;
-    if (!arrive_last(top, (unsigned)ng, flag)) return false;
+        for (int i = 0; i < NV; i++) dep += atomicAdd(&rb.gpart[(size_t)g0 * NV + i], s[i]);
+        depend_on(dep);
+        if (atomicAdd(&rb.gcount[g0], 1u) == (unsigned)(gsz - 1)) {
+            double g[NV];
+            dep = 0.0;
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
-        double t = 0.0;
-        for (int gi = threadIdx.x; gi < ng; gi += kThreads)
-            t += rb.gpart[(size_t)(k.first + gi * kGroup) * NV + i];
-        out[i] = block_sum(t, red);
+            for (int i = 0; i < NV; i++) g[i] = atomicExch(&rb.gpart[(size_t)g0 * NV + i], 0.0);
+            (void)atomicExch(&rb.gcount[g0], 0u);
+#pragma unroll
+            for (int i = 0; i < NV; i++) dep += atomicAdd(&rb.part[(size_t)k.first * NV + i], g[i]);
+            depend_on(dep);
+            const int ng = (k.count + kGroup - 1) / kGroup;
+            if (atomicAdd(top, 1u) == (unsigned)(ng - 1)) {
+#pragma unroll
+                for (int i = 0; i < NV; i++)
+                    out[i] = atomicExch(&rb.part[(size_t)k.first * NV + i], 0.0);
+                (void)atomicExch(top, 0u);
+                last = 1;
+            }
+        }
+        *flag = last;
     }
-    if (threadIdx.x == 0) *top = 0;
-    return true;
+    __syncthreads();
+    return *flag != 0;
 }
 
 // Jacobi-preconditioned search direction p = (1/d) r + beta p_old.  One definition so
@@ -317,7 +332,7 @@ __global__ __launch_bounds__(kThreads) void k_apply(SssDev<T> s, BatchDev bd,
     }
     if (!PCG) return;
     double v[1] = {part}, out[1];
-    if (reduce_hier<1>(v, k, rbuf, &st->counter, red, &flag, out)) {
+    if (reduce_sys<1>(v, k, rbuf, &st->counter, red, &flag, out)) {
         const double pq = out[0];
         if (threadIdx.x == 0) {
             double alpha = st->rz / pq;
@@ -395,7 +410,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev 
     }
     if (!PCG) return;
     double v[1] = {part}, out[1];
-    if (reduce_hier<1>(v, k, rbuf, &st->counter, red, &flag, out)) {
+    if (reduce_sys<1>(v, k, rbuf, &st->counter, red, &flag, out)) {
         const double pq = out[0];
         if (threadIdx.x == 0) {
             double alpha = st->rz / pq;
@@ -438,7 +453,7 @@ __global__ __launch_bounds__(kThreads) void k_update_f64(int n, BatchDev bd,
         prr += rn * rn;
     }
     double v[2] = {prz, prr}, out[2];
-    if (reduce_hier<2>(v, k, rbuf, &st->counter, red, &flag, out)) {
+    if (reduce_sys<2>(v, k, rbuf, &st->counter, red, &flag, out)) {
         const double rz = out[0], rr = out[1];
         if (threadIdx.x == 0) {
             st->iters += 1;
@@ -491,7 +506,7 @@ __global__ __launch_bounds__(kThreads) void k_init_f64(int n, BatchDev bd,
         prr += ri * ri;
     }
     double v[2] = {prz, prr}, out[2];
-    if (reduce_hier<2>(v, k, rbuf, &st->counter, red, &flag, out)) {
+    if (reduce_sys<2>(v, k, rbuf, &st->counter, red, &flag, out)) {
         const double rz = out[0], rr = out[1];
         if (threadIdx.x == 0) {
             st->rz = rz;
@@ -551,7 +566,7 @@ __global__ __launch_bounds__(kThreads) void k_mixed_init(int n, BatchDev bd,
         t += ri * ri;
     }
     double v[2] = {a, t}, out[2];
-    if (reduce_hier<2>(v, k, rbuf, &st->counter, red, &flag, out)) {
+    if (reduce_sys<2>(v, k, rbuf, &st->counter, red, &flag, out)) {
         const double rs2 = out[0], r02 = out[1];
         if (threadIdx.x == 0) {
             st->rz = rs2;
@@ -611,7 +626,7 @@ __global__ __launch_bounds__(kThreads) void k_update_mixed(int n, BatchDev bd,
         prr += (double)rn * (double)rn;
     }
     double v[1] = {prr}, out[1];
-    if (reduce_hier<1>(v, k, rbuf, &st->counter, red, &flag, out)) {
+    if (reduce_sys<1>(v, k, rbuf, &st->counter, red, &flag, out)) {
         const double rr = out[0];
         if (threadIdx.x == 0) {
             st->iters += 1;
@@ -682,7 +697,7 @@ __global__ __launch_bounds__(kThreads) void k_reliable_resid(int n, BatchDev bd,
         t += ri * ri;
     }
     double v[2] = {a, t}, out[2];
-    if (reduce_hier<2>(v, k, rbuf, &mx->counter, red, &flag, out)) {
+    if (reduce_sys<2>(v, k, rbuf, &mx->counter, red, &flag, out)) {
         const double rs2 = out[0], rt2 = out[1];
         if (threadIdx.x == 0) {
             mx->refinements += 1;
