@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--rows-per-block", type=int, default=0,
                     help="skip the autotuner and use this row-block size")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--methods", default="pcg_f64,learned_f64",
+                    help="comma list from pcg_f64, mixed, learned_f64, learned_mixed (the "
+                         "learned ones need trained weights)")
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only run the SpMV kernels (both precisions) and a 1 GiB reference "
                          "copy, for hardware-counter runs")
@@ -141,8 +144,27 @@ def main():
         sysd.close()
         ctx.close()
         return
+    catalog = {"pcg_f64": (dea.METHOD_PCG_F64, "pcg_f64", False),
+               "mixed": (dea.METHOD_MIXED, "mixed", False),
+               "learned_f64": (dea.METHOD_PCG_F64, "pcg_f64", True),
+               "learned_mixed": (dea.METHOD_MIXED, "mixed", True)}
+    wanted = [m for m in args.methods.split(",") if m]
+    bad = [m for m in wanted if m not in catalog]
+    if bad:
+        sys.exit(f"unknown method(s) {bad}; choose from {sorted(catalog)}")
+    if "pcg_f64" not in wanted:
+        wanted.insert(0, "pcg_f64")          # the plain FP64 solve is always the reference line
+    model = graph = None
+    if os.path.exists(WEIGHTS) and not args.no_learned:
+        from cicy_geom_amd import warmstart
+        theta = np.load(WEIGHTS, allow_pickle=False)["theta"]
+        model = warmstart.WarmStartNet(ctx, theta)
+    plan = [(m,) + catalog[m] for m in wanted if model is not None or not catalog[m][2]]
+
     tuned = {}
     for name, method in (("pcg_f64", dea.METHOD_PCG_F64), ("mixed", dea.METHOD_MIXED)):
+        if not any(p[2] == name for p in plan):
+            continue
         if args.rows_per_block:
             tuned[name] = {"format": dea.FORMAT_AUTO, "rows_per_block": args.rows_per_block}
         else:
@@ -152,16 +174,7 @@ def main():
             log(f"[bench] autotune {name}: {autotune.LAYOUTS.get(tuned[name]['format'])} "
                 f"rb={tuned[name]['rows_per_block']} ({time.perf_counter() - t0:.1f}s)")
 
-    results, systems = {}, {}
-    model = graph = None
-    if os.path.exists(WEIGHTS) and not args.no_learned:
-        from cicy_geom_amd import warmstart
-        theta = np.load(WEIGHTS, allow_pickle=False)["theta"]
-        model = warmstart.WarmStartNet(ctx, theta)
-    plan = [("pcg_f64", dea.METHOD_PCG_F64, "pcg_f64", False),
-            ("mixed", dea.METHOD_MIXED, "mixed", False)]
-    if model is not None:
-        plan.append(("learned_mixed", dea.METHOD_MIXED, "mixed", True))
+    results, systems, by_kind = {}, {}, {}
     upload_s = 0.0
     for name, method, tkey, learned in plan:
         autotune.apply(ctx, tuned[tkey])
@@ -184,6 +197,7 @@ def main():
         if st.status != dea.CICY_OK:
             log(f"[bench] WARNING {name} did not converge")
         systems[name] = sysd
+        by_kind.setdefault(tkey, sysd)
         if learned:
             graph.close()
 
@@ -191,18 +205,24 @@ def main():
     n, L = s.n, s.nnz
     spmv = {}
     for prec, sv, key in ((64, 8, "pcg_f64"), (32, 4, "mixed")):
-        ms = systems[key].spmv_bench(prec, 50)
+        if key not in by_kind:
+            continue
+        ms = by_kind[key].spmv_bench(prec, 50)
         algo = network.spmv_bytes(n, L, sv=sv, si=4)
         spmv[prec] = dict(ms=ms, bytes=algo, gbs=algo / (ms * 1e-3) / 1e9)
         log(f"[bench] SpMV fp{prec}: {ms * 1e3:.1f} us, {algo / 1e6:.1f} MB algorithmic, "
             f"{spmv[prec]['gbs']:.0f} GB/s")
 
     head = "pcg_f64"
-    for cand in ("mixed", "learned_mixed"):
+    for cand in ("learned_f64", "mixed", "learned_mixed"):
         if cand in results and results[cand]["status"] == "ok" and \
                 results[cand]["solves_per_s"] > results[head]["solves_per_s"]:
             head = cand
     value = results[head]["solves_per_s"]
+    hp = 64 if catalog[head][1] == "pcg_f64" else 32
+    # whole-iteration rate of the plain FP64 solve (device time / iterations)
+    it_bytes = network.pcg_iter_bytes(n, L)
+    it_us = 1e3 * results["pcg_f64"]["device_ms"] / max(1, results["pcg_f64"]["iters"])
     out = {
         "metric": "CG solves/s to 1e-10 residual at 4M nodes; learned+mixed vs plain FP64 GPU CG",
         "value": value,
@@ -214,7 +234,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "pcg_f64" else "f32+f64",
+        "dtype": "f64" if hp == 64 else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"config {args.config}: {s.n}-node DEA system (8 um RVE, 160k "
                                "straight CNTs, largest x-percolating cluster, R_flag=1)",
@@ -222,16 +242,18 @@ def main():
                    "method": head},
         "methods": results,
         "speedup_vs_plain_f64": value / results["pcg_f64"]["solves_per_s"],
-        "roofline": {"bound": "hbm", "achieved": spmv[64 if head == "pcg_f64" else 32]["gbs"],
+        "roofline": {"bound": "hbm", "achieved": spmv[hp]["gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv[64 if head == "pcg_f64" else 32]["gbs"] / HBM_PEAK_GBS,
+                     "frac": spmv[hp]["gbs"] / HBM_PEAK_GBS,
                      "traffic": None,
                      "kernel": "SSS SpMV (k_apply/k_apply_band) " +
-                               ("fp64" if head == "pcg_f64" else "fp32 Jacobi-scaled"),
-                     "algorithmic_bytes": spmv[64 if head == "pcg_f64" else 32]["bytes"],
-                     "us_per_launch": spmv[64 if head == "pcg_f64" else 32]["ms"] * 1e3},
+                               ("fp64" if hp == 64 else "fp32 Jacobi-scaled"),
+                     "algorithmic_bytes": spmv[hp]["bytes"],
+                     "us_per_launch": spmv[hp]["ms"] * 1e3},
         "roofline_spmv_f64": {"achieved": spmv[64]["gbs"], "frac": spmv[64]["gbs"] / HBM_PEAK_GBS,
                               "us": spmv[64]["ms"] * 1e3},
+        "pcg_iteration": {"algorithmic_bytes": it_bytes, "us": it_us,
+                          "gbs": it_bytes / (it_us * 1e-6) / 1e9 if it_us > 0 else None},
         "upload_s": upload_s,
     }
     if not args.no_cpu:
@@ -240,9 +262,11 @@ def main():
     for sysd in systems.values():
         sysd.close()
     if model is not None:
-        out["learned"] = {"weights": os.path.relpath(WEIGHTS, ROOT),
-                          "iters_saved_vs_mixed_zero_guess":
-                              1.0 - results["learned_mixed"]["iters"] / results["mixed"]["iters"]}
+        learned = {"weights": os.path.relpath(WEIGHTS, ROOT)}
+        for lk, zk in (("learned_f64", "pcg_f64"), ("learned_mixed", "mixed")):
+            if lk in results and zk in results:
+                learned[f"iters_saved_{lk}_vs_{zk}"] = 1.0 - results[lk]["iters"] / results[zk]["iters"]
+        out["learned"] = learned
         model.close()
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -61,30 +61,21 @@ __global__ __launch_bounds__(kThreads) void k_batch_norms(int n, BatchDev bd,
                                                           const double *__restrict__ ax,
                                                           double *__restrict__ out,
                                                           unsigned int *__restrict__ counters,
-                                                          double *__restrict__ partials) {
+                                                          RedBuf rb) {
     const Blk k = block_info<RB, true>(bd, n);
     __shared__ double red[8];
     __shared__ int flag;
-    double a = 0.0, t = 0.0;
+    double v[2] = {0.0, 0.0};
     for (int j = threadIdx.x; j < k.nrows; j += kThreads) {
         const int i = k.r0 + j;
         const double d = bvec[i] - ax[i];
-        a += d * d;
-        t += bvec[i] * bvec[i];
+        v[0] += d * d;
+        v[1] += bvec[i] * bvec[i];
     }
-    block_sum2(a, t, red);
-    if (threadIdx.x == 0) {
-        partials[2 * k.b] = a;
-        partials[2 * k.b + 1] = t;
-    }
-    if (arrive_last(&counters[k.sys], k.count, &flag)) {
-        double s = reduce_partials(partials + 2 * k.first, k.count, 2, red);
-        double u = reduce_partials(partials + 2 * k.first + 1, k.count, 2, red);
-        if (threadIdx.x == 0) {
-            out[2 * k.sys] = s;
-            out[2 * k.sys + 1] = u;
-            counters[k.sys] = 0;
-        }
+    double tot[2];
+    if (reduce_sys<2>(v, k, rb, &counters[k.sys], red, &flag, tot) && threadIdx.x == 0) {
+        out[2 * k.sys] = tot[0];
+        out[2 * k.sys + 1] = tot[1];
     }
 }
 
@@ -199,7 +190,7 @@ void final_norms(cicy_batch *B) {
     launch_apply<double, RB, false, false, true>(m64(B), B->nblocks, st, bd, B->x, nullptr,
                                                  nullptr, B->q, nullptr, RedBuf{}, nullptr);
     hipLaunchKernelGGL((k_batch_norms<RB>), dim3(B->nblocks), dim3(kThreads), 0, st, B->n, bd,
-                       B->b, B->q, B->norms, B->ncount, B->partials);
+                       B->b, B->q, B->norms, B->ncount, rbuf(B));
 }
 
 bool capture(cicy_batch *B, int method, int buf, int grid) {
@@ -437,6 +428,10 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
          hip_ok(ctx, hipMemsetAsync(B->xs, 0, N * sizeof(float), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(B->ncount, 0, nsys * sizeof(unsigned), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(B->gcount, 0, NB * sizeof(unsigned), ctx->stream), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(B->partials, 0, 2 * NB * sizeof(double), ctx->stream),
+                "memset") &&
+         hip_ok(ctx, hipMemsetAsync(B->gpart, 0, 2 * NB * sizeof(double), ctx->stream),
+                "memset") &&
          hip_ok(ctx, hipMemsetAsync(B->mix, 0, nsys * sizeof(MixState), ctx->stream), "memset") &&
          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     if (!ok) {

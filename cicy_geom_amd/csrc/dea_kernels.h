@@ -159,38 +159,6 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *lds8) {
     }
 }
 
-// Publishes this work-group's partial(s) and returns true in every thread of the last of
-// `expected` work-groups to arrive, after which all their partials are visible to it.
-__device__ __forceinline__ bool arrive_last(unsigned int *counter, unsigned int expected,
-                                            int *lds_flag) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned int v = atomicAdd(counter, 1u);
-        *lds_flag = (v == expected - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    bool last = *lds_flag != 0;
-    if (last) __threadfence();
-    return last;
-}
-
-// Fixed-order sum of nb partials by one work-group (deterministic); valid in thread 0.
-__device__ __forceinline__ double reduce_partials(const double *part, int nb, int stride,
-                                                  double *lds4) {
-    // four independent accumulators keep several loads in flight
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int i = threadIdx.x;
-    for (; i + 3 * kThreads < nb; i += 4 * kThreads) {
-        s0 += part[(size_t)i * stride];
-        s1 += part[(size_t)(i + kThreads) * stride];
-        s2 += part[(size_t)(i + 2 * kThreads) * stride];
-        s3 += part[(size_t)(i + 3 * kThreads) * stride];
-    }
-    for (; i < nb; i += kThreads) s0 += part[(size_t)i * stride];
-    return block_sum((s0 + s1) + (s2 + s3), lds4);
-}
-
 // Reduction scratch: accumulators and arrival counters at the slot of each group's first
 // work-group (groups of kGroup work-groups), and the system's total at its first slot.
 constexpr int kGroup = 64;

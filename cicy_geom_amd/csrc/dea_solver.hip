@@ -22,33 +22,32 @@ using namespace cicy;
 // kernels
 // ------------------------------------------------------------------------------------
 
-// out = sum over rows of (a[i] - b[i])^2 and b^2 (true-residual norms); single pass.
+// out = sum over rows of (a[i] - b[i])^2 and b^2 (true-residual norms); single pass,
+// grid-stride, reduced with the same fence-free accumulation as the iteration kernels.
 __global__ __launch_bounds__(kThreads) void k_resid_norms(int n, const double *__restrict__ bvec,
                                                           const double *__restrict__ ax,
                                                           double *__restrict__ out2,
                                                           unsigned int *__restrict__ counter,
-                                                          double *__restrict__ partials) {
+                                                          RedBuf rbuf) {
     __shared__ double red[8];
     __shared__ int flag;
-    double a = 0.0, bb = 0.0;
+    double v[2] = {0.0, 0.0};
     for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
         const double d = bvec[i] - ax[i];
-        a += d * d;
-        bb += bvec[i] * bvec[i];
+        v[0] += d * d;
+        v[1] += bvec[i] * bvec[i];
     }
-    block_sum2(a, bb, red);
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = a;
-        partials[2 * blockIdx.x + 1] = bb;
-    }
-    if (arrive_last(counter, gridDim.x, &flag)) {
-        double s = reduce_partials(partials, gridDim.x, 2, red);
-        double t = reduce_partials(partials + 1, gridDim.x, 2, red);
-        if (threadIdx.x == 0) {
-            out2[0] = s;
-            out2[1] = t;
-            *counter = 0;
-        }
+    Blk k;
+    k.b = blockIdx.x;
+    k.sys = 0;
+    k.r0 = 0;
+    k.nrows = 0;
+    k.first = 0;
+    k.count = gridDim.x;
+    double out[2];
+    if (reduce_sys<2>(v, k, rbuf, counter, red, &flag, out) && threadIdx.x == 0) {
+        out2[0] = out[0];
+        out2[1] = out[1];
     }
 }
 
@@ -409,7 +408,7 @@ bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel, double *b2)
     dispatch_rb<LaunchSpmv64>(s->rb, s, xdev, s->q);
     int nb = std::min(s->nblocks > 0 ? s->nblocks : 1, 1024);
     hipLaunchKernelGGL(k_resid_norms, dim3(nb), dim3(kThreads), 0, ctx->stream, s->n, s->b, s->q,
-                       s->scal, s->counter2, s->partials);
+                       s->scal, s->counter2, redbuf(s));
     double h[2];
     if (!hip_ok(ctx, hipMemcpyAsync(h, s->scal, 2 * sizeof(double), hipMemcpyDeviceToHost,
                                     ctx->stream), "D2H norms") ||
@@ -611,6 +610,10 @@ int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_pt
               dmalloc(ctx, &s->st, 1, &tot) && dmalloc(ctx, &s->scal, 4, &tot) &&
               dmalloc(ctx, &s->counter2, 1, &tot);
     ok = ok && h2d(ctx, s->diag, h.diag.data(), nv) &&
+         hip_ok(ctx, hipMemsetAsync(s->partials, 0, nparts * sizeof(double), ctx->stream),
+                "memset") &&
+         hip_ok(ctx, hipMemsetAsync(s->gpart, 0, nparts * sizeof(double), ctx->stream),
+                "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->x0, 0, nv * sizeof(double), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->p0, 0, nv * sizeof(double), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->r0, 0, nv * sizeof(double), ctx->stream), "memset") &&
