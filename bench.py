@@ -104,9 +104,10 @@ def main():
     ap.add_argument("--rows-per-block", type=int, default=0,
                     help="skip the autotuner and use this row-block size")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--methods", default="pcg_f64,learned_f64",
-                    help="comma list from pcg_f64, mixed, learned_f64, learned_mixed (the "
-                         "learned ones need trained weights)")
+    ap.add_argument("--methods", default="pcg_f64,condensed,condensed_resident,learned_f64",
+                    help="comma list from pcg_f64, mixed, resident, condensed, "
+                         "condensed_resident, learned_f64, learned_mixed (the learned ones "
+                         "need trained weights)")
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only run the SpMV kernels (both precisions) and a 1 GiB reference "
                          "copy, for hardware-counter runs")
@@ -146,6 +147,9 @@ def main():
         return
     catalog = {"pcg_f64": (dea.METHOD_PCG_F64, "pcg_f64", False),
                "mixed": (dea.METHOD_MIXED, "mixed", False),
+               "resident": (dea.METHOD_RESIDENT, "pcg_f64", False),
+               "condensed": (dea.METHOD_CONDENSED, "pcg_f64", False),
+               "condensed_resident": (dea.METHOD_CONDENSED_RESIDENT, "pcg_f64", False),
                "learned_f64": (dea.METHOD_PCG_F64, "pcg_f64", True),
                "learned_mixed": (dea.METHOD_MIXED, "mixed", True)}
     wanted = [m for m in args.methods.split(",") if m]
@@ -192,6 +196,8 @@ def main():
                              true_rel_res=st.true_rel_res, status=dea.STATUS_NAMES[st.status],
                              device_ms=st.solve_ms,
                              layout=autotune.LAYOUTS.get(sysd.size("format")),
+                             condensed_rows=sysd.size("condensed_rows"),
+                             resident_rows_per_thread=sysd.size("resident_rows"),
                              rows_per_block=tuned[tkey]["rows_per_block"])
         log(f"[bench] {name}: {results[name]}")
         if st.status != dea.CICY_OK:
@@ -214,7 +220,8 @@ def main():
             f"{spmv[prec]['gbs']:.0f} GB/s")
 
     head = "pcg_f64"
-    for cand in ("learned_f64", "mixed", "learned_mixed"):
+    for cand in ("learned_f64", "mixed", "learned_mixed", "resident", "condensed",
+                 "condensed_resident"):
         if cand in results and results[cand]["status"] == "ok" and \
                 results[cand]["solves_per_s"] > results[head]["solves_per_s"]:
             head = cand

@@ -66,10 +66,14 @@ inline void launch_apply(const DevMat<T> &m, int grid, hipStream_t st, const Bat
 
 }  // namespace cicy
 
+struct cicy_res;   // resident-CG layout (dea_resident.hip)
+struct cicy_cond;  // condensed system (dea_condensed.hip)
+
 struct cicy_ctx {
     int device = 0;
     int rows_per_block = 256;
     int format = 0;  // 0 auto, kFormatSss, kFormatBand
+    int resident_rows = 0;  // rows per thread of the resident CG layout (0: smallest that fits)
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
@@ -106,6 +110,16 @@ struct cicy_sys {
     float *rs = nullptr, *xs = nullptr, *ps0 = nullptr, *ps1 = nullptr, *qs = nullptr;
     cicy::MixState *mix = nullptr;
     hipGraphExec_t graph_exec_mixed = nullptr;
+    // register-resident FP64 CG (band layout only); res_why says why it is absent
+    cicy_res *res = nullptr;
+    const char *res_why = nullptr;
+    // condensed solve (chain-interior rows eliminated); cond_why says why it is absent
+    cicy_cond *cond = nullptr;
+    const char *cond_why = nullptr;
+    // squared reference norm of the stopping test when >= 0 (set by the condensed solver
+    // for its reduced system: the full system's ||b|| or ||r0||)
+    double ref2_override = -1.0;
+    double r0norm2 = 0.0;  // ||r0||^2 of the last cicy_sys_set_rhs
 };
 
 namespace cicy {
@@ -120,6 +134,7 @@ bool run_chunks(cicy_sys *s, hipGraphExec_t exec);
 // for (the stream is non-blocking, so a plain hipMemcpy would not wait for it)
 bool d2h_sync(cicy_ctx *ctx, void *dst, const void *src, size_t bytes);
 bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel, double *b2 = nullptr);
+void sys_spmv64(cicy_sys *s, const double *xdev, double *ydev);  // y = A x on the stream
 bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
                  cicy_solve_stats *stats, bool keep_x = false);
 
@@ -140,5 +155,26 @@ int choose_format(const cicy_ctx *ctx, const BlockedHost &h);
 DevMat<double> mat64(const cicy_sys *s);
 inline RedBuf redbuf(const cicy_sys *s) { return RedBuf{s->partials, s->gpart, s->gcount}; }
 DevMat<float> mat32(const cicy_sys *s);
+
+// dea_resident.hip
+cicy_res *resident_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &bh, int R_force,
+                         const char **why, bool *failed);
+void resident_free(cicy_res *z);
+bool solve_resident(cicy_sys *s, double tol, long long max_iter, int rel_to,
+                    cicy_solve_stats *stats);
+int resident_info(const cicy_res *z, int what);
+
+// dea_condensed.hip
+cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &bh,
+                           const char **why, bool *failed);
+void condensed_free(cicy_cond *c);
+bool solve_condensed(cicy_sys *s, int inner_method, double tol, long long max_iter, int rel_to,
+                     cicy_solve_stats *stats);
+int condensed_info(const cicy_cond *c, int what);
+// uploads a system (dea_solver.hip); `extras` also builds the resident and condensed layouts
+constexpr int kExtraResident = 1, kExtraCondensed = 2;
+int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+               const int64_t *col_ind, const double *values, const double *diagonal,
+               int extras, cicy_sys **out);
 
 }  // namespace cicy

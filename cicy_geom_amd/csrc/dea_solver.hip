@@ -399,6 +399,10 @@ bool run_chunks(cicy_sys *s, hipGraphExec_t exec) {
     return hip_ok(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
+void sys_spmv64(cicy_sys *s, const double *xdev, double *ydev) {
+    dispatch_rb<LaunchSpmv64>(s->rb, s, xdev, ydev);
+}
+
 bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel, double *b2) {
     cicy_ctx *ctx = s->ctx;
     if (!s->has_b) {
@@ -433,7 +437,7 @@ bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
         }
         return true;
     }
-    if (rel_to == CICY_REL_B && !s->has_b) {
+    if (rel_to == CICY_REL_B && !s->has_b && s->ref2_override < 0.0) {
         set_error(ctx, "CICY_REL_B needs b (cicy_sys_set_rhs)");
         return false;
     }
@@ -441,7 +445,9 @@ bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
     size_t vb = (size_t)s->n * sizeof(double);
     hipStream_t st = ctx->stream;
     if (!keep_x && !hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "hipEventRecord")) return false;
-    const double stop_ref2 = rel_to == CICY_REL_B ? s->bnorm2 : -1.0;
+    const double stop_ref2 = s->ref2_override >= 0.0   ? s->ref2_override
+                             : rel_to == CICY_REL_B ? s->bnorm2
+                                                    : -1.0;
     if (!keep_x &&
         (!hip_ok(ctx, hipMemcpyAsync(s->r, s->r0, vb, hipMemcpyDeviceToDevice, st), "copy r0") ||
          !hip_ok(ctx, hipMemcpyAsync(s->x, s->x0, vb, hipMemcpyDeviceToDevice, st), "copy x0")))
@@ -555,9 +561,16 @@ int cicy_set_format(cicy_ctx *ctx, int format) {
     return 1;
 }
 
-int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
-                    const int64_t *col_ind, const double *values, const double *diagonal,
-                    cicy_sys **out) {
+}  // extern "C"
+
+namespace cicy {
+
+// Uploads a system.  extras & kExtraResident also builds the register-resident CG layout
+// (band format systems that fit the resident grid); extras & kExtraCondensed also builds
+// the condensed system (chain-interior rows eliminated) when it is worth it.
+int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+               const int64_t *col_ind, const double *values, const double *diagonal,
+               int extras, cicy_sys **out) {
     if (!ctx || !out) return 0;
     *out = nullptr;
     if (hipSetDevice(ctx->device) != hipSuccess) return 0;
@@ -632,8 +645,42 @@ int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_pt
         cicy_sys_destroy(s);
         return 0;
     }
+    if (!(extras & kExtraResident)) {
+        s->res_why = "not requested";
+    } else if (s->format != kFormatBand) {
+        s->res_why = "system not in band + remainder format";
+    } else {
+        bool failed = false;
+        s->res = resident_setup(ctx, h, bh, ctx->resident_rows, &s->res_why, &failed);
+        if (failed) {
+            cicy_sys_destroy(s);
+            return 0;
+        }
+    }
+    if (!(extras & kExtraCondensed)) {
+        s->cond_why = "not requested";
+    } else {
+        if (s->format != kFormatBand) build_band(h, bh);
+        bool failed = false;
+        s->cond = condensed_setup(ctx, h, bh, &s->cond_why, &failed);
+        if (failed) {
+            cicy_sys_destroy(s);
+            return 0;
+        }
+    }
     *out = s;
     return 1;
+}
+
+}  // namespace cicy
+
+extern "C" {
+
+int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                    const int64_t *col_ind, const double *values, const double *diagonal,
+                    cicy_sys **out) {
+    return cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal,
+                            cicy::kExtraResident | cicy::kExtraCondensed, out);
 }
 
 void cicy_sys_destroy(cicy_sys *s) {
@@ -642,6 +689,8 @@ void cicy_sys_destroy(cicy_sys *s) {
     (void)hipStreamSynchronize(s->ctx->stream);
     if (s->graph_exec64) (void)hipGraphExecDestroy(s->graph_exec64);
     cicy::mixed_free(s);
+    cicy::resident_free(s->res);
+    cicy::condensed_free(s->cond);
     void *ptrs[] = {s->rp, s->col, s->val, s->diag, s->urp, s->ucol, s->uval, s->band,
                     s->xblk, s->xrow, s->xnbr, s->xval, s->x, s->r, s->p0, s->p1, s->q,
                     s->x0, s->r0, s->b, s->partials, s->gpart, s->gcount, s->st, s->scal,
@@ -660,6 +709,11 @@ int64_t cicy_sys_size(const cicy_sys *s, int what) {
         case 3: return (int64_t)s->device_bytes;
         case 4: return s->format;
         case 5: return s->format == kFormatBand ? s->nx : s->nremote;
+        case 6: return cicy::resident_info(s->res, 0);
+        case 7: return cicy::resident_info(s->res, 1);
+        case 8: return cicy::resident_info(s->res, 2);
+        case 9: return cicy::condensed_info(s->cond, 0);
+        case 10: return cicy::condensed_info(s->cond, 1);
         default: return -1;
     }
 }
@@ -672,6 +726,9 @@ int cicy_sys_set_rhs(cicy_sys *s, const double *r0, const double *x0, const doub
     bool ok = h2d(ctx, s->r0, r0, nv);
     if (x0) ok = ok && h2d(ctx, s->x0, x0, nv);
     else ok = ok && hip_ok(ctx, hipMemsetAsync(s->x0, 0, nv * sizeof(double), ctx->stream), "memset");
+    double r02 = 0.0;
+    for (size_t i = 0; i < nv; i++) r02 += r0[i] * r0[i];
+    s->r0norm2 = r02;
     s->has_b = b != nullptr;
     if (b) {
         ok = ok && h2d(ctx, s->b, b, nv);
@@ -697,6 +754,12 @@ int cicy_sys_solve(cicy_sys *s, int method, double tol, int64_t max_iter, int re
         ok = cicy::solve_pcg64(s, tol, max_iter, rel_to, stats);
     else if (method == CICY_METHOD_MIXED)
         ok = cicy::solve_mixed(s, tol, max_iter, rel_to, stats);
+    else if (method == CICY_METHOD_RESIDENT)
+        ok = cicy::solve_resident(s, tol, max_iter, rel_to, stats);
+    else if (method == CICY_METHOD_CONDENSED)
+        ok = cicy::solve_condensed(s, CICY_METHOD_PCG_F64, tol, max_iter, rel_to, stats);
+    else if (method == CICY_METHOD_CONDENSED_RESIDENT)
+        ok = cicy::solve_condensed(s, CICY_METHOD_RESIDENT, tol, max_iter, rel_to, stats);
     else {
         set_error(s->ctx, "unknown method %d", method);
         ok = false;
@@ -834,7 +897,7 @@ int cicy_solve_dea_cg_sss(cicy_ctx *ctx, int64_t nodes, int64_t reserved_nodes, 
     local.true_rel_res = -1.0;
     if (n > 0) {
         cicy_sys *s = nullptr;
-        if (!cicy_sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, &s)) {
+        if (!cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, 0, &s)) {
             if (stats) stats->status = CICY_BAD_INPUT;
             return 0;
         }

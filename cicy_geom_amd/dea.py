@@ -14,7 +14,10 @@ import numpy as np
 from ._libs import dea_lib
 
 CICY_OK, CICY_MAXITER, CICY_BREAKDOWN, CICY_BAD_INPUT, CICY_DEVICE_ERROR = 0, 1, 2, 3, 4
-METHOD_PCG_F64, METHOD_MIXED = 0, 1
+METHOD_PCG_F64, METHOD_MIXED, METHOD_RESIDENT = 0, 1, 2
+METHOD_CONDENSED, METHOD_CONDENSED_RESIDENT = 3, 4
+METHOD_NAMES = {0: "pcg_f64", 1: "mixed", 2: "resident", 3: "condensed",
+                4: "condensed_resident"}
 FORMAT_AUTO, FORMAT_SSS, FORMAT_BAND = 0, 1, 2
 REL_R0, REL_B = 0, 1
 STATUS_NAMES = {0: "ok", 1: "maxiter", 2: "breakdown", 3: "bad_input", 4: "device_error"}
@@ -35,7 +38,8 @@ EXPORTED_SYMBOLS = [
     "cicy_mpnn_destroy", "cicy_mpnn_set_params", "cicy_mpnn_get_params", "cicy_mpnn_get_grads",
     "cicy_mpnn_zero_grads", "cicy_graph_create", "cicy_graph_destroy", "cicy_graph_set_target",
     "cicy_mpnn_forward", "cicy_mpnn_backward", "cicy_mpnn_adam_step", "cicy_mpnn_warm_start",
-    "cicy_host_blocked_spmv",
+    "cicy_host_blocked_spmv", "cicy_set_resident_rows", "cicy_host_resident_apply",
+    "cicy_host_condense", "cicy_host_expand",
 ]
 
 
@@ -65,6 +69,16 @@ def lib():
     L.cicy_last_error.argtypes = [_VP]
     L.cicy_set_rows_per_block.argtypes = [_VP, ctypes.c_int]
     L.cicy_set_format.argtypes = [_VP, ctypes.c_int]
+    L.cicy_set_resident_rows.argtypes = [_VP, ctypes.c_int]
+    L.cicy_host_resident_apply.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
+                                           ctypes.c_int, ctypes.c_int, _F64, _F64,
+                                           ctypes.c_double, _F64]
+    L.cicy_host_condense.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
+                                     _VP, ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64), _I64, _I64, _I64, _F64,
+                                     _F64, _VP]
+    L.cicy_host_expand.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64, _F64,
+                                   _F64, _F64]
     L.cicy_copy_bench.argtypes = [_VP, ctypes.c_int64, ctypes.c_int, P(ctypes.c_double)]
     L.cicy_host_layout_spmv.argtypes = [ctypes.c_int, ctypes.c_int64, _I64, _I64, _F64, _F64,
                                         ctypes.c_int64, ctypes.c_int, _F64, _F64]
@@ -168,6 +182,11 @@ class Context:
     def set_format(self, fmt: int):
         self._check(self._L.cicy_set_format(self._h, fmt), "cicy_set_format")
 
+    def set_resident_rows(self, rows_per_thread: int):
+        """Rows per thread of the resident layout of systems created afterwards (0: auto)."""
+        self._check(self._L.cicy_set_resident_rows(self._h, rows_per_thread),
+                    "cicy_set_resident_rows")
+
     def close(self):
         if getattr(self, "_h", None):
             for child in list(getattr(self, "_children", ())):
@@ -268,7 +287,8 @@ class System:
     def size(self, what: str) -> int:
         return int(self.ctx._L.cicy_sys_size(
             self._h, {"n": 0, "nnz": 1, "remote": 2, "bytes": 3, "format": 4,
-                      "off_layout": 5}[what]))
+                      "off_layout": 5, "resident_rows": 6, "resident_groups": 7,
+                      "published": 8, "condensed_rows": 9, "condensed_nnz": 10}[what]))
 
     def set_rhs(self, r0, x0=None, b=None):
         r0 = _f64(r0)
@@ -363,3 +383,49 @@ def host_blocked_spmv(row_ptr, col_ind, values, diagonal, v, rows_per_block=256,
     if not ok:
         raise DEAError("cicy_host_blocked_spmv rejected the input")
     return out[:n]
+
+
+def host_resident_apply(row_ptr, col_ind, values, diagonal, r, pold, beta, rows_per_thread,
+                        phase):
+    """A^ (r + beta pold), A^ = D^-1/2 A D^-1/2, through the resident layout on the host."""
+    n = len(diagonal)
+    out = np.zeros(max(n, 1))
+    ok = lib().cicy_host_resident_apply(n, _i64(row_ptr), _i64(col_ind), _f64(values),
+                                        _f64(diagonal), len(values), rows_per_thread, phase,
+                                        _f64(r), _f64(pold), float(beta), out)
+    if not ok:
+        raise DEAError("cicy_host_resident_apply rejected the input")
+    return out[:n]
+
+
+def host_condense(row_ptr, col_ind, values, diagonal, rhs=None):
+    """The condensation on the host: (anchors, red_row_ptr, red_col_ind, red_values,
+    red_diagonal, red_rhs or None)."""
+    n, nnz = len(diagonal), len(values)
+    anchors = np.zeros(max(n, 1), np.int64)
+    rrp = np.zeros(n + 1, np.int64)
+    rcol = np.zeros(max(nnz + n, 1), np.int64)
+    rval = np.zeros(max(nnz + n, 1))
+    rdiag = np.zeros(max(n, 1))
+    rrhs = np.zeros(max(n, 1)) if rhs is not None else None
+    m = ctypes.c_int64()
+    k = ctypes.c_int64()
+    ok = lib().cicy_host_condense(n, _i64(row_ptr), _i64(col_ind), _f64(values), _f64(diagonal),
+                                  nnz, _ptr(None if rhs is None else _f64(rhs)), ctypes.byref(m),
+                                  ctypes.byref(k), anchors, rrp, rcol, rval, rdiag, _ptr(rrhs))
+    if not ok:
+        raise DEAError("cicy_host_condense: nothing to condense or invalid input")
+    m, k = m.value, k.value
+    return (anchors[:m], rrp[:m + 1], rcol[:k], rval[:k], rdiag[:m],
+            None if rrhs is None else rrhs[:m])
+
+
+def host_expand(row_ptr, col_ind, values, diagonal, rhs, x_anchor):
+    """Back-substitution of the condensation on the host."""
+    n = len(diagonal)
+    x = np.zeros(max(n, 1))
+    ok = lib().cicy_host_expand(n, _i64(row_ptr), _i64(col_ind), _f64(values), _f64(diagonal),
+                                len(values), _f64(rhs), _f64(x_anchor), x)
+    if not ok:
+        raise DEAError("cicy_host_expand: nothing to condense or invalid input")
+    return x[:n]

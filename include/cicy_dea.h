@@ -43,6 +43,17 @@ typedef struct cicy_sys cicy_sys;
 /* solve methods (cicy_sys_solve) */
 #define CICY_METHOD_PCG_F64 0    /* plain Jacobi-PCG in FP64, the reference's algorithm */
 #define CICY_METHOD_MIXED 1      /* FP32 Jacobi-PCG with FP64 reliable updates/refinement */
+#define CICY_METHOD_RESIDENT 2   /* the reference's FP64 Jacobi-PCG run as register-resident
+                                    cooperative launches (band + remainder systems that fit
+                                    the device's registers; cicy_sys_size 6 > 0) */
+#define CICY_METHOD_CONDENSED 3  /* exact static condensation: chain-interior rows (two band
+                                    neighbours, no junction entries) are eliminated by
+                                    tridiagonal sweeps, the reference's FP64 Jacobi-PCG runs
+                                    on the Schur complement of the junction rows, the
+                                    eliminated rows are recovered by back-substitution
+                                    (cicy_sys_size 9 > 0) */
+#define CICY_METHOD_CONDENSED_RESIDENT 4  /* the same with the resident CG on the reduced
+                                             system */
 
 /* stopping rule (cicy_sys_solve rel_to) */
 #define CICY_REL_R0 0  /* ||r_k|| <= tol*||r_0||, r_0 = b - A x0: the reference (:1571,:1619) */
@@ -70,6 +81,10 @@ int cicy_set_rows_per_block(cicy_ctx *ctx, int rows_per_block);
  * of the stored entries sit on the sub-diagonal, as for tube-by-tube numbering), 1 blocked
  * SSS with a remote transpose list, 2 band + remainder */
 int cicy_set_format(cicy_ctx *ctx, int format);
+/* rows per thread of the register-resident layout built for systems created afterwards
+ * (0: the smallest of 4 8 16 24 32 40 44 48 whose grid is co-resident).  No counterpart in
+ * the reference: a layout knob like cicy_set_rows_per_block. */
+int cicy_set_resident_rows(cicy_ctx *ctx, int rows_per_thread);
 
 /* ---- drop-in seam: Solve_DEA_equations_CG_SSS (Direct_Electrifying.cpp:1538-1659) -----
  * nodes          global node count (reserved + free), as the reference passes it
@@ -115,7 +130,12 @@ int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_pt
 void cicy_sys_destroy(cicy_sys *sys);
 int64_t cicy_sys_size(const cicy_sys *sys, int what); /* 0 n, 1 nnz, 2 remote entries,
                                                            3 device bytes, 4 format,
-                                                           5 off-layout entries */
+                                                           5 off-layout entries,
+                                                           6 resident rows per thread (0:
+                                                           no resident layout), 7 resident
+                                                           work-groups, 8 published rows,
+                                                           9 condensed rows (0: none), 10
+                                                           condensed stored entries */
 /* r0 = b - A x0 (the reference's R), x0 (may be NULL = zeros), b (may be NULL; needed for
  * the true residual and for CICY_REL_B). */
 int cicy_sys_set_rhs(cicy_sys *sys, const double *r0, const double *x0, const double *b);
@@ -195,6 +215,27 @@ int cicy_host_blocked_spmv(int64_t n, const int64_t *row_ptr, const int64_t *col
 int cicy_host_layout_spmv(int format, int64_t n, const int64_t *row_ptr,
                           const int64_t *col_ind, const double *values, const double *diagonal,
                           int64_t nnz, int rows_per_block, const double *v, double *out);
+/* One product of the resident layout on the host: out = A^ (r + beta pold), A^ = D^-1/2 A
+ * D^-1/2, walking lanes, waves and published rows as the resident kernel does (phase 0:
+ * remote p recomputed from the published r and p_old; phase 1: published p). */
+int cicy_host_resident_apply(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                             const double *values, const double *diagonal, int64_t nnz,
+                             int rows_per_thread, int phase, const double *r, const double *pold,
+                             double beta, double *out);
+/* The condensation of CICY_METHOD_CONDENSED on the host: anchors (rows with junction
+ * entries, [m]), the reduced SSS matrix S (red_row_ptr [m+1], red_col_ind/red_values
+ * [*nnz_out], red_diagonal [m]) and, when rhs is given, the reduced right-hand side [m].
+ * Buffers are sized by the caller for the worst case (m <= n, nnz_out <= nnz + n).
+ * Returns 0 when there is nothing worth condensing.  For layout tests without a GPU. */
+int cicy_host_condense(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                       const double *values, const double *diagonal, int64_t nnz,
+                       const double *rhs, int64_t *m_out, int64_t *nnz_out, int64_t *anchors,
+                       int64_t *red_row_ptr, int64_t *red_col_ind, double *red_values,
+                       double *red_diagonal, double *red_rhs);
+/* Back-substitution of the condensation on the host: x [n] from the anchor values [m]. */
+int cicy_host_expand(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                     const double *values, const double *diagonal, int64_t nnz,
+                     const double *rhs, const double *x_anchor, double *x);
 
 #ifdef __cplusplus
 }

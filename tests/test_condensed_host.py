@@ -1,0 +1,164 @@
+"""Host forms of the condensed solve and of the resident layout, checked against dense
+linear algebra and closed forms.  No GPU calls."""
+import numpy as np
+import pytest
+
+import oracle
+from cicy_geom_amd import dea, network
+
+
+def dense(rp, col, val, diag):
+    n = len(diag)
+    A = np.diag(np.asarray(diag, float))
+    for r in range(n):
+        for k in range(rp[r], rp[r + 1]):
+            A[r, col[k]] += val[k]
+            A[col[k], r] += val[k]
+    return A
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return network.largest_percolating_system(network.generate(1), 0)
+
+
+def test_reduced_matrix_is_the_schur_complement(c1):
+    s = c1
+    anchors, rrp, rcol, rval, rdiag, rrhs = dea.host_condense(s.row_ptr, s.col_ind, s.values,
+                                                              s.diag, s.b)
+    A = dense(s.row_ptr, s.col_ind, s.values, s.diag)
+    n = s.n
+    J = anchors
+    I = np.setdiff1d(np.arange(n), J)
+    # the anchors are exactly the rows with a junction (off-sub-diagonal) entry
+    rows = np.repeat(np.arange(n), np.diff(s.row_ptr))
+    off = s.col_ind != rows - 1
+    assert set(J) == set(rows[off]) | set(s.col_ind[off])
+    AII, AIJ, AJJ = A[np.ix_(I, I)], A[np.ix_(I, J)], A[np.ix_(J, J)]
+    S = AJJ - AIJ.T @ np.linalg.solve(AII, AIJ)
+    Sred = dense(rrp, rcol, rval, rdiag)
+    assert np.abs(Sred - S).max() <= 1e-10 * np.abs(S).max()
+    bred = s.b[J] - AIJ.T @ np.linalg.solve(AII, s.b[I])
+    assert np.allclose(rrhs, bred, rtol=1e-10, atol=1e-12 * np.abs(bred).max())
+    # reduced matrix stays strictly lower SSS with the anchors in order
+    assert np.all(np.diff(J) > 0)
+    for r in range(len(J)):
+        assert np.all(rcol[rrp[r]:rrp[r + 1]] < r)
+
+
+def test_expand_recovers_the_full_solution(c1):
+    s = c1
+    anchors, rrp, rcol, rval, rdiag, rrhs = dea.host_condense(s.row_ptr, s.col_ind, s.values,
+                                                              s.diag, s.b)
+    xj = np.linalg.solve(dense(rrp, rcol, rval, rdiag), rrhs)
+    x = dea.host_expand(s.row_ptr, s.col_ind, s.values, s.diag, s.b, xj)
+    want = np.linalg.solve(dense(s.row_ptr, s.col_ind, s.values, s.diag), s.b)
+    assert np.linalg.norm(x - want) <= 1e-10 * np.linalg.norm(want)
+    ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
+    assert np.linalg.norm(s.b - ax) <= 1e-12 * np.linalg.norm(s.b)
+
+
+def chain_system(k, g, junction=0.5):
+    """Two anchors (rows 0 and k+1) joined by a chain of k interior rows of conductance g,
+    plus a junction of conductance `junction` between the anchors and a ground leak on
+    each anchor so the system is definite."""
+    n = k + 2
+    rp, col, val = [0], [], []
+    for r in range(n):
+        if r == k + 1:
+            col.append(0)
+            val.append(-junction)
+        if r >= 1:
+            col.append(r - 1)
+            val.append(-g)
+        rp.append(len(col))
+    # sort each row's columns
+    rp = np.array(rp, np.int64)
+    col = np.array(col, np.int64)
+    val = np.array(val)
+    for r in range(n):
+        o = np.argsort(col[rp[r]:rp[r + 1]])
+        col[rp[r]:rp[r + 1]] = col[rp[r]:rp[r + 1]][o]
+        val[rp[r]:rp[r + 1]] = val[rp[r]:rp[r + 1]][o]
+    diag = np.full(n, 2 * g)
+    diag[0] = diag[-1] = g + junction + 1.0
+    return rp, col, val, diag
+
+
+@pytest.mark.parametrize("k", [2, 3, 7, 40])
+def test_series_law(k):
+    """k equal conductances g in series between two anchors act as one of g/(k+1)."""
+    g = 3.0
+    rp, col, val, diag = chain_system(k, g)
+    anchors, rrp, rcol, rval, rdiag, _ = dea.host_condense(rp, col, val, diag)
+    assert list(anchors) == [0, k + 1]
+    geff = g / (k + 1)
+    assert rval[0] == pytest.approx(-0.5 - geff, rel=1e-13)   # junction + series path
+    # each anchor keeps its leak and junction, and sees the path as geff to the other
+    assert rdiag == pytest.approx([1.0 + 0.5 + geff] * 2, rel=1e-13)
+
+
+def test_nothing_to_condense_is_refused():
+    rng = np.random.default_rng(3)
+    n = 200
+    pairs = {}
+    for a, b in rng.integers(0, n, (800, 2)):
+        if abs(a - b) > 1:
+            pairs[(max(a, b), min(a, b))] = -1.0
+    keys = sorted(pairs)
+    rp = np.zeros(n + 1, np.int64)
+    for r, _ in keys:
+        rp[r + 1] += 1
+    rp = np.cumsum(rp)
+    col = np.array([c for _, c in keys], np.int64)
+    with pytest.raises(dea.DEAError):
+        dea.host_condense(rp, col, -np.ones(len(col)), np.full(n, 10.0))
+
+
+def band_system(n, seed):
+    """Tube-like system: chains along the sub-diagonal with breaks, random junctions."""
+    rng = np.random.default_rng(seed)
+    lower = {}
+    for r in range(1, n):
+        if rng.random() > 0.05:
+            lower[(r, r - 1)] = -rng.uniform(0.5, 2.0)
+    for _ in range(n // 12):
+        a, b = rng.integers(0, n, 2)
+        if abs(a - b) > 1:
+            lower[(max(a, b), min(a, b))] = -10.0 ** rng.uniform(-6, 0)
+    keys = sorted(lower)
+    rp = np.zeros(n + 1, np.int64)
+    for r, _ in keys:
+        rp[r + 1] += 1
+    rp = np.cumsum(rp)
+    col = np.array([c for _, c in keys], np.int64)
+    val = np.array([lower[k] for k in keys])
+    diag = np.full(n, 0.1)
+    for (r, c), v in lower.items():
+        diag[r] -= v
+        diag[c] -= v
+    return rp, col, val, diag
+
+
+@pytest.mark.parametrize("R", [4, 8])
+@pytest.mark.parametrize("phase", [0, 1])
+def test_resident_layout_on_host(R, phase):
+    """The resident kernel's walk (lanes, wave edges, published rows of neighbouring
+    work-groups, remainder slots) gives A^ (r + beta p_old) over several work-groups."""
+    n = 512 * R * 3 + 777           # four work-groups, the last one ragged
+    rp, col, val, diag = band_system(n, R + phase)
+    rng = np.random.default_rng(7)
+    r = rng.standard_normal(n)
+    pold = rng.standard_normal(n)
+    beta = 0.37
+    got = dea.host_resident_apply(rp, col, val, diag, r, pold, beta, R, phase)
+    sc = 1.0 / np.sqrt(diag)
+    p = r + beta * pold
+    want = sc * oracle.spmv_sss(rp, col, val, diag, sc * p)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+
+
+def test_resident_layout_rejects_unknown_rows_per_thread(c1):
+    s = c1
+    with pytest.raises(dea.DEAError):
+        dea.host_resident_apply(s.row_ptr, s.col_ind, s.values, s.diag, s.b, s.b, 0.0, 3, 0)

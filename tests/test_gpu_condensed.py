@@ -1,0 +1,168 @@
+"""GPU parity of the register-resident CG and of the condensed solve (through the C-ABI)
+against the CPU oracle.
+
+Bars (the tolerance `north_star` sets for the voltages): ||b - A x|| <= 1e-10 ||b|| checked
+with the oracle's SSS product, and the voltages within 1e-7 relative of the FP64 truth
+(the truth itself is a CG iterate at 1e-13; these systems span seven decades of
+conductance, so a 1e-10 residual leaves ~1e-8 of error).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cicy_geom_amd import dea, network
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dea.Context(0)
+    yield c
+    c.close()
+
+
+def networks():
+    out = {"c1": network.largest_percolating_system(network.generate(1), 0)}
+    p = network.config_params(4, 0, 6000)
+    net = network.Network(p)
+    ks = net.percolating_clusters(0)
+    c = max(ks, key=lambda q: net.cluster(q)[0])
+    out["rve6k"] = net.assemble(c, 1, 0, direction=0)
+    net.close()
+    return out
+
+
+@pytest.fixture(scope="module")
+def systems():
+    return networks()
+
+
+@pytest.fixture(scope="module")
+def truths(systems):
+    return {k: oracle.cg_truth(s.row_ptr, s.col_ind, s.values, s.diag, s.b, tol=1e-13)[0]
+            for k, s in systems.items()}
+
+
+def check(s, x, truth, tol=1e-10):
+    ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
+    tr = np.linalg.norm(s.b - ax) / np.linalg.norm(s.b)
+    err = np.linalg.norm(x - truth) / np.linalg.norm(truth)
+    assert tr <= tol * (1 + 1e-6), tr
+    assert err < 1e-7, err
+    return tr, err
+
+
+@pytest.mark.parametrize("name", ["c1", "rve6k"])
+@pytest.mark.parametrize("method", [dea.METHOD_RESIDENT, dea.METHOD_CONDENSED,
+                                    dea.METHOD_CONDENSED_RESIDENT])
+def test_solution_and_true_residual(ctx, systems, truths, name, method):
+    s = systems[name]
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    try:
+        sysd.set_rhs(s.b, None, s.b)
+        st = sysd.solve(method, 1e-10, 0, dea.REL_B)
+        x, _ = sysd.get()
+        print(name, dea.METHOD_NAMES[method], st.as_dict(), sysd.size("resident_rows"),
+              sysd.size("condensed_rows"), flush=True)
+        assert st.status == dea.CICY_OK
+        tr, err = check(s, x, truths[name])
+        assert abs(st.true_rel_res - tr) <= 1e-3 * tr + 1e-16
+        if method == dea.METHOD_RESIDENT:
+            # the same CG as the two-kernel solve: iteration counts agree closely
+            ref = sysd.solve(dea.METHOD_PCG_F64, 1e-10, 0, dea.REL_B)
+            assert abs(ref.iters - st.iters) <= max(16, ref.iters // 10)
+        else:
+            assert 0 < sysd.size("condensed_rows") < s.n // 2
+    finally:
+        sysd.close()
+
+
+@pytest.mark.parametrize("rows", [4, 8])
+def test_resident_forced_rows_several_groups(ctx, systems, truths, rows):
+    """Forced rows per thread spread the 6k-tube system over many work-groups."""
+    s = systems["rve6k"]
+    ctx.set_resident_rows(rows)
+    try:
+        sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    finally:
+        ctx.set_resident_rows(0)
+    try:
+        assert sysd.size("resident_rows") == rows
+        assert sysd.size("resident_groups") == -(-s.n // (512 * rows))
+        sysd.set_rhs(s.b, None, s.b)
+        st = sysd.solve(dea.METHOD_RESIDENT, 1e-10, 0, dea.REL_B)
+        x, _ = sysd.get()
+        assert st.status == dea.CICY_OK
+        check(s, x, truths["rve6k"])
+        # a second solve on the same data repeats the first (state is reset per solve)
+        st2 = sysd.solve(dea.METHOD_RESIDENT, 1e-10, 0, dea.REL_B)
+        x2, _ = sysd.get()
+        assert st2.status == dea.CICY_OK
+        check(s, x2, truths["rve6k"])
+    finally:
+        sysd.close()
+
+
+def test_resident_iteration_cap_and_chunks(ctx, systems):
+    """max_iter beyond one cooperative launch (2048 iterations) and below convergence."""
+    s = systems["rve6k"]
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    try:
+        sysd.set_rhs(s.b, None, s.b)
+        st = sysd.solve(dea.METHOD_RESIDENT, 1e-30, 3000, dea.REL_B)
+        assert st.status == dea.CICY_MAXITER and st.iters == 3000
+    finally:
+        sysd.close()
+
+
+@pytest.mark.parametrize("method", [dea.METHOD_CONDENSED, dea.METHOD_CONDENSED_RESIDENT])
+def test_condensed_reference_rule_with_ramp_guess(ctx, systems, method):
+    """CICY_REL_R0 from the reference's ramp guess: ||b - A x|| <= tol ||r0||."""
+    s = systems["rve6k"]
+    assert s.has_guess
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    try:
+        sysd.set_rhs(s.r0, s.x0, s.b)
+        st = sysd.solve(method, 1e-8, 0, dea.REL_R0)
+        x, _ = sysd.get()
+        assert st.status == dea.CICY_OK
+        ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
+        assert np.linalg.norm(s.b - ax) <= 1e-8 * np.linalg.norm(s.r0) * (1 + 1e-6)
+    finally:
+        sysd.close()
+
+
+def test_condensed_zero_rhs_and_random_laplacian(ctx):
+    """Zero right-hand side returns zero at once; an unstructured matrix has no condensed
+    system and the method says so."""
+    s = network.largest_percolating_system(network.generate(1), 0)
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    try:
+        z = np.zeros(s.n)
+        sysd.set_rhs(z, None, None)
+        st = sysd.solve(dea.METHOD_CONDENSED, 1e-10, 0, dea.REL_R0)
+        x, _ = sysd.get()
+        assert st.status == dea.CICY_OK and not np.any(x)
+    finally:
+        sysd.close()
+    rng = np.random.default_rng(5)
+    n = 400
+    pairs = {}
+    for a, b in rng.integers(0, n, (2000, 2)):
+        if abs(a - b) > 1:
+            pairs[(max(a, b), min(a, b))] = -1.0
+    keys = sorted(pairs)
+    rp = np.zeros(n + 1, np.int64)
+    for r, _ in keys:
+        rp[r + 1] += 1
+    rp = np.cumsum(rp)
+    col = np.array([c for _, c in keys], np.int64)
+    sysd = ctx.system(rp, col, -np.ones(len(col)), np.full(n, 30.0))
+    try:
+        assert sysd.size("condensed_rows") == 0
+        sysd.set_rhs(np.ones(n), None, np.ones(n))
+        with pytest.raises(dea.DEAError):
+            sysd.solve(dea.METHOD_CONDENSED, 1e-10, 0, dea.REL_B)
+    finally:
+        sysd.close()
