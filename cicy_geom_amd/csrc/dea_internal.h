@@ -74,6 +74,7 @@ struct cicy_ctx {
     int rows_per_block = 256;
     int format = 0;  // 0 auto, kFormatSss, kFormatBand
     int resident_rows = 0;  // rows per thread of the resident CG layout (0: smallest that fits)
+    bool no_graphs = false;  // CICY_NO_GRAPHS set at creation: launch chunks directly
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
@@ -129,7 +130,9 @@ bool hip_ok(cicy_ctx *ctx, hipError_t e, const char *what);
 bool build_blocked(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
                    const int64_t *col_ind, const double *values, const double *diag, int rb,
                    BlockedHost &h);
-bool run_chunks(cicy_sys *s, hipGraphExec_t exec);
+// `direct` launches one chunk without a graph; used instead of the graph when the context
+// was created with CICY_NO_GRAPHS set (profilers that do not follow graph launches)
+bool run_chunks(cicy_sys *s, hipGraphExec_t exec, void (*direct)(cicy_sys *) = nullptr);
 // device -> host copy ordered after all work queued on the context's stream, then waited
 // for (the stream is non-blocking, so a plain hipMemcpy would not wait for it)
 bool d2h_sync(cicy_ctx *ctx, void *dst, const void *src, size_t bytes);

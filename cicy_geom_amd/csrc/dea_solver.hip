@@ -343,8 +343,13 @@ struct LaunchIter64 {
     }
 };
 
+// one chunk of kChunk iterations launched directly (no graph)
+void chunk64_direct(cicy_sys *s) {
+    for (int it = 0; it < kChunk; it++) dispatch_rb<LaunchIter64>(s->rb, s, it & 1);
+}
+
 bool ensure_graph64(cicy_sys *s) {
-    if (s->graph_exec64) return true;
+    if (s->graph_exec64 || s->ctx->no_graphs) return true;
     cicy_ctx *ctx = s->ctx;
     hipGraph_t g;
     if (!hip_ok(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
@@ -363,14 +368,20 @@ namespace cicy {
 
 // Runs chunks of kChunk iterations until the device flag says done.  The host polls the
 // state copied after each chunk while the next chunk is already queued.
-bool run_chunks(cicy_sys *s, hipGraphExec_t exec) {
+bool run_chunks(cicy_sys *s, hipGraphExec_t exec, void (*direct)(cicy_sys *)) {
     cicy_ctx *ctx = s->ctx;
+    const bool use_graph = !(ctx->no_graphs && direct);
+    auto launch = [&]() -> bool {
+        if (use_graph) return hip_ok(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch");
+        direct(s);
+        return hip_ok(ctx, hipGetLastError(), "chunk launch");
+    };
     PcgState *hst = ctx->host_state;  // pinned, two slots
     const int depth = 2;
     hipEvent_t ev[2] = {ctx->ev_poll[0], ctx->ev_poll[1]};
     int slot = 0;
     for (int i = 0; i < depth; i++) {
-        if (!hip_ok(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch")) return false;
+        if (!launch()) return false;
         if (!hip_ok(ctx, hipMemcpyAsync(&hst[i], s->st, sizeof(PcgState), hipMemcpyDeviceToHost,
                                         ctx->stream), "poll copy"))
             return false;
@@ -389,7 +400,7 @@ bool run_chunks(cicy_sys *s, hipGraphExec_t exec) {
             set_error(ctx, "solve stopped after %.0f s (CICY_MAX_SOLVE_SECONDS)", max_s);
             break;
         }
-        if (!hip_ok(ctx, hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch")) return false;
+        if (!launch()) return false;
         if (!hip_ok(ctx, hipMemcpyAsync(&hst[slot], s->st, sizeof(PcgState),
                                         hipMemcpyDeviceToHost, ctx->stream), "poll copy"))
             return false;
@@ -461,7 +472,7 @@ bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
         if (!hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), st), "memset state"))
             return false;
         dispatch_rb<LaunchInit64>(s->rb, s, tol, stop_ref2, max_iter - total_iters);
-        if (!run_chunks(s, s->graph_exec64)) return false;
+        if (!run_chunks(s, s->graph_exec64, chunk64_direct)) return false;
         if (!d2h_sync(ctx, &h, s->st, sizeof(PcgState)))
             return false;
         if (rr_init < 0.0) rr_init = h.rr_init;
@@ -511,6 +522,7 @@ int cicy_create(cicy_ctx **out, int device) {
     cicy_ctx *ctx = new cicy_ctx();
     ctx->device = device;
     ctx->rows_per_block = 256;
+    if (const char *e = getenv("CICY_NO_GRAPHS")) ctx->no_graphs = e[0] && e[0] != '0';
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
