@@ -46,10 +46,11 @@ constexpr unsigned kResSpinLimit = 1u << 24;  // polls before a grid barrier giv
 constexpr int kResGroup = 4;
 constexpr int kResRows[] = {4, 8, 16, 24, 32, 40, 44, 48};
 
-// Grid-barrier control block; cleared by the host before every launch.
+// Grid-barrier control block; cleared by the host before every launch.  Work-group g
+// arrives on counter g % 8 (eight counters on separate 128-byte lines, so arrivals do not
+// all contend for one address).
 struct ResCtl {
-    double acc[3][2];   // rotating reduction slots (barrier m uses slot m % 3)
-    unsigned count;     // arrivals in this launch
+    unsigned cnt[8][32];
     int error;          // set by a work-group whose barrier wait timed out
 };
 
@@ -78,6 +79,7 @@ struct ResDev {
     const int *ent_nbr;          // global slot of the entry's column
     const double *ent_val;       // a^(row, col)
     ResCtl *ctl;
+    double *part;                // [2][G][2] per-work-group partial sums, by barrier parity
 };
 
 }  // namespace cicy
@@ -86,7 +88,7 @@ struct cicy_res {
     int R = 0, G = 0, S = 0, P = 0, nent = 0, max_slots = 0;
     size_t npad = 0;
     double *band = nullptr, *dvec = nullptr, *x = nullptr, *rs = nullptr, *ps = nullptr;
-    double *rpub = nullptr, *ppub = nullptr, *ent_val = nullptr;
+    double *rpub = nullptr, *ppub = nullptr, *ent_val = nullptr, *part = nullptr;
     int *slot_base = nullptr, *ent_off = nullptr, *ent_slot = nullptr, *ent_nbr = nullptr;
     unsigned long long *jbits = nullptr;
     cicy::ResCtl *ctl = nullptr;
@@ -233,50 +235,59 @@ __device__ __forceinline__ void res_block_sum(double (&v)[NV], double *lds) {
     }
 }
 
-// Grid-wide sum of NV values and barrier number m of this launch.  Every thread first
+// Grid-wide sum of NV (<= 2) values and barrier number m of this launch.  Every thread first
 // waits for its own stores (the published rows must be at the point of coherence before
-// its work-group arrives); thread 0 adds the work-group's sums into slot m % 3 with
-// returning device-scope atomics, arrives on the counter, polls it with agent-scope loads
-// until all G work-groups have arrived, and reads the totals.  Work-group 0 then clears
-// slot (m + 2) % 3, which every work-group finished reading before arriving at barrier m
-// and which is next added to at barrier m + 2.  Returns false in every thread if the wait
-// timed out (the kernel then stops); v[] holds the totals on return.
+// its work-group arrives).  Thread 0 stores the work-group's sums to its own slot of the
+// barrier's parity with agent-scope stores, waits for them, and arrives on its counter;
+// lanes 0..7 of wave 0 poll the eight counters (bounded spin); then wave 0 reads all G
+// slots and sums them in a fixed order, so every work-group obtains bit-identical totals
+// (the run-to-run result is deterministic too).  A slot of parity m & 1 is rewritten only
+// at barrier m + 2, after every work-group has passed barrier m + 1 and hence finished
+// reading it.  Returns false in every thread if the wait timed out; v[] holds the totals.
 template <int NV>
 __device__ __forceinline__ bool grid_reduce(double (&v)[NV], const ResDev &a, unsigned m,
                                             double *lds, int *flag) {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     res_block_sum<NV>(v, lds);
-    if (threadIdx.x == 0) {
-        ResCtl *c = a.ctl;
-        const int slot = (int)(m % 3u);
-        double dep = 0.0;
+    const int G = a.G;
+    double *slot = a.part + (size_t)(m & 1u) * (size_t)G * 2;
+    if (threadIdx.x < kWave) {
+        if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; i++) dep += atomicAdd(&c->acc[slot][i], v[i]);
-        depend_on(dep);
-        (void)atomicAdd(&c->count, 1u);
-        const unsigned target = (m + 1u) * (unsigned)a.G;
-        unsigned spins = 0;
+            for (int i = 0; i < NV; i++) st_coh(&slot[(size_t)blockIdx.x * 2 + i], v[i]);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            (void)atomicAdd(&a.ctl->cnt[blockIdx.x & 7][0], 1u);
+        }
         int err = 0;
-        while (ld_coh_u(&c->count) < target) {
-            if (++spins > kResSpinLimit || ld_coh_i(&c->error)) {
-                err = 1;
-                break;
+        if (threadIdx.x < 8) {
+            const unsigned share = (int)threadIdx.x < G ? (unsigned)((G - threadIdx.x + 7) / 8) : 0u;
+            const unsigned target = (m + 1u) * share;
+            unsigned spins = 0;
+            while (ld_coh_u(&a.ctl->cnt[threadIdx.x][0]) < target) {
+                if (++spins > kResSpinLimit || ld_coh_i(&a.ctl->error)) {
+                    err = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
-            __builtin_amdgcn_s_sleep(2);
         }
-        if (err) __hip_atomic_store(&c->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        err = __any(err) ? 1 : 0;
+        double sum[NV];
 #pragma unroll
-        for (int i = 0; i < NV; i++) v[i] = ld_coh(&c->acc[slot][i]);
-        if (blockIdx.x == 0) {
-            const int z = (int)((m + 2u) % 3u);
-            dep = 0.0;
+        for (int i = 0; i < NV; i++) sum[i] = 0.0;
+        for (int k = threadIdx.x; k < G; k += kWave) {
 #pragma unroll
-            for (int i = 0; i < NV; i++) dep += atomicExch(&c->acc[z][i], 0.0);
-            depend_on(dep);
+            for (int i = 0; i < NV; i++) sum[i] += ld_coh(&slot[(size_t)k * 2 + i]);
         }
 #pragma unroll
-        for (int i = 0; i < NV; i++) lds[i] = v[i];
-        *flag = err;
+        for (int i = 0; i < NV; i++) sum[i] = wave_sum(sum[i]);
+        if (threadIdx.x == 0) {
+            if (err)
+                __hip_atomic_store(&a.ctl->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int i = 0; i < NV; i++) lds[i] = sum[i];
+            *flag = err;
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -366,6 +377,28 @@ __global__ __launch_bounds__(kResThreads) void k_cg_resident(ResDev a, ResState 
     auto lslot = [&](int j) -> unsigned {
         const unsigned long long bm = __ballot((bits >> j) & 1ull);
         return (unsigned)woff[j * W + w] + (unsigned)lane_rank(bm);
+    };
+    // acc[slot] += a^(row, col) p_col over the work-group's remainder entries; pval(nb)
+    // returns p of published row nb.  Four entries per thread are in flight together
+    // (indices, values and published rows loaded before any is used); lanes past the end
+    // add 0 to slot 0.
+    auto entries = [&](unsigned tt, auto pval) {
+        for (int e0 = ebeg + (int)tt; e0 < eend; e0 += 4 * T) {
+            int nb[4], sl[4];
+            double va[4], pv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + u * T;
+                const bool in = e < eend;
+                nb[u] = in ? a.ent_nbr[e] : 0;
+                sl[u] = in ? a.ent_slot[e] : 0;
+                va[u] = in ? a.ent_val[e] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) pv[u] = pval(nb[u]);
+#pragma unroll
+            for (int u = 0; u < 4; u++) atomicAdd(&acc[sl[u]], va[u] * pv[u]);
+        }
     };
     // row j of A^ p from its band values: unit diagonal, band neighbours from the lanes or
     // the segment edges, remainder entries from acc.  Branch-free (every lane reads the
@@ -465,11 +498,7 @@ __global__ __launch_bounds__(kResThreads) void k_cg_resident(ResDev a, ResState 
         if (t == kWave)
             firstv[NSEG] = snext >= 0 ? fma(beta, ld_coh(&pold[snext]), ld_coh(&a.rpub[snext])) : 0.0;
         __syncthreads();
-        for (int e = ebeg + (int)t; e < eend; e += T) {
-            const int nb = a.ent_nbr[e];
-            const double pn = fma(beta, ld_coh(&pold[nb]), ld_coh(&a.rpub[nb]));
-            atomicAdd(&acc[a.ent_slot[e]], a.ent_val[e] * pn);
-        }
+        entries(t, [&](int nb) { return fma(beta, ld_coh(&pold[nb]), ld_coh(&a.rpub[nb])); });
         __syncthreads();
         double v1[1] = {0.0};
         refresh();
@@ -491,8 +520,7 @@ __global__ __launch_bounds__(kResThreads) void k_cg_resident(ResDev a, ResState 
         if (t == 0) lastv[0] = sprev >= 0 ? ld_coh(&pcur[sprev]) : 0.0;
         if (t == kWave) firstv[NSEG] = snext >= 0 ? ld_coh(&pcur[snext]) : 0.0;
         __syncthreads();
-        for (int e = ebeg + (int)t; e < eend; e += T)
-            atomicAdd(&acc[a.ent_slot[e]], a.ent_val[e] * ld_coh(&pcur[a.ent_nbr[e]]));
+        entries(t, [&](int nb) { return ld_coh(&pcur[nb]); });
         __syncthreads();
         const bool check = ((s.k + 1) % kResCheck == 0) || (s.k + 1 >= s.max_iter);
         double v2[2] = {0.0, 0.0};
@@ -651,7 +679,7 @@ bool res_h2d(cicy_ctx *ctx, T *dst, const std::vector<T> &src) {
 
 void resident_free(cicy_res *z) {
     if (!z) return;
-    void *ptrs[] = {z->band, z->dvec, z->x, z->rs, z->ps, z->rpub, z->ppub, z->ent_val,
+    void *ptrs[] = {z->band, z->dvec, z->x, z->rs, z->ps, z->rpub, z->ppub, z->ent_val, z->part,
                     z->slot_base, z->ent_off, z->ent_slot, z->ent_nbr, z->jbits, z->ctl, z->st};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -705,6 +733,7 @@ cicy_res *resident_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &bh
               res_alloc(ctx, z, &z->ent_nbr, (size_t)bh.m) &&
               res_alloc(ctx, z, &z->jbits, (size_t)L.G * kResThreads) &&
               res_alloc(ctx, z, &z->ctl, 1) && res_alloc(ctx, z, &z->st, 1) &&
+              res_alloc(ctx, z, &z->part, 4 * (size_t)L.G) &&
               hip_ok(ctx, hipHostMalloc((void **)&z->h_st, sizeof(ResState), hipHostMallocDefault),
                      "hipHostMalloc");
     ok = ok && res_h2d(ctx, z->band, L.band) && res_h2d(ctx, z->dvec, L.dvec) &&
@@ -744,7 +773,8 @@ bool solve_resident(cicy_sys *s, double tol, long long max_iter, int rel_to,
     hipStream_t st = ctx->stream;
     const int n = s->n, npad = (int)z->npad;
     ResDev a{n, z->G, z->S, z->band, z->dvec, z->x, z->rs, z->ps, z->rpub, z->ppub,
-             z->slot_base, z->jbits, z->ent_off, z->ent_slot, z->ent_nbr, z->ent_val, z->ctl};
+             z->slot_base, z->jbits, z->ent_off, z->ent_slot, z->ent_nbr, z->ent_val, z->ctl,
+             z->part};
     const void *kern = res_kernel_for(z->R);
     bool ok = hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "event");
     long long total = 0;
