@@ -33,6 +33,8 @@
 
 using namespace cicy;
 
+#include <vector>
+
 struct cicy_cond {
     int n = 0, m = 0, npath = 0;
     long long nnz_red = 0;
@@ -47,7 +49,14 @@ struct cicy_cond {
     double *part = nullptr, *gpart = nullptr;
     unsigned *gcount = nullptr;
     int nb = 0;                 // work-groups of the norm kernel
-    cicy_sys *red = nullptr;    // the reduced system S on the device
+    // stage 2 (series rounds over S) and the core system the CG runs on
+    int mc = 0, nrec = 0;
+    long long nnz_core = 0;
+    std::vector<int> round_off;                       // host copy: launch bounds per round
+    int *e_node = nullptr, *e_u = nullptr, *e_v = nullptr, *core_nodes = nullptr;
+    double *e_au = nullptr, *e_av = nullptr, *e_d = nullptr;
+    double *fS = nullptr, *xS = nullptr;              // [m] right-hand side / solution on S
+    cicy_sys *red = nullptr;    // the core system on the device
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timing (the inner solve uses the context's)
 };
 
@@ -110,6 +119,47 @@ __global__ __launch_bounds__(kCondThreads) void k_scatter(int m, const int *__re
     if (j < m) x[anchors[j]] = xj[j];
 }
 
+// one round of the series forward sweep: f_u -= a_u f_k / d_k, f_v -= a_v f_k / d_k
+// (nodes of a round are pairwise non-adjacent, so f_k is final; a shared neighbour may get
+// several contributions, hence the atomics)
+__global__ __launch_bounds__(kCondThreads) void k_series_fwd(int b, int e, const int *__restrict__ node,
+                                                             const int *__restrict__ u,
+                                                             const int *__restrict__ v,
+                                                             const double *__restrict__ au,
+                                                             const double *__restrict__ av,
+                                                             const double *__restrict__ dd,
+                                                             double *__restrict__ f) {
+    const int i = b + blockIdx.x * kCondThreads + threadIdx.x;
+    if (i >= e) return;
+    const double fk = f[node[i]] / dd[i];
+    if (u[i] >= 0) atomicAdd(&f[u[i]], -au[i] * fk);
+    if (v[i] >= 0) atomicAdd(&f[v[i]], -av[i] * fk);
+}
+
+// one round of the series back-substitution: x_k = (f_k - a_u x_u - a_v x_v) / d_k
+__global__ __launch_bounds__(kCondThreads) void k_series_back(int b, int e, const int *__restrict__ node,
+                                                              const int *__restrict__ u,
+                                                              const int *__restrict__ v,
+                                                              const double *__restrict__ au,
+                                                              const double *__restrict__ av,
+                                                              const double *__restrict__ dd,
+                                                              const double *__restrict__ f,
+                                                              double *__restrict__ x) {
+    const int i = b + blockIdx.x * kCondThreads + threadIdx.x;
+    if (i >= e) return;
+    double r = f[node[i]];
+    if (u[i] >= 0) r -= au[i] * x[u[i]];
+    if (v[i] >= 0) r -= av[i] * x[v[i]];
+    x[node[i]] = r / dd[i];
+}
+
+__global__ __launch_bounds__(kCondThreads) void k_gather(int mc, const int *__restrict__ idx,
+                                                         const double *__restrict__ src,
+                                                         double *__restrict__ dst) {
+    const int i = blockIdx.x * kCondThreads + threadIdx.x;
+    if (i < mc) dst[i] = src[idx[i]];
+}
+
 // x += d; r = rhs - ad; out = (r.r, ...) via the fence-free reduction
 __global__ __launch_bounds__(kThreads) void k_correct(int n, double *__restrict__ x,
                                                       const double *__restrict__ d,
@@ -168,7 +218,8 @@ void condensed_free(cicy_cond *c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     void *ptrs[] = {c->ps, c->pe, c->pl, c->pr, c->anchor_of, c->anchors, c->lfac, c->mfac,
                     c->band, c->y, c->delta, c->rhs, c->norm, c->counter, c->part, c->gpart,
-                    c->gcount};
+                    c->gcount, c->e_node, c->e_u, c->e_v, c->e_au, c->e_av, c->e_d,
+                    c->core_nodes, c->fS, c->xS};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -176,7 +227,14 @@ void condensed_free(cicy_cond *c) {
 
 int condensed_info(const cicy_cond *c, int what) {
     if (!c) return 0;
-    return what == 0 ? c->m : (int)c->nnz_red;
+    switch (what) {
+        case 0: return c->m;                                 // rows after the path stage
+        case 1: return (int)c->nnz_red;
+        case 2: return c->mc;                                // core rows (the CG system)
+        case 3: return (int)c->nnz_core;
+        case 4: return (int)c->round_off.size() - 1;         // series rounds
+        default: return 0;
+    }
 }
 
 // Host form of the condensation: anchors, paths with their LU factors, the reduced SSS
@@ -303,6 +361,147 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
     return true;
 }
 
+// Stage 2: series reduction of S.  A node of S with at most two neighbours is eliminated
+// exactly (its neighbours' diagonals lose a^2/d, the two neighbours gain the coupling
+// -a_u a_v / d), which can leave other nodes with two neighbours.  Nodes are eliminated in
+// rounds; each round takes an independent set of such nodes (greedy, in index order), so a
+// chain of L nodes collapses in about log2 L rounds and all nodes of a round can be
+// processed in parallel in the per-solve sweeps.  What remains is the core system.
+struct SeriesHost {
+    int m = 0, mc = 0;
+    std::vector<int> round_off;               // [rounds + 1] into the elimination records
+    std::vector<int> e_node, e_u, e_v;         // S indices (u, v = -1 if absent)
+    std::vector<double> e_au, e_av, e_d;       // couplings to u, v and the pivot
+    std::vector<int> core_nodes;               // [mc] S index of each core row
+    std::vector<int64_t> crp, ccol;            // core SSS
+    std::vector<double> cval, cdiag;
+};
+
+void build_series_host(const CondHost &C, SeriesHost &Z) {
+    const int m = C.m;
+    Z.m = m;
+    std::vector<std::vector<std::pair<int, double>>> adj((size_t)m);
+    for (int r = 0; r < m; r++)
+        for (int64_t k = C.rrp[r]; k < C.rrp[r + 1]; k++) {
+            const int c = (int)C.rcol[k];
+            adj[r].push_back({c, C.rval[k]});
+            adj[c].push_back({r, C.rval[k]});
+        }
+    std::vector<double> d(C.sdiag.begin(), C.sdiag.end());
+    std::vector<char> alive((size_t)m, 1), taken((size_t)m, 0);
+    auto find = [](std::vector<std::pair<int, double>> &v, int c) -> int {
+        for (size_t i = 0; i < v.size(); i++)
+            if (v[i].first == c) return (int)i;
+        return -1;
+    };
+    auto drop = [&](int u, int k) {
+        const int i = find(adj[u], k);
+        adj[u][i] = adj[u].back();
+        adj[u].pop_back();
+    };
+    auto add = [&](int u, int v, double a) {
+        const int i = find(adj[u], v);
+        if (i >= 0) adj[u][i].second += a;
+        else adj[u].push_back({v, a});
+    };
+    Z.round_off.assign(1, 0);
+    std::vector<int> cand, sel;
+    for (int i = 0; i < m; i++)
+        if (adj[i].size() <= 2) cand.push_back(i);
+    while (!cand.empty()) {
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        sel.clear();
+        for (int k : cand) {
+            if (!alive[k] || adj[k].size() > 2) continue;
+            bool free = true;
+            for (auto &e : adj[k])
+                if (taken[e.first]) free = false;
+            if (!free) continue;
+            taken[k] = 1;
+            sel.push_back(k);
+        }
+        if (sel.empty()) break;
+        // records from the state before the round (selected nodes are not adjacent)
+        for (int k : sel) {
+            Z.e_node.push_back(k);
+            const auto &a = adj[k];
+            Z.e_u.push_back(a.size() > 0 ? a[0].first : -1);
+            Z.e_au.push_back(a.size() > 0 ? a[0].second : 0.0);
+            Z.e_v.push_back(a.size() > 1 ? a[1].first : -1);
+            Z.e_av.push_back(a.size() > 1 ? a[1].second : 0.0);
+            Z.e_d.push_back(d[k]);
+        }
+        cand.clear();
+        for (size_t q = 0; q < sel.size(); q++) {
+            const int k = sel[q];
+            const size_t rec = Z.e_node.size() - sel.size() + q;
+            const int u = Z.e_u[rec], v = Z.e_v[rec];
+            const double au = Z.e_au[rec], av = Z.e_av[rec], dk = Z.e_d[rec];
+            alive[k] = 0;
+            taken[k] = 0;
+            if (u >= 0) {
+                drop(u, k);
+                d[u] -= au * au / dk;
+            }
+            if (v >= 0) {
+                drop(v, k);
+                d[v] -= av * av / dk;
+            }
+            if (u >= 0 && v >= 0) {
+                add(u, v, -au * av / dk);
+                add(v, u, -au * av / dk);
+            }
+            if (u >= 0 && adj[u].size() <= 2) cand.push_back(u);
+            if (v >= 0 && adj[v].size() <= 2) cand.push_back(v);
+            adj[k].clear();
+        }
+        Z.round_off.push_back((int)Z.e_node.size());
+    }
+    // core: the nodes left, in S order
+    std::vector<int> core_of((size_t)m, -1);
+    for (int i = 0; i < m; i++)
+        if (alive[i]) {
+            core_of[i] = (int)Z.core_nodes.size();
+            Z.core_nodes.push_back(i);
+        }
+    Z.mc = (int)Z.core_nodes.size();
+    Z.crp.assign((size_t)Z.mc + 1, 0);
+    Z.cdiag.resize((size_t)Z.mc);
+    std::vector<std::pair<int, double>> row;
+    for (int q = 0; q < Z.mc; q++) {
+        const int i = Z.core_nodes[q];
+        Z.cdiag[q] = d[i];
+        row.clear();
+        for (auto &e : adj[i])
+            if (core_of[e.first] < q) row.push_back({core_of[e.first], e.second});
+        std::sort(row.begin(), row.end());
+        for (auto &e : row) {
+            Z.ccol.push_back(e.first);
+            Z.cval.push_back(e.second);
+        }
+        Z.crp[q + 1] = (int64_t)Z.ccol.size();
+    }
+}
+
+// Stage-2 sweeps on the host (the device kernels do the same per round).
+void host_series_forward(const SeriesHost &Z, double *f) {
+    for (size_t k = 0; k < Z.e_node.size(); k++) {
+        const double fk = f[Z.e_node[k]] / Z.e_d[k];
+        if (Z.e_u[k] >= 0) f[Z.e_u[k]] -= Z.e_au[k] * fk;
+        if (Z.e_v[k] >= 0) f[Z.e_v[k]] -= Z.e_av[k] * fk;
+    }
+}
+
+void host_series_back(const SeriesHost &Z, const double *f, double *x) {
+    for (size_t k = Z.e_node.size(); k-- > 0;) {
+        double v = f[Z.e_node[k]];
+        if (Z.e_u[k] >= 0) v -= Z.e_au[k] * x[Z.e_u[k]];
+        if (Z.e_v[k] >= 0) v -= Z.e_av[k] * x[Z.e_v[k]];
+        x[Z.e_node[k]] = v / Z.e_d[k];
+    }
+}
+
 // The sweeps of k_path_solve and k_anchor_rhs on the host (for layout tests).
 void host_path_solve(const CondHost &C, const double *rhs, const double *xanchor, double *z) {
     for (size_t k = 0; k < C.ps.size(); k++) {
@@ -344,9 +543,7 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
     const int n = C.n, m = C.m;
     const std::vector<int> &ps = C.ps, &pe = C.pe, &pl = C.pl, &pr = C.pr, &anchor_of = C.anchor_of,
                            &anchors = C.anchors;
-    const std::vector<double> &lfac = C.lfac, &mfac = C.mfac, &band = C.band, &rval = C.rval,
-                              &sdiag = C.sdiag;
-    const std::vector<int64_t> &rrp = C.rrp, &rcol = C.rcol;
+    const std::vector<double> &lfac = C.lfac, &mfac = C.mfac, &band = C.band, &rval = C.rval;
     cicy_cond *c = new cicy_cond();
     c->n = n;
     c->m = m;
@@ -369,9 +566,27 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
          up(ctx, c->pr, pr) && up(ctx, c->anchor_of, anchor_of) && up(ctx, c->anchors, anchors) &&
          up(ctx, c->lfac, lfac) && up(ctx, c->mfac, mfac) && up(ctx, c->band, band) &&
          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
-    if (ok && m > 0)
-        ok = sys_create(ctx, m, (int64_t)rval.size(), rrp.data(), rcol.data(), rval.data(),
-                        sdiag.data(), kExtraResident, &c->red) != 0;
+    // stage 2: series rounds over S, then the core system
+    SeriesHost Z;
+    if (ok && m > 0) build_series_host(C, Z);
+    c->mc = Z.mc;
+    c->nrec = (int)Z.e_node.size();
+    c->round_off = Z.round_off;
+    c->nnz_core = (long long)Z.cval.size();
+    if (ok && m > 0) {
+        ok = calloc_dev(ctx, &c->e_node, Z.e_node.size()) && calloc_dev(ctx, &c->e_u, Z.e_u.size()) &&
+             calloc_dev(ctx, &c->e_v, Z.e_v.size()) && calloc_dev(ctx, &c->e_au, Z.e_au.size()) &&
+             calloc_dev(ctx, &c->e_av, Z.e_av.size()) && calloc_dev(ctx, &c->e_d, Z.e_d.size()) &&
+             calloc_dev(ctx, &c->core_nodes, Z.core_nodes.size()) &&
+             calloc_dev(ctx, &c->fS, (size_t)m) && calloc_dev(ctx, &c->xS, (size_t)m) &&
+             up(ctx, c->e_node, Z.e_node) && up(ctx, c->e_u, Z.e_u) && up(ctx, c->e_v, Z.e_v) &&
+             up(ctx, c->e_au, Z.e_au) && up(ctx, c->e_av, Z.e_av) && up(ctx, c->e_d, Z.e_d) &&
+             up(ctx, c->core_nodes, Z.core_nodes) &&
+             hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (ok && Z.mc > 0)
+            ok = sys_create(ctx, Z.mc, (int64_t)Z.cval.size(), Z.crp.data(), Z.ccol.data(),
+                            Z.cval.data(), Z.cdiag.data(), kExtraResident, &c->red) != 0;
+    }
     if (!ok) {
         condensed_free(c);
         *failed = true;
@@ -394,26 +609,47 @@ static bool condensed_correction(cicy_sys *s, int inner, const double *rhs, doub
                        st, c->npath, c->ps, c->pe, c->pl, c->pr, c->lfac, c->mfac, c->band, rhs,
                        (const double *)nullptr, c->y);
     if (c->m > 0) {
-        cicy_sys *red = c->red;
+        // reduced right-hand side on S, pushed through the series rounds
         hipLaunchKernelGGL(k_anchor_rhs, dim3(blocks(c->m, kCondThreads)), dim3(kCondThreads), 0,
-                           st, c->m, c->n, c->anchors, c->anchor_of, c->band, rhs, c->y, red->r0);
-        if (!hip_ok(ctx, hipMemsetAsync(red->x0, 0, (size_t)c->m * sizeof(double), st), "memset"))
-            return false;
-        red->has_b = false;
-        red->ref2_override = ref2;
-        bool ok;
-        if (inner == CICY_METHOD_RESIDENT)
-            ok = solve_resident(red, tol, max_iter, CICY_REL_B, inner_st);
-        else
-            ok = solve_pcg64(red, tol, max_iter, CICY_REL_B, inner_st);
-        if (!ok) return false;
+                           st, c->m, c->n, c->anchors, c->anchor_of, c->band, rhs, c->y, c->fS);
+        const int nr = (int)c->round_off.size() - 1;
+        for (int q = 0; q < nr; q++) {
+            const int b = c->round_off[q], e = c->round_off[q + 1];
+            hipLaunchKernelGGL(k_series_fwd, dim3(blocks(e - b, kCondThreads)), dim3(kCondThreads),
+                               0, st, b, e, c->e_node, c->e_u, c->e_v, c->e_au, c->e_av, c->e_d,
+                               c->fS);
+        }
+        if (c->mc > 0) {
+            cicy_sys *red = c->red;
+            hipLaunchKernelGGL(k_gather, dim3(blocks(c->mc, kCondThreads)), dim3(kCondThreads), 0,
+                               st, c->mc, c->core_nodes, c->fS, red->r0);
+            if (!hip_ok(ctx, hipMemsetAsync(red->x0, 0, (size_t)c->mc * sizeof(double), st),
+                        "memset"))
+                return false;
+            red->has_b = false;
+            red->ref2_override = ref2;
+            bool ok;
+            if (inner == CICY_METHOD_RESIDENT)
+                ok = solve_resident(red, tol, max_iter, CICY_REL_B, inner_st);
+            else
+                ok = solve_pcg64(red, tol, max_iter, CICY_REL_B, inner_st);
+            if (!ok) return false;
+            hipLaunchKernelGGL(k_scatter, dim3(blocks(c->mc, kCondThreads)), dim3(kCondThreads), 0,
+                               st, c->mc, c->core_nodes, red->x, c->xS);
+        }
+        for (int q = nr - 1; q >= 0; q--) {
+            const int b = c->round_off[q], e = c->round_off[q + 1];
+            hipLaunchKernelGGL(k_series_back, dim3(blocks(e - b, kCondThreads)), dim3(kCondThreads),
+                               0, st, b, e, c->e_node, c->e_u, c->e_v, c->e_au, c->e_av, c->e_d,
+                               c->fS, c->xS);
+        }
         hipLaunchKernelGGL(k_scatter, dim3(blocks(c->m, kCondThreads)), dim3(kCondThreads), 0, st,
-                           c->m, c->anchors, red->x, c->delta);
+                           c->m, c->anchors, c->xS, c->delta);
     }
     // d_I = T^-1 (rhs_I - A_IJ d_J)
     hipLaunchKernelGGL(k_path_solve, dim3(blocks(c->npath, kCondThreads)), dim3(kCondThreads), 0,
                        st, c->npath, c->ps, c->pe, c->pl, c->pr, c->lfac, c->mfac, c->band, rhs,
-                       (const double *)(c->m > 0 ? c->red->x : nullptr), c->delta);
+                       (const double *)(c->m > 0 ? c->xS : nullptr), c->delta);
     return true;
 }
 
@@ -426,7 +662,7 @@ bool solve_condensed(cicy_sys *s, int inner, double tol, long long max_iter, int
         if (stats) stats->status = CICY_BAD_INPUT;
         return false;
     }
-    if (inner == CICY_METHOD_RESIDENT && c->m > 0 && !c->red->res) {
+    if (inner == CICY_METHOD_RESIDENT && c->mc > 0 && !c->red->res) {
         set_error(ctx, "the reduced system has no resident layout (%s)",
                   c->red->res_why ? c->red->res_why : "not built");
         if (stats) stats->status = CICY_BAD_INPUT;
@@ -546,6 +782,71 @@ int cicy_host_expand(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
     if (!build_condensed_host(&dummy, h, bh, C, &why, &failed)) return 0;
     for (int j = 0; j < C.m; j++) x[C.anchors[j]] = x_anchor[j];
     host_path_solve(C, rhs, x_anchor, x);
+    return 1;
+}
+
+static bool host_both(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                      const double *values, const double *diagonal, int64_t nnz,
+                      cicy::CondHost &C, cicy::SeriesHost &Z) {
+    using namespace cicy;
+    cicy_ctx dummy;
+    BlockedHost h;
+    if (!build_blocked(&dummy, n, nnz, row_ptr, col_ind, values, diagonal, 1024, h)) return false;
+    BandHost bh;
+    build_band(h, bh);
+    const char *why = nullptr;
+    bool failed = false;
+    if (!build_condensed_host(&dummy, h, bh, C, &why, &failed)) return false;
+    build_series_host(C, Z);
+    return true;
+}
+
+int cicy_host_core(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                   const double *values, const double *diagonal, int64_t nnz, const double *rhs,
+                   int64_t *m_out, int64_t *nnz_out, int64_t *rounds_out, int64_t *core_rows,
+                   int64_t *core_row_ptr, int64_t *core_col_ind, double *core_values,
+                   double *core_diagonal, double *core_rhs) {
+    using namespace cicy;
+    CondHost C;
+    SeriesHost Z;
+    if (!host_both(n, row_ptr, col_ind, values, diagonal, nnz, C, Z)) return 0;
+    *m_out = Z.mc;
+    *nnz_out = (int64_t)Z.cval.size();
+    *rounds_out = (int64_t)Z.round_off.size() - 1;
+    for (int q = 0; q < Z.mc; q++) {
+        core_rows[q] = C.anchors[Z.core_nodes[q]];
+        core_diagonal[q] = Z.cdiag[q];
+    }
+    for (int q = 0; q <= Z.mc; q++) core_row_ptr[q] = Z.crp[q];
+    for (size_t k = 0; k < Z.cval.size(); k++) {
+        core_col_ind[k] = Z.ccol[k];
+        core_values[k] = Z.cval[k];
+    }
+    if (rhs && core_rhs) {
+        std::vector<double> y((size_t)n, 0.0), f((size_t)C.m, 0.0);
+        host_path_solve(C, rhs, nullptr, y.data());
+        host_anchor_rhs(C, rhs, y.data(), f.data());
+        host_series_forward(Z, f.data());
+        for (int q = 0; q < Z.mc; q++) core_rhs[q] = f[Z.core_nodes[q]];
+    }
+    return 1;
+}
+
+int cicy_host_core_expand(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                          const double *values, const double *diagonal, int64_t nnz,
+                          const double *rhs, const double *x_core, double *x) {
+    using namespace cicy;
+    CondHost C;
+    SeriesHost Z;
+    if (!host_both(n, row_ptr, col_ind, values, diagonal, nnz, C, Z)) return 0;
+    std::vector<double> y((size_t)n, 0.0), f((size_t)C.m, 0.0), xs((size_t)C.m, 0.0);
+    host_path_solve(C, rhs, nullptr, y.data());
+    host_anchor_rhs(C, rhs, y.data(), f.data());
+    host_series_forward(Z, f.data());
+    for (int q = 0; q < Z.mc; q++) xs[Z.core_nodes[q]] = x_core[q];
+    host_series_back(Z, f.data(), xs.data());
+    for (int j = 0; j < C.m; j++) x[C.anchors[j]] = xs[j];
+    host_path_solve(C, rhs, xs.data(), x);
     return 1;
 }
 

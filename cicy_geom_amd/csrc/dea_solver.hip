@@ -726,6 +726,9 @@ int64_t cicy_sys_size(const cicy_sys *s, int what) {
         case 8: return cicy::resident_info(s->res, 2);
         case 9: return cicy::condensed_info(s->cond, 0);
         case 10: return cicy::condensed_info(s->cond, 1);
+        case 11: return cicy::condensed_info(s->cond, 2);
+        case 12: return cicy::condensed_info(s->cond, 3);
+        case 13: return cicy::condensed_info(s->cond, 4);
         default: return -1;
     }
 }

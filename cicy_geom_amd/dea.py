@@ -39,7 +39,7 @@ EXPORTED_SYMBOLS = [
     "cicy_mpnn_zero_grads", "cicy_graph_create", "cicy_graph_destroy", "cicy_graph_set_target",
     "cicy_mpnn_forward", "cicy_mpnn_backward", "cicy_mpnn_adam_step", "cicy_mpnn_warm_start",
     "cicy_host_blocked_spmv", "cicy_set_resident_rows", "cicy_host_resident_apply",
-    "cicy_host_condense", "cicy_host_expand",
+    "cicy_host_condense", "cicy_host_expand", "cicy_host_core", "cicy_host_core_expand",
 ]
 
 
@@ -79,6 +79,12 @@ def lib():
                                      _F64, _VP]
     L.cicy_host_expand.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64, _F64,
                                    _F64, _F64]
+    L.cicy_host_core.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64, _VP,
+                                 ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(ctypes.c_int64), _I64, _I64, _I64, _F64, _F64,
+                                 _VP]
+    L.cicy_host_core_expand.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
+                                        _F64, _F64, _F64]
     L.cicy_copy_bench.argtypes = [_VP, ctypes.c_int64, ctypes.c_int, P(ctypes.c_double)]
     L.cicy_host_layout_spmv.argtypes = [ctypes.c_int, ctypes.c_int64, _I64, _I64, _F64, _F64,
                                         ctypes.c_int64, ctypes.c_int, _F64, _F64]
@@ -288,7 +294,8 @@ class System:
         return int(self.ctx._L.cicy_sys_size(
             self._h, {"n": 0, "nnz": 1, "remote": 2, "bytes": 3, "format": 4,
                       "off_layout": 5, "resident_rows": 6, "resident_groups": 7,
-                      "published": 8, "condensed_rows": 9, "condensed_nnz": 10}[what]))
+                      "published": 8, "condensed_rows": 9, "condensed_nnz": 10,
+                      "core_rows": 11, "core_nnz": 12, "series_rounds": 13}[what]))
 
     def set_rhs(self, r0, x0=None, b=None):
         r0 = _f64(r0)
@@ -428,4 +435,39 @@ def host_expand(row_ptr, col_ind, values, diagonal, rhs, x_anchor):
                                 len(values), _f64(rhs), _f64(x_anchor), x)
     if not ok:
         raise DEAError("cicy_host_expand: nothing to condense or invalid input")
+    return x[:n]
+
+
+def host_core(row_ptr, col_ind, values, diagonal, rhs=None):
+    """Both condensation stages on the host: (core_rows, core_row_ptr, core_col_ind,
+    core_values, core_diagonal, core_rhs or None, rounds)."""
+    n, nnz = len(diagonal), len(values)
+    rows = np.zeros(max(n, 1), np.int64)
+    crp = np.zeros(n + 1, np.int64)
+    ccol = np.zeros(max(nnz + n, 1), np.int64)
+    cval = np.zeros(max(nnz + n, 1))
+    cdiag = np.zeros(max(n, 1))
+    crhs = np.zeros(max(n, 1)) if rhs is not None else None
+    m, k, rounds = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    ok = lib().cicy_host_core(n, _i64(row_ptr), _i64(col_ind), _f64(values), _f64(diagonal), nnz,
+                              _ptr(None if rhs is None else _f64(rhs)), ctypes.byref(m),
+                              ctypes.byref(k), ctypes.byref(rounds), rows, crp, ccol, cval, cdiag,
+                              _ptr(crhs))
+    if not ok:
+        raise DEAError("cicy_host_core: nothing to condense or invalid input")
+    m, k = m.value, k.value
+    return (rows[:m], crp[:m + 1], ccol[:k], cval[:k], cdiag[:m],
+            None if crhs is None else crhs[:m], rounds.value)
+
+
+def host_core_expand(row_ptr, col_ind, values, diagonal, rhs, x_core):
+    """The full solution from the core solution (both stages' back-substitution)."""
+    n = len(diagonal)
+    x = np.zeros(max(n, 1))
+    ok = lib().cicy_host_core_expand(n, _i64(row_ptr), _i64(col_ind), _f64(values),
+                                     _f64(diagonal), len(values), _f64(rhs),
+                                     _f64(np.asarray(x_core, float).reshape(-1) if len(x_core)
+                                          else np.zeros(1)), x)
+    if not ok:
+        raise DEAError("cicy_host_core_expand: nothing to condense or invalid input")
     return x[:n]

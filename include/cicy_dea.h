@@ -48,10 +48,11 @@ typedef struct cicy_sys cicy_sys;
                                     the device's registers; cicy_sys_size 6 > 0) */
 #define CICY_METHOD_CONDENSED 3  /* exact static condensation: chain-interior rows (two band
                                     neighbours, no junction entries) are eliminated by
-                                    tridiagonal sweeps, the reference's FP64 Jacobi-PCG runs
-                                    on the Schur complement of the junction rows, the
-                                    eliminated rows are recovered by back-substitution
-                                    (cicy_sys_size 9 > 0) */
+                                    tridiagonal sweeps, then rows left with at most two
+                                    neighbours by rounds of series elimination; the
+                                    reference's FP64 Jacobi-PCG runs on the remaining core
+                                    (the Schur complement), the eliminated rows are recovered
+                                    by back-substitution (cicy_sys_size 9 > 0) */
 #define CICY_METHOD_CONDENSED_RESIDENT 4  /* the same with the resident CG on the reduced
                                              system */
 
@@ -134,8 +135,11 @@ int64_t cicy_sys_size(const cicy_sys *sys, int what); /* 0 n, 1 nnz, 2 remote en
                                                            6 resident rows per thread (0:
                                                            no resident layout), 7 resident
                                                            work-groups, 8 published rows,
-                                                           9 condensed rows (0: none), 10
-                                                           condensed stored entries */
+                                                           9 rows after path elimination
+                                                           (0: no condensed system), 10 their
+                                                           stored entries, 11 core rows (the
+                                                           system CG runs on), 12 core stored
+                                                           entries, 13 series rounds */
 /* r0 = b - A x0 (the reference's R), x0 (may be NULL = zeros), b (may be NULL; needed for
  * the true residual and for CICY_REL_B). */
 int cicy_sys_set_rhs(cicy_sys *sys, const double *r0, const double *x0, const double *b);
@@ -236,6 +240,18 @@ int cicy_host_condense(int64_t n, const int64_t *row_ptr, const int64_t *col_ind
 int cicy_host_expand(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
                      const double *values, const double *diagonal, int64_t nnz,
                      const double *rhs, const double *x_anchor, double *x);
+/* Both stages on the host: the core system the condensed method runs CG on (core rows as
+ * indices of the full system [*m_out], SSS arrays, diagonal and right-hand side) and the
+ * number of series rounds.  Buffer sizes as for cicy_host_condense. */
+int cicy_host_core(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                   const double *values, const double *diagonal, int64_t nnz, const double *rhs,
+                   int64_t *m_out, int64_t *nnz_out, int64_t *rounds_out, int64_t *core_rows,
+                   int64_t *core_row_ptr, int64_t *core_col_ind, double *core_values,
+                   double *core_diagonal, double *core_rhs);
+/* The full solution x [n] from the core solution x_core [m] of cicy_host_core. */
+int cicy_host_core_expand(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
+                          const double *values, const double *diagonal, int64_t nnz,
+                          const double *rhs, const double *x_core, double *x);
 
 #ifdef __cplusplus
 }

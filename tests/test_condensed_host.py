@@ -58,6 +58,59 @@ def test_expand_recovers_the_full_solution(c1):
     assert np.linalg.norm(s.b - ax) <= 1e-12 * np.linalg.norm(s.b)
 
 
+def test_core_is_the_schur_complement_and_expands_exactly(c1):
+    """Second stage (series rounds): the core matrix is the Schur complement of A onto the
+    core rows, its right-hand side the condensed one, and the back-substitution of both
+    stages gives the full solution."""
+    s = c1
+    rows, crp, ccol, cval, cdiag, crhs, rounds = dea.host_core(s.row_ptr, s.col_ind, s.values,
+                                                               s.diag, s.b)
+    anchors = dea.host_condense(s.row_ptr, s.col_ind, s.values, s.diag)[0]
+    assert rounds >= 1 and 0 < len(rows) < len(anchors)
+    assert set(rows) <= set(anchors) and np.all(np.diff(rows) > 0)
+    A = dense(s.row_ptr, s.col_ind, s.values, s.diag)
+    J = rows
+    I = np.setdiff1d(np.arange(s.n), J)
+    AII, AIJ, AJJ = A[np.ix_(I, I)], A[np.ix_(I, J)], A[np.ix_(J, J)]
+    S = AJJ - AIJ.T @ np.linalg.solve(AII, AIJ)
+    C = dense(crp, ccol, cval, cdiag)
+    assert np.abs(C - S).max() <= 1e-10 * np.abs(S).max()
+    bc = s.b[J] - AIJ.T @ np.linalg.solve(AII, s.b[I])
+    assert np.allclose(crhs, bc, rtol=1e-9, atol=1e-12 * np.abs(bc).max())
+    xc = np.linalg.solve(C, crhs)
+    x = dea.host_core_expand(s.row_ptr, s.col_ind, s.values, s.diag, s.b, xc)
+    want = np.linalg.solve(A, s.b)
+    assert np.linalg.norm(x - want) <= 1e-10 * np.linalg.norm(want)
+
+
+def test_star_reduces_completely():
+    """A hub with three spokes of series resistors to ground is series-parallel: both stages
+    eliminate every row, and the back-substitution alone solves it exactly (hub voltage of
+    a unit current source = 1 / (3 g/3))."""
+    g = 2.0
+    n = 7          # hub 0; spoke s: node 1+2s next to the hub, node 2+2s grounded through g
+    lower = {}
+    for k in range(3):
+        a, b = 1 + 2 * k, 2 + 2 * k
+        lower[(a, 0)] = -g
+        lower[(b, a)] = -g
+    keys = sorted(lower)
+    rp = np.zeros(n + 1, np.int64)
+    for r, _ in keys:
+        rp[r + 1] += 1
+    rp = np.cumsum(rp)
+    col = np.array([c for _, c in keys], np.int64)
+    val = np.array([lower[k] for k in keys])
+    diag = np.array([3 * g] + [2 * g, 2 * g] * 3)
+    rhs = np.zeros(n)
+    rhs[0] = 1.0
+    rows, *_ = dea.host_core(rp, col, val, diag, rhs)
+    assert len(rows) == 0
+    x = dea.host_core_expand(rp, col, val, diag, rhs, np.zeros(0))
+    assert x[0] == pytest.approx(1.0 / (3 * g / 3.0), rel=1e-13)
+    assert np.allclose(x, np.linalg.solve(dense(rp, col, val, diag), rhs), rtol=1e-13)
+
+
 def chain_system(k, g, junction=0.5):
     """Two anchors (rows 0 and k+1) joined by a chain of k interior rows of conductance g,
     plus a junction of conductance `junction` between the anchors and a ground leak on
