@@ -785,6 +785,12 @@ int cicy_host_expand(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
     return 1;
 }
 
+int cicy_sys_core(cicy_sys *s, cicy_sys **core) {
+    if (!s || !core) return 0;
+    *core = (s->cond && s->cond->red) ? s->cond->red : nullptr;
+    return *core ? 1 : 0;
+}
+
 static bool host_both(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
                       const double *values, const double *diagonal, int64_t nnz,
                       cicy::CondHost &C, cicy::SeriesHost &Z) {

@@ -657,11 +657,16 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
         cicy_sys_destroy(s);
         return 0;
     }
+    bool have_band = s->format == kFormatBand;
     if (!(extras & kExtraResident)) {
         s->res_why = "not requested";
-    } else if (s->format != kFormatBand) {
-        s->res_why = "system not in band + remainder format";
     } else {
+        // the resident layout splits the matrix into band + remainder whatever the
+        // device format of the two-kernel path (remainder rows become published rows)
+        if (!have_band) {
+            build_band(h, bh);
+            have_band = true;
+        }
         bool failed = false;
         s->res = resident_setup(ctx, h, bh, ctx->resident_rows, &s->res_why, &failed);
         if (failed) {
@@ -672,7 +677,7 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
     if (!(extras & kExtraCondensed)) {
         s->cond_why = "not requested";
     } else {
-        if (s->format != kFormatBand) build_band(h, bh);
+        if (!have_band) build_band(h, bh);
         bool failed = false;
         s->cond = condensed_setup(ctx, h, bh, &s->cond_why, &failed);
         if (failed) {

@@ -40,6 +40,7 @@ EXPORTED_SYMBOLS = [
     "cicy_mpnn_forward", "cicy_mpnn_backward", "cicy_mpnn_adam_step", "cicy_mpnn_warm_start",
     "cicy_host_blocked_spmv", "cicy_set_resident_rows", "cicy_host_resident_apply",
     "cicy_host_condense", "cicy_host_expand", "cicy_host_core", "cicy_host_core_expand",
+    "cicy_sys_core",
 ]
 
 
@@ -106,6 +107,7 @@ def lib():
     L.cicy_sys_solve.argtypes = [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_int64,
                                  ctypes.c_int, P(SolveStats)]
     L.cicy_sys_get.argtypes = [_VP, _VP, _VP]
+    L.cicy_sys_core.argtypes = [_VP, P(_VP)]
     L.cicy_sys_spmv_bench.argtypes = [_VP, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
     L.cicy_host_blocked_spmv.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
                                          ctypes.c_int, _F64, _F64]
@@ -323,6 +325,27 @@ class System:
                                                         ctypes.byref(ms)),
                         "cicy_sys_spmv_bench")
         return ms.value
+
+    def core(self):
+        """The core system of the condensed method (borrowed: lives as long as this
+        system), or None."""
+        h = _VP()
+        if not self.ctx._L.cicy_sys_core(self._h, ctypes.byref(h)) or not h.value:
+            return None
+        return _Borrowed(self, h)
+
+
+class _Borrowed(System):
+    """A system handle owned by another system (its core); never destroyed from here."""
+
+    def __init__(self, owner: System, h):
+        self.ctx = owner.ctx
+        self._owner = owner
+        self._h = h
+        self.n = int(self.ctx._L.cicy_sys_size(h, 0))
+
+    def close(self):
+        self._h = None
 
 
 class Batch:

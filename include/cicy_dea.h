@@ -147,6 +147,10 @@ int cicy_sys_set_rhs(cicy_sys *sys, const double *r0, const double *x0, const do
 int cicy_sys_solve(cicy_sys *sys, int method, double tol, int64_t max_iter, int rel_to,
                    cicy_solve_stats *stats);
 int cicy_sys_get(cicy_sys *sys, double *x, double *r);
+/* The core system a CICY_METHOD_CONDENSED solve runs CG on, as a borrowed handle (owned by
+ * sys, valid while sys lives; do not destroy it), for measuring its kernels; 0 if sys has
+ * no condensed system.  No counterpart in the reference. */
+int cicy_sys_core(cicy_sys *sys, cicy_sys **core);
 /* Time `reps` back-to-back SSS SpMV launches on resident data; average ms per launch. */
 int cicy_sys_spmv_bench(cicy_sys *sys, int precision /*64 or 32*/, int reps, double *avg_ms);
 /* Time a plain device copy of `bytes` (read `bytes`, write `bytes`); the reference kernel for

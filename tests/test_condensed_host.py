@@ -211,6 +211,34 @@ def test_resident_layout_on_host(R, phase):
     assert np.allclose(got, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
 
 
+@pytest.mark.parametrize("phase", [0, 1])
+def test_resident_layout_without_band_structure(phase):
+    """A junction-dominated matrix (every row published), as a core system is."""
+    rng = np.random.default_rng(11 + phase)
+    n = 2048 * 2 + 300
+    lower = {}
+    for a, b in rng.integers(0, n, (3 * n, 2)):
+        if a != b:
+            lower[(max(a, b), min(a, b))] = -10.0 ** rng.uniform(-4, 0)
+    keys = sorted(lower)
+    rp = np.zeros(n + 1, np.int64)
+    for r, _ in keys:
+        rp[r + 1] += 1
+    rp = np.cumsum(rp)
+    col = np.array([c for _, c in keys], np.int64)
+    val = np.array([lower[k] for k in keys])
+    diag = np.full(n, 0.01)
+    for (r, c), v in lower.items():
+        diag[r] -= v
+        diag[c] -= v
+    r = rng.standard_normal(n)
+    pold = rng.standard_normal(n)
+    got = dea.host_resident_apply(rp, col, val, diag, r, pold, -0.8, 4, phase)
+    sc = 1.0 / np.sqrt(diag)
+    want = sc * oracle.spmv_sss(rp, col, val, diag, sc * (r - 0.8 * pold))
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+
+
 def test_resident_layout_rejects_unknown_rows_per_thread(c1):
     s = c1
     with pytest.raises(dea.DEAError):
