@@ -6,11 +6,16 @@ percolating along x, R_flag=1 resistance solve (family := x), zero initial guess
 system is assembled on the host and uploaded once; the timed region covers only solves
 on HBM-resident data.  A "step" is one solve.
 
-Methods: plain FP64 Jacobi-PCG on the GPU (the reference's algorithm), the
-mixed-precision FP32/FP64-reliable-update solver, and the learned warm start followed by
-the mixed solver (network inference is inside the timed solve).  Each method runs on the
-device layout and row-block size the autotuner picks for it on this system.  `value` is
-the headline method's solves/s.  Prints one JSON line on stdout; progress goes to stderr.
+Methods (--methods): plain FP64 Jacobi-PCG on the GPU (the reference's algorithm, always
+timed as the reference line), the condensed solve (exact series elimination down to a core
+system, the same FP64 Jacobi-PCG on the core, back-substitution; timed end to end), the
+condensed solve with the register-resident CG on the core, the register-resident CG on
+the full system, the mixed-precision FP32/FP64 solver, and the learned warm start followed
+by a solver (network inference inside the timed solve; needs trained weights).  Each
+method runs on the device layout and row-block size the autotuner picks for it on this
+system.  `value` is the fastest converged method's solves/s; every method's true residual
+||b - A x|| / ||b|| is checked with the full system's FP64 SpMV and reported.  Prints one
+JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -96,8 +101,11 @@ def cpu_baseline(s, iters_needed, budget_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="timed solves of the plain FP64 reference line (about 19 s each at "
+                         "configuration 3)")
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -143,6 +151,13 @@ def main():
             algo = network.spmv_bytes(s.n, s.nnz, sv=sv, si=4)
             log(f"[bench] SpMV fp{prec}: {ms * 1e3:.1f} us/launch, algorithmic {algo / 1e6:.1f} MB "
                 f"-> {algo / ms / 1e6:.0f} GB/s ({sysd.size('format')=})")
+        core = sysd.core()
+        if core is not None:
+            cn, cl = core.size("n"), core.size("nnz")
+            ms = core.spmv_bench(64, 200)
+            algo = network.spmv_bytes(cn, cl, sv=8, si=4)
+            log(f"[bench] core SpMV fp64 ({cn} rows, {cl} entries): {ms * 1e3:.1f} us/launch, "
+                f"algorithmic {algo / 1e6:.2f} MB -> {algo / ms / 1e6:.0f} GB/s")
         print(json.dumps({"copy_ms": copy_ms, "reps": reps, "n": s.n, "nnz": s.nnz}))
         sysd.close()
         ctx.close()
@@ -150,8 +165,8 @@ def main():
     catalog = {"pcg_f64": (dea.METHOD_PCG_F64, "pcg_f64", False),
                "mixed": (dea.METHOD_MIXED, "mixed", False),
                "resident": (dea.METHOD_RESIDENT, "pcg_f64", False),
-               "condensed": (dea.METHOD_CONDENSED, "pcg_f64", False),
-               "condensed_resident": (dea.METHOD_CONDENSED_RESIDENT, "pcg_f64", False),
+               "condensed": (dea.METHOD_CONDENSED, "condensed", False),
+               "condensed_resident": (dea.METHOD_CONDENSED_RESIDENT, "condensed", False),
                "learned_f64": (dea.METHOD_PCG_F64, "pcg_f64", True),
                "learned_mixed": (dea.METHOD_MIXED, "mixed", True)}
     wanted = [m for m in args.methods.split(",") if m]
@@ -168,15 +183,19 @@ def main():
     plan = [(m,) + catalog[m] for m in wanted if model is not None or not catalog[m][2]]
 
     tuned = {}
-    for name, method in (("pcg_f64", dea.METHOD_PCG_F64), ("mixed", dea.METHOD_MIXED)):
+    for name, method in (("pcg_f64", dea.METHOD_PCG_F64), ("mixed", dea.METHOD_MIXED),
+                         ("condensed", dea.METHOD_CONDENSED)):
         if not any(p[2] == name for p in plan):
             continue
         if args.rows_per_block:
             tuned[name] = {"format": dea.FORMAT_AUTO, "rows_per_block": args.rows_per_block}
         else:
             t0 = time.perf_counter()
+            # the condensed methods keep the automatic layout choice: the full system and
+            # its core differ in structure (band-dominated vs junction-dominated)
+            layouts = (dea.FORMAT_AUTO,) if name == "condensed" else None
             tuned[name] = autotune.tune(ctx, s, method, iters=96, cache_path=None,
-                                        rows=(256, 512, 1024), log=log)
+                                        layouts=layouts, rows=(256, 512, 1024), log=log)
             log(f"[bench] autotune {name}: {autotune.LAYOUTS.get(tuned[name]['format'])} "
                 f"rb={tuned[name]['rows_per_block']} ({time.perf_counter() - t0:.1f}s)")
 
@@ -190,10 +209,12 @@ def main():
         upload_s = time.perf_counter() - t0
         if learned:
             graph = warmstart.Graph(ctx, s, box, 0, 1.0)
-        wall, stats = time_solves(sysd, method, args.steps, args.warmup, args.tol, s,
+        steps, warmup = (args.plain_steps, 0) if name == "pcg_f64" else (args.steps, args.warmup)
+        wall, stats = time_solves(sysd, method, steps, warmup, args.tol, s,
                                   model if learned else None, graph if learned else None)
         st = stats[-1]
-        results[name] = dict(solves_per_s=args.steps / wall, ms_per_solve=1e3 * wall / args.steps,
+        results[name] = dict(solves_per_s=steps / wall, ms_per_solve=1e3 * wall / steps,
+                             steps=steps,
                              iters=st.iters, refinements=st.refinements,
                              true_rel_res=st.true_rel_res, status=dea.STATUS_NAMES[st.status],
                              device_ms=st.solve_ms,
@@ -264,8 +285,8 @@ def main():
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": results[head]["steps"],
+        "warmup": 0 if head == "pcg_f64" else args.warmup,
         "ms_per_step": results[head]["ms_per_solve"],
         "higher_is_better": True,
         "scaling": "weak",

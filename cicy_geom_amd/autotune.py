@@ -15,7 +15,8 @@ import os
 
 from . import dea
 
-LAYOUTS = {dea.FORMAT_SSS: "blocked-sss", dea.FORMAT_BAND: "band+remainder"}
+LAYOUTS = {dea.FORMAT_AUTO: "auto", dea.FORMAT_SSS: "blocked-sss",
+           dea.FORMAT_BAND: "band+remainder"}
 ROWS_PER_BLOCK = (128, 256, 512, 1024)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "cicy_geom_amd",
                              "autotune.json")
@@ -46,7 +47,7 @@ def tune(ctx: dea.Context, s, method: int = dea.METHOD_MIXED, iters: int = 128,
          layouts=None, rows=ROWS_PER_BLOCK, log=None) -> dict:
     """Measure every candidate on `s` and return {"format", "rows_per_block", "table"};
     the winner is stored in the cache under (device, method, size bucket)."""
-    layouts = layouts or tuple(LAYOUTS)
+    layouts = layouts or (dea.FORMAT_SSS, dea.FORMAT_BAND)
     table = []
     for fmt in layouts:
         for rb in rows:
