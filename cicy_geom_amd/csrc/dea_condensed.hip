@@ -53,8 +53,8 @@ struct cicy_cond {
     int mc = 0, nrec = 0;
     long long nnz_core = 0;
     std::vector<int> round_off;                       // host copy: launch bounds per round
-    int *e_node = nullptr, *e_u = nullptr, *e_v = nullptr, *core_nodes = nullptr;
-    double *e_au = nullptr, *e_av = nullptr, *e_d = nullptr;
+    int *e_node = nullptr, *e_off = nullptr, *e_nbr = nullptr, *core_nodes = nullptr;
+    double *e_a = nullptr, *e_d = nullptr;
     double *fS = nullptr, *xS = nullptr;              // [m] right-hand side / solution on S
     cicy_sys *red = nullptr;    // the core system on the device
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timing (the inner solve uses the context's)
@@ -119,38 +119,34 @@ __global__ __launch_bounds__(kCondThreads) void k_scatter(int m, const int *__re
     if (j < m) x[anchors[j]] = xj[j];
 }
 
-// one round of the series forward sweep: f_u -= a_u f_k / d_k, f_v -= a_v f_k / d_k
-// (nodes of a round are pairwise non-adjacent, so f_k is final; a shared neighbour may get
-// several contributions, hence the atomics)
+// one round of the forward sweep: f_u -= a_uk f_k / d_k for every neighbour u of k (nodes
+// of a round are pairwise non-adjacent, so f_k is final; a shared neighbour may get several
+// contributions, hence the atomics)
 __global__ __launch_bounds__(kCondThreads) void k_series_fwd(int b, int e, const int *__restrict__ node,
-                                                             const int *__restrict__ u,
-                                                             const int *__restrict__ v,
-                                                             const double *__restrict__ au,
-                                                             const double *__restrict__ av,
+                                                             const int *__restrict__ off,
+                                                             const int *__restrict__ nbr,
+                                                             const double *__restrict__ a,
                                                              const double *__restrict__ dd,
                                                              double *__restrict__ f) {
-    const int i = b + blockIdx.x * kCondThreads + threadIdx.x;
-    if (i >= e) return;
-    const double fk = f[node[i]] / dd[i];
-    if (u[i] >= 0) atomicAdd(&f[u[i]], -au[i] * fk);
-    if (v[i] >= 0) atomicAdd(&f[v[i]], -av[i] * fk);
+    const int k = b + blockIdx.x * kCondThreads + threadIdx.x;
+    if (k >= e) return;
+    const double fk = f[node[k]] / dd[k];
+    for (int i = off[k]; i < off[k + 1]; i++) atomicAdd(&f[nbr[i]], -a[i] * fk);
 }
 
-// one round of the series back-substitution: x_k = (f_k - a_u x_u - a_v x_v) / d_k
+// one round of the back-substitution: x_k = (f_k - sum_u a_uk x_u) / d_k
 __global__ __launch_bounds__(kCondThreads) void k_series_back(int b, int e, const int *__restrict__ node,
-                                                              const int *__restrict__ u,
-                                                              const int *__restrict__ v,
-                                                              const double *__restrict__ au,
-                                                              const double *__restrict__ av,
+                                                              const int *__restrict__ off,
+                                                              const int *__restrict__ nbr,
+                                                              const double *__restrict__ a,
                                                               const double *__restrict__ dd,
                                                               const double *__restrict__ f,
                                                               double *__restrict__ x) {
-    const int i = b + blockIdx.x * kCondThreads + threadIdx.x;
-    if (i >= e) return;
-    double r = f[node[i]];
-    if (u[i] >= 0) r -= au[i] * x[u[i]];
-    if (v[i] >= 0) r -= av[i] * x[v[i]];
-    x[node[i]] = r / dd[i];
+    const int k = b + blockIdx.x * kCondThreads + threadIdx.x;
+    if (k >= e) return;
+    double r = f[node[k]];
+    for (int i = off[k]; i < off[k + 1]; i++) r -= a[i] * x[nbr[i]];
+    x[node[k]] = r / dd[k];
 }
 
 __global__ __launch_bounds__(kCondThreads) void k_gather(int mc, const int *__restrict__ idx,
@@ -218,7 +214,7 @@ void condensed_free(cicy_cond *c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     void *ptrs[] = {c->ps, c->pe, c->pl, c->pr, c->anchor_of, c->anchors, c->lfac, c->mfac,
                     c->band, c->y, c->delta, c->rhs, c->norm, c->counter, c->part, c->gpart,
-                    c->gcount, c->e_node, c->e_u, c->e_v, c->e_au, c->e_av, c->e_d,
+                    c->gcount, c->e_node, c->e_off, c->e_nbr, c->e_a, c->e_d,
                     c->core_nodes, c->fS, c->xS};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -361,23 +357,30 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
     return true;
 }
 
-// Stage 2: series reduction of S.  A node of S with at most two neighbours is eliminated
-// exactly (its neighbours' diagonals lose a^2/d, the two neighbours gain the coupling
-// -a_u a_v / d), which can leave other nodes with two neighbours.  Nodes are eliminated in
-// rounds; each round takes an independent set of such nodes (greedy, in index order), so a
-// chain of L nodes collapses in about log2 L rounds and all nodes of a round can be
-// processed in parallel in the per-solve sweeps.  What remains is the core system.
+// Stage 2: low-degree elimination on S.  A node k of S with at most kElimDegree neighbours
+// is eliminated exactly: each neighbour u loses a_uk^2 / d_k from its diagonal and every
+// pair of neighbours (u, v) gains the coupling -a_uk a_vk / d_k (for two neighbours this is
+// series reduction of the resistor network, for more the star-mesh transform).  This can
+// leave other nodes with few neighbours.  Nodes are eliminated in rounds; each round takes
+// an independent set of such nodes (greedy, in index order), so a chain of L nodes
+// collapses in about log2 L rounds and the nodes of a round are processed in parallel in
+// the per-solve sweeps.  The cap bounds the fill (a node of degree D adds at most
+// D (D - 1) / 2 couplings).  What remains is the core system.
+constexpr int kElimDegree = 4;
+
 struct SeriesHost {
     int m = 0, mc = 0;
     std::vector<int> round_off;               // [rounds + 1] into the elimination records
-    std::vector<int> e_node, e_u, e_v;         // S indices (u, v = -1 if absent)
-    std::vector<double> e_au, e_av, e_d;       // couplings to u, v and the pivot
+    std::vector<int> e_node;                   // S index of each eliminated node
+    std::vector<int> e_off;                    // [records + 1] into e_nbr / e_a
+    std::vector<int> e_nbr;                    // neighbours at elimination (S indices)
+    std::vector<double> e_a, e_d;              // couplings to them, and the pivot
     std::vector<int> core_nodes;               // [mc] S index of each core row
     std::vector<int64_t> crp, ccol;            // core SSS
     std::vector<double> cval, cdiag;
 };
 
-void build_series_host(const CondHost &C, SeriesHost &Z) {
+void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimDegree) {
     const int m = C.m;
     Z.m = m;
     std::vector<std::vector<std::pair<int, double>>> adj((size_t)m);
@@ -405,15 +408,16 @@ void build_series_host(const CondHost &C, SeriesHost &Z) {
         else adj[u].push_back({v, a});
     };
     Z.round_off.assign(1, 0);
+    Z.e_off.assign(1, 0);
     std::vector<int> cand, sel;
     for (int i = 0; i < m; i++)
-        if (adj[i].size() <= 2) cand.push_back(i);
+        if ((int)adj[i].size() <= max_degree) cand.push_back(i);
     while (!cand.empty()) {
         std::sort(cand.begin(), cand.end());
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
         sel.clear();
         for (int k : cand) {
-            if (!alive[k] || adj[k].size() > 2) continue;
+            if (!alive[k] || (int)adj[k].size() > max_degree) continue;
             bool free = true;
             for (auto &e : adj[k])
                 if (taken[e.first]) free = false;
@@ -423,37 +427,37 @@ void build_series_host(const CondHost &C, SeriesHost &Z) {
         }
         if (sel.empty()) break;
         // records from the state before the round (selected nodes are not adjacent)
+        const size_t first = Z.e_node.size();
         for (int k : sel) {
             Z.e_node.push_back(k);
-            const auto &a = adj[k];
-            Z.e_u.push_back(a.size() > 0 ? a[0].first : -1);
-            Z.e_au.push_back(a.size() > 0 ? a[0].second : 0.0);
-            Z.e_v.push_back(a.size() > 1 ? a[1].first : -1);
-            Z.e_av.push_back(a.size() > 1 ? a[1].second : 0.0);
+            for (auto &e : adj[k]) {
+                Z.e_nbr.push_back(e.first);
+                Z.e_a.push_back(e.second);
+            }
+            Z.e_off.push_back((int)Z.e_nbr.size());
             Z.e_d.push_back(d[k]);
         }
         cand.clear();
         for (size_t q = 0; q < sel.size(); q++) {
             const int k = sel[q];
-            const size_t rec = Z.e_node.size() - sel.size() + q;
-            const int u = Z.e_u[rec], v = Z.e_v[rec];
-            const double au = Z.e_au[rec], av = Z.e_av[rec], dk = Z.e_d[rec];
+            const size_t rec = first + q;
+            const int b = Z.e_off[rec], e = Z.e_off[rec + 1];
+            const double dk = Z.e_d[rec];
             alive[k] = 0;
             taken[k] = 0;
-            if (u >= 0) {
+            for (int i = b; i < e; i++) {
+                const int u = Z.e_nbr[i];
                 drop(u, k);
-                d[u] -= au * au / dk;
+                d[u] -= Z.e_a[i] * Z.e_a[i] / dk;
             }
-            if (v >= 0) {
-                drop(v, k);
-                d[v] -= av * av / dk;
-            }
-            if (u >= 0 && v >= 0) {
-                add(u, v, -au * av / dk);
-                add(v, u, -au * av / dk);
-            }
-            if (u >= 0 && adj[u].size() <= 2) cand.push_back(u);
-            if (v >= 0 && adj[v].size() <= 2) cand.push_back(v);
+            for (int i = b; i < e; i++)
+                for (int j = i + 1; j < e; j++) {
+                    const double f = -Z.e_a[i] * Z.e_a[j] / dk;
+                    add(Z.e_nbr[i], Z.e_nbr[j], f);
+                    add(Z.e_nbr[j], Z.e_nbr[i], f);
+                }
+            for (int i = b; i < e; i++)
+                if ((int)adj[Z.e_nbr[i]].size() <= max_degree) cand.push_back(Z.e_nbr[i]);
             adj[k].clear();
         }
         Z.round_off.push_back((int)Z.e_node.size());
@@ -488,16 +492,14 @@ void build_series_host(const CondHost &C, SeriesHost &Z) {
 void host_series_forward(const SeriesHost &Z, double *f) {
     for (size_t k = 0; k < Z.e_node.size(); k++) {
         const double fk = f[Z.e_node[k]] / Z.e_d[k];
-        if (Z.e_u[k] >= 0) f[Z.e_u[k]] -= Z.e_au[k] * fk;
-        if (Z.e_v[k] >= 0) f[Z.e_v[k]] -= Z.e_av[k] * fk;
+        for (int i = Z.e_off[k]; i < Z.e_off[k + 1]; i++) f[Z.e_nbr[i]] -= Z.e_a[i] * fk;
     }
 }
 
 void host_series_back(const SeriesHost &Z, const double *f, double *x) {
     for (size_t k = Z.e_node.size(); k-- > 0;) {
         double v = f[Z.e_node[k]];
-        if (Z.e_u[k] >= 0) v -= Z.e_au[k] * x[Z.e_u[k]];
-        if (Z.e_v[k] >= 0) v -= Z.e_av[k] * x[Z.e_v[k]];
+        for (int i = Z.e_off[k]; i < Z.e_off[k + 1]; i++) v -= Z.e_a[i] * x[Z.e_nbr[i]];
         x[Z.e_node[k]] = v / Z.e_d[k];
     }
 }
@@ -574,13 +576,13 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
     c->round_off = Z.round_off;
     c->nnz_core = (long long)Z.cval.size();
     if (ok && m > 0) {
-        ok = calloc_dev(ctx, &c->e_node, Z.e_node.size()) && calloc_dev(ctx, &c->e_u, Z.e_u.size()) &&
-             calloc_dev(ctx, &c->e_v, Z.e_v.size()) && calloc_dev(ctx, &c->e_au, Z.e_au.size()) &&
-             calloc_dev(ctx, &c->e_av, Z.e_av.size()) && calloc_dev(ctx, &c->e_d, Z.e_d.size()) &&
+        ok = calloc_dev(ctx, &c->e_node, Z.e_node.size()) &&
+             calloc_dev(ctx, &c->e_off, Z.e_off.size()) && calloc_dev(ctx, &c->e_nbr, Z.e_nbr.size()) &&
+             calloc_dev(ctx, &c->e_a, Z.e_a.size()) && calloc_dev(ctx, &c->e_d, Z.e_d.size()) &&
              calloc_dev(ctx, &c->core_nodes, Z.core_nodes.size()) &&
              calloc_dev(ctx, &c->fS, (size_t)m) && calloc_dev(ctx, &c->xS, (size_t)m) &&
-             up(ctx, c->e_node, Z.e_node) && up(ctx, c->e_u, Z.e_u) && up(ctx, c->e_v, Z.e_v) &&
-             up(ctx, c->e_au, Z.e_au) && up(ctx, c->e_av, Z.e_av) && up(ctx, c->e_d, Z.e_d) &&
+             up(ctx, c->e_node, Z.e_node) && up(ctx, c->e_off, Z.e_off) &&
+             up(ctx, c->e_nbr, Z.e_nbr) && up(ctx, c->e_a, Z.e_a) && up(ctx, c->e_d, Z.e_d) &&
              up(ctx, c->core_nodes, Z.core_nodes) &&
              hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
         if (ok && Z.mc > 0)
@@ -616,8 +618,7 @@ static bool condensed_correction(cicy_sys *s, int inner, const double *rhs, doub
         for (int q = 0; q < nr; q++) {
             const int b = c->round_off[q], e = c->round_off[q + 1];
             hipLaunchKernelGGL(k_series_fwd, dim3(blocks(e - b, kCondThreads)), dim3(kCondThreads),
-                               0, st, b, e, c->e_node, c->e_u, c->e_v, c->e_au, c->e_av, c->e_d,
-                               c->fS);
+                               0, st, b, e, c->e_node, c->e_off, c->e_nbr, c->e_a, c->e_d, c->fS);
         }
         if (c->mc > 0) {
             cicy_sys *red = c->red;
@@ -640,8 +641,8 @@ static bool condensed_correction(cicy_sys *s, int inner, const double *rhs, doub
         for (int q = nr - 1; q >= 0; q--) {
             const int b = c->round_off[q], e = c->round_off[q + 1];
             hipLaunchKernelGGL(k_series_back, dim3(blocks(e - b, kCondThreads)), dim3(kCondThreads),
-                               0, st, b, e, c->e_node, c->e_u, c->e_v, c->e_au, c->e_av, c->e_d,
-                               c->fS, c->xS);
+                               0, st, b, e, c->e_node, c->e_off, c->e_nbr, c->e_a, c->e_d, c->fS,
+                               c->xS);
         }
         hipLaunchKernelGGL(k_scatter, dim3(blocks(c->m, kCondThreads)), dim3(kCondThreads), 0, st,
                            c->m, c->anchors, c->xS, c->delta);
