@@ -253,7 +253,7 @@ def main():
                 results[cand]["solves_per_s"] > results[head]["solves_per_s"]:
             head = cand
     value = results[head]["solves_per_s"]
-    hp = 64 if catalog[head][1] == "pcg_f64" else 32
+    hp = 32 if catalog[head][1] == "mixed" else 64
     # whole-iteration rate of the plain FP64 solve (device time / iterations)
     it_bytes = network.pcg_iter_bytes(n, L)
     it_us = 1e3 * plain["device_ms"] / max(1, plain["iters"]) if plain else 0.0
