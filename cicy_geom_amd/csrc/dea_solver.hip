@@ -563,6 +563,18 @@ int cicy_set_rows_per_block(cicy_ctx *ctx, int rb) {
     return 1;
 }
 
+int cicy_set_dropin_method(cicy_ctx *ctx, int method) {
+    if (!ctx) return 0;
+    if (method != CICY_METHOD_PCG_F64 && method != CICY_METHOD_RESIDENT &&
+        method != CICY_METHOD_CONDENSED && method != CICY_METHOD_CONDENSED_RESIDENT) {
+        cicy::set_error(ctx, "drop-in method must be PCG_F64, RESIDENT, CONDENSED or "
+                             "CONDENSED_RESIDENT");
+        return 0;
+    }
+    ctx->dropin_method = method;
+    return 1;
+}
+
 int cicy_set_format(cicy_ctx *ctx, int format) {
     if (!ctx) return 0;
     if (format != 0 && format != kFormatSss && format != kFormatBand) {
@@ -917,12 +929,17 @@ int cicy_solve_dea_cg_sss(cicy_ctx *ctx, int64_t nodes, int64_t reserved_nodes, 
     local.true_rel_res = -1.0;
     if (n > 0) {
         cicy_sys *s = nullptr;
-        if (!cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, 0, &s)) {
+        const int method = ctx->dropin_method;
+        const int extras = method == CICY_METHOD_PCG_F64 ? 0
+                           : method == CICY_METHOD_RESIDENT
+                               ? cicy::kExtraResident
+                               : cicy::kExtraResident | cicy::kExtraCondensed;
+        if (!cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, extras, &s)) {
             if (stats) stats->status = CICY_BAD_INPUT;
             return 0;
         }
         bool ok = cicy_sys_set_rhs(s, R, voltages_sol, nullptr) &&
-                  cicy::solve_pcg64(s, tolerance, max_iter, CICY_REL_R0, &local) &&
+                  cicy_sys_solve(s, method, tolerance, max_iter, CICY_REL_R0, &local) &&
                   cicy_sys_get(s, voltages_sol, R);
         cicy_sys_destroy(s);
         if (!ok) {

@@ -40,7 +40,7 @@ EXPORTED_SYMBOLS = [
     "cicy_mpnn_forward", "cicy_mpnn_backward", "cicy_mpnn_adam_step", "cicy_mpnn_warm_start",
     "cicy_host_blocked_spmv", "cicy_set_resident_rows", "cicy_host_resident_apply",
     "cicy_host_condense", "cicy_host_expand", "cicy_host_core", "cicy_host_core_expand",
-    "cicy_sys_core",
+    "cicy_sys_core", "cicy_set_dropin_method",
 ]
 
 
@@ -71,6 +71,7 @@ def lib():
     L.cicy_set_rows_per_block.argtypes = [_VP, ctypes.c_int]
     L.cicy_set_format.argtypes = [_VP, ctypes.c_int]
     L.cicy_set_resident_rows.argtypes = [_VP, ctypes.c_int]
+    L.cicy_set_dropin_method.argtypes = [_VP, ctypes.c_int]
     L.cicy_host_resident_apply.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
                                            ctypes.c_int, ctypes.c_int, _F64, _F64,
                                            ctypes.c_double, _F64]
@@ -189,6 +190,10 @@ class Context:
 
     def set_format(self, fmt: int):
         self._check(self._L.cicy_set_format(self._h, fmt), "cicy_set_format")
+
+    def set_dropin_method(self, method: int):
+        """Method behind solve_dea_cg_sss (default METHOD_PCG_F64, the reference's)."""
+        self._check(self._L.cicy_set_dropin_method(self._h, method), "cicy_set_dropin_method")
 
     def set_resident_rows(self, rows_per_thread: int):
         """Rows per thread of the resident layout of systems created afterwards (0: auto)."""

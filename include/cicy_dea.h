@@ -82,6 +82,12 @@ int cicy_set_rows_per_block(cicy_ctx *ctx, int rows_per_block);
  * of the stored entries sit on the sub-diagonal, as for tube-by-tube numbering), 1 blocked
  * SSS with a remote transpose list, 2 band + remainder */
 int cicy_set_format(cicy_ctx *ctx, int format);
+/* method cicy_solve_dea_cg_sss solves with (default CICY_METHOD_PCG_F64, the reference's
+ * algorithm, iteration for iteration).  CICY_METHOD_CONDENSED / _CONDENSED_RESIDENT /
+ * _RESIDENT keep the call's contract (stop when ||r|| <= tol ||r0||, R out = b - A x) and are
+ * far faster on CNT networks; the iteration count then counts CG iterations of the system
+ * the method iterates on.  No counterpart in the reference. */
+int cicy_set_dropin_method(cicy_ctx *ctx, int method);
 /* rows per thread of the register-resident layout built for systems created afterwards
  * (0: the smallest of 4 8 16 24 32 40 44 48 whose grid is co-resident).  No counterpart in
  * the reference: a layout knob like cicy_set_rows_per_block. */

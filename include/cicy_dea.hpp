@@ -40,6 +40,9 @@ public:
     DirectElectrifyingGPU &operator=(const DirectElectrifyingGPU &) = delete;
 
     std::string last_error() const { return cicy_last_error(ctx_); }
+    // solver behind Solve_DEA_equations_CG_SSS: CICY_METHOD_PCG_F64 (default, the
+    // reference's algorithm) or CICY_METHOD_CONDENSED (same stopping rule, far faster)
+    bool use_method(int method) { return cicy_set_dropin_method(ctx_, method) != 0; }
     cicy_ctx *context() { return ctx_; }
 
     int Get_voltage_vector(const long int &reserved_nodes, const long int &nodes,

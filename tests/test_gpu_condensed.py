@@ -183,3 +183,25 @@ def test_condensed_zero_rhs_and_random_laplacian(ctx):
             sysd.solve(dea.METHOD_CONDENSED, 1e-10, 0, dea.REL_B)
     finally:
         sysd.close()
+
+
+def test_dropin_call_with_condensed_method(ctx, systems):
+    """Solve_DEA_equations_CG_SSS's call, solved by the condensed method: the reference's
+    stopping rule (||r|| <= tol ||r0||) holds for the true residual, R comes back as
+    b - A x and voltages = [VEF | x]."""
+    s = systems["rve6k"]
+    ctx.set_dropin_method(dea.METHOD_CONDENSED)
+    try:
+        ok, volt, x, R, st = ctx.solve_dea_cg_sss(s.nodes, s.reserved, 1e-9, s.col_ind,
+                                                  s.row_ptr, s.values, s.diag, s.r0, s.vef,
+                                                  s.x0)
+    finally:
+        ctx.set_dropin_method(dea.METHOD_PCG_F64)
+    assert ok and st.status == dea.CICY_OK
+    ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
+    r_true = s.b - ax
+    assert np.linalg.norm(r_true) <= 1e-9 * np.linalg.norm(s.r0) * (1 + 1e-6)
+    # R is r0 - A d on the device, r_true = b - A (x0 + d) on the host: equal up to the
+    # rounding of the two ways of forming it
+    assert np.linalg.norm(R - r_true) <= 0.05 * np.linalg.norm(r_true) + 1e-12 * np.linalg.norm(s.b)
+    assert np.array_equal(volt[:s.reserved], s.vef) and np.array_equal(volt[s.reserved:], x)
