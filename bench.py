@@ -282,6 +282,13 @@ def main():
                 "algorithmic_bytes": cib, "us_per_launch": cit,
                 "core_spmv": {"us": ms * 1e3, "algorithmic_bytes": cb,
                               "gbs": cb / (ms * 1e-3) / 1e9}}
+        # memory traffic of the same kernels from the committed counter runs (rocprofv3 PMC
+        # FETCH_SIZE + WRITE_SIZE, copy-kernel scaled) when they were taken on this workload
+        tpath = os.path.join(ROOT, "profiles", "traffic_5p6M.json")
+        if args.config == 3 and os.path.exists(tpath):
+            with open(tpath) as f:
+                roof["traffic"] = json.load(f)["core_iteration_traffic_bytes"]
+            roof["traffic_source"] = "profiles/condensed_5p6M_hbm_traffic.md"
         log(f"[bench] core ({cn} rows, {cl} entries): iteration {cit:.1f} us for "
             f"{cib / 1e6:.1f} MB algorithmic -> {roof['achieved']:.0f} GB/s; SpMV alone "
             f"{ms * 1e3:.1f} us")
