@@ -8,8 +8,9 @@ percolating direction of the largest cluster, R_flag = 1.  Targets are FP64 GPU 
 1e-12 (batched).  Training runs on the device (forward, backward and Adam kernels).
 
 Judging: held-out realisations (seeds 4_500_000 + k) are solved to ||b - Ax|| <= 1e-10 ||b||
-from the zero guess (the fallback and judge), from the reference's ramp and from the
-learned guess; the iterations of each are reported.
+with the plain FP64 PCG and with the condensed solve, each from the zero guess (the
+fallback and judge), from the reference's ramp and from the learned guess; the iterations
+of each are reported.
 """
 from __future__ import annotations
 
@@ -60,21 +61,25 @@ def solve_targets(ctx, items, tol=1e-12):
     return xs, ms
 
 
-def judge(ctx, model, items, method=dea.METHOD_MIXED, tol=1e-10):
+def judge(ctx, model, items, methods=(dea.METHOD_PCG_F64, dea.METHOD_CONDENSED), tol=1e-10):
+    """Iterations (and solve ms) of each method from the zero guess, the reference's ramp
+    and the learned guess, to ||b - A x|| <= tol ||b||."""
     rows = []
     for s, box, d, vapp in items:
         sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
-        res = {"n": s.n}
-        sysd.set_rhs(s.b, None, s.b)
-        st = sysd.solve(method, tol, 0, dea.REL_B)
-        res["zero"] = (st.iters, st.solve_ms, st.status)
-        sysd.set_rhs(s.r0 if s.has_guess else s.b, s.x0, s.b)
-        st = sysd.solve(method, tol, 0, dea.REL_B)
-        res["ramp"] = (st.iters, st.solve_ms, st.status)
         g = warmstart.Graph(ctx, s, box, d, vapp)
-        ms = model.warm_start(g, sysd)
-        st = sysd.solve(method, tol, 0, dea.REL_B)
-        res["learned"] = (st.iters, st.solve_ms + ms, st.status)
+        res = {"n": s.n}
+        for method in methods:
+            name = dea.METHOD_NAMES[method]
+            sysd.set_rhs(s.b, None, s.b)
+            st = sysd.solve(method, tol, 0, dea.REL_B)
+            res[f"{name}/zero"] = (st.iters, st.solve_ms, st.status)
+            sysd.set_rhs(s.r0 if s.has_guess else s.b, s.x0, s.b)
+            st = sysd.solve(method, tol, 0, dea.REL_B)
+            res[f"{name}/ramp"] = (st.iters, st.solve_ms, st.status)
+            ms = model.warm_start(g, sysd)
+            st = sysd.solve(method, tol, 0, dea.REL_B)
+            res[f"{name}/learned"] = (st.iters, st.solve_ms + ms, st.status)
         g.close()
         sysd.close()
         rows.append(res)
@@ -115,16 +120,19 @@ def main(argv=None):
         g.close()
     test = systems_for(4_500_000, args.n_test, args.min_cnt, args.max_cnt, 2)
     rows = judge(ctx, model, test)
-    for r in rows:
-        log(f"[judge] N={r['n']}: zero {r['zero'][0]} it, ramp {r['ramp'][0]} it, "
-            f"learned {r['learned'][0]} it")
-    zero = sum(r["zero"][0] for r in rows)
-    learned = sum(r["learned"][0] for r in rows)
-    ramp = sum(r["ramp"][0] for r in rows)
     summary = {"train_systems": len(graphs), "epochs": args.epochs, "train_seconds": train_s,
-               "loss_first": hist[0], "loss_last": hist[-1],
-               "iters_zero": zero, "iters_ramp": ramp, "iters_learned": learned,
-               "iters_saved_vs_zero": 1.0 - learned / zero}
+               "loss_first": hist[0], "loss_last": hist[-1]}
+    for key in rows[0]:
+        if key == "n":
+            continue
+        summary[f"iters {key}"] = int(sum(r[key][0] for r in rows))
+    for r in rows:
+        log("[judge] N=%d " % r["n"] + ", ".join(f"{k} {v[0]} it" for k, v in r.items()
+                                                 if k != "n"))
+    for name in ("pcg_f64", "condensed"):
+        z, l = summary.get(f"iters {name}/zero"), summary.get(f"iters {name}/learned")
+        if z:
+            summary[f"iters_saved_{name}"] = 1.0 - l / z
     log("[judge] " + json.dumps(summary))
     np.savez(args.out, theta=model.params, loss=np.array(hist),
              summary=json.dumps(summary), judge=json.dumps(rows))
