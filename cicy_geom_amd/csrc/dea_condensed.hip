@@ -64,6 +64,7 @@ namespace cicy {
 namespace {
 
 constexpr int kCondThreads = 256;
+constexpr int kFloating = -2;  // pl marker of a floating path (same value as kFloatingPath)
 
 // One sweep of T z = f per path, f = rhs on the path rows, minus the couplings to the
 // anchors when x is given (back-substitution): forward elimination with the stored
@@ -76,6 +77,10 @@ __global__ __launch_bounds__(kCondThreads) void k_path_solve(
     const int k = blockIdx.x * kCondThreads + threadIdx.x;
     if (k >= npath) return;
     const int s = ps[k], e = pe[k];
+    if (pl[k] == kFloating) {
+        for (int i = s; i <= e; i++) z[i] = 0.0;
+        return;
+    }
     double g = rhs[s];
     if (xanchor && pl[k] >= 0) g -= band[s] * xanchor[pl[k]];
     if (s == e && xanchor && pr[k] >= 0) g -= band[e + 1] * xanchor[pr[k]];
@@ -236,6 +241,9 @@ int condensed_info(const cicy_cond *c, int what) {
 // Host form of the condensation: anchors, paths with their LU factors, the reduced SSS
 // matrix S.  Returns false with *why set when condensing is not worth it, or with an error
 // (failed) when a pivot is not positive.
+// pl value of a path that touches no anchor and whose block is singular (floating tube)
+constexpr int kFloatingPath = -2;
+
 struct CondHost {
     int n = 0, m = 0;
     std::vector<int> anchor_of, anchors, ps, pe, pl, pr;
@@ -305,12 +313,27 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
             lfac[k] = band[k] / mfac[k - 1];
             mfac[k] = h.diag[k] - lfac[k] * band[k];
         }
+        // A pivot can only vanish on a path that touches no anchor and no electrode: a
+        // floating tube, singular on its own (its rows of A sum to zero).  Such a system is
+        // singular but consistent where b is zero there; CG from x0 leaves the component
+        // at x0, and so does the condensed solve (the path is marked and gets d = 0).
+        const double pivot_floor = 1e-12 * h.diag[e];
+        bool singular = false;
         for (int k = s; k <= e; k++)
-            if (!(mfac[k] > 0.0) || !std::isfinite(mfac[k])) {
-                set_error(ctx, "condensation: non-positive pivot on the path of rows %d..%d", s, e);
-                *failed = true;
+            if (!(mfac[k] > pivot_floor) || !std::isfinite(mfac[k])) singular = true;
+        if (singular) {
+            if (L >= 0 || R >= 0) {
+                *why = "non-positive pivot on a path coupled to the rest (not an M-matrix)";
                 return false;
             }
+            pl.back() = kFloatingPath;
+            for (int k = s; k <= e; k++) {
+                lfac[k] = 0.0;
+                mfac[k] = 1.0;
+            }
+            i = e + 1;
+            continue;
+        }
         if (L >= 0) {
             // u = T^-1 (a(s, L) e_s): forward sweep from g_s = a(s, L), then back-substitution
             g.assign((size_t)(e - s + 1), 0.0);
@@ -418,6 +441,9 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimD
         sel.clear();
         for (int k : cand) {
             if (!alive[k] || (int)adj[k].size() > max_degree) continue;
+            // never divide by a vanishing pivot (a floating group of anchors): keep the
+            // node in the core instead
+            if (!(d[k] > 1e-12 * std::fabs(C.sdiag[k])) || !std::isfinite(d[k])) continue;
             bool free = true;
             for (auto &e : adj[k])
                 if (taken[e.first]) free = false;
@@ -508,6 +534,10 @@ void host_series_back(const SeriesHost &Z, const double *f, double *x) {
 void host_path_solve(const CondHost &C, const double *rhs, const double *xanchor, double *z) {
     for (size_t k = 0; k < C.ps.size(); k++) {
         const int s = C.ps[k], e = C.pe[k];
+        if (C.pl[k] == kFloatingPath) {
+            for (int i = s; i <= e; i++) z[i] = 0.0;
+            continue;
+        }
         double g = rhs[s];
         if (xanchor && C.pl[k] >= 0) g -= C.band[s] * xanchor[C.pl[k]];
         if (s == e && xanchor && C.pr[k] >= 0) g -= C.band[e + 1] * xanchor[C.pr[k]];
@@ -585,9 +615,20 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
              up(ctx, c->e_nbr, Z.e_nbr) && up(ctx, c->e_a, Z.e_a) && up(ctx, c->e_d, Z.e_d) &&
              up(ctx, c->core_nodes, Z.core_nodes) &&
              hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
-        if (ok && Z.mc > 0)
+        if (ok && Z.mc > 0) {
+            // a core row with a vanishing diagonal (a floating group of tubes) is refused by
+            // the layout check: then there is simply no condensed system
+            bool positive = true;
+            for (double v : Z.cdiag)
+                if (!(v > 0.0) || !std::isfinite(v)) positive = false;
+            if (!positive) {
+                condensed_free(c);
+                *why = "the core system has a non-positive diagonal (floating component)";
+                return nullptr;
+            }
             ok = sys_create(ctx, Z.mc, (int64_t)Z.cval.size(), Z.crp.data(), Z.ccol.data(),
                             Z.cval.data(), Z.cdiag.data(), kExtraResident, &c->red) != 0;
+        }
     }
     if (!ok) {
         condensed_free(c);

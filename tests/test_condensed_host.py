@@ -151,6 +151,32 @@ def test_series_law(k):
     assert rdiag == pytest.approx([1.0 + 0.5 + geff] * 2, rel=1e-13)
 
 
+def test_floating_tube_is_left_at_zero():
+    """A tube touching no junction and no electrode is singular on its own; CG from zero
+    leaves it at zero and so does the condensation (the rest is solved exactly)."""
+    k, g = 6, 3.0
+    rp0, col0, val0, diag0 = chain_system(k, g)          # rows 0 .. k+1, definite
+    n0 = len(diag0)
+    m = 5                                                # floating chain rows n0 .. n0+m-1
+    rp = list(rp0)
+    col, val = list(col0), list(val0)
+    for r in range(m):
+        if r > 0:
+            col.append(n0 + r - 1)
+            val.append(-g)
+        rp.append(len(col))
+    rp, col, val = np.array(rp, np.int64), np.array(col, np.int64), np.array(val)
+    diag = np.concatenate([diag0, [g] + [2 * g] * (m - 2) + [g]])
+    rhs = np.zeros(n0 + m)
+    rhs[0] = 1.0
+    rows, crp, ccol, cval, cdiag, crhs, _ = dea.host_core(rp, col, val, diag, rhs)
+    xc = np.linalg.solve(dense(crp, ccol, cval, cdiag), crhs) if len(rows) else np.zeros(0)
+    x = dea.host_core_expand(rp, col, val, diag, rhs, xc)
+    assert np.all(x[n0:] == 0.0)
+    want = np.linalg.solve(dense(rp0, col0, val0, diag0), rhs[:n0])
+    assert np.allclose(x[:n0], want, rtol=1e-12)
+
+
 def test_nothing_to_condense_is_refused():
     rng = np.random.default_rng(3)
     n = 200
