@@ -84,7 +84,9 @@ def test_solution_and_true_residual(ctx, systems, truths, ref_errors, name, meth
               sysd.size("condensed_rows"), flush=True)
         assert st.status == dea.CICY_OK
         tr, err = check(s, x, truths[name], ref_errors[name])
-        assert abs(st.true_rel_res - tr) <= 1e-3 * tr + 1e-16
+        # both are ||b - A x|| / ||b|| in FP64, summed in different orders: they agree to
+        # 1e-3 relative, or to rounding (a few 1e-16) when the solve reaches machine precision
+        assert abs(st.true_rel_res - tr) <= 1e-3 * tr + 2e-15
         if method == dea.METHOD_RESIDENT:
             # the same CG as the two-kernel solve: iteration counts agree closely
             ref = sysd.solve(dea.METHOD_PCG_F64, 1e-10, 0, dea.REL_B)
