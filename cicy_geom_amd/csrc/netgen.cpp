@@ -1,4 +1,5 @@
-// netgen.cpp -- seeded straight-tube CNT networks and the DEA system assembly.
+// netgen.cpp -- seeded straight-tube CNT / square-plate GNP networks and the DEA system
+// assembly.
 //
 // Generation (the workload's own generator, not the reference's wavy-tube grower):
 // tubes with uniform centres, isotropic directions and U[len_min,len_max] lengths are
@@ -11,6 +12,25 @@
 // Cutoff_Wins::Where_is_with_boundary (Cutoff_Wins.cpp:716-739): x_min->4, x_max->2,
 // y_min->5, y_max->3, z_min->1, z_max->0.
 //
+// Nanoplatelets (GNPs): square plates (side gnp_side, thickness gnp_thick) with uniform
+// centres and isotropic normals, placed after the tubes with the same random stream; a
+// plate that would cross an earlier one, or has no vertex inside the cube, is drawn
+// again (the reference relocates GNPs to avoid interpenetration).  Plate points are
+// created in the reference's order: one window-boundary point per face the plate
+// crosses, the average of its edges' crossings with that face
+// (Cutoff_Wins::Find_gnp_boundary_points, Cutoff_Wins.cpp:871-988; crossings of cube
+// edges with plate faces are not added); plate-plate junction points (closest points of
+// the mid-plane squares moved onto the facing surfaces, gap = distance - (t1 + t2)/2
+// clamped at d_vdw); tube-plate junction points: every tube point whose gap to the plate
+// (distance to the cuboid - r, clamped at d_vdw; points inside a plate are ignored) is
+// within d_tunnel is a contact (Hoshen_Kopelman::Find_adjacent_labels, :1989-2142), and
+// the contacts of one tube on one plate are compressed to the closest one per run of
+// fewer than min_points points (Compress_mixed_contacts, :2181-2278), whose point on the
+// plate becomes the plate point.  Clusters join tubes and plates (union-find, smaller root
+// kept); boundary points on faces the cluster's family does not use are dropped from the
+// plate's structure (Remove_gnp_points_in_non_relevant_boundaries, :2740-2892), and each
+// plate of a cluster is triangulated in its mid-plane (Triangulation.cpp, gnp_geom.cpp).
+//
 // Assembly follows Direct_Electrifying.cpp:
 //   Get_global_nodes            :115-137   reserved nodes from the family
 //   Get_vector_of_boundaries    :238-305   boundary order -> reserved node index
@@ -22,11 +42,17 @@
 //   Add_to_existing_elements_in_2d_sparse_matrix :670-694 (off-diagonal accumulated)
 //   Fill_2d_matrices_cnt_junctions :738-852   junction drop rule for R_d == 2
 //   Calculate_junction_resistance  :854-901
+//   LM_matrix_for_gnps          :307-349   plate points -> nodes after the tube nodes
+//   Fill_2d_matrices_mixed_junctions :903-1019  tube-plate junctions (drop rule as above)
+//   Fill_2d_matrices_gnp        :1021-1196 triangulation edges, scanned in reverse
+//   Calculate_resistance_gnp    :1198-1232 rho = u.(rho_s, rho_s, rho_t)u in global axes
+//   Fill_2d_matrices_gnp_junctions :1234-1327 plate-plate junctions
 //   From_2d_to_1d_vectors       :1329-1394 SSS arrays, b = -K_EF^T VEF, reserved diag erased
 //   zero-diagonal check         :458-484
 //   Initial_guess_for_CG / Initial_guess_for_cnt_nodes :1396-1470 (ramp)
 //   Update_residual_vector      :1502-1535
-// and the boundary element rule of Electrical_analysis.cpp:499-605 for the current sums.
+// and the boundary element rules of Electrical_analysis.cpp:499-605 (tubes) and :607-675
+// (plates) for the current sums.
 // The per-row std::map of the reference is replaced by an insertion-ordered triplet list
 // that is stably sorted by (row, col) and folded with the same overwrite/accumulate
 // semantics, so values, diagonal and b are identical bit for bit.
@@ -39,6 +65,11 @@
 #include <numeric>
 #include <random>
 #include <vector>
+
+#include "geom3.h"
+
+using cicy_geom::Plate;
+using cicy_geom::seg_seg;
 
 namespace {
 
@@ -58,6 +89,18 @@ struct Tube {
     double step;      // polyline spacing
 };
 
+struct MixedJunction {
+    int64_t p_cnt, p_gnp;  // tube point, plate point
+    int64_t cnt, gnp;
+    double dist;
+};
+
+struct GnpJunction {
+    int64_t p1, p2;   // plate points
+    int64_t g1, g2;   // plates (g1 < g2)
+    double dist;
+};
+
 }  // namespace
 
 struct cicy_net {
@@ -74,12 +117,25 @@ struct cicy_net {
     std::vector<int> family;
     std::vector<int64_t> elem_ptr;      // tubes+1
     std::vector<int64_t> elem_pts;      // element points of each tube, ascending
+    // nanoplatelets
+    std::vector<Plate> gnps;
+    std::vector<double> gxyz;                      // plate points*3 (points_gnp)
+    std::vector<std::vector<int64_t>> gstruct;     // structure_gnp
+    std::vector<std::vector<std::pair<int64_t, int>>> gpts_bnd;  // (point, face) per plate
+    std::vector<std::vector<int64_t>> gtri;        // triangulation edges, 2 per edge
+    std::vector<int64_t> gbpts[6];                 // boundary_gnp_pts
+    std::vector<int64_t> gbnd[6];                  // boundary_gnp
+    std::vector<MixedJunction> mjunctions;
+    std::vector<GnpJunction> gjunctions;
+    std::vector<std::vector<int64_t>> cluster_gnps;    // plate ids, ascending
+    std::vector<std::vector<int64_t>> cluster_mixed;   // tube-plate junction ids
+    std::vector<std::vector<int64_t>> cluster_gjunc;   // plate-plate junction ids
 };
 
 struct cicy_sss_holder {
     cicy_sss_view v;
-    std::vector<int64_t> row_ptr, col_ind, lmm, node_point;
-    std::vector<double> values, diag, b, vef, x0, r0;
+    std::vector<int64_t> row_ptr, col_ind, lmm, node_point, lmm_gnp;
+    std::vector<double> values, diag, b, vef, x0, r0, node_xyz;
     std::vector<int64_t> e_bnd[2], e_int[2];
     std::vector<double> e_re[2];
 };
@@ -98,43 +154,6 @@ int face_of(const double *p, double box) {
     if (std::fabs(p[2] - 0.0) < kZero) return 1;
     if (std::fabs(p[2] - box) < kZero) return 0;
     return -1;
-}
-
-// Closest points of segments P0+s*d1 (s in [0,1]) and Q0+t*d2 (t in [0,1]); returns the
-// squared distance.  Derived from minimising |P(s)-Q(t)|^2 over the unit square.
-double seg_seg(const double *p0, const double *d1, const double *q0, const double *d2,
-               double &s, double &t) {
-    double r[3] = {p0[0] - q0[0], p0[1] - q0[1], p0[2] - q0[2]};
-    double a = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
-    double e = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
-    double f = d2[0] * r[0] + d2[1] * r[1] + d2[2] * r[2];
-    double c = d1[0] * r[0] + d1[1] * r[1] + d1[2] * r[2];
-    double bb = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
-    const double eps = 1e-30;
-    if (a <= eps && e <= eps) {
-        s = t = 0.0;
-    } else if (a <= eps) {
-        s = 0.0;
-        t = std::clamp(f / e, 0.0, 1.0);
-    } else if (e <= eps) {
-        t = 0.0;
-        s = std::clamp(-c / a, 0.0, 1.0);
-    } else {
-        double denom = a * e - bb * bb;
-        s = denom > 1e-14 * a * e ? std::clamp((bb * f - c * e) / denom, 0.0, 1.0) : 0.0;
-        t = (bb * s + f) / e;
-        if (t < 0.0) {
-            t = 0.0;
-            s = std::clamp(-c / a, 0.0, 1.0);
-        } else if (t > 1.0) {
-            t = 1.0;
-            s = std::clamp((bb - c) / a, 0.0, 1.0);
-        }
-    }
-    double dx = p0[0] + d1[0] * s - (q0[0] + d2[0] * t);
-    double dy = p0[1] + d1[1] * s - (q0[1] + d2[1] * t);
-    double dz = p0[2] + d1[2] * s - (q0[2] + d2[2] * t);
-    return dx * dx + dy * dy + dz * dz;
 }
 
 // Index (within the tube) of the polyline point nearest arc length `s_len`; points sit
@@ -197,13 +216,38 @@ double resistance_cnt(const cicy_net *net, int64_t p1, int64_t p2, double radius
     return rho * length / (kPi * radius * radius);
 }
 
+// Calculate_junction_resistance (Direct_Electrifying.cpp:854-901): l is a tube's radius
+// (doubled to its diameter) or a plate's thickness; the junction area is the square of
+// the smaller of the two.
+double junction_resistance_typed(double dist, double l1, bool cnt1, double l2, bool cnt2,
+                                 const cicy_elec_params *ep) {
+    if (!ep->junction_exponential) return ep->junction_resistance;
+    double jl1 = l1, jl2 = l2;
+    if (cnt1) jl1 = jl1 + l1;
+    if (cnt2) jl2 = jl2 + l2;
+    double a = (jl1 - jl2 < kZero) ? jl1 * jl1 : jl2 * jl2;
+    return ep->c1 * dist * std::exp(ep->c2 * dist) / a;
+}
+
 double junction_resistance(const Junction &j, double l1, double l2,
                            const cicy_elec_params *ep) {
-    // Calculate_junction_resistance (Direct_Electrifying.cpp:854-901); both sides CNTs
-    if (!ep->junction_exponential) return ep->junction_resistance;
-    double jl1 = l1 + l1, jl2 = l2 + l2;
-    double a = (jl1 - jl2 < kZero) ? jl1 * jl1 : jl2 * jl2;
-    return ep->c1 * j.dist * std::exp(ep->c2 * j.dist) / a;
+    return junction_resistance_typed(j.dist, l1, true, l2, true, ep);
+}
+
+// Calculate_resistance_gnp (Direct_Electrifying.cpp:1198-1232): the resistivity tensor is
+// diag(rho_s, rho_s, rho_t) in the global axes.  Returns false for a zero-length edge.
+bool resistance_gnp(const double *p1, const double *p2, double rad1, double rad2,
+                    const cicy_elec_params *ep, double *re) {
+    const double dx = p1[0] - p2[0], dy = p1[1] - p2[1], dz = p1[2] - p2[2];
+    const double len = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (len < kZero) return false;
+    const double ux = (p2[0] - p1[0]) / len, uy = (p2[1] - p1[1]) / len,
+                 uz = (p2[2] - p1[2]) / len;
+    const double rx = ux * ep->resistivity_gnp_surf, ry = uy * ep->resistivity_gnp_surf,
+                 rz = uz * ep->resistivity_gnp_t;
+    const double rho = rx * ux + ry * uy + rz * uz;
+    *re = rho * len / (kPi * rad1 * rad2);
+    return true;
 }
 
 struct Trip {
@@ -211,6 +255,274 @@ struct Trip {
     double val;
     int add;  // 0 overwrite (= -g), 1 accumulate (-= g)
 };
+
+}  // namespace
+
+namespace {
+
+constexpr double kSqrt2 = 1.4142135623730951;
+
+// Uniform grid over [0,box]^3 with cells of at least `min_cell`.
+struct Grid {
+    int64_t nc = 1;
+    double cs = 1.0;
+    Grid(double box, double min_cell) {
+        nc = std::max<int64_t>(1, (int64_t)std::floor(box / min_cell));
+        cs = box / (double)nc;
+    }
+    int64_t at(double v) const {
+        return std::clamp<int64_t>((int64_t)std::floor(v / cs), 0, nc - 1);
+    }
+    int64_t index(const double *p) const { return (at(p[0]) * nc + at(p[1])) * nc + at(p[2]); }
+};
+
+int64_t add_gnp_point(cicy_net *net, int64_t g, const double *x) {
+    const int64_t id = (int64_t)net->gxyz.size() / 3;
+    net->gxyz.insert(net->gxyz.end(), x, x + 3);
+    net->gstruct[(size_t)g].push_back(id);
+    return id;
+}
+
+// Plates with uniform centres, isotropic normals and a uniform in-plane angle, drawn again
+// while no vertex lies strictly inside the cube.  Plates may cross each other: the
+// generator does not relocate them (the reference's does), and crossing plates are in
+// contact at the van der Waals gap.
+void place_gnps(cicy_net *net, std::mt19937_64 &g) {
+    const cicy_net_params &p = net->prm;
+    if (p.n_gnp <= 0) return;
+    const double box = p.box, l = p.gnp_side, t = p.gnp_thick;
+    while ((int64_t)net->gnps.size() < p.n_gnp) {
+        Plate P;
+        P.l = l;
+        P.t = t;
+        for (int a = 0; a < 3; a++) P.c[a] = u01(g) * box;
+        const double cth = 2.0 * u01(g) - 1.0, phi = 2.0 * kPi * u01(g),
+                     psi = 2.0 * kPi * u01(g);
+        const double sth = std::sqrt(std::max(0.0, 1.0 - cth * cth));
+        const double n[3] = {sth * std::cos(phi), sth * std::sin(phi), cth};
+        const double e[3] = {-std::sin(phi), std::cos(phi), 0.0};
+        const double f[3] = {-cth * std::cos(phi), -cth * std::sin(phi), sth};
+        for (int a = 0; a < 3; a++) {
+            P.ax[0][a] = std::cos(psi) * e[a] + std::sin(psi) * f[a];
+            P.ax[2][a] = n[a];
+        }
+        P.ax[1][0] = n[1] * P.ax[0][2] - n[2] * P.ax[0][1];
+        P.ax[1][1] = n[2] * P.ax[0][0] - n[0] * P.ax[0][2];
+        P.ax[1][2] = n[0] * P.ax[0][1] - n[1] * P.ax[0][0];
+        bool inside = false;
+        for (int v = 0; v < 8 && !inside; v++) {
+            double x[3];
+            cicy_geom::plate_vertex(P, v, x);
+            inside = x[0] > 0.0 && x[0] < box && x[1] > 0.0 && x[1] < box && x[2] > 0.0 &&
+                     x[2] < box;
+        }
+        if (inside) net->gnps.push_back(P);
+    }
+    const size_t G = net->gnps.size();
+    net->gstruct.assign(G, {});
+    net->gpts_bnd.assign(G, {});
+    net->gtri.assign(G, {});
+    // window-boundary points, face by face (Cutoff_Wins.cpp:871-988)
+    for (size_t gi = 0; gi < G; gi++) {
+        std::vector<double> acc[6];
+        cicy_geom::plate_window_crossings(net->gnps[gi], box, acc);
+        for (int f = 0; f < 6; f++) {
+            if (acc[f].empty()) continue;
+            const size_t m = acc[f].size() / 3;
+            double avg[3] = {0.0, 0.0, 0.0};
+            for (size_t q = 0; q < m; q++)
+                for (int a = 0; a < 3; a++) avg[a] = avg[a] + acc[f][3 * q + a];
+            for (int a = 0; a < 3; a++) avg[a] = avg[a] / (double)m;
+            const int64_t id = add_gnp_point(net, (int64_t)gi, avg);
+            net->gbnd[f].push_back((int64_t)gi);
+            net->gbpts[f].push_back(id);
+            net->gpts_bnd[gi].push_back({id, f});
+        }
+    }
+}
+
+// Plate-plate junctions: the closest points of the mid-plane squares, moved onto the
+// facing surfaces; gap = distance - (t1 + t2)/2, clamped at d_vdw.
+void gnp_gnp_junctions(cicy_net *net) {
+    const cicy_net_params &p = net->prm;
+    const size_t G = net->gnps.size();
+    if (G < 2) return;
+    Grid grid(p.box, kSqrt2 * p.gnp_side + p.gnp_thick + p.d_tunnel);
+    std::vector<std::vector<int32_t>> cells((size_t)(grid.nc * grid.nc * grid.nc));
+    for (size_t gi = 0; gi < G; gi++) cells[(size_t)grid.index(net->gnps[gi].c)].push_back((int32_t)gi);
+    std::vector<std::pair<int32_t, int32_t>> pairs;
+    for (size_t gi = 0; gi < G; gi++) {
+        const double *c = net->gnps[gi].c;
+        const int64_t ci[3] = {grid.at(c[0]), grid.at(c[1]), grid.at(c[2])};
+        for (int64_t i = ci[0] - 1; i <= ci[0] + 1; i++)
+            for (int64_t j = ci[1] - 1; j <= ci[1] + 1; j++)
+                for (int64_t m = ci[2] - 1; m <= ci[2] + 1; m++) {
+                    if (i < 0 || j < 0 || m < 0 || i >= grid.nc || j >= grid.nc || m >= grid.nc)
+                        continue;
+                    for (int32_t q : cells[(size_t)((i * grid.nc + j) * grid.nc + m)])
+                        if ((size_t)q > gi) pairs.emplace_back((int32_t)gi, q);
+                }
+    }
+    std::sort(pairs.begin(), pairs.end());
+    for (auto &pr : pairs) {
+        const Plate &A = net->gnps[(size_t)pr.first], &B = net->gnps[(size_t)pr.second];
+        double pa[3], pb[3];
+        const double d = cicy_geom::midplane_distance(A, B, pa, pb);
+        double gap = d - 0.5 * A.t - 0.5 * B.t;
+        if (gap < p.d_vdw) gap = p.d_vdw;
+        if (gap - p.d_tunnel > kZero) continue;
+        const double ab[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+        const double sa = cicy_geom::dot3(A.ax[2], ab) >= 0.0 ? 1.0 : -1.0;
+        const double sb = cicy_geom::dot3(B.ax[2], ab) <= 0.0 ? 1.0 : -1.0;
+        double xa[3], xb[3];
+        for (int a = 0; a < 3; a++) {
+            xa[a] = pa[a] + sa * 0.5 * A.t * A.ax[2][a];
+            xb[a] = pb[a] + sb * 0.5 * B.t * B.ax[2][a];
+        }
+        GnpJunction J;
+        J.p1 = add_gnp_point(net, pr.first, xa);
+        J.p2 = add_gnp_point(net, pr.second, xb);
+        J.g1 = pr.first;
+        J.g2 = pr.second;
+        J.dist = gap;
+        net->gjunctions.push_back(J);
+    }
+}
+
+// Tube-plate junctions (Find_adjacent_labels + Compress_mixed_contacts).
+void mixed_junctions(cicy_net *net) {
+    const cicy_net_params &p = net->prm;
+    const size_t G = net->gnps.size();
+    if (G == 0) return;
+    const double reach = p.radius + p.d_tunnel + 1e-9;
+    Grid grid(p.box, std::min(p.box, 0.1));
+    const double half_diag = 0.5 * std::sqrt(3.0) * grid.cs;
+    // register every plate in the cells it can reach
+    std::vector<std::pair<int64_t, int32_t>> reg;
+    for (size_t gi = 0; gi < G; gi++) {
+        const Plate &P = net->gnps[gi];
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int v = 0; v < 8; v++) {
+            double x[3];
+            cicy_geom::plate_vertex(P, v, x);
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], x[a] - reach);
+                hi[a] = std::max(hi[a], x[a] + reach);
+            }
+        }
+        for (int64_t i = grid.at(lo[0]); i <= grid.at(hi[0]); i++)
+            for (int64_t j = grid.at(lo[1]); j <= grid.at(hi[1]); j++)
+                for (int64_t m = grid.at(lo[2]); m <= grid.at(hi[2]); m++) {
+                    const double cc[3] = {(i + 0.5) * grid.cs, (j + 0.5) * grid.cs,
+                                          (m + 0.5) * grid.cs};
+                    double q[3], dist;
+                    cicy_geom::plate_closest_point(P, cc, q, &dist);
+                    if (dist <= half_diag + reach)
+                        reg.emplace_back((i * grid.nc + j) * grid.nc + m, (int32_t)gi);
+                }
+    }
+    std::sort(reg.begin(), reg.end());
+    struct Contact {
+        int64_t g, t, pt;
+        double gap;
+        double q[3];
+    };
+    std::vector<Contact> cont;
+    const int64_t T = (int64_t)net->tubes.size();
+    for (int64_t t = 0; t < T; t++)
+        for (int64_t pt = net->cnt_ptr[t]; pt < net->cnt_ptr[t + 1]; pt++) {
+            const double *x = &net->xyz[3 * pt];
+            const int64_t ci = grid.index(x);
+            auto it = std::lower_bound(reg.begin(), reg.end(), std::make_pair(ci, (int32_t)-1));
+            for (; it != reg.end() && it->first == ci; ++it) {
+                Contact c;
+                double dist;
+                if (!cicy_geom::plate_closest_point(net->gnps[(size_t)it->second], x, c.q, &dist))
+                    continue;  // a tube point inside a plate makes no contact
+                double gap = dist - net->radius[(size_t)t];
+                if (gap < p.d_vdw) gap = p.d_vdw;
+                if (gap - p.d_tunnel > kZero) continue;
+                c.g = it->second;
+                c.t = t;
+                c.pt = pt;
+                c.gap = gap;
+                cont.push_back(c);
+            }
+        }
+    std::sort(cont.begin(), cont.end(), [](const Contact &a, const Contact &b) {
+        if (a.g != b.g) return a.g < b.g;
+        if (a.t != b.t) return a.t < b.t;
+        return a.pt < b.pt;
+    });
+    auto emit = [&](const Contact &c) {
+        MixedJunction J;
+        J.p_cnt = c.pt;
+        J.p_gnp = add_gnp_point(net, c.g, c.q);
+        J.cnt = c.t;
+        J.gnp = c.g;
+        J.dist = c.gap;
+        net->mjunctions.push_back(J);
+    };
+    for (size_t s = 0; s < cont.size();) {
+        size_t e = s;
+        while (e < cont.size() && cont[e].g == cont[s].g && cont[e].t == cont[s].t) e++;
+        // runs of contacts closer than min_points points to the run's first point; the
+        // closest contact of each run is the junction
+        int64_t p0 = cont[s].pt;
+        size_t best = s;
+        for (size_t q = s + 1; q < e; q++) {
+            if (cont[q].pt - p0 >= p.min_points) {
+                emit(cont[best]);
+                best = q;
+                p0 = cont[q].pt;
+            } else if (cont[q].gap < cont[best].gap) {
+                best = q;
+            }
+        }
+        emit(cont[best]);
+        s = e;
+    }
+}
+
+// Boundary points on faces the cluster's family does not use leave the plate's structure;
+// then every plate of a cluster is triangulated in its mid-plane.
+void prune_and_triangulate(cicy_net *net) {
+    std::vector<int> faces;
+    std::vector<double> uv;
+    std::vector<std::pair<int, int>> edges;
+    for (size_t c = 0; c < net->cluster_gnps.size(); c++) {
+        const int fam = net->family[c];
+        if (fam >= 0 && fam <= 5) {
+            faces = boundaries_of_family(fam);
+            for (int64_t g : net->cluster_gnps[c]) {
+                std::vector<int64_t> &st = net->gstruct[(size_t)g];
+                for (auto &pb : net->gpts_bnd[(size_t)g]) {
+                    if (std::find(faces.begin(), faces.end(), pb.second) != faces.end()) continue;
+                    auto it = std::find(st.begin(), st.end(), pb.first);
+                    if (it != st.end()) st.erase(it);
+                }
+            }
+        }
+        for (int64_t g : net->cluster_gnps[c]) {
+            const Plate &P = net->gnps[(size_t)g];
+            const std::vector<int64_t> &st = net->gstruct[(size_t)g];
+            uv.clear();
+            for (int64_t id : st) {
+                double loc[3];
+                cicy_geom::to_local(P, &net->gxyz[3 * (size_t)id], loc);
+                uv.push_back(loc[0]);
+                uv.push_back(loc[1]);
+            }
+            cicy_geom::plate_triangulation(P.l, uv, edges);
+            std::vector<int64_t> &tri = net->gtri[(size_t)g];
+            tri.clear();
+            for (auto &e : edges) {
+                tri.push_back(st[(size_t)e.first]);
+                tri.push_back(st[(size_t)e.second]);
+            }
+        }
+    }
+}
 
 }  // namespace
 
@@ -225,7 +537,8 @@ void cicy_tunnel_constants(double h, double e, double m, double lambda, double *
 
 cicy_net *cicy_net_generate(const cicy_net_params *p) {
     if (!p || p->box <= 0 || p->n_cnt < 0 || p->seg_len <= 0 || p->radius <= 0 ||
-        p->len_min <= 0 || p->len_max < p->len_min)
+        p->len_min <= 0 || p->len_max < p->len_min || p->n_gnp < 0 ||
+        (p->n_gnp > 0 && (p->gnp_side <= 0 || p->gnp_thick <= 0 || p->min_points < 1)))
         return nullptr;
     cicy_net *net = new cicy_net();
     net->prm = *p;
@@ -291,6 +604,8 @@ cicy_net *cicy_net_generate(const cicy_net_params *p) {
         made++;
     }
     const int64_t T = made;
+    place_gnps(net, g);
+    const int64_t G = (int64_t)net->gnps.size();
 
     // ---- contact search: cell list over polyline segments, expanded by half the
     // contact distance, so any two axes closer than the contact distance share a cell
@@ -361,41 +676,60 @@ cicy_net *cicy_net_generate(const cicy_net_params *p) {
         net->junctions.push_back(J);
     }
 
-    // ---- clusters: union-find keeping the smaller root (as Merge_labels does,
-    // Hoshen_Kopelman.cpp:2977-2985); clusters need >= 2 tubes (isolated tubes are
-    // never clusters)
-    std::vector<int64_t> par(T);
+    gnp_gnp_junctions(net);
+    mixed_junctions(net);
+
+    // ---- clusters: union-find over tubes (0..T-1) and plates (T..T+G-1) keeping the
+    // smaller root (as Merge_labels does, Hoshen_Kopelman.cpp:2977-2985); clusters need
+    // >= 2 particles (isolated particles are never clusters)
+    std::vector<int64_t> par(T + G);
     std::iota(par.begin(), par.end(), 0);
-    std::vector<int64_t> deg(T, 0);
-    for (const Junction &J : net->junctions) {
-        int64_t a = find_root(par, J.n1), b = find_root(par, J.n2);
+    std::vector<int64_t> deg(T + G, 0);
+    auto unite = [&](int64_t x, int64_t y) {
+        int64_t a = find_root(par, x), b = find_root(par, y);
         if (a != b) {
             if (a < b) par[b] = a; else par[a] = b;
         }
-        deg[J.n1]++;
-        deg[J.n2]++;
-    }
-    std::vector<int64_t> cid(T, -1);
-    for (int64_t t = 0; t < T; t++) {
-        if (!deg[t]) continue;
-        int64_t r = find_root(par, t);
+        deg[x]++;
+        deg[y]++;
+    };
+    for (const Junction &J : net->junctions) unite(J.n1, J.n2);
+    for (const GnpJunction &J : net->gjunctions) unite(T + J.g1, T + J.g2);
+    for (const MixedJunction &J : net->mjunctions) unite(J.cnt, T + J.gnp);
+    std::vector<int64_t> cid(T + G, -1);
+    for (int64_t x = 0; x < T + G; x++) {
+        if (!deg[x]) continue;
+        int64_t r = find_root(par, x);
         if (cid[r] < 0) {
             cid[r] = (int64_t)net->clusters.size();
             net->clusters.emplace_back();
+            net->cluster_gnps.emplace_back();
         }
-        cid[t] = cid[r];
-        net->clusters[cid[t]].push_back(t);
+        cid[x] = cid[r];
+        if (x < T) net->clusters[cid[x]].push_back(x);
+        else net->cluster_gnps[cid[x]].push_back(x - T);
     }
-    net->cluster_junctions.resize(net->clusters.size());
+    const size_t NC = net->clusters.size();
+    net->cluster_junctions.resize(NC);
+    net->cluster_mixed.resize(NC);
+    net->cluster_gjunc.resize(NC);
     for (int64_t jx = 0; jx < (int64_t)net->junctions.size(); jx++)
         net->cluster_junctions[cid[find_root(par, net->junctions[jx].n1)]].push_back(jx);
-    // percolation families (0 X, 1 Y, 2 Z, 3 XY, 4 XZ, 5 YZ, 6 XYZ)
-    std::vector<uint8_t> touch(T, 0);
-    for (int f = 0; f < 6; f++)
+    for (int64_t jx = 0; jx < (int64_t)net->mjunctions.size(); jx++)
+        net->cluster_mixed[cid[find_root(par, net->mjunctions[jx].cnt)]].push_back(jx);
+    for (int64_t jx = 0; jx < (int64_t)net->gjunctions.size(); jx++)
+        net->cluster_gjunc[cid[find_root(par, T + net->gjunctions[jx].g1)]].push_back(jx);
+    // percolation families (0 X, 1 Y, 2 Z, 3 XY, 4 XZ, 5 YZ, 6 XYZ) from the tubes and
+    // plates at each face (boundary_cnt, boundary_gnp)
+    std::vector<uint8_t> touch(T + G, 0);
+    for (int f = 0; f < 6; f++) {
         for (int64_t t : net->bcnt[f]) touch[t] |= (uint8_t)(1u << f);
-    for (auto &cl : net->clusters) {
+        for (int64_t gp : net->gbnd[f]) touch[T + gp] |= (uint8_t)(1u << f);
+    }
+    for (size_t c = 0; c < NC; c++) {
         uint8_t m = 0;
-        for (int64_t t : cl) m |= touch[t];
+        for (int64_t t : net->clusters[c]) m |= touch[t];
+        for (int64_t gp : net->cluster_gnps[c]) m |= touch[T + gp];
         bool px = (m & (1 << 2)) && (m & (1 << 4));
         bool py = (m & (1 << 3)) && (m & (1 << 5));
         bool pz = (m & (1 << 0)) && (m & (1 << 1));
@@ -418,6 +752,7 @@ cicy_net *cicy_net_generate(const cicy_net_params *p) {
             extra[J.n1].push_back(J.p1);
             extra[J.n2].push_back(J.p2);
         }
+        for (const MixedJunction &J : net->mjunctions) extra[J.cnt].push_back(J.p_cnt);
     }
     net->elem_ptr.push_back(0);
     for (int64_t t = 0; t < T; t++) {
@@ -434,6 +769,7 @@ cicy_net *cicy_net_generate(const cicy_net_params *p) {
         }
         net->elem_ptr.push_back((int64_t)net->elem_pts.size());
     }
+    prune_and_triangulate(net);
     return net;
 }
 
@@ -446,6 +782,10 @@ int64_t cicy_net_count(const cicy_net *net, int what) {
         case 1: return (int64_t)net->radius.size();
         case 2: return (int64_t)net->junctions.size();
         case 3: return (int64_t)net->clusters.size();
+        case 4: return (int64_t)net->gnps.size();
+        case 5: return (int64_t)net->gxyz.size() / 3;
+        case 6: return (int64_t)net->mjunctions.size();
+        case 7: return (int64_t)net->gjunctions.size();
         default: return -1;
     }
 }
@@ -500,6 +840,79 @@ int64_t cicy_net_elements(const cicy_net *net, int64_t t, const int64_t **pts) {
     return net->elem_ptr[t + 1] - net->elem_ptr[t];
 }
 
+const double *cicy_net_gnp_xyz(const cicy_net *net) { return net ? net->gxyz.data() : nullptr; }
+
+int cicy_net_gnp(const cicy_net *net, int64_t g, double *centre, double *axes, double *side,
+                 double *thick) {
+    if (!net || g < 0 || g >= (int64_t)net->gnps.size()) return 0;
+    const Plate &P = net->gnps[(size_t)g];
+    for (int a = 0; a < 3; a++) {
+        centre[a] = P.c[a];
+        for (int b = 0; b < 3; b++) axes[3 * a + b] = P.ax[a][b];
+    }
+    *side = P.l;
+    *thick = P.t;
+    return 1;
+}
+
+int64_t cicy_net_gnp_structure(const cicy_net *net, int64_t g, const int64_t **pts) {
+    if (!net || g < 0 || g >= (int64_t)net->gstruct.size()) return -1;
+    *pts = net->gstruct[(size_t)g].data();
+    return (int64_t)net->gstruct[(size_t)g].size();
+}
+
+int64_t cicy_net_gnp_triangulation(const cicy_net *net, int64_t g, const int64_t **edges) {
+    if (!net || g < 0 || g >= (int64_t)net->gtri.size()) return -1;
+    *edges = net->gtri[(size_t)g].data();
+    return (int64_t)net->gtri[(size_t)g].size() / 2;
+}
+
+int64_t cicy_net_gnp_boundary_points(const cicy_net *net, int b, const int64_t **pts) {
+    if (!net || b < 0 || b > 5) return -1;
+    *pts = net->gbpts[b].data();
+    return (int64_t)net->gbpts[b].size();
+}
+
+int64_t cicy_net_boundary_gnps(const cicy_net *net, int b, const int64_t **gnps) {
+    if (!net || b < 0 || b > 5) return -1;
+    *gnps = net->gbnd[b].data();
+    return (int64_t)net->gbnd[b].size();
+}
+
+int cicy_net_mixed_junction(const cicy_net *net, int64_t j, int64_t *p_cnt, int64_t *p_gnp,
+                            int64_t *cnt, int64_t *gnp, double *dist) {
+    if (!net || j < 0 || j >= (int64_t)net->mjunctions.size()) return 0;
+    const MixedJunction &J = net->mjunctions[(size_t)j];
+    *p_cnt = J.p_cnt; *p_gnp = J.p_gnp; *cnt = J.cnt; *gnp = J.gnp; *dist = J.dist;
+    return 1;
+}
+
+int cicy_net_gnp_junction(const cicy_net *net, int64_t j, int64_t *p1, int64_t *p2,
+                          int64_t *g1, int64_t *g2, double *dist) {
+    if (!net || j < 0 || j >= (int64_t)net->gjunctions.size()) return 0;
+    const GnpJunction &J = net->gjunctions[(size_t)j];
+    *p1 = J.p1; *p2 = J.p2; *g1 = J.g1; *g2 = J.g2; *dist = J.dist;
+    return 1;
+}
+
+int64_t cicy_net_cluster_gnps(const cicy_net *net, int64_t k, const int64_t **gnps) {
+    if (!net || k < 0 || k >= (int64_t)net->cluster_gnps.size()) return -1;
+    *gnps = net->cluster_gnps[(size_t)k].data();
+    return (int64_t)net->cluster_gnps[(size_t)k].size();
+}
+
+int64_t cicy_net_cluster_mixed_junctions(const cicy_net *net, int64_t k, const int64_t **junc) {
+    if (!net || k < 0 || k >= (int64_t)net->cluster_mixed.size()) return -1;
+    *junc = net->cluster_mixed[(size_t)k].data();
+    return (int64_t)net->cluster_mixed[(size_t)k].size();
+}
+
+int64_t cicy_net_cluster_gnp_junctions(const cicy_net *net, int64_t k, const int64_t **junc) {
+    if (!net || k < 0 || k >= (int64_t)net->cluster_gjunc.size()) return -1;
+    *junc = net->cluster_gjunc[(size_t)k].data();
+    return (int64_t)net->cluster_gjunc[(size_t)k].size();
+}
+
 cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int family,
                                  const cicy_elec_params *ep, int direction, char *err,
                                  int64_t err_len) {
@@ -534,6 +947,18 @@ cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int famil
         int64_t pl = net->elem_pts[b - 1];
         if (h->lmm[pl] < 0) h->lmm[pl] = gn++;
     }
+    // LM_matrix_for_gnps (Direct_Electrifying.cpp:307-349): boundary plate points of the
+    // family's faces take the reserved nodes, then the cluster's plates number their points
+    const int64_t GP = (int64_t)net->gxyz.size() / 3;
+    h->lmm_gnp.assign((size_t)GP, -1);
+    std::vector<int64_t> &lmm_g = h->lmm_gnp;
+    for (size_t nidx = 0; nidx < bnds.size(); nidx++)
+        for (int64_t q : net->gbpts[bnds[nidx]]) lmm_g[q] = (int64_t)nidx;
+    const auto &clg = net->cluster_gnps[k];
+    const int64_t first_gnp_node = gn;
+    for (int64_t gp : clg)
+        for (int64_t q : net->gstruct[gp])
+            if (lmm_g[q] < 0) lmm_g[q] = gn++;
     const int64_t nodes = gn, n = nodes - rd;
     std::vector<double> diag_full((size_t)nodes, 0.0);
     std::vector<Trip> trips;
@@ -587,6 +1012,59 @@ cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int famil
         if (n1 >= rd && n2 >= rd) add_new(n1, n2, g);
         else if (rd > 2) add_existing(n1, n2, g);
     }
+    // Fill_2d_matrices_mixed_junctions (:903-1019); points_cnt_rad records the tube radius
+    // at the plate point of each junction (negative: absent)
+    std::vector<double> cnt_rad((size_t)GP, -1.0);
+    for (int64_t jx : net->cluster_mixed[k]) {
+        const MixedJunction &J = net->mjunctions[jx];
+        int64_t n1 = h->lmm[J.p_cnt], n2 = lmm_g[J.p_gnp];
+        if (n1 < 0 || n2 < 0) continue;
+        double g = 1.0;
+        if (r_flag == 1)
+            g = 1 / junction_resistance_typed(J.dist, net->radius[J.cnt], true,
+                                              net->gnps[J.gnp].t, false, ep);
+        if (n1 >= rd && n2 >= rd) add_new(n1, n2, g);
+        else if (rd > 2) add_existing(n1, n2, g);
+        cnt_rad[J.p_gnp] = net->radius[J.cnt];
+    }
+    // Fill_2d_matrices_gnp (:1021-1196): triangulation edges from the last to the first; an
+    // end without a tube junction uses half the plate thickness as its radius
+    const double *gx = net->gxyz.data();
+    for (int64_t gp : clg) {
+        const std::vector<int64_t> &tri = net->gtri[gp];
+        const double half_t = net->gnps[gp].t / 2;
+        for (int64_t j = (int64_t)tri.size() / 2 - 1; j >= 0; j--) {
+            const int64_t v1 = tri[2 * j], v2 = tri[2 * j + 1];
+            const int64_t n1 = lmm_g[v1], n2 = lmm_g[v2];
+            if (n1 < 0 || n2 < 0) {
+                delete h;
+                return fail("Fill_2d_matrices_gnp: triangulation vertex without a node");
+            }
+            double g = 1.0;
+            if (r_flag == 1) {
+                const double rad1 = cnt_rad[v1] >= 0 ? cnt_rad[v1] : half_t;
+                const double rad2 = cnt_rad[v2] >= 0 ? cnt_rad[v2] : half_t;
+                double re;
+                if (!resistance_gnp(gx + 3 * v1, gx + 3 * v2, rad1, rad2, ep, &re)) {
+                    delete h;
+                    return fail("Calculate_resistance_gnp: triangulation edge of zero length");
+                }
+                g = 1 / re;
+            }
+            add_new(n1, n2, g);
+        }
+    }
+    // Fill_2d_matrices_gnp_junctions (:1234-1327): no reserved-node rule here
+    for (int64_t jx : net->cluster_gjunc[k]) {
+        const GnpJunction &J = net->gjunctions[jx];
+        int64_t n1 = lmm_g[J.p1], n2 = lmm_g[J.p2];
+        if (n1 < 0 || n2 < 0) continue;
+        double g = 1.0;
+        if (r_flag == 1)
+            g = 1 / junction_resistance_typed(J.dist, net->gnps[J.g1].t, false,
+                                              net->gnps[J.g2].t, false, ep);
+        add_new(n1, n2, g);
+    }
     // From_2d_to_1d_vectors: stable sort keeps insertion order inside each (row, col)
     std::stable_sort(trips.begin(), trips.end(), [](const Trip &x, const Trip &y) {
         return x.row != y.row ? x.row < y.row : x.col < y.col;
@@ -625,12 +1103,22 @@ cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int famil
             return fail("Fill_sparse_stiffness_matrix: zero element on the diagonal (floating node)");
         }
     }
-    // node -> representative point, for node features
+    // node -> representative point (tube nodes), and every node's position
     h->node_point.assign((size_t)nodes, -1);
+    h->node_xyz.assign((size_t)nodes * 3, NAN);
     for (int64_t t : cl)
         for (int64_t q = net->elem_ptr[t]; q < net->elem_ptr[t + 1]; q++) {
             int64_t nd = h->lmm[net->elem_pts[q]];
-            if (nd >= rd && h->node_point[nd] < 0) h->node_point[nd] = net->elem_pts[q];
+            if (nd >= rd && h->node_point[nd] < 0) {
+                h->node_point[nd] = net->elem_pts[q];
+                for (int a = 0; a < 3; a++) h->node_xyz[3 * nd + a] = net->xyz[3 * net->elem_pts[q] + a];
+            }
+        }
+    for (int64_t gp : clg)
+        for (int64_t q : net->gstruct[gp]) {
+            int64_t nd = lmm_g[q];
+            if (nd >= first_gnp_node)
+                for (int a = 0; a < 3; a++) h->node_xyz[3 * nd + a] = gx[3 * q + a];
         }
     // Initial_guess_for_CG: ramp, only when R_flag && R_d == 2 (Direct_Electrifying.cpp:80)
     h->x0.assign((size_t)n, 0.0);
@@ -686,6 +1174,35 @@ cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int famil
                     h->e_re[w].push_back(re);
                 }
             }
+            // Currents_through_boundary_gnps (Electrical_analysis.cpp:607-675): the
+            // triangulation edges of the boundary's plates that touch node 0 or 1, with
+            // node = 0 for b1 and 1 for b2 (plates outside the cluster have no nodes here)
+            const int64_t node = w;
+            for (int64_t gp : net->gbnd[bb[w]]) {
+                if (!std::binary_search(clg.begin(), clg.end(), gp)) continue;
+                const std::vector<int64_t> &tri = net->gtri[gp];
+                const double half_t = net->gnps[gp].t / 2;
+                for (size_t j = 0; j < tri.size() / 2; j++) {
+                    const int64_t v1 = tri[2 * j], v2 = tri[2 * j + 1];
+                    const int64_t na = lmm_g[v1], nb = lmm_g[v2];
+                    if (na > 1 && nb > 1) continue;
+                    if (na == nb) {
+                        delete h;
+                        return fail("Currents_through_boundary_gnps: both nodes at the same boundary");
+                    }
+                    const int64_t n1 = (na == node) ? na : nb, n2 = (na == node) ? nb : na;
+                    const double rad1 = cnt_rad[v1] >= 0 ? cnt_rad[v1] : half_t;
+                    const double rad2 = cnt_rad[v2] >= 0 ? cnt_rad[v2] : half_t;
+                    double re;
+                    if (!resistance_gnp(gx + 3 * v1, gx + 3 * v2, rad1, rad2, ep, &re)) {
+                        delete h;
+                        return fail("Calculate_resistance_gnp: boundary edge of zero length");
+                    }
+                    h->e_bnd[w].push_back(n1);
+                    h->e_int[w].push_back(n2);
+                    h->e_re[w].push_back(re);
+                }
+            }
         }
         h->v.direction = direction;
     }
@@ -707,6 +1224,10 @@ cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int famil
     v.n_points = P;
     v.lmm = h->lmm.data();
     v.node_point = h->node_point.data();
+    v.node_xyz = h->node_xyz.data();
+    v.gnp_nodes = nodes - first_gnp_node;
+    v.n_gnp_points = GP;
+    v.lmm_gnp = h->lmm_gnp.data();
     for (int w = 0; w < 2; w++) {
         v.ne[w] = (int64_t)h->e_bnd[w].size();
         v.e_bnd[w] = h->e_bnd[w].data();

@@ -1,4 +1,4 @@
-// netgen.h -- synthetic CNT networks and the DEA system assembly that feeds the solver.
+// netgen.h -- synthetic CNT/GNP networks and the DEA system assembly that feeds the solver.
 //
 // Host-side C++ (no GPU code).  It produces the same inputs the reference's
 // Direct_Electrifying::Compute_voltage_field builds from Hoshen_Kopelman and Cutoff_Wins
@@ -25,6 +25,11 @@ typedef struct cicy_net_params {
     uint64_t seed;         // mt19937_64 seed
     int per_point_nodes;   // 1: every polyline point is an element point (node);
                            // 0: element points are tube ends + junction points (reference rule)
+    int64_t n_gnp;         // number of nanoplatelets (square plates) to place
+    double gnp_side;       // plate side (um)
+    double gnp_thick;      // plate thickness (um)
+    int min_points;        // contact points of one tube on one plate closer than this many
+                           // points apart form one junction (Cutoff_dist::min_points)
 } cicy_net_params;
 
 typedef struct cicy_elec_params {
@@ -33,6 +38,8 @@ typedef struct cicy_elec_params {
     int junction_exponential;  // 1: C1*d*exp(C2*d)/A; 0: constant junction_resistance
     double junction_resistance;
     double c1, c2;             // tunnelling constants
+    double resistivity_gnp_t;     // Ohm.um, across the plate (Electric_para)
+    double resistivity_gnp_surf;  // Ohm.um, in the plate
 } cicy_elec_params;
 
 typedef struct cicy_net cicy_net;
@@ -61,13 +68,18 @@ typedef struct cicy_sss_view {
     int64_t *e_int[2];      // other-end global node
     double *e_re[2];        // element resistance (Ohm)
     int direction;          // current direction used for the element lists (-1: none)
+    double *node_xyz;       // [nodes*3] position of each node's point (NaN for reserved)
+    int64_t gnp_nodes;      // nodes that are nanoplatelet points
+    int64_t n_gnp_points;
+    int64_t *lmm_gnp;       // [n_gnp_points] plate point -> global node or -1 (LMM_gnps)
     void *priv;             // owner of the arrays (internal)
 } cicy_sss_view;
 
 cicy_net *cicy_net_generate(const cicy_net_params *p);
 void cicy_net_free(cicy_net *net);
 
-// what: 0 points, 1 tubes, 2 junctions, 3 clusters
+// what: 0 points, 1 tubes, 2 junctions, 3 clusters, 4 plates, 5 plate points,
+// 6 tube-plate junctions, 7 plate-plate junctions
 int64_t cicy_net_count(const cicy_net *net, int what);
 const double *cicy_net_xyz(const cicy_net *net);          // [points*3]
 const int64_t *cicy_net_cnt_ptr(const cicy_net *net);     // [tubes+1]
@@ -84,6 +96,32 @@ int64_t cicy_net_cluster_tubes(const cicy_net *net, int64_t k, const int64_t **t
 int64_t cicy_net_cluster_junctions(const cicy_net *net, int64_t k, const int64_t **junc);
 // Element points of tube t (the reference's elements_cnt[t], ascending).
 int64_t cicy_net_elements(const cicy_net *net, int64_t t, const int64_t **pts);
+
+// ---- nanoplatelets (GNPs).  Plate points form their own numbering (the reference's
+// points_gnp); structure_gnp[g] lists plate g's points in creation order: window-boundary
+// points (Cutoff_Wins::Find_gnp_boundary_points, Cutoff_Wins.cpp:871-988), plate-plate
+// junction points, tube-plate junction points (Hoshen_Kopelman::Add_mixed_junction,
+// :2280-2307), minus the boundary points on faces the cluster's family does not use
+// (Remove_gnp_points_in_non_relevant_boundaries, :2740-2892).
+const double *cicy_net_gnp_xyz(const cicy_net *net);       // [plate points*3]
+// Plate g: centre[3], axes[9] (row k = axis k; axis 2 is the normal), side, thickness.
+int cicy_net_gnp(const cicy_net *net, int64_t g, double *centre, double *axes, double *side,
+                 double *thick);
+int64_t cicy_net_gnp_structure(const cicy_net *net, int64_t g, const int64_t **pts);
+// Triangulation edges of plate g as [2*count] plate point pairs, in the reference's order
+// (empty for plates outside every cluster).
+int64_t cicy_net_gnp_triangulation(const cicy_net *net, int64_t g, const int64_t **edges);
+int64_t cicy_net_gnp_boundary_points(const cicy_net *net, int b, const int64_t **pts);
+int64_t cicy_net_boundary_gnps(const cicy_net *net, int b, const int64_t **gnps);
+// Tube-plate junction j: tube point, plate point, tube, plate, gap (um).
+int cicy_net_mixed_junction(const cicy_net *net, int64_t j, int64_t *p_cnt, int64_t *p_gnp,
+                            int64_t *cnt, int64_t *gnp, double *dist);
+// Plate-plate junction j: points on plate 1 and 2, plates, gap (um).
+int cicy_net_gnp_junction(const cicy_net *net, int64_t j, int64_t *p1, int64_t *p2,
+                          int64_t *g1, int64_t *g2, double *dist);
+int64_t cicy_net_cluster_gnps(const cicy_net *net, int64_t k, const int64_t **gnps);
+int64_t cicy_net_cluster_mixed_junctions(const cicy_net *net, int64_t k, const int64_t **junc);
+int64_t cicy_net_cluster_gnp_junctions(const cicy_net *net, int64_t k, const int64_t **junc);
 
 // Assemble the reduced system of cluster `k` with resistor flag `r_flag` (0 unit, 1 real)
 // for `family` (0..6).  `direction` (0..2 or -1) selects which boundary element lists to

@@ -1,6 +1,7 @@
-"""Synthetic CNT networks and DEA system assembly (Python side of libcicy_net.so).
+"""Synthetic CNT/GNP networks and DEA system assembly (Python side of libcicy_net.so).
 
-The generator is this package's own (straight tubes, seeded mt19937_64); the assembly
+The generator is this package's own (straight tubes and square nanoplatelets, seeded
+mt19937_64); the assembly
 follows Direct_Electrifying::Compute_voltage_field (Direct_Electrifying.cpp:11-113) and
 its helpers, so the arrays handed to the solver have the reference's layout:
 
@@ -28,14 +29,18 @@ class _NetParams(ctypes.Structure):
                 ("radius", ctypes.c_double), ("len_min", ctypes.c_double),
                 ("len_max", ctypes.c_double), ("seg_len", ctypes.c_double),
                 ("d_tunnel", ctypes.c_double), ("d_vdw", ctypes.c_double),
-                ("seed", ctypes.c_uint64), ("per_point_nodes", ctypes.c_int)]
+                ("seed", ctypes.c_uint64), ("per_point_nodes", ctypes.c_int),
+                ("n_gnp", ctypes.c_int64), ("gnp_side", ctypes.c_double),
+                ("gnp_thick", ctypes.c_double), ("min_points", ctypes.c_int)]
 
 
 class _ElecParams(ctypes.Structure):
     _fields_ = [("applied_voltage", ctypes.c_double), ("resistivity_cnt", ctypes.c_double),
                 ("junction_exponential", ctypes.c_int),
                 ("junction_resistance", ctypes.c_double),
-                ("c1", ctypes.c_double), ("c2", ctypes.c_double)]
+                ("c1", ctypes.c_double), ("c2", ctypes.c_double),
+                ("resistivity_gnp_t", ctypes.c_double),
+                ("resistivity_gnp_surf", ctypes.c_double)]
 
 
 class _SSSView(ctypes.Structure):
@@ -46,7 +51,9 @@ class _SSSView(ctypes.Structure):
                 ("has_guess", ctypes.c_int), ("n_points", ctypes.c_int64),
                 ("lmm", I64P), ("node_point", I64P),
                 ("ne", ctypes.c_int64 * 2), ("e_bnd", I64P * 2), ("e_int", I64P * 2),
-                ("e_re", F64P * 2), ("direction", ctypes.c_int), ("priv", ctypes.c_void_p)]
+                ("e_re", F64P * 2), ("direction", ctypes.c_int), ("node_xyz", F64P),
+                ("gnp_nodes", ctypes.c_int64), ("n_gnp_points", ctypes.c_int64),
+                ("lmm_gnp", I64P), ("priv", ctypes.c_void_p)]
 
 
 _bound = False
@@ -74,10 +81,22 @@ def _bind():
     lib.cicy_net_boundary_points.argtypes = [ctypes.c_void_p, ctypes.c_int,
                                              ctypes.POINTER(I64P)]
     for nm in ("cicy_net_boundary_tubes", "cicy_net_cluster_tubes",
-               "cicy_net_cluster_junctions", "cicy_net_elements"):
+               "cicy_net_cluster_junctions", "cicy_net_elements", "cicy_net_gnp_structure",
+               "cicy_net_gnp_triangulation", "cicy_net_cluster_gnps",
+               "cicy_net_cluster_mixed_junctions", "cicy_net_cluster_gnp_junctions"):
         f = getattr(lib, nm)
         f.restype = ctypes.c_int64
         f.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(I64P)]
+    for nm in ("cicy_net_gnp_boundary_points", "cicy_net_boundary_gnps"):
+        f = getattr(lib, nm)
+        f.restype = ctypes.c_int64
+        f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(I64P)]
+    lib.cicy_net_gnp_xyz.restype = F64P
+    lib.cicy_net_gnp_xyz.argtypes = [ctypes.c_void_p]
+    lib.cicy_net_gnp.argtypes = [ctypes.c_void_p, ctypes.c_int64, F64P, F64P, F64P, F64P]
+    for nm in ("cicy_net_mixed_junction", "cicy_net_gnp_junction"):
+        getattr(lib, nm).argtypes = [ctypes.c_void_p, ctypes.c_int64, I64P, I64P, I64P, I64P,
+                                     F64P]
     lib.cicy_net_assemble.restype = ctypes.POINTER(_SSSView)
     lib.cicy_net_assemble.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                                       ctypes.c_int, ctypes.POINTER(_ElecParams),
@@ -106,6 +125,8 @@ class ElectricParams:
     lambda_barrier: float = 2.0            # eV
     c1: float = field(default=0.0)
     c2: float = field(default=0.0)
+    resistivity_gnp_t: float = 1e-2        # Ohm.um across a plate (input.dat)
+    resistivity_gnp_surf: float = 1e-7     # Ohm.um in a plate (input.dat)
 
     def __post_init__(self):
         if self.c1 == 0.0 and self.c2 == 0.0:
@@ -114,7 +135,8 @@ class ElectricParams:
     def _c(self) -> _ElecParams:
         return _ElecParams(self.applied_voltage, self.resistivity_cnt,
                            1 if self.junction_exponential else 0,
-                           self.junction_resistance, self.c1, self.c2)
+                           self.junction_resistance, self.c1, self.c2,
+                           self.resistivity_gnp_t, self.resistivity_gnp_surf)
 
 
 @dataclass
@@ -129,12 +151,16 @@ class NetworkParams:
     d_vdw: float = 0.00034
     seed: int = 1_000_000
     per_point_nodes: bool = True
+    n_gnp: int = 0                # square nanoplatelets
+    gnp_side: float = 1.0         # um
+    gnp_thick: float = 0.01       # um
+    min_points: int = 5           # tube-plate contact compression (input.dat cutoffs)
 
 
 def config_params(config: int, realisation: int = 0, n_cnt: int | None = None) -> NetworkParams:
     """Network parameters of the workload configurations.
 
-    1: 1 um RVE, 300 tubes;  2: 5 um, 40k tubes;  3: 8 um, 160k tubes;
+    1: 1 um RVE, 300 tubes;  2: 5 um, 40k tubes;  3: 8 um, 160k tubes + 2k plates;
     4: batch realisations with U[2k,40k] tubes at the density of configuration 2.
     The seed is 1_000_000*config + realisation.
     """
@@ -144,7 +170,7 @@ def config_params(config: int, realisation: int = 0, n_cnt: int | None = None) -
     if config == 2:
         return NetworkParams(box=5.0, n_cnt=40_000, seed=seed)
     if config == 3:
-        return NetworkParams(box=8.0, n_cnt=160_000, seed=seed)
+        return NetworkParams(box=8.0, n_cnt=160_000, n_gnp=2_000, seed=seed)
     if config in (4, 5):
         if n_cnt is None:
             rng = np.random.default_rng(seed)
@@ -172,6 +198,8 @@ class DEASystem:
     node_point: np.ndarray
     boundary_elements: list  # [(n_bnd, n_int, re) for b1, b2]
     direction: int
+    lmm_gnp: np.ndarray | None = None    # plate point -> global node or -1
+    gnp_nodes: int = 0                   # free nodes that are plate points (numbered last)
     node_xyz: np.ndarray | None = None
 
     @property
@@ -196,7 +224,8 @@ class Network:
         lib = _bind()
         p = _NetParams(params.box, params.n_cnt, params.radius, params.len_min,
                        params.len_max, params.seg_len, params.d_tunnel, params.d_vdw,
-                       params.seed, 1 if params.per_point_nodes else 0)
+                       params.seed, 1 if params.per_point_nodes else 0, params.n_gnp,
+                       params.gnp_side, params.gnp_thick, params.min_points)
         self.params = params
         self._h = lib.cicy_net_generate(ctypes.byref(p))
         if not self._h:
@@ -213,7 +242,8 @@ class Network:
 
     def count(self, what: str) -> int:
         return int(self._lib.cicy_net_count(
-            self._h, {"points": 0, "tubes": 1, "junctions": 2, "clusters": 3}[what]))
+            self._h, {"points": 0, "tubes": 1, "junctions": 2, "clusters": 3, "gnps": 4,
+                      "gnp_points": 5, "mixed_junctions": 6, "gnp_junctions": 7}[what]))
 
     @property
     def xyz(self) -> np.ndarray:
@@ -260,6 +290,58 @@ class Network:
     def elements(self, t: int) -> np.ndarray:
         return self._list(self._lib.cicy_net_elements, t)
 
+    # ---- nanoplatelets
+    @property
+    def gnp_xyz(self) -> np.ndarray:
+        n = self.count("gnp_points")
+        return _arr(self._lib.cicy_net_gnp_xyz(self._h), 3 * n, np.float64).reshape(n, 3)
+
+    def gnp(self, g: int):
+        """(centre[3], axes[3,3] with the normal in row 2, side, thickness) of plate g."""
+        c = (ctypes.c_double * 3)()
+        ax = (ctypes.c_double * 9)()
+        l, t = ctypes.c_double(), ctypes.c_double()
+        if not self._lib.cicy_net_gnp(self._h, g, c, ax, ctypes.byref(l), ctypes.byref(t)):
+            raise IndexError(g)
+        return np.array(c[:]), np.array(ax[:]).reshape(3, 3), l.value, t.value
+
+    def gnp_structure(self, g: int) -> np.ndarray:
+        return self._list(self._lib.cicy_net_gnp_structure, g)
+
+    def gnp_triangulation(self, g: int) -> np.ndarray:
+        ptr = I64P()
+        n = self._lib.cicy_net_gnp_triangulation(self._h, g, ctypes.byref(ptr))
+        if n < 0:
+            raise IndexError(g)
+        return _arr(ptr, 2 * n, np.int64).reshape(n, 2)
+
+    def gnp_boundary_points(self, b: int) -> np.ndarray:
+        return self._list(self._lib.cicy_net_gnp_boundary_points, b)
+
+    def boundary_gnps(self, b: int) -> np.ndarray:
+        return self._list(self._lib.cicy_net_boundary_gnps, b)
+
+    def mixed_junction(self, j: int):
+        v = [ctypes.c_int64() for _ in range(4)] + [ctypes.c_double()]
+        if not self._lib.cicy_net_mixed_junction(self._h, j, *[ctypes.byref(x) for x in v]):
+            raise IndexError(j)
+        return tuple(x.value for x in v)
+
+    def gnp_junction(self, j: int):
+        v = [ctypes.c_int64() for _ in range(4)] + [ctypes.c_double()]
+        if not self._lib.cicy_net_gnp_junction(self._h, j, *[ctypes.byref(x) for x in v]):
+            raise IndexError(j)
+        return tuple(x.value for x in v)
+
+    def cluster_gnps(self, k: int) -> np.ndarray:
+        return self._list(self._lib.cicy_net_cluster_gnps, k)
+
+    def cluster_mixed_junctions(self, k: int) -> np.ndarray:
+        return self._list(self._lib.cicy_net_cluster_mixed_junctions, k)
+
+    def cluster_gnp_junctions(self, k: int) -> np.ndarray:
+        return self._list(self._lib.cicy_net_cluster_gnp_junctions, k)
+
     @property
     def radii(self) -> np.ndarray:
         return np.full(self.count("tubes"), self.params.radius)
@@ -298,13 +380,13 @@ class Network:
                 boundary_elements=[(_arr(v.e_bnd[w], v.ne[w], np.int64),
                                     _arr(v.e_int[w], v.ne[w], np.int64),
                                     _arr(v.e_re[w], v.ne[w], np.float64)) for w in range(2)],
-                direction=v.direction)
+                direction=v.direction,
+                lmm_gnp=_arr(v.lmm_gnp, v.n_gnp_points, np.int64),
+                gnp_nodes=int(v.gnp_nodes))
+            if with_xyz:
+                sys_.node_xyz = _arr(v.node_xyz, 3 * nodes, np.float64).reshape(nodes, 3)[rd:]
         finally:
             self._lib.cicy_sss_free(vp)
-        if with_xyz:
-            xyz = self.xyz
-            pts = sys_.node_point[rd:]
-            sys_.node_xyz = xyz[pts]
         return sys_
 
 

@@ -11,9 +11,16 @@ Python dict standing in for each row's std::map<long,double>:
   add_new / add_existing   Add_new_elements_to_2d_sparse_matrix :648-668,
                            Add_to_existing_elements_in_2d_sparse_matrix :670-694
   junctions                Fill_2d_matrices_cnt_junctions :738-852
+  plate nodes              LM_matrix_for_gnps :307-349
+  tube-plate junctions     Fill_2d_matrices_mixed_junctions :903-1019
+  plate triangulation      Fill_2d_matrices_gnp :1021-1196, Calculate_resistance_gnp :1198-1232
+  plate-plate junctions    Fill_2d_matrices_gnp_junctions :1234-1327
   SSS + RHS                From_2d_to_1d_vectors :1329-1394, Get_voltage_vector :1472-1500
   ramp guess               Initial_guess_for_CG / Initial_guess_for_cnt_nodes :1396-1470
-  boundary elements        Electrical_analysis.cpp:499-605
+  boundary elements        Electrical_analysis.cpp:499-605 (tubes), :607-675 (plates)
+
+The plate triangulation itself is an input here (it is pinned separately against
+scipy's Delaunay triangulation in tests/test_gnp.py).
 """
 from __future__ import annotations
 
@@ -50,18 +57,39 @@ def resistance_cnt(xyz, p1, p2, radius, rho):
     return rho * length / (PI * radius * radius)
 
 
-def junction_resistance(dist, r1, r2, elec):
+def junction_resistance(dist, r1, r2, elec, cnt1=True, cnt2=True):
+    """Calculate_junction_resistance (Direct_Electrifying.cpp:854-901): a tube's radius is
+    doubled to its diameter, a plate's thickness is used as is."""
     if not elec["junction_exponential"]:
         return elec["junction_resistance"]
-    jl1 = r1 + r1
-    jl2 = r2 + r2
+    jl1 = r1 + r1 if cnt1 else r1
+    jl2 = r2 + r2 if cnt2 else r2
     a = jl1 * jl1 if (jl1 - jl2 < ZERO) else jl2 * jl2
     return elec["c1"] * dist * math.exp(elec["c2"] * dist) / a
 
 
+def resistance_gnp(p1, p2, rad1, rad2, elec):
+    """Calculate_resistance_gnp (Direct_Electrifying.cpp:1198-1232): the resistivity
+    tensor diag(rho_s, rho_s, rho_t) is taken in the global axes."""
+    dx, dy, dz = p1[0] - p2[0], p1[1] - p2[1], p1[2] - p2[2]
+    length = math.sqrt(dx * dx + dy * dy + dz * dz)
+    if length < ZERO:
+        raise ValueError("triangulation edge of zero length")
+    ux, uy, uz = (p2[0] - p1[0]) / length, (p2[1] - p1[1]) / length, (p2[2] - p1[2]) / length
+    rx, ry, rz = ux * elec["rho_gnp_surf"], uy * elec["rho_gnp_surf"], uz * elec["rho_gnp_t"]
+    rho = rx * ux + ry * uy + rz * uz
+    return rho * length / (PI * rad1 * rad2)
+
+
 def assemble(family, r_flag, boundary_pts, cluster_tubes, elements, xyz, radii, junctions,
-             cluster_junctions, elec, box):
-    """Returns a dict with the SSS system, b, VEF, LMM (dict), ramp guess and r0."""
+             cluster_junctions, elec, box, gnp=None):
+    """Returns a dict with the SSS system, b, VEF, LMM (dict), ramp guess and r0.
+
+    `gnp` (optional) describes the cluster's plates: boundary_pts[6] (plate points at each
+    face), cluster_gnps, structure {g: points}, xyz (plate points), thick {g: t},
+    triangulation {g: [(v1, v2), ...]}, mixed (all tube-plate junctions as (p_cnt, p_gnp,
+    cnt, gnp, dist)), cluster_mixed, gjunc (all plate-plate junctions as (p1, p2, g1, g2,
+    dist)), cluster_gjunc.  Plate nodes come after the tube nodes."""
     rd = reserved_nodes(family)
     lmm = {}
     for n, b in enumerate(boundaries_of_family(family)):
@@ -81,6 +109,16 @@ def assemble(family, r_flag, boundary_pts, cluster_tubes, elements, xyz, radii, 
         if e[-1] not in lmm:
             lmm[e[-1]] = gn
             gn += 1
+    lmm_g = {}
+    if gnp is not None:
+        for n, b in enumerate(boundaries_of_family(family)):
+            for p in gnp["boundary_pts"][b]:
+                lmm_g[int(p)] = n
+        for g in gnp["cluster_gnps"]:
+            for p in gnp["structure"][g]:
+                if int(p) not in lmm_g:
+                    lmm_g[int(p)] = gn
+                    gn += 1
     nodes = gn
     col_values = [dict() for _ in range(nodes)]
     diagonal = [0.0] * nodes
@@ -132,6 +170,44 @@ def assemble(family, r_flag, boundary_pts, cluster_tubes, elements, xyz, radii, 
             add_new(n1, n2, g)
         elif rd > 2:
             add_existing(n1, n2, g)
+    points_cnt_rad = {}
+    if gnp is not None:
+        gx = gnp["xyz"]
+        for j in gnp["cluster_mixed"]:
+            p_cnt, p_gnp, cnt, gp, dist = gnp["mixed"][j]
+            n1 = lmm.get(int(p_cnt), -1)
+            n2 = lmm_g.get(int(p_gnp), -1)
+            if n1 == -1 or n2 == -1:
+                continue
+            g = 1.0
+            if r_flag == 1:
+                g = 1 / junction_resistance(dist, radii[cnt], gnp["thick"][gp], elec,
+                                            cnt1=True, cnt2=False)
+            if n1 >= rd and n2 >= rd:
+                add_new(n1, n2, g)
+            elif rd > 2:
+                add_existing(n1, n2, g)
+            points_cnt_rad[int(p_gnp)] = radii[cnt]
+        for gp in gnp["cluster_gnps"]:
+            for v1, v2 in reversed(gnp["triangulation"][gp]):
+                node1, node2 = lmm_g[int(v1)], lmm_g[int(v2)]
+                g = 1.0
+                if r_flag == 1:
+                    rad1 = points_cnt_rad.get(int(v1), gnp["thick"][gp] / 2)
+                    rad2 = points_cnt_rad.get(int(v2), gnp["thick"][gp] / 2)
+                    g = 1 / resistance_gnp(gx[int(v1)], gx[int(v2)], rad1, rad2, elec)
+                add_new(node1, node2, g)
+        for j in gnp["cluster_gjunc"]:
+            p1, p2, g1, g2, dist = gnp["gjunc"][j]
+            n1 = lmm_g.get(int(p1), -1)
+            n2 = lmm_g.get(int(p2), -1)
+            if n1 == -1 or n2 == -1:
+                continue
+            g = 1.0
+            if r_flag == 1:
+                g = 1 / junction_resistance(dist, gnp["thick"][g1], gnp["thick"][g2], elec,
+                                            cnt1=False, cnt2=False)
+            add_new(n1, n2, g)
     vmag = elec["applied_voltage"] if r_flag == 1 else float(nodes)
     vef = [0.0] * rd
     for i in range(1, rd):
@@ -163,14 +239,18 @@ def assemble(family, r_flag, boundary_pts, cluster_tubes, elements, xyz, radii, 
                     x0[nd - rd] = vpc * (box - xyz[int(p)][ax])
     return dict(nodes=nodes, reserved=rd, row_ptr=np.array(row_ptr, np.int64),
                 col_ind=np.array(col_ind, np.int64), values=np.array(values),
-                diag=np.array(diag), b=np.array(R), vef=np.array(vef), lmm=lmm, x0=x0)
+                diag=np.array(diag), b=np.array(R), vef=np.array(vef), lmm=lmm, x0=x0,
+                lmm_gnp=lmm_g, points_cnt_rad=points_cnt_rad)
 
 
-def boundary_elements(direction, boundary_tubes, elements, lmm, xyz, radii, rho):
-    """(n_bnd, n_int, re) for b1 and b2 of `direction` (Electrical_analysis.cpp:499-605)."""
+def boundary_elements(direction, boundary_tubes, elements, lmm, xyz, radii, rho, gnp=None,
+                      lmm_gnp=None, points_cnt_rad=None, elec=None):
+    """(n_bnd, n_int, re) for b1 and b2 of `direction` (Electrical_analysis.cpp:499-605),
+    followed by the plate triangulation edges at the boundary (:607-675) when `gnp` is
+    given (boundary_gnps[6] plus the keys of assemble's `gnp`)."""
     b1, b2 = {0: (2, 4), 1: (3, 5), 2: (0, 1)}[direction]
     out = []
-    for b in (b1, b2):
+    for node, b in enumerate((b1, b2)):
         nb, ni, rr = [], [], []
         for t in boundary_tubes[b]:
             e = [int(p) for p in elements[t]]
@@ -189,6 +269,25 @@ def boundary_elements(direction, boundary_tubes, elements, lmm, xyz, radii, rho)
                 nb.append(n1)
                 ni.append(n2)
                 rr.append(resistance_cnt(xyz, lo, hi, radii[t], rho))
+        if gnp is not None:
+            in_cluster = set(int(g) for g in gnp["cluster_gnps"])
+            for gp in gnp["boundary_gnps"][b]:
+                if int(gp) not in in_cluster:
+                    continue
+                for v1, v2 in gnp["triangulation"][int(gp)]:
+                    na, nb_ = lmm_gnp[int(v1)], lmm_gnp[int(v2)]
+                    if na > 1 and nb_ > 1:
+                        continue
+                    if na == nb_:
+                        raise ValueError("both nodes at the same boundary")
+                    n1, n2 = (na, nb_) if na == node else (nb_, na)
+                    t2 = gnp["thick"][int(gp)] / 2
+                    rad1 = points_cnt_rad.get(int(v1), t2)
+                    rad2 = points_cnt_rad.get(int(v2), t2)
+                    nb.append(n1)
+                    ni.append(n2)
+                    rr.append(resistance_gnp(gnp["xyz"][int(v1)], gnp["xyz"][int(v2)], rad1,
+                                             rad2, elec))
         out.append((np.array(nb, np.int64), np.array(ni, np.int64), np.array(rr)))
     return out
 
