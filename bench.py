@@ -114,10 +114,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 reference line (profiling runs of one method)")
-    ap.add_argument("--methods", default="pcg_f64,condensed,condensed_resident,learned_f64",
+    ap.add_argument("--methods", default="pcg_f64,condensed,condensed_block,learned_f64",
                     help="comma list from pcg_f64, mixed, resident, condensed, "
-                         "condensed_resident, learned_f64, learned_mixed (the learned ones "
-                         "need trained weights)")
+                         "condensed_resident, condensed_block, learned_f64, learned_mixed "
+                         "(the learned ones need trained weights)")
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only run the SpMV kernels (both precisions) and a 1 GiB reference "
                          "copy, for hardware-counter runs")
@@ -132,10 +132,13 @@ def main():
     t0 = time.perf_counter()
     net = network.generate(args.config)
     box = net.params.box
-    s = network.largest_percolating_system(net, 0, with_xyz=True)
+    k_cl = max(net.percolating_clusters(0), key=lambda c: net.cluster(c)[0])
+    s = net.assemble(k_cl, 1, 0, direction=0, with_xyz=True)
+    plates = network.plate_row_ranges(net, k_cl, s)
+    n_gnp = net.count("gnps")
     del net
-    log(f"[bench] config {args.config}: N={s.n} L={s.nnz} built in "
-        f"{time.perf_counter() - t0:.1f}s")
+    log(f"[bench] config {args.config}: N={s.n} L={s.nnz} ({s.gnp_nodes} plate nodes in "
+        f"{len(plates)} plates) built in {time.perf_counter() - t0:.1f}s")
 
     from cicy_geom_amd import autotune
     ctx = dea.Context(0)
@@ -167,6 +170,7 @@ def main():
                "resident": (dea.METHOD_RESIDENT, "pcg_f64", False),
                "condensed": (dea.METHOD_CONDENSED, "condensed", False),
                "condensed_resident": (dea.METHOD_CONDENSED_RESIDENT, "condensed", False),
+               "condensed_block": (dea.METHOD_CONDENSED_BLOCK, "condensed", False),
                "learned_f64": (dea.METHOD_PCG_F64, "pcg_f64", True),
                "learned_mixed": (dea.METHOD_MIXED, "mixed", True)}
     wanted = [m for m in args.methods.split(",") if m]
@@ -206,6 +210,8 @@ def main():
         t0 = time.perf_counter()
         sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
         sysd.set_rhs(s.b, None, s.b)
+        if method == dea.METHOD_CONDENSED_BLOCK:
+            sysd.set_blocks(plates)
         upload_s = time.perf_counter() - t0
         if learned:
             graph = warmstart.Graph(ctx, s, box, 0, 1.0)
@@ -223,6 +229,10 @@ def main():
                              core_rows=sysd.size("core_rows"),
                              core_nnz=sysd.size("core_nnz"),
                              series_rounds=sysd.size("series_rounds"),
+                             core_blocks=sysd.size("core_blocks"),
+                             core_block_rows=sysd.size("core_block_rows"),
+                             core_groups=sysd.size("core_groups"),
+                             setup_s=upload_s,
                              resident_rows_per_thread=sysd.size("resident_rows"),
                              rows_per_block=tuned[tkey]["rows_per_block"])
         log(f"[bench] {name}: {results[name]}")
@@ -249,7 +259,7 @@ def main():
     plain = results.get("pcg_f64")
     head = "pcg_f64" if plain else next(iter(results))
     for cand in ("learned_f64", "mixed", "learned_mixed", "resident", "condensed",
-                 "condensed_resident"):
+                 "condensed_resident", "condensed_block"):
         if cand in results and cand != head and results[cand]["status"] == "ok" and \
                 results[cand]["solves_per_s"] > results[head]["solves_per_s"]:
             head = cand

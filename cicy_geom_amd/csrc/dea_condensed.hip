@@ -58,6 +58,12 @@ struct cicy_cond {
     double *fS = nullptr, *xS = nullptr;              // [m] right-hand side / solution on S
     cicy_sys *red = nullptr;    // the core system on the device
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timing (the inner solve uses the context's)
+    // host copy of the core (strictly-lower SSS, diagonal, full-system row of each core
+    // row) for the block-preconditioned core CG, built on demand
+    std::vector<int64_t> h_crp, h_ccol;
+    std::vector<double> h_cval, h_cdiag;
+    std::vector<int> h_orig;
+    cicy_core *blk = nullptr;
 };
 
 namespace cicy {
@@ -215,6 +221,7 @@ int blocks(long long n, int threads) { return (int)std::max<long long>(1, (n + t
 void condensed_free(cicy_cond *c) {
     if (!c) return;
     if (c->red) cicy_sys_destroy(c->red);
+    if (c->blk) core_free(c->blk);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     void *ptrs[] = {c->ps, c->pe, c->pl, c->pr, c->anchor_of, c->anchors, c->lfac, c->mfac,
@@ -234,6 +241,10 @@ int condensed_info(const cicy_cond *c, int what) {
         case 2: return c->mc;                                // core rows (the CG system)
         case 3: return (int)c->nnz_core;
         case 4: return (int)c->round_off.size() - 1;         // series rounds
+        case 5: return core_info(c->blk, 1);                 // core rows in dense blocks
+        case 6: return core_info(c->blk, 0);                 // dense blocks
+        case 7: return core_info(c->blk, 2);                 // core CG work-groups
+        case 8: return core_info(c->blk, 3);                 // dense block entries
         default: return 0;
     }
 }
@@ -628,6 +639,12 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
             }
             ok = sys_create(ctx, Z.mc, (int64_t)Z.cval.size(), Z.crp.data(), Z.ccol.data(),
                             Z.cval.data(), Z.cdiag.data(), kExtraResident, &c->red) != 0;
+            c->h_orig.resize((size_t)Z.mc);
+            for (int q = 0; q < Z.mc; q++) c->h_orig[q] = anchors[Z.core_nodes[q]];
+            c->h_crp = std::move(Z.crp);
+            c->h_ccol = std::move(Z.ccol);
+            c->h_cval = std::move(Z.cval);
+            c->h_cdiag = std::move(Z.cdiag);
         }
     }
     if (!ok) {
@@ -671,10 +688,15 @@ static bool condensed_correction(cicy_sys *s, int inner, const double *rhs, doub
             red->has_b = false;
             red->ref2_override = ref2;
             bool ok;
-            if (inner == CICY_METHOD_RESIDENT)
+            if (inner == CICY_METHOD_RESIDENT) {
                 ok = solve_resident(red, tol, max_iter, CICY_REL_B, inner_st);
-            else
+            } else if (inner == CICY_METHOD_CONDENSED_BLOCK) {
+                if (!c->blk) ok = condensed_set_blocks(s, 0, nullptr) != 0;
+                else ok = true;
+                ok = ok && solve_core(ctx, c->blk, red->r0, tol, max_iter, ref2, red->x, inner_st);
+            } else {
                 ok = solve_pcg64(red, tol, max_iter, CICY_REL_B, inner_st);
+            }
             if (!ok) return false;
             hipLaunchKernelGGL(k_scatter, dim3(blocks(c->mc, kCondThreads)), dim3(kCondThreads), 0,
                                st, c->mc, c->core_nodes, red->x, c->xS);
@@ -693,6 +715,31 @@ static bool condensed_correction(cicy_sys *s, int inner, const double *rhs, doub
                        st, c->npath, c->ps, c->pe, c->pl, c->pr, c->lfac, c->mfac, c->band, rhs,
                        (const double *)(c->m > 0 ? c->xS : nullptr), c->delta);
     return true;
+}
+
+int condensed_set_blocks(cicy_sys *s, int64_t nblocks, const int64_t *ranges) {
+    cicy_ctx *ctx = s->ctx;
+    cicy_cond *c = s->cond;
+    if (!c || c->mc <= 0) {
+        set_error(ctx, "no condensed core (%s)", s->cond_why ? s->cond_why : "not built");
+        return 0;
+    }
+    if (nblocks < 0 || (nblocks > 0 && !ranges)) {
+        set_error(ctx, "cicy_sys_set_blocks: invalid block list");
+        return 0;
+    }
+    for (int64_t b = 0; b < nblocks; b++)
+        if (ranges[2 * b] < 0 || ranges[2 * b + 1] > s->n || ranges[2 * b + 1] < ranges[2 * b]) {
+            set_error(ctx, "cicy_sys_set_blocks: row group %lld outside [0, %d)", (long long)b, s->n);
+            return 0;
+        }
+    if (c->blk) {
+        core_free(c->blk);
+        c->blk = nullptr;
+    }
+    c->blk = core_setup(ctx, c->mc, c->h_crp, c->h_ccol, c->h_cval, c->h_cdiag, c->h_orig,
+                        nblocks, ranges);
+    return c->blk ? 1 : 0;
 }
 
 bool solve_condensed(cicy_sys *s, int inner, double tol, long long max_iter, int rel_to,

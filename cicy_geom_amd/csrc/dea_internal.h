@@ -68,6 +68,7 @@ inline void launch_apply(const DevMat<T> &m, int grid, hipStream_t st, const Bat
 
 struct cicy_res;   // resident-CG layout (dea_resident.hip)
 struct cicy_cond;  // condensed system (dea_condensed.hip)
+struct cicy_core;  // block-preconditioned persistent core CG (dea_core.hip)
 
 struct cicy_ctx {
     int device = 0;
@@ -175,6 +176,17 @@ void condensed_free(cicy_cond *c);
 bool solve_condensed(cicy_sys *s, int inner_method, double tol, long long max_iter, int rel_to,
                      cicy_solve_stats *stats);
 int condensed_info(const cicy_cond *c, int what);
+int condensed_set_blocks(cicy_sys *s, int64_t nblocks, const int64_t *ranges);
+
+// dea_core.hip
+cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
+                      const std::vector<int64_t> &ccol, const std::vector<double> &cval,
+                      const std::vector<double> &cdiag, const std::vector<int> &orig,
+                      int64_t nblocks, const int64_t *ranges);
+void core_free(cicy_core *z);
+int core_info(const cicy_core *z, int what);
+bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long long max_iter,
+                double ref2, double *x_out, cicy_solve_stats *stats);
 // uploads a system (dea_solver.hip); `extras` also builds the resident and condensed layouts
 constexpr int kExtraResident = 1, kExtraCondensed = 2;
 int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,

@@ -23,6 +23,7 @@
 //     p_k is recomputed from the published r_k and p_{k-1} (p_k = r_k + beta p_{k-1}), so
 //     the p.Ap phase needs no extra barrier after the p update.
 // Sums are accumulated in arrival order (last bits vary from run to run).
+#include "dea_grid.h"
 #include "dea_internal.h"
 
 #include <algorithm>
@@ -41,18 +42,12 @@ constexpr int kResWaves = kResThreads / kWave;
 constexpr int kResMaxSlots = 6144;          // published rows per work-group (LDS accumulator)
 constexpr int kResCheck = 8;                // unscaled-residual test every kResCheck iterations
 constexpr int kResChunk = 2048;             // iterations per cooperative launch (at most)
-constexpr unsigned kResSpinLimit = 1u << 24;  // polls before a grid barrier gives up
 // rows per thread whose loads are issued together before they are used (R is a multiple)
 constexpr int kResGroup = 4;
 constexpr int kResRows[] = {4, 8, 16, 24, 32, 40, 44, 48};
 
-// Grid-barrier control block; cleared by the host before every launch.  Work-group g
-// arrives on counter g % 8 (eight counters on separate 128-byte lines, so arrivals do not
-// all contend for one address).
-struct ResCtl {
-    unsigned cnt[8][32];
-    int error;          // set by a work-group whose barrier wait timed out
-};
+// Grid-barrier control block (dea_grid.h); cleared by the host before every launch.
+using ResCtl = GridCtl;
 
 struct ResState {
     long long k;        // iterations done in this solve
@@ -179,21 +174,6 @@ bool build_resident_host(const BlockedHost &h, const BandHost &bh, int R, ResHos
 // device helpers
 // ------------------------------------------------------------------------------------
 
-__device__ __forceinline__ double ld_coh(const double *p) {
-    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_coh(double *p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned ld_coh_u(const unsigned *p) {
-    return __hip_atomic_load(const_cast<unsigned *>(p), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int ld_coh_i(const int *p) {
-    return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Element idx of a row-indexed array, addressed as a 32-bit byte offset from a uniform base
 // (the arrays are below 4 GiB), so loads and stores use the scalar-base + vector-offset form
 // and need one offset register instead of a 64-bit address each.
@@ -213,88 +193,11 @@ __device__ __forceinline__ int lane_rank(unsigned long long m) {
                                           __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// Sum of NV values over the work-group (valid in thread 0), lds: [NV * kResWaves].
-template <int NV>
-__device__ __forceinline__ void res_block_sum(double (&v)[NV], double *lds) {
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-#pragma unroll
-    for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
-    if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; i++) lds[i * kResWaves + w] = v[i];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; i++) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < kResWaves; k++) s += lds[i * kResWaves + k];
-            v[i] = s;
-        }
-    }
-}
-
-// Grid-wide sum of NV (<= 2) values and barrier number m of this launch.  Every thread first
-// waits for its own stores (the published rows must be at the point of coherence before
-// its work-group arrives).  Thread 0 stores the work-group's sums to its own slot of the
-// barrier's parity with agent-scope stores, waits for them, and arrives on its counter;
-// lanes 0..7 of wave 0 poll the eight counters (bounded spin); then wave 0 reads all G
-// slots and sums them in a fixed order, so every work-group obtains bit-identical totals
-// (the run-to-run result is deterministic too).  A slot of parity m & 1 is rewritten only
-// at barrier m + 2, after every work-group has passed barrier m + 1 and hence finished
-// reading it.  Returns false in every thread if the wait timed out; v[] holds the totals.
+// Grid-wide sum and barrier m of this launch (grid_sum, dea_grid.h).
 template <int NV>
 __device__ __forceinline__ bool grid_reduce(double (&v)[NV], const ResDev &a, unsigned m,
                                             double *lds, int *flag) {
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    res_block_sum<NV>(v, lds);
-    const int G = a.G;
-    double *slot = a.part + (size_t)(m & 1u) * (size_t)G * 2;
-    if (threadIdx.x < kWave) {
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int i = 0; i < NV; i++) st_coh(&slot[(size_t)blockIdx.x * 2 + i], v[i]);
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            (void)atomicAdd(&a.ctl->cnt[blockIdx.x & 7][0], 1u);
-        }
-        int err = 0;
-        if (threadIdx.x < 8) {
-            const unsigned share = (int)threadIdx.x < G ? (unsigned)((G - threadIdx.x + 7) / 8) : 0u;
-            const unsigned target = (m + 1u) * share;
-            unsigned spins = 0;
-            while (ld_coh_u(&a.ctl->cnt[threadIdx.x][0]) < target) {
-                if (++spins > kResSpinLimit || ld_coh_i(&a.ctl->error)) {
-                    err = 1;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        }
-        err = __any(err) ? 1 : 0;
-        double sum[NV];
-#pragma unroll
-        for (int i = 0; i < NV; i++) sum[i] = 0.0;
-        for (int k = threadIdx.x; k < G; k += kWave) {
-#pragma unroll
-            for (int i = 0; i < NV; i++) sum[i] += ld_coh(&slot[(size_t)k * 2 + i]);
-        }
-#pragma unroll
-        for (int i = 0; i < NV; i++) sum[i] = wave_sum(sum[i]);
-        if (threadIdx.x == 0) {
-            if (err)
-                __hip_atomic_store(&a.ctl->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-            for (int i = 0; i < NV; i++) lds[i] = sum[i];
-            *flag = err;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NV; i++) v[i] = lds[i];
-    const bool ok = *flag == 0;
-    __syncthreads();  // lds is reused by the next reduction
-    return ok;
+    return grid_sum<NV, kResWaves>(v, a.G, a.part, a.ctl, m, lds, flag);
 }
 
 // ------------------------------------------------------------------------------------

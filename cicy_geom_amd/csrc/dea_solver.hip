@@ -566,9 +566,10 @@ int cicy_set_rows_per_block(cicy_ctx *ctx, int rb) {
 int cicy_set_dropin_method(cicy_ctx *ctx, int method) {
     if (!ctx) return 0;
     if (method != CICY_METHOD_PCG_F64 && method != CICY_METHOD_RESIDENT &&
-        method != CICY_METHOD_CONDENSED && method != CICY_METHOD_CONDENSED_RESIDENT) {
-        cicy::set_error(ctx, "drop-in method must be PCG_F64, RESIDENT, CONDENSED or "
-                             "CONDENSED_RESIDENT");
+        method != CICY_METHOD_CONDENSED && method != CICY_METHOD_CONDENSED_RESIDENT &&
+        method != CICY_METHOD_CONDENSED_BLOCK) {
+        cicy::set_error(ctx, "drop-in method must be PCG_F64, RESIDENT, CONDENSED, "
+                             "CONDENSED_RESIDENT or CONDENSED_BLOCK");
         return 0;
     }
     ctx->dropin_method = method;
@@ -746,6 +747,10 @@ int64_t cicy_sys_size(const cicy_sys *s, int what) {
         case 11: return cicy::condensed_info(s->cond, 2);
         case 12: return cicy::condensed_info(s->cond, 3);
         case 13: return cicy::condensed_info(s->cond, 4);
+        case 14: return cicy::condensed_info(s->cond, 5);
+        case 15: return cicy::condensed_info(s->cond, 6);
+        case 16: return cicy::condensed_info(s->cond, 7);
+        case 17: return cicy::condensed_info(s->cond, 8);
         default: return -1;
     }
 }
@@ -772,6 +777,12 @@ int cicy_sys_set_rhs(cicy_sys *s, const double *r0, const double *x0, const doub
     return ok ? 1 : 0;
 }
 
+int cicy_sys_set_blocks(cicy_sys *s, int64_t nblocks, const int64_t *ranges) {
+    if (!s) return 0;
+    (void)hipSetDevice(s->ctx->device);
+    return cicy::condensed_set_blocks(s, nblocks, ranges);
+}
+
 int cicy_sys_solve(cicy_sys *s, int method, double tol, int64_t max_iter, int rel_to,
                    cicy_solve_stats *stats) {
     if (!s) return 0;
@@ -792,6 +803,8 @@ int cicy_sys_solve(cicy_sys *s, int method, double tol, int64_t max_iter, int re
         ok = cicy::solve_condensed(s, CICY_METHOD_PCG_F64, tol, max_iter, rel_to, stats);
     else if (method == CICY_METHOD_CONDENSED_RESIDENT)
         ok = cicy::solve_condensed(s, CICY_METHOD_RESIDENT, tol, max_iter, rel_to, stats);
+    else if (method == CICY_METHOD_CONDENSED_BLOCK)
+        ok = cicy::solve_condensed(s, CICY_METHOD_CONDENSED_BLOCK, tol, max_iter, rel_to, stats);
     else {
         set_error(s->ctx, "unknown method %d", method);
         ok = false;

@@ -15,9 +15,9 @@ from ._libs import dea_lib
 
 CICY_OK, CICY_MAXITER, CICY_BREAKDOWN, CICY_BAD_INPUT, CICY_DEVICE_ERROR = 0, 1, 2, 3, 4
 METHOD_PCG_F64, METHOD_MIXED, METHOD_RESIDENT = 0, 1, 2
-METHOD_CONDENSED, METHOD_CONDENSED_RESIDENT = 3, 4
+METHOD_CONDENSED, METHOD_CONDENSED_RESIDENT, METHOD_CONDENSED_BLOCK = 3, 4, 5
 METHOD_NAMES = {0: "pcg_f64", 1: "mixed", 2: "resident", 3: "condensed",
-                4: "condensed_resident"}
+                4: "condensed_resident", 5: "condensed_block"}
 FORMAT_AUTO, FORMAT_SSS, FORMAT_BAND = 0, 1, 2
 REL_R0, REL_B = 0, 1
 STATUS_NAMES = {0: "ok", 1: "maxiter", 2: "breakdown", 3: "bad_input", 4: "device_error"}
@@ -40,7 +40,7 @@ EXPORTED_SYMBOLS = [
     "cicy_mpnn_forward", "cicy_mpnn_backward", "cicy_mpnn_adam_step", "cicy_mpnn_warm_start",
     "cicy_host_blocked_spmv", "cicy_set_resident_rows", "cicy_host_resident_apply",
     "cicy_host_condense", "cicy_host_expand", "cicy_host_core", "cicy_host_core_expand",
-    "cicy_sys_core", "cicy_set_dropin_method",
+    "cicy_sys_core", "cicy_set_dropin_method", "cicy_sys_set_blocks", "cicy_host_core_block_cg",
 ]
 
 
@@ -109,6 +109,11 @@ def lib():
                                  ctypes.c_int, P(SolveStats)]
     L.cicy_sys_get.argtypes = [_VP, _VP, _VP]
     L.cicy_sys_core.argtypes = [_VP, P(_VP)]
+    L.cicy_sys_set_blocks.argtypes = [_VP, ctypes.c_int64, _VP]
+    L.cicy_host_core_block_cg.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64,
+                                          ctypes.c_int64, ctypes.c_int64, _VP, _F64,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int64,
+                                          _F64, P(ctypes.c_int64), P(ctypes.c_int64)]
     L.cicy_sys_spmv_bench.argtypes = [_VP, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
     L.cicy_host_blocked_spmv.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
                                          ctypes.c_int, _F64, _F64]
@@ -302,7 +307,16 @@ class System:
             self._h, {"n": 0, "nnz": 1, "remote": 2, "bytes": 3, "format": 4,
                       "off_layout": 5, "resident_rows": 6, "resident_groups": 7,
                       "published": 8, "condensed_rows": 9, "condensed_nnz": 10,
-                      "core_rows": 11, "core_nnz": 12, "series_rounds": 13}[what]))
+                      "core_rows": 11, "core_nnz": 12, "series_rounds": 13,
+                      "core_block_rows": 14, "core_blocks": 15, "core_groups": 16,
+                      "core_block_entries": 17}[what]))
+
+    def set_blocks(self, ranges):
+        """Row groups [(start, end), ...] of free-node rows (e.g. the nodes of each
+        nanoplatelet) that METHOD_CONDENSED_BLOCK preconditions with exact block inverses."""
+        r = np.ascontiguousarray(np.asarray(ranges, np.int64).reshape(-1, 2))
+        self.ctx._check(self.ctx._L.cicy_sys_set_blocks(
+            self._h, len(r), r.ctypes.data if len(r) else None), "cicy_sys_set_blocks")
 
     def set_rhs(self, r0, x0=None, b=None):
         r0 = _f64(r0)
@@ -464,6 +478,25 @@ def host_expand(row_ptr, col_ind, values, diagonal, rhs, x_anchor):
     if not ok:
         raise DEAError("cicy_host_expand: nothing to condense or invalid input")
     return x[:n]
+
+
+def host_core_block_cg(row_ptr, col_ind, values, diagonal, rhs, ranges=(), tol=1e-10,
+                       ref2=None, max_iter=0):
+    """The block-preconditioned core CG (layout + the kernel's recurrences) on the host:
+    (x, iterations, blocked rows); raises if it did not converge."""
+    n = len(diagonal)
+    r = np.ascontiguousarray(np.asarray(ranges, np.int64).reshape(-1, 2))
+    x = np.zeros(max(n, 1))
+    it, br = ctypes.c_int64(), ctypes.c_int64()
+    rhs = _f64(rhs)
+    ref2 = float(rhs @ rhs) if ref2 is None else ref2
+    ok = lib().cicy_host_core_block_cg(n, _i64(row_ptr), _i64(col_ind), _f64(values),
+                                       _f64(diagonal), len(values), len(r),
+                                       r.ctypes.data if len(r) else None, rhs, tol, ref2,
+                                       max_iter, x, ctypes.byref(it), ctypes.byref(br))
+    if not ok:
+        raise DEAError("cicy_host_core_block_cg did not converge")
+    return x[:n], it.value, br.value
 
 
 def host_core(row_ptr, col_ind, values, diagonal, rhs=None):

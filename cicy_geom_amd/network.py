@@ -410,6 +410,21 @@ def largest_percolating_system(net: Network, direction: int = 0, r_flag: int = 1
     return net.assemble(k, r_flag, direction, elec, direction=direction, with_xyz=with_xyz)
 
 
+def plate_row_ranges(net: Network, cluster: int, s: DEASystem) -> np.ndarray:
+    """[(start, end), ...] free-node rows of each nanoplatelet of the cluster: the
+    reference numbers a plate's points consecutively (LM_matrix_for_gnps,
+    Direct_Electrifying.cpp:307-349), so each plate is one contiguous range."""
+    out = []
+    for g in net.cluster_gnps(cluster):
+        nodes = s.lmm_gnp[net.gnp_structure(int(g))]
+        nodes = nodes[nodes >= s.reserved] - s.reserved
+        if len(nodes):
+            lo, hi = int(nodes.min()), int(nodes.max()) + 1
+            assert hi - lo == len(nodes), "plate nodes are not contiguous"
+            out.append((lo, hi))
+    return np.array(out, np.int64).reshape(-1, 2)
+
+
 def spmv_bytes(n: int, nnz: int, sv: int = 8, si: int = 4) -> int:
     """Algorithmic bytes of one SSS SpMV: L(sv+si) + N(sv+si) + 2 N sv."""
     return nnz * (sv + si) + n * (sv + si) + 2 * n * sv
@@ -422,4 +437,5 @@ def pcg_iter_bytes(n: int, nnz: int, sv: int = 8, si: int = 4) -> int:
 
 __all__ = ["ElectricParams", "NetworkParams", "Network", "DEASystem", "config_params",
            "generate", "largest_percolating_system", "tunnel_constants", "spmv_bytes",
+           "plate_row_ranges",
            "pcg_iter_bytes", "math"]

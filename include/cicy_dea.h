@@ -55,6 +55,13 @@ typedef struct cicy_sys cicy_sys;
                                     by back-substitution (cicy_sys_size 9 > 0) */
 #define CICY_METHOD_CONDENSED_RESIDENT 4  /* the same with the resident CG on the reduced
                                              system */
+#define CICY_METHOD_CONDENSED_BLOCK 5  /* the same eliminations; the core is solved by one
+                                          persistent cooperative launch per chunk of
+                                          iterations (Chronopoulos-Gear PCG, two grid
+                                          barriers per iteration) preconditioned by exact
+                                          dense inverses of the row groups given by
+                                          cicy_sys_set_blocks (e.g. the nodes of each
+                                          nanoplatelet) and Jacobi on the other rows */
 
 /* stopping rule (cicy_sys_solve rel_to) */
 #define CICY_REL_R0 0  /* ||r_k|| <= tol*||r_0||, r_0 = b - A x0: the reference (:1571,:1619) */
@@ -145,7 +152,12 @@ int64_t cicy_sys_size(const cicy_sys *sys, int what); /* 0 n, 1 nnz, 2 remote en
                                                            (0: no condensed system), 10 their
                                                            stored entries, 11 core rows (the
                                                            system CG runs on), 12 core stored
-                                                           entries, 13 series rounds */
+                                                           entries, 13 series rounds,
+                                                           14 core rows in dense blocks,
+                                                           15 dense blocks, 16 work-groups
+                                                           of the block-preconditioned core
+                                                           CG (0: not built), 17 entries of
+                                                           its dense block inverses */
 /* r0 = b - A x0 (the reference's R), x0 (may be NULL = zeros), b (may be NULL; needed for
  * the true residual and for CICY_REL_B). */
 int cicy_sys_set_rhs(cicy_sys *sys, const double *r0, const double *x0, const double *b);
@@ -153,6 +165,15 @@ int cicy_sys_set_rhs(cicy_sys *sys, const double *r0, const double *x0, const do
 int cicy_sys_solve(cicy_sys *sys, int method, double tol, int64_t max_iter, int rel_to,
                    cicy_solve_stats *stats);
 int cicy_sys_get(cicy_sys *sys, double *x, double *r);
+/* Row groups of the full system whose core rows CICY_METHOD_CONDENSED_BLOCK
+ * preconditions with the exact inverse of their diagonal block: ranges[2 b] .. ranges[2 b
+ * + 1] (end exclusive) of free-node rows, non-overlapping.  The reference numbers the
+ * points of each nanoplatelet consecutively (LM_matrix_for_gnps, Direct_Electrifying.cpp:
+ * 307-349), so each plate's node range is one group; its edges conduct ~1e7 times more than
+ * its junctions, which Jacobi scaling cannot balance.  Groups longer than 128 core rows are
+ * split.  nblocks = 0 gives Jacobi everywhere.  Needs a condensed system (cicy_sys_size 9
+ * > 0).  No counterpart in the reference. */
+int cicy_sys_set_blocks(cicy_sys *sys, int64_t nblocks, const int64_t *ranges);
 /* The core system a CICY_METHOD_CONDENSED solve runs CG on, as a borrowed handle (owned by
  * sys, valid while sys lives; do not destroy it), for measuring its kernels; 0 if sys has
  * no condensed system.  No counterpart in the reference. */
@@ -241,6 +262,15 @@ int cicy_host_resident_apply(int64_t n, const int64_t *row_ptr, const int64_t *c
  * [*nnz_out], red_diagonal [m]) and, when rhs is given, the reduced right-hand side [m].
  * Buffers are sized by the caller for the worst case (m <= n, nnz_out <= nnz + n).
  * Returns 0 when there is nothing worth condensing.  For layout tests without a GPU. */
+/* The block-preconditioned core CG of CICY_METHOD_CONDENSED_BLOCK on the host: the layout
+ * (blocks, dense inverses) and the kernel's recurrences with sequential sums, on an SSS
+ * system of m rows solved from x = 0 until ||r|| <= tol sqrt(ref2).  Returns 1 when it
+ * converged.  For tests without a GPU. */
+int cicy_host_core_block_cg(int64_t m, const int64_t *row_ptr, const int64_t *col_ind,
+                            const double *values, const double *diagonal, int64_t nnz,
+                            int64_t nblocks, const int64_t *ranges, const double *rhs,
+                            double tol, double ref2, int64_t max_iter, double *x,
+                            int64_t *iters, int64_t *blocked_rows);
 int cicy_host_condense(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
                        const double *values, const double *diagonal, int64_t nnz,
                        const double *rhs, int64_t *m_out, int64_t *nnz_out, int64_t *anchors,

@@ -1,0 +1,162 @@
+"""GPU parity of CICY_METHOD_CONDENSED_BLOCK (persistent block-preconditioned core CG)
+through the C-ABI: networks with and without nanoplatelets against a sparse direct solve,
+the host form of the same recurrences, repeatability and the error paths.
+
+Bars as in test_gpu_condensed.py: ||b - A x|| <= 1e-10 ||b|| (oracle product) and the
+voltages no further from the truth than the reference algorithm's own solve at the same
+stopping rule, within a factor 3."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as sla
+
+import oracle
+from cicy_geom_amd import dea, network
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dea.Context(0)
+    yield c
+    c.close()
+
+
+def direct(s):
+    rows = np.repeat(np.arange(s.n), np.diff(s.row_ptr))
+    L = sp.csr_matrix((s.values, (rows, s.col_ind)), shape=(s.n, s.n))
+    return sla.spsolve((L + L.T + sp.diags(s.diag)).tocsc(), s.b)
+
+
+def plate_case():
+    p = network.NetworkParams(box=2.0, n_cnt=2560, n_gnp=31, seed=5)
+    net = network.Network(p)
+    k = max(net.percolating_clusters(0), key=lambda c: net.cluster(c)[0])
+    s = net.assemble(k, 1, 0, direction=0)
+    ranges = network.plate_row_ranges(net, k, s)
+    net.close()
+    return s, ranges
+
+
+@pytest.fixture(scope="module")
+def cases():
+    out = {"plates": plate_case()}
+    out["c1"] = (network.largest_percolating_system(network.generate(1), 0),
+                 np.zeros((0, 2), np.int64))
+    net = network.Network(network.config_params(4, 0, 6000))
+    k = max(net.percolating_clusters(0), key=lambda q: net.cluster(q)[0])
+    out["rve6k"] = (net.assemble(k, 1, 0, direction=0), np.zeros((0, 2), np.int64))
+    net.close()
+    return out
+
+
+@pytest.fixture(scope="module")
+def truths(cases):
+    return {k: direct(s) for k, (s, _) in cases.items()}
+
+
+def ref_error(s, truth):
+    tol = 1e-10 * np.linalg.norm(s.b) / np.linalg.norm(s.r0)
+    ref = oracle.pcg_sss(s.nodes, s.reserved, tol, s.row_ptr, s.col_ind, s.values, s.diag, s.r0,
+                         s.vef, s.x0)
+    return np.linalg.norm(ref["x"] - truth) / np.linalg.norm(truth)
+
+
+def solve(ctx, s, ranges, blocks=True):
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    sysd.set_rhs(s.b, None, s.b)
+    if blocks:
+        sysd.set_blocks(ranges)
+    st = sysd.solve(dea.METHOD_CONDENSED_BLOCK, 1e-10, 0, dea.REL_B)
+    x, _ = sysd.get()
+    info = {k: sysd.size(k) for k in ("core_rows", "core_block_rows", "core_blocks",
+                                      "core_groups")}
+    sysd.close()
+    return st, x, info
+
+
+@pytest.mark.parametrize("name", ["plates", "c1", "rve6k"])
+def test_condensed_block_solution(ctx, cases, truths, name):
+    s, ranges = cases[name]
+    print(f"[case] {name} N={s.n} plates={len(ranges)}", flush=True)
+    st, x, info = solve(ctx, s, ranges)
+    ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
+    tr = np.linalg.norm(s.b - ax) / np.linalg.norm(s.b)
+    err = np.linalg.norm(x - truths[name]) / np.linalg.norm(truths[name])
+    err_ref = ref_error(s, truths[name])
+    print(name, st.as_dict(), info, "true res", tr, "err", err, "reference err", err_ref)
+    assert st.status == dea.CICY_OK
+    assert tr <= 1e-10 * (1 + 1e-6)
+    assert err <= max(3.0 * err_ref, 1e-9)
+    if len(ranges):
+        assert info["core_blocks"] > 0 and info["core_block_rows"] > 0
+    else:
+        assert info["core_blocks"] == 0 and info["core_block_rows"] == 0
+
+
+def test_iterations_match_the_host_form(ctx, cases):
+    s, ranges = cases["plates"]
+    st, _, info = solve(ctx, s, ranges)
+    rows, crp, ccol, cval, cdiag, crhs, _ = dea.host_core(s.row_ptr, s.col_ind, s.values,
+                                                         s.diag, s.b)
+    cr = []
+    for a, b in ranges:
+        q0, q1 = np.searchsorted(rows, a), np.searchsorted(rows, b)
+        if q1 - q0 >= 2:
+            cr.append((q0, q1))
+    _, it, br = dea.host_core_block_cg(crp, ccol, cval, cdiag, crhs, cr, 1e-10,
+                                       float(s.b @ s.b))
+    print("device", st.iters, "host", it, "blocked rows", info["core_block_rows"], br)
+    assert info["core_block_rows"] == br
+    # the same recurrences; only the order of the dot-product sums differs
+    assert abs(st.iters - it) <= max(3, 0.05 * it)
+
+
+def test_plate_blocks_cut_device_iterations(ctx, cases):
+    s, ranges = cases["plates"]
+    st_b, _, _ = solve(ctx, s, ranges)
+    st_j, _, _ = solve(ctx, s, ranges, blocks=False)
+    print("blocks", st_b.iters, "jacobi", st_j.iters)
+    assert st_j.status == dea.CICY_OK
+    assert st_b.iters * 10 < st_j.iters
+
+
+def test_repeat_solves_are_bit_identical(ctx, cases):
+    s, ranges = cases["plates"]
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    sysd.set_rhs(s.b, None, s.b)
+    sysd.set_blocks(ranges)
+    xs = []
+    for _ in range(2):
+        st = sysd.solve(dea.METHOD_CONDENSED_BLOCK, 1e-10, 0, dea.REL_B)
+        assert st.status == dea.CICY_OK
+        xs.append(sysd.get()[0])
+    sysd.close()
+    assert np.array_equal(xs[0], xs[1])
+
+
+def test_set_blocks_rejects_bad_groups(ctx, cases):
+    s, ranges = cases["plates"]
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    with pytest.raises(dea.DEAError):
+        sysd.set_blocks([(0, s.n + 5)])
+    a, b = ranges[0]
+    with pytest.raises(dea.DEAError):
+        sysd.set_blocks([(a, b), (a + 1, b + 1)])
+    sysd.close()
+
+
+def test_dropin_with_condensed_block(ctx, cases):
+    s, _ = cases["rve6k"]
+    ctx.set_dropin_method(dea.METHOD_CONDENSED_BLOCK)
+    try:
+        ok, volt, x, r, st = ctx.solve_dea_cg_sss(s.nodes, s.reserved, 1e-10, s.col_ind,
+                                                  s.row_ptr, s.values, s.diag, s.r0, s.vef,
+                                                  s.x0)
+    finally:
+        ctx.set_dropin_method(dea.METHOD_PCG_F64)
+    assert ok and st.status == dea.CICY_OK
+    rr = oracle.update_residual(s.row_ptr, s.col_ind, s.values, s.diag, x, s.b)
+    assert np.linalg.norm(rr) <= 1e-10 * np.linalg.norm(s.r0) * (1 + 1e-6)
+    assert np.array_equal(volt[:s.reserved], s.vef)
