@@ -53,7 +53,12 @@ struct cicy_cond {
     int mc = 0, nrec = 0;
     long long nnz_core = 0;
     std::vector<int> round_off;                       // host copy: launch bounds per round
+    std::vector<int> rv_round_off;                    // host copy: receivers of each round
     int *e_node = nullptr, *e_off = nullptr, *e_nbr = nullptr, *core_nodes = nullptr;
+    // forward sweep by receiver: rv_node[j] gathers rv_a[c] * f_k / d_k over its
+    // contributions c in rv_off[j] .. rv_off[j+1] (record rv_rec[c]), in record order
+    int *rv_node = nullptr, *rv_off = nullptr, *rv_rec = nullptr;
+    double *rv_a = nullptr;
     double *e_a = nullptr, *e_d = nullptr;
     double *fS = nullptr, *xS = nullptr;              // [m] right-hand side / solution on S
     cicy_sys *red = nullptr;    // the core system on the device
@@ -130,19 +135,27 @@ __global__ __launch_bounds__(kCondThreads) void k_scatter(int m, const int *__re
     if (j < m) x[anchors[j]] = xj[j];
 }
 
-// one round of the forward sweep: f_u -= a_uk f_k / d_k for every neighbour u of k (nodes
-// of a round are pairwise non-adjacent, so f_k is final; a shared neighbour may get several
-// contributions, hence the atomics)
-__global__ __launch_bounds__(kCondThreads) void k_series_fwd(int b, int e, const int *__restrict__ node,
-                                                             const int *__restrict__ off,
-                                                             const int *__restrict__ nbr,
-                                                             const double *__restrict__ a,
+// one round of the forward sweep: f_u -= a_uk f_k / d_k for every neighbour u of an
+// eliminated node k.  Nodes of a round are pairwise non-adjacent, so f_k is final; a node u
+// next to several of them gathers their contributions itself, in record order (one thread
+// per receiving node, no atomics), so the sweep is deterministic and equals the host form
+// bit for bit.
+__global__ __launch_bounds__(kCondThreads) void k_series_fwd(int b, int e, const int *__restrict__ rv_node,
+                                                             const int *__restrict__ rv_off,
+                                                             const int *__restrict__ rv_rec,
+                                                             const double *__restrict__ rv_a,
+                                                             const int *__restrict__ node,
                                                              const double *__restrict__ dd,
                                                              double *__restrict__ f) {
-    const int k = b + blockIdx.x * kCondThreads + threadIdx.x;
-    if (k >= e) return;
-    const double fk = f[node[k]] / dd[k];
-    for (int i = off[k]; i < off[k + 1]; i++) atomicAdd(&f[nbr[i]], -a[i] * fk);
+    const int j = b + blockIdx.x * kCondThreads + threadIdx.x;
+    if (j >= e) return;
+    const int u = rv_node[j];
+    double r = f[u];
+    for (int c = rv_off[j]; c < rv_off[j + 1]; c++) {
+        const int k = rv_rec[c];
+        r -= rv_a[c] * (f[node[k]] / dd[k]);
+    }
+    f[u] = r;
 }
 
 // one round of the back-substitution: x_k = (f_k - sum_u a_uk x_u) / d_k
@@ -227,7 +240,7 @@ void condensed_free(cicy_cond *c) {
     void *ptrs[] = {c->ps, c->pe, c->pl, c->pr, c->anchor_of, c->anchors, c->lfac, c->mfac,
                     c->band, c->y, c->delta, c->rhs, c->norm, c->counter, c->part, c->gpart,
                     c->gcount, c->e_node, c->e_off, c->e_nbr, c->e_a, c->e_d,
-                    c->core_nodes, c->fS, c->xS};
+                    c->core_nodes, c->fS, c->xS, c->rv_node, c->rv_off, c->rv_rec, c->rv_a};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -615,6 +628,33 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
     c->mc = Z.mc;
     c->nrec = (int)Z.e_node.size();
     c->round_off = Z.round_off;
+    // receivers of each round, each with its contributions in record order
+    std::vector<int> rv_node, rv_off(1, 0), rv_rec;
+    std::vector<double> rv_a;
+    c->rv_round_off.assign(1, 0);
+    {
+        struct Con { int u, k, i; };
+        std::vector<Con> con;
+        for (size_t q = 0; q + 1 < Z.round_off.size(); q++) {
+            con.clear();
+            for (int k = Z.round_off[q]; k < Z.round_off[q + 1]; k++)
+                for (int i = Z.e_off[k]; i < Z.e_off[k + 1]; i++) con.push_back({Z.e_nbr[i], k, i});
+            std::stable_sort(con.begin(), con.end(),
+                             [](const Con &x, const Con &y) { return x.u < y.u; });
+            for (size_t t = 0; t < con.size();) {
+                size_t e = t;
+                while (e < con.size() && con[e].u == con[t].u) {
+                    rv_rec.push_back(con[e].k);
+                    rv_a.push_back(Z.e_a[con[e].i]);
+                    e++;
+                }
+                rv_node.push_back(con[t].u);
+                rv_off.push_back((int)rv_rec.size());
+                t = e;
+            }
+            c->rv_round_off.push_back((int)rv_node.size());
+        }
+    }
     c->nnz_core = (long long)Z.cval.size();
     if (ok && m > 0) {
         ok = calloc_dev(ctx, &c->e_node, Z.e_node.size()) &&
@@ -625,6 +665,11 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
              up(ctx, c->e_node, Z.e_node) && up(ctx, c->e_off, Z.e_off) &&
              up(ctx, c->e_nbr, Z.e_nbr) && up(ctx, c->e_a, Z.e_a) && up(ctx, c->e_d, Z.e_d) &&
              up(ctx, c->core_nodes, Z.core_nodes) &&
+             calloc_dev(ctx, &c->rv_node, rv_node.size()) &&
+             calloc_dev(ctx, &c->rv_off, rv_off.size()) &&
+             calloc_dev(ctx, &c->rv_rec, rv_rec.size()) &&
+             calloc_dev(ctx, &c->rv_a, rv_a.size()) && up(ctx, c->rv_node, rv_node) &&
+             up(ctx, c->rv_off, rv_off) && up(ctx, c->rv_rec, rv_rec) && up(ctx, c->rv_a, rv_a) &&
              hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
         if (ok && Z.mc > 0) {
             // a core row with a vanishing diagonal (a floating group of tubes) is refused by
@@ -674,9 +719,11 @@ static bool condensed_correction(cicy_sys *s, int inner, const double *rhs, doub
                            st, c->m, c->n, c->anchors, c->anchor_of, c->band, rhs, c->y, c->fS);
         const int nr = (int)c->round_off.size() - 1;
         for (int q = 0; q < nr; q++) {
-            const int b = c->round_off[q], e = c->round_off[q + 1];
-            hipLaunchKernelGGL(k_series_fwd, dim3(blocks(e - b, kCondThreads)), dim3(kCondThreads),
-                               0, st, b, e, c->e_node, c->e_off, c->e_nbr, c->e_a, c->e_d, c->fS);
+            const int b = c->rv_round_off[q], e = c->rv_round_off[q + 1];
+            if (e > b)
+                hipLaunchKernelGGL(k_series_fwd, dim3(blocks(e - b, kCondThreads)),
+                                   dim3(kCondThreads), 0, st, b, e, c->rv_node, c->rv_off,
+                                   c->rv_rec, c->rv_a, c->e_node, c->e_d, c->fS);
         }
         if (c->mc > 0) {
             cicy_sys *red = c->red;
@@ -792,7 +839,10 @@ bool solve_condensed(cicy_sys *s, int inner, double tol, long long max_iter, int
             status = CICY_BREAKDOWN;
             break;
         }
-        if (passes >= 3 || total >= max_iter) break;
+        // the block method's pipelined recursion may stall on a badly preconditioned core
+        // and is then corrected from the true residual; the others are expected to converge
+        const int max_passes = inner == CICY_METHOD_CONDENSED_BLOCK ? 10 : 3;
+        if (passes >= max_passes || total >= max_iter) break;
         passes++;
         ok = hip_ok(ctx, hipMemcpyAsync(c->rhs, s->r, vb, hipMemcpyDeviceToDevice, st), "copy r");
     }

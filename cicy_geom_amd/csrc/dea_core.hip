@@ -8,19 +8,26 @@
 // that Jacobi scaling leaves at an eigenvalue ~1e-7 and CG needs tens of thousands of
 // iterations.  Here the core is solved by
 //
-//   * a Chronopoulos-Gear formulation of PCG (one fused pair of dot products per
-//     reduction, two grid barriers per iteration) run as one cooperative launch per chunk
-//     of iterations, every work-group resident on its own CU;
+//   * the Chronopoulos-Gear formulation of PCG (both dot products of a step summed at one
+//     reduction, two grid barriers per iteration), run as one cooperative launch per chunk
+//     of iterations, every work-group resident on its own CU.  (The pipelined variant with
+//     one barrier per iteration loses too much attainable accuracy on these systems: its
+//     recursively updated vectors drift and the recursion stalls before 1e-10.);
 //   * block-Jacobi preconditioning: row groups given by the caller (cicy_sys_set_blocks;
 //     the nodes of one nanoplatelet, which the reference numbers consecutively in
 //     LM_matrix_for_gnps, Direct_Electrifying.cpp:307-349) are inverted exactly (dense
-//     FP64 inverse of the core's diagonal block, computed once per system), the other rows
-//     use the reference's Jacobi preconditioner (Direct_Electrifying.cpp:1662-1676).
-//     Each work-group owns whole blocks, so the block solve is a work-group-local dense
-//     product with the block's residual staged in LDS;
-//   * vectors in HBM: the preconditioned residual u, which other work-groups gather for the
-//     product w = A u, is written and read with device-coherent accesses (dea_grid.h); x,
-//     r, p, s, w are private to the owning work-group.
+//     inverse of the core's diagonal block, computed once per system in FP64), the other
+//     rows use the reference's Jacobi preconditioner (Direct_Electrifying.cpp:1662-1676).
+//     A plate's inverse is dominated by one constant c (its nearly rigid mode), so it is
+//     stored as c 11^T (FP64) plus the remainder R = A_bb^-1 - c 11^T in FP32: R's entries
+//     are small, its rounding (~1e-10 of ||R||) stays far below R's smallest eigenvalue, so
+//     the preconditioner stays symmetric positive definite at half the bytes.  Each
+//     work-group owns whole blocks, so the block solve is a work-group-local dense product
+//     with the block's residual staged in LDS;
+//   * vectors in HBM: u = M^-1 r, which other work-groups gather for the product w = A u,
+//     is written and read with device-coherent accesses (dea_grid.h); x, r, w, p, s are
+//     private to the owning work-group.  The product reads the matrix in sliced ELL (64-row
+//     slices, slot-major) so that its column and value loads are coalesced.
 //
 // Iteration (u = M^-1 r, w = A u, gamma = r.u, delta = w.u; Chronopoulos & Gear 1989):
 //   p = u + beta p;  s = w + beta s;  x += alpha p;  r -= alpha s;  u = M^-1 r
@@ -28,6 +35,9 @@
 //   w = A u
 //   [barrier: delta = w.u]
 //   beta = gamma' / gamma;  alpha = gamma' / (delta - beta gamma' / alpha)
+// If gamma or the denominator of alpha loses positivity (rounding on a badly
+// preconditioned core) the solve stops as "stalled" and the condensed solve, which checks
+// the full system's true residual after every pass, corrects again from it.
 // Sums over work-groups are taken in a fixed order, so solves are bit-reproducible.
 #include "dea_grid.h"
 #include "dea_internal.h"
@@ -44,36 +54,50 @@ using namespace cicy;
 
 namespace cicy {
 
-constexpr int kCoreThreads = 512;
+constexpr int kCoreThreads = 1024;
 constexpr int kCoreWaves = kCoreThreads / kWave;
+constexpr int kCorePairs = kCoreWaves / 2;  // blocks per round: two waves share one block
 constexpr int kCoreMaxBlock = 128;     // rows of a dense block (larger groups are split)
 constexpr int kCoreChunk = 4096;       // iterations per cooperative launch (at most)
 constexpr int kCoreRowsPerGroup = 192; // work-groups: about one per this many rows
+constexpr int kCoreUnroll = 12;        // block-inverse columns loaded before use
+constexpr int kCoreEnt = 8;            // row slots loaded before their gathers
+constexpr int kCoreSums = 4;           // entries per work-group slot of the grid sums
+constexpr int kCorePasses = 10;        // corrections from the true residual (stalled solves)
 
 struct CoreState {
     long long k;         // iterations done in this solve
     long long max_iter;
     double gamma, alpha, beta, rr, stop2;
-    int done;            // 0 running, 1 converged, 2 iteration cap, 3 breakdown, 4 barrier timeout
+    int done;            // 0 running, 1 converged, 2 iteration cap, 3 breakdown (not finite),
+                         // 4 barrier timeout, 5 stalled (gamma or alpha lost positivity; the
+                         // caller corrects from the true residual)
     int started;         // the prologue (u0 = M^-1 r0, w0 = A u0) has run
 };
 
 struct CoreDev {
     int m, G;
     const int *seg;       // [G+1] rows of each work-group (blocks are never split)
-    const int *rp;        // [m+1] full symmetric CSR without the diagonal
-    const int *col;
-    const double *val;
+    // the off-diagonal entries of both triangles in sliced ELL: work-group g's rows are
+    // cut into slices of 64 rows (wsl[g] .. wsl[g+1]); slice s holds sl_len[s] slots, slot
+    // t of its lane-th row at ell_*[sl_off[s] + t * 64 + lane] (column -1: padding), in
+    // the row's column order
+    const int *wsl;
+    const long long *sl_off;
+    const int *sl_len;
+    const int *ell_col;
+    const double *ell_val;
     const double *diag;   // [m]
     const int *blk_of;    // [m] block of each row, -1 for Jacobi rows
     const int *wblk;      // [G+1] blocks of each work-group
     const int *blk_row0;  // [nb]
     const int *blk_n;     // [nb]
-    const long long *blk_off;  // [nb] offset of the block's inverse (row-major, symmetric)
-    const double *binv;
+    const long long *blk_off;  // [nb] offset of the block's remainder (row-major, symmetric)
+    const double *blk_c;  // [nb] constant part c of the block's inverse
+    const float *binv;    // remainders R = A_bb^-1 - c 11^T
     double *x, *r, *u, *w, *p, *s;
     GridCtl *ctl;
-    double *part;         // [2][G][2]
+    double *part;         // [2][G][kCoreSums]
 };
 
 }  // namespace cicy
@@ -81,10 +105,12 @@ struct CoreDev {
 struct cicy_core {
     int m = 0, G = 0, nb = 0, blocked_rows = 0;
     long long nnz2 = 0, inv_entries = 0;
-    int *seg = nullptr, *rp = nullptr, *col = nullptr, *blk_of = nullptr, *wblk = nullptr;
+    int *seg = nullptr, *blk_of = nullptr, *wblk = nullptr;
     int *blk_row0 = nullptr, *blk_n = nullptr;
-    long long *blk_off = nullptr;
-    double *val = nullptr, *diag = nullptr, *binv = nullptr;
+    int *wsl = nullptr, *sl_len = nullptr, *ell_col = nullptr;
+    long long *blk_off = nullptr, *sl_off = nullptr;
+    double *ell_val = nullptr, *diag = nullptr, *blk_c = nullptr;
+    float *binv = nullptr;
     double *x = nullptr, *r = nullptr, *u = nullptr, *w = nullptr, *p = nullptr, *s = nullptr;
     double *part = nullptr;
     cicy::GridCtl *ctl = nullptr;
@@ -102,7 +128,11 @@ namespace {
 struct CoreHost {
     int m = 0, G = 1;
     std::vector<int> rp, col, blk_of, seg, wblk, blk_row0, blk_n;
-    std::vector<double> val, diag, binv;
+    std::vector<int> wsl, sl_len, ell_col;
+    std::vector<long long> sl_off;
+    std::vector<double> ell_val;
+    std::vector<double> val, diag, blk_c;
+    std::vector<float> binv;
     std::vector<long long> blk_off;
 };
 
@@ -245,6 +275,7 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
     H.blk_row0.clear();
     H.blk_n.clear();
     H.blk_off.clear();
+    H.blk_c.clear();
     H.binv.clear();
     for (size_t g = 0; g < ng; g++) {
         if (!ok[g]) continue;
@@ -254,7 +285,12 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
         H.blk_row0.push_back(q0);
         H.blk_n.push_back(nb);
         H.blk_off.push_back((long long)H.binv.size());
-        H.binv.insert(H.binv.end(), inv.begin() + off[g], inv.begin() + off[g + 1]);
+        // constant part: the mean entry of the inverse
+        double c = 0.0;
+        for (long long e = off[g]; e < off[g + 1]; e++) c += inv[(size_t)e];
+        c /= (double)nb * (double)nb;
+        H.blk_c.push_back(c);
+        for (long long e = off[g]; e < off[g + 1]; e++) H.binv.push_back((float)(inv[(size_t)e] - c));
     }
     // work-group segments: equal shares of (rows + entries + dense block work), cut only
     // between rows that are not inside the same block
@@ -284,6 +320,30 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
         while (b < (int)H.blk_row0.size() && H.blk_row0[b] < H.seg[g + 1]) b++;
     }
     H.wblk[H.G] = (int)H.blk_row0.size();
+    // sliced ELL of each work-group's rows
+    H.wsl.assign(1, 0);
+    H.sl_off.clear();
+    H.sl_len.clear();
+    H.ell_col.clear();
+    H.ell_val.clear();
+    for (int gg = 0; gg < H.G; gg++) {
+        for (int base = H.seg[gg]; base < H.seg[gg + 1]; base += kWave) {
+            const int end = std::min(H.seg[gg + 1], base + kWave);
+            int len = 0;
+            for (int r = base; r < end; r++) len = std::max(len, H.rp[r + 1] - H.rp[r]);
+            const size_t off = H.ell_col.size();
+            H.sl_off.push_back((long long)off);
+            H.sl_len.push_back(len);
+            H.ell_col.resize(off + (size_t)len * kWave, -1);
+            H.ell_val.resize(off + (size_t)len * kWave, 0.0);
+            for (int r = base; r < end; r++)
+                for (int t = 0; t < H.rp[r + 1] - H.rp[r]; t++) {
+                    H.ell_col[off + (size_t)t * kWave + (r - base)] = H.col[(size_t)H.rp[r] + t];
+                    H.ell_val[off + (size_t)t * kWave + (r - base)] = H.val[(size_t)H.rp[r] + t];
+                }
+        }
+        H.wsl.push_back((int)H.sl_len.size());
+    }
     return true;
 }
 
@@ -293,10 +353,12 @@ void host_core_precond(const CoreHost &H, const double *v, double *out) {
         if (H.blk_of[i] < 0) out[i] = (1.0 / H.diag[i]) * v[i];
     for (size_t b = 0; b < H.blk_row0.size(); b++) {
         const int q0 = H.blk_row0[b], nb = H.blk_n[b];
-        const double *M = &H.binv[(size_t)H.blk_off[b]];
+        const float *M = &H.binv[(size_t)H.blk_off[b]];
+        double sum = 0.0;
+        for (int j = 0; j < nb; j++) sum += v[q0 + j];
         for (int i = 0; i < nb; i++) {
-            double z = 0.0;
-            for (int j = 0; j < nb; j++) z = fma(M[(size_t)j * nb + i], v[q0 + j], z);
+            double z = H.blk_c[b] * sum;
+            for (int j = 0; j < nb; j++) z = fma((double)M[(size_t)j * nb + i], v[q0 + j], z);
             out[q0 + i] = z;
         }
     }
@@ -310,8 +372,10 @@ void host_core_apply(const CoreHost &H, const double *v, double *out) {
     }
 }
 
+constexpr int kStalled = -1;  // host status of a stalled recursion (kernel done == 5)
+
 // The kernel's recurrences on the host (sequential sums): solves A x = rhs from x = 0 and
-// returns the iteration count (-1 on breakdown).
+// returns the iteration count.
 long long host_core_cg(const CoreHost &H, const double *rhs, double tol, double ref2,
                        long long max_iter, double *x, int *status) {
     const int m = H.m;
@@ -330,6 +394,10 @@ long long host_core_cg(const CoreHost &H, const double *rhs, double tol, double 
     host_core_apply(H, u.data(), w.data());
     double delta = dot(w, u);
     double alpha = gamma / delta, beta = 0.0;
+    if (!(alpha > 0.0) || !std::isfinite(alpha)) {
+        *status = kStalled;
+        return 0;
+    }
     long long k = 0;
     while (true) {
         for (int i = 0; i < m; i++) {
@@ -343,8 +411,12 @@ long long host_core_cg(const CoreHost &H, const double *rhs, double tol, double 
         rr = dot(r, r);
         k++;
         if (rr <= stop2) return k;
-        if (!std::isfinite(rr) || !(g2 > 0.0)) {
+        if (!std::isfinite(rr)) {
             *status = CICY_BREAKDOWN;
+            return k;
+        }
+        if (!(g2 > 0.0)) {
+            *status = kStalled;
             return k;
         }
         if (k >= max_iter) {
@@ -356,7 +428,7 @@ long long host_core_cg(const CoreHost &H, const double *rhs, double tol, double 
         beta = g2 / gamma;
         const double den = delta - beta * g2 / alpha;
         if (!(den > 0.0) || !std::isfinite(den)) {
-            *status = CICY_BREAKDOWN;
+            *status = kStalled;
             return k;
         }
         alpha = g2 / den;
@@ -371,13 +443,15 @@ long long host_core_cg(const CoreHost &H, const double *rhs, double tol, double 
 // One chunk of iterations.  All work-groups hold the same scalars (the grid sums are
 // bit-identical everywhere), so they take the same branches and leave together.
 __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *st, int chunk) {
-    __shared__ double rb[kCoreWaves][kCoreMaxBlock];
-    __shared__ double lds[2 * kCoreWaves];
+    __shared__ double yb[kCorePairs][kCoreMaxBlock];      // block input in flight
+    __shared__ double zp[kCorePairs][2][kCoreMaxBlock];   // its two half products
+    __shared__ double lds[kCoreSums * kCoreWaves];
     __shared__ int flag;
     const int g = blockIdx.x, tid = threadIdx.x;
     const int wave = tid / kWave, lane = tid & (kWave - 1);
     const int r0 = a.seg[g], r1 = a.seg[g + 1];
     const int b0 = a.wblk[g], b1 = a.wblk[g + 1];
+    const int s0 = a.wsl[g], s1 = a.wsl[g + 1];
     long long k = st->k;
     const long long max_iter = st->max_iter;
     double gamma = st->gamma, alpha = st->alpha, beta = st->beta, rr = st->rr;
@@ -386,13 +460,95 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
     const bool started = st->started != 0;
     unsigned bar = 0;
 
-    // u = M^-1 r for the rows of this work-group (after the vector update when UPDATE),
-    // partial r.u and r.r
+    // (A v)_row for this lane's row of slice sl, v gathered from src (coherent); own = v_row
+    auto row_product = [&](int sl, int row, bool live, const double *src, double &own) {
+        const int len = a.sl_len[sl];
+        const int *cc = a.ell_col + a.sl_off[sl] + lane;
+        const double *vv = a.ell_val + a.sl_off[sl] + lane;
+        own = live ? ld_coh(&src[row]) : 0.0;
+        double acc = live ? a.diag[row] * own : 0.0;
+        for (int t = 0; t < len; t += kCoreEnt) {
+            int c[kCoreEnt];
+            double v[kCoreEnt], gu[kCoreEnt];
+#pragma unroll
+            for (int e = 0; e < kCoreEnt; e++) {
+                const bool ok = t + e < len;
+                c[e] = ok ? cc[(t + e) * kWave] : -1;
+                v[e] = ok ? vv[(t + e) * kWave] : 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < kCoreEnt; e++) gu[e] = c[e] >= 0 ? ld_coh(&src[c[e]]) : 0.0;
+#pragma unroll
+            for (int e = 0; e < kCoreEnt; e++) acc = fma(v[e], gu[e], acc);
+        }
+        return acc;
+    };
+    auto blocks_apply = [&](const double *y, double *dst, double &pg, double &pr) {
+        // dst_b = M_b^-1 y_b for the blocks of this work-group (y: this work-group's rows,
+        // already stored), to dst with coherent stores; pg += y.dst, pr += y.y over them.
+        // Blocks go kCorePairs at a time; waves 2 q and 2 q + 1 share block base + q: each
+        // stages half of its rows, multiplies half of its columns, finalises half of its rows.
+        const int slot = wave >> 1, half = wave & 1;
+        for (int base = b0; base < b1; base += kCorePairs) {
+            const int bi = base + slot;
+            const bool act = bi < b1;
+            const int nbk = act ? a.blk_n[bi] : 0;
+            const int q0 = act ? a.blk_row0[bi] : 0;
+            const int jr = half * kWave + lane;  // this lane's row of the block
+            if (jr < nbk) yb[slot][jr] = y[q0 + jr];
+            __syncthreads();
+            if (act) {
+                // partial product over this wave's half of the columns: lane i accumulates
+                // rows i and i + 64 from column j of the symmetric remainder (consecutive
+                // lanes read consecutive addresses); the loads of kCoreUnroll columns are
+                // issued before their products, the last batch predicated.  Lanes past the
+                // block read a valid address and discard the result.
+                const float *M = a.binv + a.blk_off[bi];
+                const int hc = (nbk + 1) >> 1;
+                const int jlo = half ? hc : 0, jhi = half ? nbk : hc;
+                const int c0 = min(lane, nbk - 1), c1 = min(lane + kWave, nbk - 1);
+                double z0 = 0.0, z1 = 0.0;
+                for (int j = jlo; j < jhi; j += kCoreUnroll) {
+                    float m0[kCoreUnroll], m1[kCoreUnroll];
+#pragma unroll
+                    for (int t = 0; t < kCoreUnroll; t++) {
+                        const int jj = min(j + t, jhi - 1);
+                        m0[t] = M[(size_t)jj * nbk + c0];
+                        m1[t] = M[(size_t)jj * nbk + c1];
+                    }
+#pragma unroll
+                    for (int t = 0; t < kCoreUnroll; t++) {
+                        const double yj = j + t < jhi ? yb[slot][j + t] : 0.0;
+                        z0 = fma((double)m0[t], yj, z0);
+                        z1 = fma((double)m1[t], yj, z1);
+                    }
+                }
+                zp[slot][half][lane] = z0;
+                zp[slot][half][lane + kWave] = z1;
+            }
+            __syncthreads();
+            if (act) {
+                double sum = (lane < nbk ? yb[slot][lane] : 0.0) +
+                             (lane + kWave < nbk ? yb[slot][lane + kWave] : 0.0);
+                sum = wave_sum(sum);
+                sum = __shfl(sum, 0, kWave);
+                if (jr < nbk) {
+                    const double v = (a.blk_c[bi] * sum + zp[slot][0][jr]) + zp[slot][1][jr];
+                    const double yv = yb[slot][jr];
+                    st_coh(&dst[q0 + jr], v);
+                    pg += yv * v;
+                    pr += yv * yv;
+                }
+            }
+            __syncthreads();
+        }
+    };
+    // p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s (when update) and u = M^-1 r
+    // for the rows of this work-group; partial r.u and r.r
     auto phase_u = [&](bool update, double &pg, double &pr) {
         pg = 0.0;
         pr = 0.0;
         for (int i = r0 + tid; i < r1; i += kCoreThreads) {
-            if (a.blk_of[i] >= 0) continue;
             double ri = a.r[i];
             if (update) {
                 const double pi = fma(beta, a.p[i], ld_coh(&a.u[i]));
@@ -403,67 +559,28 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
                 ri = fma(-alpha, si, ri);
                 a.r[i] = ri;
             }
-            const double ui = (1.0 / a.diag[i]) * ri;
-            st_coh(&a.u[i], ui);
-            pg += ri * ui;
-            pr += ri * ri;
-        }
-        for (int base = b0; base < b1; base += kCoreWaves) {
-            const int bi = base + wave;
-            const bool act = bi < b1;
-            const int nbk = act ? a.blk_n[bi] : 0;
-            const int q0 = act ? a.blk_row0[bi] : 0;
-            for (int j = lane; j < nbk; j += kWave) {
-                const int i = q0 + j;
-                double ri = a.r[i];
-                if (update) {
-                    const double pi = fma(beta, a.p[i], ld_coh(&a.u[i]));
-                    const double si = fma(beta, a.s[i], a.w[i]);
-                    a.p[i] = pi;
-                    a.s[i] = si;
-                    a.x[i] = fma(alpha, pi, a.x[i]);
-                    ri = fma(-alpha, si, ri);
-                    a.r[i] = ri;
-                }
-                rb[wave][j] = ri;
+            if (a.blk_of[i] < 0) {
+                const double ui = (1.0 / a.diag[i]) * ri;
+                st_coh(&a.u[i], ui);
+                pg += ri * ui;
+                pr += ri * ri;
             }
-            __syncthreads();
-            if (act) {
-                const double *M = a.binv + a.blk_off[bi];
-                const bool h0 = lane < nbk, h1 = lane + kWave < nbk;
-                double z0 = 0.0, z1 = 0.0;
-                for (int j = 0; j < nbk; j++) {
-                    const double rj = rb[wave][j];
-                    const double *Mj = M + (size_t)j * nbk;
-                    if (h0) z0 = fma(Mj[lane], rj, z0);
-                    if (h1) z1 = fma(Mj[lane + kWave], rj, z1);
-                }
-                if (h0) {
-                    const double ri = rb[wave][lane];
-                    st_coh(&a.u[q0 + lane], z0);
-                    pg += ri * z0;
-                    pr += ri * ri;
-                }
-                if (h1) {
-                    const double ri = rb[wave][lane + kWave];
-                    st_coh(&a.u[q0 + lane + kWave], z1);
-                    pg += ri * z1;
-                    pr += ri * ri;
-                }
-            }
-            __syncthreads();
         }
+        __syncthreads();
+        blocks_apply(a.r, a.u, pg, pr);
     };
     // w = A u for the rows of this work-group, partial w.u
     auto phase_w = [&]() {
         double pd = 0.0;
-        for (int i = r0 + tid; i < r1; i += kCoreThreads) {
-            const double ui = ld_coh(&a.u[i]);
-            double acc = a.diag[i] * ui;
-            const int e1 = a.rp[i + 1];
-            for (int e = a.rp[i]; e < e1; e++) acc = fma(a.val[e], ld_coh(&a.u[a.col[e]]), acc);
-            a.w[i] = acc;
-            pd += acc * ui;
+        for (int sl = s0 + wave; sl < s1; sl += kCoreWaves) {
+            const int row = r0 + (sl - s0) * kWave + lane;
+            const bool live = row < r1;
+            double uo;
+            const double wv = row_product(sl, row, live, a.u, uo);
+            if (live) {
+                a.w[row] = wv;
+                pd += wv * uo;
+            }
         }
         return pd;
     };
@@ -471,7 +588,7 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
     if (!done && !started) {
         double v[2];
         phase_u(false, v[0], v[1]);
-        if (!grid_sum<2, kCoreWaves>(v, a.G, a.part, a.ctl, bar++, lds, &flag)) {
+        if (!grid_sum<2, kCoreWaves, kCoreSums>(v, a.G, a.part, a.ctl, bar++, lds, &flag)) {
             done = 4;
         } else {
             gamma = v[0];
@@ -480,12 +597,12 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
                 done = 1;
             } else {
                 double d[1] = {phase_w()};
-                if (!grid_sum<1, kCoreWaves>(d, a.G, a.part, a.ctl, bar++, lds, &flag)) {
+                if (!grid_sum<1, kCoreWaves, kCoreSums>(d, a.G, a.part, a.ctl, bar++, lds, &flag)) {
                     done = 4;
                 } else {
                     alpha = gamma / d[0];
                     beta = 0.0;
-                    if (!(d[0] > 0.0) || !isfinite(alpha)) done = 3;
+                    if (!(alpha > 0.0) || !isfinite(alpha)) done = 5;
                 }
             }
         }
@@ -493,7 +610,7 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
     for (int it = 0; it < chunk && !done; it++) {
         double v[2];
         phase_u(true, v[0], v[1]);
-        if (!grid_sum<2, kCoreWaves>(v, a.G, a.part, a.ctl, bar++, lds, &flag)) {
+        if (!grid_sum<2, kCoreWaves, kCoreSums>(v, a.G, a.part, a.ctl, bar++, lds, &flag)) {
             done = 4;
             break;
         }
@@ -504,8 +621,12 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
             done = 1;
             break;
         }
-        if (!isfinite(rr) || !(g2 > 0.0)) {
+        if (!isfinite(rr)) {
             done = 3;
+            break;
+        }
+        if (!(g2 > 0.0)) {
+            done = 5;
             break;
         }
         if (k >= max_iter) {
@@ -513,14 +634,14 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
             break;
         }
         double d[1] = {phase_w()};
-        if (!grid_sum<1, kCoreWaves>(d, a.G, a.part, a.ctl, bar++, lds, &flag)) {
+        if (!grid_sum<1, kCoreWaves, kCoreSums>(d, a.G, a.part, a.ctl, bar++, lds, &flag)) {
             done = 4;
             break;
         }
         beta = g2 / gamma;
         const double den = d[0] - beta * g2 / alpha;
         if (!(den > 0.0) || !isfinite(den)) {
-            done = 3;
+            done = 5;
             break;
         }
         alpha = g2 / den;
@@ -559,13 +680,17 @@ bool zeros(cicy_ctx *ctx, cicy_core *z, double **dst, size_t count) {
 
 void core_free(cicy_core *z) {
     if (!z) return;
-    int *ip[] = {z->seg, z->rp, z->col, z->blk_of, z->wblk, z->blk_row0, z->blk_n};
+    int *ip[] = {z->seg, z->blk_of, z->wblk, z->blk_row0, z->blk_n, z->wsl, z->sl_len,
+                 z->ell_col};
     for (int *q : ip)
         if (q) (void)hipFree(q);
-    double *dp[] = {z->val, z->diag, z->binv, z->x, z->r, z->u, z->w, z->p, z->s, z->part};
+    double *dp[] = {z->ell_val, z->diag, z->blk_c, z->x, z->r, z->u, z->w, z->p, z->s,
+                    z->part};
     for (double *q : dp)
         if (q) (void)hipFree(q);
+    if (z->binv) (void)hipFree(z->binv);
     if (z->blk_off) (void)hipFree(z->blk_off);
+    if (z->sl_off) (void)hipFree(z->sl_off);
     if (z->ctl) (void)hipFree(z->ctl);
     if (z->st) (void)hipFree(z->st);
     delete z;
@@ -611,15 +736,17 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
     z->nnz2 = (long long)H.col.size();
     z->inv_entries = (long long)H.binv.size();
     for (int v : H.blk_n) z->blocked_rows += v;
-    bool ok = up_vec(ctx, z, &z->seg, H.seg) && up_vec(ctx, z, &z->rp, H.rp) &&
-              up_vec(ctx, z, &z->col, H.col) && up_vec(ctx, z, &z->val, H.val) &&
+    bool ok = up_vec(ctx, z, &z->seg, H.seg) && up_vec(ctx, z, &z->wsl, H.wsl) &&
+              up_vec(ctx, z, &z->sl_off, H.sl_off) && up_vec(ctx, z, &z->sl_len, H.sl_len) &&
+              up_vec(ctx, z, &z->ell_col, H.ell_col) && up_vec(ctx, z, &z->ell_val, H.ell_val) &&
               up_vec(ctx, z, &z->diag, H.diag) && up_vec(ctx, z, &z->blk_of, H.blk_of) &&
               up_vec(ctx, z, &z->wblk, H.wblk) && up_vec(ctx, z, &z->blk_row0, H.blk_row0) &&
               up_vec(ctx, z, &z->blk_n, H.blk_n) && up_vec(ctx, z, &z->blk_off, H.blk_off) &&
-              up_vec(ctx, z, &z->binv, H.binv) && zeros(ctx, z, &z->x, m) &&
+              up_vec(ctx, z, &z->blk_c, H.blk_c) && up_vec(ctx, z, &z->binv, H.binv) &&
+              zeros(ctx, z, &z->x, m) &&
               zeros(ctx, z, &z->r, m) && zeros(ctx, z, &z->u, m) && zeros(ctx, z, &z->w, m) &&
               zeros(ctx, z, &z->p, m) && zeros(ctx, z, &z->s, m) &&
-              zeros(ctx, z, &z->part, 4 * (size_t)H.G) &&
+              zeros(ctx, z, &z->part, 2 * kCoreSums * (size_t)H.G) &&
               hip_ok(ctx, hipMalloc((void **)&z->ctl, sizeof(GridCtl)), "hipMalloc (core)") &&
               hip_ok(ctx, hipMalloc((void **)&z->st, sizeof(CoreState)), "hipMalloc (core)") &&
               hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
@@ -647,13 +774,13 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
     h.stop2 = tol * tol * ref2;
     bool ok = hip_ok(ctx, hipMemcpyAsync(z->r, rhs, vb, hipMemcpyDeviceToDevice, stream), "D2D") &&
               hip_ok(ctx, hipMemsetAsync(z->x, 0, vb, stream), "memset") &&
-              hip_ok(ctx, hipMemsetAsync(z->p, 0, vb, stream), "memset") &&
               hip_ok(ctx, hipMemsetAsync(z->s, 0, vb, stream), "memset") &&
+              hip_ok(ctx, hipMemsetAsync(z->p, 0, vb, stream), "memset") &&
               hip_ok(ctx, hipMemcpyAsync(z->st, &h, sizeof(h), hipMemcpyHostToDevice, stream),
                      "H2D");
-    CoreDev a{z->m, z->G, z->seg, z->rp, z->col, z->val, z->diag, z->blk_of, z->wblk,
-              z->blk_row0, z->blk_n, z->blk_off, z->binv, z->x, z->r, z->u, z->w, z->p, z->s,
-              z->ctl, z->part};
+    CoreDev a{z->m, z->G, z->seg, z->wsl, z->sl_off, z->sl_len, z->ell_col, z->ell_val,
+              z->diag, z->blk_of, z->wblk, z->blk_row0, z->blk_n, z->blk_off, z->blk_c,
+              z->binv, z->x, z->r, z->u, z->w, z->p, z->s, z->ctl, z->part};
     while (ok && !h.done) {
         int chunk = kCoreChunk;
         void *args[] = {(void *)&a, (void *)&z->st, (void *)&chunk};
@@ -671,11 +798,13 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
     if (ok) ok = hip_ok(ctx, hipMemcpyAsync(x_out, z->x, vb, hipMemcpyDeviceToDevice, stream), "D2D");
     memset(stats, 0, sizeof(*stats));
     stats->iters = h.k;
+    // a stalled recursion is a soft stop (the caller corrects from the true residual)
     stats->status = !ok            ? CICY_DEVICE_ERROR
                     : h.done == 1 ? CICY_OK
                     : h.done == 3 ? CICY_BREAKDOWN
                     : h.done == 4 ? CICY_DEVICE_ERROR
                                   : CICY_MAXITER;
+    stats->refinements = h.done == 5 ? 1 : 0;
     if (ok && h.done == 4) {
         set_error(ctx, "core CG: grid barrier timed out (work-groups not co-resident)");
         return false;
@@ -705,9 +834,32 @@ extern "C" int cicy_host_core_block_cg(int64_t m, const int64_t *row_ptr, const 
     const char *why = nullptr;
     if (!build_core_host((int)m, crp, ccol, cval, cdiag, orig, nblocks, ranges, 256, H, &why))
         return 0;
-    int status = CICY_OK;
-    const long long k = host_core_cg(H, rhs, tol, ref2, max_iter > 0 ? max_iter : 10 * m + 100,
-                                     x, &status);
+    // passes from the true residual, as the condensed solve corrects after a stalled one
+    const long long cap = max_iter > 0 ? max_iter : 10 * m + 100;
+    const double stop2 = tol * tol * ref2;
+    std::vector<double> res(rhs, rhs + m), dx((size_t)m), ax((size_t)m);
+    std::fill(x, x + m, 0.0);
+    long long k = 0;
+    int status = CICY_MAXITER;
+    for (int pass = 0; pass < kCorePasses && k < cap; pass++) {
+        int st = CICY_OK;
+        k += host_core_cg(H, res.data(), tol, ref2, cap - k, dx.data(), &st);
+        for (int64_t i = 0; i < m; i++) x[i] += dx[i];
+        host_core_apply(H, x, ax.data());
+        double rr = 0.0;
+        for (int64_t i = 0; i < m; i++) {
+            res[i] = rhs[i] - ax[i];
+            rr += res[i] * res[i];
+        }
+        if (rr <= stop2) {
+            status = CICY_OK;
+            break;
+        }
+        if (st == CICY_BREAKDOWN) {
+            status = CICY_BREAKDOWN;
+            break;
+        }
+    }
     if (iters) *iters = k;
     if (blocked_rows) {
         long long br = 0;

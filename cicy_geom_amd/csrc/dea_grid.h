@@ -59,27 +59,28 @@ __device__ __forceinline__ void wg_sum(double (&v)[NV], double *lds) {
     }
 }
 
-// Grid-wide sum of NV (<= 2) values and barrier number m of this launch, over G
+// Grid-wide sum of NV (<= STRIDE) values and barrier number m of this launch, over G
 // work-groups of WAVES waves.  Every thread first waits for its own stores (values other
 // work-groups read after the barrier must be at the point of coherence before its
 // work-group arrives).  Thread 0 stores the work-group's sums to its own slot of the
-// barrier's parity (part: [2][G][2]) with agent-scope stores, waits for them, and arrives
+// barrier's parity (part: [2][G][STRIDE]) with agent-scope stores, waits for them, and arrives
 // on its counter; lanes 0..7 of wave 0 poll the eight counters (bounded spin); then wave 0
 // reads all G slots and sums them in a fixed order, so every work-group obtains
 // bit-identical totals (the run-to-run result is deterministic too).  A slot of parity
 // m & 1 is rewritten only at barrier m + 2, after every work-group has passed barrier
 // m + 1 and hence finished reading it.  Returns false in every thread if the wait timed
 // out (or another work-group's did); v[] holds the totals.
-template <int NV, int WAVES>
+template <int NV, int WAVES, int STRIDE = 2>
 __device__ __forceinline__ bool grid_sum(double (&v)[NV], int G, double *part, GridCtl *ctl,
                                          unsigned m, double *lds, int *flag) {
+    static_assert(NV <= STRIDE, "grid_sum: more values than slot entries");
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     wg_sum<NV, WAVES>(v, lds);
-    double *slot = part + (size_t)(m & 1u) * (size_t)G * 2;
+    double *slot = part + (size_t)(m & 1u) * (size_t)G * STRIDE;
     if (threadIdx.x < kWave) {
         if (threadIdx.x == 0) {
 #pragma unroll
-            for (int i = 0; i < NV; i++) st_coh(&slot[(size_t)blockIdx.x * 2 + i], v[i]);
+            for (int i = 0; i < NV; i++) st_coh(&slot[(size_t)blockIdx.x * STRIDE + i], v[i]);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             (void)atomicAdd(&ctl->cnt[blockIdx.x & 7][0], 1u);
         }
@@ -102,7 +103,7 @@ __device__ __forceinline__ bool grid_sum(double (&v)[NV], int G, double *part, G
         for (int i = 0; i < NV; i++) sum[i] = 0.0;
         for (int k = threadIdx.x; k < G; k += kWave) {
 #pragma unroll
-            for (int i = 0; i < NV; i++) sum[i] += ld_coh(&slot[(size_t)k * 2 + i]);
+            for (int i = 0; i < NV; i++) sum[i] += ld_coh(&slot[(size_t)k * STRIDE + i]);
         }
 #pragma unroll
         for (int i = 0; i < NV; i++) sum[i] = wave_sum(sum[i]);

@@ -76,7 +76,8 @@ def test_plate_blocks_cut_iterations_and_keep_the_solution(plate_system):
 
 def test_block_inverse_is_exact_on_a_single_block():
     """One dense block covering a whole SPD system: the preconditioned operator is the
-    identity, so CG converges in one iteration."""
+    identity up to the FP32 rounding of the inverse's remainder, so CG converges in one
+    iteration to that rounding and in two to 1e-12."""
     rng = np.random.default_rng(3)
     n = 40
     G = np.abs(rng.normal(size=(n, n)))
@@ -93,7 +94,10 @@ def test_block_inverse_is_exact_on_a_single_block():
     x, it, br = dea.host_core_block_cg(np.array(row_ptr), np.array(col_ind), np.array(values),
                                        diag, b, [(0, n)], 1e-12)
     A = np.diag(diag) - G
-    assert br == n and it == 1
+    assert br == n and it <= 2
+    _, it1, _ = dea.host_core_block_cg(np.array(row_ptr), np.array(col_ind), np.array(values),
+                                       diag, b, [(0, n)], 1e-6)
+    assert it1 == 1
     assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
 
 
