@@ -1,21 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark: DEA conductance solves/s to ||b - A x|| / ||b|| <= 1e-10 on one MI355X.
 
-Workload: the 8 um RVE with 160k straight CNTs (seed 3_000_000), largest cluster
-percolating along x, R_flag=1 resistance solve (family := x), zero initial guess.  The
-system is assembled on the host and uploaded once; the timed region covers only solves
-on HBM-resident data.  A "step" is one solve.
+Workload: the 8 um RVE with 160k straight CNTs and 2k square nanoplatelets (seed
+3_000_000), largest cluster percolating along x, R_flag=1 resistance solve (family := x),
+zero initial guess.  The system is assembled on the host and uploaded once; each method's
+one-off setup of the system (layouts, eliminations, block inverses) is timed separately
+(`setup_s`), the timed region covers solves on HBM-resident data.  A "step" is one solve.
 
 Methods (--methods): plain FP64 Jacobi-PCG on the GPU (the reference's algorithm, always
-timed as the reference line), the condensed solve (exact series elimination down to a core
-system, the same FP64 Jacobi-PCG on the core, back-substitution; timed end to end), the
-condensed solve with the register-resident CG on the core, the register-resident CG on
-the full system, the mixed-precision FP32/FP64 solver, and the learned warm start followed
-by a solver (network inference inside the timed solve; needs trained weights).  Each
-method runs on the device layout and row-block size the autotuner picks for it on this
-system.  `value` is the fastest converged method's solves/s; every method's true residual
-||b - A x|| / ||b|| is checked with the full system's FP64 SpMV and reported.  Prints one
-JSON line on stdout; progress goes to stderr.
+timed as the reference line), the condensed solves (exact series elimination down to a
+core system, back-substitution; timed end to end) with the core solved by the two-kernel
+FP64 Jacobi-PCG (condensed), the register-resident CG (condensed_resident) or the
+persistent block-preconditioned CG with one dense block per nanoplatelet
+(condensed_block), the register-resident CG on the full system, the mixed-precision
+FP32/FP64 solver, and the learned warm start followed by a solver (network inference
+inside the timed solve; needs trained weights).  Each method runs on the device layout and
+row-block size the autotuner picks for it on this system.  Methods that iterate on the
+full system (pcg_f64, learned_f64, mixed, resident) time --plain-steps solves (about 13 s
+each here), the others --steps.  `value` is the fastest converged method's solves/s; every
+method's true residual ||b - A x|| / ||b|| is checked with the full system's FP64 SpMV and
+reported.  Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -114,10 +118,13 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 reference line (profiling runs of one method)")
-    ap.add_argument("--methods", default="pcg_f64,condensed,condensed_block,learned_f64",
+    ap.add_argument("--methods", default="pcg_f64,condensed_block,learned_block,learned_f64",
                     help="comma list from pcg_f64, mixed, resident, condensed, "
-                         "condensed_resident, condensed_block, learned_f64, learned_mixed "
-                         "(the learned ones need trained weights)")
+                         "condensed_resident, condensed_block, learned_f64, learned_mixed, "
+                         "learned_block (the learned ones need trained weights)")
+    ap.add_argument("--copy-ref", action="store_true",
+                    help="first time a 1 GiB device copy (the yardstick of hardware-counter "
+                         "traffic runs)")
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only run the SpMV kernels (both precisions) and a 1 GiB reference "
                          "copy, for hardware-counter runs")
@@ -136,12 +143,16 @@ def main():
     s = net.assemble(k_cl, 1, 0, direction=0, with_xyz=True)
     plates = network.plate_row_ranges(net, k_cl, s)
     n_gnp = net.count("gnps")
+    n_cnt = net.count("tubes")
     del net
     log(f"[bench] config {args.config}: N={s.n} L={s.nnz} ({s.gnp_nodes} plate nodes in "
         f"{len(plates)} plates) built in {time.perf_counter() - t0:.1f}s")
 
     from cicy_geom_amd import autotune
     ctx = dea.Context(0)
+    if args.copy_ref:
+        cms = ctx.copy_bench(1 << 30, 3)
+        log(f"[bench] copy 1 GiB: {cms:.3f} ms -> {2 * (1 << 30) / cms / 1e6:.0f} GB/s")
     if args.profile_spmv:
         reps = 20
         copy_ms = ctx.copy_bench(1 << 30, reps)
@@ -172,7 +183,9 @@ def main():
                "condensed_resident": (dea.METHOD_CONDENSED_RESIDENT, "condensed", False),
                "condensed_block": (dea.METHOD_CONDENSED_BLOCK, "condensed", False),
                "learned_f64": (dea.METHOD_PCG_F64, "pcg_f64", True),
-               "learned_mixed": (dea.METHOD_MIXED, "mixed", True)}
+               "learned_mixed": (dea.METHOD_MIXED, "mixed", True),
+               "learned_block": (dea.METHOD_CONDENSED_BLOCK, "condensed", True)}
+    full_system = {"pcg_f64", "learned_f64", "mixed", "learned_mixed", "resident"}
     wanted = [m for m in args.methods.split(",") if m]
     bad = [m for m in wanted if m not in catalog]
     if bad:
@@ -215,7 +228,8 @@ def main():
         upload_s = time.perf_counter() - t0
         if learned:
             graph = warmstart.Graph(ctx, s, box, 0, 1.0)
-        steps, warmup = (args.plain_steps, 0) if name == "pcg_f64" else (args.steps, args.warmup)
+        steps, warmup = ((args.plain_steps, 0) if name in full_system
+                         else (args.steps, args.warmup))
         wall, stats = time_solves(sysd, method, steps, warmup, args.tol, s,
                                   model if learned else None, graph if learned else None)
         st = stats[-1]
@@ -233,6 +247,7 @@ def main():
                              core_block_rows=sysd.size("core_block_rows"),
                              core_groups=sysd.size("core_groups"),
                              setup_s=upload_s,
+                             kernel_ms=sum(x.kernel_ms for x in stats),
                              resident_rows_per_thread=sysd.size("resident_rows"),
                              rows_per_block=tuned[tkey]["rows_per_block"])
         log(f"[bench] {name}: {results[name]}")
@@ -259,7 +274,7 @@ def main():
     plain = results.get("pcg_f64")
     head = "pcg_f64" if plain else next(iter(results))
     for cand in ("learned_f64", "mixed", "learned_mixed", "resident", "condensed",
-                 "condensed_resident", "condensed_block"):
+                 "condensed_resident", "condensed_block", "learned_block"):
         if cand in results and cand != head and results[cand]["status"] == "ok" and \
                 results[cand]["solves_per_s"] > results[head]["solves_per_s"]:
             head = cand
@@ -275,33 +290,48 @@ def main():
             "kernel": "SSS SpMV (k_apply/k_apply_band) " +
                       ("fp64" if hp == 64 else "fp32 Jacobi-scaled") + " of the full system",
             "algorithmic_bytes": spmv[hp]["bytes"], "us_per_launch": spmv[hp]["ms"] * 1e3}
-    core = systems[head].core() if head.startswith("condensed") else None
-    if core is not None:
-        # dominant kernels: the core system's PCG iteration (k_apply<PCG> + k_update_f64,
-        # ~95 % of the device time); its rate is the HIP-event device time of the timed
-        # solves over their iterations (the few sweeps around the core solve included)
+    kind = catalog[head][0]
+    if kind == dea.METHOD_CONDENSED_BLOCK:
+        # dominant kernel: the persistent block-preconditioned core CG (k_core_cg), one
+        # cooperative launch per solve; its duration is the HIP-event time of its launches
+        # inside the timed solves.  Algorithmic bytes per core iteration: the off-diagonal
+        # entries of both triangles (column + value, 12 B), the diagonal and row kind (12 B
+        # per row), the FP32 dense block remainders (4 B per entry) and the vectors (x, r, u,
+        # w, p, s: 13 reads/writes of 8 B per row).
+        r_ = results[head]
+        cn, cl = r_["core_rows"], r_["core_nnz"]
+        nb2 = systems[head].size("core_block_entries")
+        cib = 12 * 2 * cl + 12 * cn + 4 * nb2 + 8 * 13 * cn
+        ksec = r_["kernel_ms"] * 1e-3 / max(1, r_["steps"])
+        ach = cib * r_["iters"] / ksec / 1e9 if ksec > 0 else 0.0
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                "kernel": f"k_core_cg: block-preconditioned core CG, {cn} rows, "
+                          f"{r_['core_blocks']} plate blocks",
+                "algorithmic_bytes": cib * r_["iters"], "algorithmic_bytes_per_iter": cib,
+                "us_per_launch": ksec * 1e6, "iters_per_launch": r_["iters"],
+                "us_per_iter": ksec * 1e6 / max(1, r_["iters"])}
+        tpath = os.path.join(ROOT, "profiles", "core_block_6p0M_traffic.json")
+        if args.config == 3 and os.path.exists(tpath):
+            with open(tpath) as f:
+                t = json.load(f)
+            roof["traffic"] = t["bytes_per_launch"]
+            roof["traffic_per_iter"] = t["bytes_per_iter"]
+            roof["traffic_source"] = t["source"]
+        log(f"[bench] k_core_cg: {roof['us_per_iter']:.1f} us per core iteration, "
+            f"{cib / 1e6:.1f} MB algorithmic -> {ach:.0f} GB/s ({100 * roof['frac']:.1f} % of peak)")
+    elif head.startswith("condensed") or kind == dea.METHOD_CONDENSED:
+        core = systems[head].core()
+        # dominant kernels: the core system's PCG iteration (k_apply<PCG> + k_update_f64);
+        # its rate is the HIP-event device time of the timed solves over their iterations
         cn, cl = core.size("n"), core.size("nnz")
-        ms = core.spmv_bench(64, 200)
-        cb = network.spmv_bytes(cn, cl, sv=8, si=4)
         cit = results[head]["device_ms"] * 1e3 / max(1, results[head]["iters"])
         cib = network.pcg_iter_bytes(cn, cl)
         roof = {"bound": "hbm", "achieved": cib / (cit * 1e-6) / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": cib / (cit * 1e-6) / 1e9 / HBM_PEAK_GBS,
                 "traffic": None,
                 "kernel": f"PCG iteration (k_apply<PCG> + k_update_f64) of the {cn}-row core",
-                "algorithmic_bytes": cib, "us_per_launch": cit,
-                "core_spmv": {"us": ms * 1e3, "algorithmic_bytes": cb,
-                              "gbs": cb / (ms * 1e-3) / 1e9}}
-        # memory traffic of the same kernels from the committed counter runs (rocprofv3 PMC
-        # FETCH_SIZE + WRITE_SIZE, copy-kernel scaled) when they were taken on this workload
-        tpath = os.path.join(ROOT, "profiles", "traffic_5p6M.json")
-        if args.config == 3 and os.path.exists(tpath):
-            with open(tpath) as f:
-                roof["traffic"] = json.load(f)["core_iteration_traffic_bytes"]
-            roof["traffic_source"] = "profiles/condensed_5p6M_hbm_traffic.md"
-        log(f"[bench] core ({cn} rows, {cl} entries): iteration {cit:.1f} us for "
-            f"{cib / 1e6:.1f} MB algorithmic -> {roof['achieved']:.0f} GB/s; SpMV alone "
-            f"{ms * 1e3:.1f} us")
+                "algorithmic_bytes": cib, "us_per_launch": cit}
     out = {
         "metric": "CG solves/s to 1e-10 residual at 4M nodes; learned+mixed vs plain FP64 GPU CG",
         "value": value,
@@ -315,12 +345,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if hp == 64 else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"config {args.config}: {s.n}-node DEA system (8 um RVE, 160k "
-                               "straight CNTs, largest x-percolating cluster, R_flag=1)",
+        "config": {"workload": f"config {args.config}: {s.n}-node DEA system ({box:g} um RVE, "
+                               f"{n_cnt} straight CNTs + {n_gnp} square plates, largest "
+                               "x-percolating cluster, R_flag=1)",
                    "N": s.n, "L": s.nnz, "tol": args.tol, "residual": "||b-Ax||/||b||",
                    "method": head},
         "methods": results,
         "speedup_vs_plain_f64": value / plain["solves_per_s"] if plain else None,
+        # one-off per-system setup of the headline method (host layouts, eliminations,
+        # block inverses, upload), outside the timed solves; a fresh system per solve runs at
+        "setup_s": results[head]["setup_s"],
+        "end_to_end_solves_per_s": 1.0 / (results[head]["setup_s"] +
+                                          results[head]["ms_per_solve"] * 1e-3),
         "roofline": roof,
         "roofline_spmv_full_f64": {"achieved": spmv[64]["gbs"],
                                    "frac": spmv[64]["gbs"] / HBM_PEAK_GBS,
@@ -337,7 +373,8 @@ def main():
         sysd.close()
     if model is not None:
         learned = {"weights": os.path.relpath(WEIGHTS, ROOT)}
-        for lk, zk in (("learned_f64", "pcg_f64"), ("learned_mixed", "mixed")):
+        for lk, zk in (("learned_f64", "pcg_f64"), ("learned_mixed", "mixed"),
+                       ("learned_block", "condensed_block")):
             if lk in results and zk in results:
                 learned[f"iters_saved_{lk}_vs_{zk}"] = 1.0 - results[lk]["iters"] / results[zk]["iters"]
         out["learned"] = learned

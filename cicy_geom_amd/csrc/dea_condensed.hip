@@ -816,12 +816,13 @@ bool solve_condensed(cicy_sys *s, int inner, double tol, long long max_iter, int
               hip_ok(ctx, hipMemcpyAsync(c->rhs, s->r0, vb, hipMemcpyDeviceToDevice, st), "copy r0");
     long long total = 0;
     int passes = 0, status = CICY_MAXITER;
-    double rel = -1.0;
+    double rel = -1.0, kernel_ms = 0.0;
     while (ok) {
         cicy_solve_stats ist;
         ok = condensed_correction(s, inner, c->rhs, tol, max_iter - total, ref2, &ist);
         if (!ok) break;
         total += ist.iters;
+        kernel_ms += ist.kernel_ms;
         // x += d; r = rhs - A d (the residual of the whole system)
         sys_spmv64(s, c->delta, s->q);
         hipLaunchKernelGGL(k_correct, dim3(c->nb), dim3(kThreads), 0, st, s->n, s->x, c->delta,
@@ -860,6 +861,7 @@ bool solve_condensed(cicy_sys *s, int inner, double tol, long long max_iter, int
         stats->true_rel_res = tr;
         stats->solve_ms = ms;
         stats->fp64_iters = total;
+        stats->kernel_ms = kernel_ms;
     }
     return ok;
 }

@@ -781,6 +781,10 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
     CoreDev a{z->m, z->G, z->seg, z->wsl, z->sl_off, z->sl_len, z->ell_col, z->ell_val,
               z->diag, z->blk_of, z->wblk, z->blk_row0, z->blk_n, z->blk_off, z->blk_c,
               z->binv, z->x, z->r, z->u, z->w, z->p, z->s, z->ctl, z->part};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ok = ok && hip_ok(ctx, hipEventCreate(&e0), "hipEventCreate") &&
+         hip_ok(ctx, hipEventCreate(&e1), "hipEventCreate") &&
+         hip_ok(ctx, hipEventRecord(e0, stream), "event");
     while (ok && !h.done) {
         int chunk = kCoreChunk;
         void *args[] = {(void *)&a, (void *)&z->st, (void *)&chunk};
@@ -795,8 +799,15 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
                 max_seconds)
             break;
     }
+    float kms = 0.f;
+    if (ok && hip_ok(ctx, hipEventRecord(e1, stream), "event") &&
+        hip_ok(ctx, hipEventSynchronize(e1), "event"))
+        (void)hipEventElapsedTime(&kms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     if (ok) ok = hip_ok(ctx, hipMemcpyAsync(x_out, z->x, vb, hipMemcpyDeviceToDevice, stream), "D2D");
     memset(stats, 0, sizeof(*stats));
+    stats->kernel_ms = kms;
     stats->iters = h.k;
     // a stalled recursion is a soft stop (the caller corrects from the true residual)
     stats->status = !ok            ? CICY_DEVICE_ERROR

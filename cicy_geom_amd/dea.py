@@ -48,7 +48,8 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int64), ("status", ctypes.c_int),
                 ("refinements", ctypes.c_int), ("rel_res", ctypes.c_double),
                 ("true_rel_res", ctypes.c_double), ("solve_ms", ctypes.c_double),
-                ("total_ms", ctypes.c_double), ("fp64_iters", ctypes.c_int64)]
+                ("total_ms", ctypes.c_double), ("fp64_iters", ctypes.c_int64),
+                ("kernel_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_} | {

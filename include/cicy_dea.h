@@ -77,6 +77,9 @@ typedef struct cicy_solve_stats {
     double total_ms;        /* wall time of the call including host<->device copies */
     int64_t fp64_iters;     /* mixed method: iterations finished in FP64 after the FP32
                                recursion stopped making progress (0: none) */
+    double kernel_ms;       /* device time (HIP events) of the launches of the solve's
+                               dominant kernel, the persistent core CG of
+                               CICY_METHOD_CONDENSED_BLOCK; 0 when not measured */
 } cicy_solve_stats;
 
 /* ---- context ---------------------------------------------------------------------- */
