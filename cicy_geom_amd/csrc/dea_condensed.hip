@@ -89,7 +89,22 @@ __global__ __launch_bounds__(kCondThreads) void k_path_solve(
     if (k >= npath) return;
     const int s = ps[k], e = pe[k];
     if (pl[k] == kFloating) {
-        for (int i = s; i <= e; i++) z[i] = 0.0;
+        // singular chain: grounded at its last row, then the zero-mean solution
+        double g = rhs[s];
+        z[s] = g;
+        for (int i = s + 1; i < e; i++) {
+            g = rhs[i] - lfac[i] * g;
+            z[i] = g;
+        }
+        double zn = 0.0, mean = 0.0;
+        z[e] = 0.0;
+        for (int i = e - 1; i >= s; i--) {
+            zn = (z[i] - band[i + 1] * zn) / mfac[i];
+            z[i] = zn;
+            mean += zn;
+        }
+        mean /= (double)(e - s + 1);
+        for (int i = s; i <= e; i++) z[i] -= mean;
         return;
     }
     double g = rhs[s];
@@ -338,9 +353,13 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
             mfac[k] = h.diag[k] - lfac[k] * band[k];
         }
         // A pivot can only vanish on a path that touches no anchor and no electrode: a
-        // floating tube, singular on its own (its rows of A sum to zero).  Such a system is
-        // singular but consistent where b is zero there; CG from x0 leaves the component
-        // at x0, and so does the condensed solve (the path is marked and gets d = 0).
+        // floating tube (for instance one whose only junction reaches an electrode and was
+        // dropped, Direct_Electrifying.cpp:817-837), singular on its own (its rows of A sum
+        // to zero).  Its block is singular but consistent: r0 = b - A x0 = -A x0 there.  CG's
+        // correction stays orthogonal to the constant null vector, so the tube ends at the
+        // mean of its initial guess; the condensed solve gives the same: the path is solved
+        // grounded at its last row (the leading block keeps its pivots) and the correction's
+        // mean is removed.
         const double pivot_floor = 1e-12 * h.diag[e];
         bool singular = false;
         for (int k = s; k <= e; k++)
@@ -350,11 +369,14 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
                 *why = "non-positive pivot on a path coupled to the rest (not an M-matrix)";
                 return false;
             }
+            for (int k = s; k < e; k++)
+                if (!(mfac[k] > pivot_floor) || !std::isfinite(mfac[k])) {
+                    *why = "non-positive pivot inside a floating path";
+                    return false;
+                }
             pl.back() = kFloatingPath;
-            for (int k = s; k <= e; k++) {
-                lfac[k] = 0.0;
-                mfac[k] = 1.0;
-            }
+            lfac[e] = 0.0;
+            mfac[e] = 1.0;
             i = e + 1;
             continue;
         }
@@ -559,7 +581,21 @@ void host_path_solve(const CondHost &C, const double *rhs, const double *xanchor
     for (size_t k = 0; k < C.ps.size(); k++) {
         const int s = C.ps[k], e = C.pe[k];
         if (C.pl[k] == kFloatingPath) {
-            for (int i = s; i <= e; i++) z[i] = 0.0;
+            double g = rhs[s];
+            z[s] = g;
+            for (int i = s + 1; i < e; i++) {
+                g = rhs[i] - C.lfac[i] * g;
+                z[i] = g;
+            }
+            double zn = 0.0, mean = 0.0;
+            z[e] = 0.0;
+            for (int i = e - 1; i >= s; i--) {
+                zn = (z[i] - C.band[i + 1] * zn) / C.mfac[i];
+                z[i] = zn;
+                mean += zn;
+            }
+            mean /= (double)(e - s + 1);
+            for (int i = s; i <= e; i++) z[i] -= mean;
             continue;
         }
         double g = rhs[s];

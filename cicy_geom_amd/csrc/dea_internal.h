@@ -77,6 +77,7 @@ struct cicy_ctx {
     int resident_rows = 0;  // rows per thread of the resident CG layout (0: smallest that fits)
     bool no_graphs = false;  // CICY_NO_GRAPHS set at creation: launch chunks directly
     int dropin_method = CICY_METHOD_PCG_F64;  // method of cicy_solve_dea_cg_sss
+    std::vector<int64_t> dropin_blocks;       // row groups for the next block-method call
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     hipEvent_t ev_poll[2] = {nullptr, nullptr};

@@ -576,6 +576,16 @@ int cicy_set_dropin_method(cicy_ctx *ctx, int method) {
     return 1;
 }
 
+int cicy_set_dropin_blocks(cicy_ctx *ctx, int64_t nblocks, const int64_t *ranges) {
+    if (!ctx) return 0;
+    if (nblocks < 0 || (nblocks > 0 && !ranges)) {
+        cicy::set_error(ctx, "cicy_set_dropin_blocks: invalid block list");
+        return 0;
+    }
+    ctx->dropin_blocks.assign(ranges, ranges + 2 * nblocks);
+    return 1;
+}
+
 int cicy_set_format(cicy_ctx *ctx, int format) {
     if (!ctx) return 0;
     if (format != 0 && format != kFormatSss && format != kFormatBand) {
@@ -948,6 +958,16 @@ int cicy_solve_dea_cg_sss(cicy_ctx *ctx, int64_t nodes, int64_t reserved_nodes, 
                                ? cicy::kExtraResident
                                : cicy::kExtraResident | cicy::kExtraCondensed;
         if (!cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, extras, &s)) {
+            if (stats) stats->status = CICY_BAD_INPUT;
+            ctx->dropin_blocks.clear();
+            return 0;
+        }
+        // row groups handed over by cicy_set_dropin_blocks apply to this call only
+        std::vector<int64_t> groups;
+        groups.swap(ctx->dropin_blocks);
+        if (method == CICY_METHOD_CONDENSED_BLOCK && !groups.empty() && s->cond &&
+            !cicy::condensed_set_blocks(s, (int64_t)groups.size() / 2, groups.data())) {
+            cicy_sys_destroy(s);
             if (stats) stats->status = CICY_BAD_INPUT;
             return 0;
         }

@@ -41,6 +41,7 @@ EXPORTED_SYMBOLS = [
     "cicy_host_blocked_spmv", "cicy_set_resident_rows", "cicy_host_resident_apply",
     "cicy_host_condense", "cicy_host_expand", "cicy_host_core", "cicy_host_core_expand",
     "cicy_sys_core", "cicy_set_dropin_method", "cicy_sys_set_blocks", "cicy_host_core_block_cg",
+    "cicy_set_dropin_blocks",
 ]
 
 
@@ -111,6 +112,7 @@ def lib():
     L.cicy_sys_get.argtypes = [_VP, _VP, _VP]
     L.cicy_sys_core.argtypes = [_VP, P(_VP)]
     L.cicy_sys_set_blocks.argtypes = [_VP, ctypes.c_int64, _VP]
+    L.cicy_set_dropin_blocks.argtypes = [_VP, ctypes.c_int64, _VP]
     L.cicy_host_core_block_cg.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64,
                                           ctypes.c_int64, ctypes.c_int64, _VP, _F64,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_int64,
@@ -200,6 +202,14 @@ class Context:
     def set_dropin_method(self, method: int):
         """Method behind solve_dea_cg_sss (default METHOD_PCG_F64, the reference's)."""
         self._check(self._L.cicy_set_dropin_method(self._h, method), "cicy_set_dropin_method")
+
+    def set_dropin_blocks(self, ranges):
+        """Row groups [(start, end), ...] (e.g. each nanoplatelet's nodes) for the next
+        solve_dea_cg_sss with METHOD_CONDENSED_BLOCK."""
+        r = np.ascontiguousarray(np.asarray(ranges, np.int64).reshape(-1, 2))
+        self._check(self._L.cicy_set_dropin_blocks(self._h, len(r),
+                                                   r.ctypes.data if len(r) else None),
+                    "cicy_set_dropin_blocks")
 
     def set_resident_rows(self, rows_per_thread: int):
         """Rows per thread of the resident layout of systems created afterwards (0: auto)."""

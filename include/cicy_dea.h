@@ -98,6 +98,13 @@ int cicy_set_format(cicy_ctx *ctx, int format);
  * far faster on CNT networks; the iteration count then counts CG iterations of the system
  * the method iterates on.  No counterpart in the reference. */
 int cicy_set_dropin_method(cicy_ctx *ctx, int method);
+/* Row groups (ranges[2 b] .. ranges[2 b + 1], free-node rows) for the next
+ * cicy_solve_dea_cg_sss call with CICY_METHOD_CONDENSED_BLOCK, as cicy_sys_set_blocks takes
+ * them: the caller that assembled the system knows the nodes of each nanoplatelet (the
+ * reference's LMM_gnps over structure_gnp[g], Direct_Electrifying.cpp:307-349), the seam
+ * itself only sees the matrix.  Consumed (cleared) by that call.  No counterpart in the
+ * reference. */
+int cicy_set_dropin_blocks(cicy_ctx *ctx, int64_t nblocks, const int64_t *ranges);
 /* rows per thread of the register-resident layout built for systems created afterwards
  * (0: the smallest of 4 8 16 24 32 40 44 48 whose grid is co-resident).  No counterpart in
  * the reference: a layout knob like cicy_set_rows_per_block. */

@@ -41,8 +41,16 @@ public:
 
     std::string last_error() const { return cicy_last_error(ctx_); }
     // solver behind Solve_DEA_equations_CG_SSS: CICY_METHOD_PCG_F64 (default, the
-    // reference's algorithm) or CICY_METHOD_CONDENSED (same stopping rule, far faster)
+    // reference's algorithm), CICY_METHOD_CONDENSED or CICY_METHOD_CONDENSED_BLOCK (same
+    // stopping rule, far faster)
     bool use_method(int method) { return cicy_set_dropin_method(ctx_, method) != 0; }
+    // node ranges [first, end) of free nodes (node - reserved_nodes) of each nanoplatelet,
+    // flattened, for the next Solve_DEA_equations_CG_SSS with CICY_METHOD_CONDENSED_BLOCK
+    bool set_plate_blocks(const std::vector<long int> &ranges) {
+        if (ranges.size() % 2) return false;
+        return cicy_set_dropin_blocks(ctx_, (int64_t)ranges.size() / 2,
+                                      reinterpret_cast<const int64_t *>(ranges.data())) != 0;
+    }
     cicy_ctx *context() { return ctx_; }
 
     int Get_voltage_vector(const long int &reserved_nodes, const long int &nodes,

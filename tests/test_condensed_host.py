@@ -151,6 +151,48 @@ def test_series_law(k):
     assert rdiag == pytest.approx([1.0 + 0.5 + geff] * 2, rel=1e-13)
 
 
+def floating_system(k=6, g=3.0, m=5):
+    """A definite chain (rows 0 .. k+1) plus a floating chain of m rows that touches no
+    junction and no electrode (singular on its own)."""
+    rp0, col0, val0, diag0 = chain_system(k, g)
+    n0 = len(diag0)
+    rp = list(rp0)
+    col, val = list(col0), list(val0)
+    for r in range(m):
+        if r > 0:
+            col.append(n0 + r - 1)
+            val.append(-g)
+        rp.append(len(col))
+    rp, col, val = np.array(rp, np.int64), np.array(col, np.int64), np.array(val)
+    diag = np.concatenate([diag0, [g] + [2 * g] * (m - 2) + [g]])
+    return (rp0, col0, val0, diag0), (rp, col, val, diag), n0
+
+
+def test_floating_tube_from_a_ramp_ends_at_its_mean():
+    """From an initial guess x0 that varies along a floating tube, r0 = -A x0 there (the
+    block is singular but consistent).  CG's correction stays orthogonal to the constant
+    null vector, so the tube ends at mean(x0); the condensed correction must be exactly
+    mean(x0) - x0 there, and it must not disturb the definite part."""
+    (rp0, col0, val0, diag0), (rp, col, val, diag), n0 = floating_system()
+    m = len(diag) - n0
+    x0 = np.zeros(n0 + m)
+    x0[n0:] = np.linspace(0.9, 0.1, m)             # a ramp over the floating tube
+    b = np.zeros(n0 + m)
+    b[0] = 1.0
+    A = dense(rp, col, val, diag)
+    r0 = b - A @ x0
+    rows, crp, ccol, cval, cdiag, crhs, _ = dea.host_core(rp, col, val, diag, r0)
+    xc = np.linalg.solve(dense(crp, ccol, cval, cdiag), crhs) if len(rows) else np.zeros(0)
+    d = dea.host_core_expand(rp, col, val, diag, r0, xc)
+    assert np.allclose(d[n0:], x0[n0:].mean() - x0[n0:], rtol=0, atol=1e-14)
+    x = x0 + d
+    assert np.linalg.norm(b - A @ x) <= 1e-13
+    want = np.linalg.solve(dense(rp0, col0, val0, diag0), b[:n0])
+    assert np.allclose(x[:n0], want, rtol=1e-12)
+    # planted mistake: leaving the tube at x0 (no correction) keeps the residual -A x0
+    assert np.linalg.norm(b - A @ (x0 + np.concatenate([d[:n0], np.zeros(m)]))) > 1e-3
+
+
 def test_floating_tube_is_left_at_zero():
     """A tube touching no junction and no electrode is singular on its own; CG from zero
     leaves it at zero and so does the condensation (the rest is solved exactly)."""

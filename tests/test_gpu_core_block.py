@@ -160,3 +160,24 @@ def test_dropin_with_condensed_block(ctx, cases):
     rr = oracle.update_residual(s.row_ptr, s.col_ind, s.values, s.diag, x, s.b)
     assert np.linalg.norm(rr) <= 1e-10 * np.linalg.norm(s.r0) * (1 + 1e-6)
     assert np.array_equal(volt[:s.reserved], s.vef)
+
+
+def test_dropin_with_plate_blocks(ctx, cases, truths):
+    """The seam as Compute_voltage_field would drive it on a plate system: plate node ranges
+    handed over before the call, the reference's stopping rule, R out = b - A x."""
+    s, ranges = cases["plates"]
+    ctx.set_dropin_method(dea.METHOD_CONDENSED_BLOCK)
+    try:
+        ctx.set_dropin_blocks(ranges)
+        ok, volt, x, r, st = ctx.solve_dea_cg_sss(s.nodes, s.reserved, 1e-10, s.col_ind,
+                                                  s.row_ptr, s.values, s.diag, s.r0, s.vef,
+                                                  s.x0)
+    finally:
+        ctx.set_dropin_method(dea.METHOD_PCG_F64)
+    print("drop-in with plate blocks:", st.as_dict())
+    assert ok and st.status == dea.CICY_OK
+    assert st.iters < 400  # the Jacobi core needs thousands on this system
+    rr = oracle.update_residual(s.row_ptr, s.col_ind, s.values, s.diag, x, s.b)
+    assert np.linalg.norm(rr) <= 1e-10 * np.linalg.norm(s.r0) * (1 + 1e-6)
+    # R out is b - A x of the returned x, up to the rounding of the two products
+    assert np.linalg.norm(r - rr) <= 1e-12 * np.linalg.norm(s.r0)
