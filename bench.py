@@ -225,6 +225,7 @@ def main():
         sysd.set_rhs(s.b, None, s.b)
         if method == dea.METHOD_CONDENSED_BLOCK:
             sysd.set_blocks(plates)
+        sysd.prepare(method)
         upload_s = time.perf_counter() - t0
         if learned:
             graph = warmstart.Graph(ctx, s, box, 0, 1.0)

@@ -61,7 +61,8 @@ struct cicy_cond {
     double *rv_a = nullptr;
     double *e_a = nullptr, *e_d = nullptr;
     double *fS = nullptr, *xS = nullptr;              // [m] right-hand side / solution on S
-    cicy_sys *red = nullptr;    // the core system on the device
+    double *core_b = nullptr, *core_x = nullptr;      // [mc] the block solver's core vectors
+    cicy_sys *red = nullptr;    // the core system on the device (built on first use)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timing (the inner solve uses the context's)
     // host copy of the core (strictly-lower SSS, diagonal, full-system row of each core
     // row) for the block-preconditioned core CG, built on demand
@@ -255,7 +256,8 @@ void condensed_free(cicy_cond *c) {
     void *ptrs[] = {c->ps, c->pe, c->pl, c->pr, c->anchor_of, c->anchors, c->lfac, c->mfac,
                     c->band, c->y, c->delta, c->rhs, c->norm, c->counter, c->part, c->gpart,
                     c->gcount, c->e_node, c->e_off, c->e_nbr, c->e_a, c->e_d,
-                    c->core_nodes, c->fS, c->xS, c->rv_node, c->rv_off, c->rv_rec, c->rv_a};
+                    c->core_nodes, c->fS, c->xS, c->rv_node, c->rv_off, c->rv_rec, c->rv_a,
+                    c->core_b, c->core_x};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete c;
@@ -273,6 +275,7 @@ int condensed_info(const cicy_cond *c, int what) {
         case 6: return core_info(c->blk, 0);                 // dense blocks
         case 7: return core_info(c->blk, 2);                 // core CG work-groups
         case 8: return core_info(c->blk, 3);                 // dense block entries
+        case 9: return c->blk ? 1 : 0;                       // block core built
         default: return 0;
     }
 }
@@ -449,50 +452,98 @@ struct SeriesHost {
     std::vector<double> cval, cdiag;
 };
 
+// Adjacency lists of S in one pool: node u's neighbours are nbr[off[u] .. off[u] + len[u])
+// with room for cap[u]; a list that outgrows its room moves to the end of the pool with
+// twice the room.  Removal swaps the last entry into the gap, insertion appends, so the
+// order of a list depends only on the sequence of operations on it.
+struct AdjPool {
+    std::vector<int> nbr, off, len, cap;
+    std::vector<double> val;
+    int find(int u, int c) const {
+        const int *p = nbr.data() + off[u];
+        for (int i = 0; i < len[u]; i++)
+            if (p[i] == c) return i;
+        return -1;
+    }
+    void drop(int u, int k) {
+        const int i = find(u, k), o = off[u], last = o + len[u] - 1;
+        nbr[o + i] = nbr[last];
+        val[o + i] = val[last];
+        len[u]--;
+    }
+    void add(int u, int v, double a) {
+        const int i = find(u, v);
+        if (i >= 0) {
+            val[off[u] + i] += a;
+            return;
+        }
+        if (len[u] == cap[u]) {
+            const int o = (int)nbr.size(), c = std::max(4, 2 * cap[u]);
+            nbr.resize((size_t)o + c);
+            val.resize((size_t)o + c);
+            std::copy(nbr.begin() + off[u], nbr.begin() + off[u] + len[u], nbr.begin() + o);
+            std::copy(val.begin() + off[u], val.begin() + off[u] + len[u], val.begin() + o);
+            off[u] = o;
+            cap[u] = c;
+        }
+        nbr[off[u] + len[u]] = v;
+        val[off[u] + len[u]] = a;
+        len[u]++;
+    }
+};
+
 void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimDegree) {
     const int m = C.m;
     Z.m = m;
-    std::vector<std::vector<std::pair<int, double>>> adj((size_t)m);
+    // S's lower triangle gives each node its lower neighbours (ascending) and, from later
+    // rows, its upper ones (ascending): every list starts sorted by index
+    AdjPool A;
+    A.len.assign((size_t)m, 0);
+    A.off.resize((size_t)m);
+    A.cap.resize((size_t)m);
+    for (int r = 0; r < m; r++)
+        for (int64_t k = C.rrp[r]; k < C.rrp[r + 1]; k++) {
+            A.len[r]++;
+            A.len[(size_t)C.rcol[k]]++;
+        }
+    {
+        size_t o = 0;
+        for (int u = 0; u < m; u++) {
+            A.off[u] = (int)o;
+            A.cap[u] = A.len[u] + 2;  // a little room for the fill of the first rounds
+            o += (size_t)A.cap[u];
+            A.len[u] = 0;
+        }
+        A.nbr.resize(o);
+        A.val.resize(o);
+    }
     for (int r = 0; r < m; r++)
         for (int64_t k = C.rrp[r]; k < C.rrp[r + 1]; k++) {
             const int c = (int)C.rcol[k];
-            adj[r].push_back({c, C.rval[k]});
-            adj[c].push_back({r, C.rval[k]});
+            A.nbr[A.off[r] + A.len[r]] = c;
+            A.val[A.off[r] + A.len[r]++] = C.rval[k];
+            A.nbr[A.off[c] + A.len[c]] = r;
+            A.val[A.off[c] + A.len[c]++] = C.rval[k];
         }
     std::vector<double> d(C.sdiag.begin(), C.sdiag.end());
     std::vector<char> alive((size_t)m, 1), taken((size_t)m, 0);
-    auto find = [](std::vector<std::pair<int, double>> &v, int c) -> int {
-        for (size_t i = 0; i < v.size(); i++)
-            if (v[i].first == c) return (int)i;
-        return -1;
-    };
-    auto drop = [&](int u, int k) {
-        const int i = find(adj[u], k);
-        adj[u][i] = adj[u].back();
-        adj[u].pop_back();
-    };
-    auto add = [&](int u, int v, double a) {
-        const int i = find(adj[u], v);
-        if (i >= 0) adj[u][i].second += a;
-        else adj[u].push_back({v, a});
-    };
     Z.round_off.assign(1, 0);
     Z.e_off.assign(1, 0);
     std::vector<int> cand, sel;
     for (int i = 0; i < m; i++)
-        if ((int)adj[i].size() <= max_degree) cand.push_back(i);
+        if (A.len[i] <= max_degree) cand.push_back(i);
     while (!cand.empty()) {
         std::sort(cand.begin(), cand.end());
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
         sel.clear();
         for (int k : cand) {
-            if (!alive[k] || (int)adj[k].size() > max_degree) continue;
+            if (!alive[k] || A.len[k] > max_degree) continue;
             // never divide by a vanishing pivot (a floating group of anchors): keep the
             // node in the core instead
             if (!(d[k] > 1e-12 * std::fabs(C.sdiag[k])) || !std::isfinite(d[k])) continue;
             bool free = true;
-            for (auto &e : adj[k])
-                if (taken[e.first]) free = false;
+            for (int i = 0; i < A.len[k]; i++)
+                if (taken[A.nbr[A.off[k] + i]]) free = false;
             if (!free) continue;
             taken[k] = 1;
             sel.push_back(k);
@@ -502,9 +553,9 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimD
         const size_t first = Z.e_node.size();
         for (int k : sel) {
             Z.e_node.push_back(k);
-            for (auto &e : adj[k]) {
-                Z.e_nbr.push_back(e.first);
-                Z.e_a.push_back(e.second);
+            for (int i = 0; i < A.len[k]; i++) {
+                Z.e_nbr.push_back(A.nbr[A.off[k] + i]);
+                Z.e_a.push_back(A.val[A.off[k] + i]);
             }
             Z.e_off.push_back((int)Z.e_nbr.size());
             Z.e_d.push_back(d[k]);
@@ -519,18 +570,18 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimD
             taken[k] = 0;
             for (int i = b; i < e; i++) {
                 const int u = Z.e_nbr[i];
-                drop(u, k);
+                A.drop(u, k);
                 d[u] -= Z.e_a[i] * Z.e_a[i] / dk;
             }
             for (int i = b; i < e; i++)
                 for (int j = i + 1; j < e; j++) {
                     const double f = -Z.e_a[i] * Z.e_a[j] / dk;
-                    add(Z.e_nbr[i], Z.e_nbr[j], f);
-                    add(Z.e_nbr[j], Z.e_nbr[i], f);
+                    A.add(Z.e_nbr[i], Z.e_nbr[j], f);
+                    A.add(Z.e_nbr[j], Z.e_nbr[i], f);
                 }
             for (int i = b; i < e; i++)
-                if ((int)adj[Z.e_nbr[i]].size() <= max_degree) cand.push_back(Z.e_nbr[i]);
-            adj[k].clear();
+                if (A.len[Z.e_nbr[i]] <= max_degree) cand.push_back(Z.e_nbr[i]);
+            A.len[k] = 0;
         }
         Z.round_off.push_back((int)Z.e_node.size());
     }
@@ -549,8 +600,10 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimD
         const int i = Z.core_nodes[q];
         Z.cdiag[q] = d[i];
         row.clear();
-        for (auto &e : adj[i])
-            if (core_of[e.first] < q) row.push_back({core_of[e.first], e.second});
+        for (int t = 0; t < A.len[i]; t++) {
+            const int c = core_of[A.nbr[A.off[i] + t]];
+            if (c < q) row.push_back({c, A.val[A.off[i] + t]});
+        }
         std::sort(row.begin(), row.end());
         for (auto &e : row) {
             Z.ccol.push_back(e.first);
@@ -718,8 +771,10 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
                 *why = "the core system has a non-positive diagonal (floating component)";
                 return nullptr;
             }
-            ok = sys_create(ctx, Z.mc, (int64_t)Z.cval.size(), Z.crp.data(), Z.ccol.data(),
-                            Z.cval.data(), Z.cdiag.data(), kExtraResident, &c->red) != 0;
+            // the core's own vectors (right-hand side and solution of the block solver; the
+            // two-kernel and resident solvers use those of the core cicy_sys)
+            ok = calloc_dev(ctx, &c->core_b, (size_t)Z.mc) &&
+                 calloc_dev(ctx, &c->core_x, (size_t)Z.mc);
             c->h_orig.resize((size_t)Z.mc);
             for (int q = 0; q < Z.mc; q++) c->h_orig[q] = anchors[Z.core_nodes[q]];
             c->h_crp = std::move(Z.crp);
@@ -762,27 +817,34 @@ static bool condensed_correction(cicy_sys *s, int inner, const double *rhs, doub
                                    c->rv_rec, c->rv_a, c->e_node, c->e_d, c->fS);
         }
         if (c->mc > 0) {
-            cicy_sys *red = c->red;
-            hipLaunchKernelGGL(k_gather, dim3(blocks(c->mc, kCondThreads)), dim3(kCondThreads), 0,
-                               st, c->mc, c->core_nodes, c->fS, red->r0);
-            if (!hip_ok(ctx, hipMemsetAsync(red->x0, 0, (size_t)c->mc * sizeof(double), st),
-                        "memset"))
-                return false;
-            red->has_b = false;
-            red->ref2_override = ref2;
             bool ok;
-            if (inner == CICY_METHOD_RESIDENT) {
-                ok = solve_resident(red, tol, max_iter, CICY_REL_B, inner_st);
-            } else if (inner == CICY_METHOD_CONDENSED_BLOCK) {
-                if (!c->blk) ok = condensed_set_blocks(s, 0, nullptr) != 0;
-                else ok = true;
-                ok = ok && solve_core(ctx, c->blk, red->r0, tol, max_iter, ref2, red->x, inner_st);
+            const double *core_x;
+            if (inner == CICY_METHOD_CONDENSED_BLOCK) {
+                hipLaunchKernelGGL(k_gather, dim3(blocks(c->mc, kCondThreads)),
+                                   dim3(kCondThreads), 0, st, c->mc, c->core_nodes, c->fS,
+                                   c->core_b);
+                ok = (c->blk || condensed_set_blocks(s, 0, nullptr)) &&
+                     solve_core(ctx, c->blk, c->core_b, tol, max_iter, ref2, c->core_x, inner_st);
+                core_x = c->core_x;
             } else {
-                ok = solve_pcg64(red, tol, max_iter, CICY_REL_B, inner_st);
+                if (!condensed_core_sys(s, inner)) return false;
+                cicy_sys *red = c->red;
+                hipLaunchKernelGGL(k_gather, dim3(blocks(c->mc, kCondThreads)),
+                                   dim3(kCondThreads), 0, st, c->mc, c->core_nodes, c->fS,
+                                   red->r0);
+                if (!hip_ok(ctx, hipMemsetAsync(red->x0, 0, (size_t)c->mc * sizeof(double), st),
+                            "memset"))
+                    return false;
+                red->has_b = false;
+                red->ref2_override = ref2;
+                ok = inner == CICY_METHOD_RESIDENT
+                         ? solve_resident(red, tol, max_iter, CICY_REL_B, inner_st)
+                         : solve_pcg64(red, tol, max_iter, CICY_REL_B, inner_st);
+                core_x = red->x;
             }
             if (!ok) return false;
             hipLaunchKernelGGL(k_scatter, dim3(blocks(c->mc, kCondThreads)), dim3(kCondThreads), 0,
-                               st, c->mc, c->core_nodes, red->x, c->xS);
+                               st, c->mc, c->core_nodes, core_x, c->xS);
         }
         for (int q = nr - 1; q >= 0; q--) {
             const int b = c->round_off[q], e = c->round_off[q + 1];
@@ -798,6 +860,18 @@ static bool condensed_correction(cicy_sys *s, int inner, const double *rhs, doub
                        st, c->npath, c->ps, c->pe, c->pl, c->pr, c->lfac, c->mfac, c->band, rhs,
                        (const double *)(c->m > 0 ? c->xS : nullptr), c->delta);
     return true;
+}
+
+bool condensed_core_sys(cicy_sys *s, int inner) {
+    cicy_cond *c = s->cond;
+    if (!c || c->mc <= 0) return true;
+    if (!c->red) {
+        if (!sys_create(s->ctx, c->mc, (int64_t)c->h_cval.size(), c->h_crp.data(),
+                        c->h_ccol.data(), c->h_cval.data(), c->h_cdiag.data(), &c->red))
+            return false;
+        c->red->resident_rows_req = s->resident_rows_req;
+    }
+    return sys_prepare(c->red, inner);
 }
 
 int condensed_set_blocks(cicy_sys *s, int64_t nblocks, const int64_t *ranges) {
@@ -834,7 +908,8 @@ bool solve_condensed(cicy_sys *s, int inner, double tol, long long max_iter, int
         if (stats) stats->status = CICY_BAD_INPUT;
         return false;
     }
-    if (inner == CICY_METHOD_RESIDENT && c->mc > 0 && !c->red->res) {
+    if (inner == CICY_METHOD_RESIDENT && c->mc > 0 &&
+        (!condensed_core_sys(s, CICY_METHOD_RESIDENT) || !c->red->res)) {
         set_error(ctx, "the reduced system has no resident layout (%s)",
                   c->red->res_why ? c->red->res_why : "not built");
         if (stats) stats->status = CICY_BAD_INPUT;
@@ -964,6 +1039,8 @@ int cicy_host_expand(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,
 
 int cicy_sys_core(cicy_sys *s, cicy_sys **core) {
     if (!s || !core) return 0;
+    (void)hipSetDevice(s->ctx->device);
+    if (!cicy::sys_prepare(s, CICY_METHOD_CONDENSED)) return 0;
     *core = (s->cond && s->cond->red) ? s->cond->red : nullptr;
     return *core ? 1 : 0;
 }

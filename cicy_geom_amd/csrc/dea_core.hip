@@ -151,28 +151,34 @@ bool dense_spd_inverse(int n, std::vector<double> &a, double *inv) {
             a[(size_t)i * n + j] = v / l;
         }
     }
-    // columns of the inverse: L y = e_c, L^T z = y
-    std::vector<double> y((size_t)n);
-    for (int c = 0; c < n; c++) {
-        for (int i = 0; i < n; i++) {
-            double v = i == c ? 1.0 : 0.0;
-            for (int k = 0; k < i; k++) v -= a[(size_t)i * n + k] * y[k];
-            y[i] = v / a[(size_t)i * n + i];
+    // W = L^-1 (lower triangular, row-major), row by row: W_i = -(sum_k<i L_ik W_k) / L_ii
+    // with W_ii = 1 / L_ii; every inner loop runs along a row
+    std::vector<double> w((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) {
+        const double *li = &a[(size_t)i * n];
+        double *wi = &w[(size_t)i * n];
+        for (int k = 0; k < i; k++) {
+            const double lik = li[k];
+            const double *wk = &w[(size_t)k * n];
+            for (int j = 0; j <= k; j++) wi[j] -= lik * wk[j];
         }
-        for (int i = n - 1; i >= 0; i--) {
-            double v = y[i];
-            for (int k = i + 1; k < n; k++) v -= a[(size_t)k * n + i] * y[k];
-            y[i] = v / a[(size_t)i * n + i];
-        }
-        for (int i = 0; i < n; i++) inv[(size_t)i * n + c] = y[i];
+        const double inv_d = 1.0 / li[i];
+        for (int j = 0; j < i; j++) wi[j] *= inv_d;
+        wi[i] = inv_d;
     }
-    // exact symmetry (the product reads the inverse by columns)
-    for (int i = 0; i < n; i++)
-        for (int j = i + 1; j < n; j++) {
-            const double v = 0.5 * (inv[(size_t)i * n + j] + inv[(size_t)j * n + i]);
-            inv[(size_t)i * n + j] = v;
-            inv[(size_t)j * n + i] = v;
+    // A^-1 = W^T W: lower half as a sum of the rank-1 terms W_k^T W_k, then mirrored, so the
+    // inverse is exactly symmetric (the product reads it by columns)
+    std::fill(inv, inv + (size_t)n * n, 0.0);
+    for (int k = 0; k < n; k++) {
+        const double *wk = &w[(size_t)k * n];
+        for (int i = 0; i <= k; i++) {
+            const double wki = wk[i];
+            double *ri = inv + (size_t)i * n;
+            for (int j = 0; j <= i; j++) ri[j] += wki * wk[j];
         }
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) inv[(size_t)i * n + j] = inv[(size_t)j * n + i];
     return true;
 }
 

@@ -39,6 +39,14 @@ struct BandHost {
     std::vector<int> xblk, xrow, xnbr;
 };
 
+// Host copy of an uploaded system, kept until every layout built on demand from it (FP32
+// copy, resident layout, condensation) has been built.
+struct SysHost {
+    BlockedHost h;
+    BandHost bh;
+    bool have_band = false;
+};
+
 constexpr int kFormatSss = 1;   // blocked SSS with remote transpose list
 constexpr int kFormatBand = 2;  // band + remainder
 
@@ -124,6 +132,11 @@ struct cicy_sys {
     // for its reduced system: the full system's ||b|| or ||r0||)
     double ref2_override = -1.0;
     double r0norm2 = 0.0;  // ||r0||^2 of the last cicy_sys_set_rhs
+    // layouts built on first use (cicy_sys_prepare): which have been attempted, the host
+    // copy they are built from, and the resident rows per thread asked for at creation
+    bool mixed_done = false, res_done = false, cond_done = false;
+    cicy::SysHost *host = nullptr;
+    int resident_rows_req = 0;
 };
 
 namespace cicy {
@@ -188,10 +201,18 @@ void core_free(cicy_core *z);
 int core_info(const cicy_core *z, int what);
 bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long long max_iter,
                 double ref2, double *x_out, cicy_solve_stats *stats);
-// uploads a system (dea_solver.hip); `extras` also builds the resident and condensed layouts
-constexpr int kExtraResident = 1, kExtraCondensed = 2;
+// uploads a system (dea_solver.hip); the layouts of the other methods are built by
+// sys_prepare, on the first solve with them at the latest
 int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
                const int64_t *col_ind, const double *values, const double *diagonal,
-               int extras, cicy_sys **out);
+               cicy_sys **out);
+// builds what `method` needs (idempotent); false on a device or allocation error, or when
+// the method cannot run on this system (the reason in the context's error)
+// core = false stops after the condensation itself (the core solver's layout is left for
+// the next call, e.g. after cicy_sys_set_blocks has chosen its row groups)
+bool sys_prepare(cicy_sys *s, int method, bool core = true);
+// the core system of a condensed system as an ordinary cicy_sys prepared for `inner`
+// (CICY_METHOD_PCG_F64 or CICY_METHOD_RESIDENT); built on first use
+bool condensed_core_sys(cicy_sys *s, int inner);
 
 }  // namespace cicy

@@ -600,19 +600,23 @@ int cicy_set_format(cicy_ctx *ctx, int format) {
 
 namespace cicy {
 
-// Uploads a system.  extras & kExtraResident also builds the register-resident CG layout
-// (band format systems that fit the resident grid); extras & kExtraCondensed also builds
-// the condensed system (chain-interior rows eliminated) when it is worth it.
+// Uploads a system in the layout of the two-kernel FP64 solve and keeps a host copy for
+// the layouts of the other methods, which sys_prepare builds when they are first needed.
 int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
                const int64_t *col_ind, const double *values, const double *diagonal,
-               int extras, cicy_sys **out) {
+               cicy_sys **out) {
     if (!ctx || !out) return 0;
     *out = nullptr;
     if (hipSetDevice(ctx->device) != hipSuccess) return 0;
-    BlockedHost h;
-    if (!build_blocked(ctx, n, nnz, row_ptr, col_ind, values, diagonal, ctx->rows_per_block, h))
+    SysHost *keep = new SysHost();
+    BlockedHost &h = keep->h;
+    if (!build_blocked(ctx, n, nnz, row_ptr, col_ind, values, diagonal, ctx->rows_per_block, h)) {
+        delete keep;
         return 0;
+    }
     cicy_sys *s = new cicy_sys();
+    s->host = keep;
+    s->resident_rows_req = ctx->resident_rows;
     s->ctx = ctx;
     s->n = h.n;
     s->nnz = h.nnz;
@@ -623,7 +627,7 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
     const size_t nv = (size_t)h.n;
     const size_t nparts = 2 * (size_t)std::max(h.nblocks, 1024);
     s->format = choose_format(ctx, h);
-    BandHost bh;
+    BandHost &bh = keep->bh;
     bool ok = dmalloc(ctx, &s->diag, nv, &tot);
     if (s->format == kFormatSss) {
         ok = ok && dmalloc(ctx, &s->rp, nv + 1, &tot) &&
@@ -637,6 +641,7 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
              h2d(ctx, s->uval, h.uval.data(), h.nremote);
     } else {
         build_band(h, bh);
+        keep->have_band = true;
         s->nx = bh.m;
         ok = ok && dmalloc(ctx, &s->band, nv + 1, &tot) &&
              dmalloc(ctx, &s->xblk, (size_t)h.nblocks + 1, &tot) &&
@@ -676,40 +681,70 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
         cicy_sys_destroy(s);
         return 0;
     }
-    if (!cicy::mixed_setup(s, h, bh)) {
-        cicy_sys_destroy(s);
-        return 0;
-    }
-    bool have_band = s->format == kFormatBand;
-    if (!(extras & kExtraResident)) {
-        s->res_why = "not requested";
-    } else {
-        // the resident layout splits the matrix into band + remainder whatever the
-        // device format of the two-kernel path (remainder rows become published rows)
-        if (!have_band) {
-            build_band(h, bh);
-            have_band = true;
-        }
-        bool failed = false;
-        s->res = resident_setup(ctx, h, bh, ctx->resident_rows, &s->res_why, &failed);
-        if (failed) {
-            cicy_sys_destroy(s);
-            return 0;
-        }
-    }
-    if (!(extras & kExtraCondensed)) {
-        s->cond_why = "not requested";
-    } else {
-        if (!have_band) build_band(h, bh);
-        bool failed = false;
-        s->cond = condensed_setup(ctx, h, bh, &s->cond_why, &failed);
-        if (failed) {
-            cicy_sys_destroy(s);
-            return 0;
-        }
-    }
+    s->res_why = s->cond_why = "not prepared";
     *out = s;
     return 1;
+}
+
+// The host copy is dropped once every layout that is built from it has been attempted.
+static void release_host(cicy_sys *s) {
+    if (s->host && s->mixed_done && s->res_done && s->cond_done) {
+        delete s->host;
+        s->host = nullptr;
+    }
+}
+
+static BandHost &host_band(SysHost &k) {
+    if (!k.have_band) {
+        build_band(k.h, k.bh);
+        k.have_band = true;
+    }
+    return k.bh;
+}
+
+bool sys_prepare(cicy_sys *s, int method, bool core) {
+    cicy_ctx *ctx = s->ctx;
+    const bool need_mixed = method == CICY_METHOD_MIXED;
+    const bool need_res = method == CICY_METHOD_RESIDENT;
+    const bool need_cond = method == CICY_METHOD_CONDENSED ||
+                           method == CICY_METHOD_CONDENSED_RESIDENT ||
+                           method == CICY_METHOD_CONDENSED_BLOCK;
+    if (method < CICY_METHOD_PCG_F64 || method > CICY_METHOD_CONDENSED_BLOCK) {
+        set_error(ctx, "unknown method %d", method);
+        return false;
+    }
+    if (need_mixed && !s->mixed_done) {
+        s->mixed_done = true;
+        // on failure the buffers allocated so far are freed with the system
+        if (!mixed_setup(s, s->host->h, host_band(*s->host))) return false;
+    }
+    if (need_res && !s->res_done) {
+        // the resident layout splits the matrix into band + remainder whatever the device
+        // format of the two-kernel path (remainder rows become published rows)
+        s->res_done = true;
+        bool failed = false;
+        s->res = resident_setup(ctx, s->host->h, host_band(*s->host), s->resident_rows_req,
+                                &s->res_why, &failed);
+        if (failed) return false;
+    }
+    if (need_cond && !s->cond_done) {
+        s->cond_done = true;
+        bool failed = false;
+        s->cond = condensed_setup(ctx, s->host->h, host_band(*s->host), &s->cond_why, &failed);
+        if (failed) return false;
+    }
+    release_host(s);
+    if (need_cond && s->cond && core) {
+        if (method == CICY_METHOD_CONDENSED || method == CICY_METHOD_CONDENSED_RESIDENT) {
+            const int inner = method == CICY_METHOD_CONDENSED ? CICY_METHOD_PCG_F64
+                                                              : CICY_METHOD_RESIDENT;
+            if (!condensed_core_sys(s, inner)) return false;
+        }
+        if (method == CICY_METHOD_CONDENSED_BLOCK && condensed_info(s->cond, 2) > 0 &&
+            !condensed_info(s->cond, 9) && !condensed_set_blocks(s, 0, nullptr))
+            return false;
+    }
+    return true;
 }
 
 }  // namespace cicy
@@ -719,8 +754,13 @@ extern "C" {
 int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
                     const int64_t *col_ind, const double *values, const double *diagonal,
                     cicy_sys **out) {
-    return cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal,
-                            cicy::kExtraResident | cicy::kExtraCondensed, out);
+    return cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, out);
+}
+
+int cicy_sys_prepare(cicy_sys *s, int method) {
+    if (!s) return 0;
+    (void)hipSetDevice(s->ctx->device);
+    return cicy::sys_prepare(s, method) ? 1 : 0;
 }
 
 void cicy_sys_destroy(cicy_sys *s) {
@@ -731,6 +771,7 @@ void cicy_sys_destroy(cicy_sys *s) {
     cicy::mixed_free(s);
     cicy::resident_free(s->res);
     cicy::condensed_free(s->cond);
+    delete s->host;
     void *ptrs[] = {s->rp, s->col, s->val, s->diag, s->urp, s->ucol, s->uval, s->band,
                     s->xblk, s->xrow, s->xnbr, s->xval, s->x, s->r, s->p0, s->p1, s->q,
                     s->x0, s->r0, s->b, s->partials, s->gpart, s->gcount, s->st, s->scal,
@@ -790,6 +831,7 @@ int cicy_sys_set_rhs(cicy_sys *s, const double *r0, const double *x0, const doub
 int cicy_sys_set_blocks(cicy_sys *s, int64_t nblocks, const int64_t *ranges) {
     if (!s) return 0;
     (void)hipSetDevice(s->ctx->device);
+    if (!cicy::sys_prepare(s, CICY_METHOD_CONDENSED_BLOCK, false)) return 0;
     return cicy::condensed_set_blocks(s, nblocks, ranges);
 }
 
@@ -800,6 +842,13 @@ int cicy_sys_solve(cicy_sys *s, int method, double tol, int64_t max_iter, int re
     if (max_iter <= 0) max_iter = 300LL * ((long long)s->n + 2);
     if (!(tol >= 0.0)) {
         set_error(s->ctx, "tolerance must be >= 0");
+        return 0;
+    }
+    if (!cicy::sys_prepare(s, method)) {
+        if (stats) {
+            memset(stats, 0, sizeof(*stats));
+            stats->status = CICY_BAD_INPUT;
+        }
         return 0;
     }
     bool ok;
@@ -846,6 +895,7 @@ int cicy_sys_spmv_bench(cicy_sys *s, int precision, int reps, double *avg_ms) {
         set_error(ctx, "precision must be 64 or 32");
         return 0;
     }
+    if (precision == 32 && !cicy::sys_prepare(s, CICY_METHOD_MIXED)) return 0;
     // warm the caches once, then time `reps` launches between two events
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 1 && !hip_ok(ctx, hipEventRecord(ctx->ev_start, ctx->stream), "event")) return 0;
@@ -953,11 +1003,7 @@ int cicy_solve_dea_cg_sss(cicy_ctx *ctx, int64_t nodes, int64_t reserved_nodes, 
     if (n > 0) {
         cicy_sys *s = nullptr;
         const int method = ctx->dropin_method;
-        const int extras = method == CICY_METHOD_PCG_F64 ? 0
-                           : method == CICY_METHOD_RESIDENT
-                               ? cicy::kExtraResident
-                               : cicy::kExtraResident | cicy::kExtraCondensed;
-        if (!cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, extras, &s)) {
+        if (!cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, &s)) {
             if (stats) stats->status = CICY_BAD_INPUT;
             ctx->dropin_blocks.clear();
             return 0;
@@ -965,8 +1011,10 @@ int cicy_solve_dea_cg_sss(cicy_ctx *ctx, int64_t nodes, int64_t reserved_nodes, 
         // row groups handed over by cicy_set_dropin_blocks apply to this call only
         std::vector<int64_t> groups;
         groups.swap(ctx->dropin_blocks);
-        if (method == CICY_METHOD_CONDENSED_BLOCK && !groups.empty() && s->cond &&
-            !cicy::condensed_set_blocks(s, (int64_t)groups.size() / 2, groups.data())) {
+        if (method == CICY_METHOD_CONDENSED_BLOCK && !groups.empty() &&
+            (!cicy::sys_prepare(s, method, false) ||
+             (s->cond &&
+              !cicy::condensed_set_blocks(s, (int64_t)groups.size() / 2, groups.data())))) {
             cicy_sys_destroy(s);
             if (stats) stats->status = CICY_BAD_INPUT;
             return 0;

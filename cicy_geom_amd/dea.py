@@ -41,6 +41,7 @@ EXPORTED_SYMBOLS = [
     "cicy_host_blocked_spmv", "cicy_set_resident_rows", "cicy_host_resident_apply",
     "cicy_host_condense", "cicy_host_expand", "cicy_host_core", "cicy_host_core_expand",
     "cicy_sys_core", "cicy_set_dropin_method", "cicy_sys_set_blocks", "cicy_host_core_block_cg",
+    "cicy_sys_prepare",
     "cicy_set_dropin_blocks",
 ]
 
@@ -112,6 +113,7 @@ def lib():
     L.cicy_sys_get.argtypes = [_VP, _VP, _VP]
     L.cicy_sys_core.argtypes = [_VP, P(_VP)]
     L.cicy_sys_set_blocks.argtypes = [_VP, ctypes.c_int64, _VP]
+    L.cicy_sys_prepare.argtypes = [_VP, ctypes.c_int]
     L.cicy_set_dropin_blocks.argtypes = [_VP, ctypes.c_int64, _VP]
     L.cicy_host_core_block_cg.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64,
                                           ctypes.c_int64, ctypes.c_int64, _VP, _F64,
@@ -321,6 +323,10 @@ class System:
                       "core_rows": 11, "core_nnz": 12, "series_rounds": 13,
                       "core_block_rows": 14, "core_blocks": 15, "core_groups": 16,
                       "core_block_entries": 17}[what]))
+
+    def prepare(self, method):
+        """Build the layouts `method` needs now instead of in its first solve."""
+        self.ctx._check(self.ctx._L.cicy_sys_prepare(self._h, method), "cicy_sys_prepare")
 
     def set_blocks(self, ranges):
         """Row groups [(start, end), ...] of free-node rows (e.g. the nodes of each

@@ -151,6 +151,12 @@ int cicy_boundary_current_f64(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
 int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
                     const int64_t *col_ind, const double *values, const double *diagonal,
                     cicy_sys **out);
+/* Builds what `method` needs on top of the uploaded system (the FP32 copy, the resident
+ * layout, the condensation and its core solver's layout); idempotent.  cicy_sys_solve
+ * calls it, so it is optional: it moves that set-up out of the first solve.  0 on a device
+ * error or when the method cannot run on this system (e.g. nothing to condense).  No
+ * counterpart in the reference. */
+int cicy_sys_prepare(cicy_sys *sys, int method);
 void cicy_sys_destroy(cicy_sys *sys);
 int64_t cicy_sys_size(const cicy_sys *sys, int what); /* 0 n, 1 nnz, 2 remote entries,
                                                            3 device bytes, 4 format,
@@ -167,7 +173,9 @@ int64_t cicy_sys_size(const cicy_sys *sys, int what); /* 0 n, 1 nnz, 2 remote en
                                                            15 dense blocks, 16 work-groups
                                                            of the block-preconditioned core
                                                            CG (0: not built), 17 entries of
-                                                           its dense block inverses */
+                                                           its dense block inverses;
+                                                           6-17 describe layouts built by
+                                                           cicy_sys_prepare (0 before) */
 /* r0 = b - A x0 (the reference's R), x0 (may be NULL = zeros), b (may be NULL; needed for
  * the true residual and for CICY_REL_B). */
 int cicy_sys_set_rhs(cicy_sys *sys, const double *r0, const double *x0, const double *b);

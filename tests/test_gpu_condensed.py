@@ -107,6 +107,7 @@ def test_resident_forced_rows_several_groups(ctx, systems, truths, ref_errors, r
     finally:
         ctx.set_resident_rows(0)
     try:
+        sysd.prepare(dea.METHOD_RESIDENT)  # built with the rows asked for at creation
         assert sysd.size("resident_rows") == rows
         assert sysd.size("resident_groups") == -(-s.n // (512 * rows))
         sysd.set_rhs(s.b, None, s.b)
