@@ -21,13 +21,23 @@
 //     A plate's inverse is dominated by one constant c (its nearly rigid mode), so it is
 //     stored as c 11^T (FP64) plus the remainder R = A_bb^-1 - c 11^T in FP32: R's entries
 //     are small, its rounding (~1e-10 of ||R||) stays far below R's smallest eigenvalue, so
-//     the preconditioner stays symmetric positive definite at half the bytes.  Each
-//     work-group owns whole blocks, so the block solve is a work-group-local dense product
-//     with the block's residual staged in LDS;
-//   * vectors in HBM: u = M^-1 r, which other work-groups gather for the product w = A u,
-//     is written and read with device-coherent accesses (dea_grid.h); x, r, w, p, s are
-//     private to the owning work-group.  The product reads the matrix in sliced ELL (64-row
-//     slices, slot-major) so that its column and value loads are coalesced.
+//     the preconditioner stays symmetric positive definite at half the bytes.  Each block
+//     is solved by one work-group: a work-group-local dense product with the block's
+//     residual staged in LDS;
+//   * two partitions of the rows, one per phase, each balanced on its own: the core rows
+//     are renumbered so that the Jacobi rows come first and the rows of each block follow
+//     as one run; the vector updates and the preconditioner (phase u) give each
+//     work-group a run of Jacobi rows plus a run of whole blocks, sized by a cost model
+//     measured on MI355X (row updates, inverse entries, rounds of block products); the
+//     product w = A u (phase w) gives each work-group an equal share of the rows.  Plate
+//     rows are few per work-group but dear in phase u, tube rows cheap there but as dear
+//     as any row in the product, so one partition for both phases leaves either phase
+//     waiting at its barrier for the work-groups the other phase favoured;
+//   * vectors in HBM: u and w, which cross between the partitions (and u is gathered by
+//     the product), are written and read with device-coherent accesses (dea_grid.h);
+//     x, r, p, s are private to the phase-u owner of the row.  The product reads the
+//     matrix in sliced ELL (64-row slices, slot-major) so that its column and value loads
+//     are coalesced.
 //
 // Iteration (u = M^-1 r, w = A u, gamma = r.u, delta = w.u; Chronopoulos & Gear 1989):
 //   p = u + beta p;  s = w + beta s;  x += alpha p;  r -= alpha s;  u = M^-1 r
@@ -64,6 +74,12 @@ constexpr int kCoreUnroll = 12;        // block-inverse columns loaded before us
 constexpr int kCoreEnt = 8;            // row slots loaded before their gathers
 constexpr int kCoreSums = 4;           // entries per work-group slot of the grid sums
 constexpr int kCorePasses = 10;        // corrections from the true residual (stalled solves)
+// Phase-u cost per work-group on MI355X, measured (us): kCostRow per row update (p, s, x,
+// r, u), kCostEntry per block-inverse entry, kCostRound per round of kCorePairs blocks.
+constexpr double kCostRow = 2.0e-3, kCostEntry = 6.3e-5, kCostRound = 3.2;
+// Phase-w cost of a row, in units of one stored entry (a row is one lane's chain of
+// dependent loads; its entries are the slots of that chain)
+constexpr int kCostWRow = 64;
 
 struct CoreState {
     long long k;         // iterations done in this solve
@@ -77,19 +93,21 @@ struct CoreState {
 
 struct CoreDev {
     int m, G;
-    const int *seg;       // [G+1] rows of each work-group (blocks are never split)
-    // the off-diagonal entries of both triangles in sliced ELL: work-group g's rows are
-    // cut into slices of 64 rows (wsl[g] .. wsl[g+1]); slice s holds sl_len[s] slots, slot
-    // t of its lane-th row at ell_*[sl_off[s] + t * 64 + lane] (column -1: padding), in
-    // the row's column order
+    // phase u: work-group g updates the Jacobi rows ju[g] .. ju[g+1] and the rows of the
+    // blocks bu[g] .. bu[g+1] (rows numbered Jacobi rows first, then block by block)
+    const int *ju, *bu;
+    // phase w: work-group g forms w for the rows wr[g] .. wr[g+1]
+    const int *wr;
+    // the off-diagonal entries of both triangles in sliced ELL: work-group g's phase-w rows
+    // are cut into slices of 64 rows (wsl[g] .. wsl[g+1]); slice s holds sl_len[s] slots,
+    // slot t of its lane-th row at ell_*[sl_off[s] + t * 64 + lane] (column -1: padding),
+    // in the row's column order
     const int *wsl;
     const long long *sl_off;
     const int *sl_len;
     const int *ell_col;
     const double *ell_val;
     const double *diag;   // [m]
-    const int *blk_of;    // [m] block of each row, -1 for Jacobi rows
-    const int *wblk;      // [G+1] blocks of each work-group
     const int *blk_row0;  // [nb]
     const int *blk_n;     // [nb]
     const long long *blk_off;  // [nb] offset of the block's remainder (row-major, symmetric)
@@ -105,7 +123,7 @@ struct CoreDev {
 struct cicy_core {
     int m = 0, G = 0, nb = 0, blocked_rows = 0;
     long long nnz2 = 0, inv_entries = 0;
-    int *seg = nullptr, *blk_of = nullptr, *wblk = nullptr;
+    int *ju = nullptr, *bu = nullptr, *wr = nullptr, *perm = nullptr;
     int *blk_row0 = nullptr, *blk_n = nullptr;
     int *wsl = nullptr, *sl_len = nullptr, *ell_col = nullptr;
     long long *blk_off = nullptr, *sl_off = nullptr;
@@ -126,8 +144,9 @@ namespace {
 // ------------------------------------------------------------------------------------
 
 struct CoreHost {
-    int m = 0, G = 1;
-    std::vector<int> rp, col, blk_of, seg, wblk, blk_row0, blk_n;
+    int m = 0, G = 1, mj = 0;    // rows, work-groups, Jacobi rows (numbered first)
+    std::vector<int> perm;       // [m] caller's core row of each row
+    std::vector<int> rp, col, blk_of, ju, bu, wr, blk_row0, blk_n;
     std::vector<int> wsl, sl_len, ell_col;
     std::vector<long long> sl_off;
     std::vector<double> ell_val;
@@ -276,76 +295,155 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
         }
         for (auto &t : th) t.join();
     }
-    // keep the blocks that factorised (a block that did not stays on Jacobi)
+    // keep the blocks that factorised (a block that did not stays on Jacobi) and renumber
+    // the rows: Jacobi rows first (in order), then the rows of each block, block by block
+    std::vector<char> in_blk((size_t)m, 0);
+    for (size_t g = 0; g < ng; g++)
+        if (ok[g])
+            for (int q = groups[g].first; q < groups[g].second; q++) in_blk[q] = 1;
+    std::vector<int> &perm = H.perm;
+    perm.clear();
+    perm.reserve((size_t)m);
+    for (int q = 0; q < m; q++)
+        if (!in_blk[q]) perm.push_back(q);
+    H.mj = (int)perm.size();
+    for (size_t g = 0; g < ng; g++)
+        if (ok[g])
+            for (int q = groups[g].first; q < groups[g].second; q++) perm.push_back(q);
+    std::vector<int> iperm((size_t)m);
+    for (int i = 0; i < m; i++) iperm[perm[i]] = i;
+    {
+        std::vector<int> rp2((size_t)m + 1, 0), col2(H.col.size());
+        std::vector<double> val2(H.val.size()), diag2((size_t)m);
+        for (int i = 0; i < m; i++) {
+            const int o = perm[i];
+            diag2[i] = H.diag[o];
+            row.clear();
+            for (int k = H.rp[o]; k < H.rp[o + 1]; k++) row.emplace_back(iperm[H.col[k]], H.val[k]);
+            std::sort(row.begin(), row.end());
+            rp2[i + 1] = rp2[i] + (int)row.size();
+            for (size_t q = 0; q < row.size(); q++) {
+                col2[(size_t)rp2[i] + q] = row[q].first;
+                val2[(size_t)rp2[i] + q] = row[q].second;
+            }
+        }
+        H.rp.swap(rp2);
+        H.col.swap(col2);
+        H.val.swap(val2);
+        H.diag.swap(diag2);
+    }
     H.blk_of.assign((size_t)m, -1);
     H.blk_row0.clear();
     H.blk_n.clear();
     H.blk_off.clear();
     H.blk_c.clear();
-    H.binv.clear();
+    long long kept = 0;
+    for (size_t g = 0; g < ng; g++)
+        if (ok[g]) kept += off[g + 1] - off[g];
+    H.binv.resize((size_t)kept);
+    int row0 = H.mj;
+    long long at = 0;
     for (size_t g = 0; g < ng; g++) {
         if (!ok[g]) continue;
-        const int q0 = groups[g].first, nb = groups[g].second - groups[g].first;
+        const int nb = groups[g].second - groups[g].first;
         const int id = (int)H.blk_row0.size();
-        for (int i = 0; i < nb; i++) H.blk_of[q0 + i] = id;
-        H.blk_row0.push_back(q0);
+        for (int i = 0; i < nb; i++) H.blk_of[row0 + i] = id;
+        H.blk_row0.push_back(row0);
         H.blk_n.push_back(nb);
-        H.blk_off.push_back((long long)H.binv.size());
+        H.blk_off.push_back(at);
         // constant part: the mean entry of the inverse
         double c = 0.0;
         for (long long e = off[g]; e < off[g + 1]; e++) c += inv[(size_t)e];
         c /= (double)nb * (double)nb;
         H.blk_c.push_back(c);
-        for (long long e = off[g]; e < off[g + 1]; e++) H.binv.push_back((float)(inv[(size_t)e] - c));
+        for (long long e = off[g]; e < off[g + 1]; e++)
+            H.binv[(size_t)at++] = (float)(inv[(size_t)e] - c);
+        row0 += nb;
     }
-    // work-group segments: equal shares of (rows + entries + dense block work), cut only
-    // between rows that are not inside the same block
+    const int nbk = (int)H.blk_row0.size();
     H.G = std::max(1, std::min(max_groups, (m + kCoreRowsPerGroup - 1) / kCoreRowsPerGroup));
-    std::vector<double> cost((size_t)m);
-    double total = 0.0;
-    for (int r = 0; r < m; r++) {
-        cost[r] = 4.0 + (H.rp[r + 1] - H.rp[r]);
-        const int b = H.blk_of[r];
-        if (b >= 0) cost[r] += 0.25 * H.blk_n[b];
-        total += cost[r];
+    const int G = H.G;
+    // phase u: the blocks cut into G runs of equal estimated cost, then the Jacobi rows
+    // shared out so that every work-group's estimate reaches the same level T
+    std::vector<double> bcost((size_t)nbk);
+    double btotal = 0.0;
+    for (int b = 0; b < nbk; b++) {
+        const double n = H.blk_n[b];
+        bcost[b] = kCostEntry * n * n + kCostRow * n + kCostRound / kCorePairs;
+        btotal += bcost[b];
     }
-    H.seg.assign(1, 0);
-    double acc = 0.0;
-    for (int r = 0; r < m; r++) {
-        acc += cost[r];
-        const bool cut_ok = r + 1 == m || H.blk_of[r] < 0 || H.blk_of[r + 1] != H.blk_of[r];
-        if (cut_ok && (int)H.seg.size() < H.G && acc >= total * (double)H.seg.size() / H.G)
-            H.seg.push_back(r + 1);
+    H.bu.assign((size_t)G + 1, nbk);
+    H.bu[0] = 0;
+    {
+        double acc = 0.0;
+        int g = 1;
+        for (int b = 0; b < nbk && g < G; b++) {
+            acc += bcost[b];
+            while (g < G && acc >= btotal * (double)g / G) H.bu[g++] = b + 1;
+        }
     }
-    if (H.seg.back() != m) H.seg.push_back(m);
-    H.G = (int)H.seg.size() - 1;
-    H.wblk.assign((size_t)H.G + 1, 0);
-    int b = 0;
-    for (int g = 0; g < H.G; g++) {
-        H.wblk[g] = b;
-        while (b < (int)H.blk_row0.size() && H.blk_row0[b] < H.seg[g + 1]) b++;
+    std::vector<double> cb((size_t)G, 0.0);
+    for (int g = 0; g < G; g++) {
+        const int cnt = H.bu[g + 1] - H.bu[g];
+        for (int b = H.bu[g]; b < H.bu[g + 1]; b++) {
+            const double n = H.blk_n[b];
+            cb[g] += kCostEntry * n * n + kCostRow * n;
+        }
+        cb[g] += kCostRound * (double)((cnt + kCorePairs - 1) / kCorePairs);
     }
-    H.wblk[H.G] = (int)H.blk_row0.size();
-    // sliced ELL of each work-group's rows
+    auto rows_at = [&](double T) {
+        double t = 0.0;
+        for (int g = 0; g < G; g++) t += std::max(0.0, (T - cb[g]) / kCostRow);
+        return t;
+    };
+    double lo = 0.0, hi = *std::max_element(cb.begin(), cb.end()) + kCostRow * (H.mj + 1.0);
+    for (int it = 0; it < 100; it++) {
+        const double mid = 0.5 * (lo + hi);
+        (rows_at(mid) < H.mj ? lo : hi) = mid;
+    }
+    H.ju.assign((size_t)G + 1, 0);
+    {
+        long long given = 0;
+        for (int g = 0; g < G; g++) {
+            long long c = (long long)std::floor(std::max(0.0, (hi - cb[g]) / kCostRow));
+            c = std::min<long long>(c, H.mj - given);
+            if (g == G - 1) c = H.mj - given;
+            given += c;
+            H.ju[g + 1] = (int)given;
+        }
+    }
+    // phase w: the rows cut into G runs of equal cost (kCostWRow per row + its entries)
+    H.wr.assign((size_t)G + 1, m);
+    H.wr[0] = 0;
+    {
+        const double total = (double)kCostWRow * m + (double)H.rp[m];
+        double acc = 0.0;
+        int g = 1;
+        for (int r = 0; r < m && g < G; r++) {
+            acc += kCostWRow + (H.rp[r + 1] - H.rp[r]);
+            while (g < G && acc >= total * (double)g / G) H.wr[g++] = r + 1;
+        }
+    }
+    // sliced ELL of each work-group's phase-w rows
     H.wsl.assign(1, 0);
     H.sl_off.clear();
     H.sl_len.clear();
     H.ell_col.clear();
     H.ell_val.clear();
-    for (int gg = 0; gg < H.G; gg++) {
-        for (int base = H.seg[gg]; base < H.seg[gg + 1]; base += kWave) {
-            const int end = std::min(H.seg[gg + 1], base + kWave);
+    for (int gg = 0; gg < G; gg++) {
+        for (int base = H.wr[gg]; base < H.wr[gg + 1]; base += kWave) {
+            const int end = std::min(H.wr[gg + 1], base + kWave);
             int len = 0;
             for (int r = base; r < end; r++) len = std::max(len, H.rp[r + 1] - H.rp[r]);
-            const size_t off = H.ell_col.size();
-            H.sl_off.push_back((long long)off);
+            const size_t o = H.ell_col.size();
+            H.sl_off.push_back((long long)o);
             H.sl_len.push_back(len);
-            H.ell_col.resize(off + (size_t)len * kWave, -1);
-            H.ell_val.resize(off + (size_t)len * kWave, 0.0);
+            H.ell_col.resize(o + (size_t)len * kWave, -1);
+            H.ell_val.resize(o + (size_t)len * kWave, 0.0);
             for (int r = base; r < end; r++)
                 for (int t = 0; t < H.rp[r + 1] - H.rp[r]; t++) {
-                    H.ell_col[off + (size_t)t * kWave + (r - base)] = H.col[(size_t)H.rp[r] + t];
-                    H.ell_val[off + (size_t)t * kWave + (r - base)] = H.val[(size_t)H.rp[r] + t];
+                    H.ell_col[o + (size_t)t * kWave + (r - base)] = H.col[(size_t)H.rp[r] + t];
+                    H.ell_val[o + (size_t)t * kWave + (r - base)] = H.val[(size_t)H.rp[r] + t];
                 }
         }
         H.wsl.push_back((int)H.sl_len.size());
@@ -446,6 +544,14 @@ long long host_core_cg(const CoreHost &H, const double *rhs, double tol, double 
 // the persistent kernel
 // ------------------------------------------------------------------------------------
 
+// dir 0: dst[q] = src[perm[q]] (caller's core order -> the kernel's); dir 1: the inverse
+__global__ void k_core_permute(int m, const int *perm, const double *src, double *dst, int dir) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    if (dir == 0) dst[q] = src[perm[q]];
+    else dst[perm[q]] = src[q];
+}
+
 // One chunk of iterations.  All work-groups hold the same scalars (the grid sums are
 // bit-identical everywhere), so they take the same branches and leave together.
 __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *st, int chunk) {
@@ -455,8 +561,9 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
     __shared__ int flag;
     const int g = blockIdx.x, tid = threadIdx.x;
     const int wave = tid / kWave, lane = tid & (kWave - 1);
-    const int r0 = a.seg[g], r1 = a.seg[g + 1];
-    const int b0 = a.wblk[g], b1 = a.wblk[g + 1];
+    const int j0 = a.ju[g], j1 = a.ju[g + 1];     // phase u: Jacobi rows
+    const int b0 = a.bu[g], b1 = a.bu[g + 1];     // phase u: blocks
+    const int w0 = a.wr[g], w1 = a.wr[g + 1];     // phase w: rows
     const int s0 = a.wsl[g], s1 = a.wsl[g + 1];
     long long k = st->k;
     const long long max_iter = st->max_iter;
@@ -489,11 +596,25 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
         }
         return acc;
     };
-    auto blocks_apply = [&](const double *y, double *dst, double &pg, double &pr) {
-        // dst_b = M_b^-1 y_b for the blocks of this work-group (y: this work-group's rows,
-        // already stored), to dst with coherent stores; pg += y.dst, pr += y.y over them.
-        // Blocks go kCorePairs at a time; waves 2 q and 2 q + 1 share block base + q: each
-        // stages half of its rows, multiplies half of its columns, finalises half of its rows.
+    // the update of row i (when update) and its new residual
+    auto update_row = [&](int i, bool update) {
+        double ri = a.r[i];
+        if (update) {
+            const double pi = fma(beta, a.p[i], ld_coh(&a.u[i]));
+            const double si = fma(beta, a.s[i], ld_coh(&a.w[i]));
+            a.p[i] = pi;
+            a.s[i] = si;
+            a.x[i] = fma(alpha, pi, a.x[i]);
+            ri = fma(-alpha, si, ri);
+            a.r[i] = ri;
+        }
+        return ri;
+    };
+    auto blocks_apply = [&](bool update, double &pg, double &pr) {
+        // for the blocks of this work-group: the rows' updates, then u_b = M_b^-1 r_b with
+        // coherent stores; pg += r.u, pr += r.r over their rows.  Blocks go kCorePairs at a
+        // time; waves 2 q and 2 q + 1 share block base + q: each updates and stages half of
+        // its rows, multiplies half of its columns, finalises half of its rows.
         const int slot = wave >> 1, half = wave & 1;
         for (int base = b0; base < b1; base += kCorePairs) {
             const int bi = base + slot;
@@ -501,7 +622,7 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
             const int nbk = act ? a.blk_n[bi] : 0;
             const int q0 = act ? a.blk_row0[bi] : 0;
             const int jr = half * kWave + lane;  // this lane's row of the block
-            if (jr < nbk) yb[slot][jr] = y[q0 + jr];
+            if (jr < nbk) yb[slot][jr] = update_row(q0 + jr, update);
             __syncthreads();
             if (act) {
                 // partial product over this wave's half of the columns: lane i accumulates
@@ -541,7 +662,7 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
                 if (jr < nbk) {
                     const double v = (a.blk_c[bi] * sum + zp[slot][0][jr]) + zp[slot][1][jr];
                     const double yv = yb[slot][jr];
-                    st_coh(&dst[q0 + jr], v);
+                    st_coh(&a.u[q0 + jr], v);
                     pg += yv * v;
                     pr += yv * yv;
                 }
@@ -550,41 +671,29 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
         }
     };
     // p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s (when update) and u = M^-1 r
-    // for the rows of this work-group; partial r.u and r.r
+    // for the phase-u rows of this work-group; partial r.u and r.r
     auto phase_u = [&](bool update, double &pg, double &pr) {
         pg = 0.0;
         pr = 0.0;
-        for (int i = r0 + tid; i < r1; i += kCoreThreads) {
-            double ri = a.r[i];
-            if (update) {
-                const double pi = fma(beta, a.p[i], ld_coh(&a.u[i]));
-                const double si = fma(beta, a.s[i], a.w[i]);
-                a.p[i] = pi;
-                a.s[i] = si;
-                a.x[i] = fma(alpha, pi, a.x[i]);
-                ri = fma(-alpha, si, ri);
-                a.r[i] = ri;
-            }
-            if (a.blk_of[i] < 0) {
-                const double ui = (1.0 / a.diag[i]) * ri;
-                st_coh(&a.u[i], ui);
-                pg += ri * ui;
-                pr += ri * ri;
-            }
+        for (int i = j0 + tid; i < j1; i += kCoreThreads) {
+            const double ri = update_row(i, update);
+            const double ui = (1.0 / a.diag[i]) * ri;
+            st_coh(&a.u[i], ui);
+            pg += ri * ui;
+            pr += ri * ri;
         }
-        __syncthreads();
-        blocks_apply(a.r, a.u, pg, pr);
+        blocks_apply(update, pg, pr);
     };
-    // w = A u for the rows of this work-group, partial w.u
+    // w = A u for the phase-w rows of this work-group, partial w.u
     auto phase_w = [&]() {
         double pd = 0.0;
         for (int sl = s0 + wave; sl < s1; sl += kCoreWaves) {
-            const int row = r0 + (sl - s0) * kWave + lane;
-            const bool live = row < r1;
+            const int row = w0 + (sl - s0) * kWave + lane;
+            const bool live = row < w1;
             double uo;
             const double wv = row_product(sl, row, live, a.u, uo);
             if (live) {
-                a.w[row] = wv;
+                st_coh(&a.w[row], wv);
                 pd += wv * uo;
             }
         }
@@ -686,7 +795,7 @@ bool zeros(cicy_ctx *ctx, cicy_core *z, double **dst, size_t count) {
 
 void core_free(cicy_core *z) {
     if (!z) return;
-    int *ip[] = {z->seg, z->blk_of, z->wblk, z->blk_row0, z->blk_n, z->wsl, z->sl_len,
+    int *ip[] = {z->ju, z->bu, z->wr, z->perm, z->blk_row0, z->blk_n, z->wsl, z->sl_len,
                  z->ell_col};
     for (int *q : ip)
         if (q) (void)hipFree(q);
@@ -742,11 +851,12 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
     z->nnz2 = (long long)H.col.size();
     z->inv_entries = (long long)H.binv.size();
     for (int v : H.blk_n) z->blocked_rows += v;
-    bool ok = up_vec(ctx, z, &z->seg, H.seg) && up_vec(ctx, z, &z->wsl, H.wsl) &&
+    bool ok = up_vec(ctx, z, &z->ju, H.ju) && up_vec(ctx, z, &z->bu, H.bu) &&
+              up_vec(ctx, z, &z->wr, H.wr) && up_vec(ctx, z, &z->perm, H.perm) &&
+              up_vec(ctx, z, &z->wsl, H.wsl) &&
               up_vec(ctx, z, &z->sl_off, H.sl_off) && up_vec(ctx, z, &z->sl_len, H.sl_len) &&
               up_vec(ctx, z, &z->ell_col, H.ell_col) && up_vec(ctx, z, &z->ell_val, H.ell_val) &&
-              up_vec(ctx, z, &z->diag, H.diag) && up_vec(ctx, z, &z->blk_of, H.blk_of) &&
-              up_vec(ctx, z, &z->wblk, H.wblk) && up_vec(ctx, z, &z->blk_row0, H.blk_row0) &&
+              up_vec(ctx, z, &z->diag, H.diag) && up_vec(ctx, z, &z->blk_row0, H.blk_row0) &&
               up_vec(ctx, z, &z->blk_n, H.blk_n) && up_vec(ctx, z, &z->blk_off, H.blk_off) &&
               up_vec(ctx, z, &z->blk_c, H.blk_c) && up_vec(ctx, z, &z->binv, H.binv) &&
               zeros(ctx, z, &z->x, m) &&
@@ -778,15 +888,18 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
     memset(&h, 0, sizeof(h));
     h.max_iter = max_iter > 0 ? max_iter : 1;
     h.stop2 = tol * tol * ref2;
-    bool ok = hip_ok(ctx, hipMemcpyAsync(z->r, rhs, vb, hipMemcpyDeviceToDevice, stream), "D2D") &&
+    const int pg = (z->m + 255) / 256;
+    hipLaunchKernelGGL(k_core_permute, dim3(pg), dim3(256), 0, stream, z->m, z->perm, rhs, z->r,
+                       0);
+    bool ok = hip_ok(ctx, hipGetLastError(), "k_core_permute") &&
               hip_ok(ctx, hipMemsetAsync(z->x, 0, vb, stream), "memset") &&
               hip_ok(ctx, hipMemsetAsync(z->s, 0, vb, stream), "memset") &&
               hip_ok(ctx, hipMemsetAsync(z->p, 0, vb, stream), "memset") &&
               hip_ok(ctx, hipMemcpyAsync(z->st, &h, sizeof(h), hipMemcpyHostToDevice, stream),
                      "H2D");
-    CoreDev a{z->m, z->G, z->seg, z->wsl, z->sl_off, z->sl_len, z->ell_col, z->ell_val,
-              z->diag, z->blk_of, z->wblk, z->blk_row0, z->blk_n, z->blk_off, z->blk_c,
-              z->binv, z->x, z->r, z->u, z->w, z->p, z->s, z->ctl, z->part};
+    CoreDev a{z->m, z->G, z->ju, z->bu, z->wr, z->wsl, z->sl_off, z->sl_len, z->ell_col,
+              z->ell_val, z->diag, z->blk_row0, z->blk_n, z->blk_off, z->blk_c, z->binv,
+              z->x, z->r, z->u, z->w, z->p, z->s, z->ctl, z->part};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ok = ok && hip_ok(ctx, hipEventCreate(&e0), "hipEventCreate") &&
          hip_ok(ctx, hipEventCreate(&e1), "hipEventCreate") &&
@@ -811,7 +924,11 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
         (void)hipEventElapsedTime(&kms, e0, e1);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    if (ok) ok = hip_ok(ctx, hipMemcpyAsync(x_out, z->x, vb, hipMemcpyDeviceToDevice, stream), "D2D");
+    if (ok) {
+        hipLaunchKernelGGL(k_core_permute, dim3(pg), dim3(256), 0, stream, z->m, z->perm, z->x,
+                           x_out, 1);
+        ok = hip_ok(ctx, hipGetLastError(), "k_core_permute");
+    }
     memset(stats, 0, sizeof(*stats));
     stats->kernel_ms = kms;
     stats->iters = h.k;
@@ -854,18 +971,20 @@ extern "C" int cicy_host_core_block_cg(int64_t m, const int64_t *row_ptr, const 
     // passes from the true residual, as the condensed solve corrects after a stalled one
     const long long cap = max_iter > 0 ? max_iter : 10 * m + 100;
     const double stop2 = tol * tol * ref2;
-    std::vector<double> res(rhs, rhs + m), dx((size_t)m), ax((size_t)m);
-    std::fill(x, x + m, 0.0);
+    // in the layout's row order (Jacobi rows first, then block by block)
+    std::vector<double> b((size_t)m), xh((size_t)m, 0.0), res((size_t)m), dx((size_t)m),
+        ax((size_t)m);
+    for (int64_t i = 0; i < m; i++) b[i] = res[i] = rhs[H.perm[i]];
     long long k = 0;
     int status = CICY_MAXITER;
     for (int pass = 0; pass < kCorePasses && k < cap; pass++) {
         int st = CICY_OK;
         k += host_core_cg(H, res.data(), tol, ref2, cap - k, dx.data(), &st);
-        for (int64_t i = 0; i < m; i++) x[i] += dx[i];
-        host_core_apply(H, x, ax.data());
+        for (int64_t i = 0; i < m; i++) xh[i] += dx[i];
+        host_core_apply(H, xh.data(), ax.data());
         double rr = 0.0;
         for (int64_t i = 0; i < m; i++) {
-            res[i] = rhs[i] - ax[i];
+            res[i] = b[i] - ax[i];
             rr += res[i] * res[i];
         }
         if (rr <= stop2) {
@@ -877,6 +996,7 @@ extern "C" int cicy_host_core_block_cg(int64_t m, const int64_t *row_ptr, const 
             break;
         }
     }
+    for (int64_t i = 0; i < m; i++) x[H.perm[i]] = xh[i];
     if (iters) *iters = k;
     if (blocked_rows) {
         long long br = 0;
