@@ -115,7 +115,7 @@ struct CoreDev {
     const float *binv;    // remainders R = A_bb^-1 - c 11^T
     double *x, *r, *u, *w, *p, *s;
     GridCtl *ctl;
-    double *part;         // [2][G][kCoreSums]
+    unsigned long long *part;  // [3][G][kCoreSums] grid-sum slots (grid_sum_slots)
 };
 
 }  // namespace cicy
@@ -130,7 +130,7 @@ struct cicy_core {
     double *ell_val = nullptr, *diag = nullptr, *blk_c = nullptr;
     float *binv = nullptr;
     double *x = nullptr, *r = nullptr, *u = nullptr, *w = nullptr, *p = nullptr, *s = nullptr;
-    double *part = nullptr;
+    unsigned long long *part = nullptr;
     cicy::GridCtl *ctl = nullptr;
     cicy::CoreState *st = nullptr;
     size_t bytes = 0;
@@ -363,8 +363,9 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
     const int nbk = (int)H.blk_row0.size();
     H.G = std::max(1, std::min(max_groups, (m + kCoreRowsPerGroup - 1) / kCoreRowsPerGroup));
     const int G = H.G;
-    // phase u: the blocks cut into G runs of equal estimated cost, then the Jacobi rows
-    // shared out so that every work-group's estimate reaches the same level T
+    // phase u: the blocks cut into G runs of about equal estimated cost, each at most one
+    // round of the block product (kCorePairs blocks) where the blocks allow it, then the
+    // Jacobi rows shared out so that every work-group's estimate reaches the same level
     std::vector<double> bcost((size_t)nbk);
     double btotal = 0.0;
     for (int b = 0; b < nbk; b++) {
@@ -373,13 +374,19 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
         btotal += bcost[b];
     }
     H.bu.assign((size_t)G + 1, nbk);
-    H.bu[0] = 0;
     {
+        const int cap = nbk <= G * kCorePairs ? kCorePairs : (nbk + G - 1) / G;
+        int b = 0;
         double acc = 0.0;
-        int g = 1;
-        for (int b = 0; b < nbk && g < G; b++) {
-            acc += bcost[b];
-            while (g < G && acc >= btotal * (double)g / G) H.bu[g++] = b + 1;
+        for (int g = 0; g < G; g++) {
+            H.bu[g] = b;
+            // at least `must` blocks here, so that the rest fits the work-groups left
+            const int must = std::max(0, (nbk - b) - (G - g - 1) * cap);
+            const double target = btotal * (double)(g + 1) / G;
+            for (int cnt = 0; b < nbk && cnt < cap &&
+                              (cnt < must || acc + 0.5 * bcost[b] <= target);
+                 cnt++)
+                acc += bcost[b++];
         }
     }
     std::vector<double> cb((size_t)G, 0.0);
@@ -703,7 +710,7 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
     if (!done && !started) {
         double v[2];
         phase_u(false, v[0], v[1]);
-        if (!grid_sum<2, kCoreWaves, kCoreSums>(v, a.G, a.part, a.ctl, bar++, lds, &flag)) {
+        if (!grid_sum_slots<2, kCoreWaves, kCoreSums>(v, a.G, a.part, a.ctl, bar++, lds, &flag)) {
             done = 4;
         } else {
             gamma = v[0];
@@ -712,7 +719,7 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
                 done = 1;
             } else {
                 double d[1] = {phase_w()};
-                if (!grid_sum<1, kCoreWaves, kCoreSums>(d, a.G, a.part, a.ctl, bar++, lds, &flag)) {
+                if (!grid_sum_slots<1, kCoreWaves, kCoreSums>(d, a.G, a.part, a.ctl, bar++, lds, &flag)) {
                     done = 4;
                 } else {
                     alpha = gamma / d[0];
@@ -725,7 +732,7 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
     for (int it = 0; it < chunk && !done; it++) {
         double v[2];
         phase_u(true, v[0], v[1]);
-        if (!grid_sum<2, kCoreWaves, kCoreSums>(v, a.G, a.part, a.ctl, bar++, lds, &flag)) {
+        if (!grid_sum_slots<2, kCoreWaves, kCoreSums>(v, a.G, a.part, a.ctl, bar++, lds, &flag)) {
             done = 4;
             break;
         }
@@ -749,7 +756,7 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
             break;
         }
         double d[1] = {phase_w()};
-        if (!grid_sum<1, kCoreWaves, kCoreSums>(d, a.G, a.part, a.ctl, bar++, lds, &flag)) {
+        if (!grid_sum_slots<1, kCoreWaves, kCoreSums>(d, a.G, a.part, a.ctl, bar++, lds, &flag)) {
             done = 4;
             break;
         }
@@ -799,11 +806,11 @@ void core_free(cicy_core *z) {
                  z->ell_col};
     for (int *q : ip)
         if (q) (void)hipFree(q);
-    double *dp[] = {z->ell_val, z->diag, z->blk_c, z->x, z->r, z->u, z->w, z->p, z->s,
-                    z->part};
+    double *dp[] = {z->ell_val, z->diag, z->blk_c, z->x, z->r, z->u, z->w, z->p, z->s};
     for (double *q : dp)
         if (q) (void)hipFree(q);
     if (z->binv) (void)hipFree(z->binv);
+    if (z->part) (void)hipFree(z->part);
     if (z->blk_off) (void)hipFree(z->blk_off);
     if (z->sl_off) (void)hipFree(z->sl_off);
     if (z->ctl) (void)hipFree(z->ctl);
@@ -838,6 +845,8 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
         set_error(ctx, "core CG kernel cannot be resident");
         return nullptr;
     }
+    // the grid sums read at most kSlotLanes slots per lane of one wave
+    cus = std::min(cus, kSlotLanes * kWave);
     CoreHost H;
     const char *why = nullptr;
     if (!build_core_host(m, crp, ccol, cval, cdiag, orig, nblocks, ranges, cus, H, &why)) {
@@ -862,7 +871,9 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
               zeros(ctx, z, &z->x, m) &&
               zeros(ctx, z, &z->r, m) && zeros(ctx, z, &z->u, m) && zeros(ctx, z, &z->w, m) &&
               zeros(ctx, z, &z->p, m) && zeros(ctx, z, &z->s, m) &&
-              zeros(ctx, z, &z->part, 2 * kCoreSums * (size_t)H.G) &&
+              hip_ok(ctx, hipMalloc((void **)&z->part, 3 * kCoreSums * sizeof(unsigned long long) *
+                                                       (size_t)H.G),
+                     "hipMalloc (core)") &&
               hip_ok(ctx, hipMalloc((void **)&z->ctl, sizeof(GridCtl)), "hipMalloc (core)") &&
               hip_ok(ctx, hipMalloc((void **)&z->st, sizeof(CoreState)), "hipMalloc (core)") &&
               hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
@@ -908,6 +919,9 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
         int chunk = kCoreChunk;
         void *args[] = {(void *)&a, (void *)&z->st, (void *)&chunk};
         ok = hip_ok(ctx, hipMemsetAsync(z->ctl, 0, sizeof(GridCtl), stream), "memset") &&
+             hip_ok(ctx, hipMemsetD32Async((hipDeviceptr_t)z->part, (int)kSlotEmptyWord,
+                                           3 * kCoreSums * 2 * (size_t)z->G, stream),
+                    "memset") &&
              hip_ok(ctx, hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_core_cg),
                                                     dim3(z->G), dim3(kCoreThreads), args, 0,
                                                     stream),

@@ -123,4 +123,103 @@ __device__ __forceinline__ bool grid_sum(double (&v)[NV], int G, double *part, G
     return ok;
 }
 
+// ---- grid sums without a counter --------------------------------------------------
+// The arrival is the value itself: slots hold 64-bit words, kSlotEmpty marks a slot not
+// yet written for the current barrier (a signalling-NaN pattern: arithmetic never yields
+// a signalling NaN, it quiets them).  Barrier m reads buffer m % 3 of part ([3][G][STRIDE]
+// words); on arriving there each work-group empties its slots of buffer (m + 1) % 3, which
+// every work-group has finished reading (they were written at barrier m - 2, and this
+// work-group has passed barrier m - 1).  The host fills part with kSlotEmpty before a
+// launch (hipMemsetD32Async with kSlotEmptyWord).
+constexpr unsigned kSlotEmptyWord = 0x7ff4deadu;
+constexpr unsigned long long kSlotEmpty = 0x7ff4dead7ff4deadull;
+constexpr int kSlotLanes = 4;  // slots read per lane: at most kSlotLanes * 64 work-groups
+
+__device__ __forceinline__ void st_coh_u64(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_coh_u64(const unsigned long long *p) {
+    return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid-wide sum of NV (<= STRIDE) values and barrier number m of this launch, over G <=
+// kSlotLanes * 64 work-groups of WAVES waves, as grid_sum (same guarantees: every store a
+// thread issued before the call is at the point of coherence before its work-group
+// arrives; totals summed in a fixed order, bit-identical in every work-group).  Thread 0
+// stores the work-group's sums (its arrival); lane l of wave 0 reads the slots of
+// work-groups l, l + 64, ... all at once and again only those still empty (bounded spin).
+template <int NV, int WAVES, int STRIDE>
+__device__ __forceinline__ bool grid_sum_slots(double (&v)[NV], int G, unsigned long long *part,
+                                               GridCtl *ctl, unsigned m, double *lds,
+                                               int *flag) {
+    static_assert(NV <= STRIDE, "grid_sum_slots: more values than slot entries");
+    const size_t gs = (size_t)G * STRIDE;
+    if (threadIdx.x < (unsigned)NV)
+        st_coh_u64(&part[((m + 1u) % 3u) * gs + (size_t)blockIdx.x * STRIDE + threadIdx.x],
+                   kSlotEmpty);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    wg_sum<NV, WAVES>(v, lds);
+    if (threadIdx.x < kWave) {
+        unsigned long long *slot = part + (m % 3u) * gs;
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; i++)
+                st_coh_u64(&slot[(size_t)blockIdx.x * STRIDE + i],
+                           (unsigned long long)__double_as_longlong(v[i]));
+        }
+        unsigned long long w[kSlotLanes][NV];
+#pragma unroll
+        for (int j = 0; j < kSlotLanes; j++) {
+            const int k = (int)threadIdx.x + j * kWave;
+#pragma unroll
+            for (int i = 0; i < NV; i++)
+                w[j][i] = k < G ? ld_coh_u64(&slot[(size_t)k * STRIDE + i]) : 0ull;
+        }
+        int err = 0;
+        unsigned spins = 0;
+        while (true) {
+            bool pending = false;
+#pragma unroll
+            for (int j = 0; j < kSlotLanes; j++)
+#pragma unroll
+                for (int i = 0; i < NV; i++) pending = pending || w[j][i] == kSlotEmpty;
+            if (!pending) break;
+            if (++spins > kGridSpinLimit || ld_coh_i(&ctl->error)) {
+                err = 1;
+                __hip_atomic_store(&ctl->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int j = 0; j < kSlotLanes; j++) {
+                const int k = (int)threadIdx.x + j * kWave;
+#pragma unroll
+                for (int i = 0; i < NV; i++)
+                    if (w[j][i] == kSlotEmpty) w[j][i] = ld_coh_u64(&slot[(size_t)k * STRIDE + i]);
+            }
+        }
+        err = __any(err) ? 1 : 0;
+        double sum[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            sum[i] = 0.0;
+#pragma unroll
+            for (int j = 0; j < kSlotLanes; j++) sum[i] += __longlong_as_double((long long)w[j][i]);
+            sum[i] = wave_sum(sum[i]);
+        }
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; i++) lds[i] = sum[i];
+            *flag = err;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; i++) v[i] = lds[i];
+    const bool ok = *flag == 0;
+    __syncthreads();  // lds is reused by the next reduction
+    return ok;
+}
+
 }  // namespace cicy
