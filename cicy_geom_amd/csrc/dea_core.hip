@@ -74,6 +74,7 @@ constexpr int kCoreUnroll = 12;        // block-inverse columns loaded before us
 constexpr int kCoreEnt = 8;            // row slots loaded before their gathers
 constexpr int kCoreSums = 4;           // entries per work-group slot of the grid sums
 constexpr int kCorePasses = 10;        // corrections from the true residual (stalled solves)
+constexpr int kCoreLdsCols = 24 * 1024; // column indices of the product a work-group keeps in LDS
 // Phase-u cost per work-group on MI355X, measured (us): kCostRow per row update (p, s, x,
 // r, u), kCostEntry per block-inverse entry, kCostRound per round of kCorePairs blocks.
 constexpr double kCostRow = 2.0e-3, kCostEntry = 6.3e-5, kCostRound = 3.2;
@@ -104,6 +105,7 @@ struct CoreDev {
     // in the row's column order
     const int *wsl;
     const long long *sl_off;
+    int lds_cols;         // column indices staged in LDS per work-group (its first slots)
     const int *sl_len;
     const int *ell_col;
     const double *ell_val;
@@ -121,7 +123,7 @@ struct CoreDev {
 }  // namespace cicy
 
 struct cicy_core {
-    int m = 0, G = 0, nb = 0, blocked_rows = 0;
+    int m = 0, G = 0, nb = 0, blocked_rows = 0, lds_cols = 0;
     long long nnz2 = 0, inv_entries = 0;
     int *ju = nullptr, *bu = nullptr, *wr = nullptr, *perm = nullptr;
     int *blk_row0 = nullptr, *blk_n = nullptr;
@@ -566,12 +568,24 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
     __shared__ double zp[kCorePairs][2][kCoreMaxBlock];   // its two half products
     __shared__ double lds[kCoreSums * kCoreWaves];
     __shared__ int flag;
+    extern __shared__ int scols[];  // [lds_cols] this work-group's first product columns
     const int g = blockIdx.x, tid = threadIdx.x;
     const int wave = tid / kWave, lane = tid & (kWave - 1);
     const int j0 = a.ju[g], j1 = a.ju[g + 1];     // phase u: Jacobi rows
     const int b0 = a.bu[g], b1 = a.bu[g + 1];     // phase u: blocks
     const int w0 = a.wr[g], w1 = a.wr[g + 1];     // phase w: rows
     const int s0 = a.wsl[g], s1 = a.wsl[g + 1];
+    // the column indices of the product are the same in every iteration: the first
+    // lds_cols of this work-group's slots are read from LDS, so the gathers of u wait for an
+    // LDS read instead of a load from HBM
+    const long long cbase = s1 > s0 ? a.sl_off[s0] : 0;
+    const int ccount = s1 > s0 ? (int)std::min<long long>(a.sl_off[s1 - 1] +
+                                                              (long long)a.sl_len[s1 - 1] * kWave -
+                                                              cbase,
+                                                          a.lds_cols)
+                               : 0;
+    for (int i = tid; i < ccount; i += kCoreThreads) scols[i] = a.ell_col[cbase + i];
+    __syncthreads();
     long long k = st->k;
     const long long max_iter = st->max_iter;
     double gamma = st->gamma, alpha = st->alpha, beta = st->beta, rr = st->rr;
@@ -583,8 +597,10 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
     // (A v)_row for this lane's row of slice sl, v gathered from src (coherent); own = v_row
     auto row_product = [&](int sl, int row, bool live, const double *src, double &own) {
         const int len = a.sl_len[sl];
-        const int *cc = a.ell_col + a.sl_off[sl] + lane;
-        const double *vv = a.ell_val + a.sl_off[sl] + lane;
+        const long long so = a.sl_off[sl];
+        const bool staged = so - cbase + (long long)len * kWave <= ccount;
+        const int *cc = staged ? scols + (so - cbase) + lane : a.ell_col + so + lane;
+        const double *vv = a.ell_val + so + lane;
         own = live ? ld_coh(&src[row]) : 0.0;
         double acc = live ? a.diag[row] * own : 0.0;
         for (int t = 0; t < len; t += kCoreEnt) {
@@ -853,9 +869,26 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
         set_error(ctx, "core blocks: %s", why);
         return nullptr;
     }
+    long long most = 0;  // slots of the work-group with the most
+    for (int g = 0; g < H.G; g++)
+        if (H.wsl[g + 1] > H.wsl[g])
+            most = std::max(most, H.sl_off[H.wsl[g + 1] - 1] +
+                                      (long long)H.sl_len[H.wsl[g + 1] - 1] * kWave -
+                                      H.sl_off[H.wsl[g]]);
+    const int lds_cols = (int)std::min<long long>(most, kCoreLdsCols);
+    if (!hip_ok(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &per_cu, reinterpret_cast<const void *>(k_core_cg), kCoreThreads,
+                         (size_t)lds_cols * sizeof(int)),
+                "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+        return nullptr;
+    if (per_cu < 1) {
+        set_error(ctx, "core CG kernel cannot be resident with %d staged columns", lds_cols);
+        return nullptr;
+    }
     cicy_core *z = new cicy_core();
     z->m = m;
     z->G = H.G;
+    z->lds_cols = lds_cols;
     z->nb = (int)H.blk_row0.size();
     z->nnz2 = (long long)H.col.size();
     z->inv_entries = (long long)H.binv.size();
@@ -908,7 +941,7 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
               hip_ok(ctx, hipMemsetAsync(z->p, 0, vb, stream), "memset") &&
               hip_ok(ctx, hipMemcpyAsync(z->st, &h, sizeof(h), hipMemcpyHostToDevice, stream),
                      "H2D");
-    CoreDev a{z->m, z->G, z->ju, z->bu, z->wr, z->wsl, z->sl_off, z->sl_len, z->ell_col,
+    CoreDev a{z->m, z->G, z->ju, z->bu, z->wr, z->wsl, z->sl_off, z->lds_cols, z->sl_len, z->ell_col,
               z->ell_val, z->diag, z->blk_row0, z->blk_n, z->blk_off, z->blk_c, z->binv,
               z->x, z->r, z->u, z->w, z->p, z->s, z->ctl, z->part};
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -923,8 +956,8 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
                                            3 * kCoreSums * 2 * (size_t)z->G, stream),
                     "memset") &&
              hip_ok(ctx, hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_core_cg),
-                                                    dim3(z->G), dim3(kCoreThreads), args, 0,
-                                                    stream),
+                                                    dim3(z->G), dim3(kCoreThreads), args,
+                                                    (size_t)z->lds_cols * sizeof(int), stream),
                     "hipLaunchCooperativeKernel (core CG)") &&
              d2h_sync(ctx, &h, z->st, sizeof(h));
         if (ok && !h.done && max_seconds > 0 &&
