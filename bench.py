@@ -295,10 +295,12 @@ def main():
     if kind == dea.METHOD_CONDENSED_BLOCK:
         # dominant kernel: the persistent block-preconditioned core CG (k_core_cg), one
         # cooperative launch per solve; its duration is the HIP-event time of its launches
-        # inside the timed solves.  Algorithmic bytes per core iteration: the off-diagonal
-        # entries of both triangles (column + value, 12 B), the diagonal and row kind (12 B
-        # per row), the FP32 dense block remainders (4 B per entry) and the vectors (x, r, u,
-        # w, p, s: 13 reads/writes of 8 B per row).
+        # inside the timed solves.  Algorithmic bytes per core iteration, counted as for a
+        # matrix stored with 4-byte indices: the off-diagonal entries of both triangles
+        # (column + value, 12 B), the diagonal with its row index (12 B per row), the FP32
+        # dense block remainders (4 B per entry) and the vectors (x, r, u, w, p, s: 13
+        # reads/writes of 8 B per row).  The traffic per iteration comes from the committed
+        # counter runs of the same workload, scaled to this launch's iterations.
         r_ = results[head]
         cn, cl = r_["core_rows"], r_["core_nnz"]
         nb2 = systems[head].size("core_block_entries")
@@ -316,7 +318,7 @@ def main():
         if args.config == 3 and os.path.exists(tpath):
             with open(tpath) as f:
                 t = json.load(f)
-            roof["traffic"] = t["bytes_per_launch"]
+            roof["traffic"] = t["bytes_per_iter"] * r_["iters"]
             roof["traffic_per_iter"] = t["bytes_per_iter"]
             roof["traffic_source"] = t["source"]
         log(f"[bench] k_core_cg: {roof['us_per_iter']:.1f} us per core iteration, "
