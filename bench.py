@@ -259,8 +259,11 @@ def main():
         if learned:
             graph.close()
 
-    # SpMV roofline (HIP events on the library's stream, back-to-back launches)
+    # SpMV roofline (HIP events on the library's stream, back-to-back launches), next to a
+    # plain 1 GiB device copy (read + write) as the achievable streaming rate
     n, L = s.n, s.nnz
+    copy_gbs = 2 * (1 << 30) / (ctx.copy_bench(1 << 30, 5) * 1e-3) / 1e9
+    log(f"[bench] copy 1 GiB: {copy_gbs:.0f} GB/s")
     spmv = {}
     fp64_sys = next((v for k, v in by_kind.items() if k != "mixed"), None)
     for prec, sv, sysx in ((64, 8, fp64_sys), (32, 4, by_kind.get("mixed"))):
@@ -364,7 +367,8 @@ def main():
         "roofline_spmv_full_f64": {"achieved": spmv[64]["gbs"],
                                    "frac": spmv[64]["gbs"] / HBM_PEAK_GBS,
                                    "us": spmv[64]["ms"] * 1e3,
-                                   "algorithmic_bytes": spmv[64]["bytes"]},
+                                   "algorithmic_bytes": spmv[64]["bytes"],
+                                   "copy_1GiB_gbs": copy_gbs},
         "pcg_iteration_full": {"algorithmic_bytes": it_bytes, "us": it_us,
                                "gbs": it_bytes / (it_us * 1e-6) / 1e9 if it_us > 0 else None},
         "upload_s": upload_s,

@@ -37,6 +37,7 @@ struct BandHost {
     int m = 0;
     std::vector<double> band, xval;
     std::vector<int> xblk, xrow, xnbr;
+    std::vector<int> order;  // launch order of the row blocks (BandDev::order)
 };
 
 // Host copy of an uploaded system, kept until every layout built on demand from it (FP32
@@ -105,6 +106,7 @@ struct cicy_sys {
     int *xblk = nullptr, *xrow = nullptr, *xnbr = nullptr;
     double *band = nullptr, *xval = nullptr;
     float *band32 = nullptr, *xval32 = nullptr;
+    int *xorder = nullptr;  // launch order of the row blocks
     // FP64 vectors
     double *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr;
     double *x0 = nullptr, *r0 = nullptr, *b = nullptr;

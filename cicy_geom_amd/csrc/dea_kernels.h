@@ -89,6 +89,8 @@ struct BandDev {
     const int *__restrict__ xrow; // [m]
     const int *__restrict__ xnbr; // [m]
     const T *__restrict__ xval;   // [m]
+    // [nblocks] row block of each work-group (heaviest blocks first); nullptr: row order
+    const int *__restrict__ order = nullptr;
 };
 
 struct BatchDev {
@@ -327,7 +329,14 @@ __global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev 
                                                          PcgState *__restrict__ st_base,
                                                          RedBuf rbuf,
                                                          const MixState *__restrict__ guard) {
-    const Blk k = block_info<RB, BATCH>(bd, s.n);
+    Blk k = block_info<RB, BATCH>(bd, s.n);
+    if (!BATCH && s.order) {
+        // blocks with many off-band entries (plate rows) go first, so that they do not
+        // run last on a part of the GPU while the rest of it idles
+        k.b = s.order[blockIdx.x];
+        k.r0 = k.b * RB;
+        k.nrows = min(RB, s.n - k.r0);
+    }
     PcgState *st = st_base ? st_base + k.sys : nullptr;
     if (PCG && st->done) return;
     if (guard && !guard[k.sys].need_update) return;
@@ -354,20 +363,48 @@ __global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev 
         if (PCG && t >= 1 && t <= nrows) pnew[i] = p;
     }
     for (int t = threadIdx.x; t < nrows + 1; t += kThreads) sb[t] = s.band[r0 + t];
-    __syncthreads();
-    for (int t = threadIdx.x; t < nrows; t += kThreads) {
-        const T pr = sp[t + 1];
-        T y = UNIT ? pr : s.diag[r0 + t] * pr;
-        y = fma(sb[t], sp[t], y);
-        y = fma(sb[t + 1], sp[t + 2], y);
-        sy[t] = y;
+    // the diagonal of this thread's rows, loaded together with the staging above
+    constexpr int kRows = (RB + kThreads - 1) / kThreads;
+    T dg[kRows];
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const int t = (int)threadIdx.x + j * kThreads;
+        dg[j] = (!UNIT && t < nrows) ? s.diag[r0 + t] : (T)1;
     }
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const int t = (int)threadIdx.x + j * kThreads;
+        if (t < nrows) {
+            const T pr = sp[t + 1];
+            T y = UNIT ? pr : dg[j] * pr;
+            y = fma(sb[t], sp[t], y);
+            y = fma(sb[t + 1], sp[t + 2], y);
+            sy[t] = y;
+        }
+    }
+    __syncthreads();
+    // remainder half-entries: four per thread in flight together (indices and values, then
+    // the gathered p, then the LDS adds), so that a block with many of them (plate rows)
+    // waits for a quarter of the dependent load chains
     const int e1 = s.xblk[k.b + 1];
-    for (int e = s.xblk[k.b] + threadIdx.x; e < e1; e += kThreads) {
-        const int c = s.xnbr[e];
-        const T pc = (c >= r0 && c < r0 + nrows) ? sp[c - r0 + 1] : pval(c);
-        atomicAdd(&sy[s.xrow[e] - r0], s.xval[e] * pc);
+    for (int e0 = s.xblk[k.b] + threadIdx.x; e0 < e1; e0 += 4 * kThreads) {
+        int c[4], rw[4];
+        T xv[4], pc[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = e0 + u * kThreads;
+            const bool in = e < e1;
+            c[u] = in ? s.xnbr[e] : r0;
+            rw[u] = in ? s.xrow[e] - r0 : 0;
+            xv[u] = in ? s.xval[e] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            pc[u] = (c[u] >= r0 && c[u] < r0 + nrows) ? sp[c[u] - r0 + 1] : pval(c[u]);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (e0 + u * kThreads < e1) atomicAdd(&sy[rw[u]], xv[u] * pc[u]);
     }
     __syncthreads();
     double part = 0.0;

@@ -70,9 +70,17 @@ __global__ __launch_bounds__(kThreads) void k_sub(int n, double *__restrict__ r,
 // and the achievable-bandwidth yardstick.
 __global__ __launch_bounds__(kThreads) void k_copy(size_t n16, const uint4 *__restrict__ src,
                                                    uint4 *__restrict__ dst) {
-    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n16;
-         i += (size_t)gridDim.x * kThreads)
-        dst[i] = src[i];
+    // four 16-byte loads per thread in flight before their stores
+    const size_t stride = (size_t)gridDim.x * kThreads;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n16; i += 4 * stride) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (i + u * stride < n16) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (i + u * stride < n16) dst[i + u * stride] = v[u];
+    }
 }
 
 // FP64 boundary current: one work-group, fixed strided order, tree reduction.
@@ -227,6 +235,24 @@ void build_band(const BlockedHost &h, BandHost &b) {
         }
     b.xblk.assign((size_t)h.nblocks + 1, 0);
     for (int k = 0; k <= h.nblocks; k++) b.xblk[k] = cnt[std::min(h.n, k * h.rb)];
+    // launch order: blocks with more than four times the mean number of half-entries first
+    // (heaviest first), then the others in row order
+    b.order.resize((size_t)h.nblocks);
+    const double mean = h.nblocks > 0 ? (double)b.m / h.nblocks : 0.0;
+    std::vector<std::pair<int, int>> heavy;
+    int at = 0;
+    for (int k = 0; k < h.nblocks; k++) {
+        const int c = b.xblk[k + 1] - b.xblk[k];
+        if (c > 4.0 * mean && c > 0) heavy.push_back({-c, k});
+    }
+    std::sort(heavy.begin(), heavy.end());
+    std::vector<char> first((size_t)h.nblocks, 0);
+    for (auto &e : heavy) {
+        b.order[at++] = e.second;
+        first[e.second] = 1;
+    }
+    for (int k = 0; k < h.nblocks; k++)
+        if (!first[k]) b.order[at++] = k;
 }
 
 // Jacobi-scaled FP32 values of the band layout (a * s_hi * s_lo, s = 1/sqrt(d))
@@ -254,7 +280,7 @@ DevMat<double> mat64(const cicy_sys *s) {
     m.sss = SssDev<double>{s->n, s->nblocks, s->rp, s->col, s->val, s->diag, s->urp, s->ucol,
                            s->uval};
     m.band = BandDev<double>{s->n, s->nblocks, s->band, s->diag, s->xblk, s->xrow, s->xnbr,
-                             s->xval};
+                             s->xval, s->xorder};
     return m;
 }
 
@@ -264,7 +290,7 @@ DevMat<float> mat32(const cicy_sys *s) {
     m.sss = SssDev<float>{s->n, s->nblocks, s->rp, s->col, s->val32, nullptr, s->urp, s->ucol,
                           s->uval32};
     m.band = BandDev<float>{s->n, s->nblocks, s->band32, nullptr, s->xblk, s->xrow, s->xnbr,
-                            s->xval32};
+                            s->xval32, s->xorder};
     return m;
 }
 
@@ -645,6 +671,8 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
         s->nx = bh.m;
         ok = ok && dmalloc(ctx, &s->band, nv + 1, &tot) &&
              dmalloc(ctx, &s->xblk, (size_t)h.nblocks + 1, &tot) &&
+             dmalloc(ctx, &s->xorder, std::max<size_t>((size_t)h.nblocks, 1), &tot) &&
+             h2d(ctx, s->xorder, bh.order.data(), (size_t)h.nblocks) &&
              dmalloc(ctx, &s->xrow, (size_t)bh.m, &tot) && dmalloc(ctx, &s->xnbr, (size_t)bh.m, &tot) &&
              dmalloc(ctx, &s->xval, (size_t)bh.m, &tot) &&
              h2d(ctx, s->band, bh.band.data(), nv + 1) &&
@@ -773,7 +801,7 @@ void cicy_sys_destroy(cicy_sys *s) {
     cicy::condensed_free(s->cond);
     delete s->host;
     void *ptrs[] = {s->rp, s->col, s->val, s->diag, s->urp, s->ucol, s->uval, s->band,
-                    s->xblk, s->xrow, s->xnbr, s->xval, s->x, s->r, s->p0, s->p1, s->q,
+                    s->xblk, s->xrow, s->xnbr, s->xval, s->xorder, s->x, s->r, s->p0, s->p1, s->q,
                     s->x0, s->r0, s->b, s->partials, s->gpart, s->gcount, s->st, s->scal,
                     s->counter2};
     for (void *p : ptrs)
