@@ -157,6 +157,8 @@ def main():
         reps = 20
         copy_ms = ctx.copy_bench(1 << 30, reps)
         log(f"[bench] copy 1 GiB: {copy_ms:.3f} ms -> {2 * (1 << 30) / copy_ms / 1e6:.0f} GB/s")
+        if args.rows_per_block:
+            ctx.set_rows_per_block(args.rows_per_block)
         sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
         sysd.set_rhs(s.b, None, s.b)
         for prec in (64, 32):

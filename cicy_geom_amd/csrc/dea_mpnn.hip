@@ -371,23 +371,68 @@ struct cicy_mpnn {
     float *theta = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
     __hip_bfloat16 *thb = nullptr;
     long long step = 0;
+    // activation workspace shared by every graph (one graph is in flight at a time): grown
+    // to the largest graph seen; ws_owner is the graph whose training forward it holds
+    float *ws = nullptr;
+    size_t ws_floats = 0;
+    long long ws_owner = -1;
 };
 
 struct cicy_graph {
     cicy_ctx *ctx = nullptr;
+    long long id = 0;  // distinct for every graph created in the process
     int n = 0, m = 0;
     bool training = false;
     double v_app = 1.0;
     int *nptr = nullptr, *nbr = nullptr;
     float *w = nullptr, *wrev = nullptr, *e = nullptr, *feat = nullptr, *u = nullptr;
     float *target = nullptr;
-    // activations: h[0..3], agg[0..2], z0, z[0..2], gradient scratch dh, dh2, dagg
-    float *h[LAYERS + 1] = {nullptr}, *agg[LAYERS] = {nullptr}, *z0 = nullptr,
-          *z[LAYERS] = {nullptr};
-    float *dh = nullptr, *dh2 = nullptr, *dagg = nullptr;
     double *vout = nullptr, *loss_part = nullptr;
     bool have_target = false;
 };
+
+namespace {
+
+// Activations of one graph in the model's workspace: h[0..3], agg[0..2], z0, z[0..2] and the
+// gradient scratch dh, dh2, dagg, each n x H.  Inference uses h[0], h[1] (ping-pong),
+// agg[0] and dh only, the first four slots; a training step uses all fourteen.
+struct Act {
+    float *h[LAYERS + 1], *agg[LAYERS], *z0, *z[LAYERS], *dh, *dh2, *dagg;
+};
+constexpr int kActInfer = 4, kActTrain = 14;
+
+bool bind_act(cicy_mpnn *M, const cicy_graph *G, bool train, Act &a) {
+    const size_t NH = (size_t)std::max(G->n, 1) * H;
+    const size_t need = (train ? kActTrain : kActInfer) * NH;
+    if (need > M->ws_floats) {
+        if (M->ws) {
+            if (!hip_ok(M->ctx, hipStreamSynchronize(M->ctx->stream), "sync")) return false;
+            (void)hipFree(M->ws);
+            M->ws = nullptr;
+            M->ws_floats = 0;
+            M->ws_owner = -1;
+        }
+        if (!hip_ok(M->ctx, hipMalloc((void **)&M->ws, need * sizeof(float)), "hipMalloc"))
+            return false;
+        M->ws_floats = need;
+    }
+    float *w = M->ws;
+    a.h[0] = w;
+    a.h[1] = w + NH;
+    a.agg[0] = w + 2 * NH;
+    a.dh = w + 3 * NH;
+    a.h[2] = w + 4 * NH;
+    a.h[3] = w + 5 * NH;
+    a.agg[1] = w + 6 * NH;
+    a.agg[2] = w + 7 * NH;
+    a.z0 = w + 8 * NH;
+    for (int l = 0; l < LAYERS; l++) a.z[l] = w + (9 + l) * NH;
+    a.dh2 = w + 12 * NH;
+    a.dagg = w + 13 * NH;
+    return true;
+}
+
+}  // namespace
 
 namespace {
 
@@ -405,23 +450,26 @@ bool gh2d(cicy_ctx *ctx, T *dst, const T *src, size_t count) {
 
 // forward; v_dev (may be null) receives V^ in FP64
 bool forward(cicy_mpnn *M, cicy_graph *G, double *v_dev, bool train) {
+    Act a;
+    if (!bind_act(M, G, train, a)) return false;
+    M->ws_owner = train ? G->id : -1;
     hipStream_t st = M->ctx->stream;
     const int gd = grid_for(G->n);
     hipLaunchKernelGGL(k_encode, dim3(gd), dim3(kThreads), 0, st, G->n, G->feat, M->thb,
-                       train ? G->z0 : nullptr, G->h[0]);
+                       train ? a.z0 : nullptr, a.h[0]);
     for (int l = 0; l < LAYERS; l++) {
         // inference reuses two h buffers (ping-pong) and one agg buffer
-        float *hin = train ? G->h[l] : G->h[l & 1];
-        float *hout = train ? G->h[l + 1] : G->h[(l + 1) & 1];
-        float *ag = train ? G->agg[l] : G->agg[0];
+        float *hin = train ? a.h[l] : a.h[l & 1];
+        float *hout = train ? a.h[l + 1] : a.h[(l + 1) & 1];
+        float *ag = train ? a.agg[l] : a.agg[0];
         hipLaunchKernelGGL(k_aggregate, dim3(gd), dim3(kThreads), 0, st, G->n, G->nptr, G->nbr,
                            G->w, G->e, M->thb, l, hin, ag);
         hipLaunchKernelGGL(k_layer, dim3(gd), dim3(kThreads), 0, st, G->n, M->thb, l, hin, ag,
-                           train ? G->z[l] : nullptr, hout);
+                           train ? a.z[l] : nullptr, hout);
     }
-    float *hl = train ? G->h[LAYERS] : G->h[LAYERS & 1];
+    float *hl = train ? a.h[LAYERS] : a.h[LAYERS & 1];
     hipLaunchKernelGGL(k_readout, dim3(gd), dim3(kThreads), 0, st, G->n, M->thb, hl, G->u,
-                       G->v_app, v_dev, train ? G->target : nullptr, G->dh, G->loss_part,
+                       G->v_app, v_dev, train ? G->target : nullptr, a.dh, G->loss_part,
                        M->grad);
     return hip_ok(M->ctx, hipGetLastError(), "mpnn forward launch");
 }
@@ -463,7 +511,7 @@ void cicy_mpnn_destroy(cicy_mpnn *M) {
     if (!M) return;
     (void)hipSetDevice(M->ctx->device);
     (void)hipStreamSynchronize(M->ctx->stream);
-    void *p[] = {M->theta, M->grad, M->m, M->v, M->thb};
+    void *p[] = {M->theta, M->grad, M->m, M->v, M->thb, M->ws};
     for (void *q : p)
         if (q) (void)hipFree(q);
     delete M;
@@ -582,20 +630,13 @@ int cicy_graph_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_
     G->m = m;
     G->training = training != 0;
     G->v_app = v_app;
-    const size_t NH = (size_t)n * H;
+    static long long next_id = 0;
+    G->id = ++next_id;
     bool ok = galloc(ctx, &G->nptr, (size_t)n + 1) && galloc(ctx, &G->nbr, (size_t)m) &&
               galloc(ctx, &G->w, (size_t)m) && galloc(ctx, &G->wrev, (size_t)m) &&
               galloc(ctx, &G->e, (size_t)m) && galloc(ctx, &G->feat, (size_t)n * F) &&
               galloc(ctx, &G->u, (size_t)n) && galloc(ctx, &G->target, (size_t)n) &&
               galloc(ctx, &G->vout, (size_t)n) && galloc(ctx, &G->loss_part, 4096);
-    const int nh = G->training ? LAYERS + 1 : 2, na = G->training ? LAYERS : 1;
-    for (int l = 0; ok && l < nh; l++) ok = galloc(ctx, &G->h[l], NH);
-    for (int l = 0; ok && l < na; l++) ok = galloc(ctx, &G->agg[l], NH);
-    if (G->training) {
-        ok = ok && galloc(ctx, &G->z0, NH) && galloc(ctx, &G->dh, NH) &&
-             galloc(ctx, &G->dh2, NH) && galloc(ctx, &G->dagg, NH);
-        for (int l = 0; ok && l < LAYERS; l++) ok = galloc(ctx, &G->z[l], NH);
-    }
     ok = ok && gh2d(ctx, G->nptr, deg.data(), (size_t)n + 1) && gh2d(ctx, G->nbr, nbr.data(), (size_t)m) &&
          gh2d(ctx, G->w, w.data(), (size_t)m) && gh2d(ctx, G->wrev, wrev.data(), (size_t)m) &&
          gh2d(ctx, G->e, e.data(), (size_t)m) && gh2d(ctx, G->feat, feat.data(), (size_t)n * F) &&
@@ -614,15 +655,9 @@ void cicy_graph_destroy(cicy_graph *G) {
     (void)hipSetDevice(G->ctx->device);
     (void)hipStreamSynchronize(G->ctx->stream);
     void *p[] = {G->nptr, G->nbr, G->w, G->wrev, G->e, G->feat, G->u, G->target, G->vout,
-                 G->loss_part, G->z0, G->dh, G->dh2, G->dagg};
+                 G->loss_part};
     for (void *q : p)
         if (q) (void)hipFree(q);
-    for (int l = 0; l <= LAYERS; l++)
-        if (G->h[l]) (void)hipFree(G->h[l]);
-    for (int l = 0; l < LAYERS; l++) {
-        if (G->agg[l]) (void)hipFree(G->agg[l]);
-        if (G->z[l]) (void)hipFree(G->z[l]);
-    }
     delete G;
 }
 
@@ -670,18 +705,24 @@ int cicy_mpnn_backward(cicy_mpnn *M, cicy_graph *G) {
     if (!M || !G || !G->training || !G->have_target) return 0;
     cicy_ctx *ctx = M->ctx;
     (void)hipSetDevice(ctx->device);
+    if (M->ws_owner != G->id) {
+        set_error(ctx, "cicy_mpnn_backward: the last training forward was not on this graph");
+        return 0;
+    }
+    Act a;
+    if (!bind_act(M, G, true, a)) return 0;  // already large enough: nothing reallocated
     hipStream_t st = ctx->stream;
     const int gd = grid_for(G->n);
-    // G->dh holds dL/dh_L from the readout of the last forward
-    float *dh = G->dh, *dh_in = G->dh2;
+    // a.dh holds dL/dh_L from the readout of the last forward
+    float *dh = a.dh, *dh_in = a.dh2;
     for (int l = LAYERS - 1; l >= 0; l--) {
         hipLaunchKernelGGL(k_layer_bwd, dim3(gd), dim3(kThreads), 0, st, G->n, M->thb, l, dh,
-                           G->z[l], G->h[l], G->agg[l], dh_in, G->dagg, M->grad);
+                           a.z[l], a.h[l], a.agg[l], dh_in, a.dagg, M->grad);
         hipLaunchKernelGGL(k_aggregate_bwd, dim3(gd), dim3(kThreads), 0, st, G->n, G->nptr,
-                           G->nbr, G->wrev, G->e, M->thb, l, G->dagg, G->h[l], dh_in, M->grad);
+                           G->nbr, G->wrev, G->e, M->thb, l, a.dagg, a.h[l], dh_in, M->grad);
         std::swap(dh, dh_in);
     }
-    hipLaunchKernelGGL(k_encode_bwd, dim3(gd), dim3(kThreads), 0, st, G->n, G->feat, G->z0, dh,
+    hipLaunchKernelGGL(k_encode_bwd, dim3(gd), dim3(kThreads), 0, st, G->n, G->feat, a.z0, dh,
                        M->grad);
     return hip_ok(ctx, hipGetLastError(), "mpnn backward launch") &&
                    hip_ok(ctx, hipStreamSynchronize(st), "sync")

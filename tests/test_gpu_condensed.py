@@ -208,3 +208,30 @@ def test_dropin_call_with_condensed_method(ctx, systems):
     # rounding of the two ways of forming it
     assert np.linalg.norm(R - r_true) <= 0.05 * np.linalg.norm(r_true) + 1e-12 * np.linalg.norm(s.b)
     assert np.array_equal(volt[:s.reserved], s.vef) and np.array_equal(volt[s.reserved:], x)
+
+
+def test_prepare_builds_layouts_on_demand(ctx, systems):
+    """cicy_sys_create uploads only the two-kernel layout; cicy_sys_prepare builds a
+    method's layouts once (idempotent), and a solve prepares by itself."""
+    s = systems["rve6k"]
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    try:
+        assert sysd.size("condensed_rows") == 0 and sysd.size("core_blocks") == 0
+        bytes0 = sysd.size("bytes")
+        sysd.prepare(dea.METHOD_CONDENSED_BLOCK)
+        m, core = sysd.size("condensed_rows"), sysd.size("core_rows")
+        assert 0 < core <= m < s.n // 2
+        assert sysd.size("core_groups") > 0
+        sysd.prepare(dea.METHOD_CONDENSED_BLOCK)  # nothing rebuilt
+        assert (sysd.size("condensed_rows"), sysd.size("core_rows")) == (m, core)
+        sysd.set_rhs(s.b, None, s.b)
+        st = sysd.solve(dea.METHOD_CONDENSED_BLOCK, 1e-10, 0, dea.REL_B)
+        assert st.status == dea.CICY_OK and st.true_rel_res <= 1e-10 * (1 + 1e-6)
+        assert sysd.size("bytes") == bytes0  # no FP32 copy yet
+        st = sysd.solve(dea.METHOD_MIXED, 1e-6, 2000, dea.REL_B)  # built by this solve
+        assert st.status in (dea.CICY_OK, dea.CICY_MAXITER)
+        assert sysd.size("bytes") > bytes0
+        with pytest.raises(dea.DEAError):
+            sysd.prepare(99)
+    finally:
+        sysd.close()
