@@ -526,9 +526,324 @@ void prune_and_triangulate(cicy_net *net) {
     }
 }
 
+// ---- backbone network (Backbone_Network.cpp) ----------------------------------------
+
+// Length of tube polyline points P1 .. P2 times the cross-section
+// (CNT_volume_between_two_points, Backbone_Network.cpp:557-574)
+double cnt_volume(const cicy_net *net, int64_t p1, int64_t p2, double r) {
+    double len = 0.0;
+    for (int64_t i = p1; i < p2; i++) {
+        const double *a = &net->xyz[3 * (size_t)i], *b = &net->xyz[3 * (size_t)(i + 1)];
+        const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+        len = len + std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    return len * kPi * r * r;
+}
+
+// Volume of plate g inside the cube: l^2 t times the share of a 20 x 20 grid over its
+// mid-plane square that lies in [0, box]^3, when any does
+// (Generate_Network::Approximate_gnp_volume_inside_sample, Generate_Network.cpp:4750-4810)
+double gnp_volume(const cicy_net *net, int64_t g) {
+    const Plate &P = net->gnps[(size_t)g];
+    const double box = net->prm.box, h = 0.5 * P.l;
+    double c[4][3];
+    for (int k = 0; k < 3; k++) {
+        c[0][k] = P.c[k] - h * P.ax[0][k] - h * P.ax[1][k];  // corner
+        c[1][k] = P.c[k] + h * P.ax[0][k] - h * P.ax[1][k];  // right
+        c[2][k] = P.c[k] - h * P.ax[0][k] + h * P.ax[1][k];  // up
+        c[3][k] = P.c[k] + h * P.ax[0][k] + h * P.ax[1][k];  // up right
+    }
+    const int np = 20;
+    int in = 0;
+    for (int i = 0; i < np; i++) {
+        const double lr = (double)i / (np - 1);
+        for (int j = 0; j < np; j++) {
+            const double lu = (double)j / (np - 1);
+            bool inside = true;
+            for (int k = 0; k < 3; k++) {
+                const double st = c[0][k] + (c[1][k] - c[0][k]) * lr;
+                const double en = c[2][k] + (c[3][k] - c[2][k]) * lr;
+                const double q = st + (en - st) * lu;
+                if (q < 0.0 || q > box) inside = false;
+            }
+            if (inside) in++;
+        }
+    }
+    double v = P.l * P.l * P.t;
+    if (in != 0) v = v * ((double)in / (np * np));
+    return v;
+}
+
+void remove_point(std::vector<int64_t> &v, int64_t p) {
+    auto it = std::find(v.begin(), v.end(), p);
+    if (it != v.end()) v.erase(it);
+}
+
 }  // namespace
 
 extern "C" {
+
+int cicy_net_backbone(cicy_net *net, int64_t k, const double *volt, int64_t nvolt,
+                      const int64_t *lmm, int64_t n_points, const int64_t *lmm_gnp,
+                      int64_t n_gnp_points, int reserved, int avoid_resistance,
+                      const double *zero_in, const int32_t *seg_in, double *out, char *err,
+                      int64_t err_len) {
+    auto fail = [&](const char *msg) {
+        if (err && err_len > 0) std::snprintf(err, (size_t)err_len, "%s", msg);
+        return 0;
+    };
+    if (!net || !volt || !lmm || !out || k < 0 || k >= (int64_t)net->clusters.size())
+        return fail("invalid network, cluster or array");
+    if (n_points != (int64_t)net->xyz.size() / 3 ||
+        (lmm_gnp == nullptr ? 0 : n_gnp_points) != (int64_t)net->gxyz.size() / 3)
+        return fail("LMM arrays do not match the network's points");
+    auto node_c = [&](int64_t p) -> int64_t {
+        const int64_t n = lmm[p];
+        return (n >= 0 && n < nvolt) ? n : -1;
+    };
+    auto node_g = [&](int64_t p) -> int64_t {
+        const int64_t n = lmm_gnp[p];
+        return (n >= 0 && n < nvolt) ? n : -1;
+    };
+    bool bad = false;
+    auto I = [&](int64_t n1, int64_t n2) {
+        if (n1 < 0 || n2 < 0) {
+            bad = true;
+            return 0.0;
+        }
+        return std::fabs(volt[n1] - volt[n2]);
+    };
+    std::vector<int64_t> &cl = net->clusters[(size_t)k];
+    std::vector<int64_t> &cj = net->cluster_junctions[(size_t)k];
+    std::vector<int64_t> &cg = net->cluster_gnps[(size_t)k];
+    std::vector<int64_t> &cm = net->cluster_mixed[(size_t)k];
+    std::vector<int64_t> &cgj = net->cluster_gjunc[(size_t)k];
+    // the tubes' element points (HoKo->elements_cnt), edited below
+    const size_t T = net->radius.size();
+    std::vector<std::vector<int64_t>> el(T);
+    for (size_t t = 0; t < T; t++)
+        el[t].assign(net->elem_pts.begin() + net->elem_ptr[t],
+                     net->elem_pts.begin() + net->elem_ptr[t + 1]);
+    // Find_zero_current (:64-221): the largest |dV| over tube elements, tube junctions,
+    // plate triangulation edges, plate junctions and mixed junctions, times 1e-9
+    double imax = kZero;
+    for (int64_t t : cl)
+        for (size_t q = 1; q < el[(size_t)t].size(); q++) {
+            const double c = I(node_c(el[(size_t)t][q - 1]), node_c(el[(size_t)t][q]));
+            if (c > imax) imax = c;
+        }
+    for (int64_t j : cj) {
+        const Junction &J = net->junctions[(size_t)j];
+        const double c = I(node_c(J.p1), node_c(J.p2));
+        if (c > imax) imax = c;
+    }
+    for (int64_t g : cg) {
+        const std::vector<int64_t> &tri = net->gtri[(size_t)g];
+        for (size_t e = 0; e + 1 < tri.size(); e += 2) {
+            const double c = I(node_g(tri[e]), node_g(tri[e + 1]));
+            if (c > imax) imax = c;
+        }
+    }
+    for (int64_t j : cgj) {
+        const GnpJunction &J = net->gjunctions[(size_t)j];
+        const double c = I(node_g(J.p1), node_g(J.p2));
+        if (c > imax) imax = c;
+    }
+    for (int64_t j : cm) {
+        const MixedJunction &J = net->mjunctions[(size_t)j];
+        const double c = I(node_c(J.p_cnt), node_g(J.p_gnp));
+        if (c > imax) imax = c;
+    }
+    if (bad) return fail("a point of the cluster has no node in the given LMM");
+    const double zero = zero_in ? *zero_in : imax * 1e-9;
+    double vol_bb = 0.0, vol_dead = 0.0, gvol_bb = 0.0, gvol_dead = 0.0;
+    // Find_backbone_cnts (:317-409) with Calculate_cnt_volumes (:412-555): the backbone of
+    // a tube is the run of elements from the first carrying more than the zero current to
+    // the next one that does not; tubes are visited last to first
+    if (!cl.empty()) {
+        for (int64_t i = (int64_t)cl.size() - 1; i >= 0; i--) {
+            const int64_t t = cl[(size_t)i];
+            std::vector<int64_t> &e = el[(size_t)t];
+            int idx1 = -1, idx2 = -1;
+            if (seg_in) {
+                idx1 = seg_in[2 * i];
+                idx2 = seg_in[2 * i + 1];
+            } else {
+                for (size_t q = 0; q + 1 < e.size(); q++) {
+                    const double c = I(node_c(e[q]), node_c(e[q + 1]));
+                    if (c - zero > kZero) {
+                        if (idx1 == -1) idx1 = (int)q;
+                    } else if (idx1 != -1 && idx2 == -1) {
+                        idx2 = (int)q;
+                        break;
+                    }
+                }
+            }
+            if (e.empty()) return fail("a tube of the cluster has no element points");
+            if (idx1 >= (int)e.size() || idx2 >= (int)e.size() || (idx2 >= 0 && idx2 <= idx1))
+                return fail("backbone segment out of the tube's elements");
+            const double r = net->radius[(size_t)t];
+            const int64_t ps = e.front(), pe = e.back();
+            if (idx1 == -1) {
+                vol_dead = vol_dead + cnt_volume(net, ps, pe, r);
+                if (!avoid_resistance) e.clear();
+            } else {
+                const int64_t p1 = e[(size_t)idx1];
+                vol_dead = vol_dead + cnt_volume(net, ps, p1, r);
+                double vb;
+                if (idx2 == -1) {
+                    vb = cnt_volume(net, p1, pe, r);
+                } else {
+                    const int64_t p2 = e[(size_t)idx2];
+                    vb = cnt_volume(net, p1, p2, r);
+                    vol_dead = vol_dead + cnt_volume(net, p2, pe, r);
+                    if (!avoid_resistance) e.erase(e.begin() + idx2 + 1, e.end());
+                }
+                vol_bb = vol_bb + vb;
+                if (!avoid_resistance && idx1 > 0) e.erase(e.begin(), e.begin() + idx1);
+            }
+            if (!avoid_resistance && e.empty()) cl.erase(cl.begin() + i);
+        }
+    }
+    if (!cg.empty()) {
+        auto in_el = [&](int64_t t, int64_t p) {
+            const std::vector<int64_t> &e = el[(size_t)t];
+            return std::binary_search(e.begin(), e.end(), p);
+        };
+        // Find_and_remove_gnp_and_mixed_junctions_below_zero_current (:663-796)
+        for (int64_t i = (int64_t)cm.size() - 1; i >= 0; i--) {
+            const MixedJunction &J = net->mjunctions[(size_t)cm[(size_t)i]];
+            std::vector<int64_t> &e = el[(size_t)J.cnt];
+            const bool present = in_el(J.cnt, J.p_cnt);
+            const double c = I(node_c(J.p_cnt), node_g(J.p_gnp));
+            if (c - zero < kZero || e.empty() || !present) {
+                remove_point(net->gstruct[(size_t)J.gnp], J.p_gnp);
+                cm.erase(cm.begin() + i);
+                if (present) {
+                    e.erase(std::lower_bound(e.begin(), e.end(), J.p_cnt));
+                    if (e.size() < 2) e.clear();
+                }
+            }
+        }
+        for (int64_t i = (int64_t)cgj.size() - 1; i >= 0; i--) {
+            const GnpJunction &J = net->gjunctions[(size_t)cgj[(size_t)i]];
+            const double c = I(node_g(J.p1), node_g(J.p2));
+            if (c - zero < kZero) {
+                remove_point(net->gstruct[(size_t)J.g1], J.p1);
+                remove_point(net->gstruct[(size_t)J.g2], J.p2);
+                cgj.erase(cgj.begin() + i);
+            }
+        }
+        // a plate's relevant boundary points: those of its structure on the reserved nodes
+        // (points on the family's other faces were removed before the solve)
+        std::vector<int64_t> nbnd(cg.size(), 0);
+        for (size_t i = 0; i < cg.size(); i++)
+            for (int64_t q : net->gstruct[(size_t)cg[i]])
+                if (lmm_gnp[q] >= 0 && lmm_gnp[q] < reserved) nbnd[i]++;
+        // Clear_structure_of_dead_gnps (:797-855)
+        auto clear_dead = [&](int &to_remove) -> bool {
+            for (size_t i = 0; i < cg.size(); i++) {
+                std::vector<int64_t> &st = net->gstruct[(size_t)cg[i]];
+                const int64_t nt = (int64_t)st.size(), nb = nbnd[i], nj = nt - nb;
+                if (nt == 0 || (nb == 0 && nj == 1) || nb == nt) {
+                    if (!st.empty()) {
+                        st.clear();
+                        to_remove++;
+                    }
+                    if (nj == 1) to_remove++;
+                } else if (st.size() < 2) {
+                    return false;
+                }
+            }
+            return true;
+        };
+        int to_remove = 0;
+        if (!clear_dead(to_remove)) return fail("a live plate with fewer than two points");
+        while (to_remove) {
+            to_remove = 0;
+            // Remove_junctions_with_dead_particles_gnps (:999-1048)
+            for (int64_t i = (int64_t)cgj.size() - 1; i >= 0; i--) {
+                const GnpJunction &J = net->gjunctions[(size_t)cgj[(size_t)i]];
+                std::vector<int64_t> &s1 = net->gstruct[(size_t)J.g1];
+                std::vector<int64_t> &s2 = net->gstruct[(size_t)J.g2];
+                if (s1.empty() || s2.empty()) {
+                    cgj.erase(cgj.begin() + i);
+                    if (!s1.empty()) remove_point(s1, J.p1);
+                    if (!s2.empty()) remove_point(s2, J.p2);
+                }
+            }
+            // Remove_junctions_with_dead_particles_mixed (:1061-1127)
+            for (int64_t i = (int64_t)cm.size() - 1; i >= 0; i--) {
+                const MixedJunction &J = net->mjunctions[(size_t)cm[(size_t)i]];
+                std::vector<int64_t> &st = net->gstruct[(size_t)J.gnp];
+                if (el[(size_t)J.cnt].empty() || st.empty()) {
+                    cm.erase(cm.begin() + i);
+                    if (!st.empty()) remove_point(st, J.p_gnp);
+                } else if (!in_el(J.cnt, J.p_cnt)) {
+                    cm.erase(cm.begin() + i);
+                    remove_point(st, J.p_gnp);
+                }
+            }
+            if (!clear_dead(to_remove)) return fail("a live plate with fewer than two points");
+        }
+        // Remove_additional_cnts_from_cluster (:857-880)
+        for (int64_t i = (int64_t)cl.size() - 1; i >= 0; i--)
+            if (el[(size_t)cl[(size_t)i]].empty()) cl.erase(cl.begin() + i);
+        // Calculate_gnp_backbone_volume (:882-951)
+        for (int64_t g : cg) {
+            if (!net->gstruct[(size_t)g].empty()) gvol_bb = gvol_bb + gnp_volume(net, g);
+            else gvol_dead = gvol_dead + gnp_volume(net, g);
+        }
+    }
+    // Remove_junctions_with_dead_particles_cnts (:953-997)
+    if (!avoid_resistance) {
+        auto in_el = [&](int64_t t, int64_t p) {
+            const std::vector<int64_t> &e = el[(size_t)t];
+            return std::binary_search(e.begin(), e.end(), p);
+        };
+        for (int64_t i = (int64_t)cj.size() - 1; i >= 0; i--) {
+            const Junction &J = net->junctions[(size_t)cj[(size_t)i]];
+            if (el[(size_t)J.n1].empty() || el[(size_t)J.n2].empty() || !in_el(J.n1, J.p1) ||
+                !in_el(J.n2, J.p2))
+                cj.erase(cj.begin() + i);
+        }
+    }
+    // write the edited element lists back; plates are triangulated again from their edited
+    // structures, as the reference's next assembly does (Direct_Electrifying.cpp:1037-1040)
+    net->elem_ptr.assign(T + 1, 0);
+    net->elem_pts.clear();
+    for (size_t t = 0; t < T; t++) {
+        net->elem_pts.insert(net->elem_pts.end(), el[t].begin(), el[t].end());
+        net->elem_ptr[t + 1] = (int64_t)net->elem_pts.size();
+    }
+    std::vector<double> uv;
+    std::vector<std::pair<int, int>> edges;
+    for (int64_t g : cg) {
+        const Plate &P = net->gnps[(size_t)g];
+        const std::vector<int64_t> &st = net->gstruct[(size_t)g];
+        uv.clear();
+        for (int64_t id : st) {
+            double loc[3];
+            cicy_geom::to_local(P, &net->gxyz[3 * (size_t)id], loc);
+            uv.push_back(loc[0]);
+            uv.push_back(loc[1]);
+        }
+        std::vector<int64_t> &tri = net->gtri[(size_t)g];
+        tri.clear();
+        if (st.empty()) continue;
+        cicy_geom::plate_triangulation(P.l, uv, edges);
+        for (auto &e : edges) {
+            tri.push_back(st[(size_t)e.first]);
+            tri.push_back(st[(size_t)e.second]);
+        }
+    }
+    out[0] = zero;
+    out[1] = vol_bb;
+    out[2] = vol_dead;
+    out[3] = gvol_bb;
+    out[4] = gvol_dead;
+    return 1;
+}
 
 void cicy_tunnel_constants(double h, double e, double m, double lambda, double *c1,
                            double *c2) {

@@ -132,6 +132,23 @@ cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int famil
                                  int64_t err_len);
 void cicy_sss_free(cicy_sss_view *v);
 
+// Backbone of cluster k (Backbone_Network::Determine_backbone_network,
+// Backbone_Network.cpp:10-61), from the voltages [nodes] of its unit-resistor solve (R_flag
+// 0: VEF first) and that assembly's point -> node maps (lmm over the tube points, lmm_gnp
+// over the plate points; reserved = its reserved nodes).  Edits the cluster as the
+// reference does: dead tube ends and tubes, junctions with dead particles, plate junction
+// points; the plates are triangulated again, so the next cicy_net_assemble gives the
+// backbone's system.  avoid_resistance: volumes only, the tubes' elements and tube
+// junctions stay.  zero_in (may be NULL): the zero current when computed elsewhere;
+// seg_in (may be NULL): the backbone run (first, end element, -1 when none) of each
+// cluster tube, in cluster order, when computed elsewhere.  out[5]: zero current, backbone
+// and dead tube volume, backbone and dead plate volume.  0 on an error (message in err).
+int cicy_net_backbone(cicy_net *net, int64_t k, const double *volt, int64_t nvolt,
+                      const int64_t *lmm, int64_t n_points, const int64_t *lmm_gnp,
+                      int64_t n_gnp_points, int reserved, int avoid_resistance,
+                      const double *zero_in, const int32_t *seg_in, double *out, char *err,
+                      int64_t err_len);
+
 // Tunnelling constants from the mantissas the reference's input file gives
 // (Input_Reader.cpp:1450-1456).
 void cicy_tunnel_constants(double h, double e, double m, double lambda, double *c1,

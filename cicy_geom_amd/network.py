@@ -102,6 +102,10 @@ def _bind():
                                       ctypes.c_int, ctypes.POINTER(_ElecParams),
                                       ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
     lib.cicy_sss_free.argtypes = [ctypes.POINTER(_SSSView)]
+    lib.cicy_net_backbone.argtypes = [ctypes.c_void_p, ctypes.c_int64, F64P, ctypes.c_int64,
+                                      I64P, ctypes.c_int64, I64P, ctypes.c_int64, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, F64P,
+                                      ctypes.c_char_p, ctypes.c_int64]
     lib.cicy_tunnel_constants.argtypes = [ctypes.c_double] * 4 + [F64P, F64P]
     _bound = True
     return lib
@@ -350,6 +354,31 @@ class Network:
         """Clusters whose family includes `direction` (0 x, 1 y, 2 z)."""
         inc = {0: {0, 3, 4, 6}, 1: {1, 3, 5, 6}, 2: {2, 4, 5, 6}}[direction]
         return [k for k in range(self.count("clusters")) if self.cluster(k)[1] in inc]
+
+    def backbone(self, cluster: int, system: "DEASystem", voltages, avoid_resistance=False,
+                 zero=None, segments=None) -> dict:
+        """Backbone of `cluster` (Backbone_Network::Determine_backbone_network,
+        Backbone_Network.cpp:10-61) from the voltages [VEF | x] of its unit-resistor system
+        `system` (an R_flag = 0 assembly of this cluster).  Edits the cluster as the reference
+        does, so the next assemble() gives the backbone's system.  `zero` and `segments`
+        (int32 [2 per cluster tube]: first and end backbone element, -1 when none) may come
+        from the device scan (dea.Context.backbone_scan)."""
+        v = np.ascontiguousarray(voltages, np.float64)
+        lmm = np.ascontiguousarray(system.lmm, np.int64)
+        lg = np.ascontiguousarray(system.lmm_gnp, np.int64)
+        out = np.zeros(5)
+        z = None if zero is None else ctypes.c_double(zero)
+        seg = None if segments is None else np.ascontiguousarray(segments, np.int32)
+        err = ctypes.create_string_buffer(512)
+        ok = self._lib.cicy_net_backbone(
+            self._h, cluster, v.ctypes.data_as(F64P), len(v), lmm.ctypes.data_as(I64P), len(lmm),
+            lg.ctypes.data_as(I64P), len(lg), system.reserved, int(bool(avoid_resistance)),
+            None if z is None else ctypes.addressof(z),
+            None if seg is None else seg.ctypes.data, out.ctypes.data_as(F64P), err, 512)
+        if not ok:
+            raise RuntimeError(err.value.decode())
+        return dict(zero_current=out[0], cnt_backbone=out[1], cnt_dead=out[2],
+                    gnp_backbone=out[3], gnp_dead=out[4])
 
     def assemble(self, cluster: int, r_flag: int, family: int,
                  elec: ElectricParams | None = None, direction: int = -1,
