@@ -42,7 +42,7 @@ EXPORTED_SYMBOLS = [
     "cicy_host_condense", "cicy_host_expand", "cicy_host_core", "cicy_host_core_expand",
     "cicy_sys_core", "cicy_set_dropin_method", "cicy_sys_set_blocks", "cicy_host_core_block_cg",
     "cicy_sys_prepare",
-    "cicy_set_dropin_blocks",
+    "cicy_set_dropin_blocks", "cicy_backbone_scan",
 ]
 
 
@@ -149,6 +149,8 @@ def lib():
     L.cicy_mpnn_backward.argtypes = [_VP, _VP]
     L.cicy_mpnn_adam_step.argtypes = [_VP] + [ctypes.c_double] * 5
     L.cicy_mpnn_warm_start.argtypes = [_VP, _VP, _VP, P(ctypes.c_double)]
+    L.cicy_backbone_scan.argtypes = [_VP, ctypes.c_int64, _F64, ctypes.c_int64, _I64, _I64,
+                                     ctypes.c_int64, _I64, _I64, P(ctypes.c_double), _VP]
     _bound = L
     return L
 
@@ -285,6 +287,21 @@ class Context:
             self._h, len(n_bnd), _i64(n_bnd), _i64(n_int), _f64(re), _f64(voltages),
             len(voltages), ctypes.byref(out)), "cicy_boundary_current_f64")
         return out.value
+
+    def backbone_scan(self, voltages, tube_off, tube_nodes, pair_a, pair_b):
+        """Zero current and per-tube backbone runs of a unit-resistor solve on the device
+        (Backbone_Network.cpp:64-221, :317-409).  tube_off[t]..tube_off[t + 1] index the
+        voltage nodes of tube t's element points in tube_nodes; (pair_a, pair_b) are the
+        other elements.  Returns (zero current, int32 array [ntubes, 2] of (idx1, idx2))."""
+        off = _i64(tube_off)
+        nt = len(np.asarray(tube_off)) - 1
+        seg = np.zeros((max(nt, 1), 2), np.int32)
+        zero = ctypes.c_double()
+        pa = np.asarray(pair_a)
+        self._check(self._L.cicy_backbone_scan(
+            self._h, len(voltages), _f64(voltages), nt, off, _i64(tube_nodes), len(pa),
+            _i64(pa), _i64(pair_b), ctypes.byref(zero), _ptr(seg)), "cicy_backbone_scan")
+        return zero.value, seg[:nt]
 
     def system(self, row_ptr, col_ind, values, diagonal) -> "System":
         return System(self, row_ptr, col_ind, values, diagonal)

@@ -355,6 +355,41 @@ class Network:
         inc = {0: {0, 3, 4, 6}, 1: {1, 3, 5, 6}, 2: {2, 4, 5, 6}}[direction]
         return [k for k in range(self.count("clusters")) if self.cluster(k)[1] in inc]
 
+    def backbone_scan_inputs(self, cluster: int, system: "DEASystem"):
+        """The arrays dea.Context.backbone_scan reads for `cluster` and its unit-resistor
+        system: (tube_off, tube_nodes) -- the voltage nodes of each cluster tube's element
+        points, in cluster order -- and (pair_a, pair_b), the nodes of the other elements the
+        zero current is taken over (Backbone_Network.cpp:64-221): tube junctions, plate
+        triangulation edges, plate junctions and mixed junctions."""
+        lmm = np.asarray(system.lmm, np.int64)
+        lg = np.asarray(system.lmm_gnp, np.int64)
+        tubes = self.cluster_tubes(cluster)
+        els = [self.elements(int(t)) for t in tubes]
+        off = np.zeros(len(els) + 1, np.int64)
+        off[1:] = np.cumsum([len(e) for e in els])
+        nodes = lmm[np.concatenate(els)] if els else np.zeros(0, np.int64)
+        pa, pb = [], []
+        for j in self.cluster_junctions(cluster):
+            p1, p2 = self.junction(int(j))[:2]
+            pa.append(lmm[p1])
+            pb.append(lmm[p2])
+        tri = [self.gnp_triangulation(int(g)) for g in self.cluster_gnps(cluster)]
+        for j in self.cluster_gnp_junctions(cluster):
+            p1, p2 = self.gnp_junction(int(j))[:2]
+            pa.append(lg[p1])
+            pb.append(lg[p2])
+        for j in self.cluster_mixed_junctions(cluster):
+            pc, pg = self.mixed_junction(int(j))[:2]
+            pa.append(lmm[pc])
+            pb.append(lg[pg])
+        pa = np.array(pa, np.int64)
+        pb = np.array(pb, np.int64)
+        if tri:
+            e = np.concatenate(tri)
+            pa = np.concatenate([pa, lg[e[:, 0]]])
+            pb = np.concatenate([pb, lg[e[:, 1]]])
+        return off, nodes, pa, pb
+
     def backbone(self, cluster: int, system: "DEASystem", voltages, avoid_resistance=False,
                  zero=None, segments=None) -> dict:
         """Backbone of `cluster` (Backbone_Network::Determine_backbone_network,

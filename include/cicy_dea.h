@@ -144,6 +144,22 @@ int cicy_boundary_current_f64(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
                               const int64_t *n_int, const double *re,
                               const double *voltages, int64_t nv, double *current);
 
+/* ---- backbone extraction (Backbone_Network.cpp) ----------------------------------------
+ * The data-parallel part of Backbone_Network::Determine_backbone_network
+ * (Backbone_Network.cpp:10-61) from the voltages [nvolt] of a cluster's unit-resistor solve:
+ * zero_out = 1e-9 * max(1e-10, largest |V[a] - V[b]|) over the elements of every tube
+ * (tube t's element points' nodes are tube_nodes[tube_off[t] .. tube_off[t + 1]], in
+ * order) and over the npairs other elements (tube and plate junctions, plate triangulation
+ * edges: pair_a[q], pair_b[q]) -- Find_zero_current, :64-221 -- and, per tube, the run of
+ * its elements from the first whose |dV| exceeds zero_out by more than 1e-10 to the next one
+ * that does not (seg_out[2 t], seg_out[2 t + 1], -1 when absent; Find_backbone_cnts,
+ * :317-409).  The ordered pruning that uses them is the host's (cicy_net_backbone in the
+ * workload library).  Node ids are checked against nvolt on the host. */
+int cicy_backbone_scan(cicy_ctx *ctx, int64_t nvolt, const double *volt, int64_t ntubes,
+                       const int64_t *tube_off, const int64_t *tube_nodes, int64_t npairs,
+                       const int64_t *pair_a, const int64_t *pair_b, double *zero_out,
+                       int32_t *seg_out);
+
 /* ---- device-resident systems (repeated solves, benchmarking) --------------------------
  * cicy_sys_create uploads one SSS system (validated on the host) and keeps it in HBM.
  * The reference instead rebuilds and re-solves a fresh object per direction/cluster

@@ -5,6 +5,7 @@ Backbone_Network.cpp step by step, on plain Python lists, sets and dicts standin
 reference's HoKo->elements_cnt sets, cluster and junction vectors and structure_gnp:
 
   zero current                Find_zero_current                        :64-221
+                              (scan: the same from voltages and node lists alone)
   same-particle junctions     Zero_current_in_same_particle_junctions_unit_resistor :225-262
   tube backbone runs          Find_backbone_cnts                       :317-409
   tube volumes, trimming      Calculate_cnt_volumes                    :412-555
@@ -61,6 +62,37 @@ def gnp_volume(plate, box):
     if inside:
         v = v * (inside / (n * n))
     return v
+
+
+def scan(volt, tubes, pairs):
+    """The zero current and each tube's backbone run, from voltages alone
+    (Find_zero_current :64-221, Find_backbone_cnts :317-409): tubes is a list of node lists
+    (one per tube, its element points in order), pairs a list of (node, node) for the other
+    elements.  Returns (zero, [(idx1, idx2)] per tube)."""
+    imax = ZERO
+    for nodes in tubes:
+        for q in range(1, len(nodes)):
+            c = abs(volt[nodes[q - 1]] - volt[nodes[q]])
+            if c > imax:
+                imax = c
+    for a, b in pairs:
+        c = abs(volt[a] - volt[b])
+        if c > imax:
+            imax = c
+    zero = imax * 1e-9
+    runs = []
+    for nodes in tubes:
+        idx1 = idx2 = -1
+        for q in range(1, len(nodes)):
+            c = abs(volt[nodes[q - 1]] - volt[nodes[q]])
+            if c - zero > ZERO:
+                if idx1 == -1:
+                    idx1 = q - 1
+            elif idx1 != -1:
+                idx2 = q - 1
+                break
+        runs.append((idx1, idx2))
+    return zero, runs
 
 
 def backbone(net, volt, lmm, lmm_gnp, reserved, avoid_resistance=False):

@@ -191,3 +191,22 @@ def test_bad_input_is_rejected():
     s0 = net.assemble(k, 0, net.cluster(k)[1])
     with pytest.raises(RuntimeError):
         net.backbone(k, s0, np.zeros(3))  # voltages too short for the nodes
+
+
+def test_scan_agrees_with_the_extraction(case):
+    """oracle.backbone.scan (the device scan's checker) gives the zero current and the tube
+    runs that the full extraction found, from the cluster's node lists and voltages."""
+    name, (net, k, before, want, got, (s0, v)) = case
+    vol, ed = want
+    lmm, lg = s0.lmm, s0.lmm_gnp
+    tubes = [[int(lmm[p]) for p in before["elements"][t]] for t in before["cluster"]]
+    pairs = [(lmm[a], lmm[b]) for a, b, _, _ in before["junctions"]]
+    pairs += [(lg[a], lg[b]) for g in before["gnps"] for a, b in before["triangulation"][g]]
+    pairs += [(lg[a], lg[b]) for a, b, _, _ in before["gjunctions"]]
+    pairs += [(lmm[a], lg[b]) for a, b, _, _ in before["mixed"]]
+    zero, runs = ob.scan(v, tubes, pairs)
+    assert zero == vol["zero_current"]
+    assert runs == [ed["segments"][t] for t in before["cluster"]]
+    # the helper that feeds the device scan lists the same nodes
+    off, nodes, pa, pb = net.backbone_scan_inputs(k, s0)  # after the edit: fewer tubes
+    assert off[-1] == len(nodes) and len(pa) == len(pb)
