@@ -42,7 +42,7 @@ EXPORTED_SYMBOLS = [
     "cicy_host_condense", "cicy_host_expand", "cicy_host_core", "cicy_host_core_expand",
     "cicy_sys_core", "cicy_set_dropin_method", "cicy_sys_set_blocks", "cicy_host_core_block_cg",
     "cicy_sys_prepare",
-    "cicy_set_dropin_blocks", "cicy_backbone_scan",
+    "cicy_set_dropin_blocks", "cicy_backbone_scan", "cicy_cnt_contacts",
 ]
 
 
@@ -151,6 +151,11 @@ def lib():
     L.cicy_mpnn_warm_start.argtypes = [_VP, _VP, _VP, P(ctypes.c_double)]
     L.cicy_backbone_scan.argtypes = [_VP, ctypes.c_int64, _F64, ctypes.c_int64, _I64, _I64,
                                      ctypes.c_int64, _I64, _I64, P(ctypes.c_double), _VP]
+    L.cicy_cnt_contacts.argtypes = [_VP, ctypes.c_int64, _F64, ctypes.c_int64, _F64, _I64,
+                                    _I64, ctypes.c_int64, _I64, _F64, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int64, _VP, P(ctypes.c_int64),
+                                    ctypes.c_int64, P(ctypes.c_int64), _VP, _VP, _I64, _I64,
+                                    _F64]
     _bound = L
     return L
 
@@ -302,6 +307,48 @@ class Context:
             self._h, len(voltages), _f64(voltages), nt, off, _i64(tube_nodes), len(pa),
             _i64(pa), _i64(pair_b), ctypes.byref(zero), _ptr(seg)), "cicy_backbone_scan")
         return zero.value, seg[:nt]
+
+    def cnt_contacts(self, xyz, radii, structure, window_min, window_len, gs_min, overlap_cnt,
+                     overlap_gnp, vdw, tunnel, min_points, cnts_inside=None):
+        """Tube contacts and cluster labels of the reference's Make_cnt_clusters
+        (Hoshen_Kopelman.cpp:203-247) over Contact_grid's subregions (Contact_grid.cpp:18-295).
+        structure: list of point-id arrays, one per tube.  Returns dict(labels, n_labels,
+        junctions [n, 4] (Pi, tube i, Pj, tube j), junction_d, elements (list of arrays),
+        stats)."""
+        xyz = _f64(np.asarray(xyz, np.float64).reshape(-1))
+        n_points = len(xyz) // 3 if np.asarray(xyz).size else 0
+        n_cnts = len(structure)
+        off = np.zeros(n_cnts + 1, np.int64)
+        off[1:] = np.cumsum([len(s) for s in structure])
+        pts = _i64(np.concatenate([np.asarray(s, np.int64) for s in structure])
+                   if n_cnts else np.zeros(0, np.int64))
+        inside = np.arange(n_cnts) if cnts_inside is None else np.asarray(cnts_inside)
+        geom = _f64(np.concatenate([window_min, window_len, gs_min, [overlap_cnt, overlap_gnp]]))
+        labels = np.zeros(max(n_cnts, 1), np.int32)
+        elem_off = np.zeros(n_cnts + 1, np.int64)
+        elem_pts = np.zeros(max(int(off[-1]), 1), np.int64)
+        stats = np.zeros(8)
+        n_labels, n_junc = ctypes.c_int64(), ctypes.c_int64()
+        cap = 2 * int(off[-1]) + 1024
+        for _ in range(2):
+            junc = np.zeros((cap, 4), np.int64)
+            jd = np.zeros(cap)
+            ok = self._L.cicy_cnt_contacts(
+                self._h, n_points, xyz, n_cnts, _f64(radii), off, pts, len(inside),
+                _i64(inside), geom, vdw, tunnel, int(min_points), _ptr(labels),
+                ctypes.byref(n_labels), cap, ctypes.byref(n_junc), _ptr(junc), _ptr(jd),
+                elem_off, elem_pts, stats)
+            if ok or n_junc.value <= cap:
+                break
+            cap = n_junc.value
+        self._check(ok, "cicy_cnt_contacts")
+        nj = n_junc.value
+        return dict(labels=labels[:n_cnts], n_labels=n_labels.value, junctions=junc[:nj],
+                    junction_d=jd[:nj],
+                    elements=[elem_pts[elem_off[i]:elem_off[i + 1]] for i in range(n_cnts)],
+                    stats=dict(zip(["subregions", "slots", "pair_tests", "contacts",
+                                    "kernel_ms", "host_regions_ms", "device_ms",
+                                    "host_labels_ms"], stats.tolist())))
 
     def system(self, row_ptr, col_ind, values, diagonal) -> "System":
         return System(self, row_ptr, col_ind, values, diagonal)

@@ -160,6 +160,32 @@ int cicy_backbone_scan(cicy_ctx *ctx, int64_t nvolt, const double *volt, int64_t
                        const int64_t *pair_a, const int64_t *pair_b, double *zero_out,
                        int32_t *seg_out);
 
+/* ---- tube contact search and cluster labels (Contact_grid.cpp, Hoshen_Kopelman.cpp) -----
+ * Replaces Contact_grid::Generate_contact_grid's tube part (Contact_grid.cpp:18-295) and
+ * Hoshen_Kopelman::Make_cnt_clusters (Hoshen_Kopelman.cpp:203-247: Label_cnts_in_window,
+ * Cleanup_labels, Compress_cnt_cnt_contact_segments, Complete_cnt_elements).
+ * Points: xyz[3 n_points]; tube i holds struct_pts[struct_off[i] .. struct_off[i + 1]] (the
+ * reference's structure[i]; a point belongs to one tube), radius radii[i]; cnts_inside lists
+ * the tubes of the window in the reference's order.  geom[11] = window corner[3], window
+ * lengths[3], gs_min[3] (background grid), gs_overlap_cnt, gs_overlap_gnp (the larger of the
+ * two sizes the subregions, the first places points in neighbours).  Cutoffs: vdw, tunnel,
+ * min_points.
+ * Out: labels[n_cnts] (the reference's labels_cnt: proper labels 0.. in its numbering, -1
+ * without contact), *n_labels; junctions in the reference's order, junc[4 q..] = (Pi, tube
+ * i, Pj, tube j), junc_d[q] = junction distance, *n_junc (when it exceeds junc_cap the call
+ * fails after setting it); elements (elements_cnt) as elem_off[n_cnts + 1] into elem_pts,
+ * ascending point ids (elem_pts may be null; it needs at most struct_off[n_cnts] entries).
+ * stats[8] (may be null): subregions, slots, pair tests, contacts, pair-kernel ms, host
+ * subregion ms, device ms (copies and kernels), host labelling ms.  Results are the
+ * reference's bit for bit: the pairs are visited in its order and the distance uses the
+ * same IEEE operations. */
+int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *xyz, int64_t n_cnts,
+                      const double *radii, const int64_t *struct_off, const int64_t *struct_pts,
+                      int64_t n_inside, const int64_t *cnts_inside, const double *geom,
+                      double vdw, double tunnel, int64_t min_points, int32_t *labels,
+                      int64_t *n_labels, int64_t junc_cap, int64_t *n_junc, int64_t *junc,
+                      double *junc_d, int64_t *elem_off, int64_t *elem_pts, double *stats);
+
 /* ---- device-resident systems (repeated solves, benchmarking) --------------------------
  * cicy_sys_create uploads one SSS system (validated on the host) and keeps it in HBM.
  * The reference instead rebuilds and re-solves a fresh object per direction/cluster
