@@ -13,77 +13,35 @@
 // two tubes are compressed into one junction each (Compress_cnt_cnt_contact_segments,
 // :443-555) and tube end points complete the element sets (Complete_cnt_elements, :557-573).
 //
-// Here:
-//   host   the subregions, in the reference's order (an O(points) pass);
-//   device k_pack gathers each slot's point into slot-ordered arrays; k_pairs runs one
-//          thread per slot s, testing it against the later slots of its subregion, and
-//          appends every contact (s, k, d) -- the O(sum m^2) part;
-//   host   sorts the contacts by (s, k), which is the reference's visiting order because
-//          slots are numbered subregion by subregion in list order, and replays HK76, the
-//          closest-point map and the compression over them in that order.
-// Labels, junctions and elements are therefore those of the reference, bit for bit: the
-// distance is evaluated with the same IEEE operations in the same order (no contraction
-// into fused multiply-adds; the FP64 square root is correctly rounded).
+// Device, in HBM:
+//   k_place  one thread per visit v (the window's tubes in order, their points in order):
+//            home subregion and face neighbours, as keys (subregion << 32 | v);
+//   radix sort of the keys: slot order = the reference's subregion lists, since within a
+//            subregion the reference appends points in visiting order;
+//   k_slots  slot-ordered arrays of coordinates, tube, radius and subregion;
+//   k_pairs  one thread per slot, against the later slots of its subregion: a counting
+//            pass, an exclusive scan, and a writing pass that puts every contact at its
+//            place in the reference's visiting order (slot, then later slot).
+// Host: HK76, the closest-point map and the compression replayed over the contacts in that
+// order -- O(contacts) sequential bookkeeping, as the reference's.  Labels, junctions and
+// elements are the reference's bit for bit: the distance is evaluated with the same IEEE
+// operations in the same order (no contraction into fused multiply-adds; the FP64 square
+// root is correctly rounded), and so are the subregion coordinates and overlap flags.
 #include "dea_internal.h"
+
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
-#include <numeric>
-#include <unordered_map>
 #include <vector>
 
 namespace cicy {
 namespace {
 
 constexpr double kZeroTol = 1.0e-10;  // Zero, MathMatrix.h:18
-
-__global__ __launch_bounds__(kThreads) void k_pack(
-    int64_t slots, const int *__restrict__ slot_pt, const double *__restrict__ xyz,
-    const int *__restrict__ flag, const double *__restrict__ radii, double *__restrict__ sx,
-    double *__restrict__ sy, double *__restrict__ sz, double *__restrict__ sr,
-    int *__restrict__ sc) {
-    const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (s >= slots) return;
-    const int p = slot_pt[s];
-    const int c = flag[p];
-    sx[s] = xyz[3 * (int64_t)p];
-    sy[s] = xyz[3 * (int64_t)p + 1];
-    sz[s] = xyz[3 * (int64_t)p + 2];
-    sc[s] = c;
-    sr[s] = radii[c];
-}
-
-// One thread per slot: tests slot s against slots s+1 .. end[s]-1 of its subregion.
-__global__ __launch_bounds__(kThreads) void k_pairs(
-    int64_t slots, const int *__restrict__ end, const double *__restrict__ sx,
-    const double *__restrict__ sy, const double *__restrict__ sz, const double *__restrict__ sr,
-    const int *__restrict__ sc, double vdw, double tunnel, unsigned long long cap,
-    unsigned long long *__restrict__ count, int2 *__restrict__ pairs, double *__restrict__ dj) {
-#pragma clang fp contract(off)
-    const int64_t s = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (s >= slots) return;
-    const int e = end[s];
-    const double x = sx[s], y = sy[s], z = sz[s], r = sr[s];
-    const int c = sc[s];
-    for (int k = (int)s + 1; k < e; k++) {
-        if (sc[k] == c) continue;
-        const double dx = x - sx[k];
-        const double dy = y - sy[k];
-        const double dz = z - sz[k];
-        const double dist = sqrt(dx * dx + dy * dy + dz * dz);
-        double d = dist - r - sr[k];
-        if (d < vdw) d = vdw;
-        if (d <= tunnel) {
-            const unsigned long long i = atomicAdd(count, 1ull);
-            if (i < cap) {
-                pairs[i] = make_int2((int)s, k);
-                dj[i] = d;
-            }
-        }
-    }
-}
+constexpr unsigned long long kNoKey = ~0ull;
 
 struct Grid {
     int n[3];
@@ -109,21 +67,109 @@ Grid make_grid(const double *geom) {
     return g;
 }
 
-// Calculate_region_coordinates + Calculate_overlapping_flags (Contact_grid.cpp:163-212):
-// home cell co[3] and the neighbour offset f[3] in {-1, 0, 1} per axis.
-void place(const Grid &g, const double *p, int co[3], int f[3]) {
-    for (int k = 0; k < 3; k++) {
-        const double q = (p[k] - g.lo[k]) / g.l[k];
-        int a = q >= (double)g.n[k] ? g.n[k] - 1 : (q <= -1.0 ? 0 : (int)q);
-        if (a < 0) a = 0;
-        co[k] = a;
-        const double x1 = (double)a * g.l[k] + g.lo[k];
-        const double x2 = x1 + g.l[k];
-        f[k] = 0;
-        if (a > 0 && p[k] >= x1 && p[k] <= x1 + g.ov)
-            f[k] = -1;
-        else if (a < g.n[k] - 1 && p[k] >= x2 - g.ov && p[k] <= x2)
-            f[k] = 1;
+// Calculate_region_coordinates + Calculate_overlapping_flags + Assign_point_to_regions_cnts
+// (Contact_grid.cpp:163-295) for one point: up to four subregions, home first.
+__global__ __launch_bounds__(kThreads) void k_place(
+    long long nv, const int *__restrict__ vis_pt, const double *__restrict__ xyz, Grid g,
+    unsigned long long *__restrict__ keys, unsigned long long *__restrict__ nslots) {
+#pragma clang fp contract(off)
+    const long long v = (long long)blockIdx.x * kThreads + threadIdx.x;
+    int mine = 0;
+    if (v < nv) {
+        const double *p = xyz + 3 * (long long)vis_pt[v];
+        int co[3], f[3];
+        for (int k = 0; k < 3; k++) {
+            const double q = (p[k] - g.lo[k]) / g.l[k];
+            int a = q >= (double)g.n[k] ? g.n[k] - 1 : (q <= -1.0 ? 0 : (int)q);
+            if (a < 0) a = 0;
+            co[k] = a;
+            const double x1 = (double)a * g.l[k] + g.lo[k];
+            const double x2 = x1 + g.l[k];
+            f[k] = 0;
+            if (a > 0 && p[k] >= x1 && p[k] <= x1 + g.ov)
+                f[k] = -1;
+            else if (a < g.n[k] - 1 && p[k] >= x2 - g.ov && p[k] <= x2)
+                f[k] = 1;
+        }
+        const long long sxy = (long long)g.n[0] * g.n[1];
+        const long long t = co[0] + (long long)co[1] * g.n[0] + co[2] * sxy;
+        const unsigned long long lo = (unsigned long long)v;
+        keys[4 * v] = ((unsigned long long)t << 32) | lo;
+        keys[4 * v + 1] = f[0] ? ((unsigned long long)(t + f[0]) << 32) | lo : kNoKey;
+        keys[4 * v + 2] = f[1] ? ((unsigned long long)(t + (long long)f[1] * g.n[0]) << 32) | lo
+                               : kNoKey;
+        keys[4 * v + 3] = f[2] ? ((unsigned long long)(t + f[2] * sxy) << 32) | lo : kNoKey;
+        mine = 1 + (f[0] != 0) + (f[1] != 0) + (f[2] != 0);
+    }
+    // one atomic per wave
+    for (int o = kWave / 2; o > 0; o >>= 1) mine += __shfl_down(mine, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && mine) atomicAdd(nslots, (unsigned long long)mine);
+}
+
+__global__ __launch_bounds__(kThreads) void k_slots(
+    long long slots, const unsigned long long *__restrict__ keys, const int *__restrict__ vis_pt,
+    const double *__restrict__ xyz, const int *__restrict__ flag,
+    const double *__restrict__ radii, double *__restrict__ sx, double *__restrict__ sy,
+    double *__restrict__ sz, double *__restrict__ sr, int *__restrict__ sc,
+    int *__restrict__ sreg, int *__restrict__ spt) {
+    const long long s = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (s >= slots) return;
+    const unsigned long long key = keys[s];
+    const int p = vis_pt[(long long)(key & 0xffffffffull)];
+    const int c = flag[p];
+    sx[s] = xyz[3 * (long long)p];
+    sy[s] = xyz[3 * (long long)p + 1];
+    sz[s] = xyz[3 * (long long)p + 2];
+    sc[s] = c;
+    sr[s] = radii[c];
+    sreg[s] = (int)(key >> 32);
+    spt[s] = p;
+}
+
+// One thread per slot s: tests it against the later slots of its subregion.  WRITE = false
+// counts contacts (and pair tests); WRITE = true stores them from off[s] on, in k order.
+template <bool WRITE>
+__global__ __launch_bounds__(kThreads) void k_pairs(
+    long long slots, const int *__restrict__ sreg, const double *__restrict__ sx,
+    const double *__restrict__ sy, const double *__restrict__ sz, const double *__restrict__ sr,
+    const int *__restrict__ sc, const int *__restrict__ spt, double vdw, double tunnel,
+    int *__restrict__ cnt, unsigned long long *__restrict__ tests,
+    const long long *__restrict__ off, int *__restrict__ p1, int *__restrict__ p2,
+    double *__restrict__ dj) {
+#pragma clang fp contract(off)
+    const long long s = (long long)blockIdx.x * kThreads + threadIdx.x;
+    int found = 0;
+    unsigned long long tried = 0;
+    if (s < slots) {
+        const int t = sreg[s];
+        const double x = sx[s], y = sy[s], z = sz[s], r = sr[s];
+        const int c = sc[s];
+        long long w = WRITE ? off[s] : 0;
+        const int ps = WRITE ? spt[s] : 0;
+        for (long long k = s + 1; k < slots && sreg[k] == t; k++) {
+            tried++;
+            if (sc[k] == c) continue;
+            const double dx = x - sx[k];
+            const double dy = y - sy[k];
+            const double dz = z - sz[k];
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            double d = dist - r - sr[k];
+            if (d < vdw) d = vdw;
+            if (d <= tunnel) {
+                if (WRITE) {
+                    p1[w] = ps;
+                    p2[w] = spt[k];
+                    dj[w] = d;
+                    w++;
+                }
+                found++;
+            }
+        }
+        if (!WRITE) cnt[s] = found;
+    }
+    if (!WRITE) {
+        for (int o = kWave / 2; o > 0; o >>= 1) tried += __shfl_down(tried, o, kWave);
+        if ((threadIdx.x & (kWave - 1)) == 0 && tried) atomicAdd(tests, tried);
     }
 }
 
@@ -135,6 +181,23 @@ int find_root(int L, const std::vector<int> &ll) {
     }
     return r;
 }
+
+// Device buffers of one call, freed together.
+struct Bufs {
+    std::vector<void *> p;
+    template <class T>
+    bool get(cicy_ctx *ctx, T *&ptr, size_t n) {
+        ptr = nullptr;
+        if (!hip_ok(ctx, hipMalloc((void **)&ptr, std::max<size_t>(n, 1) * sizeof(T)),
+                    "hipMalloc"))
+            return false;
+        p.push_back(ptr);
+        return true;
+    }
+    ~Bufs() {
+        for (void *q : p) (void)hipFree(q);
+    }
+};
 
 }  // namespace
 }  // namespace cicy
@@ -199,158 +262,148 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
         set_error(ctx, "cicy_cnt_contacts: %.0f subregions (at most 1e8)", tot_d);
         return 0;
     }
-    const int64_t tot = (int64_t)tot_d;
-    // Fill_sectioned_domain_cnts (Contact_grid.cpp:104-161): counting pass, then placement in
-    // the same visiting order, so each subregion lists its points as the reference's does
-    std::vector<int64_t> roff((size_t)tot + 1, 0);
-    auto visit = [&](auto &&emit) {
-        for (int64_t i = 0; i < n_inside; i++) {
-            const int64_t cnt = cnts_inside[i];
-            for (int64_t q = struct_off[cnt]; q < struct_off[cnt + 1]; q++) {
-                const int64_t P = struct_pts[q];
-                int co[3], f[3];
-                place(g, xyz + 3 * P, co, f);
-                const int64_t sxy = (int64_t)g.n[0] * g.n[1];
-                const int64_t t = co[0] + (int64_t)co[1] * g.n[0] + co[2] * sxy;
-                emit(t, P);
-                if (f[0]) emit(t + f[0], P);
-                if (f[1]) emit(t + (int64_t)f[1] * g.n[0], P);
-                if (f[2]) emit(t + f[2] * sxy, P);
-            }
-        }
-    };
-    for (int64_t i = 0; i < n_inside; i++)
-        if (cnts_inside[i] < 0 || cnts_inside[i] >= n_cnts) {
+    // the visiting order of Fill_sectioned_domain_cnts (Contact_grid.cpp:104-161)
+    std::vector<int> vis;
+    for (int64_t i = 0; i < n_inside; i++) {
+        const int64_t c = cnts_inside[i];
+        if (c < 0 || c >= n_cnts) {
             set_error(ctx, "cicy_cnt_contacts: cnts_inside[%lld] = %lld out of range",
-                      (long long)i, (long long)cnts_inside[i]);
+                      (long long)i, (long long)c);
             return 0;
         }
-    visit([&](int64_t t, int64_t) { roff[(size_t)t + 1]++; });
-    for (int64_t t = 0; t < tot; t++) roff[(size_t)t + 1] += roff[(size_t)t];
-    const int64_t slots = roff[(size_t)tot];
-    if (slots >= (1LL << 31)) {
-        set_error(ctx, "cicy_cnt_contacts: %lld subregion slots (at most 2^31)",
-                  (long long)slots);
+        for (int64_t q = struct_off[c]; q < struct_off[c + 1]; q++)
+            vis.push_back((int)struct_pts[q]);
+    }
+    const long long nv = (long long)vis.size();
+    if (4 * nv >= (1LL << 31) || nv >= (1LL << 30)) {
+        set_error(ctx, "cicy_cnt_contacts: %lld visits (at most 2^29)", nv);
         return 0;
     }
-    std::vector<int> slot_pt((size_t)std::max<int64_t>(slots, 1)), send(slot_pt.size());
-    {
-        std::vector<int64_t> fill(roff.begin(), roff.end() - 1);
-        visit([&](int64_t t, int64_t P) { slot_pt[(size_t)fill[(size_t)t]++] = (int)P; });
-        for (int64_t t = 0; t < tot; t++)
-            for (int64_t s = roff[(size_t)t]; s < roff[(size_t)t + 1]; s++)
-                send[(size_t)s] = (int)roff[(size_t)t + 1];
-    }
-    double pair_tests = 0.0;
-    for (int64_t t = 0; t < tot; t++) {
-        const double m = (double)(roff[(size_t)t + 1] - roff[(size_t)t]);
-        pair_tests += 0.5 * m * (m - 1.0);
-    }
+    for (int &f : flag)
+        if (f < 0) f = 0;  // points outside every tube are never visited
     const auto t_host = clk::now();
 
-    // ---- device: pack the slots and test every pair of each subregion
+    // ---- device
     if (hipSetDevice(ctx->device) != hipSuccess) return 0;
     hipStream_t st = ctx->stream;
-    double *dxyz = nullptr, *drad = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr,
-           *sr = nullptr, *ddj = nullptr;
-    int *dflag = nullptr, *dslot = nullptr, *dend = nullptr, *sc = nullptr;
-    int2 *dpairs = nullptr;
-    unsigned long long *dcount = nullptr;
-    const size_t S = slot_pt.size();
-    const size_t NP = (size_t)std::max<int64_t>(n_points, 1), NC = (size_t)std::max<int64_t>(n_cnts, 1);
-    std::vector<double> rad(NC, 0.0);
-    if (n_cnts > 0) std::memcpy(rad.data(), radii, (size_t)n_cnts * 8);
-    std::vector<double> xyzv(3 * NP, 0.0);
-    if (n_points > 0) std::memcpy(xyzv.data(), xyz, (size_t)(3 * n_points) * 8);
-    for (int &f : flag)
-        if (f < 0) f = 0;  // points outside every tube are never in a slot
-    bool ok = hip_ok(ctx, hipMalloc((void **)&dxyz, 3 * NP * 8), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&drad, NC * 8), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&dflag, NP * 4), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&dslot, S * 4), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&dend, S * 4), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&sx, S * 8), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&sy, S * 8), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&sz, S * 8), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&sr, S * 8), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&sc, S * 4), "hipMalloc") &&
-              hip_ok(ctx, hipMalloc((void **)&dcount, 8), "hipMalloc");
+    Bufs B;
+    const size_t NP = (size_t)std::max<int64_t>(n_points, 1);
+    const size_t NC = (size_t)std::max<int64_t>(n_cnts, 1);
+    double *dxyz, *drad;
+    int *dflag, *dvis;
+    unsigned long long *keys, *keys2, *dctr;
+    bool ok = B.get(ctx, dxyz, 3 * NP) && B.get(ctx, drad, NC) && B.get(ctx, dflag, NP) &&
+              B.get(ctx, dvis, (size_t)nv) && B.get(ctx, keys, 4 * (size_t)nv) &&
+              B.get(ctx, keys2, 4 * (size_t)nv) && B.get(ctx, dctr, 2);
     ok = ok &&
-         hip_ok(ctx, hipMemcpyAsync(dxyz, xyzv.data(), 3 * NP * 8, hipMemcpyHostToDevice, st),
-                "H2D") &&
-         hip_ok(ctx, hipMemcpyAsync(drad, rad.data(), NC * 8, hipMemcpyHostToDevice, st), "H2D") &&
+         (n_points == 0 ||
+          hip_ok(ctx, hipMemcpyAsync(dxyz, xyz, 3 * (size_t)n_points * 8, hipMemcpyHostToDevice,
+                                     st), "H2D")) &&
+         (n_cnts == 0 || hip_ok(ctx, hipMemcpyAsync(drad, radii, (size_t)n_cnts * 8,
+                                                    hipMemcpyHostToDevice, st), "H2D")) &&
          hip_ok(ctx, hipMemcpyAsync(dflag, flag.data(), NP * 4, hipMemcpyHostToDevice, st),
                 "H2D") &&
-         hip_ok(ctx, hipMemcpyAsync(dslot, slot_pt.data(), S * 4, hipMemcpyHostToDevice, st),
-                "H2D") &&
-         hip_ok(ctx, hipMemcpyAsync(dend, send.data(), S * 4, hipMemcpyHostToDevice, st), "H2D");
-    const unsigned blocks = (unsigned)((std::max<int64_t>(slots, 1) + kThreads - 1) / kThreads);
-    if (ok && slots > 0)
-        hipLaunchKernelGGL(k_pack, dim3(blocks), dim3(kThreads), 0, st, (int64_t)slots, dslot,
-                           dxyz, dflag, drad, sx, sy, sz, sr, sc);
-    // contacts: a first guess of the room, then once more with the exact count if short
-    unsigned long long cap = (unsigned long long)std::max<int64_t>(4 * slots, 1024);
-    unsigned long long count = 0;
+         (nv == 0 || hip_ok(ctx, hipMemcpyAsync(dvis, vis.data(), (size_t)nv * 4,
+                                                hipMemcpyHostToDevice, st), "H2D")) &&
+         hip_ok(ctx, hipMemsetAsync(dctr, 0, 16, st), "memset");
+    int tbits = 1;
+    while ((1LL << tbits) <= (long long)tot_d) tbits++;  // kNoKey's field exceeds every t
+    auto grid_of = [](long long n) {
+        return dim3((unsigned)((std::max<long long>(n, 1) + kThreads - 1) / kThreads));
+    };
+    if (ok && nv > 0) {
+        hipLaunchKernelGGL(k_place, grid_of(nv), dim3(kThreads), 0, st, nv, dvis, dxyz, g, keys,
+                           dctr);
+        size_t tmp = 0;
+        ok = hip_ok(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, keys, keys2,
+                                                           (int)(4 * nv), 0, 32 + tbits, st),
+                    "radix sort size");
+        char *dtmp = nullptr;
+        ok = ok && B.get(ctx, dtmp, tmp) &&
+             hip_ok(ctx, hipcub::DeviceRadixSort::SortKeys(dtmp, tmp, keys, keys2, (int)(4 * nv),
+                                                           0, 32 + tbits, st),
+                    "radix sort");
+    }
+    unsigned long long ctr[2] = {0, 0};
+    ok = ok && hip_ok(ctx, hipMemcpyAsync(ctr, dctr, 16, hipMemcpyDeviceToHost, st), "D2H") &&
+         hip_ok(ctx, hipStreamSynchronize(st), "sync");
+    if (!ok) return 0;
+    const long long slots = (long long)ctr[0];
+    double *sx, *sy, *sz, *sr;
+    int *sc, *sreg, *spt, *cnt;
+    long long *off;
+    ok = B.get(ctx, sx, (size_t)slots) && B.get(ctx, sy, (size_t)slots) &&
+         B.get(ctx, sz, (size_t)slots) && B.get(ctx, sr, (size_t)slots) &&
+         B.get(ctx, sc, (size_t)slots) && B.get(ctx, sreg, (size_t)slots) &&
+         B.get(ctx, spt, (size_t)slots) && B.get(ctx, cnt, (size_t)slots + 1) &&
+         B.get(ctx, off, (size_t)slots + 1);
     float kernel_ms = 0.f;
-    for (int attempt = 0; ok && attempt < 2; attempt++) {
-        if (dpairs) (void)hipFree(dpairs);
-        if (ddj) (void)hipFree(ddj);
-        dpairs = nullptr;
-        ddj = nullptr;
-        ok = hip_ok(ctx, hipMalloc((void **)&dpairs, cap * sizeof(int2)), "hipMalloc") &&
-             hip_ok(ctx, hipMalloc((void **)&ddj, cap * 8), "hipMalloc") &&
-             hip_ok(ctx, hipMemsetAsync(dcount, 0, 8, st), "memset") &&
-             hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "event");
-        if (ok && slots > 0)
-            hipLaunchKernelGGL(k_pairs, dim3(blocks), dim3(kThreads), 0, st, (int64_t)slots,
-                               dend, sx, sy, sz, sr, sc, vdw, tunnel, cap, dcount, dpairs, ddj);
-        ok = ok && hip_ok(ctx, hipGetLastError(), "k_pairs launch") &&
-             hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "event") &&
-             hip_ok(ctx, hipMemcpyAsync(&count, dcount, 8, hipMemcpyDeviceToHost, st), "D2H") &&
+    long long count = 0;
+    if (ok && slots > 0) {
+        hipLaunchKernelGGL(k_slots, grid_of(slots), dim3(kThreads), 0, st, slots, keys2, dvis,
+                           dxyz, dflag, drad, sx, sy, sz, sr, sc, sreg, spt);
+        ok = hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "event");
+        hipLaunchKernelGGL(k_pairs<false>, grid_of(slots), dim3(kThreads), 0, st, slots, sreg,
+                           sx, sy, sz, sr, sc, spt, vdw, tunnel, cnt, dctr + 1, nullptr, nullptr,
+                           nullptr, nullptr);
+        ok = ok && hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "event") &&
+             hip_ok(ctx, hipMemsetAsync(cnt + slots, 0, 4, st), "memset");
+        // exclusive sum over slots + 1 counts (the last one 0): off[slots] = total
+        size_t tmp = 0;
+        ok = ok && hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt, off,
+                                                                (int)slots + 1, st),
+                          "scan size");
+        char *dtmp = nullptr;
+        ok = ok && B.get(ctx, dtmp, tmp) &&
+             hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(dtmp, tmp, cnt, off, (int)slots + 1,
+                                                          st), "scan") &&
+             hip_ok(ctx, hipMemcpyAsync(&count, off + slots, 8, hipMemcpyDeviceToHost, st),
+                    "D2H") &&
+             hip_ok(ctx, hipMemcpyAsync(ctr, dctr, 16, hipMemcpyDeviceToHost, st), "D2H") &&
              hip_ok(ctx, hipStreamSynchronize(st), "sync") &&
              hip_ok(ctx, hipEventElapsedTime(&kernel_ms, ctx->ev_start, ctx->ev_stop), "event");
-        if (!ok || count <= cap) break;
-        cap = count;
     }
-    std::vector<int2> pairs;
-    std::vector<double> dj;
-    if (ok) {
-        pairs.resize((size_t)count);
-        dj.resize((size_t)count);
-        ok = (count == 0 ||
-              (hip_ok(ctx, hipMemcpyAsync(pairs.data(), dpairs, count * sizeof(int2),
-                                          hipMemcpyDeviceToHost, st), "D2H") &&
-               hip_ok(ctx, hipMemcpyAsync(dj.data(), ddj, count * 8, hipMemcpyDeviceToHost, st),
-                      "D2H"))) &&
+    if (!ok) return 0;
+    std::vector<int> p1((size_t)count), p2((size_t)count);
+    std::vector<double> dj((size_t)count);
+    if (count > 0) {
+        int *dp1, *dp2;
+        double *ddj;
+        ok = B.get(ctx, dp1, (size_t)count) && B.get(ctx, dp2, (size_t)count) &&
+             B.get(ctx, ddj, (size_t)count);
+        if (ok)
+            hipLaunchKernelGGL(k_pairs<true>, grid_of(slots), dim3(kThreads), 0, st, slots, sreg,
+                               sx, sy, sz, sr, sc, spt, vdw, tunnel, nullptr, nullptr, off, dp1,
+                               dp2, ddj);
+        ok = ok && hip_ok(ctx, hipGetLastError(), "k_pairs launch") &&
+             hip_ok(ctx, hipMemcpyAsync(p1.data(), dp1, (size_t)count * 4,
+                                        hipMemcpyDeviceToHost, st), "D2H") &&
+             hip_ok(ctx, hipMemcpyAsync(p2.data(), dp2, (size_t)count * 4,
+                                        hipMemcpyDeviceToHost, st), "D2H") &&
+             hip_ok(ctx, hipMemcpyAsync(dj.data(), ddj, (size_t)count * 8,
+                                        hipMemcpyDeviceToHost, st), "D2H") &&
              hip_ok(ctx, hipStreamSynchronize(st), "sync");
     }
-    void *ptrs[] = {dxyz, drad, dflag, dslot, dend, sx, sy, sz, sr, sc, dcount, dpairs, ddj};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    ok = ok && hip_ok(ctx, hipGetLastError(), "contacts launch");
     if (!ok) return 0;
     const auto t_dev = clk::now();
 
-    // ---- host: the reference's visiting order, then its sequential bookkeeping
-    std::vector<uint32_t> order((size_t)count);
-    std::iota(order.begin(), order.end(), 0u);
-    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        const int2 pa = pairs[a], pb = pairs[b];
-        return pa.x != pb.x ? pa.x < pb.x : pa.y < pb.y;
-    });
+    // ---- host: the reference's sequential bookkeeping over the contacts in visiting order
     std::vector<int> lab((size_t)n_cnts, -1), ll;
     int new_label = 0;
-    struct Tri {
-        int a, b;
-        int64_t pa;
+    // point_contacts / point_contacts_dist: per point Pa a short list of (tube b, d, Pb)
+    struct Entry {
+        int b, next;
+        int64_t pb;
+        double d;
     };
-    std::vector<Tri> ce;
-    ce.reserve((size_t)count);
-    std::unordered_map<uint64_t, std::pair<double, int64_t>> pc;
-    pc.reserve((size_t)count);
-    for (uint32_t id : order) {
-        const int64_t P1 = slot_pt[(size_t)pairs[id].x], P2 = slot_pt[(size_t)pairs[id].y];
+    std::vector<Entry> ent;
+    ent.reserve((size_t)count);
+    std::vector<int> head((size_t)std::max<int64_t>(n_points, 1), -1);
+    for (long long id = 0; id < count; id++) {
+        const int64_t P1 = p1[(size_t)id], P2 = p2[(size_t)id];
         const int c1 = flag[(size_t)P1], c2 = flag[(size_t)P2];
-        const double d = dj[id];
+        const double d = dj[(size_t)id];
         // HK76 (Hoshen_Kopelman.cpp:2908-2953)
         if (lab[(size_t)c1] == -1 && lab[(size_t)c2] == -1) {
             lab[(size_t)c1] = lab[(size_t)c2] = new_label++;
@@ -376,15 +429,17 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
                 ll[(size_t)r1] = -r2;
             }
         }
-        const int a = std::min(c1, c2), b = std::max(c1, c2);
+        const int b = std::max(c1, c2);
         const int64_t Pa = c1 < c2 ? P1 : P2, Pb = c2 < c1 ? P1 : P2;
-        ce.push_back({a, b, Pa});
-        const uint64_t key = (uint64_t)Pa * (uint64_t)n_cnts + (uint64_t)b;
-        auto it = pc.find(key);
-        if (it == pc.end())
-            pc.emplace(key, std::make_pair(d, Pb));
-        else if (it->second.first > d)
-            it->second = std::make_pair(d, Pb);
+        int e = head[(size_t)Pa];
+        while (e != -1 && ent[(size_t)e].b != b) e = ent[(size_t)e].next;
+        if (e == -1) {
+            ent.push_back({b, head[(size_t)Pa], Pb, d});
+            head[(size_t)Pa] = (int)ent.size() - 1;
+        } else if (ent[(size_t)e].d > d) {
+            ent[(size_t)e].d = d;
+            ent[(size_t)e].pb = Pb;
+        }
     }
     // Cleanup_labels (:374-425)
     std::vector<int> label_map(ll.size(), -1);
@@ -397,16 +452,24 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
         labels[i] = L;
     }
     *n_labels = proper;
-    // Compress_cnt_cnt_contact_segments (:443-555): tube i ascending, then the other tube,
-    // then the points of tube i in contact with it, ascending
-    std::sort(ce.begin(), ce.end(), [](const Tri &x, const Tri &y) {
-        return x.a != y.a ? x.a < y.a : (x.b != y.b ? x.b < y.b : x.pa < y.pa);
-    });
-    ce.erase(std::unique(ce.begin(), ce.end(),
-                         [](const Tri &x, const Tri &y) {
-                             return x.a == y.a && x.b == y.b && x.pa == y.pa;
-                         }),
-             ce.end());
+    // contact_elements in Compress_cnt_cnt_contact_segments' order (:443-555): tube a
+    // ascending (counting sort), then the other tube b, then the point Pa of a, ascending
+    struct Tri {
+        int b;
+        int64_t pa;
+        int e;
+    };
+    std::vector<int64_t> toff((size_t)n_cnts + 1, 0);
+    for (int64_t P = 0; P < n_points; P++)
+        for (int e = head[(size_t)P]; e != -1; e = ent[(size_t)e].next) toff[(size_t)flag[(size_t)P] + 1]++;
+    for (int64_t i = 0; i < n_cnts; i++) toff[(size_t)i + 1] += toff[(size_t)i];
+    std::vector<Tri> tri(ent.size());
+    {
+        std::vector<int64_t> fill(toff.begin(), toff.end() - 1);
+        for (int64_t P = 0; P < n_points; P++)
+            for (int e = head[(size_t)P]; e != -1; e = ent[(size_t)e].next)
+                tri[(size_t)fill[(size_t)flag[(size_t)P]]++] = {ent[(size_t)e].b, P, e};
+    }
     std::vector<std::vector<int64_t>> el((size_t)n_cnts);
     int64_t nj = 0;
     auto emit = [&](int64_t pi, int i, int64_t pj, int j, double d) {
@@ -421,37 +484,39 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
         el[(size_t)i].push_back(pi);
         el[(size_t)j].push_back(pj);
     };
-    auto look = [&](int64_t pa, int b) -> const std::pair<double, int64_t> & {
-        return pc.at((uint64_t)pa * (uint64_t)n_cnts + (uint64_t)b);
-    };
-    for (size_t q = 0; q < ce.size();) {
-        size_t e = q;
-        while (e < ce.size() && ce[e].a == ce[q].a && ce[e].b == ce[q].b) e++;
-        const int i = ce[q].a, cj = ce[q].b;
-        int64_t Pi0 = ce[q].pa;
-        int64_t Pj0 = look(Pi0, cj).second;
-        double dmin = look(Pi0, cj).first;
-        int64_t Pi_j = Pi0, Pj_j = Pj0;
-        for (size_t u = q + 1; u < e; u++) {
-            const int64_t Pi1 = ce[u].pa;
-            const auto &v = look(Pi1, cj);
-            const int64_t Pj1 = v.second;
-            const double d = v.first;
-            if (Pi1 - Pi0 >= min_points || std::llabs(Pj1 - Pj0) >= min_points) {
-                emit(Pi_j, i, Pj_j, cj, dmin);
-                dmin = d;
-                Pi_j = Pi1;
-                Pj_j = Pj1;
-            } else if (d < dmin) {
-                dmin = d;
-                Pi_j = Pi1;
-                Pj_j = Pj1;
+    for (int64_t i = 0; i < n_cnts; i++) {
+        Tri *a0 = tri.data() + toff[(size_t)i], *a1 = tri.data() + toff[(size_t)i + 1];
+        std::sort(a0, a1, [](const Tri &x, const Tri &y) {
+            return x.b != y.b ? x.b < y.b : x.pa < y.pa;
+        });
+        for (Tri *q = a0; q < a1;) {
+            Tri *e = q;
+            while (e < a1 && e->b == q->b) e++;
+            const int cj = q->b;
+            int64_t Pi0 = q->pa;
+            int64_t Pj0 = ent[(size_t)q->e].pb;
+            double dmin = ent[(size_t)q->e].d;
+            int64_t Pi_j = Pi0, Pj_j = Pj0;
+            for (Tri *u = q + 1; u < e; u++) {
+                const int64_t Pi1 = u->pa;
+                const int64_t Pj1 = ent[(size_t)u->e].pb;
+                const double d = ent[(size_t)u->e].d;
+                if (Pi1 - Pi0 >= min_points || std::llabs(Pj1 - Pj0) >= min_points) {
+                    emit(Pi_j, (int)i, Pj_j, cj, dmin);
+                    dmin = d;
+                    Pi_j = Pi1;
+                    Pj_j = Pj1;
+                } else if (d < dmin) {
+                    dmin = d;
+                    Pi_j = Pi1;
+                    Pj_j = Pj1;
+                }
+                Pi0 = Pi1;
+                Pj0 = Pj1;
             }
-            Pi0 = Pi1;
-            Pj0 = Pj1;
+            emit(Pi_j, (int)i, Pj_j, cj, dmin);
+            q = e;
         }
-        emit(Pi_j, i, Pj_j, cj, dmin);
-        q = e;
     }
     *n_junc = nj;
     // Complete_cnt_elements (:557-573), element sets in ascending point order
@@ -472,9 +537,9 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
         auto ms = [](clk::time_point a, clk::time_point b) {
             return std::chrono::duration<double, std::milli>(b - a).count();
         };
-        stats[0] = (double)tot;
+        stats[0] = tot_d;
         stats[1] = (double)slots;
-        stats[2] = pair_tests;
+        stats[2] = (double)ctr[1];
         stats[3] = (double)count;
         stats[4] = kernel_ms;
         stats[5] = ms(t_start, t_host);
