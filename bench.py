@@ -102,6 +102,76 @@ def cpu_baseline(s, iters_needed, budget_s=12.0):
             "s_per_iter": t_iter, "host_nproc": os.cpu_count()}
 
 
+# Contact search workload: the reference's defaults (input.dat) -- 0.01 um steps,
+# 0.05 um background grid, van der Waals 0.00034 um, tunnelling 0.0018 um, min_points 5 --
+# on tubes of radius 0.004-0.006 um at about 8 % volume.
+CONTACT_CUTOFFS = (0.00034, 0.0018, 5)
+
+
+def contact_args(box):
+    return dict(window_min=[0.0, 0.0, 0.0], window_len=[box] * 3, gs_min=[0.05] * 3,
+                overlap_cnt=2 * 0.006 + CONTACT_CUTOFFS[1], overlap_gnp=0.0)
+
+
+def bench_contacts(args):
+    """One step = one full contact search + labelling (cicy_cnt_contacts) of a synthetic
+    wavy-tube network with its inputs in host memory, as the reference's call receives them;
+    value = points per second over the K timed steps."""
+    from cicy_geom_amd import dea, network
+    box = args.contacts_box
+    n_tubes = int(10_000 * (box / 2.0) ** 3)
+    t0 = time.perf_counter()
+    xyz, radii, structure = network.wavy_tubes(11, n_tubes, box, n_min=40, n_max=120)
+    log(f"[bench] contacts: {len(xyz)} points in {len(structure)} tubes, {box} um cube, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+    g = contact_args(box)
+    ctx = dea.Context(0)
+
+    def step():
+        return ctx.cnt_contacts(xyz, radii, structure, g["window_min"], g["window_len"],
+                                g["gs_min"], g["overlap_cnt"], g["overlap_gnp"],
+                                *CONTACT_CUTOFFS)
+    for _ in range(args.warmup):
+        res = step()
+    times, stats = [], []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        res = step()
+        times.append(time.perf_counter() - t)
+        stats.append(res["stats"])
+        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms {res['stats']}")
+    ms = 1e3 * sum(times) / len(times)
+    mean = {k: sum(s_[k] for s_ in stats) / len(stats) for k in stats[0]}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import hk
+        sb = 2.0
+        sx, sr, ss = network.wavy_tubes(11, int(10_000 * (sb / 2.0) ** 3), sb, n_min=40,
+                                        n_max=120)
+        gb = contact_args(sb)
+        t = time.perf_counter()
+        hk.cnt_clusters(gb["window_min"], gb["window_len"], gb["gs_min"], gb["overlap_cnt"],
+                        gb["overlap_gnp"], CONTACT_CUTOFFS, sr, ss, sx)
+        dt = time.perf_counter() - t
+        cpu = {"value": len(sx) / dt, "unit": "points/s", "cores": 1, "kind": "port",
+               "sample": f"oracle/hk.py (numpy per subregion, Python bookkeeping) on the same "
+                         f"generator at a {sb} um cube ({len(sx)} points): {dt:.2f} s"}
+    ctx.close()
+    print(json.dumps({
+        "metric": "contact_search_points_per_s", "value": len(xyz) / (ms * 1e-3),
+        "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"tube contact search + HK labels, {len(xyz)} points, "
+                               f"{len(structure)} tubes, {box} um cube, 0.05 um grid",
+                   "points": len(xyz), "tubes": len(structure)},
+        "stages_ms": {k: mean[k] for k in ("kernel_ms", "host_regions_ms", "device_ms",
+                                            "host_labels_ms")},
+        "counts": {k: mean[k] for k in ("subregions", "slots", "pair_tests", "contacts")},
+        "labels": int(res["n_labels"]), "junctions": int(len(res["junctions"])),
+        "cpu_baseline": cpu}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +198,11 @@ def main():
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only run the SpMV kernels (both precisions) and a 1 GiB reference "
                          "copy, for hardware-counter runs")
+    ap.add_argument("--workload", default="dea", choices=["dea", "contacts"],
+                    help="dea: the conductance solve (the default line); contacts: the tube "
+                         "contact search and cluster labels on synthetic wavy tubes")
+    ap.add_argument("--contacts-box", type=float, default=4.0,
+                    help="cube side (um) of the contact-search workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); multi-GPU "
@@ -136,6 +211,9 @@ def main():
 
     from cicy_geom_amd import dea, network
 
+    if args.workload == "contacts":
+        bench_contacts(args)
+        return
     t0 = time.perf_counter()
     net = network.generate(args.config)
     box = net.params.box

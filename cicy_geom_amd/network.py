@@ -489,6 +489,38 @@ def plate_row_ranges(net: Network, cluster: int, s: DEASystem) -> np.ndarray:
     return np.array(out, np.int64).reshape(-1, 2)
 
 
+def wavy_tubes(seed: int, n_tubes: int, box: float, step: float = 0.01, n_min: int = 20,
+               n_max: int = 80, r=(0.004, 0.006), kink: float = 0.09):
+    """Synthetic wavy tubes for the point-based contact search: each starts at a uniform
+    point with an isotropic direction and grows up to U[n_min, n_max) steps of `step` um,
+    turning by a Gaussian kick of `kink` per step, and ends at its first point outside
+    [0, box]^3 (as a window cut would).  Tubes with fewer than two points are dropped.
+    Returns (xyz [points, 3], radii [tubes], structure: point-id ranges per tube, consecutive
+    ids), the layout the network file readers produce."""
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(0.0, box, (n_tubes, 3))
+    d = rng.standard_normal((n_tubes, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    steps = rng.integers(n_min, n_max, n_tubes)
+    radii = rng.uniform(r[0], r[1], n_tubes)
+    pts = np.empty((n_tubes, n_max, 3))
+    pts[:, 0] = p
+    length = np.ones(n_tubes, np.int64)
+    alive = np.ones(n_tubes, bool)
+    for i in range(1, n_max):
+        d = d + kink * rng.standard_normal((n_tubes, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        p = p + step * d
+        alive &= (i <= steps) & (p >= 0.0).all(axis=1) & (p <= box).all(axis=1)
+        pts[:, i] = p
+        length += alive
+    keep = length >= 2
+    pts, length, radii = pts[keep], length[keep], radii[keep]
+    xyz = pts[np.arange(n_max)[None, :] < length[:, None]]
+    off = np.concatenate([[0], np.cumsum(length)])
+    return xyz, radii, [np.arange(off[i], off[i + 1]) for i in range(len(length))]
+
+
 def spmv_bytes(n: int, nnz: int, sv: int = 8, si: int = 4) -> int:
     """Algorithmic bytes of one SSS SpMV: L(sv+si) + N(sv+si) + 2 N sv."""
     return nnz * (sv + si) + n * (sv + si) + 2 * n * sv

@@ -10,7 +10,7 @@ Make_cnt_clusters) pinned by checks it cannot pass by repeating its own mistakes
 * the compression rule at its two ends: min_points = 1 gives one junction per (point, other
   tube), an unreachable min_points one per tube pair, each at the pair's closest points;
 * two crossing straight tubes give one junction at the known closest points.
-Also holds the wavy-tube generator the device tests use."""
+The networks are network.wavy_tubes (0.01 um steps, the reference's default)."""
 import numpy as np
 import pytest
 from scipy.sparse import coo_matrix
@@ -18,36 +18,12 @@ from scipy.sparse.csgraph import connected_components
 from scipy.spatial import cKDTree
 
 from oracle import hk
+from cicy_geom_amd import network
 
 VDW, TUNNEL, MINP = 0.00034, 0.0018, 5  # input.dat defaults (Cutoff_Distances)
 
 
-def wavy_tubes(seed, n_tubes, box, step=0.01, n_min=20, n_max=80, r=(0.004, 0.006), kink=0.3):
-    """Random tubes grown in `step` increments with small direction changes, each ended at
-    its first point outside [0, box]^3 (as a window cut would).  Returns (xyz, radii,
-    structure) with consecutive point ids per tube, the layout the file readers produce."""
-    rng = np.random.default_rng(seed)
-    pts, structure, radii = [], [], []
-    n = 0
-    for _ in range(n_tubes):
-        p = rng.uniform(0, box, 3)
-        d = rng.standard_normal(3)
-        d /= np.linalg.norm(d)
-        tube = [p]
-        for _ in range(int(rng.integers(n_min, n_max))):
-            d = d + kink * rng.standard_normal(3) * step / 0.01 * 0.3
-            d /= np.linalg.norm(d)
-            p = p + step * d
-            if (p < 0).any() or (p > box).any():
-                break
-            tube.append(p)
-        if len(tube) < 2:
-            continue
-        pts.extend(tube)
-        structure.append(np.arange(n, n + len(tube)))
-        radii.append(rng.uniform(*r))
-        n += len(tube)
-    return np.array(pts), np.array(radii), structure
+wavy_tubes = network.wavy_tubes
 
 
 def grid_args(box, gs=0.05, rmax=0.006):
