@@ -126,11 +126,13 @@ def bench_contacts(args):
         f"built in {time.perf_counter() - t0:.1f}s")
     g = contact_args(box)
     ctx = dea.Context(0)
+    off = np.concatenate([[0], np.cumsum([len(t) for t in structure])]).astype(np.int64)
+    tubes = (off, np.arange(off[-1], dtype=np.int64))  # consecutive ids per tube
 
     def step():
-        return ctx.cnt_contacts(xyz, radii, structure, g["window_min"], g["window_len"],
+        return ctx.cnt_contacts(xyz, radii, tubes, g["window_min"], g["window_len"],
                                 g["gs_min"], g["overlap_cnt"], g["overlap_gnp"],
-                                *CONTACT_CUTOFFS)
+                                *CONTACT_CUTOFFS, as_arrays=True)
     for _ in range(args.warmup):
         res = step()
     times, stats = [], []

@@ -21,9 +21,13 @@
 //   k_slots  slot-ordered arrays of coordinates, tube, radius and subregion;
 //   k_pairs  one thread per slot, against the later slots of its subregion: a counting
 //            pass, an exclusive scan, and a writing pass that puts every contact at its
-//            place in the reference's visiting order (slot, then later slot).
-// Host: HK76, the closest-point map and the compression replayed over the contacts in that
-// order -- O(contacts) sequential bookkeeping, as the reference's.  Labels, junctions and
+//            place in the reference's visiting order (slot, then later slot);
+//   point_contacts: a stable radix sort of the contacts by (Pa, other tube) keeps visiting
+//            order inside each group; k_closest keeps the group's closest contact (the first
+//            on ties); a stable sort of those entries by (tube of Pa, other tube) gives the
+//            order Compress_cnt_cnt_contact_segments walks, Pa ascending inside each pair.
+// Host: HK76 over the contacts' tube pairs in visiting order, and the compression over the
+// sorted entries -- the sequential parts of the reference, O(contacts).  Labels, junctions and
 // elements are the reference's bit for bit: the distance is evaluated with the same IEEE
 // operations in the same order (no contraction into fused multiply-adds; the FP64 square
 // root is correctly rounded), and so are the subregion coordinates and overlap flags.
@@ -71,10 +75,9 @@ Grid make_grid(const double *geom) {
 // (Contact_grid.cpp:163-295) for one point: up to four subregions, home first.
 __global__ __launch_bounds__(kThreads) void k_place(
     long long nv, const int *__restrict__ vis_pt, const double *__restrict__ xyz, Grid g,
-    unsigned long long *__restrict__ keys, unsigned long long *__restrict__ nslots) {
+    unsigned long long *__restrict__ keys) {
 #pragma clang fp contract(off)
     const long long v = (long long)blockIdx.x * kThreads + threadIdx.x;
-    int mine = 0;
     if (v < nv) {
         const double *p = xyz + 3 * (long long)vis_pt[v];
         int co[3], f[3];
@@ -99,11 +102,22 @@ __global__ __launch_bounds__(kThreads) void k_place(
         keys[4 * v + 2] = f[1] ? ((unsigned long long)(t + (long long)f[1] * g.n[0]) << 32) | lo
                                : kNoKey;
         keys[4 * v + 3] = f[2] ? ((unsigned long long)(t + f[2] * sxy) << 32) | lo : kNoKey;
-        mine = 1 + (f[0] != 0) + (f[1] != 0) + (f[2] != 0);
     }
-    // one atomic per wave
-    for (int o = kWave / 2; o > 0; o >>= 1) mine += __shfl_down(mine, o, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0 && mine) atomicAdd(nslots, (unsigned long long)mine);
+}
+
+// The sorted keys end with the absent neighbours (kNoKey): the slot count is the position of
+// the first of them.
+__global__ void k_first_absent(long long n, const unsigned long long *__restrict__ keys,
+                               unsigned long long *__restrict__ out) {
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        const long long mid = lo + (hi - lo) / 2;
+        if (keys[mid] == kNoKey)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    *out = (unsigned long long)lo;
 }
 
 __global__ __launch_bounds__(kThreads) void k_slots(
@@ -133,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void k_pairs(
     long long slots, const int *__restrict__ sreg, const double *__restrict__ sx,
     const double *__restrict__ sy, const double *__restrict__ sz, const double *__restrict__ sr,
     const int *__restrict__ sc, const int *__restrict__ spt, double vdw, double tunnel,
-    int *__restrict__ cnt, unsigned long long *__restrict__ tests,
+    int *__restrict__ cnt, unsigned long long *__restrict__ tests,  // per slot
     const long long *__restrict__ off, int *__restrict__ p1, int *__restrict__ p2,
     double *__restrict__ dj) {
 #pragma clang fp contract(off)
@@ -165,12 +179,79 @@ __global__ __launch_bounds__(kThreads) void k_pairs(
                 found++;
             }
         }
-        if (!WRITE) cnt[s] = found;
+        if (!WRITE) {
+            cnt[s] = found;
+            tests[s] = tried;
+        }
     }
-    if (!WRITE) {
-        for (int o = kWave / 2; o > 0; o >>= 1) tried += __shfl_down(tried, o, kWave);
-        if ((threadIdx.x & (kWave - 1)) == 0 && tried) atomicAdd(tests, tried);
+}
+
+// Per contact i (visiting order): its two tubes, for the host's HK76 replay, and the sort key
+// of point_contacts[Pa][b] -- Pa the point of the lower-numbered tube, b the other tube.
+__global__ __launch_bounds__(kThreads) void k_contact_keys(
+    long long n, const int *__restrict__ p1, const int *__restrict__ p2,
+    const int *__restrict__ flag, int cb, int2 *__restrict__ tubes,
+    unsigned long long *__restrict__ key, unsigned *__restrict__ val) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int a1 = p1[i], a2 = p2[i];
+    const int c1 = flag[a1], c2 = flag[a2];
+    tubes[i] = make_int2(c1, c2);
+    const int pa = c1 < c2 ? a1 : a2;
+    const int b = c1 < c2 ? c2 : c1;
+    key[i] = ((unsigned long long)pa << cb) | (unsigned long long)b;
+    val[i] = (unsigned)i;
+}
+
+__global__ __launch_bounds__(kThreads) void k_heads(long long n,
+                                                    const unsigned long long *__restrict__ key,
+                                                    int *__restrict__ head) {
+    const long long j = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (j >= n) return;
+    head[j] = (j == 0 || key[j] != key[j - 1]) ? 1 : 0;
+}
+
+// One thread per (Pa, b) group of the stably sorted contacts: the closest contact, the first
+// in visiting order on ties (point_contacts_dist keeps a value unless a strictly smaller one
+// comes), as entry pos[j], with the key (tube of Pa, b) of the compression order.
+__global__ __launch_bounds__(kThreads) void k_closest(
+    long long n, const unsigned long long *__restrict__ key, const unsigned *__restrict__ val,
+    const int *__restrict__ head, const int *__restrict__ pos, const int *__restrict__ p1,
+    const int *__restrict__ p2, const double *__restrict__ dj, const int *__restrict__ flag,
+    int cb, int *__restrict__ epa, int *__restrict__ epb, double *__restrict__ ed,
+    unsigned long long *__restrict__ key2, unsigned *__restrict__ val2) {
+    const long long j = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (j >= n || !head[j]) return;
+    const unsigned long long k = key[j];
+    unsigned best = val[j];
+    double bd = dj[best];
+    for (long long u = j + 1; u < n && key[u] == k; u++) {
+        const unsigned c = val[u];
+        if (dj[c] < bd) {
+            bd = dj[c];
+            best = c;
+        }
     }
+    const int pa = (int)(k >> cb);
+    const unsigned long long b = k & ((1ull << cb) - 1);
+    const int e = pos[j];
+    epa[e] = pa;
+    epb[e] = p1[best] == pa ? p2[best] : p1[best];
+    ed[e] = bd;
+    key2[e] = ((unsigned long long)flag[pa] << cb) | b;
+    val2[e] = (unsigned)e;
+}
+
+__global__ __launch_bounds__(kThreads) void k_gather_entries(
+    long long n, const unsigned *__restrict__ order, const int *__restrict__ epa,
+    const int *__restrict__ epb, const double *__restrict__ ed, int *__restrict__ opa,
+    int *__restrict__ opb, double *__restrict__ od) {
+    const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= n) return;
+    const unsigned e = order[q];
+    opa[q] = epa[e];
+    opb[q] = epb[e];
+    od[q] = ed[e];
 }
 
 int find_root(int L, const std::vector<int> &ll) {
@@ -289,6 +370,19 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
     Bufs B;
     const size_t NP = (size_t)std::max<int64_t>(n_points, 1);
     const size_t NC = (size_t)std::max<int64_t>(n_cnts, 1);
+    int cb = 1, pb = 1;  // bits of a tube id and of a point id
+    while ((1LL << cb) < n_cnts) cb++;
+    while ((1LL << pb) < n_points) pb++;
+    auto grid_of = [](long long n) {
+        return dim3((unsigned)((std::max<long long>(n, 1) + kThreads - 1) / kThreads));
+    };
+    // hipCUB calls: size query, scratch, run
+    auto cub = [&](auto &&call, const char *what) {
+        size_t tmp = 0;
+        if (!hip_ok(ctx, call(nullptr, tmp), what)) return false;
+        char *d = nullptr;
+        return B.get(ctx, d, tmp) && hip_ok(ctx, call(d, tmp), what);
+    };
     double *dxyz, *drad;
     int *dflag, *dvis;
     unsigned long long *keys, *keys2, *dctr;
@@ -308,35 +402,32 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
          hip_ok(ctx, hipMemsetAsync(dctr, 0, 16, st), "memset");
     int tbits = 1;
     while ((1LL << tbits) <= (long long)tot_d) tbits++;  // kNoKey's field exceeds every t
-    auto grid_of = [](long long n) {
-        return dim3((unsigned)((std::max<long long>(n, 1) + kThreads - 1) / kThreads));
-    };
+    const int nkeys = (int)(4 * nv);
     if (ok && nv > 0) {
-        hipLaunchKernelGGL(k_place, grid_of(nv), dim3(kThreads), 0, st, nv, dvis, dxyz, g, keys,
-                           dctr);
-        size_t tmp = 0;
-        ok = hip_ok(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, keys, keys2,
-                                                           (int)(4 * nv), 0, 32 + tbits, st),
-                    "radix sort size");
-        char *dtmp = nullptr;
-        ok = ok && B.get(ctx, dtmp, tmp) &&
-             hip_ok(ctx, hipcub::DeviceRadixSort::SortKeys(dtmp, tmp, keys, keys2, (int)(4 * nv),
-                                                           0, 32 + tbits, st),
-                    "radix sort");
+        hipLaunchKernelGGL(k_place, grid_of(nv), dim3(kThreads), 0, st, nv, dvis, dxyz, g, keys);
+        ok = cub([&](void *d, size_t &n) {
+                 return hipcub::DeviceRadixSort::SortKeys(d, n, keys, keys2, nkeys, 0,
+                                                          32 + tbits, st);
+             }, "radix sort") ;
+        if (ok)
+            hipLaunchKernelGGL(k_first_absent, dim3(1), dim3(1), 0, st, (long long)nkeys, keys2,
+                               dctr);
     }
     unsigned long long ctr[2] = {0, 0};
-    ok = ok && hip_ok(ctx, hipMemcpyAsync(ctr, dctr, 16, hipMemcpyDeviceToHost, st), "D2H") &&
+    ok = ok && hip_ok(ctx, hipGetLastError(), "launch") &&
+         hip_ok(ctx, hipMemcpyAsync(ctr, dctr, 16, hipMemcpyDeviceToHost, st), "D2H") &&
          hip_ok(ctx, hipStreamSynchronize(st), "sync");
     if (!ok) return 0;
     const long long slots = (long long)ctr[0];
     double *sx, *sy, *sz, *sr;
     int *sc, *sreg, *spt, *cnt;
     long long *off;
+    unsigned long long *tries;
     ok = B.get(ctx, sx, (size_t)slots) && B.get(ctx, sy, (size_t)slots) &&
          B.get(ctx, sz, (size_t)slots) && B.get(ctx, sr, (size_t)slots) &&
          B.get(ctx, sc, (size_t)slots) && B.get(ctx, sreg, (size_t)slots) &&
          B.get(ctx, spt, (size_t)slots) && B.get(ctx, cnt, (size_t)slots + 1) &&
-         B.get(ctx, off, (size_t)slots + 1);
+         B.get(ctx, off, (size_t)slots + 1) && B.get(ctx, tries, (size_t)slots);
     float kernel_ms = 0.f;
     long long count = 0;
     if (ok && slots > 0) {
@@ -344,19 +435,18 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
                            dxyz, dflag, drad, sx, sy, sz, sr, sc, sreg, spt);
         ok = hip_ok(ctx, hipEventRecord(ctx->ev_start, st), "event");
         hipLaunchKernelGGL(k_pairs<false>, grid_of(slots), dim3(kThreads), 0, st, slots, sreg,
-                           sx, sy, sz, sr, sc, spt, vdw, tunnel, cnt, dctr + 1, nullptr, nullptr,
+                           sx, sy, sz, sr, sc, spt, vdw, tunnel, cnt, tries, nullptr, nullptr,
                            nullptr, nullptr);
         ok = ok && hip_ok(ctx, hipEventRecord(ctx->ev_stop, st), "event") &&
              hip_ok(ctx, hipMemsetAsync(cnt + slots, 0, 4, st), "memset");
         // exclusive sum over slots + 1 counts (the last one 0): off[slots] = total
-        size_t tmp = 0;
-        ok = ok && hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt, off,
-                                                                (int)slots + 1, st),
-                          "scan size");
-        char *dtmp = nullptr;
-        ok = ok && B.get(ctx, dtmp, tmp) &&
-             hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(dtmp, tmp, cnt, off, (int)slots + 1,
-                                                          st), "scan") &&
+        ok = ok &&
+             cub([&](void *d, size_t &n) {
+                 return hipcub::DeviceScan::ExclusiveSum(d, n, cnt, off, (int)slots + 1, st);
+             }, "scan") &&
+             cub([&](void *d, size_t &n) {
+                 return hipcub::DeviceReduce::Sum(d, n, tries, dctr + 1, (int)slots, st);
+             }, "reduce") &&
              hip_ok(ctx, hipMemcpyAsync(&count, off + slots, 8, hipMemcpyDeviceToHost, st),
                     "D2H") &&
              hip_ok(ctx, hipMemcpyAsync(ctr, dctr, 16, hipMemcpyDeviceToHost, st), "D2H") &&
@@ -364,47 +454,94 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
              hip_ok(ctx, hipEventElapsedTime(&kernel_ms, ctx->ev_start, ctx->ev_stop), "event");
     }
     if (!ok) return 0;
-    std::vector<int> p1((size_t)count), p2((size_t)count);
-    std::vector<double> dj((size_t)count);
+    if (count >= (1LL << 31)) {
+        set_error(ctx, "cicy_cnt_contacts: %lld contacts (at most 2^31)", count);
+        return 0;
+    }
+    // contacts in visiting order; then point_contacts on the device: a stable sort by
+    // (Pa, b) keeps visiting order inside each group, the group's closest contact becomes an
+    // entry, and a stable sort of the entries by (tube of Pa, b) gives the compression order
+    // (Pa stays ascending inside each (tube, b), as it was after the first sort)
+    std::vector<int2> tubes((size_t)count);
+    long long ne = 0;
+    std::vector<int> opa, opb;
+    std::vector<double> od;
     if (count > 0) {
-        int *dp1, *dp2;
-        double *ddj;
-        ok = B.get(ctx, dp1, (size_t)count) && B.get(ctx, dp2, (size_t)count) &&
-             B.get(ctx, ddj, (size_t)count);
+        const int C = (int)count;
+        int *dp1, *dp2, *head, *pos, *epa, *epb, *qa, *qb;
+        double *ddj, *ed, *qd;
+        int2 *dtubes;
+        unsigned long long *k1, *k1s, *k2, *k2s;
+        unsigned *v1, *v1s, *v2, *v2s;
+        ok = B.get(ctx, dp1, (size_t)C) && B.get(ctx, dp2, (size_t)C) &&
+             B.get(ctx, ddj, (size_t)C) && B.get(ctx, dtubes, (size_t)C) &&
+             B.get(ctx, k1, (size_t)C) && B.get(ctx, k1s, (size_t)C) &&
+             B.get(ctx, v1, (size_t)C) && B.get(ctx, v1s, (size_t)C) &&
+             B.get(ctx, head, (size_t)C + 1) && B.get(ctx, pos, (size_t)C + 1);
+        if (ok) {
+            hipLaunchKernelGGL(k_pairs<true>, grid_of(slots), dim3(kThreads), 0, st, slots,
+                               sreg, sx, sy, sz, sr, sc, spt, vdw, tunnel, nullptr, nullptr, off,
+                               dp1, dp2, ddj);
+            hipLaunchKernelGGL(k_contact_keys, grid_of(C), dim3(kThreads), 0, st, (long long)C,
+                               dp1, dp2, dflag, cb, dtubes, k1, v1);
+        }
+        ok = ok &&
+             cub([&](void *d, size_t &n) {
+                 return hipcub::DeviceRadixSort::SortPairs(d, n, k1, k1s, v1, v1s, C, 0,
+                                                           pb + cb, st);
+             }, "radix sort") &&
+             hip_ok(ctx, hipMemsetAsync(head + C, 0, 4, st), "memset");
         if (ok)
-            hipLaunchKernelGGL(k_pairs<true>, grid_of(slots), dim3(kThreads), 0, st, slots, sreg,
-                               sx, sy, sz, sr, sc, spt, vdw, tunnel, nullptr, nullptr, off, dp1,
-                               dp2, ddj);
-        ok = ok && hip_ok(ctx, hipGetLastError(), "k_pairs launch") &&
-             hip_ok(ctx, hipMemcpyAsync(p1.data(), dp1, (size_t)count * 4,
+            hipLaunchKernelGGL(k_heads, grid_of(C), dim3(kThreads), 0, st, (long long)C, k1s,
+                               head);
+        ok = ok &&
+             cub([&](void *d, size_t &n) {
+                 return hipcub::DeviceScan::ExclusiveSum(d, n, head, pos, C + 1, st);
+             }, "scan");
+        int E = 0;
+        ok = ok && hip_ok(ctx, hipMemcpyAsync(&E, pos + C, 4, hipMemcpyDeviceToHost, st), "D2H") &&
+             hip_ok(ctx, hipStreamSynchronize(st), "sync");
+        ok = ok && B.get(ctx, epa, (size_t)E) && B.get(ctx, epb, (size_t)E) &&
+             B.get(ctx, ed, (size_t)E) && B.get(ctx, k2, (size_t)E) &&
+             B.get(ctx, k2s, (size_t)E) && B.get(ctx, v2, (size_t)E) &&
+             B.get(ctx, v2s, (size_t)E) && B.get(ctx, qa, (size_t)E) &&
+             B.get(ctx, qb, (size_t)E) && B.get(ctx, qd, (size_t)E);
+        if (ok)
+            hipLaunchKernelGGL(k_closest, grid_of(C), dim3(kThreads), 0, st, (long long)C, k1s,
+                               v1s, head, pos, dp1, dp2, ddj, dflag, cb, epa, epb, ed, k2, v2);
+        ok = ok &&
+             cub([&](void *d, size_t &n) {
+                 return hipcub::DeviceRadixSort::SortPairs(d, n, k2, k2s, v2, v2s, E, 0, 2 * cb,
+                                                           st);
+             }, "radix sort");
+        if (ok && E > 0)
+            hipLaunchKernelGGL(k_gather_entries, grid_of(E), dim3(kThreads), 0, st,
+                               (long long)E, v2s, epa, epb, ed, qa, qb, qd);
+        ne = E;
+        opa.resize((size_t)E);
+        opb.resize((size_t)E);
+        od.resize((size_t)E);
+        ok = ok && hip_ok(ctx, hipGetLastError(), "contacts launch") &&
+             hip_ok(ctx, hipMemcpyAsync(tubes.data(), dtubes, (size_t)C * sizeof(int2),
                                         hipMemcpyDeviceToHost, st), "D2H") &&
-             hip_ok(ctx, hipMemcpyAsync(p2.data(), dp2, (size_t)count * 4,
-                                        hipMemcpyDeviceToHost, st), "D2H") &&
-             hip_ok(ctx, hipMemcpyAsync(dj.data(), ddj, (size_t)count * 8,
-                                        hipMemcpyDeviceToHost, st), "D2H") &&
+             (E == 0 ||
+              (hip_ok(ctx, hipMemcpyAsync(opa.data(), qa, (size_t)E * 4, hipMemcpyDeviceToHost,
+                                          st), "D2H") &&
+               hip_ok(ctx, hipMemcpyAsync(opb.data(), qb, (size_t)E * 4, hipMemcpyDeviceToHost,
+                                          st), "D2H") &&
+               hip_ok(ctx, hipMemcpyAsync(od.data(), qd, (size_t)E * 8, hipMemcpyDeviceToHost,
+                                          st), "D2H"))) &&
              hip_ok(ctx, hipStreamSynchronize(st), "sync");
     }
     ok = ok && hip_ok(ctx, hipGetLastError(), "contacts launch");
     if (!ok) return 0;
     const auto t_dev = clk::now();
 
-    // ---- host: the reference's sequential bookkeeping over the contacts in visiting order
+    // ---- host: HK76 over the contacts in visiting order (Hoshen_Kopelman.cpp:2908-2953)
     std::vector<int> lab((size_t)n_cnts, -1), ll;
     int new_label = 0;
-    // point_contacts / point_contacts_dist: per point Pa a short list of (tube b, d, Pb)
-    struct Entry {
-        int b, next;
-        int64_t pb;
-        double d;
-    };
-    std::vector<Entry> ent;
-    ent.reserve((size_t)count);
-    std::vector<int> head((size_t)std::max<int64_t>(n_points, 1), -1);
     for (long long id = 0; id < count; id++) {
-        const int64_t P1 = p1[(size_t)id], P2 = p2[(size_t)id];
-        const int c1 = flag[(size_t)P1], c2 = flag[(size_t)P2];
-        const double d = dj[(size_t)id];
-        // HK76 (Hoshen_Kopelman.cpp:2908-2953)
+        const int c1 = tubes[(size_t)id].x, c2 = tubes[(size_t)id].y;
         if (lab[(size_t)c1] == -1 && lab[(size_t)c2] == -1) {
             lab[(size_t)c1] = lab[(size_t)c2] = new_label++;
             ll.push_back(2);
@@ -429,17 +566,6 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
                 ll[(size_t)r1] = -r2;
             }
         }
-        const int b = std::max(c1, c2);
-        const int64_t Pa = c1 < c2 ? P1 : P2, Pb = c2 < c1 ? P1 : P2;
-        int e = head[(size_t)Pa];
-        while (e != -1 && ent[(size_t)e].b != b) e = ent[(size_t)e].next;
-        if (e == -1) {
-            ent.push_back({b, head[(size_t)Pa], Pb, d});
-            head[(size_t)Pa] = (int)ent.size() - 1;
-        } else if (ent[(size_t)e].d > d) {
-            ent[(size_t)e].d = d;
-            ent[(size_t)e].pb = Pb;
-        }
     }
     // Cleanup_labels (:374-425)
     std::vector<int> label_map(ll.size(), -1);
@@ -452,26 +578,11 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
         labels[i] = L;
     }
     *n_labels = proper;
-    // contact_elements in Compress_cnt_cnt_contact_segments' order (:443-555): tube a
-    // ascending (counting sort), then the other tube b, then the point Pa of a, ascending
-    struct Tri {
-        int b;
-        int64_t pa;
-        int e;
-    };
-    std::vector<int64_t> toff((size_t)n_cnts + 1, 0);
-    for (int64_t P = 0; P < n_points; P++)
-        for (int e = head[(size_t)P]; e != -1; e = ent[(size_t)e].next) toff[(size_t)flag[(size_t)P] + 1]++;
-    for (int64_t i = 0; i < n_cnts; i++) toff[(size_t)i + 1] += toff[(size_t)i];
-    std::vector<Tri> tri(ent.size());
-    {
-        std::vector<int64_t> fill(toff.begin(), toff.end() - 1);
-        for (int64_t P = 0; P < n_points; P++)
-            for (int e = head[(size_t)P]; e != -1; e = ent[(size_t)e].next)
-                tri[(size_t)fill[(size_t)flag[(size_t)P]]++] = {ent[(size_t)e].b, P, e};
-    }
-    std::vector<std::vector<int64_t>> el((size_t)n_cnts);
+    // Compress_cnt_cnt_contact_segments (:443-555) over the entries, already in its order:
+    // tube a ascending, then the other tube b, then the point Pa of a ascending
+    std::vector<int64_t> ecount((size_t)n_cnts + 1, 0);
     int64_t nj = 0;
+    std::vector<int64_t> ji, jpts;  // junction tubes and points, for the element sets
     auto emit = [&](int64_t pi, int i, int64_t pj, int j, double d) {
         if (nj < junc_cap) {
             junc[4 * nj] = pi;
@@ -481,56 +592,65 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
             junc_d[nj] = d;
         }
         nj++;
-        el[(size_t)i].push_back(pi);
-        el[(size_t)j].push_back(pj);
+        ji.push_back(i);
+        jpts.push_back(pi);
+        ji.push_back(j);
+        jpts.push_back(pj);
     };
-    for (int64_t i = 0; i < n_cnts; i++) {
-        Tri *a0 = tri.data() + toff[(size_t)i], *a1 = tri.data() + toff[(size_t)i + 1];
-        std::sort(a0, a1, [](const Tri &x, const Tri &y) {
-            return x.b != y.b ? x.b < y.b : x.pa < y.pa;
-        });
-        for (Tri *q = a0; q < a1;) {
-            Tri *e = q;
-            while (e < a1 && e->b == q->b) e++;
-            const int cj = q->b;
-            int64_t Pi0 = q->pa;
-            int64_t Pj0 = ent[(size_t)q->e].pb;
-            double dmin = ent[(size_t)q->e].d;
-            int64_t Pi_j = Pi0, Pj_j = Pj0;
-            for (Tri *u = q + 1; u < e; u++) {
-                const int64_t Pi1 = u->pa;
-                const int64_t Pj1 = ent[(size_t)u->e].pb;
-                const double d = ent[(size_t)u->e].d;
-                if (Pi1 - Pi0 >= min_points || std::llabs(Pj1 - Pj0) >= min_points) {
-                    emit(Pi_j, (int)i, Pj_j, cj, dmin);
-                    dmin = d;
-                    Pi_j = Pi1;
-                    Pj_j = Pj1;
-                } else if (d < dmin) {
-                    dmin = d;
-                    Pi_j = Pi1;
-                    Pj_j = Pj1;
-                }
-                Pi0 = Pi1;
-                Pj0 = Pj1;
+    for (long long q = 0; q < ne;) {
+        // a group shares (tube of Pa, tube of Pb): Pb lies on the higher-numbered tube b
+        const int i = flag[(size_t)opa[(size_t)q]];
+        const int cj = flag[(size_t)opb[(size_t)q]];
+        long long e = q + 1;
+        while (e < ne && flag[(size_t)opa[(size_t)e]] == i && flag[(size_t)opb[(size_t)e]] == cj)
+            e++;
+        int64_t Pi0 = opa[(size_t)q], Pj0 = opb[(size_t)q];
+        double dmin = od[(size_t)q];
+        int64_t Pi_j = Pi0, Pj_j = Pj0;
+        for (long long u = q + 1; u < e; u++) {
+            const int64_t Pi1 = opa[(size_t)u], Pj1 = opb[(size_t)u];
+            const double d = od[(size_t)u];
+            if (Pi1 - Pi0 >= min_points || std::llabs(Pj1 - Pj0) >= min_points) {
+                emit(Pi_j, i, Pj_j, cj, dmin);
+                dmin = d;
+                Pi_j = Pi1;
+                Pj_j = Pj1;
+            } else if (d < dmin) {
+                dmin = d;
+                Pi_j = Pi1;
+                Pj_j = Pj1;
             }
-            emit(Pi_j, (int)i, Pj_j, cj, dmin);
-            q = e;
+            Pi0 = Pi1;
+            Pj0 = Pj1;
         }
+        emit(Pi_j, i, Pj_j, cj, dmin);
+        q = e;
     }
     *n_junc = nj;
-    // Complete_cnt_elements (:557-573), element sets in ascending point order
+    // Complete_cnt_elements (:557-573): junction points plus the end points of every tube
+    // with one, as ascending sets -- a counting sort of the junction points by tube
+    for (size_t q = 0; q < ji.size(); q++) ecount[(size_t)ji[q] + 1]++;
+    for (int64_t i = 0; i < n_cnts; i++)
+        if (ecount[(size_t)i + 1]) ecount[(size_t)i + 1] += 2;
+    for (int64_t i = 0; i < n_cnts; i++) ecount[(size_t)i + 1] += ecount[(size_t)i];
+    std::vector<int64_t> epts((size_t)ecount[(size_t)n_cnts]);
+    {
+        std::vector<int64_t> fill(ecount.begin(), ecount.end() - 1);
+        for (size_t q = 0; q < ji.size(); q++) epts[(size_t)fill[(size_t)ji[q]]++] = jpts[q];
+        for (int64_t i = 0; i < n_cnts; i++)
+            if (ecount[(size_t)i + 1] > ecount[(size_t)i]) {
+                epts[(size_t)fill[(size_t)i]++] = struct_pts[struct_off[i]];
+                epts[(size_t)fill[(size_t)i]++] = struct_pts[struct_off[i + 1] - 1];
+            }
+    }
     elem_off[0] = 0;
     for (int64_t i = 0; i < n_cnts; i++) {
-        std::vector<int64_t> &v = el[(size_t)i];
-        if (!v.empty()) {
-            v.push_back(struct_pts[struct_off[i]]);
-            v.push_back(struct_pts[struct_off[i + 1] - 1]);
-            std::sort(v.begin(), v.end());
-            v.erase(std::unique(v.begin(), v.end()), v.end());
-        }
-        elem_off[i + 1] = elem_off[i] + (int64_t)v.size();
-        if (elem_pts) std::copy(v.begin(), v.end(), elem_pts + elem_off[i]);
+        int64_t *a0 = epts.data() + ecount[(size_t)i], *a1 = epts.data() + ecount[(size_t)i + 1];
+        std::sort(a0, a1);
+        a1 = std::unique(a0, a1);
+        const int64_t m = a1 - a0;
+        if (elem_pts) std::copy(a0, a1, elem_pts + elem_off[i]);
+        elem_off[i + 1] = elem_off[i] + m;
     }
     const auto t_end = clk::now();
     if (stats) {

@@ -309,19 +309,26 @@ class Context:
         return zero.value, seg[:nt]
 
     def cnt_contacts(self, xyz, radii, structure, window_min, window_len, gs_min, overlap_cnt,
-                     overlap_gnp, vdw, tunnel, min_points, cnts_inside=None):
+                     overlap_gnp, vdw, tunnel, min_points, cnts_inside=None, as_arrays=False):
         """Tube contacts and cluster labels of the reference's Make_cnt_clusters
         (Hoshen_Kopelman.cpp:203-247) over Contact_grid's subregions (Contact_grid.cpp:18-295).
-        structure: list of point-id arrays, one per tube.  Returns dict(labels, n_labels,
-        junctions [n, 4] (Pi, tube i, Pj, tube j), junction_d, elements (list of arrays),
-        stats)."""
+        structure: list of point-id arrays, one per tube, or a pair (offsets [tubes + 1],
+        point ids).  Returns dict(labels, n_labels, junctions [n, 4] (Pi, tube i, Pj, tube j),
+        junction_d, elements (list of arrays; with as_arrays, elements_off and elements_pts
+        instead), stats)."""
+        npts_xyz = np.asarray(xyz).size
         xyz = _f64(np.asarray(xyz, np.float64).reshape(-1))
-        n_points = len(xyz) // 3 if np.asarray(xyz).size else 0
-        n_cnts = len(structure)
-        off = np.zeros(n_cnts + 1, np.int64)
-        off[1:] = np.cumsum([len(s) for s in structure])
-        pts = _i64(np.concatenate([np.asarray(s, np.int64) for s in structure])
-                   if n_cnts else np.zeros(0, np.int64))
+        n_points = npts_xyz // 3
+        if isinstance(structure, tuple):
+            off = np.ascontiguousarray(structure[0], np.int64)
+            pts = _i64(structure[1])
+            n_cnts = len(off) - 1
+        else:
+            n_cnts = len(structure)
+            off = np.zeros(n_cnts + 1, np.int64)
+            off[1:] = np.cumsum([len(s) for s in structure])
+            pts = _i64(np.concatenate([np.asarray(s, np.int64) for s in structure])
+                       if n_cnts else np.zeros(0, np.int64))
         inside = np.arange(n_cnts) if cnts_inside is None else np.asarray(cnts_inside)
         geom = _f64(np.concatenate([window_min, window_len, gs_min, [overlap_cnt, overlap_gnp]]))
         labels = np.zeros(max(n_cnts, 1), np.int32)
@@ -343,12 +350,16 @@ class Context:
             cap = n_junc.value
         self._check(ok, "cicy_cnt_contacts")
         nj = n_junc.value
-        return dict(labels=labels[:n_cnts], n_labels=n_labels.value, junctions=junc[:nj],
-                    junction_d=jd[:nj],
-                    elements=[elem_pts[elem_off[i]:elem_off[i + 1]] for i in range(n_cnts)],
-                    stats=dict(zip(["subregions", "slots", "pair_tests", "contacts",
-                                    "kernel_ms", "host_regions_ms", "device_ms",
-                                    "host_labels_ms"], stats.tolist())))
+        out = dict(labels=labels[:n_cnts], n_labels=n_labels.value, junctions=junc[:nj],
+                   junction_d=jd[:nj],
+                   stats=dict(zip(["subregions", "slots", "pair_tests", "contacts",
+                                   "kernel_ms", "host_regions_ms", "device_ms",
+                                   "host_labels_ms"], stats.tolist())))
+        if as_arrays:
+            out.update(elements_off=elem_off, elements_pts=elem_pts[:elem_off[-1]])
+        else:
+            out["elements"] = [elem_pts[elem_off[i]:elem_off[i + 1]] for i in range(n_cnts)]
+        return out
 
     def system(self, row_ptr, col_ind, values, diagonal) -> "System":
         return System(self, row_ptr, col_ind, values, diagonal)

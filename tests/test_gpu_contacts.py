@@ -84,6 +84,22 @@ def test_network_read_from_files(ctx, tmp_path, fmt):
     compare(got, want)
 
 
+def test_array_form_of_the_tubes(ctx):
+    """(offsets, point ids) in, element arrays out: the same result as the list form."""
+    xyz, radii, structure = th.wavy_tubes(12, 160, 0.5)
+    off = np.concatenate([[0], np.cumsum([len(s) for s in structure])])
+    g = th.grid_args(0.5)
+    got = ctx.cnt_contacts(xyz, radii, (off, np.concatenate(structure)), g["window_min"],
+                           g["window_len"], g["gs_min"], g["overlap_cnt"], g["overlap_gnp"],
+                           th.VDW, th.TUNNEL, th.MINP, as_arrays=True)
+    want = device(ctx, xyz, radii, structure, 0.5)
+    assert np.array_equal(got["labels"], want["labels"])
+    assert np.array_equal(got["junctions"], want["junctions"])
+    eo, ep = got["elements_off"], got["elements_pts"]
+    assert [ep[eo[i]:eo[i + 1]].tolist() for i in range(len(structure))] == \
+        [e.tolist() for e in want["elements"]]
+
+
 def test_empty_and_contactless(ctx):
     got = device(ctx, np.zeros((0, 3)), np.zeros(0), [], 0.5)
     assert got["n_labels"] == 0 and len(got["junctions"]) == 0
