@@ -26,8 +26,11 @@
 //            order inside each group; k_closest keeps the group's closest contact (the first
 //            on ties); a stable sort of those entries by (tube of Pa, other tube) gives the
 //            order Compress_cnt_cnt_contact_segments walks, Pa ascending inside each pair.
-// Host: HK76 over the contacts' tube pairs in visiting order, and the compression over the
-// sorted entries -- the sequential parts of the reference, O(contacts).  Labels, junctions and
+//   labels   union-find over the contacts' tube pairs (lock-free, larger root linked under
+//            the smaller), the first contact of every component, and a sort of the
+//            components by it: HK76 + Cleanup_labels' numbering (see k_uf_union below).
+// Host: the segment compression over the sorted entries -- the sequential part of the
+// reference, O(contacts).  Labels, junctions and
 // elements are the reference's bit for bit: the distance is evaluated with the same IEEE
 // operations in the same order (no contraction into fused multiply-adds; the FP64 square
 // root is correctly rounded), and so are the subregion coordinates and overlap flags.
@@ -75,11 +78,17 @@ Grid make_grid(const double *geom) {
 // (Contact_grid.cpp:163-295) for one point: up to four subregions, home first.
 __global__ __launch_bounds__(kThreads) void k_place(
     long long nv, const int *__restrict__ vis_pt, const double *__restrict__ xyz, Grid g,
-    unsigned long long *__restrict__ keys) {
+    unsigned long long *__restrict__ keys, int *__restrict__ bad) {
 #pragma clang fp contract(off)
     const long long v = (long long)blockIdx.x * kThreads + threadIdx.x;
     if (v < nv) {
-        const double *p = xyz + 3 * (long long)vis_pt[v];
+        const double *p0 = xyz + 3 * (long long)vis_pt[v];
+        // a coordinate that is not finite fails the call; it is placed at the window corner
+        // meanwhile, so every key stays a valid subregion
+        const bool fin = isfinite(p0[0]) && isfinite(p0[1]) && isfinite(p0[2]);
+        if (!fin) atomicOr(bad, 1);
+        const double p[3] = {fin ? p0[0] : g.lo[0], fin ? p0[1] : g.lo[1],
+                             fin ? p0[2] : g.lo[2]};
         int co[3], f[3];
         for (int k = 0; k < 3; k++) {
             const double q = (p[k] - g.lo[k]) / g.l[k];
@@ -254,13 +263,139 @@ __global__ __launch_bounds__(kThreads) void k_gather_entries(
     od[q] = ed[e];
 }
 
-int find_root(int L, const std::vector<int> &ll) {
-    int r = L;
-    while (ll[(size_t)r] <= 0) {
-        r = -ll[(size_t)r];
-        if (ll[(size_t)r] == 0) return 0;
+// ---- cluster labels on the device
+// HK76 with smaller-root merging numbers clusters by their first contact: the first contact of
+// a cluster (visiting order) always meets two unlabelled tubes, so it creates the cluster's
+// smallest label, which stays its root, and Cleanup_labels (Hoshen_Kopelman.cpp:374-425)
+// numbers roots in increasing order.  So: union-find over the contacts' tube pairs, the first
+// contact of every component, and components ranked by it -- the reference's labels_cnt.
+constexpr unsigned kNoFirst = 0xffffffffu;
+constexpr int kUnionTries = 1 << 20;  // a retry bound no union reaches; exceeding it fails
+
+__device__ __forceinline__ int ld_agent(const int *p) {
+    return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// parent[x] <= x always (a root is only ever linked under a smaller one), so the walk ends.
+// Path halving: a non-root is pointed at its grandparent on the way.  That store races only
+// with other halving stores (a compare-and-swap targets roots, and a non-root never becomes
+// one again), and every value written is an ancestor, so the partition is unchanged.
+__device__ __forceinline__ int uf_find(int *parent, int x) {
+    int p = ld_agent(parent + x);
+    while (p != x) {
+        const int gp = ld_agent(parent + p);
+        if (gp != p)
+            __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = gp;
+        p = ld_agent(parent + x);
     }
-    return r;
+    return x;
+}
+
+__global__ __launch_bounds__(kThreads) void k_uf_init(long long n, int *__restrict__ parent,
+                                                      unsigned *__restrict__ first_tube,
+                                                      unsigned *__restrict__ first_comp) {
+    const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= n) return;
+    parent[t] = (int)t;
+    first_tube[t] = kNoFirst;
+    first_comp[t] = kNoFirst;
+}
+
+// The first contact of every tube (visiting order).
+__global__ __launch_bounds__(kThreads) void k_first_contact(long long n,
+                                                            const int2 *__restrict__ tubes,
+                                                            unsigned *first_tube) {
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int2 c = tubes[i];
+    atomicMin(first_tube + c.x, (unsigned)i);
+    atomicMin(first_tube + c.y, (unsigned)i);
+}
+
+// Every distinct tube pair in contact -- the heads of the entries sorted by (a, b) -- joins
+// its two tubes: lock-free, the larger root linked under the smaller with a compare-and-swap,
+// retried on a lost race.
+__global__ __launch_bounds__(kThreads) void k_uf_union(long long n,
+                                                       const unsigned long long *__restrict__ ab,
+                                                       int cb, int *parent, int *fail) {
+    const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= n || (q > 0 && ab[q] == ab[q - 1])) return;
+    int a = (int)(ab[q] >> cb), b = (int)(ab[q] & ((1ull << cb) - 1));
+    for (int it = 0; it < kUnionTries; it++) {
+        a = uf_find(parent, a);
+        b = uf_find(parent, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        int expected = b;
+        if (__hip_atomic_compare_exchange_strong(parent + b, &expected, a, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return;
+    }
+    atomicOr(fail, 1);
+}
+
+// root of every tube; the first contact of each component (one atomic per wave when the
+// wave's tubes share a component, the common case inside a large cluster)
+__global__ __launch_bounds__(kThreads) void k_uf_roots(long long n, int *parent,
+                                                       const unsigned *__restrict__ first_tube,
+                                                       int *__restrict__ root,
+                                                       unsigned *first_comp) {
+    const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const bool in = t < n;
+    const int r = in ? uf_find(parent, (int)t) : -1;
+    if (in) root[t] = r;
+    const unsigned f = in ? first_tube[t] : kNoFirst;
+    const bool valid = f != kNoFirst;
+    const unsigned long long vm = __ballot(valid);
+    if (!vm) return;
+    const int src = __ffsll((long long)vm) - 1;
+    const int r0 = __shfl(r, src, kWave);
+    if (__all(!valid || r == r0)) {
+        unsigned m = f;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const unsigned u = __shfl_down(m, o, kWave);
+            m = u < m ? u : m;
+        }
+        if ((threadIdx.x & (kWave - 1)) == 0) atomicMin(first_comp + r0, m);
+    } else if (valid) {
+        atomicMin(first_comp + r, f);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_rank_keys(long long n, const int *__restrict__ root,
+                                                        const unsigned *__restrict__ first_comp,
+                                                        unsigned *__restrict__ key,
+                                                        int *__restrict__ val) {
+    const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= n) return;
+    key[t] = root[t] == (int)t ? first_comp[t] : kNoFirst;
+    val[t] = (int)t;
+}
+
+// labels: rank of the tube's component among components ordered by first contact; -1 for a
+// tube without contact.  keys sorted ascending, absent ones (kNoFirst) last.
+__global__ __launch_bounds__(kThreads) void k_rank(long long n, const unsigned *__restrict__ key,
+                                                   const int *__restrict__ val,
+                                                   int *__restrict__ rank,
+                                                   unsigned long long *__restrict__ n_labels) {
+    const long long q = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= n) return;
+    if (key[q] != kNoFirst) rank[val[q]] = (int)q;
+    if (key[q] != kNoFirst && (q + 1 == n || key[q + 1] == kNoFirst)) *n_labels = q + 1;
+}
+
+__global__ __launch_bounds__(kThreads) void k_labels(long long n, const int *__restrict__ root,
+                                                     const unsigned *__restrict__ first_tube,
+                                                     const int *__restrict__ rank,
+                                                     int *__restrict__ labels) {
+    const long long t = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (t >= n) return;
+    labels[t] = first_tube[t] != kNoFirst ? rank[root[t]] : -1;
 }
 
 // Device buffers of one call, freed together.
@@ -331,12 +466,6 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
             flag[(size_t)P] = (int)i;
         }
     }
-    for (int64_t q = 0; q < 3 * n_points; q++)
-        if (!std::isfinite(xyz[q])) {
-            set_error(ctx, "cicy_cnt_contacts: point %lld has a coordinate that is not finite",
-                      (long long)(q / 3));
-            return 0;
-        }
     const Grid g = make_grid(geom);
     const double tot_d = (double)g.n[0] * g.n[1] * g.n[2];
     if (tot_d > 1.0e8) {
@@ -344,7 +473,7 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
         return 0;
     }
     // the visiting order of Fill_sectioned_domain_cnts (Contact_grid.cpp:104-161)
-    std::vector<int> vis;
+    int64_t nvis = 0;
     for (int64_t i = 0; i < n_inside; i++) {
         const int64_t c = cnts_inside[i];
         if (c < 0 || c >= n_cnts) {
@@ -352,8 +481,15 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
                       (long long)i, (long long)c);
             return 0;
         }
-        for (int64_t q = struct_off[c]; q < struct_off[c + 1]; q++)
-            vis.push_back((int)struct_pts[q]);
+        nvis += struct_off[c + 1] - struct_off[c];
+    }
+    std::vector<int> vis((size_t)nvis);
+    {
+        int *w = vis.data();
+        for (int64_t i = 0; i < n_inside; i++) {
+            const int64_t c = cnts_inside[i];
+            w = std::copy(struct_pts + struct_off[c], struct_pts + struct_off[c + 1], w);
+        }
     }
     const long long nv = (long long)vis.size();
     if (4 * nv >= (1LL << 31) || nv >= (1LL << 30)) {
@@ -384,11 +520,11 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
         return B.get(ctx, d, tmp) && hip_ok(ctx, call(d, tmp), what);
     };
     double *dxyz, *drad;
-    int *dflag, *dvis;
+    int *dflag, *dvis, *dbad;
     unsigned long long *keys, *keys2, *dctr;
     bool ok = B.get(ctx, dxyz, 3 * NP) && B.get(ctx, drad, NC) && B.get(ctx, dflag, NP) &&
               B.get(ctx, dvis, (size_t)nv) && B.get(ctx, keys, 4 * (size_t)nv) &&
-              B.get(ctx, keys2, 4 * (size_t)nv) && B.get(ctx, dctr, 2);
+              B.get(ctx, keys2, 4 * (size_t)nv) && B.get(ctx, dctr, 2) && B.get(ctx, dbad, 1);
     ok = ok &&
          (n_points == 0 ||
           hip_ok(ctx, hipMemcpyAsync(dxyz, xyz, 3 * (size_t)n_points * 8, hipMemcpyHostToDevice,
@@ -399,12 +535,14 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
                 "H2D") &&
          (nv == 0 || hip_ok(ctx, hipMemcpyAsync(dvis, vis.data(), (size_t)nv * 4,
                                                 hipMemcpyHostToDevice, st), "H2D")) &&
-         hip_ok(ctx, hipMemsetAsync(dctr, 0, 16, st), "memset");
+         hip_ok(ctx, hipMemsetAsync(dctr, 0, 16, st), "memset") &&
+         hip_ok(ctx, hipMemsetAsync(dbad, 0, 4, st), "memset");
     int tbits = 1;
     while ((1LL << tbits) <= (long long)tot_d) tbits++;  // kNoKey's field exceeds every t
     const int nkeys = (int)(4 * nv);
     if (ok && nv > 0) {
-        hipLaunchKernelGGL(k_place, grid_of(nv), dim3(kThreads), 0, st, nv, dvis, dxyz, g, keys);
+        hipLaunchKernelGGL(k_place, grid_of(nv), dim3(kThreads), 0, st, nv, dvis, dxyz, g, keys,
+                           dbad);
         ok = cub([&](void *d, size_t &n) {
                  return hipcub::DeviceRadixSort::SortKeys(d, n, keys, keys2, nkeys, 0,
                                                           32 + tbits, st);
@@ -414,10 +552,17 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
                                dctr);
     }
     unsigned long long ctr[2] = {0, 0};
+    int bad = 0;
     ok = ok && hip_ok(ctx, hipGetLastError(), "launch") &&
          hip_ok(ctx, hipMemcpyAsync(ctr, dctr, 16, hipMemcpyDeviceToHost, st), "D2H") &&
+         hip_ok(ctx, hipMemcpyAsync(&bad, dbad, 4, hipMemcpyDeviceToHost, st), "D2H") &&
          hip_ok(ctx, hipStreamSynchronize(st), "sync");
     if (!ok) return 0;
+    if (bad) {
+        set_error(ctx, "cicy_cnt_contacts: a point of a window tube has a coordinate that is "
+                  "not finite");
+        return 0;
+    }
     const long long slots = (long long)ctr[0];
     double *sx, *sy, *sz, *sr;
     int *sc, *sreg, *spt, *cnt;
@@ -462,9 +607,13 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
     // (Pa, b) keeps visiting order inside each group, the group's closest contact becomes an
     // entry, and a stable sort of the entries by (tube of Pa, b) gives the compression order
     // (Pa stays ascending inside each (tube, b), as it was after the first sort)
-    std::vector<int2> tubes((size_t)count);
     long long ne = 0;
+    if (count == 0) {
+        for (int64_t i = 0; i < n_cnts; i++) labels[i] = -1;
+        *n_labels = 0;
+    }
     std::vector<int> opa, opb;
+    std::vector<unsigned long long> oab;  // (tube of Pa) << cb | b, the entries' sort keys
     std::vector<double> od;
     if (count > 0) {
         const int C = (int)count;
@@ -478,12 +627,29 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
              B.get(ctx, k1, (size_t)C) && B.get(ctx, k1s, (size_t)C) &&
              B.get(ctx, v1, (size_t)C) && B.get(ctx, v1s, (size_t)C) &&
              B.get(ctx, head, (size_t)C + 1) && B.get(ctx, pos, (size_t)C + 1);
+        int *parent, *root, *rval, *rval_s, *rank, *dlab, *dfail;
+        unsigned *first_tube, *first_comp, *rkey, *rkey_s;
+        unsigned long long *dnlab;
+        const long long T = n_cnts;
+        ok = ok && B.get(ctx, parent, (size_t)T) && B.get(ctx, root, (size_t)T) &&
+             B.get(ctx, rval, (size_t)T) && B.get(ctx, rval_s, (size_t)T) &&
+             B.get(ctx, rank, (size_t)T) && B.get(ctx, dlab, (size_t)T) &&
+             B.get(ctx, first_tube, (size_t)T) && B.get(ctx, first_comp, (size_t)T) &&
+             B.get(ctx, rkey, (size_t)T) && B.get(ctx, rkey_s, (size_t)T) &&
+             B.get(ctx, dfail, 1) && B.get(ctx, dnlab, 1) &&
+             hip_ok(ctx, hipMemsetAsync(dfail, 0, 4, st), "memset") &&
+             hip_ok(ctx, hipMemsetAsync(dnlab, 0, 8, st), "memset");
         if (ok) {
             hipLaunchKernelGGL(k_pairs<true>, grid_of(slots), dim3(kThreads), 0, st, slots,
                                sreg, sx, sy, sz, sr, sc, spt, vdw, tunnel, nullptr, nullptr, off,
                                dp1, dp2, ddj);
             hipLaunchKernelGGL(k_contact_keys, grid_of(C), dim3(kThreads), 0, st, (long long)C,
                                dp1, dp2, dflag, cb, dtubes, k1, v1);
+            // labels_cnt, part 1: the first contact of every tube
+            hipLaunchKernelGGL(k_uf_init, grid_of(T), dim3(kThreads), 0, st, T, parent,
+                               first_tube, first_comp);
+            hipLaunchKernelGGL(k_first_contact, grid_of(C), dim3(kThreads), 0, st,
+                               (long long)C, dtubes, first_tube);
         }
         ok = ok &&
              cub([&](void *d, size_t &n) {
@@ -514,70 +680,62 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
                  return hipcub::DeviceRadixSort::SortPairs(d, n, k2, k2s, v2, v2s, E, 0, 2 * cb,
                                                            st);
              }, "radix sort");
-        if (ok && E > 0)
+        if (ok && E > 0) {
             hipLaunchKernelGGL(k_gather_entries, grid_of(E), dim3(kThreads), 0, st,
                                (long long)E, v2s, epa, epb, ed, qa, qb, qd);
+            // labels_cnt, part 2: union-find over the distinct tube pairs, the first contact
+            // of every component, components ranked by it
+            hipLaunchKernelGGL(k_uf_union, grid_of(E), dim3(kThreads), 0, st, (long long)E,
+                               k2s, cb, parent, dfail);
+            hipLaunchKernelGGL(k_uf_roots, grid_of(T), dim3(kThreads), 0, st, T, parent,
+                               first_tube, root, first_comp);
+            hipLaunchKernelGGL(k_rank_keys, grid_of(T), dim3(kThreads), 0, st, T, root,
+                               first_comp, rkey, rval);
+        }
+        ok = ok && cub([&](void *d, size_t &n) {
+                 return hipcub::DeviceRadixSort::SortPairs(d, n, rkey, rkey_s, rval, rval_s,
+                                                           (int)T, 0, 32, st);
+             }, "radix sort");
+        if (ok) {
+            hipLaunchKernelGGL(k_rank, grid_of(T), dim3(kThreads), 0, st, T, rkey_s, rval_s,
+                               rank, dnlab);
+            hipLaunchKernelGGL(k_labels, grid_of(T), dim3(kThreads), 0, st, T, root,
+                               first_tube, rank, dlab);
+        }
+        int fail = 0;
+        unsigned long long nlab = 0;
+        ok = ok && hip_ok(ctx, hipGetLastError(), "labels launch") &&
+             hip_ok(ctx, hipMemcpyAsync(labels, dlab, (size_t)T * 4, hipMemcpyDeviceToHost, st),
+                    "D2H") &&
+             hip_ok(ctx, hipMemcpyAsync(&fail, dfail, 4, hipMemcpyDeviceToHost, st), "D2H") &&
+             hip_ok(ctx, hipMemcpyAsync(&nlab, dnlab, 8, hipMemcpyDeviceToHost, st), "D2H");
         ne = E;
+        oab.resize((size_t)E);
         opa.resize((size_t)E);
         opb.resize((size_t)E);
         od.resize((size_t)E);
         ok = ok && hip_ok(ctx, hipGetLastError(), "contacts launch") &&
-             hip_ok(ctx, hipMemcpyAsync(tubes.data(), dtubes, (size_t)C * sizeof(int2),
-                                        hipMemcpyDeviceToHost, st), "D2H") &&
              (E == 0 ||
-              (hip_ok(ctx, hipMemcpyAsync(opa.data(), qa, (size_t)E * 4, hipMemcpyDeviceToHost,
+              (hip_ok(ctx, hipMemcpyAsync(oab.data(), k2s, (size_t)E * 8, hipMemcpyDeviceToHost,
+                                          st), "D2H") &&
+               hip_ok(ctx, hipMemcpyAsync(opa.data(), qa, (size_t)E * 4, hipMemcpyDeviceToHost,
                                           st), "D2H") &&
                hip_ok(ctx, hipMemcpyAsync(opb.data(), qb, (size_t)E * 4, hipMemcpyDeviceToHost,
                                           st), "D2H") &&
                hip_ok(ctx, hipMemcpyAsync(od.data(), qd, (size_t)E * 8, hipMemcpyDeviceToHost,
                                           st), "D2H"))) &&
              hip_ok(ctx, hipStreamSynchronize(st), "sync");
+        if (ok && fail) {
+            set_error(ctx, "cicy_cnt_contacts: a union of two clusters did not settle");
+            return 0;
+        }
+        *n_labels = (int64_t)nlab;
     }
     ok = ok && hip_ok(ctx, hipGetLastError(), "contacts launch");
     if (!ok) return 0;
     const auto t_dev = clk::now();
 
-    // ---- host: HK76 over the contacts in visiting order (Hoshen_Kopelman.cpp:2908-2953)
-    std::vector<int> lab((size_t)n_cnts, -1), ll;
-    int new_label = 0;
-    for (long long id = 0; id < count; id++) {
-        const int c1 = tubes[(size_t)id].x, c2 = tubes[(size_t)id].y;
-        if (lab[(size_t)c1] == -1 && lab[(size_t)c2] == -1) {
-            lab[(size_t)c1] = lab[(size_t)c2] = new_label++;
-            ll.push_back(2);
-        } else if (lab[(size_t)c1] == -1 || lab[(size_t)c2] == -1) {
-            const int known = lab[(size_t)c1] == -1 ? c2 : c1;
-            const int fresh = known == c2 ? c1 : c2;
-            int L = lab[(size_t)known];
-            if (ll[(size_t)L] <= 0) L = find_root(L, ll);
-            lab[(size_t)fresh] = L;
-            ll[(size_t)L] += 1;
-        } else if (lab[(size_t)c1] != lab[(size_t)c2]) {
-            const int r1 = find_root(lab[(size_t)c1], ll), r2 = find_root(lab[(size_t)c2], ll);
-            if (ll[(size_t)r1] <= 0 || ll[(size_t)r2] <= 0) {
-                set_error(ctx, "cicy_cnt_contacts: merging labels that are not roots");
-                return 0;
-            }
-            if (r1 < r2) {
-                ll[(size_t)r1] += ll[(size_t)r2];
-                ll[(size_t)r2] = -r1;
-            } else if (r2 < r1) {
-                ll[(size_t)r2] += ll[(size_t)r1];
-                ll[(size_t)r1] = -r2;
-            }
-        }
-    }
-    // Cleanup_labels (:374-425)
-    std::vector<int> label_map(ll.size(), -1);
-    int proper = 0;
-    for (size_t i = 0; i < ll.size(); i++)
-        if (ll[i] > 0) label_map[i] = proper++;
-    for (int64_t i = 0; i < n_cnts; i++) {
-        int L = lab[(size_t)i];
-        if (L != -1) L = label_map[(size_t)find_root(L, ll)];
-        labels[i] = L;
-    }
-    *n_labels = proper;
+    // ---- host: the sequential compression
     // Compress_cnt_cnt_contact_segments (:443-555) over the entries, already in its order:
     // tube a ascending, then the other tube b, then the point Pa of a ascending
     std::vector<int64_t> ecount((size_t)n_cnts + 1, 0);
@@ -598,12 +756,11 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
         jpts.push_back(pj);
     };
     for (long long q = 0; q < ne;) {
-        // a group shares (tube of Pa, tube of Pb): Pb lies on the higher-numbered tube b
-        const int i = flag[(size_t)opa[(size_t)q]];
-        const int cj = flag[(size_t)opb[(size_t)q]];
+        // a group shares its key (tube of Pa, other tube b)
+        const unsigned long long key = oab[(size_t)q];
+        const int i = (int)(key >> cb), cj = (int)(key & ((1ull << cb) - 1));
         long long e = q + 1;
-        while (e < ne && flag[(size_t)opa[(size_t)e]] == i && flag[(size_t)opb[(size_t)e]] == cj)
-            e++;
+        while (e < ne && oab[(size_t)e] == key) e++;
         int64_t Pi0 = opa[(size_t)q], Pj0 = opb[(size_t)q];
         double dmin = od[(size_t)q];
         int64_t Pi_j = Pi0, Pj_j = Pj0;

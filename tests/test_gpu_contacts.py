@@ -117,6 +117,10 @@ def test_bad_input_is_rejected(ctx):
         device(ctx, xyz, np.array([0.005]), [np.array([0, 7])], 0.5)
     with pytest.raises(dea.DEAError, match="invalid"):
         device(ctx, xyz, np.array([0.005]), [np.arange(4)], -1.0)
+    bad = xyz.copy()
+    bad[2, 1] = np.nan
+    with pytest.raises(dea.DEAError, match="not finite"):
+        device(ctx, bad, np.array([0.005]), [np.arange(4)], 0.5)
 
 
 def test_large_network_labels_follow_junctions(ctx):
