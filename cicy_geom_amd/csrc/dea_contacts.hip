@@ -30,7 +30,7 @@
 //            the smaller), the first contact of every component, and a sort of the
 //            components by it: HK76 + Cleanup_labels' numbering (see k_uf_union below).
 // Host: the segment compression over the sorted entries -- the sequential part of the
-// reference, O(contacts).  Labels, junctions and
+// reference, O(contacts), split over host threads at (tube, other tube) boundaries.  Labels, junctions and
 // elements are the reference's bit for bit: the distance is evaluated with the same IEEE
 // operations in the same order (no contraction into fused multiply-adds; the FP64 square
 // root is correctly rounded), and so are the subregion coordinates and overlap flags.
@@ -42,6 +42,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace cicy {
@@ -735,80 +736,136 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
     if (!ok) return 0;
     const auto t_dev = clk::now();
 
-    // ---- host: the sequential compression
-    // Compress_cnt_cnt_contact_segments (:443-555) over the entries, already in its order:
-    // tube a ascending, then the other tube b, then the point Pa of a ascending
-    std::vector<int64_t> ecount((size_t)n_cnts + 1, 0);
-    int64_t nj = 0;
-    std::vector<int64_t> ji, jpts;  // junction tubes and points, for the element sets
-    auto emit = [&](int64_t pi, int i, int64_t pj, int j, double d) {
-        if (nj < junc_cap) {
-            junc[4 * nj] = pi;
-            junc[4 * nj + 1] = i;
-            junc[4 * nj + 2] = pj;
-            junc[4 * nj + 3] = j;
-            junc_d[nj] = d;
-        }
-        nj++;
-        ji.push_back(i);
-        jpts.push_back(pi);
-        ji.push_back(j);
-        jpts.push_back(pj);
+    // ---- host: Compress_cnt_cnt_contact_segments (:443-555) over the entries, already in
+    // its order (tube a ascending, then the other tube b, then the point Pa of a ascending).
+    // Groups (a, b) are independent: the entries are cut into one chunk per host thread at
+    // group boundaries, each chunk compresses its groups, and the junctions are concatenated
+    // in chunk order -- the reference's order.
+    struct Junc {
+        int64_t pi, pj;
+        int i, j;
+        double d;
     };
-    for (long long q = 0; q < ne;) {
-        // a group shares its key (tube of Pa, other tube b)
-        const unsigned long long key = oab[(size_t)q];
-        const int i = (int)(key >> cb), cj = (int)(key & ((1ull << cb) - 1));
-        long long e = q + 1;
-        while (e < ne && oab[(size_t)e] == key) e++;
-        int64_t Pi0 = opa[(size_t)q], Pj0 = opb[(size_t)q];
-        double dmin = od[(size_t)q];
-        int64_t Pi_j = Pi0, Pj_j = Pj0;
-        for (long long u = q + 1; u < e; u++) {
-            const int64_t Pi1 = opa[(size_t)u], Pj1 = opb[(size_t)u];
-            const double d = od[(size_t)u];
-            if (Pi1 - Pi0 >= min_points || std::llabs(Pj1 - Pj0) >= min_points) {
-                emit(Pi_j, i, Pj_j, cj, dmin);
-                dmin = d;
-                Pi_j = Pi1;
-                Pj_j = Pj1;
-            } else if (d < dmin) {
-                dmin = d;
-                Pi_j = Pi1;
-                Pj_j = Pj1;
+    const int nth = (int)std::max<long long>(
+        1, std::min<long long>({16, (long long)std::max(1u, std::thread::hardware_concurrency()),
+                                ne / 65536 + 1}));
+    auto parallel = [&](auto &&fn) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nth; t++) th.emplace_back(fn, t);
+        fn(0);
+        for (auto &x : th) x.join();
+    };
+    std::vector<long long> cut((size_t)nth + 1, ne);
+    for (int t = 0; t < nth; t++) {
+        long long c = ne * t / nth;
+        while (c > 0 && c < ne && oab[(size_t)c] == oab[(size_t)c - 1]) c++;
+        cut[(size_t)t] = c;
+    }
+    for (int t = nth - 1; t > 0; t--) cut[(size_t)t - 1] = std::min(cut[(size_t)t - 1], cut[(size_t)t]);
+    std::vector<std::vector<Junc>> part((size_t)nth);
+    parallel([&](int t) {
+        std::vector<Junc> &out = part[(size_t)t];
+        for (long long q = cut[(size_t)t]; q < cut[(size_t)t + 1];) {
+            const unsigned long long key = oab[(size_t)q];
+            const int i = (int)(key >> cb), cj = (int)(key & ((1ull << cb) - 1));
+            long long e = q + 1;
+            while (e < ne && oab[(size_t)e] == key) e++;
+            int64_t Pi0 = opa[(size_t)q], Pj0 = opb[(size_t)q];
+            double dmin = od[(size_t)q];
+            int64_t Pi_j = Pi0, Pj_j = Pj0;
+            for (long long u = q + 1; u < e; u++) {
+                const int64_t Pi1 = opa[(size_t)u], Pj1 = opb[(size_t)u];
+                const double d = od[(size_t)u];
+                if (Pi1 - Pi0 >= min_points || std::llabs(Pj1 - Pj0) >= min_points) {
+                    out.push_back({Pi_j, Pj_j, i, cj, dmin});
+                    dmin = d;
+                    Pi_j = Pi1;
+                    Pj_j = Pj1;
+                } else if (d < dmin) {
+                    dmin = d;
+                    Pi_j = Pi1;
+                    Pj_j = Pj1;
+                }
+                Pi0 = Pi1;
+                Pj0 = Pj1;
             }
-            Pi0 = Pi1;
-            Pj0 = Pj1;
+            out.push_back({Pi_j, Pj_j, i, cj, dmin});
+            q = e;
         }
-        emit(Pi_j, i, Pj_j, cj, dmin);
-        q = e;
-    }
+    });
+    std::vector<int64_t> joff((size_t)nth + 1, 0);
+    for (int t = 0; t < nth; t++) joff[(size_t)t + 1] = joff[(size_t)t] + (int64_t)part[(size_t)t].size();
+    const int64_t nj = joff[(size_t)nth];
     *n_junc = nj;
-    // Complete_cnt_elements (:557-573): junction points plus the end points of every tube
-    // with one, as ascending sets -- a counting sort of the junction points by tube
-    for (size_t q = 0; q < ji.size(); q++) ecount[(size_t)ji[q] + 1]++;
-    for (int64_t i = 0; i < n_cnts; i++)
-        if (ecount[(size_t)i + 1]) ecount[(size_t)i + 1] += 2;
-    for (int64_t i = 0; i < n_cnts; i++) ecount[(size_t)i + 1] += ecount[(size_t)i];
-    std::vector<int64_t> epts((size_t)ecount[(size_t)n_cnts]);
-    {
-        std::vector<int64_t> fill(ecount.begin(), ecount.end() - 1);
-        for (size_t q = 0; q < ji.size(); q++) epts[(size_t)fill[(size_t)ji[q]]++] = jpts[q];
-        for (int64_t i = 0; i < n_cnts; i++)
-            if (ecount[(size_t)i + 1] > ecount[(size_t)i]) {
-                epts[(size_t)fill[(size_t)i]++] = struct_pts[struct_off[i]];
-                epts[(size_t)fill[(size_t)i]++] = struct_pts[struct_off[i + 1] - 1];
+    // Complete_cnt_elements (:557-573): each tube with a junction gets its junction points and
+    // its two end points, as an ascending set.  Per-thread tube histograms, one scatter, then
+    // a sort of every tube's points.
+    const size_t T1 = (size_t)n_cnts + 1;
+    std::vector<int64_t> hist((size_t)nth * T1, 0);
+    parallel([&](int t) {
+        int64_t *h = hist.data() + (size_t)t * T1;
+        for (const Junc &x : part[(size_t)t]) {
+            h[x.i]++;
+            h[x.j]++;
+        }
+        // the junctions themselves, at their place in the output
+        int64_t q = joff[(size_t)t];
+        for (const Junc &x : part[(size_t)t]) {
+            if (q < junc_cap) {
+                junc[4 * q] = x.pi;
+                junc[4 * q + 1] = x.i;
+                junc[4 * q + 2] = x.pj;
+                junc[4 * q + 3] = x.j;
+                junc_d[q] = x.d;
             }
-    }
-    elem_off[0] = 0;
+            q++;
+        }
+    });
+    // tube offsets: junction points + 2 end points per tube that has any
+    std::vector<int64_t> ecount(T1, 0);
     for (int64_t i = 0; i < n_cnts; i++) {
-        int64_t *a0 = epts.data() + ecount[(size_t)i], *a1 = epts.data() + ecount[(size_t)i + 1];
-        std::sort(a0, a1);
-        a1 = std::unique(a0, a1);
-        const int64_t m = a1 - a0;
-        if (elem_pts) std::copy(a0, a1, elem_pts + elem_off[i]);
-        elem_off[i + 1] = elem_off[i] + m;
+        int64_t c = 0;
+        for (int t = 0; t < nth; t++) c += hist[(size_t)t * T1 + (size_t)i];
+        ecount[(size_t)i + 1] = ecount[(size_t)i] + (c ? c + 2 : 0);
     }
+    // per-thread write positions inside each tube: thread t after threads 0..t-1
+    for (int64_t i = 0; i < n_cnts; i++) {
+        int64_t at = ecount[(size_t)i];
+        for (int t = 0; t < nth; t++) {
+            const int64_t c = hist[(size_t)t * T1 + (size_t)i];
+            hist[(size_t)t * T1 + (size_t)i] = at;
+            at += c;
+        }
+    }
+    std::vector<int64_t> epts((size_t)ecount[(size_t)n_cnts]);
+    parallel([&](int t) {
+        int64_t *h = hist.data() + (size_t)t * T1;
+        for (const Junc &x : part[(size_t)t]) {
+            epts[(size_t)h[x.i]++] = x.pi;
+            epts[(size_t)h[x.j]++] = x.pj;
+        }
+    });
+    std::vector<int64_t> ecnt((size_t)n_cnts, 0);
+    parallel([&](int t) {
+        const int64_t i0 = n_cnts * t / nth, i1 = n_cnts * (t + 1) / nth;
+        for (int64_t i = i0; i < i1; i++) {
+            int64_t *a0 = epts.data() + ecount[(size_t)i], *a1 = epts.data() + ecount[(size_t)i + 1];
+            if (a0 == a1) continue;
+            a1[-2] = struct_pts[struct_off[i]];
+            a1[-1] = struct_pts[struct_off[i + 1] - 1];
+            std::sort(a0, a1);
+            ecnt[(size_t)i] = std::unique(a0, a1) - a0;
+        }
+    });
+    elem_off[0] = 0;
+    for (int64_t i = 0; i < n_cnts; i++) elem_off[i + 1] = elem_off[i] + ecnt[(size_t)i];
+    if (elem_pts)
+        parallel([&](int t) {
+            const int64_t i0 = n_cnts * t / nth, i1 = n_cnts * (t + 1) / nth;
+            for (int64_t i = i0; i < i1; i++)
+                std::copy(epts.data() + ecount[(size_t)i],
+                          epts.data() + ecount[(size_t)i] + ecnt[(size_t)i], elem_pts + elem_off[i]);
+        });
     const auto t_end = clk::now();
     if (stats) {
         auto ms = [](clk::time_point a, clk::time_point b) {

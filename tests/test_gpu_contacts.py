@@ -123,6 +123,16 @@ def test_bad_input_is_rejected(ctx):
         device(ctx, bad, np.array([0.005]), [np.arange(4)], 0.5)
 
 
+def test_large_network_matches_the_oracle(ctx):
+    """0.59 M points: enough entries for the compression to run on several host threads (it
+    splits above 65 536 entries); the result is still the oracle's, exactly."""
+    xyz, radii, structure = th.wavy_tubes(11, 10_000, 2.0, n_min=40, n_max=120)
+    got = device(ctx, xyz, radii, structure, 2.0)
+    want = th.run(xyz, radii, structure, 2.0)
+    assert len(want["contacts"]) > 3 * 65536
+    compare(got, want)
+
+
 def test_large_network_labels_follow_junctions(ctx):
     """2 um cube at ~6 % tubes (about 0.3 M points): tubes share a label exactly when the
     junctions connect them, labels run 0 .. n-1 in first-appearance order of their tubes'
