@@ -29,12 +29,15 @@ def test_dea_header_symbols_exported():
 
 
 def test_all_headers_exported():
-    libs = {"cicy_dea.h": "libcicy_dea.so"}
-    for h in os.listdir(os.path.join(ROOT, "include")):
-        if h.endswith(".h") and h in libs:
-            lib = ctypes.CDLL(os.path.join(ROOT, "cicy_geom_amd", libs[h]))
-            for n in declared(h):
-                assert hasattr(lib, n), (h, n)
+    libs = {"cicy_dea.h": "libcicy_dea.so", "cicy_netio.h": "libcicy_net.so"}
+    headers = [h for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h")]
+    assert sorted(headers) == sorted(libs)  # every C header is checked against its library
+    for h in headers:
+        lib = ctypes.CDLL(os.path.join(ROOT, "cicy_geom_amd", libs[h]))
+        names = declared(h)
+        assert names, h
+        for n in names:
+            assert hasattr(lib, n), (h, n)
 
 
 @pytest.mark.parametrize("fmt", [dea.FORMAT_SSS, dea.FORMAT_BAND])
