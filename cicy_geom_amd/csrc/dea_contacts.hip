@@ -445,27 +445,53 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
             set_error(ctx, "cicy_cnt_contacts: window, grid or overlap value %d invalid", k);
             return 0;
         }
+    // host threads for the O(points) checks and copies (up to 16)
+    auto run_threads = [](int nt, auto &&fn) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(fn, t);
+        fn(0);
+        for (auto &x : th) x.join();
+    };
+    const int64_t n_struct = struct_off[n_cnts];
+    const int hth = (int)std::max<long long>(
+        1, std::min<long long>({16, (long long)std::max(1u, std::thread::hardware_concurrency()),
+                                (long long)(n_struct / (1 << 18)) + 1}));
     // the tube of every point (the reference's Point_3D::flag); a point may be in one tube
     std::vector<int> flag((size_t)std::max<int64_t>(n_points, 1), -1);
     if (struct_off[0] != 0) {
         set_error(ctx, "cicy_cnt_contacts: structure offsets must start at 0");
         return 0;
     }
-    for (int64_t i = 0; i < n_cnts; i++) {
+    for (int64_t i = 0; i < n_cnts; i++)
         if (struct_off[i + 1] < struct_off[i]) {
             set_error(ctx, "cicy_cnt_contacts: structure offsets decrease at tube %lld",
                       (long long)i);
             return 0;
         }
-        for (int64_t q = struct_off[i]; q < struct_off[i + 1]; q++) {
-            const int64_t P = struct_pts[q];
-            if (P < 0 || P >= n_points || flag[(size_t)P] != -1) {
+    {
+        std::vector<int64_t> bad_pt((size_t)hth, -1), bad_tube((size_t)hth, -1);
+        run_threads(hth, [&](int t) {
+            const int64_t i0 = n_cnts * t / hth, i1 = n_cnts * (t + 1) / hth;
+            for (int64_t i = i0; i < i1; i++)
+                for (int64_t q = struct_off[i]; q < struct_off[i + 1]; q++) {
+                    const int64_t P = struct_pts[q];
+                    int expected = -1;
+                    if (P < 0 || P >= n_points ||
+                        !__atomic_compare_exchange_n(&flag[(size_t)P], &expected, (int)i, false,
+                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                        bad_pt[(size_t)t] = P;
+                        bad_tube[(size_t)t] = i;
+                        return;
+                    }
+                }
+        });
+        for (int t = 0; t < hth; t++)
+            if (bad_tube[(size_t)t] >= 0) {
                 set_error(ctx, "cicy_cnt_contacts: point %lld of tube %lld is out of range "
-                          "or in two tubes", (long long)P, (long long)i);
+                          "or in two tubes", (long long)bad_pt[(size_t)t],
+                          (long long)bad_tube[(size_t)t]);
                 return 0;
             }
-            flag[(size_t)P] = (int)i;
-        }
     }
     const Grid g = make_grid(geom);
     const double tot_d = (double)g.n[0] * g.n[1] * g.n[2];
@@ -474,7 +500,7 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
         return 0;
     }
     // the visiting order of Fill_sectioned_domain_cnts (Contact_grid.cpp:104-161)
-    int64_t nvis = 0;
+    std::vector<int64_t> voff((size_t)n_inside + 1, 0);
     for (int64_t i = 0; i < n_inside; i++) {
         const int64_t c = cnts_inside[i];
         if (c < 0 || c >= n_cnts) {
@@ -482,16 +508,17 @@ extern "C" int cicy_cnt_contacts(cicy_ctx *ctx, int64_t n_points, const double *
                       (long long)i, (long long)c);
             return 0;
         }
-        nvis += struct_off[c + 1] - struct_off[c];
+        voff[(size_t)i + 1] = voff[(size_t)i] + struct_off[c + 1] - struct_off[c];
     }
-    std::vector<int> vis((size_t)nvis);
-    {
-        int *w = vis.data();
-        for (int64_t i = 0; i < n_inside; i++) {
+    std::vector<int> vis((size_t)voff[(size_t)n_inside]);
+    run_threads(hth, [&](int t) {
+        const int64_t i0 = n_inside * t / hth, i1 = n_inside * (t + 1) / hth;
+        for (int64_t i = i0; i < i1; i++) {
             const int64_t c = cnts_inside[i];
-            w = std::copy(struct_pts + struct_off[c], struct_pts + struct_off[c + 1], w);
+            std::copy(struct_pts + struct_off[c], struct_pts + struct_off[c + 1],
+                      vis.data() + voff[(size_t)i]);
         }
-    }
+    });
     const long long nv = (long long)vis.size();
     if (4 * nv >= (1LL << 31) || nv >= (1LL << 30)) {
         set_error(ctx, "cicy_cnt_contacts: %lld visits (at most 2^29)", nv);
