@@ -2,7 +2,8 @@
 
     python -m cicy_geom_amd.build          # both libraries
     libcicy_dea.so  <- csrc/dea_solver.hip csrc/dea_mixed.hip ... (hipcc, gfx950)
-    libcicy_net.so  <- csrc/netgen.cpp csrc/gnp_geom.cpp csrc/netio.cpp (g++, host only)
+    libcicy_net.so  <- csrc/netgen.cpp csrc/gnp_geom.cpp csrc/netio.cpp csrc/percolation.cpp
+                     (g++, host only)
 
 Every HIP source is compiled with packed-FP32 instruction generation disabled
 (-target-feature -packed-fp32-ops).
@@ -37,7 +38,7 @@ def build_net(out_dir: str = HERE) -> str:
     out = os.path.join(out_dir, "libcicy_net.so")
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
           f"-I{INCLUDE}", os.path.join(CSRC, "netgen.cpp"), os.path.join(CSRC, "gnp_geom.cpp"),
-          os.path.join(CSRC, "netio.cpp"), "-o", out])
+          os.path.join(CSRC, "netio.cpp"), os.path.join(CSRC, "percolation.cpp"), "-o", out])
     return out
 
 

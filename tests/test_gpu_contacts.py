@@ -156,3 +156,24 @@ def test_large_network_labels_follow_junctions(ctx):
     assert (got["junction_d"] >= th.VDW).all() and (got["junction_d"] <= th.TUNNEL).all()
     print({k: round(v, 3) for k, v in got["stats"].items()}, "labels", got["n_labels"],
           "junctions", len(j), flush=True)
+
+
+def test_percolation_from_device_labels(ctx):
+    """Contact search on the device, then cicy_cnt_percolation: the same clusters, families
+    and junction groups as the oracle chain (oracle.hk, then oracle.percolation)."""
+    from cicy_geom_amd import clusters
+    from oracle import percolation as op
+    import test_percolation as tp
+    xyz, radii, structure = th.wavy_tubes(21, 700, 0.5, n_min=30, n_max=90)
+    bl = tp.boundary_lists(xyz, structure, 0.5, 0.011)
+    got = device(ctx, xyz, radii, structure, 0.5)
+    perc = clusters.cnt_percolation(got["labels"], got["n_labels"], bl, got["junctions"])
+    want = th.run(xyz, radii, structure, 0.5)
+    tmp, iso, pl, fam, cl, groups = op.cnt_percolation(
+        want["labels"], want["n_labels"], bl, [j[:4] for j in want["junctions"]],
+        list(range(len(structure))))
+    assert perc["n_perc"] == len(pl) > 0
+    assert [int(perc["family"][L]) for L in sorted(pl)] == fam
+    assert [c.tolist() for c in perc["clusters"]] == tmp
+    jc = perc["junction_cluster"]
+    assert [list(np.nonzero(jc == k)[0]) for k in range(len(groups))] == groups
