@@ -7,6 +7,8 @@
 // in dea_kernels.h.
 #include "dea_internal.h"
 
+#include <thread>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -148,47 +150,84 @@ bool build_blocked(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr
     h.nblocks = n > 0 ? (int)((n + rb - 1) / rb) : 0;
     h.rp.assign((size_t)n + 1, 0);
     h.col.resize((size_t)nnz);
-    h.val.assign(values, values + nnz);
-    h.diag.assign(diag, diag + n);
-    std::vector<int> ucount((size_t)n + 1, 0);
-    for (int64_t r = 0; r < n; r++) {
-        if (row_ptr[r + 1] < row_ptr[r]) {
-            set_error(ctx, "row_ptr decreases at row %lld", (long long)r);
-            return false;
+    // rows are checked and copied on host threads, each over a range of rows; the entries
+    // whose column lies in another row block (the transposed "remote" part) are collected per
+    // thread in row order and then placed by one stable counting pass, so the layout does not
+    // depend on the thread count
+    struct Remote {
+        int c, r;
+        double v;
+    };
+    const int nth = (int)std::max<int64_t>(
+        1, std::min<int64_t>({16, (int64_t)std::max(1u, std::thread::hardware_concurrency()),
+                              n / 65536 + 1}));
+    std::vector<std::vector<Remote>> rem((size_t)nth);
+    // first error of each thread: row, kind (1 row_ptr decreases, 2 diagonal, 3 column), entry
+    std::vector<int64_t> err_row((size_t)nth, -1), err_j((size_t)nth, 0);
+    std::vector<int> err_kind((size_t)nth, 0);
+    auto work = [&](int t) {
+        const int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
+        for (int64_t r = r0; r < r1; r++) {
+            if (row_ptr[r + 1] < row_ptr[r] || row_ptr[r + 1] > nnz) {
+                err_row[(size_t)t] = r;
+                err_kind[(size_t)t] = 1;
+                return;
+            }
+            if (!(diag[r] != 0.0) || !std::isfinite(diag[r])) {
+                err_row[(size_t)t] = r;
+                err_kind[(size_t)t] = 2;
+                return;
+            }
+            h.rp[(size_t)r + 1] = (int)row_ptr[r + 1];
+            for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; j++) {
+                const int64_t c = col_ind[j];
+                if (c < 0 || c >= r) {
+                    err_row[(size_t)t] = r;
+                    err_kind[(size_t)t] = 3;
+                    err_j[(size_t)t] = j;
+                    return;
+                }
+                h.col[(size_t)j] = (int)c;
+                if (c / rb != r / rb) rem[(size_t)t].push_back({(int)c, (int)r, values[j]});
+            }
         }
-        if (!(diag[r] != 0.0) || !std::isfinite(diag[r])) {
+    };
+    {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nth; t++) th.emplace_back(work, t);
+        work(0);
+        for (auto &x : th) x.join();
+    }
+    for (int t = 0; t < nth; t++) {
+        const int64_t r = err_row[(size_t)t];
+        if (r < 0) continue;
+        if (err_kind[(size_t)t] == 1)
+            set_error(ctx, "row_ptr decreases or exceeds nnz at row %lld", (long long)r);
+        else if (err_kind[(size_t)t] == 2)
             set_error(ctx, "zero or non-finite diagonal at row %lld (floating node)",
                       (long long)r);
-            return false;
-        }
-        h.rp[r + 1] = (int)row_ptr[r + 1];
-        for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; j++) {
-            int64_t c = col_ind[j];
-            if (c < 0 || c >= r) {
-                set_error(ctx, "col_ind[%lld]=%lld is not strictly below row %lld", (long long)j,
-                          (long long)c, (long long)r);
-                return false;
-            }
-            h.col[j] = (int)c;
-            if (c / rb != r / rb) ucount[c + 1]++;
-        }
+        else
+            set_error(ctx, "col_ind[%lld]=%lld is not strictly below row %lld",
+                      (long long)err_j[(size_t)t], (long long)col_ind[err_j[(size_t)t]],
+                      (long long)r);
+        return false;
     }
+    h.val.assign(values, values + nnz);
+    h.diag.assign(diag, diag + n);
     h.urp.assign((size_t)n + 1, 0);
-    for (int64_t i = 0; i < n; i++) h.urp[i + 1] = h.urp[i] + ucount[i + 1];
-    h.nremote = n > 0 ? h.urp[n] : 0;
+    for (const auto &v : rem)
+        for (const Remote &e : v) h.urp[(size_t)e.c + 1]++;
+    for (int64_t i = 0; i < n; i++) h.urp[(size_t)i + 1] += h.urp[(size_t)i];
+    h.nremote = n > 0 ? h.urp[(size_t)n] : 0;
     h.ucol.resize((size_t)h.nremote);
     h.uval.resize((size_t)h.nremote);
     std::vector<int> fill(h.urp.begin(), h.urp.end() - (n > 0 ? 1 : 0));
-    for (int64_t r = 0; r < n; r++) {
-        for (int64_t j = row_ptr[r]; j < row_ptr[r + 1]; j++) {
-            int64_t c = col_ind[j];
-            if (c / rb != r / rb) {
-                int at = fill[c]++;
-                h.ucol[at] = (int)r;
-                h.uval[at] = values[j];
-            }
+    for (const auto &v : rem)
+        for (const Remote &e : v) {
+            const int at = fill[(size_t)e.c]++;
+            h.ucol[(size_t)at] = e.r;
+            h.uval[(size_t)at] = e.v;
         }
-    }
     return true;
 }
 
