@@ -3,7 +3,7 @@
 //
 // Not a reference component: the reference's warm start is the linear ramp of
 // Initial_guess_for_CG (Direct_Electrifying.cpp:1396-1470).  The network predicts a
-// correction of that ramp; zero-guess CG stays the fallback and the judge of its error.
+// correction of that ramp; zero-guess CG stays the fallback and the yardstick of its error.
 //
 // Network (H = 32 channels, F = 4 node features, 3 layers), weights stored in BF16,
 // activations and accumulation in FP32:

@@ -1,4 +1,4 @@
-"""Train the learned warm start on solves the solver itself generates, then judge it.
+"""Train the learned warm start on solves the solver itself generates, then assess it.
 
     python -m cicy_geom_amd.train_warmstart --n-train 64 --epochs 40 --out weights.npz
 
@@ -10,7 +10,7 @@ to condense).  Training runs on the device (forward, backward and Adam kernels).
 
 Judging: held-out realisations (seeds 4_500_000 + k) are solved to ||b - Ax|| <= 1e-10 ||b||
 with the plain FP64 PCG and with the condensed solve, each from the zero guess (the
-fallback and judge), from the reference's ramp and from the learned guess; the iterations
+fallback and yardstick), from the reference's ramp and from the learned guess; the iterations
 of each are reported.
 """
 from __future__ import annotations
@@ -70,7 +70,7 @@ def solve_targets(ctx, items, tol=1e-12):
     return xs, ms
 
 
-def judge(ctx, model, items, methods=(dea.METHOD_PCG_F64, dea.METHOD_CONDENSED), tol=1e-10):
+def assess(ctx, model, items, methods=(dea.METHOD_PCG_F64, dea.METHOD_CONDENSED), tol=1e-10):
     """Iterations (and solve ms) of each method from the zero guess, the reference's ramp
     and the learned guess, to ||b - A x|| <= tol ||b||."""
     rows = []
@@ -129,7 +129,7 @@ def main(argv=None):
     for g in graphs:
         g.close()
     test = systems_for(4_500_000, args.n_test, args.min_cnt, args.max_cnt, 2)
-    rows = judge(ctx, model, test)
+    rows = assess(ctx, model, test)
     summary = {"train_systems": len(graphs), "epochs": args.epochs, "train_seconds": train_s,
                "loss_first": hist[0], "loss_last": hist[-1]}
     for key in rows[0]:
@@ -137,15 +137,15 @@ def main(argv=None):
             continue
         summary[f"iters {key}"] = int(sum(r[key][0] for r in rows))
     for r in rows:
-        log("[judge] N=%d " % r["n"] + ", ".join(f"{k} {v[0]} it" for k, v in r.items()
+        log("[assess] N=%d " % r["n"] + ", ".join(f"{k} {v[0]} it" for k, v in r.items()
                                                  if k != "n"))
     for name in ("pcg_f64", "condensed"):
         z, l = summary.get(f"iters {name}/zero"), summary.get(f"iters {name}/learned")
         if z:
             summary[f"iters_saved_{name}"] = 1.0 - l / z
-    log("[judge] " + json.dumps(summary))
+    log("[assess] " + json.dumps(summary))
     np.savez(args.out, theta=model.params, loss=np.array(hist),
-             summary=json.dumps(summary), judge=json.dumps(rows))
+             summary=json.dumps(summary), assessment=json.dumps(rows))
     print(json.dumps(summary))
     model.close()
     ctx.close()

@@ -5,7 +5,7 @@ Jacobi-normalised log-conductance edges, 32 channels, BF16 weights, FP32 accumul
 predicts a correction of the reference's linear ramp guess
 (Initial_guess_for_CG, Direct_Electrifying.cpp:1396-1470).  It is trained on the device
 (forward, backward and Adam are HIP kernels) on solves that the solver itself produces.
-Zero-guess CG is the fallback and the judge: ``warm_start_gain`` solves every held-out
+Zero-guess CG is the fallback and the yardstick: ``warm_start_gain`` solves every held-out
 system both ways and reports the iterations saved.
 """
 from __future__ import annotations

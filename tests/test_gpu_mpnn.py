@@ -1,5 +1,5 @@
 """Warm-start network on the GPU: forward/backward against the float64 oracle
-(oracle/mpnn.py), training on solver-generated data, and the warm start judged against
+(oracle/mpnn.py), training on solver-generated data, and the warm start assessed against
 zero-guess CG."""
 import numpy as np
 import pytest
@@ -69,7 +69,7 @@ def test_forward_and_gradients_match_oracle(ctx, c1sys):
     net.close()
 
 
-def test_training_reduces_loss_and_warm_start_is_judged(ctx):
+def test_training_reduces_loss_and_warm_start_is_assessed(ctx):
     train = []
     for k, ncnt in enumerate([400, 600, 800, 500, 700, 900]):
         net = network.generate(4, realisation=100 + k, n_cnt=ncnt)
