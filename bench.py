@@ -174,6 +174,85 @@ def bench_contacts(args):
         "cpu_baseline": cpu}), flush=True)
 
 
+def bench_backbone(args):
+    """Backbone extraction of the configuration's largest x-percolating cluster: the
+    unit-resistor (R_flag = 0) system solved with the block method, then the device scan
+    (zero current and tube runs, cicy_backbone_scan, inputs in host memory) timed over K
+    calls, then the host pruning (cicy_net_backbone) with the scan's results, once (it edits
+    the cluster)."""
+    from cicy_geom_amd import dea, network
+    t0 = time.perf_counter()
+    net = network.generate(args.config)
+    k = max(net.percolating_clusters(0), key=lambda c: net.cluster(c)[0])
+    s0 = net.assemble(k, 0, net.cluster(k)[1])
+    plates = network.plate_row_ranges(net, k, s0)
+    log(f"[bench] backbone: config {args.config}, N={s0.n} built in "
+        f"{time.perf_counter() - t0:.1f}s")
+    ctx = dea.Context(0)
+    sysd = ctx.system(s0.row_ptr, s0.col_ind, s0.values, s0.diag)
+    sysd.set_rhs(s0.b, None, s0.b)
+    sysd.set_blocks(plates)
+    st = sysd.solve(dea.METHOD_CONDENSED_BLOCK, args.tol, 0, dea.REL_B)
+    volt = np.concatenate([s0.vef, sysd.get()[0]])
+    sysd.close()
+    t = time.perf_counter()
+    off, nodes, pa, pb = net.backbone_scan_inputs(k, s0)
+    inputs_s = time.perf_counter() - t
+    for _ in range(args.warmup):
+        ctx.backbone_scan(volt, off, nodes, pa, pb)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        zero, seg = ctx.backbone_scan(volt, off, nodes, pa, pb)
+    scan_ms = 1e3 * (time.perf_counter() - t) / args.steps
+    t = time.perf_counter()
+    vol = net.backbone(k, s0, volt, zero=zero, segments=seg)
+    prune_s = time.perf_counter() - t
+    ctx.close()
+    kept = int((seg[:, 0] >= 0).sum())
+    print(json.dumps({
+        "metric": "backbone_scan_ms", "value": scan_ms, "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"backbone of config {args.config}'s largest cluster, "
+                               f"R_flag=0, N={s0.n}", "tubes": int(len(off) - 1),
+                   "element_points": int(off[-1]), "pairs": int(len(pa))},
+        "unit_solve": {"iters": int(st.iters), "ms": st.solve_ms,
+                       "true_rel_res": st.true_rel_res},
+        "scan_inputs_s": inputs_s, "host_pruning_s": prune_s, "zero_current": zero,
+        "tubes_with_backbone": kept, "volumes": {k_: float(v) for k_, v in vol.items()}}),
+        flush=True)
+
+
+def bench_netio(args):
+    """Network files of the contact-search workload (5.5 M points in a 4 um cube): write and
+    read cnt_struct/cnt_coordinates as csv (6 digits) and dat; host only."""
+    import tempfile
+    from cicy_geom_amd import netio, network
+    box = args.contacts_box
+    xyz, radii, structure = network.wavy_tubes(11, int(10_000 * (box / 2.0) ** 3), box,
+                                               n_min=40, n_max=120)
+    npts = [len(t) for t in structure]
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for fmt in ("csv", "dat"):
+            t = time.perf_counter()
+            netio.write_network(d, fmt, [0, 0, 0], [box] * 3, npts, radii, xyz, prec_cnt=6)
+            w = time.perf_counter() - t
+            size = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d)
+                       if f.startswith("cnt_") and f.endswith(fmt))
+            t = time.perf_counter()
+            r = netio.read_cnts(d, fmt)
+            rd = time.perf_counter() - t
+            assert len(r["xyz"]) == len(xyz)
+            out[fmt] = {"write_s": w, "read_s": rd, "bytes": size,
+                        "read_points_per_s": len(xyz) / rd, "read_MB_per_s": size / rd / 1e6}
+    print(json.dumps({
+        "metric": "network_file_read_points_per_s", "value": out["dat"]["read_points_per_s"],
+        "unit": "points/s", "higher_is_better": True, "data": "synthetic",
+        "config": {"workload": f"cnt_struct + cnt_coordinates, {len(xyz)} points, "
+                               f"{len(structure)} tubes"}, "formats": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,9 +279,12 @@ def main():
     ap.add_argument("--profile-spmv", action="store_true",
                     help="only run the SpMV kernels (both precisions) and a 1 GiB reference "
                          "copy, for hardware-counter runs")
-    ap.add_argument("--workload", default="dea", choices=["dea", "contacts"],
+    ap.add_argument("--workload", default="dea",
+                    choices=["dea", "contacts", "backbone", "netio"],
                     help="dea: the conductance solve (the default line); contacts: the tube "
-                         "contact search and cluster labels on synthetic wavy tubes")
+                         "contact search and cluster labels on synthetic wavy tubes; backbone: "
+                         "the backbone scan and pruning of the configuration's largest "
+                         "cluster; netio: reading and writing the network files")
     ap.add_argument("--contacts-box", type=float, default=4.0,
                     help="cube side (um) of the contact-search workload")
     args = ap.parse_args()
@@ -215,6 +297,12 @@ def main():
 
     if args.workload == "contacts":
         bench_contacts(args)
+        return
+    if args.workload == "backbone":
+        bench_backbone(args)
+        return
+    if args.workload == "netio":
+        bench_netio(args)
         return
     t0 = time.perf_counter()
     net = network.generate(args.config)
