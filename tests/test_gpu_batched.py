@@ -90,7 +90,7 @@ def test_batch_matches_truth_and_single_solves(ctx, systems, truths, method, fmt
     B.close()
 
 
-def test_batch_reference_stopping_rule_with_warm_start(ctx, systems):
+def test_batch_reference_stopping_rule_with_warm_start(ctx, systems, truths):
     """REL_R0 with the ramp guess: each system stops as Solve_DEA_equations_CG_SSS would."""
     sub = systems[:4]
     B = ctx.batch([(s.row_ptr, s.col_ind, s.values, s.diag) for s in sub])
@@ -103,6 +103,10 @@ def test_batch_reference_stopping_rule_with_warm_start(ctx, systems):
         x, r = B.get(i)
         assert stats[i].status == dea.CICY_OK
         assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(s.r0)
-        assert np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"]) < 1e-8
+        # the two runs stop where their recursions cross the same residual; their distance
+        # is held to the scale of the reference's own error at that residual
+        truth, err_ref = truths[i]
+        dist = np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"])
+        assert dist < max(1e-8, 100.0 * err_ref), (dist, err_ref)
         assert abs(stats[i].iters - ref["iters"]) <= max(5, ref["iters"] // 20)
     B.close()

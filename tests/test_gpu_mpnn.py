@@ -109,6 +109,10 @@ def test_training_reduces_loss_and_warm_start_is_assessed(ctx):
     # both solutions sit within CG's error at this residual of the FP64 truth
     import oracle
     xt, _, _ = oracle.cg_truth(s.row_ptr, s.col_ind, s.values, s.diag, s.b, tol=1e-13)
+    ref = oracle.pcg_sss(s.nodes, s.reserved, 1e-10, s.row_ptr, s.col_ind, s.values, s.diag,
+                         s.b, s.vef, np.zeros(s.n))
+    err_ref = np.linalg.norm(ref["x"] - xt) / np.linalg.norm(xt)
     for xs in (x0sol, x1sol):
-        assert np.linalg.norm(xs - xt) / np.linalg.norm(xt) < 1e-7
+        err = np.linalg.norm(xs - xt) / np.linalg.norm(xt)
+        assert err < max(1e-7, 100.0 * err_ref), (err, err_ref)
     sysd.close()
