@@ -43,11 +43,22 @@ def build_net(out_dir: str = HERE) -> str:
 
 
 def build_dea(out_dir: str = HERE) -> str:
+    """Each HIP source is compiled to an object on its own process (in parallel), then the
+    objects are linked into the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+    import tempfile
     out = os.path.join(out_dir, "libcicy_dea.so")
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, *srcs, "-o", tmp])
-    os.replace(tmp, out)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="cicy_build_") as tmpdir:
+        objs = [os.path.join(tmpdir, os.path.basename(s) + ".o") for s in srcs]
+        jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+        with ThreadPoolExecutor(jobs) as ex:
+            list(ex.map(lambda so: _run([HIPCC, *cflags, "-c", so[0], "-o", so[1]]),
+                        zip(srcs, objs)))
+        tmp = out + ".tmp"
+        _run([HIPCC, f"--offload-arch={ARCH}", "-fPIC", "-shared", *objs, "-o", tmp])
+        os.replace(tmp, out)
     return out
 
 

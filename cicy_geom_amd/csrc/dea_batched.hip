@@ -32,7 +32,7 @@ struct cicy_batch {
     double *val = nullptr, *diag = nullptr, *uval = nullptr;
     float *val32 = nullptr, *uval32 = nullptr;
     int format = kFormatSss, nx = 0;
-    int *xblk = nullptr, *xrow = nullptr, *xnbr = nullptr;
+    int *xblk = nullptr, *xrp = nullptr, *xnbr = nullptr;
     double *band = nullptr, *xval = nullptr;
     float *band32 = nullptr, *xval32 = nullptr;
     int *blk_sys = nullptr, *blk_row0 = nullptr, *sys_row0 = nullptr, *sys_blk0 = nullptr;
@@ -90,7 +90,7 @@ DevMat<double> m64(const cicy_batch *B) {
     m.format = B->format;
     m.sss = SssDev<double>{B->n, B->nblocks, B->rp, B->col, B->val, B->diag, B->urp, B->ucol,
                            B->uval};
-    m.band = BandDev<double>{B->n, B->nblocks, B->band, B->diag, B->xblk, B->xrow, B->xnbr,
+    m.band = BandDev<double>{B->n, B->nblocks, B->band, B->diag, B->xblk, B->xrp, B->xnbr,
                              B->xval};
     return m;
 }
@@ -100,7 +100,7 @@ DevMat<float> m32(const cicy_batch *B) {
     m.format = B->format;
     m.sss = SssDev<float>{B->n, B->nblocks, B->rp, B->col, B->val32, nullptr, B->urp, B->ucol,
                           B->uval32};
-    m.band = BandDev<float>{B->n, B->nblocks, B->band32, nullptr, B->xblk, B->xrow, B->xnbr,
+    m.band = BandDev<float>{B->n, B->nblocks, B->band32, nullptr, B->xblk, B->xrp, B->xnbr,
                             B->xval32};
     return m;
 }
@@ -288,13 +288,8 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
     B->nremote = (int)trem;
     B->bnorm2.assign((size_t)nsys, -1.0);
     // one layout for the whole batch: band when most entries of the batch are sub-diagonal
-    if (ctx->format == kFormatSss || ctx->format == kFormatBand) {
-        B->format = ctx->format;
-    } else {
-        double on = 0.0;
-        for (int s = 0; s < nsys; s++) on += band_fraction(hs[s]) * hs[s].nnz;
-        B->format = (tnnz == 0 || on >= 0.5 * (double)tnnz) ? kFormatBand : kFormatSss;
-    }
+    // (the band layout unless the blocked SSS layout is asked for, as for single systems)
+    B->format = ctx->format == kFormatSss ? kFormatSss : kFormatBand;
     const bool band_fmt = B->format == kFormatBand;
     std::vector<BandHost> bhs(band_fmt ? (size_t)nsys : 0);
     long long tx = 0;
@@ -318,7 +313,8 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
     const size_t nb1 = band_fmt ? 1 : 0;
     std::vector<double> band(nb1 * ((size_t)tn + 1), 0.0), xval(nb1 * (size_t)tx);
     std::vector<float> band32(nb1 * ((size_t)tn + 1), 0.f), xval32(nb1 * (size_t)tx);
-    std::vector<int> xblk(nb1 * ((size_t)tb + 1)), xrow(nb1 * (size_t)tx), xnbr(nb1 * (size_t)tx);
+    std::vector<int> xblk(nb1 * ((size_t)tb + 1)), xrp(nb1 * ((size_t)tn + 1)),
+        xnbr(nb1 * (size_t)tx);
     int xo = 0;
     B->h_sys_row0.assign((size_t)nsys + 1, 0);
     B->h_sys_blk0.assign((size_t)nsys + 1, 0);
@@ -351,8 +347,8 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
                 band[ro + i] = bh.band[i];
                 band32[ro + i] = a32[i];
             }
+            for (int i = 0; i < h.n; i++) xrp[ro + i] = xo + bh.xrp[i];
             for (int k = 0; k < bh.m; k++) {
-                xrow[xo + k] = ro + bh.xrow[k];
                 xnbr[xo + k] = ro + bh.xnbr[k];
                 xval[xo + k] = bh.xval[k];
                 xval32[xo + k] = x32[k];
@@ -388,12 +384,13 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
              bh2d(B, B->uval32, uval32.data(), (size_t)trem);
     } else {
         xblk[tb] = xo;
+        xrp[tn] = xo;
         ok = ok && balloc(B, &B->band, N + 1) && balloc(B, &B->band32, N + 1) &&
-             balloc(B, &B->xblk, NB + 1) && balloc(B, &B->xrow, (size_t)tx) &&
+             balloc(B, &B->xblk, NB + 1) && balloc(B, &B->xrp, N + 1) &&
              balloc(B, &B->xnbr, (size_t)tx) && balloc(B, &B->xval, (size_t)tx) &&
              balloc(B, &B->xval32, (size_t)tx) && bh2d(B, B->band, band.data(), N + 1) &&
              bh2d(B, B->band32, band32.data(), N + 1) && bh2d(B, B->xblk, xblk.data(), NB + 1) &&
-             bh2d(B, B->xrow, xrow.data(), (size_t)tx) && bh2d(B, B->xnbr, xnbr.data(), (size_t)tx) &&
+             bh2d(B, B->xrp, xrp.data(), N + 1) && bh2d(B, B->xnbr, xnbr.data(), (size_t)tx) &&
              bh2d(B, B->xval, xval.data(), (size_t)tx) &&
              bh2d(B, B->xval32, xval32.data(), (size_t)tx);
     }
@@ -448,7 +445,7 @@ void cicy_batch_destroy(cicy_batch *B) {
     (void)hipStreamSynchronize(B->ctx->stream);
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     void *dev[] = {B->rp, B->col, B->urp, B->ucol, B->val, B->diag, B->uval, B->val32, B->uval32,
-                   B->band, B->band32, B->xblk, B->xrow, B->xnbr, B->xval, B->xval32,
+                   B->band, B->band32, B->xblk, B->xrp, B->xnbr, B->xval, B->xval32,
                    B->blk_sys, B->blk_row0, B->sys_row0, B->sys_blk0, B->active[0],
                    B->active[1], B->x, B->r, B->p0, B->p1, B->q, B->x0, B->r0, B->b,
                    B->stop_ref2, B->rs, B->xs, B->ps0, B->ps1, B->qs, B->partials, B->norms,

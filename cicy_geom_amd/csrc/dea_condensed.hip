@@ -928,9 +928,14 @@ bool solve_condensed(cicy_sys *s, int inner, double tol, long long max_iter, int
     long long total = 0;
     int passes = 0, status = CICY_MAXITER;
     double rel = -1.0, kernel_ms = 0.0;
+    // core iterations allowed per correction pass (default: the remaining cap)
+    long long pass_cap = 0;
+    if (const char *e = getenv("CICY_CORE_PASS_ITERS")) pass_cap = atoll(e);
     while (ok) {
         cicy_solve_stats ist;
-        ok = condensed_correction(s, inner, c->rhs, tol, max_iter - total, ref2, &ist);
+        const long long left = max_iter - total;
+        ok = condensed_correction(s, inner, c->rhs, tol,
+                                  pass_cap > 0 ? std::min(pass_cap, left) : left, ref2, &ist);
         if (!ok) break;
         total += ist.iters;
         kernel_ms += ist.kernel_ms;
@@ -958,6 +963,24 @@ bool solve_condensed(cicy_sys *s, int inner, double tol, long long max_iter, int
         passes++;
         ok = hip_ok(ctx, hipMemcpyAsync(c->rhs, s->r, vb, hipMemcpyDeviceToDevice, st), "copy r");
     }
+    // Correction passes used up short of the tolerance and of the iteration cap (a stalled
+    // core recursion): continue from x with the reference's algorithm, FP64 Jacobi-PCG on
+    // the whole system (s->r = b - A x after the last pass), stopping against the same
+    // reference norm, until it converges or the remaining iterations of the cap
+    // (300 * nodes, Direct_Electrifying.cpp:1559) are spent.
+    long long fallback_iters = 0;
+    if (ok && status == CICY_MAXITER && total < max_iter) {
+        cicy_solve_stats fst;
+        s->ref2_override = ref2;
+        ok = solve_pcg64(s, tol, max_iter - total, rel_to, &fst, /*keep_x=*/true);
+        s->ref2_override = -1.0;
+        if (ok) {
+            fallback_iters = fst.iters;
+            total += fst.iters;
+            status = fst.status;
+            rel = fst.rel_res;
+        }
+    }
     double tr = -1.0;
     if (ok && s->has_b) ok = sys_true_residual(s, s->x, &tr);
     float ms = 0.f;
@@ -971,7 +994,7 @@ bool solve_condensed(cicy_sys *s, int inner, double tol, long long max_iter, int
         stats->rel_res = rel;
         stats->true_rel_res = tr;
         stats->solve_ms = ms;
-        stats->fp64_iters = total;
+        stats->fp64_iters = fallback_iters;
         stats->kernel_ms = kernel_ms;
     }
     return ok;

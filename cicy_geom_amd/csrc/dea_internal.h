@@ -32,11 +32,12 @@ struct BlockedHost {
 };
 
 // Band + remainder layout (BandDev): band[r] = a(r, r-1), the other entries as symmetric
-// half-entries sorted by row, xblk[b] = first half-entry of row block b.
+// half-entries sorted by row, xrp[r] = first half-entry of row r, xblk[b] = first
+// half-entry of row block b.
 struct BandHost {
     int m = 0;
     std::vector<double> band, xval;
-    std::vector<int> xblk, xrow, xnbr;
+    std::vector<int> xblk, xrp, xrow, xnbr;
     std::vector<int> order;  // launch order of the row blocks (BandDev::order)
 };
 
@@ -103,7 +104,7 @@ struct cicy_sys {
     double *val = nullptr, *diag = nullptr, *uval = nullptr;
     // band + remainder (format kFormatBand)
     int nx = 0;
-    int *xblk = nullptr, *xrow = nullptr, *xnbr = nullptr;
+    int *xblk = nullptr, *xrp = nullptr, *xnbr = nullptr;
     double *band = nullptr, *xval = nullptr;
     float *band32 = nullptr, *xval32 = nullptr;
     int *xorder = nullptr;  // launch order of the row blocks

@@ -79,14 +79,15 @@ struct SssDev {
 
 // "Band + remainder" layout of the same SSS matrix: band[r] = a(r, r-1) (0 when row r
 // has no entry in column r-1; band[0] = band[n] = 0) plus every other stored entry as two
-// half-entries (row, nbr, val), sorted by row and indexed per row block by xblk.
+// half-entries (nbr, val), one in each of its rows, sorted by row and indexed per row by
+// xrp (per row block by xblk).
 template <typename T>
 struct BandDev {
     int n, nblocks;
     const T *__restrict__ band;   // [n+1]
     const T *__restrict__ diag;   // nullptr: unit diagonal
-    const int *__restrict__ xblk; // [nblocks+1]
-    const int *__restrict__ xrow; // [m]
+    const int *__restrict__ xblk; // [nblocks+1] first half-entry of each row block
+    const int *__restrict__ xrp;  // [n+1] first half-entry of each row
     const int *__restrict__ xnbr; // [m]
     const T *__restrict__ xval;   // [m]
     // [nblocks] row block of each work-group (heaviest blocks first); nullptr: row order
@@ -161,28 +162,35 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *lds8) {
     }
 }
 
-// Reduction scratch: accumulators and arrival counters at the slot of each group's first
-// work-group (groups of kGroup work-groups), and the system's total at its first slot.
+// Reduction scratch: one slot of NV partial sums per work-group (gpart), one per group of
+// kGroup consecutive work-groups of a system (part, at the slot of the group's first
+// work-group), and the group arrival counters (gcount, same indexing).
 constexpr int kGroup = 64;
 struct RedBuf {
-    double *part;            // [2 * blocks] system totals (slot of the system's first block)
-    double *gpart;           // [2 * blocks] group accumulators (slot of the group's first block)
+    double *part;            // [2 * blocks] group sums
+    double *gpart;           // [2 * blocks] work-group sums
     unsigned int *gcount;    // [blocks]     group arrival counters
 };
 
-// Makes `v` a register operand of an empty asm statement with a memory clobber: the value
-// must have returned (s_waitcnt) before any later memory operation is issued.
-__device__ __forceinline__ void depend_on(double v) { asm volatile("" ::"v"(v) : "memory"); }
+__device__ __forceinline__ double ld_dev(const double *p) {
+    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
-// Reduction of NV values over the work-groups of one system without cache-wide fences.
-// Thread 0 adds the work-group's sums into its group's accumulators with returning
-// device-scope FP64 atomics; once they have returned it arrives on the group counter.  The
-// last arriver of a group takes the group sums (atomicExch, which also resets them), adds
-// them into the system totals and arrives on `top`; the last group leader takes the totals.
-// Every step is a device-scope atomic, performed at the point of coherence, so no L2
-// write-back/invalidate (which an agent-scope __threadfence costs on a multi-XCD part) is
-// needed.  Returns true in the final work-group with out[] valid in thread 0.  The FP64
-// atomic adds make the last bits of the sums depend on arrival order.
+// Reduction of NV values over the work-groups of one system, in a fixed order and without
+// cache-wide fences.  Thread 0 stores the work-group's sums into its slot with
+// device-coherent (agent-scope) stores, waits for them to complete, and arrives on its
+// group's counter.  The last arriver of a group sums the group's slots in work-group order
+// (lane j of wave 0 reads slot j, then a fixed shuffle tree), stores the group sum, and
+// arrives on `top`; the last group sums the group sums in group order.  Each partial is
+// summed in the same order whichever work-group arrives last, so the totals are bit-identical
+// from run to run.  Stores and loads of shared slots are performed at the point of
+// coherence (the eight XCDs' L2 caches are not coherent with each other), so no L2
+// write-back/invalidate fence is needed.  Returns true in the final work-group with out[]
+// valid in thread 0.
 template <int NV>
 __device__ __forceinline__ bool reduce_sys(double (&v)[NV], const Blk &k, const RedBuf &rb,
                                            unsigned int *top, double *red, int *flag,
@@ -190,33 +198,54 @@ __device__ __forceinline__ bool reduce_sys(double (&v)[NV], const Blk &k, const 
     double s[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) s[i] = block_sum(v[i], red);
+    const int g0 = k.first + ((k.b - k.first) / kGroup) * kGroup;
+    const int gsz = min(kGroup, k.first + k.count - g0);
+    const int ng = (k.count + kGroup - 1) / kGroup;
     if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) st_dev(&rb.gpart[(size_t)k.b * NV + i], s[i]);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        *flag = atomicAdd(&rb.gcount[g0], 1u) == (unsigned)(gsz - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    const int group_last = *flag;
+    __syncthreads();  // every thread has read the flag before it is rewritten below
+    if (!group_last) return false;
+    if (threadIdx.x < kWave) {
+        const int lane = threadIdx.x;
+        double g[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            g[i] = lane < gsz ? ld_dev(&rb.gpart[(size_t)(g0 + lane) * NV + i]) : 0.0;
+            g[i] = wave_sum(g[i]);
+        }
         int last = 0;
-        const int g0 = k.first + ((k.b - k.first) / kGroup) * kGroup;
-        const int gsz = min(kGroup, k.first + k.count - g0);
-        double dep = 0.0;
+        if (lane == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; i++) dep += atomicAdd(&rb.gpart[(size_t)g0 * NV + i], s[i]);
-        depend_on(dep);
-        if (atomicAdd(&rb.gcount[g0], 1u) == (unsigned)(gsz - 1)) {
-            double g[NV];
-            dep = 0.0;
+            for (int i = 0; i < NV; i++) st_dev(&rb.part[(size_t)g0 * NV + i], g[i]);
+            __hip_atomic_store(&rb.gcount[g0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            last = atomicAdd(top, 1u) == (unsigned)(ng - 1) ? 1 : 0;
+        }
+        last = __shfl(last, 0, kWave);
+        if (last) {
+            double t[NV];
 #pragma unroll
-            for (int i = 0; i < NV; i++) g[i] = atomicExch(&rb.gpart[(size_t)g0 * NV + i], 0.0);
-            (void)atomicExch(&rb.gcount[g0], 0u);
-#pragma unroll
-            for (int i = 0; i < NV; i++) dep += atomicAdd(&rb.part[(size_t)k.first * NV + i], g[i]);
-            depend_on(dep);
-            const int ng = (k.count + kGroup - 1) / kGroup;
-            if (atomicAdd(top, 1u) == (unsigned)(ng - 1)) {
+            for (int i = 0; i < NV; i++) t[i] = 0.0;
+            for (int gi = lane; gi < ng; gi += kWave) {
 #pragma unroll
                 for (int i = 0; i < NV; i++)
-                    out[i] = atomicExch(&rb.part[(size_t)k.first * NV + i], 0.0);
-                (void)atomicExch(top, 0u);
-                last = 1;
+                    t[i] += ld_dev(&rb.part[(size_t)(k.first + gi * kGroup) * NV + i]);
+            }
+#pragma unroll
+            for (int i = 0; i < NV; i++) t[i] = wave_sum(t[i]);
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < NV; i++) out[i] = t[i];
+                __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
-        *flag = last;
+        if (lane == 0) *flag = last;
     }
     __syncthreads();
     return *flag != 0;
@@ -318,8 +347,10 @@ __global__ __launch_bounds__(kThreads) void k_apply(SssDev<T> s, BatchDev bd,
 
 // q = A p over one row block in the band + remainder layout.  The band part is a
 // three-point stencil on LDS (p[r-1], p[r], p[r+1] with a halo row on each side, band[r],
-// band[r+1]); the remainder half-entries of the block are added with LDS atomics.  Same
-// PCG/SPMV/UNIT/BATCH/guard semantics as k_apply.
+// band[r+1]); each row's remainder half-entries (xrp[r]..xrp[r+1]) are then added by the
+// thread that owns the row, in stored order, four loads in flight.  Every sum has a fixed
+// order, so the product is bit-identical from run to run.  Same PCG/SPMV/UNIT/BATCH/guard
+// semantics as k_apply.
 template <typename T, int RB, bool PCG, bool UNIT, bool BATCH>
 __global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev bd,
                                                          const T *__restrict__ src,
@@ -342,7 +373,6 @@ __global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev 
     if (guard && !guard[k.sys].need_update) return;
     __shared__ T sp[RB + 2];
     __shared__ T sb[RB + 1];
-    __shared__ T sy[RB];
     __shared__ double red[8];
     __shared__ int flag;
     const int r0 = k.r0;
@@ -363,15 +393,19 @@ __global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev 
         if (PCG && t >= 1 && t <= nrows) pnew[i] = p;
     }
     for (int t = threadIdx.x; t < nrows + 1; t += kThreads) sb[t] = s.band[r0 + t];
-    // the diagonal of this thread's rows, loaded together with the staging above
+    // the diagonal and remainder range of this thread's rows, loaded with the staging above
     constexpr int kRows = (RB + kThreads - 1) / kThreads;
     T dg[kRows];
+    int xb[kRows], xe[kRows];
 #pragma unroll
     for (int j = 0; j < kRows; j++) {
         const int t = (int)threadIdx.x + j * kThreads;
         dg[j] = (!UNIT && t < nrows) ? s.diag[r0 + t] : (T)1;
+        xb[j] = t < nrows ? s.xrp[r0 + t] : 0;
+        xe[j] = t < nrows ? s.xrp[r0 + t + 1] : 0;
     }
     __syncthreads();
+    double part = 0.0;
 #pragma unroll
     for (int j = 0; j < kRows; j++) {
         const int t = (int)threadIdx.x + j * kThreads;
@@ -380,38 +414,25 @@ __global__ __launch_bounds__(kThreads) void k_apply_band(BandDev<T> s, BatchDev 
             T y = UNIT ? pr : dg[j] * pr;
             y = fma(sb[t], sp[t], y);
             y = fma(sb[t + 1], sp[t + 2], y);
-            sy[t] = y;
+            for (int e = xb[j]; e < xe[j]; e += 4) {
+                int c[4];
+                T xv[4], pc[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const bool in = e + u < xe[j];
+                    c[u] = in ? s.xnbr[e + u] : r0;
+                    xv[u] = in ? s.xval[e + u] : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    pc[u] = (c[u] >= r0 && c[u] < r0 + nrows) ? sp[c[u] - r0 + 1] : pval(c[u]);
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (e + u < xe[j]) y = fma(xv[u], pc[u], y);
+            }
+            q[r0 + t] = y;
+            if (PCG) part += (double)pr * (double)y;
         }
-    }
-    __syncthreads();
-    // remainder half-entries: four per thread in flight together (indices and values, then
-    // the gathered p, then the LDS adds), so that a block with many of them (plate rows)
-    // waits for a quarter of the dependent load chains
-    const int e1 = s.xblk[k.b + 1];
-    for (int e0 = s.xblk[k.b] + threadIdx.x; e0 < e1; e0 += 4 * kThreads) {
-        int c[4], rw[4];
-        T xv[4], pc[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int e = e0 + u * kThreads;
-            const bool in = e < e1;
-            c[u] = in ? s.xnbr[e] : r0;
-            rw[u] = in ? s.xrow[e] - r0 : 0;
-            xv[u] = in ? s.xval[e] : (T)0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            pc[u] = (c[u] >= r0 && c[u] < r0 + nrows) ? sp[c[u] - r0 + 1] : pval(c[u]);
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (e0 + u * kThreads < e1) atomicAdd(&sy[rw[u]], xv[u] * pc[u]);
-    }
-    __syncthreads();
-    double part = 0.0;
-    for (int t = threadIdx.x; t < nrows; t += kThreads) {
-        const T qv = sy[t];
-        q[r0 + t] = qv;
-        if (PCG) part += (double)sp[t + 1] * (double)qv;
     }
     if (!PCG) return;
     double v[1] = {part}, out[1];

@@ -168,7 +168,9 @@ bool build_blocked(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr
     auto work = [&](int t) {
         const int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
         for (int64_t r = r0; r < r1; r++) {
-            if (row_ptr[r + 1] < row_ptr[r] || row_ptr[r + 1] > nnz) {
+            // every bound a row's entries are read with is checked by the thread that reads
+            // them: row_ptr[r0] is also the end of the previous thread's last row
+            if (row_ptr[r] < 0 || row_ptr[r + 1] < row_ptr[r] || row_ptr[r + 1] > nnz) {
                 err_row[(size_t)t] = r;
                 err_kind[(size_t)t] = 1;
                 return;
@@ -255,6 +257,7 @@ void build_band(const BlockedHost &h, BandHost &b) {
         }
     for (int i = 0; i < h.n; i++) cnt[i + 1] += cnt[i];
     b.m = h.n > 0 ? cnt[h.n] : 0;
+    b.xrp = cnt;
     b.xrow.resize((size_t)b.m);
     b.xnbr.resize((size_t)b.m);
     b.xval.resize((size_t)b.m);
@@ -308,9 +311,13 @@ void scaled_band(const BlockedHost &h, const BandHost &b, std::vector<float> &ba
     }
 }
 
+// The band + remainder layout is the default for every matrix: its products sum each row
+// in a fixed order (bit-identical from run to run).  The blocked SSS layout, whose in-block
+// transposes are added with LDS atomics in arrival order, is used only when asked for.
 int choose_format(const cicy_ctx *ctx, const BlockedHost &h) {
+    (void)h;
     if (ctx->format == kFormatSss || ctx->format == kFormatBand) return ctx->format;
-    return band_fraction(h) >= 0.5 ? kFormatBand : kFormatSss;
+    return kFormatBand;
 }
 
 DevMat<double> mat64(const cicy_sys *s) {
@@ -318,7 +325,7 @@ DevMat<double> mat64(const cicy_sys *s) {
     m.format = s->format;
     m.sss = SssDev<double>{s->n, s->nblocks, s->rp, s->col, s->val, s->diag, s->urp, s->ucol,
                            s->uval};
-    m.band = BandDev<double>{s->n, s->nblocks, s->band, s->diag, s->xblk, s->xrow, s->xnbr,
+    m.band = BandDev<double>{s->n, s->nblocks, s->band, s->diag, s->xblk, s->xrp, s->xnbr,
                              s->xval, s->xorder};
     return m;
 }
@@ -328,7 +335,7 @@ DevMat<float> mat32(const cicy_sys *s) {
     m.format = s->format;
     m.sss = SssDev<float>{s->n, s->nblocks, s->rp, s->col, s->val32, nullptr, s->urp, s->ucol,
                           s->uval32};
-    m.band = BandDev<float>{s->n, s->nblocks, s->band32, nullptr, s->xblk, s->xrow, s->xnbr,
+    m.band = BandDev<float>{s->n, s->nblocks, s->band32, nullptr, s->xblk, s->xrp, s->xnbr,
                             s->xval32, s->xorder};
     return m;
 }
@@ -564,7 +571,9 @@ bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,
         stats->iters = total_iters;
         stats->status = h.status;
         stats->refinements = restarts;
-        stats->rel_res = rr_init > 0 ? std::sqrt(h.rr / rr_init) : 0.0;
+        // relative to the norm the stop was taken against
+        const double rel2 = s->ref2_override >= 0.0 ? s->ref2_override : rr_init;
+        stats->rel_res = rel2 > 0 ? std::sqrt(h.rr / rel2) : 0.0;
         stats->solve_ms = ms;
         stats->true_rel_res = tr;
     }
@@ -712,11 +721,11 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
              dmalloc(ctx, &s->xblk, (size_t)h.nblocks + 1, &tot) &&
              dmalloc(ctx, &s->xorder, std::max<size_t>((size_t)h.nblocks, 1), &tot) &&
              h2d(ctx, s->xorder, bh.order.data(), (size_t)h.nblocks) &&
-             dmalloc(ctx, &s->xrow, (size_t)bh.m, &tot) && dmalloc(ctx, &s->xnbr, (size_t)bh.m, &tot) &&
+             dmalloc(ctx, &s->xrp, nv + 1, &tot) && dmalloc(ctx, &s->xnbr, (size_t)bh.m, &tot) &&
              dmalloc(ctx, &s->xval, (size_t)bh.m, &tot) &&
              h2d(ctx, s->band, bh.band.data(), nv + 1) &&
              h2d(ctx, s->xblk, bh.xblk.data(), (size_t)h.nblocks + 1) &&
-             h2d(ctx, s->xrow, bh.xrow.data(), (size_t)bh.m) &&
+             h2d(ctx, s->xrp, bh.xrp.data(), nv + 1) &&
              h2d(ctx, s->xnbr, bh.xnbr.data(), (size_t)bh.m) &&
              h2d(ctx, s->xval, bh.xval.data(), (size_t)bh.m);
     }
@@ -840,7 +849,7 @@ void cicy_sys_destroy(cicy_sys *s) {
     cicy::condensed_free(s->cond);
     delete s->host;
     void *ptrs[] = {s->rp, s->col, s->val, s->diag, s->urp, s->ucol, s->uval, s->band,
-                    s->xblk, s->xrow, s->xnbr, s->xval, s->xorder, s->x, s->r, s->p0, s->p1, s->q,
+                    s->xblk, s->xrp, s->xnbr, s->xval, s->xorder, s->x, s->r, s->p0, s->p1, s->q,
                     s->x0, s->r0, s->b, s->partials, s->gpart, s->gcount, s->st, s->scal,
                     s->counter2};
     for (void *p : ptrs)

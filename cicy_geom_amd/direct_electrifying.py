@@ -15,7 +15,9 @@ DirectElectrifying.get_voltage_vector       Get_voltage_vector, :1472-1500
 calculate_parallel_resistor                 Electrical_analysis.cpp:355-473 (+ :499-605)
 get_boundaries_from_direction               Electrical_analysis.cpp:477-497
 vector_of_directions                        Electrical_analysis.cpp:292-346
-calculate_resistance                        Calculate_resistances_and_resistivities, :873-932
+calculate_resistances_and_resistivities     Calculate_resistances_and_resistivities, :873-932
+calculate_matrix_resistance                 Calculate_matrix_resistance, :935-958
+calculate_resistivity                       Calculate_resistivity, :960-987
 electrical_resistance_along_each_percolated_direction   :229-289
 =========================================  ================================================
 
@@ -127,19 +129,31 @@ class DirectElectrifying:
             return 0
         self.system = s
         self.LMM_cnts = s.lmm
+        self.LMM_gnps = s.lmm_gnp if s.lmm_gnp is not None else np.zeros(0, np.int64)
         R = s.r0.copy()
         x = s.x0.copy()
         if self.method == dea.METHOD_PCG_F64:
             return self.solve_dea_equations_cg_sss(s.nodes, s.reserved, tolerance, s.col_ind,
                                                    s.row_ptr, s.values, s.diag, R, s.vef, x)
-        # mixed precision on a device-resident system, same stopping rule (||r|| vs ||r0||)
+        # the other methods on a device-resident system, same stopping rule (||r|| <=
+        # tol * ||r0||, r0 = b - A x0 of the ramp guess)
         sysd = self.ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
         try:
+            if self.method == dea.METHOD_CONDENSED_BLOCK and s.gnp_nodes > 0:
+                from .network import plate_row_ranges
+                sysd.set_blocks(plate_row_ranges(network, n_cluster, s))
             sysd.set_rhs(s.r0, s.x0, s.b)
             self.stats = sysd.solve(self.method, tolerance, 0, dea.REL_R0)
             x, r = sysd.get()
+        except dea.DEAError as e:
+            self.last_error = f"Error in Compute_voltage_field: {e}"
+            return 0
         finally:
             sysd.close()
+        if self.stats.status not in (dea.CICY_OK, dea.CICY_MAXITER):
+            self.last_error = ("Error in Compute_voltage_field: solve ended with status "
+                               + self.stats.as_dict()["status_name"])
+            return 0
         self.voltages = np.concatenate([s.vef, x])
         return 1
 
@@ -178,17 +192,72 @@ def calculate_parallel_resistor(direction: int, dea_obj: DirectElectrifying,
     return ep.applied_voltage / i_total, i_total, i_check
 
 
-def calculate_resistance(resistors, r_matrix: float = float("inf")) -> float:
-    """Per-direction parallel combination (Electrical_analysis.cpp:873-932): 0 clusters ->
-    the matrix value, 1 -> that R, more -> 1/sum(1/R_k)."""
+def window_dimensions(window) -> tuple[float, float, float]:
+    """(len_x, wid_y, hei_z) of a window: a cube side (float), a 3-sequence, or an object
+    with the reference's cuboid members len_x/wid_y/hei_z (Geometry_3D.h)."""
+    if hasattr(window, "len_x"):
+        return float(window.len_x), float(window.wid_y), float(window.hei_z)
+    if np.ndim(window) == 0:
+        w = float(window)
+        return w, w, w
+    lx, wy, hz = (float(v) for v in window)
+    return lx, wy, hz
+
+
+def _length_and_area(direction: int, window):
+    lx, wy, hz = window_dimensions(window)
+    if direction == 0:
+        return lx, wy * hz
+    if direction == 1:
+        return wy, lx * hz
+    if direction == 2:
+        return hz, lx * wy
+    raise ValueError(f"invalid direction {direction}: only 0, 1 or 2")
+
+
+def calculate_matrix_resistance(direction: int, window, matrix_resistivity: float) -> float:
+    """Resistance of the polymer alone along `direction`: rho_m * length / area
+    (Calculate_matrix_resistance, Electrical_analysis.cpp:935-958)."""
+    length, area = _length_and_area(direction, window)
+    return matrix_resistivity * length / area
+
+
+def calculate_resistivity(direction: int, window, resistance: float) -> float:
+    """rho = R * area / length along `direction` (Calculate_resistivity,
+    Electrical_analysis.cpp:960-987)."""
+    length, area = _length_and_area(direction, window)
+    return resistance * area / length
+
+
+def calculate_resistance(resistors, r_matrix: float) -> float:
+    """Per-direction combination (Electrical_analysis.cpp:883-917): no cluster -> the
+    matrix resistance `r_matrix`, one -> that R, more -> 1/sum(1/R_k) summed in list order."""
     if len(resistors) == 0:
-        return r_matrix
+        return float(r_matrix)
     if len(resistors) == 1:
         return float(resistors[0])
     inv = 0.0
     for r in resistors:
         inv = inv + 1 / r
     return 1 / inv
+
+
+def calculate_resistances_and_resistivities(window, electric_param: ElectricParams,
+                                            parallel_resistors):
+    """Resistances and resistivities along x, y, z from each direction's list of cluster
+    resistances (Calculate_resistances_and_resistivities, Electrical_analysis.cpp:873-932).
+    A direction without clusters takes the matrix resistance and the matrix resistivity."""
+    resistors, resistivities = [0.0] * 3, [0.0] * 3
+    for d in range(3):
+        rk = parallel_resistors[d]
+        if len(rk) == 0:
+            resistivities[d] = electric_param.resistivity_matrix
+            resistors[d] = calculate_matrix_resistance(d, window,
+                                                       electric_param.resistivity_matrix)
+        else:
+            resistors[d] = calculate_resistance(rk, 0.0)
+            resistivities[d] = calculate_resistivity(d, window, resistors[d])
+    return resistors, resistivities
 
 
 def electrical_resistance_along_each_percolated_direction(
@@ -214,17 +283,28 @@ def electrical_resistance_along_each_percolated_direction(
 
 def network_resistances(network: Network, electric_param: ElectricParams | None = None,
                         tolerance: float = 1e-10, method: int = dea.METHOD_PCG_F64,
-                        ctx: dea.Context | None = None):
+                        ctx: dea.Context | None = None, with_details: bool = False):
     """Resistance of the network along x, y, z: every percolating cluster solved per
-    direction and combined in parallel (Perform_analysis_on_clusters,
-    Electrical_analysis.cpp:10-145, without the backbone step)."""
+    direction and combined in parallel, a direction without one taking the matrix
+    resistance of the window (Perform_analysis_on_clusters, Electrical_analysis.cpp:10-145,
+    without the backbone step; then Calculate_resistances_and_resistivities, :873-932).
+
+    Returns [R_x, R_y, R_z]; with `with_details`, (resistances, resistivities,
+    {direction: [(cluster, R, I, I_check, stats), ...]})."""
+    ep = electric_param or ElectricParams()
     per_dir = {0: [], 1: [], 2: []}
+    details = {0: [], 1: [], 2: []}
     for k in range(network.count("clusters")):
         _, fam = network.cluster(k)
         if fam < 0:
             continue
         res = electrical_resistance_along_each_percolated_direction(
-            network, k, fam, electric_param, tolerance, method=method, ctx=ctx)
-        for d, (R, _, _, _) in res.items():
+            network, k, fam, ep, tolerance, method=method, ctx=ctx)
+        for d, (R, I, Ic, st) in res.items():
             per_dir[d].append(R)
-    return [calculate_resistance(per_dir[d]) for d in range(3)]
+            details[d].append((k, R, I, Ic, st))
+    resistors, resistivities = calculate_resistances_and_resistivities(
+        network.params.box, ep, [per_dir[d] for d in range(3)])
+    if with_details:
+        return resistors, resistivities, details
+    return resistors

@@ -131,6 +131,7 @@ class ElectricParams:
     c2: float = field(default=0.0)
     resistivity_gnp_t: float = 1e-2        # Ohm.um across a plate (input.dat)
     resistivity_gnp_surf: float = 1e-7     # Ohm.um in a plate (input.dat)
+    resistivity_matrix: float = 1.03e20    # Ohm.um, the polymer (input.dat)
 
     def __post_init__(self):
         if self.c1 == 0.0 and self.c2 == 0.0:

@@ -228,19 +228,33 @@ def assemble(family, r_flag, boundary_pts, cluster_tubes, elements, xyz, radii, 
     diag = diagonal[rd:]
     if any(abs(d) < ZERO for d in diag):
         raise ValueError("zero element on the diagonal")
+    x0 = ramp_guess(n, rd, r_flag, family, cluster_tubes, elements, lmm, xyz,
+                    elec["applied_voltage"], box)
+    return dict(nodes=nodes, reserved=rd, row_ptr=np.array(row_ptr, np.int64),
+                col_ind=np.array(col_ind, np.int64), values=np.array(values),
+                diag=np.array(diag), b=np.array(R), vef=np.array(vef), lmm=lmm, x0=x0,
+                lmm_gnp=lmm_g, points_cnt_rad=points_cnt_rad)
+
+
+def ramp_guess(n, rd, r_flag, family, cluster_tubes, elements, lmm, xyz, applied_voltage,
+               box):
+    """Initial guess of the free nodes (Direct_Electrifying.cpp:80-95 with
+    Initial_guess_for_CG / Select_coordinate / Initial_guess_for_cnt_nodes, :1396-1470).
+
+    Only when R_flag == 1 and there are two reserved nodes: for every element point of every
+    cluster tube whose node is not reserved (node >= 2), x0[node - rd] = V_app / L_window *
+    (max_coord - coord) along the family's axis (0 x, 1 y, 2 z).  The window is [0, box]^3,
+    so L_window = max_coord = box.  Plate nodes keep 0."""
     x0 = np.zeros(n)
     if r_flag == 1 and rd == 2:
-        vpc = elec["applied_voltage"] / box
+        vpc = applied_voltage / box
         ax = family
         for t in cluster_tubes:
             for p in elements[t]:
                 nd = lmm[int(p)]
                 if nd >= 2:
                     x0[nd - rd] = vpc * (box - xyz[int(p)][ax])
-    return dict(nodes=nodes, reserved=rd, row_ptr=np.array(row_ptr, np.int64),
-                col_ind=np.array(col_ind, np.int64), values=np.array(values),
-                diag=np.array(diag), b=np.array(R), vef=np.array(vef), lmm=lmm, x0=x0,
-                lmm_gnp=lmm_g, points_cnt_rad=points_cnt_rad)
+    return x0
 
 
 def boundary_elements(direction, boundary_tubes, elements, lmm, xyz, radii, rho, gnp=None,

@@ -95,3 +95,30 @@ def test_layouts_on_unstructured_matrix(fmt):
     got = dea.host_blocked_spmv(rp, col, val, diag, v, 256, fmt)
     want = oracle.spmv_sss(rp, col, val, diag, v)
     assert np.allclose(got, want, rtol=1e-13, atol=1e-13)
+
+
+def test_negative_row_ptr_at_thread_boundaries_is_rejected():
+    """build_blocked checks rows on host threads (up to 16, when n >= 65536).  A row_ptr
+    entry that is negative where a thread's range starts must be rejected before any entry
+    is read: each thread checks row_ptr[r] >= 0 for its own rows, so no thread reads
+    col_ind at a negative index while the neighbouring thread reports the error.  The
+    entry is set at every start a thread count of 2..16 would use, with a magnitude that
+    would fault on any read through it."""
+    n = 4 * 65536
+    rp = np.arange(n + 1, dtype=np.int64)
+    rp[0] = 0
+    rp[1] = 0                      # row 0 has no entry; row r > 0 holds (r, r-1)
+    rp[2:] = np.arange(1, n, dtype=np.int64)
+    nnz = int(rp[-1])
+    col = np.arange(nnz, dtype=np.int64)   # entry j belongs to row j + 1: column j
+    val = -np.ones(nnz)
+    diag = np.full(n, 4.0)
+    v = np.ones(n)
+    assert np.allclose(dea.host_blocked_spmv(rp, col, val, diag, v), oracle.spmv_sss(
+        rp, col, val, diag, v))
+    bad = rp.copy()
+    for nth in range(2, 17):
+        for t in range(1, nth):
+            bad[n * t // nth] = -(1 << 40)
+    with pytest.raises(dea.DEAError):
+        dea.host_blocked_spmv(bad, col, val, diag, v)
