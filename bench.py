@@ -79,6 +79,45 @@ def time_solves(sysd, method, steps, warmup, tol, s=None, model=None, graph=None
     return wall, stats
 
 
+def time_fresh(ctx, s, method, steps, warmup, tol, plates=None, model=None, box=None):
+    """`steps` solves of the system as a new one each time (after `warmup` untimed ones):
+    from the host arrays, upload and layout (cicy_sys_create), right-hand side, plate row
+    groups, the method's set-up (cicy_sys_prepare: eliminations, core layout, block
+    inverses), the learned guess when a model is given, the solve, and the read-back of x;
+    the system is destroyed after each step.  Returns per-step timings (s) and the stats."""
+    from cicy_geom_amd import dea, warmstart
+    out = []
+    for i in range(warmup + steps):
+        t0 = time.perf_counter()
+        sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+        sysd.set_rhs(s.b, None, s.b)
+        if method == dea.METHOD_CONDENSED_BLOCK and plates is not None:
+            sysd.set_blocks(plates)
+        t1 = time.perf_counter()
+        sysd.prepare(method)
+        t2 = time.perf_counter()
+        graph = None
+        if model is not None:
+            graph = warmstart.Graph(ctx, s, box, 0, 1.0)
+            model.warm_start(graph, sysd)
+        st = sysd.solve(method, tol, MAX_ITER, dea.REL_B)
+        t3 = time.perf_counter()
+        x, _ = sysd.get()
+        t4 = time.perf_counter()
+        info = {k: sysd.size(k) for k in ("condensed_rows", "core_rows", "core_nnz",
+                                          "series_rounds", "core_blocks", "core_block_rows",
+                                          "core_groups", "core_block_entries", "format",
+                                          "resident_rows")}
+        if graph is not None:
+            graph.close()
+        sysd.close()
+        if i >= warmup:
+            out.append(dict(create_s=t1 - t0, prepare_s=t2 - t1, solve_s=t3 - t2,
+                            get_s=t4 - t3, total_s=time.perf_counter() - t0, stats=st,
+                            info=info))
+    return out
+
+
 def cpu_baseline(s, iters_needed, budget_s=12.0):
     """The reference's CPU algorithm (oracle restatement, 1 thread) on the same system:
     a bounded number of PCG iterations, timed, scaled to a full solve."""
@@ -387,47 +426,62 @@ def main():
                 f"rb={tuned[name]['rows_per_block']} ({time.perf_counter() - t0:.1f}s)")
 
     results, systems, by_kind = {}, {}, {}
-    upload_s = 0.0
     for name, method, tkey, learned in plan:
         autotune.apply(ctx, tuned[tkey])
-        t0 = time.perf_counter()
-        sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
-        sysd.set_rhs(s.b, None, s.b)
-        if method == dea.METHOD_CONDENSED_BLOCK:
-            sysd.set_blocks(plates)
-        sysd.prepare(method)
-        upload_s = time.perf_counter() - t0
-        if learned:
-            graph = warmstart.Graph(ctx, s, box, 0, 1.0)
         steps, warmup = ((args.plain_steps, 0) if name in full_system
                          else (args.steps, args.warmup))
-        wall, stats = time_solves(sysd, method, steps, warmup, args.tol, s,
-                                  model if learned else None, graph if learned else None)
-        st = stats[-1]
+        # the metric's step: a new system each time (set-up inside the timed step)
+        fresh = time_fresh(ctx, s, method, steps, 1 if name not in full_system else 0,
+                           args.tol, plates, model if learned else None, box)
+        st = fresh[-1]["stats"]
+        wall = sum(f["total_s"] for f in fresh)
+        mean = {k: sum(f[k] for f in fresh) / len(fresh)
+                for k in ("create_s", "prepare_s", "solve_s", "get_s", "total_s")}
+        info = fresh[-1]["info"]
         results[name] = dict(solves_per_s=steps / wall, ms_per_solve=1e3 * wall / steps,
                              steps=steps,
                              iters=st.iters, refinements=st.refinements,
                              true_rel_res=st.true_rel_res, status=dea.STATUS_NAMES[st.status],
                              device_ms=st.solve_ms,
-                             layout=autotune.LAYOUTS.get(sysd.size("format")),
-                             condensed_rows=sysd.size("condensed_rows"),
-                             core_rows=sysd.size("core_rows"),
-                             core_nnz=sysd.size("core_nnz"),
-                             series_rounds=sysd.size("series_rounds"),
-                             core_blocks=sysd.size("core_blocks"),
-                             core_block_rows=sysd.size("core_block_rows"),
-                             core_groups=sysd.size("core_groups"),
-                             setup_s=upload_s,
-                             kernel_ms=sum(x.kernel_ms for x in stats),
-                             resident_rows_per_thread=sysd.size("resident_rows"),
+                             layout=autotune.LAYOUTS.get(info["format"]),
+                             condensed_rows=info["condensed_rows"],
+                             core_rows=info["core_rows"], core_nnz=info["core_nnz"],
+                             series_rounds=info["series_rounds"],
+                             core_blocks=info["core_blocks"],
+                             core_block_rows=info["core_block_rows"],
+                             core_block_entries=info["core_block_entries"],
+                             core_groups=info["core_groups"],
+                             step_breakdown_s=mean,
+                             setup_s=mean["create_s"] + mean["prepare_s"],
+                             kernel_ms=sum(f["stats"].kernel_ms for f in fresh),
+                             resident_rows_per_thread=info["resident_rows"],
                              rows_per_block=tuned[tkey]["rows_per_block"])
+        if name not in full_system:
+            # the same method re-solving an already prepared, resident system (the rate a
+            # caller gets when one system is solved many times)
+            sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+            sysd.set_rhs(s.b, None, s.b)
+            if method == dea.METHOD_CONDENSED_BLOCK:
+                sysd.set_blocks(plates)
+            sysd.prepare(method)
+            graph = warmstart.Graph(ctx, s, box, 0, 1.0) if learned else None
+            rwall, rstats = time_solves(sysd, method, steps, args.warmup, args.tol, s,
+                                        model if learned else None, graph)
+            results[name]["resident_solves_per_s"] = steps / rwall
+            results[name]["resident_ms_per_solve"] = 1e3 * rwall / steps
+            results[name]["resident_kernel_ms"] = sum(x.kernel_ms for x in rstats)
+            if graph is not None:
+                graph.close()
+            systems[name] = sysd
+            by_kind.setdefault(tkey, sysd)
         log(f"[bench] {name}: {results[name]}")
         if st.status != dea.CICY_OK:
             log(f"[bench] WARNING {name} did not converge")
-        systems[name] = sysd
-        by_kind.setdefault(tkey, sysd)
-        if learned:
-            graph.close()
+    if "pcg_f64" not in by_kind and "pcg_f64" in results:
+        sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+        sysd.set_rhs(s.b, None, s.b)
+        systems["pcg_f64"] = sysd
+        by_kind["pcg_f64"] = sysd
 
     # SpMV roofline (HIP events on the library's stream, back-to-back launches), next to a
     # plain 1 GiB device copy (read + write) as the achievable streaming rate
@@ -476,7 +530,7 @@ def main():
         # counter runs of the same workload, scaled to this launch's iterations.
         r_ = results[head]
         cn, cl = r_["core_rows"], r_["core_nnz"]
-        nb2 = systems[head].size("core_block_entries")
+        nb2 = r_["core_block_entries"]
         cib = 12 * 2 * cl + 12 * cn + 4 * nb2 + 8 * 13 * cn
         ksec = r_["kernel_ms"] * 1e-3 / max(1, r_["steps"])
         ach = cib * r_["iters"] / ksec / 1e9 if ksec > 0 else 0.0
@@ -497,7 +551,7 @@ def main():
         log(f"[bench] k_core_cg: {roof['us_per_iter']:.1f} us per core iteration, "
             f"{cib / 1e6:.1f} MB algorithmic -> {ach:.0f} GB/s ({100 * roof['frac']:.1f} % of peak)")
     elif head.startswith("condensed") or kind == dea.METHOD_CONDENSED:
-        core = systems[head].core()
+        core = systems[head].core()  # the resident system of the same method
         # dominant kernels: the core system's PCG iteration (k_apply<PCG> + k_update_f64);
         # its rate is the HIP-event device time of the timed solves over their iterations
         cn, cl = core.size("n"), core.size("nnz")
@@ -528,11 +582,12 @@ def main():
                    "method": head},
         "methods": results,
         "speedup_vs_plain_f64": value / plain["solves_per_s"] if plain else None,
-        # one-off per-system setup of the headline method (host layouts, eliminations,
-        # block inverses, upload), outside the timed solves; a fresh system per solve runs at
+        # each step is a new system: upload and layouts, set-up (eliminations, core layout,
+        # block inverses), solve and read-back of x, from the host arrays
+        "step": "new system from host arrays: upload + set-up + solve + read-back",
+        "step_breakdown_s": results[head]["step_breakdown_s"],
         "setup_s": results[head]["setup_s"],
-        "end_to_end_solves_per_s": 1.0 / (results[head]["setup_s"] +
-                                          results[head]["ms_per_solve"] * 1e-3),
+        "resident_solves_per_s": results[head].get("resident_solves_per_s"),
         "roofline": roof,
         "roofline_spmv_full_f64": {"achieved": spmv[64]["gbs"],
                                    "frac": spmv[64]["gbs"] / HBM_PEAK_GBS,
@@ -541,7 +596,6 @@ def main():
                                    "copy_1GiB_gbs": copy_gbs},
         "pcg_iteration_full": {"algorithmic_bytes": it_bytes, "us": it_us,
                                "gbs": it_bytes / (it_us * 1e-6) / 1e9 if it_us > 0 else None},
-        "upload_s": upload_s,
     }
     if not args.no_cpu and plain:
         out["cpu_baseline"] = cpu_baseline(s, plain["iters"])

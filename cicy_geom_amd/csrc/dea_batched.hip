@@ -26,7 +26,7 @@ struct cicy_batch {
     int nsys = 0, rb = 256;
     int n = 0, nnz = 0, nblocks = 0, nremote = 0;
     std::vector<int> h_sys_row0, h_sys_blk0, h_blk_sys, h_blk_row0;
-    std::vector<double> bnorm2;
+    std::vector<double> bnorm2, r0norm2;
     // device: matrix (FP64 + Jacobi-scaled FP32 values), tables, vectors, state
     int *rp = nullptr, *col = nullptr, *urp = nullptr, *ucol = nullptr;
     double *val = nullptr, *diag = nullptr, *uval = nullptr;
@@ -287,6 +287,7 @@ int cicy_batch_create(cicy_ctx *ctx, int nsys, const int64_t *n, const int64_t *
     B->nblocks = (int)tb;
     B->nremote = (int)trem;
     B->bnorm2.assign((size_t)nsys, -1.0);
+    B->r0norm2.assign((size_t)nsys, 0.0);
     // one layout for the whole batch: band when most entries of the batch are sub-diagonal
     // (the band layout unless the blocked SSS layout is asked for, as for single systems)
     B->format = ctx->format == kFormatSss ? kFormatSss : kFormatBand;
@@ -490,9 +491,11 @@ int cicy_batch_set_rhs(cicy_batch *B, int sys, const double *r0, const double *x
         return 0;
     }
     ok = ok && bh2d(B, B->b + off, bb, ns);
-    double nb2 = 0.0;
+    double nb2 = 0.0, r2 = 0.0;
     for (size_t i = 0; i < ns; i++) nb2 += bb[i] * bb[i];
+    for (size_t i = 0; i < ns; i++) r2 += r0[i] * r0[i];
     B->bnorm2[sys] = nb2;
+    B->r0norm2[sys] = r2;
     ok = ok && hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return ok ? 1 : 0;
 }
@@ -609,13 +612,17 @@ int cicy_batch_solve(cicy_batch *B, int method, double tol, int64_t max_iter, in
     if (!read_state() || !read_norms()) return 0;
     for (int s = 0; s < B->nsys; s++) iters[s] = hs[s].iters;
     // Continue in FP64, from x with r = b - A x, every system whose FP32 recursion stalled
-    // (status CICY_MAXITER below the cap) or whose true residual misses tol*||b||.
+    // or broke down (status CICY_MAXITER below the cap, or CICY_BREAKDOWN with a finite
+    // residual) or whose true residual misses tol*||b||.
     for (int pass = 0; pass < 3; pass++) {
         int m = 0;
         std::vector<char> redo((size_t)B->nsys, 0);
         for (int s = 0; s < B->nsys; s++) {
-            const bool stalled = method == CICY_METHOD_MIXED && hs[s].status == CICY_MAXITER &&
-                                 iters[s] < max_iter;
+            // an FP32 breakdown (p.q no longer positive in FP32) is continued in FP64 too
+            const bool stalled = method == CICY_METHOD_MIXED && pass == 0 &&
+                                 (hs[s].status == CICY_MAXITER ||
+                                  hs[s].status == CICY_BREAKDOWN) &&
+                                 iters[s] < max_iter && std::isfinite(true_rel(s));
             const bool short_of_b = rel_to == CICY_REL_B && hs[s].status == CICY_OK &&
                                     true_rel(s) > tol;
             if (!stalled && !short_of_b) continue;
@@ -623,6 +630,12 @@ int cicy_batch_solve(cicy_batch *B, int method, double tol, int64_t max_iter, in
             for (int b = B->h_sys_blk0[s]; b < B->h_sys_blk0[s + 1]; b++) B->h_active[0][m++] = b;
         }
         if (m == 0) break;
+        // the continuation stops against the same reference norm as the solve (||b|| or the
+        // given ||r0||), not against its own starting residual
+        if (pass == 0) {
+            const std::vector<double> &cref = rel_to == CICY_REL_B ? B->bnorm2 : B->r0norm2;
+            if (!bh2d(B, B->stop_ref2, cref.data(), cref.size())) return 0;
+        }
         if (!bh2d(B, B->active[0], B->h_active[0], (size_t)m)) return 0;
         CICY_RB_SWITCH(B->rb, resid_blocks<RB>(B, B->active[0], m));
         CICY_RB_SWITCH(B->rb, init_blocks<RB>(B, CICY_METHOD_PCG_F64, tol, max_iter,

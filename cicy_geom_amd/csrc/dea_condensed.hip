@@ -683,8 +683,10 @@ void host_anchor_rhs(const CondHost &C, const double *rhs, const double *y, doub
 
 cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &bh,
                            const char **why, bool *failed) {
+    StageClock clk;
     CondHost C;
     if (!build_condensed_host(ctx, h, bh, C, why, failed)) return nullptr;
+    clk.mark("condense: paths");
     const int n = C.n, m = C.m;
     const std::vector<int> &ps = C.ps, &pe = C.pe, &pl = C.pl, &pr = C.pr, &anchor_of = C.anchor_of,
                            &anchors = C.anchors;
@@ -712,8 +714,10 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
          up(ctx, c->lfac, lfac) && up(ctx, c->mfac, mfac) && up(ctx, c->band, band) &&
          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     // stage 2: series rounds over S, then the core system
+    clk.mark("condense: path upload");
     SeriesHost Z;
     if (ok && m > 0) build_series_host(C, Z);
+    clk.mark("condense: series rounds");
     c->mc = Z.mc;
     c->nrec = (int)Z.e_node.size();
     c->round_off = Z.round_off;
@@ -744,6 +748,7 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
             c->rv_round_off.push_back((int)rv_node.size());
         }
     }
+    clk.mark("condense: receiver lists");
     c->nnz_core = (long long)Z.cval.size();
     if (ok && m > 0) {
         ok = calloc_dev(ctx, &c->e_node, Z.e_node.size()) &&

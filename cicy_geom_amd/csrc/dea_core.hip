@@ -863,6 +863,7 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
     }
     // the grid sums read at most kSlotLanes slots per lane of one wave
     cus = std::min(cus, kSlotLanes * kWave);
+    StageClock clk;
     CoreHost H;
     const char *why = nullptr;
     if (!build_core_host(m, crp, ccol, cval, cdiag, orig, nblocks, ranges, cus, H, &why)) {
@@ -885,6 +886,7 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
         set_error(ctx, "core CG kernel cannot be resident with %d staged columns", lds_cols);
         return nullptr;
     }
+    clk.mark("core: layout + block inverses");
     cicy_core *z = new cicy_core();
     z->m = m;
     z->G = H.G;
@@ -910,6 +912,7 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
               hip_ok(ctx, hipMalloc((void **)&z->ctl, sizeof(GridCtl)), "hipMalloc (core)") &&
               hip_ok(ctx, hipMalloc((void **)&z->st, sizeof(CoreState)), "hipMalloc (core)") &&
               hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    clk.mark("core: upload");
     if (!ok) {
         core_free(z);
         return nullptr;

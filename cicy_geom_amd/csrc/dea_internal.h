@@ -2,12 +2,29 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "cicy_dea.h"
 #include "dea_kernels.h"
 
 namespace cicy {
+
+// Wall-clock marks of a system's set-up stages, printed to stderr when CICY_SETUP_TIMING is
+// set (a diagnostic of where a fresh system's set-up time goes).
+struct StageClock {
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    bool on = getenv("CICY_SETUP_TIMING") != nullptr;
+    void mark(const char *what) {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        fprintf(stderr, "[cicy setup] %-28s %8.2f ms\n", what,
+                std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
 
 // iterations per captured hipGraph chunk (even: the p double buffer returns to parity 0)
 constexpr int kChunk = 32;
@@ -92,6 +109,8 @@ struct cicy_ctx {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
     cicy::PcgState *host_state = nullptr;  // pinned, 2 slots
+    double *bc_ws = nullptr;   // boundary-current workspace (grows, kept until destroy)
+    size_t bc_words = 0;
     char err[512] = {0};
 };
 

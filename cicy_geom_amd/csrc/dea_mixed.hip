@@ -214,10 +214,16 @@ bool solve_mixed(cicy_sys *s, double tol, long long max_iter, int rel_to,
     int fallback_iters = 0;
     const bool had_b = s->has_b;
     s->has_b = true;  // b is valid here (given or formed above)
-    if (status != CICY_OK && status != CICY_BREAKDOWN && iters < max_iter) {
-        // the FP32 recursion stopped making progress: finish in FP64 from x, r = b - A x
+    if (status != CICY_OK && iters < max_iter) {
+        // the FP32 recursion stopped making progress, or broke down (p.q no longer positive
+        // in FP32 once the scaled residual nears FP32's range): finish in FP64 from x,
+        // r = b - A x; a breakdown of the FP64 iteration itself is reported as such
+        // stopping against the solve's reference norm (||b||, or ||r0|| of the given start),
+        // not against the residual the continuation starts from
         cicy_solve_stats fs;
+        s->ref2_override = rel_to == CICY_REL_B ? s->bnorm2 : s->r0norm2;
         const bool ok = solve_pcg64(s, tol, max_iter - iters, rel_to, &fs, true);
+        s->ref2_override = -1.0;
         if (!ok) {
             s->has_b = had_b;
             return false;

@@ -85,16 +85,80 @@ __global__ __launch_bounds__(kThreads) void k_copy(size_t n16, const uint4 *__re
     }
 }
 
-// FP64 boundary current: one work-group, fixed strided order, tree reduction.
+// FP64 boundary current I = sum_e (v[ni[e]] - v[nb[e]]) / re[e]: grid-stride over the
+// elements, per-work-group sums reduced in a fixed order (reduce_sys: bit-identical from run
+// to run), the total written to out[0].  An element whose node lies outside [0, nv) adds
+// nothing and sets *bad.  The reduction words (RedBuf slots and counters, the counters at
+// zero) are laid out by bc_red().
+constexpr int kBcGroups = 1024;  // most work-groups of one launch
+// reduction words of a launch: work-group sums, group sums, then kBcGroups + 2 counters
+// (group arrivals, the final arrival, the bad-node flag) packed as 32-bit words
+constexpr size_t kBcRedWords = 2 * (size_t)kBcGroups + ((size_t)kBcGroups + 2 + 1) / 2;
+
 __global__ __launch_bounds__(kThreads) void k_boundary_current(
     long long ne, const long long *__restrict__ nb, const long long *__restrict__ ni,
-    const double *__restrict__ re, const double *__restrict__ v, double *__restrict__ out) {
+    const double *__restrict__ re, const double *__restrict__ v, long long nv,
+    double *__restrict__ out, RedBuf rb, unsigned int *__restrict__ top,
+    unsigned int *__restrict__ bad) {
     __shared__ double red[8];
-    double s = 0.0;
-    for (long long e = threadIdx.x; e < ne; e += kThreads) s += (v[ni[e]] - v[nb[e]]) / re[e];
-    s = block_sum(s, red);
-    if (threadIdx.x == 0) out[0] = s;
+    __shared__ int flag;
+    double s[1] = {0.0};
+    for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x; e < ne;
+         e += (long long)gridDim.x * kThreads) {
+        const long long a = ni[e], b = nb[e];
+        if ((unsigned long long)a >= (unsigned long long)nv ||
+            (unsigned long long)b >= (unsigned long long)nv) {
+            atomicOr(bad, 1u);
+            continue;
+        }
+        s[0] += (v[a] - v[b]) / re[e];
+    }
+    Blk k;
+    k.b = blockIdx.x;
+    k.sys = 0;
+    k.r0 = 0;
+    k.nrows = 0;
+    k.first = 0;
+    k.count = gridDim.x;
+    double o[1];
+    if (reduce_sys<1>(s, k, rb, top, red, &flag, o) && threadIdx.x == 0) out[0] = o[0];
 }
+
+// p[0 .. n) = 0 (clears counters with a kernel, so the sequence may be captured in a graph)
+__global__ __launch_bounds__(kThreads) void k_zero_u32(unsigned int *__restrict__ p, int n) {
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+namespace {
+
+struct BcRed {
+    RedBuf rb;
+    unsigned int *top, *bad;
+};
+
+// the reduction words of one launch inside `words` (kBcRedWords doubles)
+BcRed bc_red(double *words) {
+    unsigned int *cnt = reinterpret_cast<unsigned int *>(words + 2 * kBcGroups);
+    return BcRed{RedBuf{words + kBcGroups, words, cnt}, cnt + kBcGroups, cnt + kBcGroups + 1};
+}
+
+int bc_grid(long long ne) {
+    return (int)std::max<long long>(1, std::min<long long>((ne + kThreads - 1) / kThreads,
+                                                           kBcGroups));
+}
+
+// clears the counters, launches the sum into out on the context's stream
+void bc_launch(hipStream_t st, long long ne, const long long *nb, const long long *ni,
+               const double *re, const double *v, long long nv, double *out, double *words) {
+    const BcRed r = bc_red(words);
+    hipLaunchKernelGGL(k_zero_u32, dim3((kBcGroups + 2 + kThreads - 1) / kThreads),
+                       dim3(kThreads), 0, st, r.rb.gcount, kBcGroups + 2);
+    hipLaunchKernelGGL(k_boundary_current, dim3(bc_grid(ne)), dim3(kThreads), 0, st, ne, nb, ni,
+                       re, v, nv, out, r.rb, r.top, r.bad);
+}
+
+}  // namespace
 
 // ------------------------------------------------------------------------------------
 // host side
@@ -616,6 +680,7 @@ void cicy_destroy(cicy_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->host_state) (void)hipHostFree(ctx->host_state);
+    if (ctx->bc_ws) (void)hipFree(ctx->bc_ws);
     (void)hipEventDestroy(ctx->ev_start);
     (void)hipEventDestroy(ctx->ev_stop);
     (void)hipEventDestroy(ctx->ev_poll[0]);
@@ -682,12 +747,14 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
     if (!ctx || !out) return 0;
     *out = nullptr;
     if (hipSetDevice(ctx->device) != hipSuccess) return 0;
+    StageClock clk;
     SysHost *keep = new SysHost();
     BlockedHost &h = keep->h;
     if (!build_blocked(ctx, n, nnz, row_ptr, col_ind, values, diagonal, ctx->rows_per_block, h)) {
         delete keep;
         return 0;
     }
+    clk.mark("create: row checks + blocks");
     cicy_sys *s = new cicy_sys();
     s->host = keep;
     s->resident_rows_req = ctx->resident_rows;
@@ -716,6 +783,7 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
     } else {
         build_band(h, bh);
         keep->have_band = true;
+        clk.mark("create: band layout");
         s->nx = bh.m;
         ok = ok && dmalloc(ctx, &s->band, nv + 1, &tot) &&
              dmalloc(ctx, &s->xblk, (size_t)h.nblocks + 1, &tot) &&
@@ -753,6 +821,7 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
          hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), ctx->stream), "memset") &&
          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     s->device_bytes = tot;
+    clk.mark("create: allocate + upload");
     if (!ok) {
         cicy_sys_destroy(s);
         return 0;
@@ -1139,23 +1208,29 @@ int cicy_boundary_current_f64(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
         return 1;
     }
     (void)hipSetDevice(ctx->device);
-    long long *dnb = nullptr, *dni = nullptr;
-    double *dre = nullptr, *dv = nullptr, *dout = nullptr;
-    bool ok = dmalloc(ctx, &dnb, (size_t)ne, nullptr) && dmalloc(ctx, &dni, (size_t)ne, nullptr) &&
-              dmalloc(ctx, &dre, (size_t)ne, nullptr) && dmalloc(ctx, &dv, (size_t)nv, nullptr) &&
-              dmalloc(ctx, &dout, 1, nullptr);
-    ok = ok && h2d(ctx, dnb, (const long long *)n_bnd, (size_t)ne) &&
-         h2d(ctx, dni, (const long long *)n_int, (size_t)ne) && h2d(ctx, dre, re, (size_t)ne) &&
-         h2d(ctx, dv, voltages, (size_t)nv);
+    // inputs, result and reduction words in the context's workspace, which only grows
+    const size_t words = 3 * (size_t)ne + (size_t)nv + 1 + kBcRedWords;
+    if (words > ctx->bc_words) {
+        if (ctx->bc_ws) (void)hipFree(ctx->bc_ws);
+        ctx->bc_ws = nullptr;
+        ctx->bc_words = 0;
+        if (!dmalloc(ctx, &ctx->bc_ws, words, nullptr)) return 0;
+        ctx->bc_words = words;
+    }
+    double *red = ctx->bc_ws;
+    double *dout = red + kBcRedWords;
+    double *dre = dout + 1;
+    double *dv = dre + ne;
+    long long *dnb = reinterpret_cast<long long *>(dv + nv);
+    long long *dni = dnb + ne;
+    bool ok = h2d(ctx, dnb, (const long long *)n_bnd, (size_t)ne) &&
+              h2d(ctx, dni, (const long long *)n_int, (size_t)ne) &&
+              h2d(ctx, dre, re, (size_t)ne) && h2d(ctx, dv, voltages, (size_t)nv);
     if (ok) {
-        hipLaunchKernelGGL(k_boundary_current, dim3(1), dim3(kThreads), 0, ctx->stream,
-                           (long long)ne, dnb, dni, dre, dv, dout);
+        bc_launch(ctx->stream, ne, dnb, dni, dre, dv, nv, dout, red);
         ok = hip_ok(ctx, hipGetLastError(), "launch") && d2h(ctx, current, dout, 1) &&
              hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     }
-    void *ptrs[] = {dnb, dni, dre, dv, dout};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
     return ok ? 1 : 0;
 }
 

@@ -55,9 +55,16 @@ def get_global_nodes(family: int) -> int:
 
 
 class DirectElectrifying:
-    def __init__(self, ctx: dea.Context | None = None, method: int = dea.METHOD_PCG_F64):
+    """One solve's object, as the reference creates one per (cluster, R_flag, family)
+    (Electrical_analysis.cpp:42, :252).  `method` selects the device method; `warm_start`
+    (a warmstart.WarmStartNet) replaces the ramp guess by the learned prediction, with the
+    stop kept at the reference's absolute threshold tol * ||r0(ramp)||."""
+
+    def __init__(self, ctx: dea.Context | None = None, method: int = dea.METHOD_PCG_F64,
+                 warm_start=None):
         self.ctx = ctx or default_context()
         self.method = method
+        self.warm_start = warm_start
         self.voltages = np.zeros(0)
         self.LMM_cnts = np.zeros(0, dtype=np.int64)   # point -> node, -1 when unmapped
         self.LMM_gnps = np.zeros(0, dtype=np.int64)
@@ -123,7 +130,7 @@ class DirectElectrifying:
             return 0
         try:
             s = network.assemble(n_cluster, r_flag, family, electric_param,
-                                 direction=direction)
+                                 direction=direction, with_xyz=self.warm_start is not None)
         except RuntimeError as e:
             self.last_error = f"Error in Compute_voltage_field: {e}"
             return 0
@@ -132,7 +139,7 @@ class DirectElectrifying:
         self.LMM_gnps = s.lmm_gnp if s.lmm_gnp is not None else np.zeros(0, np.int64)
         R = s.r0.copy()
         x = s.x0.copy()
-        if self.method == dea.METHOD_PCG_F64:
+        if self.method == dea.METHOD_PCG_F64 and self.warm_start is None:
             return self.solve_dea_equations_cg_sss(s.nodes, s.reserved, tolerance, s.col_ind,
                                                    s.row_ptr, s.values, s.diag, R, s.vef, x)
         # the other methods on a device-resident system, same stopping rule (||r|| <=
@@ -143,7 +150,20 @@ class DirectElectrifying:
                 from .network import plate_row_ranges
                 sysd.set_blocks(plate_row_ranges(network, n_cluster, s))
             sysd.set_rhs(s.r0, s.x0, s.b)
-            self.stats = sysd.solve(self.method, tolerance, 0, dea.REL_R0)
+            if self.warm_start is None:
+                self.stats = sysd.solve(self.method, tolerance, 0, dea.REL_R0)
+            else:
+                from .warmstart import Graph
+                ep = electric_param or ElectricParams()
+                axis = family if 0 <= family <= 2 else max(direction, 0)
+                g = Graph(self.ctx, s, network.params.box, axis, ep.applied_voltage)
+                try:
+                    self.warm_start.warm_start(g, sysd)
+                finally:
+                    g.close()
+                nb = float(np.linalg.norm(s.b))
+                tol_b = tolerance * float(np.linalg.norm(s.r0)) / nb if nb > 0 else tolerance
+                self.stats = sysd.solve(self.method, tol_b, 0, dea.REL_B)
             x, r = sysd.get()
         except dea.DEAError as e:
             self.last_error = f"Error in Compute_voltage_field: {e}"
@@ -264,7 +284,7 @@ def electrical_resistance_along_each_percolated_direction(
         network: Network, n_cluster: int, family: int,
         electric_param: ElectricParams | None = None, tolerance: float = 1e-10,
         resistances=(1, 1, 1), method: int = dea.METHOD_PCG_F64,
-        ctx: dea.Context | None = None):
+        ctx: dea.Context | None = None, warm_start=None):
     """One R_flag=1 solve per percolated direction with family := direction
     (Electrical_analysis.cpp:229-289).  Returns {direction: (R, I, I_check, stats)}."""
     ep = electric_param or ElectricParams()
@@ -273,7 +293,7 @@ def electrical_resistance_along_each_percolated_direction(
         raise ValueError(f"Invalid family: {family}")
     out = {}
     for d in dirs:
-        de = DirectElectrifying(ctx, method)
+        de = DirectElectrifying(ctx, method, warm_start)
         if not de.compute_voltage_field(n_cluster, 1, network, d, ep, tolerance, direction=d):
             raise RuntimeError(de.last_error)
         R, I, Ic = calculate_parallel_resistor(d, de, ep)
@@ -283,7 +303,8 @@ def electrical_resistance_along_each_percolated_direction(
 
 def network_resistances(network: Network, electric_param: ElectricParams | None = None,
                         tolerance: float = 1e-10, method: int = dea.METHOD_PCG_F64,
-                        ctx: dea.Context | None = None, with_details: bool = False):
+                        ctx: dea.Context | None = None, with_details: bool = False,
+                        warm_start=None):
     """Resistance of the network along x, y, z: every percolating cluster solved per
     direction and combined in parallel, a direction without one taking the matrix
     resistance of the window (Perform_analysis_on_clusters, Electrical_analysis.cpp:10-145,
@@ -299,10 +320,63 @@ def network_resistances(network: Network, electric_param: ElectricParams | None 
         if fam < 0:
             continue
         res = electrical_resistance_along_each_percolated_direction(
-            network, k, fam, ep, tolerance, method=method, ctx=ctx)
+            network, k, fam, ep, tolerance, method=method, ctx=ctx, warm_start=warm_start)
         for d, (R, I, Ic, st) in res.items():
             per_dir[d].append(R)
             details[d].append((k, R, I, Ic, st))
+    resistors, resistivities = calculate_resistances_and_resistivities(
+        network.params.box, ep, [per_dir[d] for d in range(3)])
+    if with_details:
+        return resistors, resistivities, details
+    return resistors
+
+
+def network_resistances_batched(network: Network,
+                                electric_param: ElectricParams | None = None,
+                                tolerance: float = 1e-10,
+                                method: int = dea.METHOD_PCG_F64,
+                                ctx: dea.Context | None = None, with_details: bool = False):
+    """network_resistances with every (cluster, direction) system solved together in one
+    batch (cicy_batch: concatenated systems, per-system scalars and stopping tests in shared
+    launches) instead of one after another as the reference's cluster and direction loops do
+    (Electrical_analysis.cpp:34-105, :249-289).  Each system starts from its ramp guess and
+    stops by the reference's rule ||r|| <= tol * ||r0||; currents and the combination are as
+    in network_resistances.  `method` is METHOD_PCG_F64 or METHOD_MIXED."""
+    ep = electric_param or ElectricParams()
+    ctx = ctx or default_context()
+    jobs = []
+    for k in range(network.count("clusters")):
+        _, fam = network.cluster(k)
+        if fam < 0:
+            continue
+        dirs = vector_of_directions(fam)
+        if dirs is None:
+            raise ValueError(f"Invalid family: {fam}")
+        for d in dirs:
+            jobs.append((k, d, network.assemble(k, 1, d, ep, direction=d)))
+    per_dir = {0: [], 1: [], 2: []}
+    details = {0: [], 1: [], 2: []}
+    stats = []
+    if jobs:
+        B = ctx.batch([(s.row_ptr, s.col_ind, s.values, s.diag) for _, _, s in jobs])
+        try:
+            for i, (_, _, s) in enumerate(jobs):
+                B.set_rhs(i, s.r0, s.x0, s.b)
+            stats, _ = B.solve(method, tolerance, 0, dea.REL_R0)
+            for i, (k, d, s) in enumerate(jobs):
+                st = stats[i]
+                if st.status not in (dea.CICY_OK, dea.CICY_MAXITER):
+                    raise RuntimeError(f"batched solve of cluster {k}, direction {d}: "
+                                       + st.as_dict()["status_name"])
+                x, _ = B.get(i)
+                volt = np.concatenate([s.vef, x])
+                i1 = ctx.boundary_current(*s.boundary_elements[0], volt)
+                i2 = ctx.boundary_current(*s.boundary_elements[1], volt)
+                R = ep.applied_voltage / i1
+                per_dir[d].append(R)
+                details[d].append((k, R, i1, i2, st))
+        finally:
+            B.close()
     resistors, resistivities = calculate_resistances_and_resistivities(
         network.params.box, ep, [per_dir[d] for d in range(3)])
     if with_details:

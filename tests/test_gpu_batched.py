@@ -65,28 +65,32 @@ def test_batch_matches_truth_and_single_solves(ctx, systems, truths, method, fmt
               f"{tr:.2e} {err:.2e}")
         assert st.status == dea.CICY_OK
         assert tr <= 1e-10 * (1 + 1e-6) and abs(st.true_rel_res - tr) <= 1e-3 * tr + 1e-16
-        # error against the FP64 truth: a guard against gross errors, scaled by what the
-        # reference algorithm itself leaves at the same stopping rule.  The residual checks
-        # above are the criterion; at a fixed residual the error of these ill-conditioned
-        # systems depends on where the recursion crosses it (the 23 213-row z system: the
-        # reference stops after 8 854 iterations at 3.6e-9, a device run after 8 468 at
-        # 1.0e-7, both with ||b - Ax|| <= 1e-10 ||b||).  The FP32 recursion of the mixed
-        # method drifts more.
+        # error against the FP64 truth, against what the reference algorithm itself leaves
+        # at the same stopping rule: the device FP64 PCG sums in a fixed order and follows the
+        # reference's recursion, so it lands where the reference does (within a factor 3);
+        # the FP32 recursion of the mixed method ends elsewhere at the same residual and is
+        # held to the looser bound
         if method == dea.METHOD_PCG_F64:
-            assert err <= max(100.0 * err_ref, 1e-7), (err, err_ref)
+            assert err <= 3.0 * err_ref, (err, err_ref)
         else:
             assert err < 1e-4
-    # each system's iteration count matches its stand-alone solve closely (the device sums
-    # are accumulated in arrival order, so the last bits and hence a few percent of the
-    # iterations of these ill-conditioned systems vary from run to run)
+    # each system's solve in the batch is its stand-alone solve: the sums of a system have
+    # the same fixed order in both (same row blocks, slots summed in block order)
     if method == dea.METHOD_PCG_F64:
         for i in (0, len(systems) - 1):
             s = systems[i]
+            ctx.set_format(fmt)
             sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+            ctx.set_format(dea.FORMAT_AUTO)
             sysd.set_rhs(s.b, None, s.b)
             st1 = sysd.solve(method, 1e-10, 200_000, dea.REL_B)
+            x1, _ = sysd.get()
             sysd.close()
-            assert abs(st1.iters - stats[i].iters) <= max(3, st1.iters // 10)
+            xb, _ = B.get(i)
+            print("stand-alone", i, st1.iters, "batch", stats[i].iters)
+            assert st1.iters == stats[i].iters
+            if fmt == dea.FORMAT_BAND:
+                assert np.array_equal(x1, xb)
     B.close()
 
 
@@ -103,10 +107,10 @@ def test_batch_reference_stopping_rule_with_warm_start(ctx, systems, truths):
         x, r = B.get(i)
         assert stats[i].status == dea.CICY_OK
         assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(s.r0)
-        # the two runs stop where their recursions cross the same residual; their distance
-        # is held to the scale of the reference's own error at that residual
-        truth, err_ref = truths[i]
+        # the device follows the reference's recursion (fixed-order sums): same iterations,
+        # voltages within 1e-8
         dist = np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"])
-        assert dist < max(1e-8, 100.0 * err_ref), (dist, err_ref)
-        assert abs(stats[i].iters - ref["iters"]) <= max(5, ref["iters"] // 20)
+        print(i, s.n, "iters", stats[i].iters, "oracle", ref["iters"], f"dist {dist:.1e}")
+        assert dist < 1e-8, dist
+        assert abs(stats[i].iters - ref["iters"]) <= max(5, ref["iters"] // 100)
     B.close()

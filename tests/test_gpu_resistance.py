@@ -1,0 +1,254 @@
+"""End-to-end resistance and voltage parity, every device method, through the host mirror of
+the reference's analysis (cicy_geom_amd.direct_electrifying):
+
+  product assembly (libcicy_net.so) -> device solve -> device boundary current
+  (cicy_boundary_current_f64) -> R = V_app / I -> per-direction parallel combination,
+
+as Perform_analysis_on_clusters / Electrical_resistance_along_each_percolated_direction /
+Calculate_parallel_resistor / Calculate_resistances_and_resistivities chain them
+(Electrical_analysis.cpp:10-145, :229-289, :355-473, :873-932).
+
+References:
+* the truth: a sparse LU solve of each system, refined with residuals summed in extended
+  precision until it stops moving, with the boundary current and the combination taken by
+  the oracle (oracle.boundary_current, oracle.parallel_resistance);
+* the reference's algorithm: oracle.pcg_sss (Solve_DEA_equations_CG_SSS restated in its
+  loop and summation order) on the same ramp-guess system, on configuration 1.
+
+Bars:
+* converged (tol 1e-14 for every method, the stop as the reference takes it: ||r|| <=
+  tol * ||r0(ramp)||): every cluster resistance and every direction's combined resistance
+  within 1e-10 relative of the truth -- or, on a system where the reference's algorithm
+  itself (oracle.pcg_sss at the same tolerance) ends further away than that, no further
+  than it does -- and the voltages within 1e-8 relative of the truth on every node of a
+  component tied to an electrode.  A floating component (a tube group whose only junction
+  to the rest reached an electrode and was dropped, Direct_Electrifying.cpp:817-837) has
+  a singular, consistent block: CG leaves it at the mean of its initial guess and an LU
+  solve at an arbitrary point of the null space, so its voltages are left out (their count
+  is printed);
+* reference rule (tol 1e-10): the device form of the reference's algorithm (drop-in seam and
+  batch) takes exactly the oracle's iterations and gives its resistances within 1e-10 and
+  its voltages within 1e-8; every other method's resistance error against the truth is
+  stated, and that of the condensed methods, which start from the same ramp guess, is
+  asserted no larger than the reference algorithm's own on the same system.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csgraph
+import scipy.sparse.linalg as sla
+
+import oracle
+from cicy_geom_amd import dea, network, warmstart
+from cicy_geom_amd import direct_electrifying as de
+
+pytestmark = pytest.mark.gpu
+
+TOL_TIGHT = 1e-14
+TOL_REF = 1e-10
+# directions solved per family (Vector_of_directions, Electrical_analysis.cpp:292-346)
+FAMILY_DIRECTIONS = {0: [0], 1: [1], 2: [2], 3: [0, 1], 4: [0, 2], 5: [1, 2], 6: [0, 1, 2]}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dea.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def model(ctx):
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "cicy_geom_amd", "weights", "warmstart.npz")
+    theta = np.load(path, allow_pickle=False)["theta"]
+    m = warmstart.WarmStartNet(ctx, theta)
+    yield m
+    m.close()
+
+
+def make_net(name):
+    if name == "c1":
+        return network.generate(1)
+    if name == "plates":
+        return network.Network(network.NetworkParams(box=2.0, n_cnt=2560, n_gnp=31, seed=5))
+    return network.Network(network.config_params(4, 0, 6000))      # "rve6k"
+
+
+def grounded_rows(s):
+    """Rows in a connected component of the free-node graph that couples to an electrode
+    (a row of A with a positive sum); the others float."""
+    rows = np.repeat(np.arange(s.n), np.diff(s.row_ptr))
+    G = sp.csr_matrix((np.ones(len(rows)), (rows, s.col_ind)), shape=(s.n, s.n))
+    ncomp, lab = csgraph.connected_components(G + G.T, directed=False)
+    rs = s.diag.copy()
+    np.add.at(rs, rows, s.values)
+    np.add.at(rs, s.col_ind, s.values)
+    tied = np.zeros(ncomp, bool)
+    tied[lab[rs > 1e-9 * s.diag]] = True
+    return tied[lab]
+
+
+def refined_truth(s):
+    """LU solve of the free block, refined with b - A x summed in extended precision."""
+    rows = np.repeat(np.arange(s.n), np.diff(s.row_ptr))
+    L = sp.csr_matrix((s.values, (rows, s.col_ind)), shape=(s.n, s.n))
+    A = (L + L.T + sp.diags(s.diag)).tocsc()
+    lu = sla.splu(A)
+    x = lu.solve(s.b)
+    Ac = A.tocoo()
+    data = Ac.data.astype(np.longdouble)
+    for _ in range(3):
+        ax = np.zeros(s.n, np.longdouble)
+        np.add.at(ax, Ac.row, data * x.astype(np.longdouble)[Ac.col])
+        x = x + lu.solve((s.b.astype(np.longdouble) - ax).astype(np.float64))
+    return x
+
+
+@pytest.fixture(scope="module")
+def truths():
+    """Per network: {(cluster, direction): (system, x_truth, R_truth)} and the per-direction
+    combination of the truth resistances."""
+    ep = network.ElectricParams()
+    out = {}
+    for name in ("c1", "plates", "rve6k"):
+        net = make_net(name)
+        per = {}
+        rk = {0: [], 1: [], 2: []}
+        for k in range(net.count("clusters")):
+            fam = net.cluster(k)[1]
+            if fam < 0:
+                continue
+            for d in FAMILY_DIRECTIONS[fam]:
+                s = net.assemble(k, 1, d, ep, direction=d)
+                x = refined_truth(s)
+                I = oracle.boundary_current(*s.boundary_elements[0], np.concatenate([s.vef, x]))
+                per[(k, d)] = (s, x, ep.applied_voltage / I, grounded_rows(s))
+                rk[d].append(ep.applied_voltage / I)
+        box = net.params.box
+        combined = [oracle.parallel_resistance(rk[d], ep.resistivity_matrix * box / (box * box))
+                    for d in range(3)]
+        out[name] = dict(net=net, per=per, combined=combined, oracle_err={})
+    return out
+
+
+def oracle_error(t, k, d, tol):
+    """|R - R_truth| / R_truth of the reference's algorithm (oracle.pcg_sss, ramp guess)
+    stopped at `tol`, computed once per system."""
+    key = (k, d, tol)
+    if key not in t["oracle_err"]:
+        s, _, Rt, _ = t["per"][(k, d)]
+        ref = oracle.pcg_sss(s.nodes, s.reserved, tol, s.row_ptr, s.col_ind, s.values, s.diag,
+                             s.r0, s.vef, s.x0)
+        I = oracle.boundary_current(*s.boundary_elements[0], ref["voltages"])
+        t["oracle_err"][key] = abs(1.0 / I - Rt) / Rt
+    return t["oracle_err"][key]
+
+
+METHODS = ["pcg_f64", "mixed", "resident", "condensed", "condensed_block", "learned_block",
+           "learned_pcg", "batched_pcg", "batched_mixed"]
+CODES = {"pcg_f64": dea.METHOD_PCG_F64, "mixed": dea.METHOD_MIXED,
+         "resident": dea.METHOD_RESIDENT, "condensed": dea.METHOD_CONDENSED,
+         "condensed_block": dea.METHOD_CONDENSED_BLOCK,
+         "learned_block": dea.METHOD_CONDENSED_BLOCK, "learned_pcg": dea.METHOD_PCG_F64,
+         "batched_pcg": dea.METHOD_PCG_F64, "batched_mixed": dea.METHOD_MIXED}
+
+
+def run_method(ctx, model, net, method, tol):
+    ep = network.ElectricParams()
+    code = CODES[method]
+    if method.startswith("batched"):
+        return de.network_resistances_batched(net, ep, tol, code, ctx, with_details=True)
+    ws = model if method.startswith("learned") else None
+    return de.network_resistances(net, ep, tol, code, ctx, with_details=True, warm_start=ws)
+
+
+def voltages_of(ctx, model, net, method, k, d, tol):
+    """The voltages [VEF | x] of one cluster and direction as the method computes them."""
+    ep = network.ElectricParams()
+    code = CODES[method]
+    if method.startswith("batched"):
+        s = net.assemble(k, 1, d, ep, direction=d)
+        B = ctx.batch([(s.row_ptr, s.col_ind, s.values, s.diag)])
+        B.set_rhs(0, s.r0, s.x0, s.b)
+        B.solve(code, tol, 0, dea.REL_R0)
+        x, _ = B.get(0)
+        B.close()
+        return np.concatenate([s.vef, x])
+    obj = de.DirectElectrifying(ctx, code, model if method.startswith("learned") else None)
+    assert obj.compute_voltage_field(k, 1, net, d, ep, tol, direction=d), obj.last_error
+    return obj.voltages
+
+
+@pytest.mark.parametrize("name", ["c1", "rve6k", "plates"])
+@pytest.mark.parametrize("method", METHODS)
+def test_resistance_converged(ctx, model, truths, name, method):
+    t = truths[name]
+    R, rho, det = run_method(ctx, model, t["net"], method, TOL_TIGHT)
+    worst = 0.0
+    for d in range(3):
+        got = sorted((k, Rk) for k, Rk, _, _, _ in det[d])
+        want = sorted((k, t["per"][(k, dd)][2]) for (k, dd) in t["per"] if dd == d)
+        assert [k for k, _ in got] == [k for k, _ in want]
+        bar_dir = 1e-10
+        for (k, Rk), (_, Rt) in zip(got, want):
+            e = abs(Rk - Rt) / Rt
+            bar = 1e-10
+            if e > bar:
+                bar = max(bar, oracle_error(t, k, d, TOL_TIGHT))
+            bar_dir = max(bar_dir, bar)
+            print(f"{name} {method} cluster {k} dir {d}: R={Rk:.12e} |R-R_truth|/R={e:.1e} "
+                  f"(bar {bar:.1e})")
+            assert e <= bar, (k, d, e, bar)
+        assert abs(R[d] - t["combined"][d]) <= bar_dir * t["combined"][d]
+        assert rho[d] == pytest.approx(R[d] * t["net"].params.box, rel=1e-15)
+    for (k, d), (s, x, _, tied) in t["per"].items():
+        v = voltages_of(ctx, model, t["net"], method, k, d, TOL_TIGHT)[s.reserved:]
+        e = np.linalg.norm(v[tied] - x[tied]) / np.linalg.norm(x[tied])
+        print(f"{name} {method} cluster {k} dir {d}: |x-x_truth|/|x_truth|={e:.1e} on "
+              f"{tied.sum()} rows ({(~tied).sum()} floating)")
+        assert e <= 1e-8, (k, d, e)
+
+
+@pytest.mark.parametrize("method", ["pcg_f64", "batched_pcg"])
+def test_reference_rule_matches_oracle_c1(ctx, model, truths, method):
+    """The reference's algorithm on the device at the reference's stopping rule: same
+    iteration counts as the oracle, resistances within 1e-10, voltages within 1e-8."""
+    t = truths["c1"]
+    R, _, det = run_method(ctx, model, t["net"], method, TOL_REF)
+    for d in range(3):
+        for k, Rk, _, _, st in det[d]:
+            s = t["per"][(k, d)][0]
+            ref = oracle.pcg_sss(s.nodes, s.reserved, TOL_REF, s.row_ptr, s.col_ind, s.values,
+                                 s.diag, s.r0, s.vef, s.x0)
+            I = oracle.boundary_current(*s.boundary_elements[0], ref["voltages"])
+            R_ref = 1.0 / I
+            print(f"c1 {method} cluster {k} dir {d}: iters {st.iters} (oracle {ref['iters']}) "
+                  f"|R-R_ref|/R={abs(Rk - R_ref) / R_ref:.1e}")
+            assert st.iters == ref["iters"]
+            assert abs(Rk - R_ref) <= 1e-10 * R_ref
+            v = voltages_of(ctx, model, t["net"], method, k, d, TOL_REF)
+            assert np.linalg.norm(v - ref["voltages"]) <= 1e-8 * np.linalg.norm(ref["voltages"])
+
+
+@pytest.mark.parametrize("name", ["c1", "rve6k", "plates"])
+def test_reference_rule_errors(ctx, model, truths, name):
+    """At the reference's stopping rule (tol 1e-10 against the ramp guess's residual) every
+    method's resistance error against the truth is stated; the condensed methods' error is no
+    larger than that of the reference's algorithm (the device form, shown identical to the
+    oracle's on configuration 1) on each system."""
+    t = truths[name]
+    errs = {}
+    for method in METHODS:
+        _, _, det = run_method(ctx, model, t["net"], method, TOL_REF)
+        for d in range(3):
+            for k, Rk, _, _, st in det[d]:
+                Rt = t["per"][(k, d)][2]
+                errs[(method, k, d)] = abs(Rk - Rt) / Rt
+    for (method, k, d), e in sorted(errs.items()):
+        print(f"{name} tol 1e-10 {method:16s} cluster {k} dir {d}: |R-R_truth|/R={e:.2e} "
+              f"(reference algorithm {errs[('pcg_f64', k, d)]:.2e})")
+    for (method, k, d), e in errs.items():
+        if method in ("condensed", "condensed_block"):
+            assert e <= max(errs[("pcg_f64", k, d)], 1e-12), (method, k, d, e)

@@ -215,3 +215,44 @@ def test_cpp_dropin_program():
     out = subprocess.run([exe, "3000"], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
     assert out.returncode == 0 and out.stdout.startswith("OK")
+
+
+def test_repeat_solves_are_bit_identical(ctx, c1):
+    """The device sums have a fixed order (per-row products, slot reductions summed in
+    work-group order), so two solves of the same system give the same bits and the same
+    iteration count."""
+    runs = []
+    for _ in range(2):
+        ok, volt, x, r, st = ctx.solve_dea_cg_sss(c1.nodes, c1.reserved, 1e-10, c1.col_ind,
+                                                  c1.row_ptr, c1.values, c1.diag, c1.r0,
+                                                  c1.vef, c1.x0)
+        assert ok and st.status == dea.CICY_OK
+        runs.append((x.copy(), r.copy(), st.iters))
+    assert runs[0][2] == runs[1][2]
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+def test_condensed_passes_used_up_continue_with_fp64_pcg(ctx, c1, monkeypatch):
+    """When the condensed solve's correction passes run out short of the tolerance (forced
+    here by capping each pass at 3 core iterations), the solve continues from x with the
+    reference's FP64 Jacobi-PCG and reports convergence only when it is reached."""
+    monkeypatch.setenv("CICY_CORE_PASS_ITERS", "3")
+    for method in (dea.METHOD_CONDENSED, dea.METHOD_CONDENSED_BLOCK):
+        sysd = ctx.system(c1.row_ptr, c1.col_ind, c1.values, c1.diag)
+        sysd.set_rhs(c1.r0, c1.x0, c1.b)
+        st = sysd.solve(method, 1e-10, 0, dea.REL_R0)
+        x, r = sysd.get()
+        sysd.close()
+        print(dea.METHOD_NAMES[method], st.as_dict())
+        assert st.status == dea.CICY_OK
+        assert st.fp64_iters > 0          # the FP64 continuation ran
+        # the reference's stopping rule holds for the returned state
+        assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(c1.r0) * (1 + 1e-6)
+        ax = oracle.spmv_sss(c1.row_ptr, c1.col_ind, c1.values, c1.diag, x)
+        assert np.linalg.norm(c1.b - ax) <= 1e-9 * np.linalg.norm(c1.r0)
+    monkeypatch.delenv("CICY_CORE_PASS_ITERS")
+    sysd = ctx.system(c1.row_ptr, c1.col_ind, c1.values, c1.diag)
+    sysd.set_rhs(c1.r0, c1.x0, c1.b)
+    st = sysd.solve(dea.METHOD_CONDENSED_BLOCK, 1e-10, 0, dea.REL_R0)
+    sysd.close()
+    assert st.status == dea.CICY_OK and st.fp64_iters == 0
