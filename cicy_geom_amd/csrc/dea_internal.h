@@ -110,6 +110,7 @@ struct cicy_ctx {
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
     cicy::PcgState *host_state = nullptr;  // pinned, 2 slots
     double *bc_ws = nullptr;   // boundary-current workspace (grows, kept until destroy)
+    double *scratch = nullptr; // this device's copy of the allocation-free reduction words
     size_t bc_words = 0;
     char err[512] = {0};
 };

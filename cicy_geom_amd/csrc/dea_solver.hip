@@ -7,6 +7,7 @@
 // in dea_kernels.h.
 #include "dea_internal.h"
 
+#include <mutex>
 #include <thread>
 
 #include <algorithm>
@@ -94,6 +95,8 @@ constexpr int kBcGroups = 1024;  // most work-groups of one launch
 // reduction words of a launch: work-group sums, group sums, then kBcGroups + 2 counters
 // (group arrivals, the final arrival, the bad-node flag) packed as 32-bit words
 constexpr size_t kBcRedWords = 2 * (size_t)kBcGroups + ((size_t)kBcGroups + 2 + 1) / 2;
+// largest element or node count of the allocation-free entry point
+constexpr int64_t kBcMaxElements = (1LL << 40);
 
 __global__ __launch_bounds__(kThreads) void k_boundary_current(
     long long ne, const long long *__restrict__ nb, const long long *__restrict__ ni,
@@ -156,6 +159,32 @@ void bc_launch(hipStream_t st, long long ne, const long long *nb, const long lon
                        dim3(kThreads), 0, st, r.rb.gcount, kBcGroups + 2);
     hipLaunchKernelGGL(k_boundary_current, dim3(bc_grid(ne)), dim3(kThreads), 0, st, ne, nb, ni,
                        re, v, nv, out, r.rb, r.top, r.bad);
+}
+
+}  // namespace
+
+// The reduction words of cicy_boundary_current_f64_noalloc: a file-scope device array, one
+// copy on every GPU the library runs on, held until the process ends (kBcRedWords doubles,
+// about 20 KB).  Host code never uses this name as an address: the device address of each
+// GPU's copy is looked up once (scratch_address) and kept in g_scratch.
+static __device__ __align__(256) double scratch[kBcRedWords];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// With `device` current: the device address of its copy of `scratch`, looked up once.
+double *scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch[device] = static_cast<double *>(p);
+    }
+    return g_scratch[device];
 }
 
 }  // namespace
@@ -667,7 +696,8 @@ int cicy_create(cicy_ctx **out, int device) {
         hipEventCreateWithFlags(&ctx->ev_poll[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_poll[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&ctx->host_state, 2 * sizeof(PcgState), hipHostMallocDefault) !=
-            hipSuccess) {
+            hipSuccess ||
+        !(ctx->scratch = scratch_address(device))) {
         delete ctx;
         return 0;
     }
@@ -1232,6 +1262,45 @@ int cicy_boundary_current_f64(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
              hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     }
     return ok ? 1 : 0;
+}
+
+int cicy_boundary_current_f64_noalloc(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
+                                      const int64_t *n_int, const double *re,
+                                      const double *voltages, int64_t nv, double *current) {
+    if (!ctx) return 0;
+    if (ne < 0 || nv < 0 || ne > kBcMaxElements || nv > kBcMaxElements) {
+        set_error(ctx, "cicy_boundary_current_f64_noalloc: sizes must lie in [0, %lld] "
+                       "(ne=%lld nv=%lld)", (long long)kBcMaxElements, (long long)ne,
+                  (long long)nv);
+        return 0;
+    }
+    if (!current || (ne > 0 && (!n_bnd || !n_int || !re || !voltages))) {
+        set_error(ctx, "cicy_boundary_current_f64_noalloc: null device array");
+        return 0;
+    }
+    if (!ctx->scratch) {
+        set_error(ctx, "cicy_boundary_current_f64_noalloc: no scratch address for device %d",
+                  ctx->device);
+        return 0;
+    }
+    if (!hip_ok(ctx, hipSetDevice(ctx->device), "hipSetDevice")) return 0;
+    // every work-group's partial, the group sums, the counters and the bad-node flag live in
+    // this GPU's scratch; bc_launch clears the counters and the flag with a kernel first
+    bc_launch(ctx->stream, ne, (const long long *)n_bnd, (const long long *)n_int, re, voltages,
+              nv, current, ctx->scratch);
+    unsigned int bad = 0;
+    const BcRed r = bc_red(ctx->scratch);
+    if (!hip_ok(ctx, hipGetLastError(), "launch") ||
+        !hip_ok(ctx, hipMemcpyAsync(&bad, r.bad, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream),
+                "D2H") ||
+        !hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync"))
+        return 0;
+    if (bad) {
+        set_error(ctx, "cicy_boundary_current_f64_noalloc: a boundary element has a node "
+                       "outside [0, %lld)", (long long)nv);
+        return 0;
+    }
+    return 1;
 }
 
 int cicy_host_blocked_spmv(int64_t n, const int64_t *row_ptr, const int64_t *col_ind,

@@ -43,6 +43,7 @@ EXPORTED_SYMBOLS = [
     "cicy_sys_core", "cicy_set_dropin_method", "cicy_sys_set_blocks", "cicy_host_core_block_cg",
     "cicy_sys_prepare",
     "cicy_set_dropin_blocks", "cicy_backbone_scan", "cicy_cnt_contacts",
+    "cicy_boundary_current_f64_noalloc",
 ]
 
 
@@ -102,6 +103,8 @@ def lib():
                                     _F64, _F64]
     L.cicy_boundary_current_f64.argtypes = [_VP, ctypes.c_int64, _I64, _I64, _F64, _F64,
                                             ctypes.c_int64, P(ctypes.c_double)]
+    L.cicy_boundary_current_f64_noalloc.argtypes = [_VP, ctypes.c_int64, _VP, _VP, _VP, _VP,
+                                                    ctypes.c_int64, _VP]
     L.cicy_sys_create.argtypes = [_VP, ctypes.c_int64, ctypes.c_int64, _I64, _I64, _F64, _F64,
                                   P(_VP)]
     L.cicy_sys_destroy.argtypes = [_VP]
@@ -292,6 +295,13 @@ class Context:
             self._h, len(n_bnd), _i64(n_bnd), _i64(n_int), _f64(re), _f64(voltages),
             len(voltages), ctypes.byref(out)), "cicy_boundary_current_f64")
         return out.value
+
+    def boundary_current_noalloc(self, ne, d_n_bnd, d_n_int, d_re, d_voltages, nv, d_current):
+        """cicy_boundary_current_f64_noalloc: every argument a device address (int); the
+        result is written to d_current on the device."""
+        self._check(self._L.cicy_boundary_current_f64_noalloc(
+            self._h, ne, d_n_bnd, d_n_int, d_re, d_voltages, nv, d_current),
+            "cicy_boundary_current_f64_noalloc")
 
     def backbone_scan(self, voltages, tube_off, tube_nodes, pair_a, pair_b):
         """Zero current and per-tube backbone runs of a unit-resistor solve on the device

@@ -143,6 +143,18 @@ int cicy_spmv_sss_f64(cicy_ctx *ctx, int64_t n, const int64_t *row_ptr,
 int cicy_boundary_current_f64(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
                               const int64_t *n_int, const double *re,
                               const double *voltages, int64_t nv, double *current);
+/* The same sum with every array in device memory (n_bnd, n_int, re: ne entries; voltages: nv
+ * entries; current: one double, written) and no allocation at all: no hipMalloc, hipFree or
+ * allocating library call.  It has no counterpart in the reference; it is for callers that
+ * cannot allocate (code under graph capture, tight serving loops).  Its reduction words sit in
+ * a file-scope device array of about 20 KB, one copy per GPU, held until the process ends;
+ * every call on a GPU shares that GPU's copy, so calls that overlap on one GPU race.  Runs on
+ * the context's stream and returns after it has finished.  Sizes above 2^40 and a node index
+ * outside [0, nv) are refused: the call returns 0 and cicy_last_error says why.  The result
+ * is bit-identical to cicy_boundary_current_f64's on the same data (same kernel and grid). */
+int cicy_boundary_current_f64_noalloc(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
+                                      const int64_t *n_int, const double *re,
+                                      const double *voltages, int64_t nv, double *current);
 
 /* ---- backbone extraction (Backbone_Network.cpp) ----------------------------------------
  * The data-parallel part of Backbone_Network::Determine_backbone_network
