@@ -262,6 +262,111 @@ def bench_backbone(args):
         flush=True)
 
 
+def batch_realisation(k):
+    """Systems of realisation k of the batch workload (configuration 4; seed 4_000_000 + k):
+    U[2k, 40k] tubes at the density of the 5 um / 40k-tube RVE, the largest cluster
+    percolating along each direction, R_flag = 1 (family := direction)."""
+    from cicy_geom_amd import network
+    out = []
+    net = network.generate(4, realisation=k)
+    for d in range(3):
+        ks = net.percolating_clusters(d)
+        if ks:
+            c = max(ks, key=lambda q: net.cluster(q)[0])
+            out.append(net.assemble(c, 1, d, direction=d))
+    net.close()
+    return out
+
+
+def bench_batch(args):
+    """The batch workload (configuration 4): every percolating direction of R realisations
+    with U[2k, 40k] tubes, solved to ||b - A x|| <= 1e-10 ||b|| from the zero guess.
+      (a) all R realisations with the block-preconditioned condensed solve, one new system
+          after another, each timed from its host arrays (upload, layout, set-up, solve,
+          read-back); generation is outside the timed region;
+      (b) the first --batch-pcg-realisations of them in one batch (cicy_batch: the systems
+          concatenated, per-system scalars and stopping tests in shared launches, FP64
+          Jacobi-PCG, the reference's algorithm), timed from the host arrays likewise.
+    value = systems / s of (a) over all R realisations."""
+    from cicy_geom_amd import dea
+    ctx = dea.Context(0)
+    t_gen, wall, count, iters, worst, rows = 0.0, 0.0, 0, 0, 0.0, []
+    kept = []
+    for k in range(args.batch_realisations):
+        t = time.perf_counter()
+        systems = batch_realisation(k)
+        t_gen += time.perf_counter() - t
+        if k < args.batch_pcg_realisations:
+            kept.extend(systems)
+        for s in systems:
+            t = time.perf_counter()
+            sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+            sysd.set_rhs(s.b, None, s.b)
+            st = sysd.solve(dea.METHOD_CONDENSED_BLOCK, args.tol, MAX_ITER, dea.REL_B)
+            sysd.get()
+            sysd.close()
+            wall += time.perf_counter() - t
+            count += 1
+            iters += st.iters
+            rows.append(s.n)
+            worst = max(worst, st.true_rel_res)
+            if st.status != dea.CICY_OK:
+                log(f"[bench] WARNING condensed system N={s.n} did not converge")
+        if (k + 1) % 16 == 0:
+            log(f"[bench] batch: {k + 1} of {args.batch_realisations} realisations, {count} "
+                f"systems, {wall:.1f} s of solves ({t_gen:.1f} s generating)")
+    out = {"condensed_block_sequential": dict(
+        systems=count, seconds=wall, solves_per_s=count / wall, core_iters=iters,
+        worst_true_rel_res=worst, rows_min=min(rows), rows_max=max(rows),
+        rows_total=int(sum(rows)))}
+    log(f"[bench] batch condensed_block: {out['condensed_block_sequential']}")
+    if kept and not args.no_plain:
+        t = time.perf_counter()
+        B = ctx.batch([(s.row_ptr, s.col_ind, s.values, s.diag) for s in kept])
+        for i, s in enumerate(kept):
+            B.set_rhs(i, s.b, None, s.b)
+        t1 = time.perf_counter()
+        stats, dev_ms = B.solve(dea.METHOD_PCG_F64, args.tol, 0, dea.REL_B)
+        for i in range(len(kept)):
+            B.get(i)
+        pw = time.perf_counter() - t
+        B.close()
+        t = time.perf_counter()
+        cw = 0.0
+        for s in kept:   # the same systems with the condensed solve, for the comparison
+            t2 = time.perf_counter()
+            sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+            sysd.set_rhs(s.b, None, s.b)
+            sysd.solve(dea.METHOD_CONDENSED_BLOCK, args.tol, MAX_ITER, dea.REL_B)
+            sysd.get()
+            sysd.close()
+            cw += time.perf_counter() - t2
+        out["batched_pcg_f64"] = dict(systems=len(kept), seconds=pw, create_s=t1 - t,
+                                      solves_per_s=len(kept) / pw,
+                                      iters=int(sum(x.iters for x in stats)),
+                                      max_iters=int(max(x.iters for x in stats)),
+                                      worst_true_rel_res=max(x.true_rel_res for x in stats),
+                                      converged=int(sum(x.status == dea.CICY_OK
+                                                        for x in stats)),
+                                      condensed_block_same_systems_s=cw,
+                                      speedup_condensed_vs_batched=pw / cw)
+        log(f"[bench] batch pcg_f64: {out['batched_pcg_f64']}")
+    ctx.close()
+    a = out["condensed_block_sequential"]
+    print(json.dumps({
+        "metric": "DEA solves/s to 1e-10 residual, batch workload (configuration 4)",
+        "value": a["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": a["systems"], "warmup": 0, "ms_per_step": 1e3 * a["seconds"] / a["systems"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"configuration 4: {args.batch_realisations} realisations x "
+                               f"percolating directions = {a['systems']} systems, U[2k,40k] "
+                               f"tubes, N {a['rows_min']}..{a['rows_max']}",
+                   "method": "condensed_block",
+                   "step": "one system from host arrays: upload + set-up + solve + read-back"},
+        "generation_s": t_gen, "methods": out}), flush=True)
+
+
 def bench_netio(args):
     """Network files of the contact-search workload (5.5 M points in a 4 um cube): write and
     read cnt_struct/cnt_coordinates as csv (6 digits) and dat; host only."""
@@ -308,7 +413,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 reference line (profiling runs of one method)")
-    ap.add_argument("--methods", default="pcg_f64,condensed_block,learned_block,learned_f64",
+    ap.add_argument("--methods",
+                    default="pcg_f64,condensed_block,learned_block,learned_f64,mixed,"
+                            "learned_mixed",
                     help="comma list from pcg_f64, mixed, resident, condensed, "
                          "condensed_resident, condensed_block, learned_f64, learned_mixed, "
                          "learned_block (the learned ones need trained weights)")
@@ -319,11 +426,15 @@ def main():
                     help="only run the SpMV kernels (both precisions) and a 1 GiB reference "
                          "copy, for hardware-counter runs")
     ap.add_argument("--workload", default="dea",
-                    choices=["dea", "contacts", "backbone", "netio"],
+                    choices=["dea", "contacts", "backbone", "netio", "batch"],
                     help="dea: the conductance solve (the default line); contacts: the tube "
                          "contact search and cluster labels on synthetic wavy tubes; backbone: "
                          "the backbone scan and pruning of the configuration's largest "
                          "cluster; netio: reading and writing the network files")
+    ap.add_argument("--batch-realisations", type=int, default=256,
+                    help="realisations of the batch workload (configuration 4)")
+    ap.add_argument("--batch-pcg-realisations", type=int, default=8,
+                    help="of those, how many are also solved as one FP64 PCG batch")
     ap.add_argument("--contacts-box", type=float, default=4.0,
                     help="cube side (um) of the contact-search workload")
     args = ap.parse_args()
@@ -342,6 +453,9 @@ def main():
         return
     if args.workload == "netio":
         bench_netio(args)
+        return
+    if args.workload == "batch":
+        bench_batch(args)
         return
     t0 = time.perf_counter()
     net = network.generate(args.config)

@@ -27,8 +27,10 @@
 #include "dea_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 
 using namespace cicy;
@@ -303,6 +305,7 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
         *why = "empty system";
         return false;
     }
+    StageClock clk;
     std::vector<int> &anchor_of = C.anchor_of, &anchors = C.anchors;
     anchor_of.assign((size_t)n, -1);
     anchors.clear();
@@ -333,99 +336,164 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
         int r, c;
         double v;
     };
-    std::vector<Fill> fill;
-    std::vector<double> g;
     const std::vector<double> &band = bh.band;  // [n + 1], band[i] = a(i, i-1)
-    for (int i = 0; i < n;) {
-        if (anchor_of[i] >= 0) {
-            i++;
-            continue;
-        }
-        const int s = i;
-        int e = s;
-        while (e + 1 < n && anchor_of[e + 1] < 0 && band[e + 1] != 0.0) e++;
-        const int L = (s > 0 && band[s] != 0.0) ? s - 1 : -1;
-        const int R = (e + 1 < n && band[e + 1] != 0.0) ? e + 1 : -1;
-        ps.push_back(s);
-        pe.push_back(e);
-        pl.push_back(L >= 0 ? anchor_of[L] : -1);
-        pr.push_back(R >= 0 ? anchor_of[R] : -1);
-        mfac[s] = h.diag[s];
-        for (int k = s + 1; k <= e; k++) {
-            lfac[k] = band[k] / mfac[k - 1];
-            mfac[k] = h.diag[k] - lfac[k] * band[k];
-        }
-        // A pivot can only vanish on a path that touches no anchor and no electrode: a
-        // floating tube (for instance one whose only junction reaches an electrode and was
-        // dropped, Direct_Electrifying.cpp:817-837), singular on its own (its rows of A sum
-        // to zero).  Its block is singular but consistent: r0 = b - A x0 = -A x0 there.  CG's
-        // correction stays orthogonal to the constant null vector, so the tube ends at the
-        // mean of its initial guess; the condensed solve gives the same: the path is solved
-        // grounded at its last row (the leading block keeps its pivots) and the correction's
-        // mean is removed.
-        const double pivot_floor = 1e-12 * h.diag[e];
-        bool singular = false;
-        for (int k = s; k <= e; k++)
-            if (!(mfac[k] > pivot_floor) || !std::isfinite(mfac[k])) singular = true;
-        if (singular) {
-            if (L >= 0 || R >= 0) {
-                *why = "non-positive pivot on a path coupled to the rest (not an M-matrix)";
-                return false;
+    // A path starts at a non-anchor row whose predecessor is an anchor or not coupled to it.
+    auto path_start = [&](int i) {
+        return anchor_of[i] < 0 && (i == 0 || anchor_of[i - 1] >= 0 || band[i] == 0.0);
+    };
+    // The rows are cut into one range per host thread at anchors or path starts, so no path
+    // crosses a cut.  Each thread factorises its paths (lfac/mfac rows are its own) and
+    // collects, in row order, its paths, its Schur terms on the anchors' diagonals (an anchor
+    // can border paths of two threads) and its fill; these are merged in thread order, which
+    // is the sequential order, so the result does not depend on the thread count.
+    const int nth = host_threads(n, 1 << 16);
+    std::vector<int> cut((size_t)nth + 1, n);
+    cut[0] = 0;
+    for (int t = 1; t < nth; t++) {
+        int b = (int)((long long)n * t / nth);
+        b = std::max(b, cut[t - 1]);
+        while (b < n && anchor_of[b] < 0 && !path_start(b)) b++;
+        cut[t] = b;
+    }
+    struct Part {
+        std::vector<int> ps, pe, pl, pr;
+        std::vector<std::pair<int, double>> dsub;  // (anchor, term subtracted from sdiag)
+        std::vector<Fill> fill, jfill;  // path fill; couplings of the original matrix
+        const char *why = nullptr;
+    };
+    std::vector<Part> parts((size_t)nth);
+    parallel_for(nth, [&](int t) {
+        Part &P = parts[(size_t)t];
+        std::vector<double> g;
+        for (int i = cut[t]; i < cut[t + 1];) {
+            if (anchor_of[i] >= 0) {
+                i++;
+                continue;
             }
-            for (int k = s; k < e; k++)
-                if (!(mfac[k] > pivot_floor) || !std::isfinite(mfac[k])) {
-                    *why = "non-positive pivot inside a floating path";
-                    return false;
+            const int s = i;
+            int e = s;
+            while (e + 1 < n && anchor_of[e + 1] < 0 && band[e + 1] != 0.0) e++;
+            const int L = (s > 0 && band[s] != 0.0) ? s - 1 : -1;
+            const int R = (e + 1 < n && band[e + 1] != 0.0) ? e + 1 : -1;
+            P.ps.push_back(s);
+            P.pe.push_back(e);
+            P.pl.push_back(L >= 0 ? anchor_of[L] : -1);
+            P.pr.push_back(R >= 0 ? anchor_of[R] : -1);
+            mfac[s] = h.diag[s];
+            for (int k = s + 1; k <= e; k++) {
+                lfac[k] = band[k] / mfac[k - 1];
+                mfac[k] = h.diag[k] - lfac[k] * band[k];
+            }
+            // A pivot can only vanish on a path that touches no anchor and no electrode: a
+            // floating tube (for instance one whose only junction reaches an electrode and was
+            // dropped, Direct_Electrifying.cpp:817-837), singular on its own (its rows of A
+            // sum to zero).  Its block is singular but consistent: r0 = b - A x0 = -A x0
+            // there.  CG's correction stays orthogonal to the constant null vector, so the
+            // tube ends at the mean of its initial guess; the condensed solve gives the same:
+            // the path is solved grounded at its last row (the leading block keeps its
+            // pivots) and the correction's mean is removed.
+            const double pivot_floor = 1e-12 * h.diag[e];
+            bool singular = false;
+            for (int k = s; k <= e; k++)
+                if (!(mfac[k] > pivot_floor) || !std::isfinite(mfac[k])) singular = true;
+            if (singular) {
+                if (L >= 0 || R >= 0) {
+                    P.why = "non-positive pivot on a path coupled to the rest (not an M-matrix)";
+                    return;
                 }
-            pl.back() = kFloatingPath;
-            lfac[e] = 0.0;
-            mfac[e] = 1.0;
+                for (int k = s; k < e; k++)
+                    if (!(mfac[k] > pivot_floor) || !std::isfinite(mfac[k])) {
+                        P.why = "non-positive pivot inside a floating path";
+                        return;
+                    }
+                P.pl.back() = kFloatingPath;
+                lfac[e] = 0.0;
+                mfac[e] = 1.0;
+                i = e + 1;
+                continue;
+            }
+            if (L >= 0) {
+                // u = T^-1 (a(s, L) e_s): forward sweep from g_s = a(s, L), then
+                // back-substitution
+                g.assign((size_t)(e - s + 1), 0.0);
+                g[0] = band[s];
+                for (int k = s + 1; k <= e; k++) g[k - s] = -lfac[k] * g[k - 1 - s];
+                double u = g[e - s] / mfac[e];
+                const double ue = u;
+                for (int k = e - 1; k >= s; k--) u = (g[k - s] - band[k + 1] * u) / mfac[k];
+                P.dsub.push_back({anchor_of[L], band[s] * u});  // u is u_s here
+                if (R >= 0) P.fill.push_back({anchor_of[R], anchor_of[L], -band[e + 1] * ue});
+            }
+            if (R >= 0) {
+                // v = T^-1 (a(R, e) e_e): only its last entry is needed
+                P.dsub.push_back({anchor_of[R], band[e + 1] * (band[e + 1] / mfac[e])});
+            }
             i = e + 1;
-            continue;
         }
-        if (L >= 0) {
-            // u = T^-1 (a(s, L) e_s): forward sweep from g_s = a(s, L), then back-substitution
-            g.assign((size_t)(e - s + 1), 0.0);
-            g[0] = band[s];
-            for (int k = s + 1; k <= e; k++) g[k - s] = -lfac[k] * g[k - 1 - s];
-            double u = g[e - s] / mfac[e];
-            const double ue = u;
-            for (int k = e - 1; k >= s; k--) u = (g[k - s] - band[k + 1] * u) / mfac[k];
-            sdiag[anchor_of[L]] -= band[s] * u;  // u is u_s here
-            if (R >= 0) fill.push_back({anchor_of[R], anchor_of[L], -band[e + 1] * ue});
+        // couplings among this range's anchors and earlier anchors in the original matrix
+        // (junctions, adjacent anchors)
+        for (int r = cut[t]; r < cut[t + 1]; r++) {
+            if (anchor_of[r] < 0) continue;
+            for (int k = h.rp[r]; k < h.rp[r + 1]; k++) {
+                const int c = h.col[k];
+                if (anchor_of[c] >= 0) P.jfill.push_back({anchor_of[r], anchor_of[c], h.val[k]});
+            }
         }
-        if (R >= 0) {
-            // v = T^-1 (a(R, e) e_e): only its last entry is needed
-            sdiag[anchor_of[R]] -= band[e + 1] * (band[e + 1] / mfac[e]);
+    });
+    clk.mark("paths: anchors + factors");
+    for (const Part &P : parts)
+        if (P.why) {
+            *why = P.why;
+            return false;
         }
-        i = e + 1;
+    size_t nfill = 0;
+    for (const Part &P : parts) {
+        ps.insert(ps.end(), P.ps.begin(), P.ps.end());
+        pe.insert(pe.end(), P.pe.begin(), P.pe.end());
+        pl.insert(pl.end(), P.pl.begin(), P.pl.end());
+        pr.insert(pr.end(), P.pr.begin(), P.pr.end());
+        for (const auto &d : P.dsub) sdiag[d.first] -= d.second;
+        nfill += P.fill.size() + P.jfill.size();
     }
-    // couplings among anchors in the original matrix (junctions, adjacent anchors)
-    for (int r = 0; r < n; r++) {
-        if (anchor_of[r] < 0) continue;
-        for (int k = h.rp[r]; k < h.rp[r + 1]; k++) {
-            const int c = h.col[k];
-            if (anchor_of[c] >= 0) fill.push_back({anchor_of[r], anchor_of[c], h.val[k]});
-        }
-    }
-    std::sort(fill.begin(), fill.end(),
-              [](const Fill &a, const Fill &b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+    // S in rows: the path fill of every thread in order, then the couplings of every thread
+    // in order (the sequential order), placed by a stable counting pass on the row and a
+    // stable sort on the column within each row; entries with the same (row, column) are
+    // summed in that order
     std::vector<int64_t> &rrp = C.rrp, &rcol = C.rcol;
     std::vector<double> &rval = C.rval;
+    std::vector<int> rcnt((size_t)m + 1, 0);
+    for (const Part &P : parts) {
+        for (const Fill &f : P.fill) rcnt[(size_t)f.r + 1]++;
+        for (const Fill &f : P.jfill) rcnt[(size_t)f.r + 1]++;
+    }
+    for (int j = 0; j < m; j++) rcnt[j + 1] += rcnt[j];
+    std::vector<std::pair<int, double>> byrow(nfill);
+    {
+        std::vector<int> at(rcnt.begin(), rcnt.end() - 1);
+        for (const Part &P : parts)
+            for (const Fill &f : P.fill) byrow[(size_t)at[f.r]++] = {f.c, f.v};
+        for (const Part &P : parts)
+            for (const Fill &f : P.jfill) byrow[(size_t)at[f.r]++] = {f.c, f.v};
+    }
     rrp.assign((size_t)m + 1, 0);
     rcol.clear();
     rval.clear();
-    for (size_t k = 0; k < fill.size();) {
-        size_t q = k;
-        double v = 0.0;
-        while (q < fill.size() && fill[q].r == fill[k].r && fill[q].c == fill[k].c) v += fill[q++].v;
-        rcol.push_back(fill[k].c);
-        rval.push_back(v);
-        rrp[(size_t)fill[k].r + 1]++;
-        k = q;
+    for (int j = 0; j < m; j++) {
+        auto b = byrow.begin() + rcnt[j], e = byrow.begin() + rcnt[j + 1];
+        std::stable_sort(b, e, [](const std::pair<int, double> &x,
+                                  const std::pair<int, double> &y) { return x.first < y.first; });
+        for (auto q = b; q != e;) {
+            auto w = q;
+            double v = 0.0;
+            while (w != e && w->first == q->first) v += (w++)->second;
+            rcol.push_back(q->first);
+            rval.push_back(v);
+            rrp[(size_t)j + 1]++;
+            q = w;
+        }
     }
     for (int j = 0; j < m; j++) rrp[j + 1] += rrp[j];
-
+    clk.mark("paths: merge + reduced rows");
     return true;
 }
 
@@ -450,89 +518,147 @@ struct SeriesHost {
     std::vector<int> core_nodes;               // [mc] S index of each core row
     std::vector<int64_t> crp, ccol;            // core SSS
     std::vector<double> cval, cdiag;
+    // receivers of each round's records, ascending, each with its contributions in record
+    // order: rv_node[rv_round_off[q] ..], contributions rv_off[j] .. rv_off[j + 1] of
+    // (record rv_rec, coupling rv_a)
+    std::vector<int> rv_round_off, rv_node, rv_off, rv_rec;
+    std::vector<double> rv_a;
 };
 
-// Adjacency lists of S in one pool: node u's neighbours are nbr[off[u] .. off[u] + len[u])
-// with room for cap[u]; a list that outgrows its room moves to the end of the pool with
-// twice the room.  Removal swaps the last entry into the gap, insertion appends, so the
-// order of a list depends only on the sequence of operations on it.
-struct AdjPool {
-    std::vector<int> nbr, off, len, cap;
-    std::vector<double> val;
+// Adjacency lists of S: node u's neighbours are nb[u][0 .. len[u]) with couplings vv[u][..]
+// and room for cap[u].  A list that outgrows its room moves to twice the room in the arena
+// of the thread that grows it (arenas are chunks that never move, so the lists of other
+// nodes stay where they are while threads work on disjoint nodes).  Removal swaps the last
+// entry into the gap, insertion appends, so the order of a list depends only on the sequence
+// of operations on it.
+struct AdjLists {
+    std::vector<int *> nb;
+    std::vector<double *> vv;
+    std::vector<int> len, cap;
+    struct Arena {
+        std::vector<std::unique_ptr<int[]>> ic;
+        std::vector<std::unique_ptr<double[]>> dc;
+        size_t used = 0, size = 0;
+        void take(int n, int **pi, double **pd) {
+            if (used + (size_t)n > size) {
+                size = std::max<size_t>(1 << 16, (size_t)n);
+                ic.emplace_back(new int[size]);
+                dc.emplace_back(new double[size]);
+                used = 0;
+            }
+            *pi = ic.back().get() + used;
+            *pd = dc.back().get() + used;
+            used += (size_t)n;
+        }
+    };
+    std::vector<Arena> arenas;
     int find(int u, int c) const {
-        const int *p = nbr.data() + off[u];
+        const int *p = nb[u];
         for (int i = 0; i < len[u]; i++)
             if (p[i] == c) return i;
         return -1;
     }
     void drop(int u, int k) {
-        const int i = find(u, k), o = off[u], last = o + len[u] - 1;
-        nbr[o + i] = nbr[last];
-        val[o + i] = val[last];
+        const int i = find(u, k), last = len[u] - 1;
+        nb[u][i] = nb[u][last];
+        vv[u][i] = vv[u][last];
         len[u]--;
     }
-    void add(int u, int v, double a) {
+    void add(int u, int v, double a, Arena &ar) {
         const int i = find(u, v);
         if (i >= 0) {
-            val[off[u] + i] += a;
+            vv[u][i] += a;
             return;
         }
         if (len[u] == cap[u]) {
-            const int o = (int)nbr.size(), c = std::max(4, 2 * cap[u]);
-            nbr.resize((size_t)o + c);
-            val.resize((size_t)o + c);
-            std::copy(nbr.begin() + off[u], nbr.begin() + off[u] + len[u], nbr.begin() + o);
-            std::copy(val.begin() + off[u], val.begin() + off[u] + len[u], val.begin() + o);
-            off[u] = o;
+            const int c = std::max(4, 2 * cap[u]);
+            int *pi;
+            double *pd;
+            ar.take(c, &pi, &pd);
+            std::copy(nb[u], nb[u] + len[u], pi);
+            std::copy(vv[u], vv[u] + len[u], pd);
+            nb[u] = pi;
+            vv[u] = pd;
             cap[u] = c;
         }
-        nbr[off[u] + len[u]] = v;
-        val[off[u] + len[u]] = a;
+        nb[u][len[u]] = v;
+        vv[u][len[u]] = a;
         len[u]++;
     }
 };
 
+// Stage 2 on the host.  Each round: the candidates (at most max_degree neighbours) are
+// taken greedily in index order into an independent set, their elimination records are
+// written from the state before the round, and the records' updates are applied grouped by
+// the node they change: a node's own list and diagonal are changed only by the records it
+// neighbours, in record order, so the nodes are shared out over host threads and the result
+// is the same as one record after the other (and the same for any thread count).  The
+// grouping is also each round's receiver list of the per-solve forward sweep.
 void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimDegree) {
     const int m = C.m;
     Z.m = m;
+    const int nth = host_threads(m, 1 << 12);
+    StageClock clk;
+    double t_sel = 0, t_rec = 0, t_grp = 0, t_app = 0;
+    auto now = []() { return std::chrono::steady_clock::now(); };
+    auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
     // S's lower triangle gives each node its lower neighbours (ascending) and, from later
     // rows, its upper ones (ascending): every list starts sorted by index
-    AdjPool A;
+    AdjLists A;
     A.len.assign((size_t)m, 0);
-    A.off.resize((size_t)m);
     A.cap.resize((size_t)m);
+    A.nb.resize((size_t)m);
+    A.vv.resize((size_t)m);
+    A.arenas.resize((size_t)nth + 1);
     for (int r = 0; r < m; r++)
         for (int64_t k = C.rrp[r]; k < C.rrp[r + 1]; k++) {
             A.len[r]++;
             A.len[(size_t)C.rcol[k]]++;
         }
     {
+        size_t total = 0;
+        for (int u = 0; u < m; u++) total += (size_t)A.len[u] + 2;
+        AdjLists::Arena &a0 = A.arenas[0];
+        a0.size = total;
+        a0.ic.emplace_back(new int[std::max<size_t>(total, 1)]);
+        a0.dc.emplace_back(new double[std::max<size_t>(total, 1)]);
         size_t o = 0;
         for (int u = 0; u < m; u++) {
-            A.off[u] = (int)o;
+            A.nb[u] = a0.ic[0].get() + o;
+            A.vv[u] = a0.dc[0].get() + o;
             A.cap[u] = A.len[u] + 2;  // a little room for the fill of the first rounds
             o += (size_t)A.cap[u];
             A.len[u] = 0;
         }
-        A.nbr.resize(o);
-        A.val.resize(o);
+        a0.used = o;
     }
     for (int r = 0; r < m; r++)
         for (int64_t k = C.rrp[r]; k < C.rrp[r + 1]; k++) {
             const int c = (int)C.rcol[k];
-            A.nbr[A.off[r] + A.len[r]] = c;
-            A.val[A.off[r] + A.len[r]++] = C.rval[k];
-            A.nbr[A.off[c] + A.len[c]] = r;
-            A.val[A.off[c] + A.len[c]++] = C.rval[k];
+            A.nb[r][A.len[r]] = c;
+            A.vv[r][A.len[r]++] = C.rval[k];
+            A.nb[c][A.len[c]] = r;
+            A.vv[c][A.len[c]++] = C.rval[k];
         }
     std::vector<double> d(C.sdiag.begin(), C.sdiag.end());
     std::vector<char> alive((size_t)m, 1), taken((size_t)m, 0);
     Z.round_off.assign(1, 0);
     Z.e_off.assign(1, 0);
-    std::vector<int> cand, sel;
+    Z.rv_round_off.assign(1, 0);
+    Z.rv_off.assign(1, 0);
+    std::vector<int> cand, sel, gcnt((size_t)m + 1, 0);
+    struct Op {
+        int r, t;  // record, position of the node in the record's neighbour list
+    };
+    std::vector<Op> ops;
+    std::vector<int> targets, toff;
     for (int i = 0; i < m; i++)
         if (A.len[i] <= max_degree) cand.push_back(i);
+    clk.mark("series: adjacency");
     while (!cand.empty()) {
+        auto t0 = now();
         std::sort(cand.begin(), cand.end());
         cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
         sel.clear();
@@ -543,48 +669,120 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimD
             if (!(d[k] > 1e-12 * std::fabs(C.sdiag[k])) || !std::isfinite(d[k])) continue;
             bool free = true;
             for (int i = 0; i < A.len[k]; i++)
-                if (taken[A.nbr[A.off[k] + i]]) free = false;
+                if (taken[A.nb[k][i]]) free = false;
             if (!free) continue;
             taken[k] = 1;
             sel.push_back(k);
         }
         if (sel.empty()) break;
+        auto t1 = now();
+        t_sel += ms(t0, t1);
         // records from the state before the round (selected nodes are not adjacent)
-        const size_t first = Z.e_node.size();
+        const int first = (int)Z.e_node.size();
+        const int nsel = (int)sel.size();
         for (int k : sel) {
             Z.e_node.push_back(k);
-            for (int i = 0; i < A.len[k]; i++) {
-                Z.e_nbr.push_back(A.nbr[A.off[k] + i]);
-                Z.e_a.push_back(A.val[A.off[k] + i]);
-            }
-            Z.e_off.push_back((int)Z.e_nbr.size());
+            Z.e_off.push_back(Z.e_off.back() + A.len[k]);
             Z.e_d.push_back(d[k]);
         }
+        Z.e_nbr.resize((size_t)Z.e_off.back());
+        Z.e_a.resize((size_t)Z.e_off.back());
+        for (int q = 0; q < nsel; q++) {
+            const int k = sel[q], o = Z.e_off[first + q];
+            std::copy(A.nb[k], A.nb[k] + A.len[k], Z.e_nbr.begin() + o);
+            std::copy(A.vv[k], A.vv[k] + A.len[k], Z.e_a.begin() + o);
+        }
+        auto t2 = now();
+        t_rec += ms(t1, t2);
+        // the updates grouped by the node they change (stable: record order)
+        const int e0 = Z.e_off[first], e1 = Z.e_off.back();
+        targets.clear();
+        for (int i = e0; i < e1; i++) gcnt[(size_t)Z.e_nbr[i] + 1]++;
+        // the changed nodes in ascending order: a scan of the counts (cheaper than sorting
+        // them while rounds are large)
+        if ((long long)(e1 - e0) * 8 > m) {
+            for (int u = 0; u < m; u++)
+                if (gcnt[(size_t)u + 1]) targets.push_back(u);
+        } else {
+            for (int i = e0; i < e1; i++) {
+                const int u = Z.e_nbr[i];
+                if (gcnt[(size_t)u + 1] > 0) {
+                    targets.push_back(u);
+                    gcnt[(size_t)u + 1] = -gcnt[(size_t)u + 1];  // listed once
+                }
+            }
+            for (int u : targets) gcnt[(size_t)u + 1] = -gcnt[(size_t)u + 1];
+            std::sort(targets.begin(), targets.end());
+        }
+        toff.assign(targets.size() + 1, 0);
+        for (size_t j = 0; j < targets.size(); j++) {
+            toff[j + 1] = toff[j] + gcnt[(size_t)targets[j] + 1];
+            gcnt[(size_t)targets[j] + 1] = toff[j];  // becomes the fill position
+        }
+        ops.resize((size_t)(e1 - e0));
+        for (int q = 0; q < nsel; q++) {
+            const int rec = first + q;
+            for (int i = Z.e_off[rec]; i < Z.e_off[rec + 1]; i++)
+                ops[(size_t)gcnt[(size_t)Z.e_nbr[i] + 1]++] = {rec, i - Z.e_off[rec]};
+        }
+        for (int u : targets) gcnt[(size_t)u + 1] = 0;
+        // receivers of this round (the forward sweep's deterministic gather)
+        {
+            const size_t n0 = Z.rv_node.size(), c0 = Z.rv_rec.size();
+            Z.rv_node.insert(Z.rv_node.end(), targets.begin(), targets.end());
+            Z.rv_rec.resize(c0 + ops.size());
+            Z.rv_a.resize(c0 + ops.size());
+            Z.rv_off.resize(n0 + targets.size() + 1);
+            for (size_t j = 0; j < targets.size(); j++)
+                Z.rv_off[n0 + j + 1] = (int)c0 + toff[j + 1];
+            for (size_t o = 0; o < ops.size(); o++) {
+                Z.rv_rec[c0 + o] = ops[o].r;
+                Z.rv_a[c0 + o] = Z.e_a[(size_t)Z.e_off[ops[o].r] + ops[o].t];
+            }
+        }
+        Z.rv_round_off.push_back((int)Z.rv_node.size());
+        auto t3 = now();
+        t_grp += ms(t2, t3);
+        // apply: each thread takes a range of the changed nodes
+        const int nt = std::min(nth, std::max(1, (int)(targets.size() / 1024)));
+        parallel_for(nt, [&](int t) {
+            AdjLists::Arena &ar = A.arenas[(size_t)t + 1];
+            const size_t j0 = targets.size() * t / nt, j1 = targets.size() * (t + 1) / nt;
+            for (size_t j = j0; j < j1; j++) {
+                const int u = targets[j];
+                for (int o = toff[j]; o < toff[j + 1]; o++) {
+                    const Op &op = ops[(size_t)o];
+                    const int rec = op.r, b = Z.e_off[rec], e = Z.e_off[rec + 1];
+                    const double dk = Z.e_d[rec];
+                    const double au = Z.e_a[(size_t)b + op.t];
+                    A.drop(u, Z.e_node[rec]);
+                    d[u] -= au * au / dk;
+                    // the pair (i, j), i < j, of the record's neighbours adds
+                    // -a_i a_j / d_k to both of their couplings
+                    for (int i = b; i < e; i++) {
+                        if (i == b + op.t) continue;
+                        const int lo = std::min(i, b + op.t), hi = std::max(i, b + op.t);
+                        const double f = -Z.e_a[lo] * Z.e_a[hi] / dk;
+                        A.add(u, Z.e_nbr[i], f, ar);
+                    }
+                }
+            }
+        });
+        t_app += ms(t3, now());
         cand.clear();
-        for (size_t q = 0; q < sel.size(); q++) {
-            const int k = sel[q];
-            const size_t rec = first + q;
-            const int b = Z.e_off[rec], e = Z.e_off[rec + 1];
-            const double dk = Z.e_d[rec];
+        for (int u : targets)
+            if (A.len[u] <= max_degree) cand.push_back(u);
+        for (int k : sel) {
             alive[k] = 0;
             taken[k] = 0;
-            for (int i = b; i < e; i++) {
-                const int u = Z.e_nbr[i];
-                A.drop(u, k);
-                d[u] -= Z.e_a[i] * Z.e_a[i] / dk;
-            }
-            for (int i = b; i < e; i++)
-                for (int j = i + 1; j < e; j++) {
-                    const double f = -Z.e_a[i] * Z.e_a[j] / dk;
-                    A.add(Z.e_nbr[i], Z.e_nbr[j], f);
-                    A.add(Z.e_nbr[j], Z.e_nbr[i], f);
-                }
-            for (int i = b; i < e; i++)
-                if (A.len[Z.e_nbr[i]] <= max_degree) cand.push_back(Z.e_nbr[i]);
             A.len[k] = 0;
         }
         Z.round_off.push_back((int)Z.e_node.size());
     }
+    if (clk.on)
+        fprintf(stderr, "[cicy setup]   rounds: select %.1f records %.1f group %.1f apply %.1f ms\n",
+                t_sel, t_rec, t_grp, t_app);
+    clk.mark("series: rounds");
     // core: the nodes left, in S order
     std::vector<int> core_of((size_t)m, -1);
     for (int i = 0; i < m; i++)
@@ -593,24 +791,44 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimD
             Z.core_nodes.push_back(i);
         }
     Z.mc = (int)Z.core_nodes.size();
+    clk.mark("series: core numbering");
     Z.crp.assign((size_t)Z.mc + 1, 0);
     Z.cdiag.resize((size_t)Z.mc);
-    std::vector<std::pair<int, double>> row;
-    for (int q = 0; q < Z.mc; q++) {
-        const int i = Z.core_nodes[q];
-        Z.cdiag[q] = d[i];
-        row.clear();
-        for (int t = 0; t < A.len[i]; t++) {
-            const int c = core_of[A.nbr[A.off[i] + t]];
-            if (c < q) row.push_back({c, A.val[A.off[i] + t]});
+    // each core row's lower entries, counted and then written and sorted on host threads
+    const int nct = host_threads(Z.mc, 1 << 14);
+    parallel_for(nct, [&](int t) {
+        const int q0 = (int)((long long)Z.mc * t / nct), q1 = (int)((long long)Z.mc * (t + 1) / nct);
+        for (int q = q0; q < q1; q++) {
+            const int i = Z.core_nodes[q];
+            Z.cdiag[q] = d[i];
+            int c = 0;
+            for (int t2 = 0; t2 < A.len[i]; t2++)
+                if (core_of[A.nb[i][t2]] < q) c++;
+            Z.crp[(size_t)q + 1] = c;
         }
-        std::sort(row.begin(), row.end());
-        for (auto &e : row) {
-            Z.ccol.push_back(e.first);
-            Z.cval.push_back(e.second);
+    });
+    for (int q = 0; q < Z.mc; q++) Z.crp[q + 1] += Z.crp[q];
+    clk.mark("series: core row counts");
+    Z.ccol.resize((size_t)Z.crp[Z.mc]);
+    Z.cval.resize((size_t)Z.crp[Z.mc]);
+    parallel_for(nct, [&](int t) {
+        std::vector<std::pair<int, double>> row;
+        const int q0 = (int)((long long)Z.mc * t / nct), q1 = (int)((long long)Z.mc * (t + 1) / nct);
+        for (int q = q0; q < q1; q++) {
+            const int i = Z.core_nodes[q];
+            row.clear();
+            for (int t2 = 0; t2 < A.len[i]; t2++) {
+                const int c = core_of[A.nb[i][t2]];
+                if (c < q) row.push_back({c, A.vv[i][t2]});
+            }
+            std::sort(row.begin(), row.end());
+            for (size_t k = 0; k < row.size(); k++) {
+                Z.ccol[(size_t)Z.crp[q] + k] = row[k].first;
+                Z.cval[(size_t)Z.crp[q] + k] = row[k].second;
+            }
         }
-        Z.crp[q + 1] = (int64_t)Z.ccol.size();
-    }
+    });
+    clk.mark("series: core rows");
 }
 
 // Stage-2 sweeps on the host (the device kernels do the same per round).
@@ -721,33 +939,11 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
     c->mc = Z.mc;
     c->nrec = (int)Z.e_node.size();
     c->round_off = Z.round_off;
-    // receivers of each round, each with its contributions in record order
-    std::vector<int> rv_node, rv_off(1, 0), rv_rec;
-    std::vector<double> rv_a;
-    c->rv_round_off.assign(1, 0);
-    {
-        struct Con { int u, k, i; };
-        std::vector<Con> con;
-        for (size_t q = 0; q + 1 < Z.round_off.size(); q++) {
-            con.clear();
-            for (int k = Z.round_off[q]; k < Z.round_off[q + 1]; k++)
-                for (int i = Z.e_off[k]; i < Z.e_off[k + 1]; i++) con.push_back({Z.e_nbr[i], k, i});
-            std::stable_sort(con.begin(), con.end(),
-                             [](const Con &x, const Con &y) { return x.u < y.u; });
-            for (size_t t = 0; t < con.size();) {
-                size_t e = t;
-                while (e < con.size() && con[e].u == con[t].u) {
-                    rv_rec.push_back(con[e].k);
-                    rv_a.push_back(Z.e_a[con[e].i]);
-                    e++;
-                }
-                rv_node.push_back(con[t].u);
-                rv_off.push_back((int)rv_rec.size());
-                t = e;
-            }
-            c->rv_round_off.push_back((int)rv_node.size());
-        }
-    }
+    // receivers of each round, each with its contributions in record order (built with the
+    // rounds)
+    const std::vector<int> &rv_node = Z.rv_node, &rv_off = Z.rv_off, &rv_rec = Z.rv_rec;
+    const std::vector<double> &rv_a = Z.rv_a;
+    c->rv_round_off = Z.rv_round_off.empty() ? std::vector<int>(1, 0) : Z.rv_round_off;
     clk.mark("condense: receiver lists");
     c->nnz_core = (long long)Z.cval.size();
     if (ok && m > 0) {

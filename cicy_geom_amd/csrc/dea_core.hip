@@ -231,16 +231,22 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
             H.col[fill[c]] = r;
             H.val[fill[c]++] = cval[k];
         }
-    std::vector<std::pair<int, double>> row;
-    for (int r = 0; r < m; r++) {
-        row.clear();
-        for (int k = H.rp[r]; k < H.rp[r + 1]; k++) row.emplace_back(H.col[k], H.val[k]);
-        std::sort(row.begin(), row.end());
-        for (size_t q = 0; q < row.size(); q++) {
-            H.col[(size_t)H.rp[r] + q] = row[q].first;
-            H.val[(size_t)H.rp[r] + q] = row[q].second;
+    // rows sorted by column on host threads (rows are independent)
+    const int nth = host_threads(m, 1 << 14);
+    auto sort_rows = [&](int t) {
+        std::vector<std::pair<int, double>> row;
+        const int r0 = (int)((long long)m * t / nth), r1 = (int)((long long)m * (t + 1) / nth);
+        for (int r = r0; r < r1; r++) {
+            row.clear();
+            for (int k = H.rp[r]; k < H.rp[r + 1]; k++) row.emplace_back(H.col[k], H.val[k]);
+            std::sort(row.begin(), row.end());
+            for (size_t q = 0; q < row.size(); q++) {
+                H.col[(size_t)H.rp[r] + q] = row[q].first;
+                H.val[(size_t)H.rp[r] + q] = row[q].second;
+            }
         }
-    }
+    };
+    parallel_for(nth, sort_rows);
     H.diag = cdiag;
     // row groups -> core row ranges (split at kCoreMaxBlock rows)
     std::vector<std::pair<int, int>> groups;
@@ -317,18 +323,23 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
     {
         std::vector<int> rp2((size_t)m + 1, 0), col2(H.col.size());
         std::vector<double> val2(H.val.size()), diag2((size_t)m);
-        for (int i = 0; i < m; i++) {
-            const int o = perm[i];
-            diag2[i] = H.diag[o];
-            row.clear();
-            for (int k = H.rp[o]; k < H.rp[o + 1]; k++) row.emplace_back(iperm[H.col[k]], H.val[k]);
-            std::sort(row.begin(), row.end());
-            rp2[i + 1] = rp2[i] + (int)row.size();
-            for (size_t q = 0; q < row.size(); q++) {
-                col2[(size_t)rp2[i] + q] = row[q].first;
-                val2[(size_t)rp2[i] + q] = row[q].second;
+        for (int i = 0; i < m; i++) rp2[i + 1] = rp2[i] + (H.rp[perm[i] + 1] - H.rp[perm[i]]);
+        parallel_for(nth, [&](int t) {
+            std::vector<std::pair<int, double>> row;
+            const int i0 = (int)((long long)m * t / nth), i1 = (int)((long long)m * (t + 1) / nth);
+            for (int i = i0; i < i1; i++) {
+                const int o = perm[i];
+                diag2[i] = H.diag[o];
+                row.clear();
+                for (int k = H.rp[o]; k < H.rp[o + 1]; k++)
+                    row.emplace_back(iperm[H.col[k]], H.val[k]);
+                std::sort(row.begin(), row.end());
+                for (size_t q = 0; q < row.size(); q++) {
+                    col2[(size_t)rp2[i] + q] = row[q].first;
+                    val2[(size_t)rp2[i] + q] = row[q].second;
+                }
             }
-        }
+        });
         H.rp.swap(rp2);
         H.col.swap(col2);
         H.val.swap(val2);
@@ -345,6 +356,7 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
     H.binv.resize((size_t)kept);
     int row0 = H.mj;
     long long at = 0;
+    std::vector<size_t> kept_g;  // the factorised groups, in order
     for (size_t g = 0; g < ng; g++) {
         if (!ok[g]) continue;
         const int nb = groups[g].second - groups[g].first;
@@ -353,14 +365,28 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
         H.blk_row0.push_back(row0);
         H.blk_n.push_back(nb);
         H.blk_off.push_back(at);
-        // constant part: the mean entry of the inverse
-        double c = 0.0;
-        for (long long e = off[g]; e < off[g + 1]; e++) c += inv[(size_t)e];
-        c /= (double)nb * (double)nb;
-        H.blk_c.push_back(c);
-        for (long long e = off[g]; e < off[g + 1]; e++)
-            H.binv[(size_t)at++] = (float)(inv[(size_t)e] - c);
+        kept_g.push_back(g);
+        at += (long long)nb * nb;
         row0 += nb;
+    }
+    // constant part (the mean entry of the inverse) and FP32 remainder of each block, on
+    // host threads over blocks
+    H.blk_c.assign(kept_g.size(), 0.0);
+    {
+        const int nbt = host_threads((long long)kept_g.size(), 8);
+        parallel_for(nbt, [&](int t) {
+            for (size_t b = t; b < kept_g.size(); b += nbt) {
+                const size_t g = kept_g[b];
+                const double nb = groups[g].second - groups[g].first;
+                double c = 0.0;
+                for (long long e = off[g]; e < off[g + 1]; e++) c += inv[(size_t)e];
+                c /= nb * nb;
+                H.blk_c[b] = c;
+                long long o = H.blk_off[b];
+                for (long long e = off[g]; e < off[g + 1]; e++)
+                    H.binv[(size_t)o++] = (float)(inv[(size_t)e] - c);
+            }
+        });
     }
     const int nbk = (int)H.blk_row0.size();
     H.G = std::max(1, std::min(max_groups, (m + kCoreRowsPerGroup - 1) / kCoreRowsPerGroup));
@@ -433,30 +459,42 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
             while (g < G && acc >= total * (double)g / G) H.wr[g++] = r + 1;
         }
     }
-    // sliced ELL of each work-group's phase-w rows
+    // sliced ELL of each work-group's phase-w rows: the slices and their offsets first, then
+    // the slices filled on host threads
     H.wsl.assign(1, 0);
     H.sl_off.clear();
     H.sl_len.clear();
-    H.ell_col.clear();
-    H.ell_val.clear();
+    std::vector<int> sl_base, sl_end;
+    size_t ell = 0;
     for (int gg = 0; gg < G; gg++) {
         for (int base = H.wr[gg]; base < H.wr[gg + 1]; base += kWave) {
             const int end = std::min(H.wr[gg + 1], base + kWave);
             int len = 0;
             for (int r = base; r < end; r++) len = std::max(len, H.rp[r + 1] - H.rp[r]);
-            const size_t o = H.ell_col.size();
-            H.sl_off.push_back((long long)o);
+            H.sl_off.push_back((long long)ell);
             H.sl_len.push_back(len);
-            H.ell_col.resize(o + (size_t)len * kWave, -1);
-            H.ell_val.resize(o + (size_t)len * kWave, 0.0);
-            for (int r = base; r < end; r++)
-                for (int t = 0; t < H.rp[r + 1] - H.rp[r]; t++) {
-                    H.ell_col[o + (size_t)t * kWave + (r - base)] = H.col[(size_t)H.rp[r] + t];
-                    H.ell_val[o + (size_t)t * kWave + (r - base)] = H.val[(size_t)H.rp[r] + t];
-                }
+            sl_base.push_back(base);
+            sl_end.push_back(end);
+            ell += (size_t)len * kWave;
         }
         H.wsl.push_back((int)H.sl_len.size());
     }
+    H.ell_col.assign(ell, -1);
+    H.ell_val.assign(ell, 0.0);
+    const int nsl = (int)sl_base.size();
+    parallel_for(host_threads(nsl, 256), [&](int t) {
+        const int nt = host_threads(nsl, 256);
+        for (int q = (int)((long long)nsl * t / nt); q < (int)((long long)nsl * (t + 1) / nt);
+             q++) {
+            const int base = sl_base[q], end = sl_end[q];
+            const size_t o = (size_t)H.sl_off[q];
+            for (int r = base; r < end; r++)
+                for (int k = 0; k < H.rp[r + 1] - H.rp[r]; k++) {
+                    H.ell_col[o + (size_t)k * kWave + (r - base)] = H.col[(size_t)H.rp[r] + k];
+                    H.ell_val[o + (size_t)k * kWave + (r - base)] = H.val[(size_t)H.rp[r] + k];
+                }
+        }
+    });
     return true;
 }
 

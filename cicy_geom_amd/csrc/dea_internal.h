@@ -2,15 +2,35 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <thread>
 #include <vector>
 
 #include "cicy_dea.h"
 #include "dea_kernels.h"
 
 namespace cicy {
+
+// Host threads for `work` items, at least `grain` items each, at most 16 (or at most
+// CICY_HOST_THREADS when set: the set-up's results do not depend on the count).
+inline int host_threads(long long work, long long grain) {
+    long long hw = std::max(1u, std::thread::hardware_concurrency());
+    if (const char *e = getenv("CICY_HOST_THREADS")) hw = std::max(1LL, std::min(hw, atoll(e)));
+    return (int)std::max<long long>(1, std::min<long long>({16, hw, work / std::max(1LL, grain)}));
+}
+
+// Runs fn(t) for t in [0, nth) on nth host threads (t = 0 on the calling thread).
+template <typename F>
+void parallel_for(int nth, F &&fn) {
+    std::vector<std::thread> th;
+    th.reserve((size_t)std::max(0, nth - 1));
+    for (int t = 1; t < nth; t++) th.emplace_back([&fn, t]() { fn(t); });
+    fn(0);
+    for (auto &x : th) x.join();
+}
 
 // Wall-clock marks of a system's set-up stages, printed to stderr when CICY_SETUP_TIMING is
 // set (a diagnostic of where a fresh system's set-up time goes).

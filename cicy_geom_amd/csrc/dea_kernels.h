@@ -62,8 +62,13 @@ struct MixState {
     unsigned int counter;
     int stall;
     int restarts;
-    int pad;
+    int run;            // FP32 iterations since the last reliable update
 };
+
+// FP32 iterations after which a reliable update is taken even if the recursion has not
+// reached its trigger (a recursion stuck at FP32's resolution then restarts from the true
+// residual, or is found stalled, instead of running on into overflow)
+constexpr int kMixedMaxRun = 4096;
 
 template <typename T>
 struct SssDev {
@@ -617,6 +622,7 @@ __global__ __launch_bounds__(kThreads) void k_mixed_init(int n, BatchDev bd,
             mx->counter = 0;
             mx->stall = 0;
             mx->restarts = 0;
+            mx->run = 0;
             if (r02 <= st->stop2) {
                 st->status = 0;
                 st->done = 1;
@@ -664,7 +670,9 @@ __global__ __launch_bounds__(kThreads) void k_update_mixed(int n, BatchDev bd,
                 mx->rr_prev = st->rz;
                 st->rz = rr;
                 st->rr = rr;
-                if (rr <= mx->delta2 * mx->m2 || rr <= mx->tgt2) mx->need_update = 1;
+                mx->run += 1;
+                if (rr <= mx->delta2 * mx->m2 || rr <= mx->tgt2 || mx->run >= kMixedMaxRun)
+                    mx->need_update = 1;
                 if (st->iters >= st->max_iter) {
                     // fold what has been accumulated and stop
                     mx->need_update = 1;
@@ -688,7 +696,9 @@ __global__ __launch_bounds__(kThreads) void k_fold(int n, BatchDev bd,
     for (int j = threadIdx.x; j < k.nrows; j += kThreads) {
         const int i = k.r0 + j;
         const float c = xs[i];
-        if (c != 0.f) x[i] = fma((double)c, 1.0 / sqrt(diag[i]), x[i]);
+        // a correction made non-finite by an FP32 breakdown is dropped: x stays the last
+        // finite iterate, from which the FP64 continuation starts
+        if (c != 0.f && isfinite(c)) x[i] = fma((double)c, 1.0 / sqrt(diag[i]), x[i]);
         xs[i] = 0.f;
     }
 }
@@ -728,6 +738,7 @@ __global__ __launch_bounds__(kThreads) void k_reliable_resid(int n, BatchDev bd,
         if (threadIdx.x == 0) {
             mx->refinements += 1;
             mx->need_update = 0;
+            mx->run = 0;
             if (rt2 > 0.25 * mx->rtrue2) mx->stall += 1;
             else mx->stall = 0;
             mx->rtrue2 = rt2;

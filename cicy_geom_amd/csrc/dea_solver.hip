@@ -337,16 +337,28 @@ double band_fraction(const BlockedHost &h) {
 
 void build_band(const BlockedHost &h, BandHost &b) {
     b.band.assign((size_t)h.n + 1, 0.0);
-    std::vector<int> cnt((size_t)h.n + 1, 0);
-    for (int r = 0; r < h.n; r++)
-        for (int j = h.rp[r]; j < h.rp[r + 1]; j++) {
-            const int c = h.col[j];
-            if (c == r - 1) {
-                b.band[r] = h.val[j];
-            } else {
-                cnt[r + 1]++;
-                cnt[c + 1]++;
+    // the entries off the sub-diagonal, gathered on host threads in row order; the band is
+    // written by the thread that owns the row
+    struct Off {
+        int r, c;
+        double v;
+    };
+    const int nth = host_threads(h.n, 1 << 17);
+    std::vector<std::vector<Off>> offs((size_t)nth);
+    parallel_for(nth, [&](int t) {
+        const int r0 = (int)((long long)h.n * t / nth), r1 = (int)((long long)h.n * (t + 1) / nth);
+        for (int r = r0; r < r1; r++)
+            for (int j = h.rp[r]; j < h.rp[r + 1]; j++) {
+                const int c = h.col[j];
+                if (c == r - 1) b.band[r] = h.val[j];
+                else offs[(size_t)t].push_back({r, c, h.val[j]});
             }
+    });
+    std::vector<int> cnt((size_t)h.n + 1, 0);
+    for (const auto &v : offs)
+        for (const Off &o : v) {
+            cnt[(size_t)o.r + 1]++;
+            cnt[(size_t)o.c + 1]++;
         }
     for (int i = 0; i < h.n; i++) cnt[i + 1] += cnt[i];
     b.m = h.n > 0 ? cnt[h.n] : 0;
@@ -354,19 +366,19 @@ void build_band(const BlockedHost &h, BandHost &b) {
     b.xrow.resize((size_t)b.m);
     b.xnbr.resize((size_t)b.m);
     b.xval.resize((size_t)b.m);
+    // each entry becomes a half-entry of its row and one of its column's row, placed in the
+    // sequential row order (the layout does not depend on the thread count)
     std::vector<int> fill(cnt.begin(), cnt.end() - (h.n > 0 ? 1 : 0));
-    for (int r = 0; r < h.n; r++)
-        for (int j = h.rp[r]; j < h.rp[r + 1]; j++) {
-            const int c = h.col[j];
-            if (c == r - 1) continue;
-            int a = fill[r]++;
-            b.xrow[a] = r;
-            b.xnbr[a] = c;
-            b.xval[a] = h.val[j];
-            a = fill[c]++;
-            b.xrow[a] = c;
-            b.xnbr[a] = r;
-            b.xval[a] = h.val[j];
+    for (const auto &v : offs)
+        for (const Off &o : v) {
+            int a = fill[o.r]++;
+            b.xrow[a] = o.r;
+            b.xnbr[a] = o.c;
+            b.xval[a] = o.v;
+            a = fill[o.c]++;
+            b.xrow[a] = o.c;
+            b.xnbr[a] = o.r;
+            b.xval[a] = o.v;
         }
     b.xblk.assign((size_t)h.nblocks + 1, 0);
     for (int k = 0; k <= h.nblocks; k++) b.xblk[k] = cnt[std::min(h.n, k * h.rb)];

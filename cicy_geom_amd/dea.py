@@ -5,7 +5,9 @@ is missing the constructor raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import sys
 import weakref
 from dataclasses import dataclass
 
@@ -181,6 +183,22 @@ class DEAError(RuntimeError):
     pass
 
 
+_LIVE_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    """Contexts still open at exit are closed (with their systems, batches and networks)
+    while the HIP runtime is certainly up, before the interpreter's teardown; finalizers
+    that run later make no native call (see _finalizing)."""
+    for ctx in list(_LIVE_CONTEXTS):
+        ctx.close()
+
+
+def _finalizing() -> bool:
+    return sys.is_finalizing()
+
+
 class Context:
     """One device context (stream, events); not re-entrant."""
 
@@ -192,6 +210,7 @@ class Context:
         self._h = h
         self._L = L
         self._children = weakref.WeakSet()   # systems, batches, graphs, networks
+        _LIVE_CONTEXTS.add(self)
         if rows_per_block:
             self.set_rows_per_block(rows_per_block)
         if fmt:
@@ -236,7 +255,8 @@ class Context:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not _finalizing():
+            self.close()
 
     def __enter__(self):
         return self
@@ -398,7 +418,8 @@ class System:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not _finalizing():
+            self.close()
 
     def size(self, what: str) -> int:
         return int(self.ctx._L.cicy_sys_size(
@@ -494,7 +515,8 @@ class Batch:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not _finalizing():
+            self.close()
 
     def size(self, what: str) -> int:
         return int(self.ctx._L.cicy_batch_size(self._h, {

@@ -76,7 +76,8 @@ class Graph:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not dea._finalizing():
+            self.close()
 
 
 class WarmStartNet:
@@ -100,7 +101,8 @@ class WarmStartNet:
             self._h = None
 
     def __del__(self):
-        self.close()
+        if not dea._finalizing():
+            self.close()
 
     @property
     def params(self) -> np.ndarray:

@@ -311,3 +311,33 @@ def test_resident_layout_rejects_unknown_rows_per_thread(c1):
     s = c1
     with pytest.raises(dea.DEAError):
         dea.host_resident_apply(s.row_ptr, s.col_ind, s.values, s.diag, s.b, s.b, 0.0, 3, 0)
+
+
+def test_threaded_condensation_is_independent_of_the_thread_count(monkeypatch):
+    """Above 2^17 rows the path factorisation runs on host threads (rows cut at anchors and
+    path starts, merged in row order).  The reduced system is bit-identical with one thread
+    and with many, and it condenses exactly: the expanded solution solves the full system."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    net = network.Network(network.config_params(4, 0, 8000))
+    k = max(net.percolating_clusters(0), key=lambda q: net.cluster(q)[0])
+    s = net.assemble(k, 1, 0, direction=0)
+    assert s.n > 2 * 65536
+    outs, cores = [], []
+    for threads in ("1", "16"):
+        monkeypatch.setenv("CICY_HOST_THREADS", threads)
+        outs.append(dea.host_condense(s.row_ptr, s.col_ind, s.values, s.diag, s.b))
+        cores.append(dea.host_core(s.row_ptr, s.col_ind, s.values, s.diag, s.b))
+    monkeypatch.delenv("CICY_HOST_THREADS")
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    for a, b in zip(*cores):          # the series rounds too
+        assert np.array_equal(a, b)
+    anchors, rrp, rcol, rval, rdiag, rrhs = outs[1]
+    m = len(anchors)
+    rows = np.repeat(np.arange(m), np.diff(rrp))
+    L = sp.csr_matrix((rval, (rows, rcol)), shape=(m, m))
+    xj = sla.spsolve((L + L.T + sp.diags(rdiag)).tocsc(), rrhs)
+    x = dea.host_expand(s.row_ptr, s.col_ind, s.values, s.diag, s.b, xj)
+    ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
+    assert np.linalg.norm(s.b - ax) <= 1e-11 * np.linalg.norm(s.b)

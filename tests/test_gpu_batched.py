@@ -32,15 +32,23 @@ def systems():
 
 @pytest.fixture(scope="module")
 def truths(systems):
-    """Per system: the FP64 truth (CG to 1e-13) and the error against it that the reference's
-    own algorithm (FP64 Jacobi-PCG, zero guess) has when it reaches ||b - Ax|| <= 1e-10 ||b||:
-    how far a 1e-10 residual is from the solution depends on each system's conditioning."""
+    """Per system: the FP64 truth (CG to 1e-13), the error against it that the reference's
+    own algorithm (FP64 Jacobi-PCG, zero guess) has when it reaches ||b - Ax|| <= 1e-10 ||b||,
+    and, on the rows of components tied to an electrode, the smallest eigenvalue of A
+    (shift-invert Lanczos), which bounds the error any solution with a given residual can
+    have there: ||x - x*|| <= ||b - Ax|| / lambda_min.  (A floating tube group has b = 0 and
+    stays at the zero guess.)"""
+    from linalg_helpers import grounded_rows, smallest_eigenvalue
     out = []
     for s in systems:
         truth = oracle.cg_truth(s.row_ptr, s.col_ind, s.values, s.diag, s.b, tol=1e-13)[0]
         ref = oracle.pcg_sss(s.nodes, s.reserved, 1e-10, s.row_ptr, s.col_ind, s.values,
                              s.diag, s.b, s.vef, np.zeros(s.n))
-        out.append((truth, np.linalg.norm(ref["x"] - truth) / np.linalg.norm(truth)))
+        tied = grounded_rows(s)
+        lam = smallest_eigenvalue(s, tied)
+        assert lam > 0.0
+        out.append((truth, np.linalg.norm(ref["x"] - truth) / np.linalg.norm(truth),
+                    (lam, tied)))
     return out
 
 
@@ -59,23 +67,29 @@ def test_batch_matches_truth_and_single_solves(ctx, systems, truths, method, fmt
         st = stats[i]
         ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
         tr = np.linalg.norm(s.b - ax) / np.linalg.norm(s.b)
-        truth, err_ref = truths[i]
+        truth, err_ref, (lam, tied) = truths[i]
         err = np.linalg.norm(x - truth) / np.linalg.norm(truth)
         print(i, s.n, st.as_dict()["status_name"], st.iters, st.refinements, st.fp64_iters,
               f"{tr:.2e} {err:.2e}")
         assert st.status == dea.CICY_OK
         assert tr <= 1e-10 * (1 + 1e-6) and abs(st.true_rel_res - tr) <= 1e-3 * tr + 1e-16
-        # error against the FP64 truth, against what the reference algorithm itself leaves
-        # at the same stopping rule: the device FP64 PCG sums in a fixed order and follows the
-        # reference's recursion, so it lands where the reference does (within a factor 3);
-        # the FP32 recursion of the mixed method ends elsewhere at the same residual and is
-        # held to the looser bound
-        if method == dea.METHOD_PCG_F64:
-            assert err <= 3.0 * err_ref, (err, err_ref)
-        else:
-            assert err < 1e-4
-    # each system's solve in the batch is its stand-alone solve: the sums of a system have
-    # the same fixed order in both (same row blocks, slots summed in block order)
+        # error against the FP64 truth: within what the residual it stops at allows,
+        # ||x - x*|| <= ||b - A x|| / lambda_min (every method), stated next to the error the
+        # reference's algorithm leaves at the same stopping rule.  Two FP64 CG runs with
+        # different rounding (the reference's scatter order and the device's per-row order)
+        # cross the 1e-10 residual at different points of these ill-conditioned systems (the
+        # 23 213-row z system: the reference after 8 854 iterations at 3.6e-9, the device
+        # after 8 528 at 2.8e-8), so the error itself is not held to the reference's.
+        bound = np.linalg.norm((s.b - ax)[tied]) / lam
+        print(f"   error {err:.1e} (reference {err_ref:.1e}); bound from the residual "
+              f"{bound / np.linalg.norm(truth):.1e}; {(~tied).sum()} floating rows")
+        assert np.linalg.norm((x - truth)[tied]) <= bound * (1 + 1e-6) + \
+            1e-12 * np.linalg.norm(truth)
+        assert np.all(x[~tied] == 0.0)
+    # each system's solve in the batch is its stand-alone solve: in the band layout the sums
+    # of a system have the same fixed order in both (same row blocks, slots summed in block
+    # order); the blocked SSS layout adds its in-block transposes with LDS atomics in
+    # arrival order, so there the counts only stay close
     if method == dea.METHOD_PCG_F64:
         for i in (0, len(systems) - 1):
             s = systems[i]
@@ -88,9 +102,11 @@ def test_batch_matches_truth_and_single_solves(ctx, systems, truths, method, fmt
             sysd.close()
             xb, _ = B.get(i)
             print("stand-alone", i, st1.iters, "batch", stats[i].iters)
-            assert st1.iters == stats[i].iters
             if fmt == dea.FORMAT_BAND:
+                assert st1.iters == stats[i].iters
                 assert np.array_equal(x1, xb)
+            else:
+                assert abs(st1.iters - stats[i].iters) <= max(3, st1.iters // 10)
     B.close()
 
 
@@ -113,4 +129,20 @@ def test_batch_reference_stopping_rule_with_warm_start(ctx, systems, truths):
         print(i, s.n, "iters", stats[i].iters, "oracle", ref["iters"], f"dist {dist:.1e}")
         assert dist < 1e-8, dist
         assert abs(stats[i].iters - ref["iters"]) <= max(5, ref["iters"] // 100)
+    B.close()
+
+
+def test_batch_converged_voltages_within_1e8_of_truth(ctx, systems, truths):
+    """Converged further (||b - A x|| <= 1e-13 ||b||), every system of the batch is within
+    1e-8 relative of the FP64 truth, whatever the conditioning."""
+    B = ctx.batch([(s.row_ptr, s.col_ind, s.values, s.diag) for s in systems])
+    for i, s in enumerate(systems):
+        B.set_rhs(i, s.b, None, s.b)
+    stats, _ = B.solve(dea.METHOD_PCG_F64, 1e-13, 0, dea.REL_B)
+    for i, s in enumerate(systems):
+        x, _ = B.get(i)
+        truth = truths[i][0]
+        err = np.linalg.norm(x - truth) / np.linalg.norm(truth)
+        print(i, s.n, stats[i].iters, f"{err:.1e}")
+        assert stats[i].status == dea.CICY_OK and err <= 1e-8, err
     B.close()

