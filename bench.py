@@ -601,17 +601,21 @@ def main():
     # plain 1 GiB device copy (read + write) as the achievable streaming rate
     n, L = s.n, s.nnz
     copy_gbs = 2 * (1 << 30) / (ctx.copy_bench(1 << 30, 5) * 1e-3) / 1e9
-    log(f"[bench] copy 1 GiB: {copy_gbs:.0f} GB/s")
+    read_gbs = (1 << 30) / (ctx.read_bench(1 << 30, 10) * 1e-3) / 1e9
+    log(f"[bench] copy 1 GiB: {copy_gbs:.0f} GB/s; read 1 GiB: {read_gbs:.0f} GB/s")
     spmv = {}
     fp64_sys = next((v for k, v in by_kind.items() if k != "mixed"), None)
     for prec, sv, sysx in ((64, 8, fp64_sys), (32, 4, by_kind.get("mixed"))):
         if sysx is None:
             continue
         ms = sysx.spmv_bench(prec, 50)
+        cold = sysx.spmv_bench_cold(prec, 20, 1 << 30)
         algo = network.spmv_bytes(n, L, sv=sv, si=4)
-        spmv[prec] = dict(ms=ms, bytes=algo, gbs=algo / (ms * 1e-3) / 1e9)
-        log(f"[bench] SpMV fp{prec}: {ms * 1e3:.1f} us, {algo / 1e6:.1f} MB algorithmic, "
-            f"{spmv[prec]['gbs']:.0f} GB/s")
+        spmv[prec] = dict(ms=ms, bytes=algo, gbs=algo / (ms * 1e-3) / 1e9, cold_ms=cold,
+                          cold_gbs=algo / (cold * 1e-3) / 1e9)
+        log(f"[bench] SpMV fp{prec}: {ms * 1e3:.1f} us back to back, {cold * 1e3:.1f} us with the "
+            f"cache flushed; {algo / 1e6:.1f} MB algorithmic: {spmv[prec]['gbs']:.0f} / "
+            f"{spmv[prec]['cold_gbs']:.0f} GB/s")
 
     plain = results.get("pcg_f64")
     head = "pcg_f64" if plain else next(iter(results))
@@ -706,8 +710,11 @@ def main():
         "roofline_spmv_full_f64": {"achieved": spmv[64]["gbs"],
                                    "frac": spmv[64]["gbs"] / HBM_PEAK_GBS,
                                    "us": spmv[64]["ms"] * 1e3,
+                                   "cold_achieved": spmv[64]["cold_gbs"],
+                                   "cold_frac": spmv[64]["cold_gbs"] / HBM_PEAK_GBS,
+                                   "cold_us": spmv[64]["cold_ms"] * 1e3,
                                    "algorithmic_bytes": spmv[64]["bytes"],
-                                   "copy_1GiB_gbs": copy_gbs},
+                                   "copy_1GiB_gbs": copy_gbs, "read_1GiB_gbs": read_gbs},
         "pcg_iteration_full": {"algorithmic_bytes": it_bytes, "us": it_us,
                                "gbs": it_bytes / (it_us * 1e-6) / 1e9 if it_us > 0 else None},
     }

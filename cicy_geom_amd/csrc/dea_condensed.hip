@@ -307,19 +307,33 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
     }
     StageClock clk;
     std::vector<int> &anchor_of = C.anchor_of, &anchors = C.anchors;
-    anchor_of.assign((size_t)n, -1);
-    anchors.clear();
-    {
-        std::vector<char> is_anchor((size_t)n, 0);
-        for (int k = 0; k < bh.m; k++) is_anchor[bh.xrow[k]] = 1;
-        for (int i = 0; i < n; i++)
-            if (is_anchor[i]) {
-                anchor_of[i] = (int)anchors.size();
-                anchors.push_back(i);
-            }
-    }
-    const int m = (int)anchors.size();
+    // anchors: the rows with an entry off the sub-diagonal (a half-entry of the remainder),
+    // numbered in row order; per-thread counts then a prefix, on host threads
+    anchor_of.resize((size_t)n);
+    const int nat = host_threads(n, 1 << 16);
+    std::vector<int> acount((size_t)nat + 1, 0);
+    parallel_for(nat, [&](int t) {
+        const int i0 = (int)((long long)n * t / nat), i1 = (int)((long long)n * (t + 1) / nat);
+        int c = 0;
+        for (int i = i0; i < i1; i++) c += bh.xrp[(size_t)i + 1] > bh.xrp[(size_t)i];
+        acount[(size_t)t + 1] = c;
+    });
+    for (int t = 0; t < nat; t++) acount[(size_t)t + 1] += acount[(size_t)t];
+    const int m = acount[(size_t)nat];
     C.m = m;
+    anchors.resize((size_t)m);
+    parallel_for(nat, [&](int t) {
+        const int i0 = (int)((long long)n * t / nat), i1 = (int)((long long)n * (t + 1) / nat);
+        int a = acount[(size_t)t];
+        for (int i = i0; i < i1; i++) {
+            if (bh.xrp[(size_t)i + 1] > bh.xrp[(size_t)i]) {
+                anchor_of[i] = a;
+                anchors[(size_t)a++] = i;
+            } else {
+                anchor_of[i] = -1;
+            }
+        }
+    });
     if ((long long)m * 2 > n) {
         *why = "more than half of the rows carry junction entries: nothing worth condensing";
         return false;
@@ -327,11 +341,19 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
     // paths, their LU factors and their Schur complement terms
     std::vector<int> &ps = C.ps, &pe = C.pe, &pl = C.pl, &pr = C.pr;
     std::vector<double> &lfac = C.lfac, &mfac = C.mfac, &sdiag = C.sdiag;
-    lfac.assign((size_t)n, 0.0);
-    mfac.assign((size_t)n, 1.0);
+    lfac.resize((size_t)n);
+    mfac.resize((size_t)n);
+    C.band.resize(bh.band.size());
+    parallel_for(nat, [&](int t) {
+        const size_t i0 = (size_t)n * t / nat, i1 = (size_t)n * (t + 1) / nat;
+        std::fill(lfac.begin() + i0, lfac.begin() + i1, 0.0);
+        std::fill(mfac.begin() + i0, mfac.begin() + i1, 1.0);
+        std::copy(bh.band.begin() + i0, bh.band.begin() + i1, C.band.begin() + i0);
+    });
+    C.band[(size_t)n] = bh.band[(size_t)n];
     sdiag.resize((size_t)m);
-    C.band = bh.band;
     for (int j = 0; j < m; j++) sdiag[j] = h.diag[anchors[j]];
+    clk.mark("paths: anchors");
     struct Fill {
         int r, c;
         double v;
@@ -475,24 +497,41 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
         for (const Part &P : parts)
             for (const Fill &f : P.jfill) byrow[(size_t)at[f.r]++] = {f.c, f.v};
     }
+    // each row sorted (stable insertion sort: the rows hold a few entries) and its equal
+    // columns summed in order, counted then written, on host threads over rows
     rrp.assign((size_t)m + 1, 0);
-    rcol.clear();
-    rval.clear();
-    for (int j = 0; j < m; j++) {
+    const int nrt = host_threads(m, 1 << 14);
+    auto sort_row = [&](int j) {
         auto b = byrow.begin() + rcnt[j], e = byrow.begin() + rcnt[j + 1];
-        std::stable_sort(b, e, [](const std::pair<int, double> &x,
-                                  const std::pair<int, double> &y) { return x.first < y.first; });
-        for (auto q = b; q != e;) {
-            auto w = q;
-            double v = 0.0;
-            while (w != e && w->first == q->first) v += (w++)->second;
-            rcol.push_back(q->first);
-            rval.push_back(v);
-            rrp[(size_t)j + 1]++;
-            q = w;
-        }
-    }
+        for (auto q = b + (b != e); q < e; q++)
+            for (auto w = q; w > b && (w - 1)->first > w->first; w--) std::iter_swap(w - 1, w);
+        int u = 0;
+        for (auto q = b; q != e; q++)
+            if (q == b || q->first != (q - 1)->first) u++;
+        rrp[(size_t)j + 1] = u;
+    };
+    parallel_for(nrt, [&](int t) {
+        const int j0 = (int)((long long)m * t / nrt), j1 = (int)((long long)m * (t + 1) / nrt);
+        for (int j = j0; j < j1; j++) sort_row(j);
+    });
     for (int j = 0; j < m; j++) rrp[j + 1] += rrp[j];
+    rcol.resize((size_t)rrp[m]);
+    rval.resize((size_t)rrp[m]);
+    parallel_for(nrt, [&](int t) {
+        const int j0 = (int)((long long)m * t / nrt), j1 = (int)((long long)m * (t + 1) / nrt);
+        for (int j = j0; j < j1; j++) {
+            auto b = byrow.begin() + rcnt[j], e = byrow.begin() + rcnt[j + 1];
+            int64_t o = rrp[j];
+            for (auto q = b; q != e;) {
+                auto w = q;
+                double v = 0.0;
+                while (w != e && w->first == q->first) v += (w++)->second;
+                rcol[(size_t)o] = q->first;
+                rval[(size_t)o++] = v;
+                q = w;
+            }
+        }
+    });
     clk.mark("paths: merge + reduced rows");
     return true;
 }

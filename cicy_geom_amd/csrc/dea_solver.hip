@@ -86,6 +86,30 @@ __global__ __launch_bounds__(kThreads) void k_copy(size_t n16, const uint4 *__re
     }
 }
 
+// Read-mostly stream: each thread sums its 16-byte vectors (four loads in flight), one word
+// per work-group is written.  The yardstick of a kernel that mostly reads (an SpMV).
+__global__ __launch_bounds__(kThreads) void k_read(size_t n16, const uint4 *__restrict__ src,
+                                                   unsigned int *__restrict__ out) {
+    const size_t stride = (size_t)gridDim.x * kThreads;
+    unsigned int acc = 0;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n16; i += 4 * stride) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            v[u] = i + u * stride < n16 ? src[i + u * stride] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    __shared__ unsigned int sacc[kThreads];
+    sacc[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int a = 0;
+        for (int t = 0; t < kThreads; t++) a ^= sacc[t];
+        out[blockIdx.x] = a;
+    }
+}
+
 // FP64 boundary current I = sum_e (v[ni[e]] - v[nb[e]]) / re[e]: grid-stride over the
 // elements, per-work-group sums reduced in a fixed order (reduce_sys: bit-identical from run
 // to run), the total written to out[0].  An element whose node lies outside [0, nv) adds
@@ -1099,6 +1123,76 @@ int cicy_sys_spmv_bench(cicy_sys *s, int precision, int reps, double *avg_ms) {
     (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
     *avg_ms = ms / reps;
     return hip_ok(ctx, hipGetLastError(), "kernel launch") ? 1 : 0;
+}
+
+int cicy_sys_spmv_bench_cold(cicy_sys *s, int precision, int reps, int64_t flush_bytes,
+                             double *avg_ms) {
+    if (!s || reps <= 0 || !avg_ms || flush_bytes < 0) return 0;
+    cicy_ctx *ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    if (s->n == 0) {
+        *avg_ms = 0.0;
+        return 1;
+    }
+    if (precision != 64 && precision != 32) {
+        set_error(ctx, "precision must be 64 or 32");
+        return 0;
+    }
+    if (precision == 32 && !cicy::sys_prepare(s, CICY_METHOD_MIXED)) return 0;
+    // a copy of flush_bytes through the memory-side cache before every timed launch, so the
+    // matrix and vectors come from HBM; each launch timed alone between two events
+    const size_t n16 = (size_t)flush_bytes / 32;
+    uint4 *a = nullptr, *b = nullptr;
+    if (n16 && (!hip_ok(ctx, hipMalloc((void **)&a, n16 * 16), "hipMalloc") ||
+                !hip_ok(ctx, hipMalloc((void **)&b, n16 * 16), "hipMalloc"))) {
+        if (a) (void)hipFree(a);
+        return 0;
+    }
+    bool ok = !n16 || hip_ok(ctx, hipMemsetAsync(a, 0, n16 * 16, ctx->stream), "memset");
+    const int grid = (int)std::min<size_t>((n16 + kThreads - 1) / kThreads, 256 * 64);
+    double total = 0.0;
+    for (int i = 0; ok && i <= reps; i++) {
+        if (n16) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kThreads), 0, ctx->stream, n16, a, b);
+        ok = hip_ok(ctx, hipEventRecord(ctx->ev_start, ctx->stream), "event");
+        if (precision == 64) dispatch_rb<LaunchSpmv64>(s->rb, s, s->p0, s->q);
+        else cicy::mixed_spmv32(s);
+        ok = ok && hip_ok(ctx, hipEventRecord(ctx->ev_stop, ctx->stream), "event") &&
+             hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "sync");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+        if (i > 0) total += ms;  // the first launch warms the code and the TLB
+    }
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    *avg_ms = total / reps;
+    return ok && hip_ok(ctx, hipGetLastError(), "kernel launch") ? 1 : 0;
+}
+
+int cicy_read_bench(cicy_ctx *ctx, int64_t bytes, int reps, double *avg_ms) {
+    if (!ctx || bytes < 16 || reps <= 0 || !avg_ms) return 0;
+    (void)hipSetDevice(ctx->device);
+    const size_t n16 = (size_t)bytes / 16;
+    uint4 *a = nullptr;
+    unsigned int *o = nullptr;
+    const int grid = (int)std::min<size_t>((n16 + kThreads - 1) / kThreads, 256 * 64);
+    if (!hip_ok(ctx, hipMalloc((void **)&a, n16 * 16), "hipMalloc") ||
+        !hip_ok(ctx, hipMalloc((void **)&o, (size_t)grid * sizeof(unsigned)), "hipMalloc")) {
+        if (a) (void)hipFree(a);
+        return 0;
+    }
+    bool ok = hip_ok(ctx, hipMemsetAsync(a, 1, n16 * 16, ctx->stream), "memset");
+    hipLaunchKernelGGL(k_read, dim3(grid), dim3(kThreads), 0, ctx->stream, n16, a, o);
+    ok = ok && hip_ok(ctx, hipEventRecord(ctx->ev_start, ctx->stream), "event");
+    for (int i = 0; i < reps; i++)
+        hipLaunchKernelGGL(k_read, dim3(grid), dim3(kThreads), 0, ctx->stream, n16, a, o);
+    ok = ok && hip_ok(ctx, hipEventRecord(ctx->ev_stop, ctx->stream), "event") &&
+         hip_ok(ctx, hipEventSynchronize(ctx->ev_stop), "sync");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop);
+    *avg_ms = ms / reps;
+    (void)hipFree(a);
+    (void)hipFree(o);
+    return ok ? 1 : 0;
 }
 
 int cicy_copy_bench(cicy_ctx *ctx, int64_t bytes, int reps, double *avg_ms) {

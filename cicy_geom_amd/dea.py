@@ -45,7 +45,7 @@ EXPORTED_SYMBOLS = [
     "cicy_sys_core", "cicy_set_dropin_method", "cicy_sys_set_blocks", "cicy_host_core_block_cg",
     "cicy_sys_prepare",
     "cicy_set_dropin_blocks", "cicy_backbone_scan", "cicy_cnt_contacts",
-    "cicy_boundary_current_f64_noalloc",
+    "cicy_boundary_current_f64_noalloc", "cicy_sys_spmv_bench_cold", "cicy_read_bench",
 ]
 
 
@@ -94,6 +94,9 @@ def lib():
     L.cicy_host_core_expand.argtypes = [ctypes.c_int64, _I64, _I64, _F64, _F64, ctypes.c_int64,
                                         _F64, _F64, _F64]
     L.cicy_copy_bench.argtypes = [_VP, ctypes.c_int64, ctypes.c_int, P(ctypes.c_double)]
+    L.cicy_read_bench.argtypes = [_VP, ctypes.c_int64, ctypes.c_int, P(ctypes.c_double)]
+    L.cicy_sys_spmv_bench_cold.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                           P(ctypes.c_double)]
     L.cicy_host_layout_spmv.argtypes = [ctypes.c_int, ctypes.c_int64, _I64, _I64, _F64, _F64,
                                         ctypes.c_int64, ctypes.c_int, _F64, _F64]
     L.cicy_solve_dea_cg_sss.argtypes = [_VP, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
@@ -225,6 +228,13 @@ class Context:
         ms = ctypes.c_double()
         self._check(self._L.cicy_copy_bench(self._h, nbytes, reps, ctypes.byref(ms)),
                     "cicy_copy_bench")
+        return ms.value
+
+    def read_bench(self, nbytes: int, reps: int = 20) -> float:
+        """Average ms of a read-mostly stream over nbytes."""
+        ms = ctypes.c_double()
+        self._check(self._L.cicy_read_bench(self._h, nbytes, reps, ctypes.byref(ms)),
+                    "cicy_read_bench")
         return ms.value
 
     def set_format(self, fmt: int):
@@ -466,6 +476,14 @@ class System:
         self.ctx._check(self.ctx._L.cicy_sys_spmv_bench(self._h, precision, reps,
                                                         ctypes.byref(ms)),
                         "cicy_sys_spmv_bench")
+        return ms.value
+
+    def spmv_bench_cold(self, precision=64, reps=20, flush_bytes=1 << 30) -> float:
+        """Average ms of one SpMV launch with the memory-side cache flushed before it."""
+        ms = ctypes.c_double()
+        self.ctx._check(self.ctx._L.cicy_sys_spmv_bench_cold(self._h, precision, reps,
+                                                             flush_bytes, ctypes.byref(ms)),
+                        "cicy_sys_spmv_bench_cold")
         return ms.value
 
     def core(self):

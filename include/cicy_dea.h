@@ -252,9 +252,17 @@ int cicy_sys_set_blocks(cicy_sys *sys, int64_t nblocks, const int64_t *ranges);
 int cicy_sys_core(cicy_sys *sys, cicy_sys **core);
 /* Time `reps` back-to-back SSS SpMV launches on resident data; average ms per launch. */
 int cicy_sys_spmv_bench(cicy_sys *sys, int precision /*64 or 32*/, int reps, double *avg_ms);
+/* The same with the memory-side (Infinity) cache flushed before every launch: a device copy
+ * of flush_bytes (use more than twice its 256 MB) runs before each timed launch, and each
+ * launch is timed alone, so the matrix and vectors come from HBM. */
+int cicy_sys_spmv_bench_cold(cicy_sys *sys, int precision, int reps, int64_t flush_bytes,
+                             double *avg_ms);
 /* Time a plain device copy of `bytes` (read `bytes`, write `bytes`); the reference kernel for
  * traffic counters and the achievable-bandwidth yardstick. */
 int cicy_copy_bench(cicy_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
+/* Time a read-mostly stream over `bytes` (read `bytes`, write a word per work-group): the
+ * achievable-bandwidth yardstick of a kernel that mostly reads, as an SpMV does. */
+int cicy_read_bench(cicy_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
 
 /* ---- batches of independent systems (clusters x directions x realisations) -----------
  * The reference solves each (cluster, direction) system in turn with a fresh object
