@@ -3,23 +3,28 @@
 
 Workload: the 8 um RVE with 160k straight CNTs and 2k square nanoplatelets (seed
 3_000_000), largest cluster percolating along x, R_flag=1 resistance solve (family := x),
-zero initial guess.  The system is assembled on the host and uploaded once; each method's
-one-off setup of the system (layouts, eliminations, block inverses) is timed separately
-(`setup_s`), the timed region covers solves on HBM-resident data.  A "step" is one solve.
+zero initial guess.  The system's SSS arrays (the reference's row_ptr, col_ind, values,
+diagonal, with int64 indices) and b are uploaded to HBM once, before anything is timed.  A
+"step" is one new system, as the reference solves each (cluster, direction) system once:
+create it from the arrays in HBM (checks and band layout on the device), set its right-hand
+side, build the method's set-up on the device (condensation, series rounds, core layout,
+plate-block inverses), solve it (x stays in HBM) and destroy it.  The rate from host arrays
+(upload and host layout inside the step) and the rate of re-solving one prepared system are
+reported beside it.
 
 Methods (--methods): plain FP64 Jacobi-PCG on the GPU (the reference's algorithm, always
 timed as the reference line), the condensed solves (exact series elimination down to a
-core system, back-substitution; timed end to end) with the core solved by the two-kernel
-FP64 Jacobi-PCG (condensed), the register-resident CG (condensed_resident) or the
-persistent block-preconditioned CG with one dense block per nanoplatelet
-(condensed_block), the register-resident CG on the full system, the mixed-precision
-FP32/FP64 solver, and the learned warm start followed by a solver (network inference
-inside the timed solve; needs trained weights).  Each method runs on the device layout and
-row-block size the autotuner picks for it on this system.  Methods that iterate on the
-full system (pcg_f64, learned_f64, mixed, resident) time --plain-steps solves (about 13 s
-each here), the others --steps.  `value` is the fastest converged method's solves/s; every
-method's true residual ||b - A x|| / ||b|| is checked with the full system's FP64 SpMV and
-reported.  Prints one JSON line on stdout; progress goes to stderr.
+core system, back-substitution) with the core solved by the two-kernel FP64 Jacobi-PCG
+(condensed), the register-resident CG (condensed_resident) or the persistent
+block-preconditioned CG with one dense block per nanoplatelet (condensed_block), the
+register-resident CG on the full system, the mixed-precision FP32/FP64 solver, and the
+learned warm start followed by a solver (network inference inside the step; needs trained
+weights).  Methods that iterate on the full system (pcg_f64, learned_f64, mixed, resident)
+time --plain-steps systems (about 13 s each here), the others --steps.  The mixed and
+resident methods build their layouts from host arrays, so their steps start from those.
+`value` is the fastest converged method's new systems per second; every method's true
+residual ||b - A x|| / ||b|| is checked with the full system's FP64 SpMV and reported.
+Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -79,18 +84,29 @@ def time_solves(sysd, method, steps, warmup, tol, s=None, model=None, graph=None
     return wall, stats
 
 
-def time_fresh(ctx, s, method, steps, warmup, tol, plates=None, model=None, box=None):
-    """`steps` solves of the system as a new one each time (after `warmup` untimed ones):
-    from the host arrays, upload and layout (cicy_sys_create), right-hand side, plate row
-    groups, the method's set-up (cicy_sys_prepare: eliminations, core layout, block
-    inverses), the learned guess when a model is given, the solve, and the read-back of x;
-    the system is destroyed after each step.  Returns per-step timings (s) and the stats."""
+def time_fresh(ctx, s, method, steps, warmup, tol, plates=None, model=None, box=None,
+               dev=None):
+    """`steps` solves of the system as a new one each time (after `warmup` untimed ones).
+
+    With `dev` (the reference's SSS arrays and b already in device memory, uploaded once
+    before the timed steps): the system is created from them on the device
+    (cicy_sys_create_device: checks and layout), its right-hand side set from device memory,
+    the plate row groups given, the method's set-up built on the device (cicy_sys_prepare:
+    eliminations, core layout, block inverses), the learned guess written when a model is
+    given, the system solved (x stays in HBM) and destroyed.  Without `dev` the same from the
+    host arrays (cicy_sys_create: upload and host layout, as a caller with host vectors
+    sees it).  Every step is timed whole; returns per-step timings (s) and the stats."""
     from cicy_geom_amd import dea, warmstart
     out = []
     for i in range(warmup + steps):
         t0 = time.perf_counter()
-        sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
-        sysd.set_rhs(s.b, None, s.b)
+        if dev is not None:
+            sysd = ctx.system_device(s.n, len(s.values), dev["row_ptr"], dev["col_ind"],
+                                     dev["values"], dev["diag"])
+            sysd.set_rhs_device(dev["b"], None, dev["b"])
+        else:
+            sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+            sysd.set_rhs(s.b, None, s.b)
         if method == dea.METHOD_CONDENSED_BLOCK and plates is not None:
             sysd.set_blocks(plates)
         t1 = time.perf_counter()
@@ -102,8 +118,6 @@ def time_fresh(ctx, s, method, steps, warmup, tol, plates=None, model=None, box=
             model.warm_start(graph, sysd)
         st = sysd.solve(method, tol, MAX_ITER, dea.REL_B)
         t3 = time.perf_counter()
-        x, _ = sysd.get()
-        t4 = time.perf_counter()
         info = {k: sysd.size(k) for k in ("condensed_rows", "core_rows", "core_nnz",
                                           "series_rounds", "core_blocks", "core_block_rows",
                                           "core_groups", "core_block_entries", "format",
@@ -111,10 +125,10 @@ def time_fresh(ctx, s, method, steps, warmup, tol, plates=None, model=None, box=
         if graph is not None:
             graph.close()
         sysd.close()
+        t4 = time.perf_counter()
         if i >= warmup:
             out.append(dict(create_s=t1 - t0, prepare_s=t2 - t1, solve_s=t3 - t2,
-                            get_s=t4 - t3, total_s=time.perf_counter() - t0, stats=st,
-                            info=info))
+                            destroy_s=t4 - t3, total_s=t4 - t0, stats=st, info=info))
     return out
 
 
@@ -539,18 +553,26 @@ def main():
             log(f"[bench] autotune {name}: {autotune.LAYOUTS.get(tuned[name]['format'])} "
                 f"rb={tuned[name]['rows_per_block']} ({time.perf_counter() - t0:.1f}s)")
 
+    # the reference's arrays of the system, resident in HBM before any timed step
+    dev = {k: ctx.upload(np.ascontiguousarray(a, dt)) for k, a, dt in
+           (("row_ptr", s.row_ptr, np.int64), ("col_ind", s.col_ind, np.int64),
+            ("values", s.values, np.float64), ("diag", s.diag, np.float64),
+            ("b", s.b, np.float64))}
+    host_only = {"mixed", "learned_mixed", "resident"}   # layouts built from host arrays
     results, systems, by_kind = {}, {}, {}
     for name, method, tkey, learned in plan:
         autotune.apply(ctx, tuned[tkey])
         steps, warmup = ((args.plain_steps, 0) if name in full_system
                          else (args.steps, args.warmup))
-        # the metric's step: a new system each time (set-up inside the timed step)
+        # the metric's step: a new system each time, set-up inside the timed step, from the
+        # arrays in HBM
         fresh = time_fresh(ctx, s, method, steps, 1 if name not in full_system else 0,
-                           args.tol, plates, model if learned else None, box)
+                           args.tol, plates, model if learned else None, box,
+                           dev=None if name in host_only else dev)
         st = fresh[-1]["stats"]
         wall = sum(f["total_s"] for f in fresh)
         mean = {k: sum(f[k] for f in fresh) / len(fresh)
-                for k in ("create_s", "prepare_s", "solve_s", "get_s", "total_s")}
+                for k in ("create_s", "prepare_s", "solve_s", "destroy_s", "total_s")}
         info = fresh[-1]["info"]
         results[name] = dict(solves_per_s=steps / wall, ms_per_solve=1e3 * wall / steps,
                              steps=steps,
@@ -566,11 +588,20 @@ def main():
                              core_block_entries=info["core_block_entries"],
                              core_groups=info["core_groups"],
                              step_breakdown_s=mean,
+                             inputs="host arrays" if name in host_only else "device arrays",
                              setup_s=mean["create_s"] + mean["prepare_s"],
                              kernel_ms=sum(f["stats"].kernel_ms for f in fresh),
                              resident_rows_per_thread=info["resident_rows"],
                              rows_per_block=tuned[tkey]["rows_per_block"])
         if name not in full_system:
+            # a new system from the host arrays each step (upload and host-side layout
+            # included: the rate of a caller that hands over host vectors)
+            hfresh = time_fresh(ctx, s, method, min(steps, 5), 1, args.tol, plates,
+                                model if learned else None, box)
+            results[name]["host_arrays_solves_per_s"] = len(hfresh) / sum(
+                f["total_s"] for f in hfresh)
+            results[name]["host_arrays_create_s"] = sum(
+                f["create_s"] for f in hfresh) / len(hfresh)
             # the same method re-solving an already prepared, resident system (the rate a
             # caller gets when one system is solved many times)
             sysd = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
@@ -700,9 +731,10 @@ def main():
                    "method": head},
         "methods": results,
         "speedup_vs_plain_f64": value / plain["solves_per_s"] if plain else None,
-        # each step is a new system: upload and layouts, set-up (eliminations, core layout,
-        # block inverses), solve and read-back of x, from the host arrays
-        "step": "new system from host arrays: upload + set-up + solve + read-back",
+        # each step is a new system from the reference's arrays in HBM: checks and layout,
+        # set-up (eliminations, core layout, block inverses), solve (x in HBM), destroy
+        "step": "new system from its SSS arrays resident in HBM: device layout + set-up + "
+                "solve + destroy",
         "step_breakdown_s": results[head]["step_breakdown_s"],
         "setup_s": results[head]["setup_s"],
         "resident_solves_per_s": results[head].get("resident_solves_per_s"),
@@ -723,6 +755,8 @@ def main():
         log(f"[bench] cpu baseline: {out['cpu_baseline']}")
     for sysd in systems.values():
         sysd.close()
+    for a in dev.values():
+        a.close()
     if model is not None:
         learned = {"weights": os.path.relpath(WEIGHTS, ROOT)}
         for lk, zk in (("learned_f64", "pcg_f64"), ("learned_mixed", "mixed"),

@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["dea_solver.hip", "dea_mixed.hip", "dea_batched.hip", "dea_mpnn.hip",
+HIP_SOURCES = ["dea_solver.hip", "dea_setup.hip", "dea_mixed.hip", "dea_batched.hip", "dea_mpnn.hip",
                "dea_resident.hip", "dea_condensed.hip", "dea_core.hip", "dea_backbone.hip",
                "dea_contacts.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",

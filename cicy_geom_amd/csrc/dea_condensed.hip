@@ -37,42 +37,6 @@ using namespace cicy;
 
 #include <vector>
 
-struct cicy_cond {
-    int n = 0, m = 0, npath = 0;
-    long long nnz_red = 0;
-    int *ps = nullptr, *pe = nullptr, *pl = nullptr, *pr = nullptr;  // [npath]
-    int *anchor_of = nullptr;   // [n] reduced index or -1
-    int *anchors = nullptr;     // [m] full index
-    double *lfac = nullptr, *mfac = nullptr;  // [n] Thomas multipliers and pivots (path rows)
-    double *band = nullptr;     // [n + 1] a(i, i-1) (0 where absent)
-    double *y = nullptr, *delta = nullptr, *rhs = nullptr;  // [n]
-    double *norm = nullptr;     // [2]
-    unsigned *counter = nullptr;
-    double *part = nullptr, *gpart = nullptr;
-    unsigned *gcount = nullptr;
-    int nb = 0;                 // work-groups of the norm kernel
-    // stage 2 (series rounds over S) and the core system the CG runs on
-    int mc = 0, nrec = 0;
-    long long nnz_core = 0;
-    std::vector<int> round_off;                       // host copy: launch bounds per round
-    std::vector<int> rv_round_off;                    // host copy: receivers of each round
-    int *e_node = nullptr, *e_off = nullptr, *e_nbr = nullptr, *core_nodes = nullptr;
-    // forward sweep by receiver: rv_node[j] gathers rv_a[c] * f_k / d_k over its
-    // contributions c in rv_off[j] .. rv_off[j+1] (record rv_rec[c]), in record order
-    int *rv_node = nullptr, *rv_off = nullptr, *rv_rec = nullptr;
-    double *rv_a = nullptr;
-    double *e_a = nullptr, *e_d = nullptr;
-    double *fS = nullptr, *xS = nullptr;              // [m] right-hand side / solution on S
-    double *core_b = nullptr, *core_x = nullptr;      // [mc] the block solver's core vectors
-    cicy_sys *red = nullptr;    // the core system on the device (built on first use)
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timing (the inner solve uses the context's)
-    // host copy of the core (strictly-lower SSS, diagonal, full-system row of each core
-    // row) for the block-preconditioned core CG, built on demand
-    std::vector<int64_t> h_crp, h_ccol;
-    std::vector<double> h_cval, h_cdiag;
-    std::vector<int> h_orig;
-    cicy_core *blk = nullptr;
-};
 
 namespace cicy {
 namespace {
@@ -262,6 +226,7 @@ void condensed_free(cicy_cond *c) {
                     c->core_b, c->core_x};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    condensed_free_device(c);
     delete c;
 }
 
@@ -454,11 +419,16 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
         }
         // couplings among this range's anchors and earlier anchors in the original matrix
         // (junctions, adjacent anchors)
+        // (from the band layout: the band entry to the previous row, then the row's own
+        // half-entries in stored order; an entry whose value is exactly zero is absent)
         for (int r = cut[t]; r < cut[t + 1]; r++) {
             if (anchor_of[r] < 0) continue;
-            for (int k = h.rp[r]; k < h.rp[r + 1]; k++) {
-                const int c = h.col[k];
-                if (anchor_of[c] >= 0) P.jfill.push_back({anchor_of[r], anchor_of[c], h.val[k]});
+            if (r > 0 && band[r] != 0.0 && anchor_of[r - 1] >= 0)
+                P.jfill.push_back({anchor_of[r], anchor_of[r - 1], band[r]});
+            for (int k = bh.xrp[r]; k < bh.xrp[r + 1]; k++) {
+                const int c = bh.xnbr[k];
+                if (c < r && anchor_of[c] >= 0)
+                    P.jfill.push_back({anchor_of[r], anchor_of[c], bh.xval[k]});
             }
         }
     });
@@ -541,11 +511,10 @@ bool build_condensed_host(cicy_ctx *ctx, const BlockedHost &h, const BandHost &b
 // pair of neighbours (u, v) gains the coupling -a_uk a_vk / d_k (for two neighbours this is
 // series reduction of the resistor network, for more the star-mesh transform).  This can
 // leave other nodes with few neighbours.  Nodes are eliminated in rounds; each round takes
-// an independent set of such nodes (greedy, in index order), so a chain of L nodes
-// collapses in about log2 L rounds and the nodes of a round are processed in parallel in
-// the per-solve sweeps.  The cap bounds the fill (a node of degree D adds at most
+// an independent set of such nodes (greedy, in elim_priority order, a fixed hash of the
+// node), so a chain of L nodes collapses in O(log L) rounds and the nodes of a round are
+// processed in parallel in the per-solve sweeps.  The cap bounds the fill (a node of degree D adds at most
 // D (D - 1) / 2 couplings).  What remains is the core system.
-constexpr int kElimDegree = 4;
 
 struct SeriesHost {
     int m = 0, mc = 0;
@@ -564,265 +533,160 @@ struct SeriesHost {
     std::vector<double> rv_a;
 };
 
-// Adjacency lists of S: node u's neighbours are nb[u][0 .. len[u]) with couplings vv[u][..]
-// and room for cap[u].  A list that outgrows its room moves to twice the room in the arena
-// of the thread that grows it (arenas are chunks that never move, so the lists of other
-// nodes stay where they are while threads work on disjoint nodes).  Removal swaps the last
-// entry into the gap, insertion appends, so the order of a list depends only on the sequence
-// of operations on it.
-struct AdjLists {
-    std::vector<int *> nb;
-    std::vector<double *> vv;
-    std::vector<int> len, cap;
-    struct Arena {
-        std::vector<std::unique_ptr<int[]>> ic;
-        std::vector<std::unique_ptr<double[]>> dc;
-        size_t used = 0, size = 0;
-        void take(int n, int **pi, double **pd) {
-            if (used + (size_t)n > size) {
-                size = std::max<size_t>(1 << 16, (size_t)n);
-                ic.emplace_back(new int[size]);
-                dc.emplace_back(new double[size]);
-                used = 0;
-            }
-            *pi = ic.back().get() + used;
-            *pd = dc.back().get() + used;
-            used += (size_t)n;
-        }
-    };
-    std::vector<Arena> arenas;
-    int find(int u, int c) const {
-        const int *p = nb[u];
-        for (int i = 0; i < len[u]; i++)
-            if (p[i] == c) return i;
-        return -1;
-    }
-    void drop(int u, int k) {
-        const int i = find(u, k), last = len[u] - 1;
-        nb[u][i] = nb[u][last];
-        vv[u][i] = vv[u][last];
-        len[u]--;
-    }
-    void add(int u, int v, double a, Arena &ar) {
-        const int i = find(u, v);
-        if (i >= 0) {
-            vv[u][i] += a;
-            return;
-        }
-        if (len[u] == cap[u]) {
-            const int c = std::max(4, 2 * cap[u]);
-            int *pi;
-            double *pd;
-            ar.take(c, &pi, &pd);
-            std::copy(nb[u], nb[u] + len[u], pi);
-            std::copy(vv[u], vv[u] + len[u], pd);
-            nb[u] = pi;
-            vv[u] = pd;
-            cap[u] = c;
-        }
-        nb[u][len[u]] = v;
-        vv[u][len[u]] = a;
-        len[u]++;
-    }
-};
-
-// Stage 2 on the host.  Each round: the candidates (at most max_degree neighbours) are
-// taken greedily in index order into an independent set, their elimination records are
-// written from the state before the round, and the records' updates are applied grouped by
-// the node they change: a node's own list and diagonal are changed only by the records it
-// neighbours, in record order, so the nodes are shared out over host threads and the result
-// is the same as one record after the other (and the same for any thread count).  The
-// grouping is also each round's receiver list of the per-solve forward sweep.
+// Stage 2 on the host, the device set-up's rounds (dea_setup.hip) in sequential form.  S is
+// kept as a symmetric CSR with ascending columns.  Each round: the candidates (alive, at most
+// max_degree neighbours, a usable pivot) are taken greedily in elim_priority order into an
+// independent set; its nodes, ascending, are the round's records (their rows before the
+// round); every node next to a record is a receiver, whose diagonal loses a_uk^2 / d_k and
+// whose row gains the fill -a_uk a_vk / d_k, record after record; equal columns are summed
+// after the old value, in record order.
 void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimDegree) {
     const int m = C.m;
     Z.m = m;
-    const int nth = host_threads(m, 1 << 12);
     StageClock clk;
-    double t_sel = 0, t_rec = 0, t_grp = 0, t_app = 0;
-    auto now = []() { return std::chrono::steady_clock::now(); };
-    auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
-        return std::chrono::duration<double, std::milli>(b - a).count();
-    };
-    // S's lower triangle gives each node its lower neighbours (ascending) and, from later
-    // rows, its upper ones (ascending): every list starts sorted by index
-    AdjLists A;
-    A.len.assign((size_t)m, 0);
-    A.cap.resize((size_t)m);
-    A.nb.resize((size_t)m);
-    A.vv.resize((size_t)m);
-    A.arenas.resize((size_t)nth + 1);
+    // symmetric CSR: each row's lower entries, then the rows below it that couple to it
+    std::vector<int> arp((size_t)m + 1, 0), acol;
+    std::vector<double> aval;
     for (int r = 0; r < m; r++)
         for (int64_t k = C.rrp[r]; k < C.rrp[r + 1]; k++) {
-            A.len[r]++;
-            A.len[(size_t)C.rcol[k]]++;
+            arp[(size_t)r + 1]++;
+            arp[(size_t)C.rcol[k] + 1]++;
         }
+    for (int r = 0; r < m; r++) arp[(size_t)r + 1] += arp[(size_t)r];
+    acol.resize((size_t)arp[(size_t)m]);
+    aval.resize((size_t)arp[(size_t)m]);
     {
-        size_t total = 0;
-        for (int u = 0; u < m; u++) total += (size_t)A.len[u] + 2;
-        AdjLists::Arena &a0 = A.arenas[0];
-        a0.size = total;
-        a0.ic.emplace_back(new int[std::max<size_t>(total, 1)]);
-        a0.dc.emplace_back(new double[std::max<size_t>(total, 1)]);
-        size_t o = 0;
-        for (int u = 0; u < m; u++) {
-            A.nb[u] = a0.ic[0].get() + o;
-            A.vv[u] = a0.dc[0].get() + o;
-            A.cap[u] = A.len[u] + 2;  // a little room for the fill of the first rounds
-            o += (size_t)A.cap[u];
-            A.len[u] = 0;
-        }
-        a0.used = o;
+        std::vector<int> at(arp.begin(), arp.end() - 1);
+        for (int r = 0; r < m; r++)
+            for (int64_t k = C.rrp[r]; k < C.rrp[r + 1]; k++) {
+                const int c = (int)C.rcol[k];
+                acol[(size_t)at[r]] = c;
+                aval[(size_t)at[r]++] = C.rval[k];
+                acol[(size_t)at[c]] = r;
+                aval[(size_t)at[c]++] = C.rval[k];
+            }
     }
-    for (int r = 0; r < m; r++)
-        for (int64_t k = C.rrp[r]; k < C.rrp[r + 1]; k++) {
-            const int c = (int)C.rcol[k];
-            A.nb[r][A.len[r]] = c;
-            A.vv[r][A.len[r]++] = C.rval[k];
-            A.nb[c][A.len[c]] = r;
-            A.vv[c][A.len[c]++] = C.rval[k];
-        }
     std::vector<double> d(C.sdiag.begin(), C.sdiag.end());
-    std::vector<char> alive((size_t)m, 1), taken((size_t)m, 0);
+    std::vector<char> alive((size_t)m, 1), in((size_t)m, 0);
+    std::vector<int> rec_of((size_t)m, -1);
     Z.round_off.assign(1, 0);
     Z.e_off.assign(1, 0);
     Z.rv_round_off.assign(1, 0);
     Z.rv_off.assign(1, 0);
-    std::vector<int> cand, sel, gcnt((size_t)m + 1, 0);
-    struct Op {
-        int r, t;  // record, position of the node in the record's neighbour list
-    };
-    std::vector<Op> ops;
-    std::vector<int> targets, toff;
-    for (int i = 0; i < m; i++)
-        if (A.len[i] <= max_degree) cand.push_back(i);
+    std::vector<int> cand, nrp, ncol;
+    std::vector<double> nval;
+    std::vector<std::pair<int, double>> fill;
     clk.mark("series: adjacency");
-    while (!cand.empty()) {
-        auto t0 = now();
-        std::sort(cand.begin(), cand.end());
-        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-        sel.clear();
+    for (;;) {
+        cand.clear();
+        for (int k = 0; k < m; k++)
+            if (alive[k] && arp[k + 1] - arp[k] <= max_degree &&
+                d[k] > 1e-12 * std::fabs(C.sdiag[k]) && std::isfinite(d[k]))
+                cand.push_back(k);
+        std::sort(cand.begin(), cand.end(), [](int a, int b) {
+            return elim_priority(a) < elim_priority(b);
+        });
+        int nsel = 0;
         for (int k : cand) {
-            if (!alive[k] || A.len[k] > max_degree) continue;
-            // never divide by a vanishing pivot (a floating group of anchors): keep the
-            // node in the core instead
-            if (!(d[k] > 1e-12 * std::fabs(C.sdiag[k])) || !std::isfinite(d[k])) continue;
             bool free = true;
-            for (int i = 0; i < A.len[k]; i++)
-                if (taken[A.nb[k][i]]) free = false;
-            if (!free) continue;
-            taken[k] = 1;
-            sel.push_back(k);
+            for (int t = arp[k]; t < arp[k + 1]; t++)
+                if (in[acol[t]]) free = false;
+            if (free) {
+                in[k] = 1;
+                nsel++;
+            }
         }
-        if (sel.empty()) break;
-        auto t1 = now();
-        t_sel += ms(t0, t1);
-        // records from the state before the round (selected nodes are not adjacent)
-        const int first = (int)Z.e_node.size();
-        const int nsel = (int)sel.size();
-        for (int k : sel) {
+        if (nsel == 0) break;
+        // records, ascending
+        for (int k = 0; k < m; k++) {
+            if (!in[k]) continue;
+            rec_of[k] = (int)Z.e_node.size();
             Z.e_node.push_back(k);
-            Z.e_off.push_back(Z.e_off.back() + A.len[k]);
             Z.e_d.push_back(d[k]);
-        }
-        Z.e_nbr.resize((size_t)Z.e_off.back());
-        Z.e_a.resize((size_t)Z.e_off.back());
-        for (int q = 0; q < nsel; q++) {
-            const int k = sel[q], o = Z.e_off[first + q];
-            std::copy(A.nb[k], A.nb[k] + A.len[k], Z.e_nbr.begin() + o);
-            std::copy(A.vv[k], A.vv[k] + A.len[k], Z.e_a.begin() + o);
-        }
-        auto t2 = now();
-        t_rec += ms(t1, t2);
-        // the updates grouped by the node they change (stable: record order)
-        const int e0 = Z.e_off[first], e1 = Z.e_off.back();
-        targets.clear();
-        for (int i = e0; i < e1; i++) gcnt[(size_t)Z.e_nbr[i] + 1]++;
-        // the changed nodes in ascending order: a scan of the counts (cheaper than sorting
-        // them while rounds are large)
-        if ((long long)(e1 - e0) * 8 > m) {
-            for (int u = 0; u < m; u++)
-                if (gcnt[(size_t)u + 1]) targets.push_back(u);
-        } else {
-            for (int i = e0; i < e1; i++) {
-                const int u = Z.e_nbr[i];
-                if (gcnt[(size_t)u + 1] > 0) {
-                    targets.push_back(u);
-                    gcnt[(size_t)u + 1] = -gcnt[(size_t)u + 1];  // listed once
-                }
+            for (int t = arp[k]; t < arp[k + 1]; t++) {
+                Z.e_nbr.push_back(acol[t]);
+                Z.e_a.push_back(aval[t]);
             }
-            for (int u : targets) gcnt[(size_t)u + 1] = -gcnt[(size_t)u + 1];
-            std::sort(targets.begin(), targets.end());
+            Z.e_off.push_back((int)Z.e_nbr.size());
         }
-        toff.assign(targets.size() + 1, 0);
-        for (size_t j = 0; j < targets.size(); j++) {
-            toff[j + 1] = toff[j] + gcnt[(size_t)targets[j] + 1];
-            gcnt[(size_t)targets[j] + 1] = toff[j];  // becomes the fill position
-        }
-        ops.resize((size_t)(e1 - e0));
-        for (int q = 0; q < nsel; q++) {
-            const int rec = first + q;
-            for (int i = Z.e_off[rec]; i < Z.e_off[rec + 1]; i++)
-                ops[(size_t)gcnt[(size_t)Z.e_nbr[i] + 1]++] = {rec, i - Z.e_off[rec]};
-        }
-        for (int u : targets) gcnt[(size_t)u + 1] = 0;
-        // receivers of this round (the forward sweep's deterministic gather)
-        {
-            const size_t n0 = Z.rv_node.size(), c0 = Z.rv_rec.size();
-            Z.rv_node.insert(Z.rv_node.end(), targets.begin(), targets.end());
-            Z.rv_rec.resize(c0 + ops.size());
-            Z.rv_a.resize(c0 + ops.size());
-            Z.rv_off.resize(n0 + targets.size() + 1);
-            for (size_t j = 0; j < targets.size(); j++)
-                Z.rv_off[n0 + j + 1] = (int)c0 + toff[j + 1];
-            for (size_t o = 0; o < ops.size(); o++) {
-                Z.rv_rec[c0 + o] = ops[o].r;
-                Z.rv_a[c0 + o] = Z.e_a[(size_t)Z.e_off[ops[o].r] + ops[o].t];
-            }
-        }
-        Z.rv_round_off.push_back((int)Z.rv_node.size());
-        auto t3 = now();
-        t_grp += ms(t2, t3);
-        // apply: each thread takes a range of the changed nodes
-        const int nt = std::min(nth, std::max(1, (int)(targets.size() / 1024)));
-        parallel_for(nt, [&](int t) {
-            AdjLists::Arena &ar = A.arenas[(size_t)t + 1];
-            const size_t j0 = targets.size() * t / nt, j1 = targets.size() * (t + 1) / nt;
-            for (size_t j = j0; j < j1; j++) {
-                const int u = targets[j];
-                for (int o = toff[j]; o < toff[j + 1]; o++) {
-                    const Op &op = ops[(size_t)o];
-                    const int rec = op.r, b = Z.e_off[rec], e = Z.e_off[rec + 1];
-                    const double dk = Z.e_d[rec];
-                    const double au = Z.e_a[(size_t)b + op.t];
-                    A.drop(u, Z.e_node[rec]);
-                    d[u] -= au * au / dk;
-                    // the pair (i, j), i < j, of the record's neighbours adds
-                    // -a_i a_j / d_k to both of their couplings
-                    for (int i = b; i < e; i++) {
-                        if (i == b + op.t) continue;
-                        const int lo = std::min(i, b + op.t), hi = std::max(i, b + op.t);
-                        const double f = -Z.e_a[lo] * Z.e_a[hi] / dk;
-                        A.add(u, Z.e_nbr[i], f, ar);
+        // receivers, diagonals and the new rows
+        nrp.assign((size_t)m + 1, 0);
+        ncol.clear();
+        nval.clear();
+        for (int u = 0; u < m; u++) {
+            if (alive[u] && !in[u]) {
+                bool target = false;
+                for (int t = arp[u]; t < arp[u + 1]; t++)
+                    if (in[acol[t]]) target = true;
+                if (!target) {
+                    for (int t = arp[u]; t < arp[u + 1]; t++) {
+                        ncol.push_back(acol[t]);
+                        nval.push_back(aval[t]);
+                    }
+                } else {
+                    Z.rv_node.push_back(u);
+                    fill.clear();
+                    double du = d[u];
+                    for (int t = arp[u]; t < arp[u + 1]; t++) {
+                        const int j = acol[t];
+                        if (!in[j]) continue;
+                        const int rec = rec_of[j];
+                        const double a = aval[t], dk = Z.e_d[rec];
+                        Z.rv_rec.push_back(rec);
+                        Z.rv_a.push_back(a);
+                        du = du - a * a / dk;
+                        for (int q = Z.e_off[rec]; q < Z.e_off[rec + 1]; q++) {
+                            const int v = Z.e_nbr[q];
+                            if (v != u) fill.push_back({v, -a * Z.e_a[q] / dk});
+                        }
+                    }
+                    d[u] = du;
+                    Z.rv_off.push_back((int)Z.rv_rec.size());
+                    std::stable_sort(fill.begin(), fill.end(),
+                                     [](const std::pair<int, double> &x,
+                                        const std::pair<int, double> &y) {
+                                         return x.first < y.first;
+                                     });
+                    size_t p = 0;
+                    int t = arp[u];
+                    const int te = arp[u + 1];
+                    for (;;) {
+                        while (t < te && in[acol[t]]) t++;
+                        if (t >= te && p >= fill.size()) break;
+                        int c;
+                        double v;
+                        if (p >= fill.size() || (t < te && acol[t] < fill[p].first)) {
+                            c = acol[t];
+                            v = aval[t++];
+                        } else if (t >= te || fill[p].first < acol[t]) {
+                            c = fill[p].first;
+                            v = fill[p++].second;
+                            while (p < fill.size() && fill[p].first == c) v += fill[p++].second;
+                        } else {
+                            c = acol[t];
+                            v = aval[t++];
+                            while (p < fill.size() && fill[p].first == c) v += fill[p++].second;
+                        }
+                        ncol.push_back(c);
+                        nval.push_back(v);
                     }
                 }
             }
-        });
-        t_app += ms(t3, now());
-        cand.clear();
-        for (int u : targets)
-            if (A.len[u] <= max_degree) cand.push_back(u);
-        for (int k : sel) {
-            alive[k] = 0;
-            taken[k] = 0;
-            A.len[k] = 0;
+            nrp[(size_t)u + 1] = (int)ncol.size();
         }
+        for (int k = 0; k < m; k++)
+            if (in[k]) {
+                alive[k] = 0;
+                in[k] = 0;
+            }
+        arp.swap(nrp);
+        acol.swap(ncol);
+        aval.swap(nval);
         Z.round_off.push_back((int)Z.e_node.size());
+        Z.rv_round_off.push_back((int)Z.rv_node.size());
     }
-    if (clk.on)
-        fprintf(stderr, "[cicy setup]   rounds: select %.1f records %.1f group %.1f apply %.1f ms\n",
-                t_sel, t_rec, t_grp, t_app);
     clk.mark("series: rounds");
-    // core: the nodes left, in S order
+    // core: the nodes left, in S order, with their lower entries
     std::vector<int> core_of((size_t)m, -1);
     for (int i = 0; i < m; i++)
         if (alive[i]) {
@@ -830,43 +694,20 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimD
             Z.core_nodes.push_back(i);
         }
     Z.mc = (int)Z.core_nodes.size();
-    clk.mark("series: core numbering");
     Z.crp.assign((size_t)Z.mc + 1, 0);
     Z.cdiag.resize((size_t)Z.mc);
-    // each core row's lower entries, counted and then written and sorted on host threads
-    const int nct = host_threads(Z.mc, 1 << 14);
-    parallel_for(nct, [&](int t) {
-        const int q0 = (int)((long long)Z.mc * t / nct), q1 = (int)((long long)Z.mc * (t + 1) / nct);
-        for (int q = q0; q < q1; q++) {
-            const int i = Z.core_nodes[q];
-            Z.cdiag[q] = d[i];
-            int c = 0;
-            for (int t2 = 0; t2 < A.len[i]; t2++)
-                if (core_of[A.nb[i][t2]] < q) c++;
-            Z.crp[(size_t)q + 1] = c;
-        }
-    });
-    for (int q = 0; q < Z.mc; q++) Z.crp[q + 1] += Z.crp[q];
-    clk.mark("series: core row counts");
-    Z.ccol.resize((size_t)Z.crp[Z.mc]);
-    Z.cval.resize((size_t)Z.crp[Z.mc]);
-    parallel_for(nct, [&](int t) {
-        std::vector<std::pair<int, double>> row;
-        const int q0 = (int)((long long)Z.mc * t / nct), q1 = (int)((long long)Z.mc * (t + 1) / nct);
-        for (int q = q0; q < q1; q++) {
-            const int i = Z.core_nodes[q];
-            row.clear();
-            for (int t2 = 0; t2 < A.len[i]; t2++) {
-                const int c = core_of[A.nb[i][t2]];
-                if (c < q) row.push_back({c, A.vv[i][t2]});
-            }
-            std::sort(row.begin(), row.end());
-            for (size_t k = 0; k < row.size(); k++) {
-                Z.ccol[(size_t)Z.crp[q] + k] = row[k].first;
-                Z.cval[(size_t)Z.crp[q] + k] = row[k].second;
+    for (int q = 0; q < Z.mc; q++) {
+        const int i = Z.core_nodes[q];
+        Z.cdiag[q] = d[i];
+        for (int t = arp[i]; t < arp[i + 1]; t++) {
+            const int c = core_of[acol[t]];
+            if (c < q) {
+                Z.ccol.push_back(c);
+                Z.cval.push_back(aval[t]);
             }
         }
-    });
+        Z.crp[(size_t)q + 1] = (int64_t)Z.ccol.size();
+    }
     clk.mark("series: core rows");
 }
 
@@ -1102,9 +943,38 @@ static bool condensed_correction(cicy_sys *s, int inner, const double *rhs, doub
     return true;
 }
 
+// host copy of the core's strictly-lower rows, diagonal and full-system rows (kept on the
+// device by the device set-up)
+static bool core_to_host(cicy_ctx *ctx, cicy_cond *c) {
+    if (!c->on_device || !c->h_crp.empty()) return true;
+    const int mc = c->mc;
+    std::vector<int> frp((size_t)mc + 1), orig((size_t)mc);
+    c->h_cdiag.resize((size_t)mc);
+    if (!d2h_sync(ctx, frp.data(), c->d_frp, frp.size() * sizeof(int))) return false;
+    std::vector<int> fcol((size_t)frp[(size_t)mc]);
+    std::vector<double> fval(fcol.size());
+    if (!d2h_sync(ctx, fcol.data(), c->d_fcol, fcol.size() * sizeof(int)) ||
+        !d2h_sync(ctx, fval.data(), c->d_fval, fval.size() * sizeof(double)) ||
+        !d2h_sync(ctx, c->h_cdiag.data(), c->d_cdiag, (size_t)mc * sizeof(double)) ||
+        !d2h_sync(ctx, orig.data(), c->d_orig, (size_t)mc * sizeof(int)))
+        return false;
+    c->h_crp.assign((size_t)mc + 1, 0);
+    for (int q = 0; q < mc; q++) {
+        for (int k = frp[(size_t)q]; k < frp[(size_t)q + 1]; k++)
+            if (fcol[(size_t)k] < q) {
+                c->h_ccol.push_back(fcol[(size_t)k]);
+                c->h_cval.push_back(fval[(size_t)k]);
+            }
+        c->h_crp[(size_t)q + 1] = (int64_t)c->h_ccol.size();
+    }
+    c->h_orig = std::move(orig);
+    return true;
+}
+
 bool condensed_core_sys(cicy_sys *s, int inner) {
     cicy_cond *c = s->cond;
     if (!c || c->mc <= 0) return true;
+    if (!core_to_host(s->ctx, c)) return false;
     if (!c->red) {
         if (!sys_create(s->ctx, c->mc, (int64_t)c->h_cval.size(), c->h_crp.data(),
                         c->h_ccol.data(), c->h_cval.data(), c->h_cdiag.data(), &c->red))
@@ -1134,8 +1004,11 @@ int condensed_set_blocks(cicy_sys *s, int64_t nblocks, const int64_t *ranges) {
         core_free(c->blk);
         c->blk = nullptr;
     }
-    c->blk = core_setup(ctx, c->mc, c->h_crp, c->h_ccol, c->h_cval, c->h_cdiag, c->h_orig,
-                        nblocks, ranges);
+    c->blk = c->on_device
+                 ? core_setup_device(ctx, c->mc, c->d_frp, c->d_fcol, c->d_fval, c->d_cdiag,
+                                     c->d_orig, nblocks, ranges)
+                 : core_setup(ctx, c->mc, c->h_crp, c->h_ccol, c->h_cval, c->h_cdiag, c->h_orig,
+                              nblocks, ranges);
     return c->blk ? 1 : 0;
 }
 

@@ -52,6 +52,8 @@
 #include "dea_grid.h"
 #include "dea_internal.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -201,6 +203,106 @@ bool dense_spd_inverse(int n, std::vector<double> &a, double *inv) {
     for (int i = 0; i < n; i++)
         for (int j = i + 1; j < n; j++) inv[(size_t)i * n + j] = inv[(size_t)j * n + i];
     return true;
+}
+
+// The work-groups and their row partitions (phase u: runs of whole blocks, then Jacobi rows
+// by the cost model; phase w: runs of equal cost) and the sliced-ELL slices of the phase-w
+// rows, for a layout whose rows (rp, renumbered), Jacobi row count mj and blocks are set.
+// Returns the ELL size; sl_base / sl_end receive each slice's rows.
+size_t core_partitions(CoreHost &H, int max_groups, std::vector<int> &sl_base,
+                       std::vector<int> &sl_end) {
+    const int m = H.m;
+    const int nbk = (int)H.blk_row0.size();
+    H.G = std::max(1, std::min(max_groups, (m + kCoreRowsPerGroup - 1) / kCoreRowsPerGroup));
+    const int G = H.G;
+    // phase u: the blocks cut into G runs of about equal estimated cost, each at most one
+    // round of the block product (kCorePairs blocks) where the blocks allow it, then the
+    // Jacobi rows shared out so that every work-group's estimate reaches the same level
+    std::vector<double> bcost((size_t)nbk);
+    double btotal = 0.0;
+    for (int b = 0; b < nbk; b++) {
+        const double n = H.blk_n[b];
+        bcost[b] = kCostEntry * n * n + kCostRow * n + kCostRound / kCorePairs;
+        btotal += bcost[b];
+    }
+    H.bu.assign((size_t)G + 1, nbk);
+    {
+        const int cap = nbk <= G * kCorePairs ? kCorePairs : (nbk + G - 1) / G;
+        int b = 0;
+        double acc = 0.0;
+        for (int g = 0; g < G; g++) {
+            H.bu[g] = b;
+            // at least `must` blocks here, so that the rest fits the work-groups left
+            const int must = std::max(0, (nbk - b) - (G - g - 1) * cap);
+            const double target = btotal * (double)(g + 1) / G;
+            for (int cnt = 0; b < nbk && cnt < cap &&
+                              (cnt < must || acc + 0.5 * bcost[b] <= target);
+                 cnt++)
+                acc += bcost[b++];
+        }
+    }
+    std::vector<double> cb((size_t)G, 0.0);
+    for (int g = 0; g < G; g++) {
+        const int cnt = H.bu[g + 1] - H.bu[g];
+        for (int b = H.bu[g]; b < H.bu[g + 1]; b++) {
+            const double n = H.blk_n[b];
+            cb[g] += kCostEntry * n * n + kCostRow * n;
+        }
+        cb[g] += kCostRound * (double)((cnt + kCorePairs - 1) / kCorePairs);
+    }
+    auto rows_at = [&](double T) {
+        double t = 0.0;
+        for (int g = 0; g < G; g++) t += std::max(0.0, (T - cb[g]) / kCostRow);
+        return t;
+    };
+    double lo = 0.0, hi = *std::max_element(cb.begin(), cb.end()) + kCostRow * (H.mj + 1.0);
+    for (int it = 0; it < 100; it++) {
+        const double mid = 0.5 * (lo + hi);
+        (rows_at(mid) < H.mj ? lo : hi) = mid;
+    }
+    H.ju.assign((size_t)G + 1, 0);
+    {
+        long long given = 0;
+        for (int g = 0; g < G; g++) {
+            long long c = (long long)std::floor(std::max(0.0, (hi - cb[g]) / kCostRow));
+            c = std::min<long long>(c, H.mj - given);
+            if (g == G - 1) c = H.mj - given;
+            given += c;
+            H.ju[g + 1] = (int)given;
+        }
+    }
+    // phase w: the rows cut into G runs of equal cost (kCostWRow per row + its entries)
+    H.wr.assign((size_t)G + 1, m);
+    H.wr[0] = 0;
+    {
+        const double total = (double)kCostWRow * m + (double)H.rp[m];
+        double acc = 0.0;
+        int g = 1;
+        for (int r = 0; r < m && g < G; r++) {
+            acc += kCostWRow + (H.rp[r + 1] - H.rp[r]);
+            while (g < G && acc >= total * (double)g / G) H.wr[g++] = r + 1;
+        }
+    }
+    // sliced ELL of each work-group's phase-w rows: the slices and their offsets first, then
+    // the slices filled on host threads
+    H.wsl.assign(1, 0);
+    H.sl_off.clear();
+    H.sl_len.clear();
+    size_t ell = 0;
+    for (int gg = 0; gg < G; gg++) {
+        for (int base = H.wr[gg]; base < H.wr[gg + 1]; base += kWave) {
+            const int end = std::min(H.wr[gg + 1], base + kWave);
+            int len = 0;
+            for (int r = base; r < end; r++) len = std::max(len, H.rp[r + 1] - H.rp[r]);
+            H.sl_off.push_back((long long)ell);
+            H.sl_len.push_back(len);
+            sl_base.push_back(base);
+            sl_end.push_back(end);
+            ell += (size_t)len * kWave;
+        }
+        H.wsl.push_back((int)H.sl_len.size());
+    }
+    return ell;
 }
 
 // Core layout from the core's strictly-lower SSS triangle (crp/ccol/cval), its diagonal,
@@ -388,97 +490,8 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
             }
         });
     }
-    const int nbk = (int)H.blk_row0.size();
-    H.G = std::max(1, std::min(max_groups, (m + kCoreRowsPerGroup - 1) / kCoreRowsPerGroup));
-    const int G = H.G;
-    // phase u: the blocks cut into G runs of about equal estimated cost, each at most one
-    // round of the block product (kCorePairs blocks) where the blocks allow it, then the
-    // Jacobi rows shared out so that every work-group's estimate reaches the same level
-    std::vector<double> bcost((size_t)nbk);
-    double btotal = 0.0;
-    for (int b = 0; b < nbk; b++) {
-        const double n = H.blk_n[b];
-        bcost[b] = kCostEntry * n * n + kCostRow * n + kCostRound / kCorePairs;
-        btotal += bcost[b];
-    }
-    H.bu.assign((size_t)G + 1, nbk);
-    {
-        const int cap = nbk <= G * kCorePairs ? kCorePairs : (nbk + G - 1) / G;
-        int b = 0;
-        double acc = 0.0;
-        for (int g = 0; g < G; g++) {
-            H.bu[g] = b;
-            // at least `must` blocks here, so that the rest fits the work-groups left
-            const int must = std::max(0, (nbk - b) - (G - g - 1) * cap);
-            const double target = btotal * (double)(g + 1) / G;
-            for (int cnt = 0; b < nbk && cnt < cap &&
-                              (cnt < must || acc + 0.5 * bcost[b] <= target);
-                 cnt++)
-                acc += bcost[b++];
-        }
-    }
-    std::vector<double> cb((size_t)G, 0.0);
-    for (int g = 0; g < G; g++) {
-        const int cnt = H.bu[g + 1] - H.bu[g];
-        for (int b = H.bu[g]; b < H.bu[g + 1]; b++) {
-            const double n = H.blk_n[b];
-            cb[g] += kCostEntry * n * n + kCostRow * n;
-        }
-        cb[g] += kCostRound * (double)((cnt + kCorePairs - 1) / kCorePairs);
-    }
-    auto rows_at = [&](double T) {
-        double t = 0.0;
-        for (int g = 0; g < G; g++) t += std::max(0.0, (T - cb[g]) / kCostRow);
-        return t;
-    };
-    double lo = 0.0, hi = *std::max_element(cb.begin(), cb.end()) + kCostRow * (H.mj + 1.0);
-    for (int it = 0; it < 100; it++) {
-        const double mid = 0.5 * (lo + hi);
-        (rows_at(mid) < H.mj ? lo : hi) = mid;
-    }
-    H.ju.assign((size_t)G + 1, 0);
-    {
-        long long given = 0;
-        for (int g = 0; g < G; g++) {
-            long long c = (long long)std::floor(std::max(0.0, (hi - cb[g]) / kCostRow));
-            c = std::min<long long>(c, H.mj - given);
-            if (g == G - 1) c = H.mj - given;
-            given += c;
-            H.ju[g + 1] = (int)given;
-        }
-    }
-    // phase w: the rows cut into G runs of equal cost (kCostWRow per row + its entries)
-    H.wr.assign((size_t)G + 1, m);
-    H.wr[0] = 0;
-    {
-        const double total = (double)kCostWRow * m + (double)H.rp[m];
-        double acc = 0.0;
-        int g = 1;
-        for (int r = 0; r < m && g < G; r++) {
-            acc += kCostWRow + (H.rp[r + 1] - H.rp[r]);
-            while (g < G && acc >= total * (double)g / G) H.wr[g++] = r + 1;
-        }
-    }
-    // sliced ELL of each work-group's phase-w rows: the slices and their offsets first, then
-    // the slices filled on host threads
-    H.wsl.assign(1, 0);
-    H.sl_off.clear();
-    H.sl_len.clear();
     std::vector<int> sl_base, sl_end;
-    size_t ell = 0;
-    for (int gg = 0; gg < G; gg++) {
-        for (int base = H.wr[gg]; base < H.wr[gg + 1]; base += kWave) {
-            const int end = std::min(H.wr[gg + 1], base + kWave);
-            int len = 0;
-            for (int r = base; r < end; r++) len = std::max(len, H.rp[r + 1] - H.rp[r]);
-            H.sl_off.push_back((long long)ell);
-            H.sl_len.push_back(len);
-            sl_base.push_back(base);
-            sl_end.push_back(end);
-            ell += (size_t)len * kWave;
-        }
-        H.wsl.push_back((int)H.sl_len.size());
-    }
+    const size_t ell = core_partitions(H, max_groups, sl_base, sl_end);
     H.ell_col.assign(ell, -1);
     H.ell_val.assign(ell, 0.0);
     const int nsl = (int)sl_base.size();
@@ -955,6 +968,503 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
         core_free(z);
         return nullptr;
     }
+    return z;
+}
+
+namespace {
+
+// ---- the layout on the device (core_setup_device) ------------------------------------
+
+constexpr int kLT = 256;  // threads of the layout kernels
+
+#define CORE_LOOP(i, n) \
+    for (long long i = (long long)blockIdx.x * kLT + threadIdx.x; i < (long long)(n); \
+         i += (long long)gridDim.x * kLT)
+
+inline int core_grid(long long n) {
+    return (int)std::max<long long>(1, std::min<long long>((n + kLT - 1) / kLT, 1 << 16));
+}
+
+#pragma clang fp contract(off)
+
+// out[t] = first core row whose full-system row is >= vals[t] (lower_bound in orig)
+__global__ __launch_bounds__(kLT) void k_group_bounds(int nv, const int64_t *__restrict__ vals,
+                                                      const int *__restrict__ orig, int m,
+                                                      int *__restrict__ out) {
+    CORE_LOOP(t, nv) {
+        const int64_t v = vals[t];
+        int lo = 0, hi = m;
+        while (lo < hi) {
+            const int mid = lo + (hi - lo) / 2;
+            if ((int64_t)orig[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        out[t] = lo;
+    }
+}
+
+// Dense inverse of one diagonal block per work-group, with dense_spd_inverse's operations in
+// its order: Cholesky column by column, W = L^-1 row by row, A^-1 = W^T W, then the mean
+// entry c (summed in row-major order).  LDS holds the block: L in the lower triangle, W in
+// the strict upper one (W[i][j] at [j][i]) and W's diagonal after it.
+__global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ gq0,
+                                                       const int *__restrict__ gnb,
+                                                       const long long *__restrict__ goff,
+                                                       const int *__restrict__ frp,
+                                                       const int *__restrict__ fcol,
+                                                       const double *__restrict__ fval,
+                                                       const double *__restrict__ cdiag,
+                                                       double *__restrict__ inv,
+                                                       int *__restrict__ ok,
+                                                       double *__restrict__ cg) {
+    extern __shared__ double A[];
+    __shared__ int fail;
+    const int g = blockIdx.x, q0 = gq0[g], nb = gnb[g], tid = threadIdx.x;
+    double *wd = A + (size_t)nb * nb;
+    double *out = inv + goff[g];
+    for (int e = tid; e < nb * nb; e += kLT) A[e] = 0.0;
+    if (tid == 0) fail = 0;
+    __syncthreads();
+    for (int i = tid; i < nb; i += kLT) {
+        A[i * nb + i] = cdiag[q0 + i];
+        for (int k = frp[q0 + i]; k < frp[q0 + i + 1]; k++) {
+            const int c = fcol[k] - q0;
+            if (c >= 0 && c < nb) A[i * nb + c] = fval[k];
+        }
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; j++) {
+        if (tid == 0) {
+            double d = A[j * nb + j];
+            for (int k = 0; k < j; k++) d = d - A[j * nb + k] * A[j * nb + k];
+            if (!(d > 0.0) || !isfinite(d)) fail = 1;
+            else A[j * nb + j] = sqrt(d);
+        }
+        __syncthreads();
+        if (fail) break;
+        const double l = A[j * nb + j];
+        for (int i = j + 1 + tid; i < nb; i += kLT) {
+            double v = A[i * nb + j];
+            for (int k = 0; k < j; k++) v = v - A[i * nb + k] * A[j * nb + k];
+            A[i * nb + j] = v / l;
+        }
+        __syncthreads();
+    }
+    if (fail) {
+        if (tid == 0) {
+            ok[g] = 0;
+            cg[g] = 0.0;
+        }
+        return;
+    }
+    for (int i = 0; i < nb; i++) {
+        const double inv_d = 1.0 / A[i * nb + i];
+        for (int j = tid; j < i; j += kLT) {
+            double acc = 0.0;
+            for (int k = j; k < i; k++) {
+                const double wkj = k == j ? wd[j] : A[j * nb + k];
+                acc = acc - A[i * nb + k] * wkj;
+            }
+            A[j * nb + i] = acc * inv_d;
+        }
+        if (tid == 0) wd[i] = inv_d;
+        __syncthreads();
+    }
+    for (int i = 0; i < nb; i++)
+        for (int j = tid; j <= i; j += kLT) {
+            double acc = 0.0;
+            for (int k = i; k < nb; k++) {
+                const double wki = k == i ? wd[i] : A[i * nb + k];
+                const double wkj = k == j ? wd[j] : A[j * nb + k];
+                acc = acc + wki * wkj;
+            }
+            out[(size_t)i * nb + j] = acc;
+            out[(size_t)j * nb + i] = acc;
+        }
+    __syncthreads();
+    if (tid == 0) {
+        double c = 0.0;
+        for (int e = 0; e < nb * nb; e++) c = c + out[e];
+        const double dn = nb;
+        cg[g] = c / (dn * dn);
+        ok[g] = 1;
+    }
+}
+
+// kept block b: its rows' new numbers, its constant, its FP32 remainder
+__global__ __launch_bounds__(kLT) void k_block_keep(const int *__restrict__ kq0,
+                                                    const int *__restrict__ knb,
+                                                    const int *__restrict__ krow0,
+                                                    const long long *__restrict__ kgoff,
+                                                    const long long *__restrict__ kboff,
+                                                    const int *__restrict__ kg,
+                                                    const double *__restrict__ inv,
+                                                    const double *__restrict__ cg,
+                                                    int *__restrict__ in_blk,
+                                                    int *__restrict__ iperm, int *__restrict__ perm,
+                                                    double *__restrict__ blk_c,
+                                                    float *__restrict__ binv) {
+    const int b = blockIdx.x, q0 = kq0[b], nb = knb[b], row0 = krow0[b];
+    const double c = cg[kg[b]];
+    for (int t = threadIdx.x; t < nb; t += kLT) {
+        in_blk[q0 + t] = 1;
+        iperm[q0 + t] = row0 + t;
+        perm[row0 + t] = q0 + t;
+    }
+    const double *src = inv + kgoff[b];
+    float *dst = binv + kboff[b];
+    for (int e = threadIdx.x; e < nb * nb; e += kLT) dst[e] = (float)(src[e] - c);
+    if (threadIdx.x == 0) blk_c[b] = c;
+}
+
+__global__ __launch_bounds__(kLT) void k_jacobi_flags(int m, const int *__restrict__ in_blk,
+                                                      int *__restrict__ flag) {
+    CORE_LOOP(q, m + 1) flag[q] = q < m && !in_blk[q];
+}
+
+__global__ __launch_bounds__(kLT) void k_jacobi_perm(int m, const int *__restrict__ in_blk,
+                                                     const int *__restrict__ jpos,
+                                                     int *__restrict__ iperm,
+                                                     int *__restrict__ perm) {
+    CORE_LOOP(q, m) {
+        if (in_blk[q]) continue;
+        iperm[q] = jpos[q];
+        perm[jpos[q]] = (int)q;
+    }
+}
+
+__global__ __launch_bounds__(kLT) void k_perm_len(int m, const int *__restrict__ perm,
+                                                  const int *__restrict__ frp,
+                                                  int *__restrict__ len) {
+    CORE_LOOP(i, m + 1) len[i] = i < m ? frp[perm[i] + 1] - frp[perm[i]] : 0;
+}
+
+// Row i of the renumbered layout is row perm[i]: its Jacobi columns (renumbered below mj,
+// in the old order, which the renumbering keeps) then its block columns.
+__global__ __launch_bounds__(kLT) void k_permute_rows(int m, int mj, const int *__restrict__ perm,
+                                                      const int *__restrict__ iperm,
+                                                      const int *__restrict__ frp,
+                                                      const int *__restrict__ fcol,
+                                                      const double *__restrict__ fval,
+                                                      const double *__restrict__ cdiag,
+                                                      const int *__restrict__ rp2,
+                                                      int *__restrict__ col2,
+                                                      double *__restrict__ val2,
+                                                      double *__restrict__ diag2) {
+    CORE_LOOP(i, m) {
+        const int o = perm[i];
+        int d = rp2[i];
+        for (int k = frp[o]; k < frp[o + 1]; k++) {
+            const int c = iperm[fcol[k]];
+            if (c < mj) {
+                col2[d] = c;
+                val2[d++] = fval[k];
+            }
+        }
+        for (int k = frp[o]; k < frp[o + 1]; k++) {
+            const int c = iperm[fcol[k]];
+            if (c >= mj) {
+                col2[d] = c;
+                val2[d++] = fval[k];
+            }
+        }
+        diag2[i] = cdiag[o];
+    }
+}
+
+// sliced ELL: slice q, lane l holds row sl_base[q] + l (padding: column -1, value 0)
+__global__ __launch_bounds__(kLT) void k_fill_ell(int nsl, const int *__restrict__ sl_base,
+                                                  const int *__restrict__ sl_end,
+                                                  const long long *__restrict__ sl_off,
+                                                  const int *__restrict__ sl_len,
+                                                  const int *__restrict__ rp2,
+                                                  const int *__restrict__ col2,
+                                                  const double *__restrict__ val2,
+                                                  int *__restrict__ ell_col,
+                                                  double *__restrict__ ell_val) {
+    CORE_LOOP(t, (long long)nsl * kWave) {
+        const int q = (int)(t / kWave), lane = (int)(t % kWave);
+        const int r = sl_base[q] + lane;
+        const int len = r < sl_end[q] ? rp2[r + 1] - rp2[r] : 0;
+        const long long o = sl_off[q];
+        for (int k = 0; k < sl_len[q]; k++) {
+            const long long pos = o + (long long)k * kWave + lane;
+            ell_col[pos] = k < len ? col2[rp2[r] + k] : -1;
+            ell_val[pos] = k < len ? val2[rp2[r] + k] : 0.0;
+        }
+    }
+}
+
+#pragma clang fp contract(on)
+
+template <typename T>
+bool core_dev(cicy_ctx *ctx, cicy_core *z, T **dst, size_t count) {
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    if (!hip_ok(ctx, hipMalloc((void **)dst, bytes), "hipMalloc (core)")) return false;
+    z->bytes += bytes;
+    return true;
+}
+
+// the kernel's residency: work-groups (at most one per CU, at most the grid sums' slots)
+bool core_groups(cicy_ctx *ctx, int *cus) {
+    int per_cu = 0;
+    if (!hip_ok(ctx, hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount,
+                                           ctx->device),
+                "hipDeviceGetAttribute") ||
+        !hip_ok(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &per_cu, reinterpret_cast<const void *>(k_core_cg), kCoreThreads, 0),
+                "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+        return false;
+    if (per_cu < 1) {
+        set_error(ctx, "core CG kernel cannot be resident");
+        return false;
+    }
+    *cus = std::min(*cus, kSlotLanes * kWave);
+    return true;
+}
+
+// staged LDS columns per work-group (its first slots, at most kCoreLdsCols), checked for
+// residency
+bool core_lds_cols(cicy_ctx *ctx, const CoreHost &H, int *lds_cols) {
+    long long most = 0;  // slots of the work-group with the most
+    for (int g = 0; g < H.G; g++)
+        if (H.wsl[g + 1] > H.wsl[g])
+            most = std::max(most, H.sl_off[H.wsl[g + 1] - 1] +
+                                      (long long)H.sl_len[H.wsl[g + 1] - 1] * kWave -
+                                      H.sl_off[H.wsl[g]]);
+    *lds_cols = (int)std::min<long long>(most, kCoreLdsCols);
+    int per_cu = 0;
+    if (!hip_ok(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &per_cu, reinterpret_cast<const void *>(k_core_cg), kCoreThreads,
+                         (size_t)*lds_cols * sizeof(int)),
+                "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+        return false;
+    if (per_cu < 1) {
+        set_error(ctx, "core CG kernel cannot be resident with %d staged columns", *lds_cols);
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+// The layout of core_setup built on the device from the core's symmetric CSR (frp, fcol,
+// fval: both triangles, columns ascending), its diagonal and the full-system row of each
+// core row (ascending), all on the device: equal to build_core_host's bit for bit.  The
+// caller's row groups (ranges, host) are cut and checked as there; the dense inverses, the
+// renumbering, the renumbered rows and the sliced ELL are built on the device, the
+// work-group partitions on the host from the renumbered row lengths.
+cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fcol,
+                             const double *fval, const double *cdiag, const int *orig,
+                             int64_t nblocks, const int64_t *ranges) {
+    hipStream_t st = ctx->stream;
+    int cus = 0;
+    if (!core_groups(ctx, &cus)) return nullptr;
+    StageClock clk;
+    PoolScope ps(ctx);
+    // row groups -> core row ranges (split at kCoreMaxBlock rows)
+    for (int64_t b = 0; b < nblocks; b++)
+        if (ranges[2 * b] < 0 || ranges[2 * b + 1] < ranges[2 * b]) {
+            set_error(ctx, "core blocks: a row group has a negative start or ends before it "
+                           "starts");
+            return nullptr;
+        }
+    std::vector<int> qb((size_t)std::max<int64_t>(2 * nblocks, 1));
+    if (nblocks > 0) {
+        int64_t *dr;
+        int *dq;
+        if (!ps.get(dr, 2 * (size_t)nblocks) || !ps.get(dq, 2 * (size_t)nblocks) ||
+            !hip_ok(ctx, hipMemcpyAsync(dr, ranges, 2 * (size_t)nblocks * sizeof(int64_t),
+                                        hipMemcpyHostToDevice, st), "H2D"))
+            return nullptr;
+        hipLaunchKernelGGL(k_group_bounds, dim3(core_grid(2 * nblocks)), dim3(kLT), 0, st,
+                           (int)(2 * nblocks), dr, orig, m, dq);
+        if (!hip_ok(ctx, hipGetLastError(), "k_group_bounds") ||
+            !d2h_sync(ctx, qb.data(), dq, 2 * (size_t)nblocks * sizeof(int)))
+            return nullptr;
+    }
+    std::vector<std::pair<int, int>> groups;
+    for (int64_t b = 0; b < nblocks; b++) {
+        const int q0 = qb[(size_t)(2 * b)], q1 = qb[(size_t)(2 * b + 1)];
+        for (int q = q0; q < q1; q += kCoreMaxBlock) {
+            const int e = std::min(q1, q + kCoreMaxBlock);
+            if (e - q >= 2) groups.emplace_back(q, e);
+        }
+    }
+    std::sort(groups.begin(), groups.end());
+    for (size_t g = 1; g < groups.size(); g++)
+        if (groups[g].first < groups[g - 1].second) {
+            set_error(ctx, "core blocks: row groups overlap");
+            return nullptr;
+        }
+    const int ng = (int)groups.size();
+    std::vector<int> gq0((size_t)ng), gnb((size_t)ng);
+    std::vector<long long> goff((size_t)ng + 1, 0);
+    int nbmax = 0;
+    for (int g = 0; g < ng; g++) {
+        gq0[(size_t)g] = groups[(size_t)g].first;
+        gnb[(size_t)g] = groups[(size_t)g].second - groups[(size_t)g].first;
+        nbmax = std::max(nbmax, gnb[(size_t)g]);
+        goff[(size_t)g + 1] = goff[(size_t)g] + (long long)gnb[(size_t)g] * gnb[(size_t)g];
+    }
+    // dense inverses
+    int *dgq0, *dgnb, *dok;
+    long long *dgoff;
+    double *dinv, *dcg;
+    std::vector<int> okh((size_t)ng, 0);
+    if (!ps.get(dgq0, (size_t)ng) || !ps.get(dgnb, (size_t)ng) || !ps.get(dok, (size_t)ng) ||
+        !ps.get(dgoff, (size_t)ng + 1) || !ps.get(dinv, (size_t)goff[(size_t)ng]) ||
+        !ps.get(dcg, (size_t)ng))
+        return nullptr;
+    if (ng > 0) {
+        const size_t lds = ((size_t)nbmax * nbmax + nbmax) * sizeof(double);
+        if (!hip_ok(ctx, hipMemcpyAsync(dgq0, gq0.data(), (size_t)ng * sizeof(int),
+                                        hipMemcpyHostToDevice, st), "H2D") ||
+            !hip_ok(ctx, hipMemcpyAsync(dgnb, gnb.data(), (size_t)ng * sizeof(int),
+                                        hipMemcpyHostToDevice, st), "H2D") ||
+            !hip_ok(ctx, hipMemcpyAsync(dgoff, goff.data(), ((size_t)ng + 1) * sizeof(long long),
+                                        hipMemcpyHostToDevice, st), "H2D") ||
+            !hip_ok(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_inverse),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds), "hipFuncSetAttribute"))
+            return nullptr;
+        hipLaunchKernelGGL(k_block_inverse, dim3(ng), dim3(kLT), lds, st, dgq0, dgnb, dgoff, frp,
+                           fcol, fval, cdiag, dinv, dok, dcg);
+        if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
+            !d2h_sync(ctx, okh.data(), dok, (size_t)ng * sizeof(int)))
+            return nullptr;
+    }
+    clk.mark("device core: block inverses");
+    // kept blocks, numbered after the Jacobi rows in group order
+    CoreHost H;
+    H.m = m;
+    std::vector<int> kq0, knb, krow0, kg;
+    std::vector<long long> kgoff, kboff;
+    int blocked = 0;
+    for (int g = 0; g < ng; g++)
+        if (okh[(size_t)g]) blocked += gnb[(size_t)g];
+    H.mj = m - blocked;
+    {
+        int row0 = H.mj;
+        long long at = 0;
+        for (int g = 0; g < ng; g++) {
+            if (!okh[(size_t)g]) continue;
+            const int nb = gnb[(size_t)g];
+            kq0.push_back(gq0[(size_t)g]);
+            knb.push_back(nb);
+            krow0.push_back(row0);
+            kg.push_back(g);
+            kgoff.push_back(goff[(size_t)g]);
+            kboff.push_back(at);
+            H.blk_row0.push_back(row0);
+            H.blk_n.push_back(nb);
+            H.blk_off.push_back(at);
+            at += (long long)nb * nb;
+            row0 += nb;
+        }
+    }
+    const int nbk = (int)kq0.size();
+    cicy_core *z = new cicy_core();
+    z->m = m;
+    z->nb = nbk;
+    z->blocked_rows = blocked;
+    z->inv_entries = nbk > 0 ? kboff.back() + (long long)knb.back() * knb.back() : 0;
+    auto fail = [&]() -> cicy_core * {
+        core_free(z);
+        return nullptr;
+    };
+    int *in_blk, *iperm, *jflag, *jpos, *len2, *rp2, *col2, *dk[4];
+    long long *dkl[2];
+    double *val2;
+    if (!ps.get(in_blk, (size_t)m + 1) || !ps.get(iperm, (size_t)m + 1) ||
+        !ps.get(jflag, (size_t)m + 1) || !ps.get(jpos, (size_t)m + 1) ||
+        !ps.get(len2, (size_t)m + 1) || !ps.get(rp2, (size_t)m + 1) ||
+        !core_dev(ctx, z, &z->perm, (size_t)m) || !core_dev(ctx, z, &z->diag, (size_t)m) ||
+        !core_dev(ctx, z, &z->blk_c, (size_t)nbk) ||
+        !core_dev(ctx, z, &z->binv, (size_t)z->inv_entries) ||
+        !hip_ok(ctx, hipMemsetAsync(in_blk, 0, ((size_t)m + 1) * sizeof(int), st), "memset"))
+        return fail();
+    for (int k = 0; k < 4; k++)
+        if (!ps.get(dk[k], (size_t)nbk)) return fail();
+    for (int k = 0; k < 2; k++)
+        if (!ps.get(dkl[k], (size_t)nbk)) return fail();
+    if (nbk > 0) {
+        const std::vector<int> *iv[] = {&kq0, &knb, &krow0, &kg};
+        for (int k = 0; k < 4; k++)
+            if (!hip_ok(ctx, hipMemcpyAsync(dk[k], iv[k]->data(), (size_t)nbk * sizeof(int),
+                                            hipMemcpyHostToDevice, st), "H2D"))
+                return fail();
+        if (!hip_ok(ctx, hipMemcpyAsync(dkl[0], kgoff.data(), (size_t)nbk * sizeof(long long),
+                                        hipMemcpyHostToDevice, st), "H2D") ||
+            !hip_ok(ctx, hipMemcpyAsync(dkl[1], kboff.data(), (size_t)nbk * sizeof(long long),
+                                        hipMemcpyHostToDevice, st), "H2D"))
+            return fail();
+        hipLaunchKernelGGL(k_block_keep, dim3(nbk), dim3(kLT), 0, st, dk[0], dk[1], dk[2], dkl[0],
+                           dkl[1], dk[3], dinv, dcg, in_blk, iperm, z->perm, z->blk_c, z->binv);
+    }
+    hipLaunchKernelGGL(k_jacobi_flags, dim3(core_grid(m + 1)), dim3(kLT), 0, st, m, in_blk, jflag);
+    size_t tmp_bytes = 0;
+    unsigned char *tmp = nullptr;
+    auto scan = [&](const int *in, int *out, int count) {
+        tmp_bytes = 0;
+        return hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, in, out, count, st),
+                      "scan") &&
+               ps.get(tmp, tmp_bytes) &&
+               hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, count, st),
+                      "scan");
+    };
+    if (!scan(jflag, jpos, m + 1)) return fail();
+    hipLaunchKernelGGL(k_jacobi_perm, dim3(core_grid(m)), dim3(kLT), 0, st, m, in_blk, jpos, iperm,
+                       z->perm);
+    hipLaunchKernelGGL(k_perm_len, dim3(core_grid(m + 1)), dim3(kLT), 0, st, m, z->perm, frp, len2);
+    if (!scan(len2, rp2, m + 1)) return fail();
+    H.rp.resize((size_t)m + 1);
+    if (!hip_ok(ctx, hipGetLastError(), "core renumbering") ||
+        !d2h_sync(ctx, H.rp.data(), rp2, ((size_t)m + 1) * sizeof(int)))
+        return fail();
+    const int nnz2 = H.rp[(size_t)m];
+    if (!ps.get(col2, (size_t)nnz2) || !ps.get(val2, (size_t)nnz2)) return fail();
+    hipLaunchKernelGGL(k_permute_rows, dim3(core_grid(m)), dim3(kLT), 0, st, m, H.mj, z->perm,
+                       iperm, frp, fcol, fval, cdiag, rp2, col2, val2, z->diag);
+    clk.mark("device core: renumbered rows");
+    // partitions and slices on the host (from the row lengths), the ELL on the device
+    std::vector<int> sl_base, sl_end;
+    const size_t ell = core_partitions(H, cus, sl_base, sl_end);
+    z->G = H.G;
+    z->nnz2 = nnz2;
+    int lds_cols = 0;
+    if (!core_lds_cols(ctx, H, &lds_cols)) return fail();
+    z->lds_cols = lds_cols;
+    const int nsl = (int)sl_base.size();
+    int *dsb, *dse;
+    if (!ps.get(dsb, (size_t)nsl) || !ps.get(dse, (size_t)nsl) ||
+        !up_vec(ctx, z, &z->ju, H.ju) || !up_vec(ctx, z, &z->bu, H.bu) ||
+        !up_vec(ctx, z, &z->wr, H.wr) || !up_vec(ctx, z, &z->wsl, H.wsl) ||
+        !up_vec(ctx, z, &z->sl_off, H.sl_off) || !up_vec(ctx, z, &z->sl_len, H.sl_len) ||
+        !up_vec(ctx, z, &z->blk_row0, H.blk_row0) || !up_vec(ctx, z, &z->blk_n, H.blk_n) ||
+        !up_vec(ctx, z, &z->blk_off, H.blk_off) || !core_dev(ctx, z, &z->ell_col, ell) ||
+        !core_dev(ctx, z, &z->ell_val, ell) ||
+        (nsl > 0 && (!hip_ok(ctx, hipMemcpyAsync(dsb, sl_base.data(), (size_t)nsl * sizeof(int),
+                                                 hipMemcpyHostToDevice, st), "H2D") ||
+                     !hip_ok(ctx, hipMemcpyAsync(dse, sl_end.data(), (size_t)nsl * sizeof(int),
+                                                 hipMemcpyHostToDevice, st), "H2D"))))
+        return fail();
+    if (nsl > 0)
+        hipLaunchKernelGGL(k_fill_ell, dim3(core_grid((long long)nsl * kWave)), dim3(kLT), 0, st,
+                           nsl, dsb, dse, z->sl_off, z->sl_len, rp2, col2, val2, z->ell_col,
+                           z->ell_val);
+    const bool ok = hip_ok(ctx, hipGetLastError(), "k_fill_ell") &&
+                    zeros(ctx, z, &z->x, m) && zeros(ctx, z, &z->r, m) && zeros(ctx, z, &z->u, m) &&
+                    zeros(ctx, z, &z->w, m) && zeros(ctx, z, &z->p, m) && zeros(ctx, z, &z->s, m) &&
+                    hip_ok(ctx, hipMalloc((void **)&z->part, 3 * kCoreSums *
+                                                               sizeof(unsigned long long) *
+                                                               (size_t)H.G),
+                           "hipMalloc (core)") &&
+                    hip_ok(ctx, hipMalloc((void **)&z->ctl, sizeof(GridCtl)), "hipMalloc (core)") &&
+                    hip_ok(ctx, hipMalloc((void **)&z->st, sizeof(CoreState)), "hipMalloc (core)") &&
+                    hip_ok(ctx, hipStreamSynchronize(st), "sync");
+    clk.mark("device core: slices + upload");
+    if (!ok) return fail();
     return z;
 }
 

@@ -114,8 +114,88 @@ inline void launch_apply(const DevMat<T> &m, int grid, hipStream_t st, const Bat
 }  // namespace cicy
 
 struct cicy_res;   // resident-CG layout (dea_resident.hip)
-struct cicy_cond;  // condensed system (dea_condensed.hip)
 struct cicy_core;  // block-preconditioned persistent core CG (dea_core.hip)
+
+namespace cicy {
+
+// Device memory reused from one set-up to the next: blocks are handed out best-fit and
+// returned when a set-up ends, so repeated set-ups of systems of one size allocate nothing
+// after the first.  Held by the context until it is destroyed.
+struct DevPool {
+    struct Blk {
+        void *p;
+        size_t bytes;
+        bool used;
+    };
+    std::vector<Blk> blocks;
+    void *get(size_t bytes);  // nullptr when hipMalloc fails
+    void put(void *p);
+    void clear();             // frees every block
+};
+
+// Nodes of the reduced system with at most this many neighbours are eliminated exactly
+// (series reduction / star-mesh transform) before the core CG.
+constexpr int kElimDegree = 4;
+
+// Elimination priority of node k of the reduced system in the series rounds: a fixed
+// integer hash of k with k itself as tie-break (lower value = earlier).  The rounds take
+// the greedy independent set of their candidates in this order, which the device finds in
+// a few parallel passes and the host form by one sweep in the same order.
+__host__ __device__ inline unsigned long long elim_priority(int k) {
+    unsigned int h = (unsigned int)k;
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return ((unsigned long long)h << 32) | (unsigned int)k;
+}
+
+}  // namespace cicy
+
+// The condensed system (dea_condensed.hip), set up on the device (dea_setup.hip) or, for
+// the blocked SSS layout, on the host.
+struct cicy_cond {
+    int n = 0, m = 0, npath = 0;
+    long long nnz_red = 0;
+    int *ps = nullptr, *pe = nullptr, *pl = nullptr, *pr = nullptr;  // [npath]
+    int *anchor_of = nullptr;   // [n] reduced index or -1
+    int *anchors = nullptr;     // [m] full index
+    double *lfac = nullptr, *mfac = nullptr;  // [n] Thomas multipliers and pivots (path rows)
+    double *band = nullptr;     // [n + 1] a(i, i-1) (0 where absent)
+    double *y = nullptr, *delta = nullptr, *rhs = nullptr;  // [n]
+    double *norm = nullptr;     // [2]
+    unsigned *counter = nullptr;
+    double *part = nullptr, *gpart = nullptr;
+    unsigned *gcount = nullptr;
+    int nb = 0;                 // work-groups of the norm kernel
+    // stage 2 (series rounds over S) and the core system the CG runs on
+    int mc = 0, nrec = 0;
+    long long nnz_core = 0;
+    std::vector<int> round_off;                       // host copy: launch bounds per round
+    std::vector<int> rv_round_off;                    // host copy: receivers of each round
+    int *e_node = nullptr, *e_off = nullptr, *e_nbr = nullptr, *core_nodes = nullptr;
+    // forward sweep by receiver: rv_node[j] gathers rv_a[c] * f_k / d_k over its
+    // contributions c in rv_off[j] .. rv_off[j+1] (record rv_rec[c]), in record order
+    int *rv_node = nullptr, *rv_off = nullptr, *rv_rec = nullptr;
+    double *rv_a = nullptr;
+    double *e_a = nullptr, *e_d = nullptr;
+    double *fS = nullptr, *xS = nullptr;              // [m] right-hand side / solution on S
+    double *core_b = nullptr, *core_x = nullptr;      // [mc] the block solver's core vectors
+    cicy_sys *red = nullptr;    // the core system on the device (built on first use)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timing (the inner solve uses the context's)
+    // host copy of the core (strictly-lower SSS, diagonal, full-system row of each core
+    // row) for the block-preconditioned core CG, built on demand
+    std::vector<int64_t> h_crp, h_ccol;
+    std::vector<double> h_cval, h_cdiag;
+    std::vector<int> h_orig;
+    // the core on the device (device set-up): both triangles with columns ascending
+    // (d_frp, d_fcol, d_fval), its diagonal and the full-system row of each core row
+    int *d_frp = nullptr, *d_fcol = nullptr, *d_orig = nullptr;
+    double *d_fval = nullptr, *d_cdiag = nullptr;
+    bool on_device = false;
+    cicy_core *blk = nullptr;
+};
 
 struct cicy_ctx {
     int device = 0;
@@ -132,6 +212,8 @@ struct cicy_ctx {
     double *bc_ws = nullptr;   // boundary-current workspace (grows, kept until destroy)
     double *scratch = nullptr; // this device's copy of the allocation-free reduction words
     size_t bc_words = 0;
+    bool setup_on_host = false;  // cicy_set_setup: build layouts and condensation on the host
+    cicy::DevPool pool;          // temporaries of the device set-up
     char err[512] = {0};
 };
 
@@ -235,7 +317,37 @@ bool solve_condensed(cicy_sys *s, int inner_method, double tol, long long max_it
 int condensed_info(const cicy_cond *c, int what);
 int condensed_set_blocks(cicy_sys *s, int64_t nblocks, const int64_t *ranges);
 
+// dea_setup.hip: the band + remainder layout from the caller's device arrays (validated on
+// the device; false with the context's error set on a malformed system), and the
+// condensation of a system whose band layout is on the device
+bool band_from_device(cicy_sys *s, const int64_t *row_ptr, const int64_t *col_ind,
+                      const double *values, const double *diag);
+cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed);
+void condensed_free_device(cicy_cond *c);
+// device memory for one set-up, returned to the context's pool when the scope ends
+struct PoolScope {
+    cicy_ctx *ctx;
+    std::vector<void *> got;
+    explicit PoolScope(cicy_ctx *c) : ctx(c) {}
+    template <class T>
+    bool get(T *&p, size_t count) {
+        p = static_cast<T *>(ctx->pool.get(std::max<size_t>(count, 1) * sizeof(T)));
+        if (!p) {
+            set_error(ctx, "hipMalloc of %zu bytes (set-up) failed", count * sizeof(T));
+            return false;
+        }
+        got.push_back(p);
+        return true;
+    }
+    ~PoolScope() {
+        for (void *q : got) ctx->pool.put(q);
+    }
+};
+
 // dea_core.hip
+cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fcol,
+                             const double *fval, const double *cdiag, const int *orig,
+                             int64_t nblocks, const int64_t *ranges);
 cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
                       const std::vector<int64_t> &ccol, const std::vector<double> &cval,
                       const std::vector<double> &cdiag, const std::vector<int> &orig,
@@ -249,6 +361,14 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
 int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
                const int64_t *col_ind, const double *values, const double *diagonal,
                cicy_sys **out);
+// the same from the reference's arrays in device memory (layout built on the device, no
+// host copy kept)
+int sys_create_device(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                      const int64_t *col_ind, const double *values, const double *diagonal,
+                      cicy_sys **out);
+bool sys_alloc_vectors(cicy_sys *s, size_t *total);
+// sum of v[i]^2 over n device entries, in a fixed order
+bool sum_squares(cicy_ctx *ctx, const double *v, size_t n, double *out);
 // builds what `method` needs (idempotent); false on a device or allocation error, or when
 // the method cannot run on this system (the reason in the context's error)
 // core = false stops after the condensation itself (the core solver's layout is left for

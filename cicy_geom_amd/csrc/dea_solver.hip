@@ -110,6 +110,25 @@ __global__ __launch_bounds__(kThreads) void k_read(size_t n16, const uint4 *__re
     }
 }
 
+// Sum of squares, per work-group in a fixed order (grid-stride rows, then a tree over the
+// work-group's lanes); the work-group sums are added on the host in work-group order.
+constexpr int kSqGroups = 256;
+__global__ __launch_bounds__(kThreads) void k_sum_squares(size_t n, const double *__restrict__ v,
+                                                          double *__restrict__ out) {
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * kThreads)
+        acc += v[i] * v[i];
+    __shared__ double red[kThreads];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
 // FP64 boundary current I = sum_e (v[ni[e]] - v[nb[e]]) / re[e]: grid-stride over the
 // elements, per-work-group sums reduced in a fixed order (reduce_sys: bit-identical from run
 // to run), the total written to out[0].  An element whose node lies outside [0, nv) adds
@@ -615,6 +634,23 @@ void sys_spmv64(cicy_sys *s, const double *xdev, double *ydev) {
     dispatch_rb<LaunchSpmv64>(s->rb, s, xdev, ydev);
 }
 
+bool sum_squares(cicy_ctx *ctx, const double *v, size_t n, double *out) {
+    double *part = nullptr;
+    double h[kSqGroups];
+    *out = 0.0;
+    if (n == 0) return true;
+    PoolScope ps(ctx);
+    if (!ps.get(part, kSqGroups)) return false;
+    hipLaunchKernelGGL(k_sum_squares, dim3(kSqGroups), dim3(kThreads), 0, ctx->stream, n, v, part);
+    if (!hip_ok(ctx, hipGetLastError(), "k_sum_squares") ||
+        !d2h_sync(ctx, h, part, sizeof(h)))
+        return false;
+    double t = 0.0;
+    for (int g = 0; g < kSqGroups; g++) t += h[g];
+    *out = t;
+    return true;
+}
+
 bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel, double *b2) {
     cicy_ctx *ctx = s->ctx;
     if (!s->has_b) {
@@ -747,6 +783,7 @@ void cicy_destroy(cicy_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->host_state) (void)hipHostFree(ctx->host_state);
     if (ctx->bc_ws) (void)hipFree(ctx->bc_ws);
+    ctx->pool.clear();
     (void)hipEventDestroy(ctx->ev_start);
     (void)hipEventDestroy(ctx->ev_stop);
     (void)hipEventDestroy(ctx->ev_poll[0]);
@@ -832,7 +869,6 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
     s->nremote = h.nremote;
     size_t tot = 0;
     const size_t nv = (size_t)h.n;
-    const size_t nparts = 2 * (size_t)std::max(h.nblocks, 1024);
     s->format = choose_format(ctx, h);
     BandHost &bh = keep->bh;
     bool ok = dmalloc(ctx, &s->diag, nv, &tot);
@@ -863,16 +899,34 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
              h2d(ctx, s->xnbr, bh.xnbr.data(), (size_t)bh.m) &&
              h2d(ctx, s->xval, bh.xval.data(), (size_t)bh.m);
     }
-    ok = ok && dmalloc(ctx, &s->x, nv, &tot) &&
+    ok = ok && h2d(ctx, s->diag, h.diag.data(), nv) && sys_alloc_vectors(s, &tot);
+    s->device_bytes = tot;
+    clk.mark("create: allocate + upload");
+    if (!ok) {
+        cicy_sys_destroy(s);
+        return 0;
+    }
+    s->res_why = s->cond_why = "not prepared";
+    *out = s;
+    return 1;
+}
+
+// The vectors and reduction words of a system whose matrix layout is in place.
+bool sys_alloc_vectors(cicy_sys *s, size_t *total) {
+    cicy_ctx *ctx = s->ctx;
+    const size_t nv = (size_t)s->n;
+    const size_t nparts = 2 * (size_t)std::max(s->nblocks, 1024);
+    size_t tot = 0;
+    bool ok = dmalloc(ctx, &s->x, nv, &tot) &&
               dmalloc(ctx, &s->r, nv, &tot) && dmalloc(ctx, &s->p0, nv, &tot) &&
               dmalloc(ctx, &s->p1, nv, &tot) && dmalloc(ctx, &s->q, nv, &tot) &&
               dmalloc(ctx, &s->x0, nv, &tot) && dmalloc(ctx, &s->r0, nv, &tot) &&
               dmalloc(ctx, &s->b, nv, &tot) && dmalloc(ctx, &s->partials, nparts, &tot) &&
               dmalloc(ctx, &s->gpart, nparts, &tot) &&
-              dmalloc(ctx, &s->gcount, std::max<size_t>((size_t)h.nblocks, 1), &tot) &&
+              dmalloc(ctx, &s->gcount, std::max<size_t>((size_t)s->nblocks, 1), &tot) &&
               dmalloc(ctx, &s->st, 1, &tot) && dmalloc(ctx, &s->scal, 4, &tot) &&
               dmalloc(ctx, &s->counter2, 1, &tot);
-    ok = ok && h2d(ctx, s->diag, h.diag.data(), nv) &&
+    ok = ok &&
          hip_ok(ctx, hipMemsetAsync(s->partials, 0, nparts * sizeof(double), ctx->stream),
                 "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->gpart, 0, nparts * sizeof(double), ctx->stream),
@@ -882,17 +936,65 @@ int sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
          hip_ok(ctx, hipMemsetAsync(s->r0, 0, nv * sizeof(double), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->counter2, 0, sizeof(unsigned), ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->gcount, 0,
-                                    std::max<size_t>((size_t)h.nblocks, 1) * sizeof(unsigned),
+                                    std::max<size_t>((size_t)s->nblocks, 1) * sizeof(unsigned),
                                     ctx->stream), "memset") &&
          hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), ctx->stream), "memset") &&
          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    *total += tot;
+    return ok;
+}
+
+// A system from the reference's arrays resident in device memory: checked and laid out
+// (band + remainder) on the device; no host copy is kept, so the methods whose layouts are
+// built on the host (mixed, resident, the blocked SSS layout) are not available for it.
+int sys_create_device(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr,
+                      const int64_t *col_ind, const double *values, const double *diagonal,
+                      cicy_sys **out) {
+    if (!ctx || !out) return 0;
+    *out = nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return 0;
+    if (n < 0 || nnz < 0) {
+        set_error(ctx, "negative size (n=%lld nnz=%lld)", (long long)n, (long long)nnz);
+        return 0;
+    }
+    if (n >= (1LL << 31) - 1 || 2 * nnz >= (1LL << 31) - 1) {
+        set_error(ctx, "system too large for 32-bit device indices (n=%lld nnz=%lld)",
+                  (long long)n, (long long)nnz);
+        return 0;
+    }
+    if (n > 0 && (!row_ptr || !diagonal || (nnz > 0 && (!col_ind || !values)))) {
+        set_error(ctx, "null array");
+        return 0;
+    }
+    if (ctx->format == kFormatSss) {
+        set_error(ctx, "the blocked SSS layout is built from host arrays (cicy_sys_create)");
+        return 0;
+    }
+    StageClock clk;
+    cicy_sys *s = new cicy_sys();
+    s->ctx = ctx;
+    s->resident_rows_req = ctx->resident_rows;
+    s->n = (int)n;
+    s->nnz = (int)nnz;
+    s->rb = ctx->rows_per_block;
+    s->nblocks = n > 0 ? (int)((n + s->rb - 1) / s->rb) : 0;
+    s->format = kFormatBand;
+    s->mixed_done = s->res_done = true;  // no host copy to build them from
+    s->res_why = "the register-resident layout is built from host arrays (cicy_sys_create)";
+    size_t tot = 0;
+    bool ok = dmalloc(ctx, &s->diag, (size_t)n, &tot);
     s->device_bytes = tot;
-    clk.mark("create: allocate + upload");
+    ok = ok && band_from_device(s, row_ptr, col_ind, values, diagonal);
+    clk.mark("create (device): layout");
+    tot = 0;
+    ok = ok && sys_alloc_vectors(s, &tot);
+    s->device_bytes += tot;
+    clk.mark("create (device): vectors");
     if (!ok) {
         cicy_sys_destroy(s);
         return 0;
     }
-    s->res_why = s->cond_why = "not prepared";
+    s->cond_why = "not prepared";
     *out = s;
     return 1;
 }
@@ -924,6 +1026,11 @@ bool sys_prepare(cicy_sys *s, int method, bool core) {
         set_error(ctx, "unknown method %d", method);
         return false;
     }
+    if ((need_mixed || need_res) && !s->host) {
+        set_error(ctx, "method %d needs the host arrays of the system (create it with "
+                       "cicy_sys_create)", method);
+        return false;
+    }
     if (need_mixed && !s->mixed_done) {
         s->mixed_done = true;
         // on failure the buffers allocated so far are freed with the system
@@ -941,7 +1048,11 @@ bool sys_prepare(cicy_sys *s, int method, bool core) {
     if (need_cond && !s->cond_done) {
         s->cond_done = true;
         bool failed = false;
-        s->cond = condensed_setup(ctx, s->host->h, host_band(*s->host), &s->cond_why, &failed);
+        if (s->format == kFormatBand && (!ctx->setup_on_host || !s->host))
+            s->cond = condensed_setup_device(s, &s->cond_why, &failed);
+        else
+            s->cond = condensed_setup(ctx, s->host->h, host_band(*s->host), &s->cond_why,
+                                      &failed);
         if (failed) return false;
     }
     release_host(s);
@@ -966,6 +1077,72 @@ int cicy_sys_create(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_pt
                     const int64_t *col_ind, const double *values, const double *diagonal,
                     cicy_sys **out) {
     return cicy::sys_create(ctx, n, nnz, row_ptr, col_ind, values, diagonal, out);
+}
+
+int cicy_sys_create_device(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *d_row_ptr,
+                           const int64_t *d_col_ind, const double *d_values,
+                           const double *d_diagonal, cicy_sys **out) {
+    return cicy::sys_create_device(ctx, n, nnz, d_row_ptr, d_col_ind, d_values, d_diagonal, out);
+}
+
+int cicy_sys_set_rhs_device(cicy_sys *s, const double *d_r0, const double *d_x0,
+                            const double *d_b) {
+    if (!s || !d_r0) return 0;
+    cicy_ctx *ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    const size_t nv = (size_t)s->n;
+    hipStream_t st = ctx->stream;
+    bool ok = hip_ok(ctx, hipMemcpyAsync(s->r0, d_r0, nv * sizeof(double),
+                                         hipMemcpyDeviceToDevice, st), "copy r0");
+    ok = ok && (d_x0 ? hip_ok(ctx, hipMemcpyAsync(s->x0, d_x0, nv * sizeof(double),
+                                                  hipMemcpyDeviceToDevice, st), "copy x0")
+                     : hip_ok(ctx, hipMemsetAsync(s->x0, 0, nv * sizeof(double), st), "memset"));
+    if (d_b)
+        ok = ok && hip_ok(ctx, hipMemcpyAsync(s->b, d_b, nv * sizeof(double),
+                                              hipMemcpyDeviceToDevice, st), "copy b");
+    double r02 = 0.0, b2 = 0.0;
+    ok = ok && cicy::sum_squares(ctx, s->r0, nv, &r02) &&
+         (!d_b || cicy::sum_squares(ctx, s->b, nv, &b2));
+    if (!ok) return 0;
+    s->r0norm2 = r02;
+    s->has_b = d_b != nullptr;
+    if (d_b) s->bnorm2 = b2;
+    return 1;
+}
+
+int cicy_set_setup(cicy_ctx *ctx, int where) {
+    if (!ctx || where < 0 || where > 1) return 0;
+    ctx->setup_on_host = where == 1;
+    return 1;
+}
+
+int cicy_device_alloc(cicy_ctx *ctx, int64_t bytes, void **out) {
+    if (!ctx || !out || bytes < 0) return 0;
+    (void)hipSetDevice(ctx->device);
+    *out = nullptr;
+    return hip_ok(ctx, hipMalloc(out, std::max<int64_t>(bytes, 1)), "hipMalloc") ? 1 : 0;
+}
+
+int cicy_device_free(cicy_ctx *ctx, void *p) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    return p == nullptr || hip_ok(ctx, hipFree(p), "hipFree") ? 1 : 0;
+}
+
+int cicy_device_upload(cicy_ctx *ctx, void *dst, const void *src, int64_t bytes) {
+    if (!ctx || bytes < 0 || (bytes > 0 && (!dst || !src))) return 0;
+    (void)hipSetDevice(ctx->device);
+    return bytes == 0 || (hip_ok(ctx, hipMemcpyAsync(dst, src, (size_t)bytes,
+                                                     hipMemcpyHostToDevice, ctx->stream), "H2D") &&
+                          hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync"))
+               ? 1 : 0;
+}
+
+int cicy_device_download(cicy_ctx *ctx, void *dst, const void *src, int64_t bytes) {
+    if (!ctx || bytes < 0 || (bytes > 0 && (!dst || !src))) return 0;
+    (void)hipSetDevice(ctx->device);
+    return bytes == 0 || d2h_sync(ctx, dst, src, (size_t)bytes) ? 1 : 0;
 }
 
 int cicy_sys_prepare(cicy_sys *s, int method) {

@@ -46,6 +46,8 @@ EXPORTED_SYMBOLS = [
     "cicy_sys_prepare",
     "cicy_set_dropin_blocks", "cicy_backbone_scan", "cicy_cnt_contacts",
     "cicy_boundary_current_f64_noalloc", "cicy_sys_spmv_bench_cold", "cicy_read_bench",
+    "cicy_sys_create_device", "cicy_sys_set_rhs_device", "cicy_set_setup", "cicy_device_alloc",
+    "cicy_device_free", "cicy_device_upload", "cicy_device_download",
 ]
 
 
@@ -112,6 +114,14 @@ def lib():
                                                     ctypes.c_int64, _VP]
     L.cicy_sys_create.argtypes = [_VP, ctypes.c_int64, ctypes.c_int64, _I64, _I64, _F64, _F64,
                                   P(_VP)]
+    L.cicy_sys_create_device.argtypes = [_VP, ctypes.c_int64, ctypes.c_int64, _VP, _VP, _VP,
+                                         _VP, P(_VP)]
+    L.cicy_sys_set_rhs_device.argtypes = [_VP, _VP, _VP, _VP]
+    L.cicy_set_setup.argtypes = [_VP, ctypes.c_int]
+    L.cicy_device_alloc.argtypes = [_VP, ctypes.c_int64, P(_VP)]
+    L.cicy_device_free.argtypes = [_VP, _VP]
+    L.cicy_device_upload.argtypes = [_VP, _VP, _VP, ctypes.c_int64]
+    L.cicy_device_download.argtypes = [_VP, _VP, _VP, ctypes.c_int64]
     L.cicy_sys_destroy.argtypes = [_VP]
     L.cicy_sys_size.restype = ctypes.c_int64
     L.cicy_sys_size.argtypes = [_VP, ctypes.c_int]
@@ -404,21 +414,88 @@ class Context:
     def system(self, row_ptr, col_ind, values, diagonal) -> "System":
         return System(self, row_ptr, col_ind, values, diagonal)
 
+    def set_setup(self, where: str):
+        """Where systems created afterwards are set up: "device" (default) or "host" (the
+        host forms, bit-identical; for checking one against the other)."""
+        self._check(self._L.cicy_set_setup(self._h, {"device": 0, "host": 1}[where]),
+                    "cicy_set_setup")
+
+    def upload(self, a) -> "DeviceArray":
+        """A copy of the numpy array `a` in device memory."""
+        return DeviceArray(self, a)
+
+    def system_device(self, n: int, nnz: int, d_row_ptr, d_col_ind, d_values,
+                      d_diagonal) -> "System":
+        """A system from the reference's SSS arrays already in device memory (DeviceArray
+        or int addresses; int64 indices): checked and set up on the device."""
+        return System(self, None, None, None, None,
+                      device=(n, nnz, d_row_ptr, d_col_ind, d_values, d_diagonal))
+
     def batch(self, systems) -> "Batch":
         """systems: sequence of (row_ptr, col_ind, values, diagonal)."""
         return Batch(self, systems)
 
 
+def _addr(a):
+    """Device address of a DeviceArray, an int address, or None."""
+    if a is None:
+        return None
+    return a.ptr if isinstance(a, DeviceArray) else int(a)
+
+
+class DeviceArray:
+    """A numpy array copied into device memory of a context (cicy_device_alloc)."""
+
+    def __init__(self, ctx: Context, a):
+        a = np.ascontiguousarray(a)
+        self.ctx = ctx
+        self.dtype, self.shape, self.nbytes = a.dtype, a.shape, a.nbytes
+        p = _VP()
+        ctx._check(ctx._L.cicy_device_alloc(ctx._h, a.nbytes, ctypes.byref(p)),
+                   "cicy_device_alloc")
+        self._p = p
+        ctx._adopt(self)
+        ctx._check(ctx._L.cicy_device_upload(ctx._h, p, a.ctypes.data_as(_VP), a.nbytes),
+                   "cicy_device_upload")
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value
+
+    def get(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(self.ctx._L.cicy_device_download(self.ctx._h, out.ctypes.data_as(_VP),
+                                                         self._p, self.nbytes),
+                        "cicy_device_download")
+        return out
+
+    def close(self):
+        if getattr(self, "_p", None):
+            self.ctx._L.cicy_device_free(self.ctx._h, self._p)
+            self._p = None
+
+    def __del__(self):
+        if not _finalizing():
+            self.close()
+
+
 class System:
     """A device-resident SSS system (cicy_sys)."""
 
-    def __init__(self, ctx: Context, row_ptr, col_ind, values, diagonal):
+    def __init__(self, ctx: Context, row_ptr, col_ind, values, diagonal, device=None):
         self.ctx = ctx
         h = _VP()
-        self.n = len(diagonal)
-        ctx._check(ctx._L.cicy_sys_create(ctx._h, self.n, len(values), _i64(row_ptr),
-                                          _i64(col_ind), _f64(values), _f64(diagonal),
-                                          ctypes.byref(h)), "cicy_sys_create")
+        if device is not None:
+            n, nnz, rp, ci, va, dg = device
+            self.n = int(n)
+            ctx._check(ctx._L.cicy_sys_create_device(ctx._h, self.n, int(nnz), _addr(rp),
+                                                     _addr(ci), _addr(va), _addr(dg),
+                                                     ctypes.byref(h)), "cicy_sys_create_device")
+        else:
+            self.n = len(diagonal)
+            ctx._check(ctx._L.cicy_sys_create(ctx._h, self.n, len(values), _i64(row_ptr),
+                                              _i64(col_ind), _f64(values), _f64(diagonal),
+                                              ctypes.byref(h)), "cicy_sys_create")
         self._h = h
         ctx._adopt(self)
 
@@ -458,6 +535,12 @@ class System:
         self.ctx._check(self.ctx._L.cicy_sys_set_rhs(self._h, r0, _ptr(x0), _ptr(b)),
                         "cicy_sys_set_rhs")
         self._keep = (r0, x0, b)
+
+    def set_rhs_device(self, d_r0, d_x0=None, d_b=None):
+        """set_rhs from device arrays (DeviceArray or int addresses)."""
+        self.ctx._check(self.ctx._L.cicy_sys_set_rhs_device(self._h, _addr(d_r0), _addr(d_x0),
+                                                            _addr(d_b)),
+                        "cicy_sys_set_rhs_device")
 
     def solve(self, method=METHOD_PCG_F64, tol=1e-10, max_iter=0, rel_to=REL_R0) -> SolveStats:
         st = SolveStats()

@@ -233,6 +233,33 @@ int64_t cicy_sys_size(const cicy_sys *sys, int what); /* 0 n, 1 nnz, 2 remote en
 /* r0 = b - A x0 (the reference's R), x0 (may be NULL = zeros), b (may be NULL; needed for
  * the true residual and for CICY_REL_B). */
 int cicy_sys_set_rhs(cicy_sys *sys, const double *r0, const double *x0, const double *b);
+
+/* The same system from the reference's arrays already in device memory (the arrays
+ * From_2d_to_1d_vectors builds, Direct_Electrifying.cpp:1329-1394, with int64 indices as
+ * its std::vector<long>): they are checked and laid out (band + remainder) on the device,
+ * and the condensation, series rounds and core layout of the condensed methods are built on
+ * the device too, so a new system costs no host work proportional to its size.  The
+ * caller's arrays are only read during the call.  Malformed arrays give 0 with the first
+ * bad row in cicy_last_error, as cicy_sys_create.  No host copy is kept: the methods whose
+ * layouts are built on the host (CICY_METHOD_MIXED, CICY_METHOD_RESIDENT, the blocked SSS
+ * format) are refused for such a system.  No counterpart in the reference. */
+int cicy_sys_create_device(cicy_ctx *ctx, int64_t n, int64_t nnz, const int64_t *d_row_ptr,
+                           const int64_t *d_col_ind, const double *d_values,
+                           const double *d_diagonal, cicy_sys **out);
+/* cicy_sys_set_rhs with device arrays (d_x0, d_b may be NULL); the reference norms are
+ * summed on the device.  No counterpart in the reference. */
+int cicy_sys_set_rhs_device(cicy_sys *sys, const double *d_r0, const double *d_x0,
+                            const double *d_b);
+/* Where the set-up of systems created afterwards runs: 0 on the device (default; the
+ * condensation of systems in the band layout), 1 on the host (the host forms, bit-identical;
+ * for checking one against the other).  No counterpart in the reference. */
+int cicy_set_setup(cicy_ctx *ctx, int where);
+/* Plain device memory for callers that hold no device allocator of their own (tests,
+ * benchmarks): allocate, copy in and out, free.  No counterpart in the reference. */
+int cicy_device_alloc(cicy_ctx *ctx, int64_t bytes, void **out);
+int cicy_device_free(cicy_ctx *ctx, void *p);
+int cicy_device_upload(cicy_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int cicy_device_download(cicy_ctx *ctx, void *dst, const void *src, int64_t bytes);
 /* Solve on device-resident data.  max_iter <= 0: 300*(n+2). */
 int cicy_sys_solve(cicy_sys *sys, int method, double tol, int64_t max_iter, int rel_to,
                    cicy_solve_stats *stats);
