@@ -193,7 +193,8 @@ __global__ __launch_bounds__(kThreads) void k_correct(int n, double *__restrict_
 
 template <typename T>
 bool calloc_dev(cicy_ctx *ctx, T **p, size_t count) {
-    if (hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+    *p = static_cast<T *>(ctx->pool.get(std::max<size_t>(count, 1) * sizeof(T)));
+    if (!*p) {
         set_error(ctx, "hipMalloc of %zu bytes failed (condensed system)", count * sizeof(T));
         return false;
     }
@@ -224,8 +225,7 @@ void condensed_free(cicy_cond *c) {
                     c->gcount, c->e_node, c->e_off, c->e_nbr, c->e_a, c->e_d,
                     c->core_nodes, c->fS, c->xS, c->rv_node, c->rv_off, c->rv_rec, c->rv_a,
                     c->core_b, c->core_x};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    for (void *p : ptrs) dev_release(c->ctx, p);
     condensed_free_device(c);
     delete c;
 }
@@ -790,6 +790,7 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
                            &anchors = C.anchors;
     const std::vector<double> &lfac = C.lfac, &mfac = C.mfac, &band = C.band, &rval = C.rval;
     cicy_cond *c = new cicy_cond();
+    c->ctx = ctx;
     c->n = n;
     c->m = m;
     c->npath = (int)ps.size();

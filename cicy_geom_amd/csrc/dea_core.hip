@@ -125,6 +125,7 @@ struct CoreDev {
 }  // namespace cicy
 
 struct cicy_core {
+    cicy_ctx *ctx = nullptr;  // its arrays come from the context's pool
     int m = 0, G = 0, nb = 0, blocked_rows = 0, lds_cols = 0;
     long long nnz2 = 0, inv_entries = 0;
     int *ju = nullptr, *bu = nullptr, *wr = nullptr, *perm = nullptr;
@@ -480,8 +481,14 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
             for (size_t b = t; b < kept_g.size(); b += nbt) {
                 const size_t g = kept_g[b];
                 const double nb = groups[g].second - groups[g].first;
+                // the sum of the row sums (each in column order)
                 double c = 0.0;
-                for (long long e = off[g]; e < off[g + 1]; e++) c += inv[(size_t)e];
+                const int n_ = groups[g].second - groups[g].first;
+                for (int i = 0; i < n_; i++) {
+                    double t = 0.0;
+                    for (int j = 0; j < n_; j++) t += inv[(size_t)off[g] + (size_t)i * n_ + j];
+                    c += t;
+                }
                 c /= nb * nb;
                 H.blk_c[b] = c;
                 long long o = H.blk_off[b];
@@ -847,10 +854,16 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
     }
 }
 
+bool core_alloc(cicy_ctx *ctx, void **dst, size_t bytes) {
+    *dst = ctx->pool.get(bytes);
+    if (!*dst) set_error(ctx, "hipMalloc of %zu bytes (core) failed", bytes);
+    return *dst != nullptr;
+}
+
 template <typename T>
 bool up_vec(cicy_ctx *ctx, cicy_core *z, T **dst, const std::vector<T> &src) {
     const size_t bytes = std::max<size_t>(1, src.size()) * sizeof(T);
-    if (!hip_ok(ctx, hipMalloc((void **)dst, bytes), "hipMalloc (core)")) return false;
+    if (!core_alloc(ctx, (void **)dst, bytes)) return false;
     z->bytes += bytes;
     if (src.empty()) return true;
     return hip_ok(ctx, hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T),
@@ -860,7 +873,7 @@ bool up_vec(cicy_ctx *ctx, cicy_core *z, T **dst, const std::vector<T> &src) {
 
 bool zeros(cicy_ctx *ctx, cicy_core *z, double **dst, size_t count) {
     const size_t bytes = std::max<size_t>(1, count) * sizeof(double);
-    if (!hip_ok(ctx, hipMalloc((void **)dst, bytes), "hipMalloc (core)")) return false;
+    if (!core_alloc(ctx, (void **)dst, bytes)) return false;
     z->bytes += bytes;
     return hip_ok(ctx, hipMemsetAsync(*dst, 0, bytes, ctx->stream), "memset (core)");
 }
@@ -869,19 +882,10 @@ bool zeros(cicy_ctx *ctx, cicy_core *z, double **dst, size_t count) {
 
 void core_free(cicy_core *z) {
     if (!z) return;
-    int *ip[] = {z->ju, z->bu, z->wr, z->perm, z->blk_row0, z->blk_n, z->wsl, z->sl_len,
-                 z->ell_col};
-    for (int *q : ip)
-        if (q) (void)hipFree(q);
-    double *dp[] = {z->ell_val, z->diag, z->blk_c, z->x, z->r, z->u, z->w, z->p, z->s};
-    for (double *q : dp)
-        if (q) (void)hipFree(q);
-    if (z->binv) (void)hipFree(z->binv);
-    if (z->part) (void)hipFree(z->part);
-    if (z->blk_off) (void)hipFree(z->blk_off);
-    if (z->sl_off) (void)hipFree(z->sl_off);
-    if (z->ctl) (void)hipFree(z->ctl);
-    if (z->st) (void)hipFree(z->st);
+    void *ptrs[] = {z->ju, z->bu, z->wr, z->perm, z->blk_row0, z->blk_n, z->wsl, z->sl_len,
+                    z->ell_col, z->ell_val, z->diag, z->blk_c, z->x, z->r, z->u, z->w, z->p,
+                    z->s, z->binv, z->part, z->blk_off, z->sl_off, z->ctl, z->st};
+    for (void *q : ptrs) dev_release(z->ctx, q);
     delete z;
 }
 
@@ -939,6 +943,7 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
     }
     clk.mark("core: layout + block inverses");
     cicy_core *z = new cicy_core();
+    z->ctx = ctx;
     z->m = m;
     z->G = H.G;
     z->lds_cols = lds_cols;
@@ -957,11 +962,10 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
               zeros(ctx, z, &z->x, m) &&
               zeros(ctx, z, &z->r, m) && zeros(ctx, z, &z->u, m) && zeros(ctx, z, &z->w, m) &&
               zeros(ctx, z, &z->p, m) && zeros(ctx, z, &z->s, m) &&
-              hip_ok(ctx, hipMalloc((void **)&z->part, 3 * kCoreSums * sizeof(unsigned long long) *
-                                                       (size_t)H.G),
-                     "hipMalloc (core)") &&
-              hip_ok(ctx, hipMalloc((void **)&z->ctl, sizeof(GridCtl)), "hipMalloc (core)") &&
-              hip_ok(ctx, hipMalloc((void **)&z->st, sizeof(CoreState)), "hipMalloc (core)") &&
+              core_alloc(ctx, (void **)&z->part,
+                         3 * kCoreSums * sizeof(unsigned long long) * (size_t)H.G) &&
+              core_alloc(ctx, (void **)&z->ctl, sizeof(GridCtl)) &&
+              core_alloc(ctx, (void **)&z->st, sizeof(CoreState)) &&
               hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
     clk.mark("core: upload");
     if (!ok) {
@@ -1003,11 +1007,27 @@ __global__ __launch_bounds__(kLT) void k_group_bounds(int nv, const int64_t *__r
     }
 }
 
-// Dense inverse of one diagonal block per work-group, with dense_spd_inverse's operations in
-// its order: Cholesky column by column, W = L^-1 row by row, A^-1 = W^T W, then the mean
-// entry c (summed in row-major order).  LDS holds the block: L in the lower triangle, W in
-// the strict upper one (W[i][j] at [j][i]) and W's diagonal after it.
-__global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ gq0,
+// (i, j), 0 <= j <= i, of the e-th entry of a lower triangle stored by rows
+__device__ __forceinline__ void tri_index(int e, int *i, int *j) {
+    int a = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while ((a + 1) * (a + 2) / 2 <= e) a++;
+    while (a * (a + 1) / 2 > e) a--;
+    *i = a;
+    *j = e - a * (a + 1) / 2;
+}
+
+// Dense inverse of one diagonal block per work-group (work-group b takes group order[b],
+// the largest first).  Every entry gets dense_spd_inverse's operations in its order, run
+// right-looking so that whole steps go in parallel: Cholesky (at step k the column of L,
+// then the trailing update a_ij -= l_ik l_jk: each entry sees k = 0, 1, ... as in the host's
+// inner loop), W = L^-1 by forward substitution (acc_ij -= l_ik w_kj at step k, k from j up,
+// then times 1 / l_ii), A^-1 = W^T W (each entry summed over k ascending by one thread),
+// then c = the sum of the row sums (each in column order) over the block's n^2 entries.
+// LDS: the block with an odd row stride (ld = n | 1, no bank conflicts on columns): L in
+// the lower triangle, W in the strict upper one (W[i][j] at [j][i]) and W's diagonal after
+// the block; the inverse's lower triangle then replaces L.
+__global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ order,
+                                                       const int *__restrict__ gq0,
                                                        const int *__restrict__ gnb,
                                                        const long long *__restrict__ goff,
                                                        const int *__restrict__ frp,
@@ -1019,34 +1039,40 @@ __global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ g
                                                        double *__restrict__ cg) {
     extern __shared__ double A[];
     __shared__ int fail;
-    const int g = blockIdx.x, q0 = gq0[g], nb = gnb[g], tid = threadIdx.x;
-    double *wd = A + (size_t)nb * nb;
+    const int g = order[blockIdx.x], q0 = gq0[g], nb = gnb[g], tid = threadIdx.x;
+    const int ld = nb | 1;
+    double *wd = A + (size_t)nb * ld;
+    double *rs = wd + nb;
     double *out = inv + goff[g];
-    for (int e = tid; e < nb * nb; e += kLT) A[e] = 0.0;
+    for (int e = tid; e < nb * ld; e += kLT) A[e] = 0.0;
     if (tid == 0) fail = 0;
     __syncthreads();
     for (int i = tid; i < nb; i += kLT) {
-        A[i * nb + i] = cdiag[q0 + i];
+        A[i * ld + i] = cdiag[q0 + i];
         for (int k = frp[q0 + i]; k < frp[q0 + i + 1]; k++) {
             const int c = fcol[k] - q0;
-            if (c >= 0 && c < nb) A[i * nb + c] = fval[k];
+            if (c >= 0 && c < nb) A[i * ld + c] = fval[k];
         }
     }
     __syncthreads();
-    for (int j = 0; j < nb; j++) {
+    // Cholesky
+    for (int k = 0; k < nb; k++) {
         if (tid == 0) {
-            double d = A[j * nb + j];
-            for (int k = 0; k < j; k++) d = d - A[j * nb + k] * A[j * nb + k];
+            const double d = A[k * ld + k];
             if (!(d > 0.0) || !isfinite(d)) fail = 1;
-            else A[j * nb + j] = sqrt(d);
+            else A[k * ld + k] = sqrt(d);
         }
         __syncthreads();
         if (fail) break;
-        const double l = A[j * nb + j];
-        for (int i = j + 1 + tid; i < nb; i += kLT) {
-            double v = A[i * nb + j];
-            for (int k = 0; k < j; k++) v = v - A[i * nb + k] * A[j * nb + k];
-            A[i * nb + j] = v / l;
+        const double l = A[k * ld + k];
+        for (int i = k + 1 + tid; i < nb; i += kLT) A[i * ld + k] = A[i * ld + k] / l;
+        __syncthreads();
+        const int r = nb - k - 1, cnt = r * (r + 1) / 2;
+        for (int e = tid; e < cnt; e += kLT) {
+            int a, c;
+            tri_index(e, &a, &c);
+            const int i = k + 1 + a, j = k + 1 + c;
+            A[i * ld + j] = A[i * ld + j] - A[i * ld + k] * A[j * ld + k];
         }
         __syncthreads();
     }
@@ -1057,34 +1083,50 @@ __global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ g
         }
         return;
     }
-    for (int i = 0; i < nb; i++) {
-        const double inv_d = 1.0 / A[i * nb + i];
-        for (int j = tid; j < i; j += kLT) {
-            double acc = 0.0;
-            for (int k = j; k < i; k++) {
-                const double wkj = k == j ? wd[j] : A[j * nb + k];
-                acc = acc - A[i * nb + k] * wkj;
-            }
-            A[j * nb + i] = acc * inv_d;
+    // W = L^-1: the strict upper triangle holds the accumulators, then W
+    for (int e = tid; e < nb * ld; e += kLT) {
+        const int i = e / ld, j = e % ld;
+        if (j > i && j < nb) A[e] = 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < nb; k++) {
+        const double inv_d = 1.0 / A[k * ld + k];
+        for (int j = tid; j < k; j += kLT) A[j * ld + k] = A[j * ld + k] * inv_d;
+        if (tid == 0) wd[k] = inv_d;
+        __syncthreads();
+        const int rows = nb - k - 1, cols = k + 1;
+        for (int e = tid; e < rows * cols; e += kLT) {
+            const int i = k + 1 + e / cols, j = e % cols;
+            const double wkj = j == k ? wd[k] : A[j * ld + k];
+            A[j * ld + i] = A[j * ld + i] - A[i * ld + k] * wkj;
         }
-        if (tid == 0) wd[i] = inv_d;
         __syncthreads();
     }
-    for (int i = 0; i < nb; i++)
-        for (int j = tid; j <= i; j += kLT) {
-            double acc = 0.0;
-            for (int k = i; k < nb; k++) {
-                const double wki = k == i ? wd[i] : A[i * nb + k];
-                const double wkj = k == j ? wd[j] : A[j * nb + k];
-                acc = acc + wki * wkj;
-            }
-            out[(size_t)i * nb + j] = acc;
-            out[(size_t)j * nb + i] = acc;
+    // A^-1 = W^T W into the lower triangle (L is no longer needed) and to global memory
+    const int ntri = nb * (nb + 1) / 2;
+    for (int e = tid; e < ntri; e += kLT) {
+        int i, j;
+        tri_index(e, &i, &j);
+        double acc = 0.0;
+        for (int k = i; k < nb; k++) {
+            const double wki = k == i ? wd[i] : A[i * ld + k];
+            const double wkj = k == j ? wd[j] : A[j * ld + k];
+            acc = acc + wki * wkj;
         }
+        A[i * ld + j] = acc;
+        out[(size_t)i * nb + j] = acc;
+        out[(size_t)j * nb + i] = acc;
+    }
+    __syncthreads();
+    for (int i = tid; i < nb; i += kLT) {
+        double t = 0.0;
+        for (int j = 0; j < nb; j++) t = t + (j <= i ? A[i * ld + j] : A[j * ld + i]);
+        rs[i] = t;
+    }
     __syncthreads();
     if (tid == 0) {
         double c = 0.0;
-        for (int e = 0; e < nb * nb; e++) c = c + out[e];
+        for (int i = 0; i < nb; i++) c = c + rs[i];
         const double dn = nb;
         cg[g] = c / (dn * dn);
         ok[g] = 1;
@@ -1200,7 +1242,7 @@ __global__ __launch_bounds__(kLT) void k_fill_ell(int nsl, const int *__restrict
 template <typename T>
 bool core_dev(cicy_ctx *ctx, cicy_core *z, T **dst, size_t count) {
     const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
-    if (!hip_ok(ctx, hipMalloc((void **)dst, bytes), "hipMalloc (core)")) return false;
+    if (!core_alloc(ctx, (void **)dst, bytes)) return false;
     z->bytes += bytes;
     return true;
 }
@@ -1308,19 +1350,25 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
         goff[(size_t)g + 1] = goff[(size_t)g] + (long long)gnb[(size_t)g] * gnb[(size_t)g];
     }
     // dense inverses
-    int *dgq0, *dgnb, *dok;
+    int *dgq0, *dgnb, *dok, *dord;
     long long *dgoff;
     double *dinv, *dcg;
-    std::vector<int> okh((size_t)ng, 0);
+    std::vector<int> okh((size_t)ng, 0), gord((size_t)ng);
+    for (int g = 0; g < ng; g++) gord[(size_t)g] = g;
+    std::stable_sort(gord.begin(), gord.end(),
+                     [&](int a, int b) { return gnb[(size_t)a] > gnb[(size_t)b]; });
     if (!ps.get(dgq0, (size_t)ng) || !ps.get(dgnb, (size_t)ng) || !ps.get(dok, (size_t)ng) ||
+        !ps.get(dord, (size_t)ng) ||
         !ps.get(dgoff, (size_t)ng + 1) || !ps.get(dinv, (size_t)goff[(size_t)ng]) ||
         !ps.get(dcg, (size_t)ng))
         return nullptr;
     if (ng > 0) {
-        const size_t lds = ((size_t)nbmax * nbmax + nbmax) * sizeof(double);
+        const size_t lds = ((size_t)nbmax * (nbmax | 1) + 2 * (size_t)nbmax) * sizeof(double);
         if (!hip_ok(ctx, hipMemcpyAsync(dgq0, gq0.data(), (size_t)ng * sizeof(int),
                                         hipMemcpyHostToDevice, st), "H2D") ||
             !hip_ok(ctx, hipMemcpyAsync(dgnb, gnb.data(), (size_t)ng * sizeof(int),
+                                        hipMemcpyHostToDevice, st), "H2D") ||
+            !hip_ok(ctx, hipMemcpyAsync(dord, gord.data(), (size_t)ng * sizeof(int),
                                         hipMemcpyHostToDevice, st), "H2D") ||
             !hip_ok(ctx, hipMemcpyAsync(dgoff, goff.data(), ((size_t)ng + 1) * sizeof(long long),
                                         hipMemcpyHostToDevice, st), "H2D") ||
@@ -1328,8 +1376,8 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)lds), "hipFuncSetAttribute"))
             return nullptr;
-        hipLaunchKernelGGL(k_block_inverse, dim3(ng), dim3(kLT), lds, st, dgq0, dgnb, dgoff, frp,
-                           fcol, fval, cdiag, dinv, dok, dcg);
+        hipLaunchKernelGGL(k_block_inverse, dim3(ng), dim3(kLT), lds, st, dord, dgq0, dgnb, dgoff,
+                           frp, fcol, fval, cdiag, dinv, dok, dcg);
         if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
             !d2h_sync(ctx, okh.data(), dok, (size_t)ng * sizeof(int)))
             return nullptr;
@@ -1365,6 +1413,7 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
     }
     const int nbk = (int)kq0.size();
     cicy_core *z = new cicy_core();
+    z->ctx = ctx;
     z->m = m;
     z->nb = nbk;
     z->blocked_rows = blocked;
@@ -1456,12 +1505,10 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
     const bool ok = hip_ok(ctx, hipGetLastError(), "k_fill_ell") &&
                     zeros(ctx, z, &z->x, m) && zeros(ctx, z, &z->r, m) && zeros(ctx, z, &z->u, m) &&
                     zeros(ctx, z, &z->w, m) && zeros(ctx, z, &z->p, m) && zeros(ctx, z, &z->s, m) &&
-                    hip_ok(ctx, hipMalloc((void **)&z->part, 3 * kCoreSums *
-                                                               sizeof(unsigned long long) *
-                                                               (size_t)H.G),
-                           "hipMalloc (core)") &&
-                    hip_ok(ctx, hipMalloc((void **)&z->ctl, sizeof(GridCtl)), "hipMalloc (core)") &&
-                    hip_ok(ctx, hipMalloc((void **)&z->st, sizeof(CoreState)), "hipMalloc (core)") &&
+                    core_alloc(ctx, (void **)&z->part,
+                               3 * kCoreSums * sizeof(unsigned long long) * (size_t)H.G) &&
+                    core_alloc(ctx, (void **)&z->ctl, sizeof(GridCtl)) &&
+                    core_alloc(ctx, (void **)&z->st, sizeof(CoreState)) &&
                     hip_ok(ctx, hipStreamSynchronize(st), "sync");
     clk.mark("device core: slices + upload");
     if (!ok) return fail();

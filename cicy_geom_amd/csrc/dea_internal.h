@@ -129,7 +129,7 @@ struct DevPool {
     };
     std::vector<Blk> blocks;
     void *get(size_t bytes);  // nullptr when hipMalloc fails
-    void put(void *p);
+    bool put(void *p);        // false when p is not a block of this pool
     void clear();             // frees every block
 };
 
@@ -156,6 +156,7 @@ __host__ __device__ inline unsigned long long elim_priority(int k) {
 // The condensed system (dea_condensed.hip), set up on the device (dea_setup.hip) or, for
 // the blocked SSS layout, on the host.
 struct cicy_cond {
+    cicy_ctx *ctx = nullptr;    // its arrays come from the context's pool
     int n = 0, m = 0, npath = 0;
     long long nnz_red = 0;
     int *ps = nullptr, *pe = nullptr, *pl = nullptr, *pr = nullptr;  // [npath]
@@ -316,6 +317,11 @@ bool solve_condensed(cicy_sys *s, int inner_method, double tol, long long max_it
                      cicy_solve_stats *stats);
 int condensed_info(const cicy_cond *c, int what);
 int condensed_set_blocks(cicy_sys *s, int64_t nblocks, const int64_t *ranges);
+
+// Returns a device block to the context's pool (or frees it when it is not one of the
+// pool's blocks).  A system's arrays, its condensation's and its core's come from the pool,
+// so a new system of the size of an earlier one allocates nothing.
+void dev_release(cicy_ctx *ctx, void *p);
 
 // dea_setup.hip: the band + remainder layout from the caller's device arrays (validated on
 // the device; false with the context's error set on a malformed system), and the

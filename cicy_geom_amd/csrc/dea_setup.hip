@@ -41,9 +41,11 @@ namespace cicy {
 
 void *DevPool::get(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
+    // best fit among the free blocks that are not much larger than the request
+    const size_t most = 2 * bytes + (64 << 10);
     int best = -1;
     for (size_t i = 0; i < blocks.size(); i++)
-        if (!blocks[i].used && blocks[i].bytes >= bytes &&
+        if (!blocks[i].used && blocks[i].bytes >= bytes && blocks[i].bytes <= most &&
             (best < 0 || blocks[i].bytes < blocks[(size_t)best].bytes))
             best = (int)i;
     if (best >= 0) {
@@ -59,9 +61,18 @@ void *DevPool::get(size_t bytes) {
     return p;
 }
 
-void DevPool::put(void *p) {
+bool DevPool::put(void *p) {
     for (auto &b : blocks)
-        if (b.p == p) b.used = false;
+        if (b.p == p) {
+            b.used = false;
+            return true;
+        }
+    return false;
+}
+
+void dev_release(cicy_ctx *ctx, void *p) {
+    if (!p) return;
+    if (!ctx || !ctx->pool.put(p)) (void)hipFree(p);
 }
 
 void DevPool::clear() {
@@ -75,6 +86,7 @@ constexpr int kST = 256;  // threads per work-group of the set-up kernels
 constexpr int kFloatingPathDev = -2;  // pl of a floating path (the host form's kFloatingPath)
 constexpr int kUndecided = 1, kIn = 2, kOut = 3;  // candidate states of a round
 constexpr int kMaxMisPasses = 4096;
+constexpr int kMisBatch = 3;  // independent-set passes launched per read-back
 
 inline int grid_of(long long n) {
     return (int)std::max<long long>(1, std::min<long long>((n + kST - 1) / kST, 1 << 16));
@@ -715,6 +727,14 @@ __global__ __launch_bounds__(kST) void k_set_int(int *__restrict__ p, int v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
 
+// the entries at `at` of up to eight arrays (scan totals), gathered for one read-back
+struct LastArgs {
+    const int *p[8];
+};
+__global__ __launch_bounds__(64) void k_last(LastArgs a, int k, int at, int *__restrict__ out) {
+    if ((int)threadIdx.x < k) out[threadIdx.x] = a.p[threadIdx.x][at];
+}
+
 // the core: the nodes left, in S order, with their rows renumbered (both triangles)
 __global__ __launch_bounds__(kST) void k_core_len(int m, const int *__restrict__ alive,
                                                   const int *__restrict__ arp,
@@ -798,10 +818,22 @@ bool last_int(cicy_ctx *ctx, const int *p, long long at, int *out) {
     return d2h_sync(ctx, out, p + at, sizeof(int));
 }
 
+// the entries at `at` of several device int arrays, with one wait; `buf` holds 8 ints
+bool last_ints(cicy_ctx *ctx, int *buf, std::initializer_list<const int *> arrs, int at,
+               int *out) {
+    LastArgs a{};
+    int k = 0;
+    for (const int *p : arrs) a.p[k++] = p;
+    hipLaunchKernelGGL(k_last, dim3(1), dim3(64), 0, ctx->stream, a, k, at, buf);
+    return hip_ok(ctx, hipGetLastError(), "k_last") &&
+           d2h_sync(ctx, out, buf, (size_t)k * sizeof(int));
+}
+
 template <typename T>
 bool dev_alloc(cicy_ctx *ctx, T **p, size_t count) {
-    *p = nullptr;
-    return hip_ok(ctx, hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+    *p = static_cast<T *>(ctx->pool.get(std::max<size_t>(count, 1) * sizeof(T)));
+    if (!*p) set_error(ctx, "hipMalloc of %zu bytes failed", count * sizeof(T));
+    return *p != nullptr;
 }
 
 #define LAUNCH(kernel, count, ...) \
@@ -898,10 +930,12 @@ bool band_from_device(cicy_sys *s, const int64_t *row_ptr, const int64_t *col_in
         return false;
     LAUNCH(k_add_counts, nv + 1, n, own, colcnt, cnt);
     int nhalf = 0, noff = 0;
-    if (!cub.scan(cnt, s->xrp, n + 1) || !cub.scan(own, ostart, n + 1) ||
-        !cub.scan(colcnt, cstart, n + 1) || !last_int(ctx, s->xrp, n, &nhalf) ||
-        !last_int(ctx, ostart, n, &noff))
+    int *totbuf, tot[8];
+    if (!ps.get(totbuf, 8) || !cub.scan(cnt, s->xrp, n + 1) || !cub.scan(own, ostart, n + 1) ||
+        !cub.scan(colcnt, cstart, n + 1) || !last_ints(ctx, totbuf, {s->xrp, ostart}, n, tot))
         return false;
+    nhalf = tot[0];
+    noff = tot[1];
     s->nx = nhalf;
     int *tkey, *tidx, *trow, *skey, *sidx;
     double *tval;
@@ -954,12 +988,8 @@ bool band_from_device(cicy_sys *s, const int64_t *row_ptr, const int64_t *col_in
 // ------------------------------------------------------------------------------------
 
 void condensed_free_device(cicy_cond *c) {
-    int *ip[] = {c->d_frp, c->d_fcol, c->d_orig};
-    for (int *q : ip)
-        if (q) (void)hipFree(q);
-    double *dp[] = {c->d_fval, c->d_cdiag};
-    for (double *q : dp)
-        if (q) (void)hipFree(q);
+    void *ptrs[] = {c->d_frp, c->d_fcol, c->d_orig, c->d_fval, c->d_cdiag};
+    for (void *q : ptrs) dev_release(c->ctx, q);
     c->d_frp = c->d_fcol = c->d_orig = nullptr;
     c->d_fval = c->d_cdiag = nullptr;
 }
@@ -978,6 +1008,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
     Cub cub{ctx, ps};
     const size_t nv = (size_t)n;
     cicy_cond *c = new cicy_cond();
+    c->ctx = ctx;
     c->on_device = true;
     c->n = n;
     c->nb = std::max(1, std::min(s->nblocks, 1024));
@@ -1085,7 +1116,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
     int *arp, *acol, *brp, *bcol;
     double *aval, *bval;
     int *alive, *state, *sel, *selpos, *sdeg, *degoff, *rec_of, *tflag, *tpos, *scnt, *cpos,
-        *fcnt, *fpos, *ub, *upos, *len;
+        *fcnt, *fpos, *ub, *upos, *len, *totbuf;
     double *d;
     unsigned int *undecided;
     if (!ps.get(arp, mv + 1) || !ps.get(acol, (size_t)cap) || !ps.get(aval, (size_t)cap) ||
@@ -1095,7 +1126,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         !ps.get(tpos, mv + 1) || !ps.get(scnt, mv + 1) || !ps.get(cpos, mv + 1) ||
         !ps.get(fcnt, mv + 1) || !ps.get(fpos, mv + 1) || !ps.get(ub, mv + 1) ||
         !ps.get(upos, mv + 1) || !ps.get(len, mv + 1) || !ps.get(d, mv + 1) ||
-        !ps.get(undecided, 1))
+        !ps.get(undecided, 1) || !ps.get(totbuf, 8))
         return fail();
     if (!lower_to_full(ctx, ps, m, nred, rrp, rcol, rval, arp, acol, aval) ||
         !hip_ok(ctx, hipMemcpyAsync(d, sdiag, mv * sizeof(double), hipMemcpyDeviceToDevice, st),
@@ -1125,9 +1156,13 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
     int R0 = 0, E0 = 0, RV0 = 0, C0 = 0;
     while (m > 0) {
         LAUNCH(k_cand, m, m, arp, d, sdiag, alive, state, kElimDegree);
+        // passes in batches (a pass after the set has settled changes nothing); the count
+        // of undecided candidates is that of the batch's last pass
         int passes = 0;
         for (;;) {
             unsigned int und = 0;
+            for (int q = 0; q < kMisBatch - 1; q++)
+                LAUNCH(k_mis_pass, m, m, arp, acol, state, undecided);
             if (!hip_ok(ctx, hipMemsetAsync(undecided, 0, sizeof(unsigned), st), "memset"))
                 return fail();
             LAUNCH(k_mis_pass, m, m, arp, acol, state, undecided);
@@ -1135,17 +1170,20 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
                 !d2h_sync(ctx, &und, undecided, sizeof(unsigned)))
                 return fail();
             if (und == 0) break;
-            if (++passes > kMaxMisPasses) {
+            passes += kMisBatch;
+            if (passes > kMaxMisPasses) {
                 set_error(ctx, "series rounds: independent set not settled after %d passes",
                           kMaxMisPasses);
                 return fail();
             }
         }
         LAUNCH(k_sel_flags, mv + 1, m, state, arp, sel, sdeg);
-        int nsel = 0, nent = 0;
+        int nsel = 0, nent = 0, tot[8];
         if (!cub.scan(sel, selpos, m + 1) || !cub.scan(sdeg, degoff, m + 1) ||
-            !last_int(ctx, selpos, m, &nsel) || !last_int(ctx, degoff, m, &nent))
+            !last_ints(ctx, totbuf, {selpos, degoff}, m, tot))
             return fail();
+        nsel = tot[0];
+        nent = tot[1];
         if (nsel == 0) break;
         if ((size_t)R0 + nsel > rcap - 1 || (size_t)E0 + nent > ecap - 1) {
             set_error(ctx, "series rounds: record arrays overflow");
@@ -1154,12 +1192,11 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         LAUNCH(k_records, m, m, sel, selpos, degoff, arp, acol, aval, d, R0, E0, c->e_node,
                c->e_off, c->e_d, c->e_nbr, c->e_a, rec_of);
         LAUNCH(k_targets, mv + 1, m, alive, sel, arp, acol, tflag, scnt, fcnt, ub);
-        int ntarg = 0, ncontrib = 0, nfill = 0, nub = 0;
         if (!cub.scan(tflag, tpos, m + 1) || !cub.scan(scnt, cpos, m + 1) ||
             !cub.scan(fcnt, fpos, m + 1) || !cub.scan(ub, upos, m + 1) ||
-            !last_int(ctx, tpos, m, &ntarg) || !last_int(ctx, cpos, m, &ncontrib) ||
-            !last_int(ctx, fpos, m, &nfill) || !last_int(ctx, upos, m, &nub))
+            !last_ints(ctx, totbuf, {tpos, cpos, fpos, upos}, m, tot))
             return fail();
+        const int ntarg = tot[0], ncontrib = tot[1], nfill = tot[2], nub = tot[3];
         if ((size_t)RV0 + ntarg > ecap - 1 || (size_t)C0 + ncontrib > ecap - 1) {
             set_error(ctx, "series rounds: receiver arrays overflow");
             return fail();

@@ -495,7 +495,11 @@ namespace {
 template <typename T>
 bool dmalloc(cicy_ctx *ctx, T **p, size_t count, size_t *total) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    if (!hip_ok(ctx, hipMalloc((void **)p, bytes), "hipMalloc")) return false;
+    *p = static_cast<T *>(ctx->pool.get(bytes));
+    if (!*p) {
+        set_error(ctx, "hipMalloc of %zu bytes failed", bytes);
+        return false;
+    }
     if (total) *total += bytes;
     return true;
 }
@@ -782,7 +786,7 @@ void cicy_destroy(cicy_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->host_state) (void)hipHostFree(ctx->host_state);
-    if (ctx->bc_ws) (void)hipFree(ctx->bc_ws);
+    cicy::dev_release(ctx, ctx->bc_ws);
     ctx->pool.clear();
     (void)hipEventDestroy(ctx->ev_start);
     (void)hipEventDestroy(ctx->ev_stop);
@@ -1164,8 +1168,7 @@ void cicy_sys_destroy(cicy_sys *s) {
                     s->xblk, s->xrp, s->xnbr, s->xval, s->xorder, s->x, s->r, s->p0, s->p1, s->q,
                     s->x0, s->r0, s->b, s->partials, s->gpart, s->gcount, s->st, s->scal,
                     s->counter2};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    for (void *p : ptrs) cicy::dev_release(s->ctx, p);
     delete s;
 }
 
@@ -1524,7 +1527,7 @@ int cicy_boundary_current_f64(cicy_ctx *ctx, int64_t ne, const int64_t *n_bnd,
     // inputs, result and reduction words in the context's workspace, which only grows
     const size_t words = 3 * (size_t)ne + (size_t)nv + 1 + kBcRedWords;
     if (words > ctx->bc_words) {
-        if (ctx->bc_ws) (void)hipFree(ctx->bc_ws);
+        cicy::dev_release(ctx, ctx->bc_ws);
         ctx->bc_ws = nullptr;
         ctx->bc_words = 0;
         if (!dmalloc(ctx, &ctx->bc_ws, words, nullptr)) return 0;
