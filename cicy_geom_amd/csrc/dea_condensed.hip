@@ -540,7 +540,7 @@ struct SeriesHost {
 // round); every node next to a record is a receiver, whose diagonal loses a_uk^2 / d_k and
 // whose row gains the fill -a_uk a_vk / d_k, record after record; equal columns are summed
 // after the old value, in record order.
-void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = kElimDegree) {
+void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = elim_degree()) {
     const int m = C.m;
     Z.m = m;
     StageClock clk;

@@ -135,7 +135,13 @@ struct DevPool {
 
 // Nodes of the reduced system with at most this many neighbours are eliminated exactly
 // (series reduction / star-mesh transform) before the core CG.
-constexpr int kElimDegree = 4;
+constexpr int kElimDegree = 6;
+// the degree cap in effect: kElimDegree, or CICY_ELIM_DEGREE (2..8) when set (a tuning knob;
+// host and device forms read the same value)
+inline int elim_degree() {
+    if (const char *e = getenv("CICY_ELIM_DEGREE")) return std::max(2, std::min(8, atoi(e)));
+    return kElimDegree;
+}
 
 // Elimination priority of node k of the reduced system in the series rounds: a fixed
 // integer hash of k with k itself as tie-break (lower value = earlier).  The rounds take

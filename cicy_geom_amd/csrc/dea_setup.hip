@@ -1141,9 +1141,10 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
             return fail();
         if (!hip_ok(ctx, hipStreamSynchronize(st), "sync")) return fail();
     }
-    // record and receiver arrays: at most m records, 4 m record entries, 4 m contributions
-    // (each entry of a record is one contribution), 4 m receivers
-    const size_t rcap = mv + 1, ecap = 4 * mv + 1;
+    // record and receiver arrays: at most m records, D m record entries, D m contributions
+    // (each entry of a record is one contribution), D m receivers (D the degree cap)
+    const int max_degree = elim_degree();
+    const size_t rcap = mv + 1, ecap = (size_t)max_degree * mv + 1;
     if (!dev_alloc(ctx, &c->e_node, rcap) || !dev_alloc(ctx, &c->e_off, rcap + 1) ||
         !dev_alloc(ctx, &c->e_d, rcap) || !dev_alloc(ctx, &c->e_nbr, ecap) ||
         !dev_alloc(ctx, &c->e_a, ecap) || !dev_alloc(ctx, &c->rv_node, ecap) ||
@@ -1155,7 +1156,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
     c->rv_round_off.assign(1, 0);
     int R0 = 0, E0 = 0, RV0 = 0, C0 = 0;
     while (m > 0) {
-        LAUNCH(k_cand, m, m, arp, d, sdiag, alive, state, kElimDegree);
+        LAUNCH(k_cand, m, m, arp, d, sdiag, alive, state, max_degree);
         // passes in batches (a pass after the set has settled changes nothing); the count
         // of undecided candidates is that of the batch's last pass
         int passes = 0;
