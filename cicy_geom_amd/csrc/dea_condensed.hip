@@ -43,6 +43,7 @@ namespace {
 
 constexpr int kCondThreads = 256;
 constexpr int kFloating = -2;  // pl marker of a floating path (same value as kFloatingPath)
+constexpr int kPathChunk = 8;  // path rows whose loads a sweep issues together
 
 // One sweep of T z = f per path, f = rhs on the path rows, minus the couplings to the
 // anchors when x is given (back-substitution): forward elimination with the stored
@@ -74,21 +75,48 @@ __global__ __launch_bounds__(kCondThreads) void k_path_solve(
         for (int i = s; i <= e; i++) z[i] -= mean;
         return;
     }
+    // the same recurrences with the loads of kPathChunk rows issued before their arithmetic
+    // (a path can be a whole tube: its rows would otherwise be a chain of dependent misses)
     double g = rhs[s];
     if (xanchor && pl[k] >= 0) g -= band[s] * xanchor[pl[k]];
-    if (s == e && xanchor && pr[k] >= 0) g -= band[e + 1] * xanchor[pr[k]];
+    const double fe = (xanchor && pr[k] >= 0) ? band[e + 1] * xanchor[pr[k]] : 0.0;
+    if (s == e) g -= fe;
     z[s] = g;
-    for (int i = s + 1; i <= e; i++) {
-        double f = rhs[i];
-        if (i == e && xanchor && pr[k] >= 0) f -= band[e + 1] * xanchor[pr[k]];
-        g = f - lfac[i] * g;
-        z[i] = g;
+    for (int i0 = s + 1; i0 <= e; i0 += kPathChunk) {
+        double f[kPathChunk], l[kPathChunk];
+#pragma unroll
+        for (int t = 0; t < kPathChunk; t++) {
+            const int i = min(i0 + t, e);
+            f[t] = rhs[i];
+            l[t] = lfac[i];
+        }
+#pragma unroll
+        for (int t = 0; t < kPathChunk; t++) {
+            const int i = i0 + t;
+            if (i > e) break;
+            const double fi = i == e ? f[t] - fe : f[t];
+            g = fi - l[t] * g;
+            z[i] = g;
+        }
     }
     double zn = z[e] / mfac[e];
     z[e] = zn;
-    for (int i = e - 1; i >= s; i--) {
-        zn = (z[i] - band[i + 1] * zn) / mfac[i];
-        z[i] = zn;
+    for (int i0 = e - 1; i0 >= s; i0 -= kPathChunk) {
+        double zz[kPathChunk], bb[kPathChunk], mm[kPathChunk];
+#pragma unroll
+        for (int t = 0; t < kPathChunk; t++) {
+            const int i = max(i0 - t, s);
+            zz[t] = z[i];
+            bb[t] = band[i + 1];
+            mm[t] = mfac[i];
+        }
+#pragma unroll
+        for (int t = 0; t < kPathChunk; t++) {
+            const int i = i0 - t;
+            if (i < s) break;
+            zn = (zz[t] - bb[t] * zn) / mm[t];
+            z[i] = zn;
+        }
     }
 }
 
@@ -569,6 +597,7 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = elim_d
     std::vector<double> d(C.sdiag.begin(), C.sdiag.end());
     std::vector<char> alive((size_t)m, 1), in((size_t)m, 0);
     std::vector<int> rec_of((size_t)m, -1);
+    long long left = m;  // nodes not eliminated yet
     Z.round_off.assign(1, 0);
     Z.e_off.assign(1, 0);
     Z.rv_round_off.assign(1, 0);
@@ -596,7 +625,13 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = elim_d
                 nsel++;
             }
         }
-        if (nsel == 0) break;
+        // a round that would take fewer than 1 % of the nodes left is not taken: the rounds
+        // stop (as on the device)
+        if (nsel == 0 || (long long)nsel * 100 < left) {
+            std::fill(in.begin(), in.end(), 0);
+            break;
+        }
+        left -= nsel;
         // records, ascending
         for (int k = 0; k < m; k++) {
             if (!in[k]) continue;

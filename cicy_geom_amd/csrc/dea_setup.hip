@@ -86,7 +86,10 @@ constexpr int kST = 256;  // threads per work-group of the set-up kernels
 constexpr int kFloatingPathDev = -2;  // pl of a floating path (the host form's kFloatingPath)
 constexpr int kUndecided = 1, kIn = 2, kOut = 3;  // candidate states of a round
 constexpr int kMaxMisPasses = 4096;
-constexpr int kMisBatch = 3;  // independent-set passes launched per read-back
+constexpr int kMisBatch = 16;  // independent-set passes launched per read-back
+// the rounds stop at the first that would eliminate fewer than 1 / kMinRoundFraction of the
+// nodes left
+constexpr long long kMinRoundFraction = 100;
 
 inline int grid_of(long long n) {
     return (int)std::max<long long>(1, std::min<long long>((n + kST - 1) / kST, 1 << 16));
@@ -498,40 +501,71 @@ __global__ __launch_bounds__(kST) void k_cand(int m, const int *__restrict__ arp
 
 // One pass toward the greedy independent set in elim_priority order: an undecided candidate
 // with a selected neighbour is out; one whose earlier candidate neighbours are all out is
-// in; the others wait for the next pass (counted in *undecided).  Decisions are final, so
+// in; the others wait for the next pass (*undecided is set when there are any).  Decisions are final, so
 // reading a neighbour's state while it changes only delays a decision.
 __global__ __launch_bounds__(kST) void k_mis_pass(int m, const int *__restrict__ arp,
                                                   const int *__restrict__ acol, int *state,
+                                                  const unsigned int *__restrict__ before,
                                                   unsigned int *__restrict__ undecided) {
+    // the set settled in an earlier pass of this batch (no candidate was left waiting)
+    if (before && *before == 0) return;
+    bool any_wait = false;
     GRID_LOOP(k, m) {
-        if (state[k] != kUndecided) continue;
-        const unsigned long long pk = elim_priority((int)k);
-        bool blocked = false, wait = false;
-        for (int t = arp[k]; t < arp[k + 1]; t++) {
-            const int j = acol[t];
-            const int sj = __hip_atomic_load(&state[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (sj == kIn) {
-                blocked = true;
-                break;
+        bool wait = false;
+        if (state[k] == kUndecided) {
+            const unsigned long long pk = elim_priority((int)k);
+            bool blocked = false;
+            for (int t = arp[k]; t < arp[k + 1]; t++) {
+                const int j = acol[t];
+                const int sj = state[j];
+                if (sj == kIn) {
+                    blocked = true;
+                    break;
+                }
+                if (sj == kUndecided && elim_priority(j) < pk) wait = true;
             }
-            if (sj == kUndecided && elim_priority(j) < pk) wait = true;
+            if (blocked) {
+                state[k] = kOut;
+                wait = false;
+            } else if (!wait) {
+                state[k] = kIn;
+            }
         }
-        if (blocked)
-            __hip_atomic_store(&state[k], kOut, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else if (!wait)
-            __hip_atomic_store(&state[k], kIn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else
-            atomicAdd(undecided, 1u);
+        any_wait = any_wait || wait;
+    }
+    // one store per work-group with a waiting candidate (only "any left" is read)
+    if (__syncthreads_or(any_wait) && threadIdx.x == 0) *undecided = 1u;
+}
+
+// four counts per node, prefix-summed together in one scan
+struct Cnt4 {
+    int v[4];
+};
+struct Cnt4Sum {
+    __host__ __device__ Cnt4 operator()(const Cnt4 &a, const Cnt4 &b) const {
+        return Cnt4{{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2], a.v[3] + b.v[3]}};
+    }
+};
+
+__global__ __launch_bounds__(kST) void k_unpack4(int count, const Cnt4 *__restrict__ in,
+                                                 int *__restrict__ o0, int *__restrict__ o1,
+                                                 int *__restrict__ o2, int *__restrict__ o3) {
+    GRID_LOOP(i, count) {
+        const Cnt4 c = in[i];
+        o0[i] = c.v[0];
+        if (o1) o1[i] = c.v[1];
+        if (o2) o2[i] = c.v[2];
+        if (o3) o3[i] = c.v[3];
     }
 }
 
 __global__ __launch_bounds__(kST) void k_sel_flags(int m, const int *__restrict__ state,
                                                    const int *__restrict__ arp,
-                                                   int *__restrict__ sel, int *__restrict__ sdeg) {
+                                                   int *__restrict__ sel, Cnt4 *__restrict__ c4) {
     GRID_LOOP(k, m + 1) {
         const int s = k < m && state[k] == kIn;
         sel[k] = s;
-        sdeg[k] = s ? arp[k + 1] - arp[k] : 0;
+        c4[k] = Cnt4{{s, s ? arp[k + 1] - arp[k] : 0, 0, 0}};
     }
 }
 
@@ -567,8 +601,7 @@ __global__ __launch_bounds__(kST) void k_targets(int m, const int *__restrict__ 
                                                  const int *__restrict__ sel,
                                                  const int *__restrict__ arp,
                                                  const int *__restrict__ acol,
-                                                 int *__restrict__ tflag, int *__restrict__ scnt,
-                                                 int *__restrict__ fcnt, int *__restrict__ ub) {
+                                                 int *__restrict__ tflag, Cnt4 *__restrict__ c4) {
     GRID_LOOP(u, m + 1) {
         int t = 0, s = 0, f = 0, b = 0;
         if (u < m && alive[u] && !sel[u]) {
@@ -583,9 +616,7 @@ __global__ __launch_bounds__(kST) void k_targets(int m, const int *__restrict__ 
             b = arp[u + 1] - arp[u] - s + f;
         }
         tflag[u] = t;
-        scnt[u] = s;
-        fcnt[u] = f;
-        ub[u] = b;
+        c4[u] = Cnt4{{t, s, f, b}};
     }
 }
 
@@ -796,6 +827,18 @@ struct Cub {
         tmp = t;
         return hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, count,
                                                             ctx->stream), "scan");
+    }
+    // exclusive prefix sums of count Cnt4 entries
+    bool scan4(const Cnt4 *in, Cnt4 *out, int count) {
+        size_t bytes = 0;
+        const Cnt4 zero{{0, 0, 0, 0}};
+        if (!hip_ok(ctx, hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, in, out, Cnt4Sum(),
+                                                           zero, count, ctx->stream), "scan"))
+            return false;
+        unsigned char *t = nullptr;
+        if (!ps.get(t, bytes)) return false;
+        return hip_ok(ctx, hipcub::DeviceScan::ExclusiveScan(t, bytes, in, out, Cnt4Sum(), zero,
+                                                             count, ctx->stream), "scan");
     }
     // stable radix sort of (key, value) pairs on the low `bits` bits of the keys
     bool sort(const int *k_in, int *k_out, const int *v_in, int *v_out, int count, int bits) {
@@ -1115,18 +1158,20 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
     int cap = std::max(2 * nred, 1);
     int *arp, *acol, *brp, *bcol;
     double *aval, *bval;
-    int *alive, *state, *sel, *selpos, *sdeg, *degoff, *rec_of, *tflag, *tpos, *scnt, *cpos,
-        *fcnt, *fpos, *ub, *upos, *len, *totbuf;
+    int *alive, *state, *sel, *selpos, *degoff, *rec_of, *tflag, *tpos, *cpos, *fpos, *upos, *len,
+        *totbuf;
+    Cnt4 *c4, *c4s;
     double *d;
     unsigned int *undecided;
     if (!ps.get(arp, mv + 1) || !ps.get(acol, (size_t)cap) || !ps.get(aval, (size_t)cap) ||
         !ps.get(brp, mv + 1) || !ps.get(alive, mv + 1) || !ps.get(state, mv + 1) ||
-        !ps.get(sel, mv + 1) || !ps.get(selpos, mv + 1) || !ps.get(sdeg, mv + 1) ||
+        !ps.get(sel, mv + 1) || !ps.get(selpos, mv + 1) || !ps.get(c4, mv + 1) ||
+        !ps.get(c4s, mv + 1) ||
         !ps.get(degoff, mv + 1) || !ps.get(rec_of, mv + 1) || !ps.get(tflag, mv + 1) ||
-        !ps.get(tpos, mv + 1) || !ps.get(scnt, mv + 1) || !ps.get(cpos, mv + 1) ||
-        !ps.get(fcnt, mv + 1) || !ps.get(fpos, mv + 1) || !ps.get(ub, mv + 1) ||
+        !ps.get(tpos, mv + 1) || !ps.get(cpos, mv + 1) ||
+        !ps.get(fpos, mv + 1) ||
         !ps.get(upos, mv + 1) || !ps.get(len, mv + 1) || !ps.get(d, mv + 1) ||
-        !ps.get(undecided, 1) || !ps.get(totbuf, 8))
+        !ps.get(undecided, kMisBatch) || !ps.get(totbuf, 8))
         return fail();
     if (!lower_to_full(ctx, ps, m, nred, rrp, rcol, rval, arp, acol, aval) ||
         !hip_ok(ctx, hipMemcpyAsync(d, sdiag, mv * sizeof(double), hipMemcpyDeviceToDevice, st),
@@ -1155,48 +1200,56 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
     c->round_off.assign(1, 0);
     c->rv_round_off.assign(1, 0);
     int R0 = 0, E0 = 0, RV0 = 0, C0 = 0;
+    long long left = m;  // nodes not eliminated yet
     while (m > 0) {
         LAUNCH(k_cand, m, m, arp, d, sdiag, alive, state, max_degree);
-        // passes in batches (a pass after the set has settled changes nothing); the count
-        // of undecided candidates is that of the batch's last pass
+        // passes in batches of kMisBatch launches, each counting the candidates left waiting
+        // into its own word; a pass after one that left none returns at once
         int passes = 0;
         for (;;) {
-            unsigned int und = 0;
-            for (int q = 0; q < kMisBatch - 1; q++)
-                LAUNCH(k_mis_pass, m, m, arp, acol, state, undecided);
-            if (!hip_ok(ctx, hipMemsetAsync(undecided, 0, sizeof(unsigned), st), "memset"))
+            unsigned int und[kMisBatch];
+            if (!hip_ok(ctx, hipMemsetAsync(undecided, 0, kMisBatch * sizeof(unsigned), st),
+                        "memset"))
                 return fail();
-            LAUNCH(k_mis_pass, m, m, arp, acol, state, undecided);
+            for (int q = 0; q < kMisBatch; q++)
+                LAUNCH(k_mis_pass, m, m, arp, acol, state,
+                       q > 0 ? undecided + q - 1 : (const unsigned *)nullptr, undecided + q);
             if (!hip_ok(ctx, hipGetLastError(), "k_mis_pass") ||
-                !d2h_sync(ctx, &und, undecided, sizeof(unsigned)))
+                !d2h_sync(ctx, und, undecided, sizeof(und)))
                 return fail();
-            if (und == 0) break;
-            passes += kMisBatch;
+            bool settled = false;
+            for (int q = 0; q < kMisBatch && !settled; q++) {
+                passes++;
+                settled = und[q] == 0;
+            }
+            if (settled) break;
             if (passes > kMaxMisPasses) {
                 set_error(ctx, "series rounds: independent set not settled after %d passes",
                           kMaxMisPasses);
                 return fail();
             }
         }
-        LAUNCH(k_sel_flags, mv + 1, m, state, arp, sel, sdeg);
+        LAUNCH(k_sel_flags, mv + 1, m, state, arp, sel, c4);
         int nsel = 0, nent = 0, tot[8];
-        if (!cub.scan(sel, selpos, m + 1) || !cub.scan(sdeg, degoff, m + 1) ||
-            !last_ints(ctx, totbuf, {selpos, degoff}, m, tot))
-            return fail();
+        if (!cub.scan4(c4, c4s, m + 1)) return fail();
+        LAUNCH(k_unpack4, mv + 1, m + 1, c4s, selpos, degoff, (int *)nullptr, (int *)nullptr);
+        if (!last_ints(ctx, totbuf, {selpos, degoff}, m, tot)) return fail();
         nsel = tot[0];
         nent = tot[1];
-        if (nsel == 0) break;
+        // a round that would take fewer than 1 % of the nodes left is not worth its pass over
+        // the whole system: the rounds stop (the host form stops by the same rule)
+        if (nsel == 0 || (long long)nsel * kMinRoundFraction < left) break;
+        left -= nsel;
         if ((size_t)R0 + nsel > rcap - 1 || (size_t)E0 + nent > ecap - 1) {
             set_error(ctx, "series rounds: record arrays overflow");
             return fail();
         }
         LAUNCH(k_records, m, m, sel, selpos, degoff, arp, acol, aval, d, R0, E0, c->e_node,
                c->e_off, c->e_d, c->e_nbr, c->e_a, rec_of);
-        LAUNCH(k_targets, mv + 1, m, alive, sel, arp, acol, tflag, scnt, fcnt, ub);
-        if (!cub.scan(tflag, tpos, m + 1) || !cub.scan(scnt, cpos, m + 1) ||
-            !cub.scan(fcnt, fpos, m + 1) || !cub.scan(ub, upos, m + 1) ||
-            !last_ints(ctx, totbuf, {tpos, cpos, fpos, upos}, m, tot))
-            return fail();
+        LAUNCH(k_targets, mv + 1, m, alive, sel, arp, acol, tflag, c4);
+        if (!cub.scan4(c4, c4s, m + 1)) return fail();
+        LAUNCH(k_unpack4, mv + 1, m + 1, c4s, tpos, cpos, fpos, upos);
+        if (!last_ints(ctx, totbuf, {tpos, cpos, fpos, upos}, m, tot)) return fail();
         const int ntarg = tot[0], ncontrib = tot[1], nfill = tot[2], nub = tot[3];
         if ((size_t)RV0 + ntarg > ecap - 1 || (size_t)C0 + ncontrib > ecap - 1) {
             set_error(ctx, "series rounds: receiver arrays overflow");
@@ -1213,9 +1266,10 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, c->e_off + R0 + nsel, E0 + nent);
         LAUNCH(k_merge, mv + 1, m, alive, sel, tflag, arp, acol, aval, upos, fpos, rec_of, c->e_off,
                c->e_nbr, c->e_a, c->e_d, fc, fvv, oc, ov, len);
-        int nlen = 0;
-        if (!cub.scan(len, brp, m + 1) || !last_int(ctx, brp, m, &nlen)) return fail();
-        if (!ps.get(bcol, (size_t)std::max(nlen, 1)) || !ps.get(bval, (size_t)std::max(nlen, 1)))
+        // the new rows hold at most nub entries (the merge's bound), so no read-back of their
+        // true total is needed
+        if (!cub.scan(len, brp, m + 1) || !ps.get(bcol, (size_t)std::max(nub, 1)) ||
+            !ps.get(bval, (size_t)std::max(nub, 1)))
             return fail();
         LAUNCH(k_compact, m, m, upos, brp, oc, ov, bcol, bval);
         LAUNCH(k_kill, m, m, sel, alive);
@@ -1234,6 +1288,13 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         c->round_off.push_back(R0);
         c->rv_round_off.push_back(RV0);
         if (!hip_ok(ctx, hipGetLastError(), "series round")) return fail();
+        if (clk.on) {
+            (void)hipStreamSynchronize(st);
+            fprintf(stderr, "[cicy setup]   round %zu: %d eliminated, %d receivers, %d fill, "
+                    "%d independent-set passes\n", c->round_off.size() - 1, nsel, ntarg, nfill,
+                    passes);
+            clk.mark("device condense: round");
+        }
     }
     // closing offsets of the record and receiver lists
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, c->e_off + R0, E0);
