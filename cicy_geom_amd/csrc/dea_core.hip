@@ -70,6 +70,14 @@ constexpr int kCoreThreads = 1024;
 constexpr int kCoreWaves = kCoreThreads / kWave;
 constexpr int kCorePairs = kCoreWaves / 2;  // blocks per round: two waves share one block
 constexpr int kCoreMaxBlock = 128;     // rows of a dense block (larger groups are split)
+constexpr int kSmallBlock = 64;        // blocks inverted beside the large ones (second stream)
+// the split size in effect: kCoreMaxBlock, or CICY_CORE_MAX_BLOCK (16..kCoreMaxBlock) when set
+// (a tuning knob; host and device layouts read the same value)
+inline int core_max_block() {
+    if (const char *e = getenv("CICY_CORE_MAX_BLOCK"))
+        return std::max(16, std::min(kCoreMaxBlock, atoi(e)));
+    return kCoreMaxBlock;
+}
 constexpr int kCoreChunk = 4096;       // iterations per cooperative launch (at most)
 constexpr int kCoreRowsPerGroup = 192; // work-groups: about one per this many rows
 constexpr int kCoreUnroll = 12;        // block-inverse columns loaded before use
@@ -351,7 +359,8 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
     };
     parallel_for(nth, sort_rows);
     H.diag = cdiag;
-    // row groups -> core row ranges (split at kCoreMaxBlock rows)
+    // row groups -> core row ranges (split at core_max_block() rows)
+    const int maxb = core_max_block();
     std::vector<std::pair<int, int>> groups;
     for (int64_t b = 0; b < nblocks; b++) {
         const int64_t s0 = ranges[2 * b], s1 = ranges[2 * b + 1];
@@ -361,8 +370,8 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
         }
         const int q0 = (int)(std::lower_bound(orig.begin(), orig.end(), s0) - orig.begin());
         const int q1 = (int)(std::lower_bound(orig.begin(), orig.end(), s1) - orig.begin());
-        for (int q = q0; q < q1; q += kCoreMaxBlock) {
-            const int e = std::min(q1, q + kCoreMaxBlock);
+        for (int q = q0; q < q1; q += maxb) {
+            const int e = std::min(q1, q + maxb);
             if (e - q >= 2) groups.emplace_back(q, e);
         }
     }
@@ -1262,6 +1271,8 @@ bool core_groups(cicy_ctx *ctx, int *cus) {
         return false;
     }
     *cus = std::min(*cus, kSlotLanes * kWave);
+    // CICY_CORE_GROUPS caps the work-groups (a tuning knob; the layout follows it)
+    if (const char *e = getenv("CICY_CORE_GROUPS")) *cus = std::max(1, std::min(*cus, atoi(e)));
     return true;
 }
 
@@ -1304,7 +1315,8 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
     if (!core_groups(ctx, &cus)) return nullptr;
     StageClock clk;
     PoolScope ps(ctx);
-    // row groups -> core row ranges (split at kCoreMaxBlock rows)
+    // row groups -> core row ranges (split at core_max_block() rows)
+    const int maxb = core_max_block();
     for (int64_t b = 0; b < nblocks; b++)
         if (ranges[2 * b] < 0 || ranges[2 * b + 1] < ranges[2 * b]) {
             set_error(ctx, "core blocks: a row group has a negative start or ends before it "
@@ -1328,8 +1340,8 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
     std::vector<std::pair<int, int>> groups;
     for (int64_t b = 0; b < nblocks; b++) {
         const int q0 = qb[(size_t)(2 * b)], q1 = qb[(size_t)(2 * b + 1)];
-        for (int q = q0; q < q1; q += kCoreMaxBlock) {
-            const int e = std::min(q1, q + kCoreMaxBlock);
+        for (int q = q0; q < q1; q += maxb) {
+            const int e = std::min(q1, q + maxb);
             if (e - q >= 2) groups.emplace_back(q, e);
         }
     }
@@ -1376,8 +1388,33 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)lds), "hipFuncSetAttribute"))
             return nullptr;
-        hipLaunchKernelGGL(k_block_inverse, dim3(ng), dim3(kLT), lds, st, dord, dgq0, dgnb, dgoff,
-                           frp, fcol, fval, cdiag, dinv, dok, dcg);
+        // blocks of at most kSmallBlock rows need a quarter of the LDS of a 128-row block,
+        // so several share a CU: they run on the context's second stream beside the large
+        // ones (the groups are sorted by size, the small ones last in gord)
+        int nbig = 0;
+        while (nbig < ng && gnb[(size_t)gord[(size_t)nbig]] > kSmallBlock) nbig++;
+        const int nsmall = ng - nbig;
+        const size_t lds_small = ((size_t)kSmallBlock * (kSmallBlock | 1) + 2 * kSmallBlock) *
+                                 sizeof(double);
+        bool launched = true;
+        if (nsmall > 0 && nbig > 0 && ctx->stream2) {
+            launched = hip_ok(ctx, hipEventRecord(ctx->ev_fork, st), "event") &&
+                       hip_ok(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0), "wait");
+            if (launched) {
+                hipLaunchKernelGGL(k_block_inverse, dim3(nsmall), dim3(kLT), lds_small,
+                                   ctx->stream2, dord + nbig, dgq0, dgnb, dgoff, frp, fcol, fval,
+                                   cdiag, dinv, dok, dcg);
+                hipLaunchKernelGGL(k_block_inverse, dim3(nbig), dim3(kLT), lds, st, dord, dgq0,
+                                   dgnb, dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
+                launched = hip_ok(ctx, hipGetLastError(), "k_block_inverse") &&
+                           hip_ok(ctx, hipEventRecord(ctx->ev_join, ctx->stream2), "event") &&
+                           hip_ok(ctx, hipStreamWaitEvent(st, ctx->ev_join, 0), "wait");
+            }
+        } else {
+            hipLaunchKernelGGL(k_block_inverse, dim3(ng), dim3(kLT), lds, st, dord, dgq0, dgnb,
+                               dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
+        }
+        if (!launched) return nullptr;
         if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
             !d2h_sync(ctx, okh.data(), dok, (size_t)ng * sizeof(int)))
             return nullptr;
@@ -1479,6 +1516,13 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
     // partitions and slices on the host (from the row lengths), the ELL on the device
     std::vector<int> sl_base, sl_end;
     const size_t ell = core_partitions(H, cus, sl_base, sl_end);
+    if (clk.on) {
+        int longest = 0;
+        for (int r = 0; r < m; r++) longest = std::max(longest, H.rp[(size_t)r + 1] - H.rp[(size_t)r]);
+        fprintf(stderr, "[cicy setup]   core: %d rows, %d entries, %zu ELL slots (x%.2f), "
+                "longest row %d, %d Jacobi rows, %zu blocks\n", m, nnz2, ell,
+                nnz2 > 0 ? (double)ell / nnz2 : 0.0, longest, H.mj, H.blk_n.size());
+    }
     z->G = H.G;
     z->nnz2 = nnz2;
     int lds_cols = 0;

@@ -213,6 +213,10 @@ struct cicy_ctx {
     int dropin_method = CICY_METHOD_PCG_F64;  // method of cicy_solve_dea_cg_sss
     std::vector<int64_t> dropin_blocks;       // row groups for the next block-method call
     hipStream_t stream = nullptr;
+    // a second stream (same priority) for set-up work that runs beside the first, joined
+    // back through ev_fork / ev_join
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
     cicy::PcgState *host_state = nullptr;  // pinned, 2 slots

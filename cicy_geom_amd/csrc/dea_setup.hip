@@ -87,6 +87,7 @@ constexpr int kFloatingPathDev = -2;  // pl of a floating path (the host form's 
 constexpr int kUndecided = 1, kIn = 2, kOut = 3;  // candidate states of a round
 constexpr int kMaxMisPasses = 4096;
 constexpr int kMisBatch = 16;  // independent-set passes launched per read-back
+constexpr int kPathChunkS = 8;  // path rows whose loads the factorisation issues together
 // the rounds stop at the first that would eliminate fewer than 1 / kMinRoundFraction of the
 // nodes left
 constexpr long long kMinRoundFraction = 100;
@@ -284,10 +285,26 @@ __global__ __launch_bounds__(kST) void k_paths(
         termL[p] = 0.0;
         termR[p] = 0.0;
         fillv[p] = 0.0;
-        mfac[s] = diag[s];
-        for (int k = s + 1; k <= e; k++) {
-            lfac[k] = band[k] / mfac[k - 1];
-            mfac[k] = diag[k] - lfac[k] * band[k];
+        // the Thomas factors, the loads of kPathChunkS rows issued before their arithmetic
+        double mprev = diag[s];
+        mfac[s] = mprev;
+        for (int k0 = s + 1; k0 <= e; k0 += kPathChunkS) {
+            double bb[kPathChunkS], dd[kPathChunkS];
+#pragma unroll
+            for (int t = 0; t < kPathChunkS; t++) {
+                const int k = min(k0 + t, e);
+                bb[t] = band[k];
+                dd[t] = diag[k];
+            }
+#pragma unroll
+            for (int t = 0; t < kPathChunkS; t++) {
+                const int k = k0 + t;
+                if (k > e) break;
+                const double l = bb[t] / mprev;
+                mprev = dd[t] - l * bb[t];
+                lfac[k] = l;
+                mfac[k] = mprev;
+            }
         }
         const double pivot_floor = 1e-12 * diag[e];
         bool singular = false;
@@ -764,6 +781,14 @@ struct LastArgs {
 };
 __global__ __launch_bounds__(64) void k_last(LastArgs a, int k, int at, int *__restrict__ out) {
     if ((int)threadIdx.x < k) out[threadIdx.x] = a.p[threadIdx.x][at];
+}
+
+// *bad = 1 when some v[i] is not positive and finite
+__global__ __launch_bounds__(kST) void k_nonpositive(int n, const double *__restrict__ v,
+                                                     int *__restrict__ bad) {
+    bool any = false;
+    GRID_LOOP(i, n) any = any || !(v[i] > 0.0) || !isfinite(v[i]);
+    if (__syncthreads_or(any) && threadIdx.x == 0) *bad = 1;
 }
 
 // the core: the nodes left, in S order, with their rows renumbered (both triangles)
@@ -1326,16 +1351,16 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, c->d_frp + mc, nnz2);
         // a core row with a vanishing diagonal (a floating group of tubes): no condensed
         // system, as on the host
-        std::vector<double> cd((size_t)mc);
-        if (!hip_ok(ctx, hipGetLastError(), "core") ||
-            !d2h_sync(ctx, cd.data(), c->d_cdiag, (size_t)mc * sizeof(double)))
+        int bad = 0;
+        if (!hip_ok(ctx, hipMemsetAsync(totbuf, 0, sizeof(int), st), "memset")) return fail();
+        LAUNCH(k_nonpositive, mc, mc, c->d_cdiag, totbuf);
+        if (!hip_ok(ctx, hipGetLastError(), "core") || !d2h_sync(ctx, &bad, totbuf, sizeof(int)))
             return fail();
-        for (double v : cd)
-            if (!(v > 0.0) || !std::isfinite(v)) {
-                condensed_free(c);
-                *why = "the core system has a non-positive diagonal (floating component)";
-                return nullptr;
-            }
+        if (bad) {
+            condensed_free(c);
+            *why = "the core system has a non-positive diagonal (floating component)";
+            return nullptr;
+        }
     }
     if (!hip_ok(ctx, hipStreamSynchronize(st), "sync")) return fail();
     clk.mark("device condense: core");

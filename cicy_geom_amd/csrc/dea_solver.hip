@@ -768,6 +768,9 @@ int cicy_create(cicy_ctx **out, int device) {
     if (const char *e = getenv("CICY_NO_GRAPHS")) ctx->no_graphs = e[0] && e[0] != '0';
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_poll[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_poll[1], hipEventDisableTiming) != hipSuccess ||
@@ -792,6 +795,10 @@ void cicy_destroy(cicy_ctx *ctx) {
     (void)hipEventDestroy(ctx->ev_stop);
     (void)hipEventDestroy(ctx->ev_poll[0]);
     (void)hipEventDestroy(ctx->ev_poll[1]);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
