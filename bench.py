@@ -150,8 +150,10 @@ def cpu_baseline(s, iters_needed, budget_s=12.0):
     solve_s = t_iter * iters_needed
     return {"value": 1.0 / solve_s, "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": f"{k} Jacobi-PCG iterations of the same N={s.n} system in "
-                      f"{t:.2f} s (oracle/dea_oracle.c, -O2, 1 thread, {cpu_model()}), "
-                      f"scaled by the {iters_needed} iterations the FP64 GPU solve needed",
+                      f"{t:.2f} s (oracle/dea_oracle.c, -O3 -ffp-contract=off, 1 thread, "
+                      f"{cpu_model()}), scaled by the {iters_needed} iterations the FP64 GPU "
+                      f"solve needed",
+            "extrapolated": True, "iterations_timed": k, "iterations_scaled_to": iters_needed,
             "s_per_iter": t_iter, "host_nproc": os.cpu_count()}
 
 

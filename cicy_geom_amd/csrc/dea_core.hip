@@ -630,7 +630,12 @@ __global__ void k_core_permute(int m, const int *perm, const double *src, double
 
 // One chunk of iterations.  All work-groups hold the same scalars (the grid sums are
 // bit-identical everywhere), so they take the same branches and leave together.
-__global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *st, int chunk) {
+// TIMED: each work-group adds the wall-clock ticks of phase u, the first grid sum, phase w
+// and the second grid sum of its iterations to tim[4 g .. 4 g + 3] (a diagnostic variant,
+// launched when CICY_CORE_TIMING is set)
+template <bool TIMED>
+__global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *st, int chunk,
+                                                          unsigned long long *tim) {
     __shared__ double yb[kCorePairs][kCoreMaxBlock];      // block input in flight
     __shared__ double zp[kCorePairs][2][kCoreMaxBlock];   // its two half products
     __shared__ double lds[kCoreSums * kCoreWaves];
@@ -812,13 +817,23 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
             }
         }
     }
+    unsigned long long tacc[4] = {0, 0, 0, 0}, tprev = TIMED ? wall_clock64() : 0;
+    auto tick = [&](int j) {
+        if (TIMED) {
+            const unsigned long long now = wall_clock64();
+            tacc[j] += now - tprev;
+            tprev = now;
+        }
+    };
     for (int it = 0; it < chunk && !done; it++) {
         double v[2];
         phase_u(true, v[0], v[1]);
+        tick(0);
         if (!grid_sum_slots<2, kCoreWaves, kCoreSums>(v, a.G, a.part, a.ctl, bar++, lds, &flag)) {
             done = 4;
             break;
         }
+        tick(1);
         const double g2 = v[0];
         rr = v[1];
         k++;
@@ -839,10 +854,12 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
             break;
         }
         double d[1] = {phase_w()};
+        tick(2);
         if (!grid_sum_slots<1, kCoreWaves, kCoreSums>(d, a.G, a.part, a.ctl, bar++, lds, &flag)) {
             done = 4;
             break;
         }
+        tick(3);
         beta = g2 / gamma;
         const double den = d[0] - beta * g2 / alpha;
         if (!(den > 0.0) || !isfinite(den)) {
@@ -852,6 +869,8 @@ __global__ __launch_bounds__(kCoreThreads) void k_core_cg(CoreDev a, CoreState *
         alpha = g2 / den;
         gamma = g2;
     }
+    if (TIMED && tid == 0)
+        for (int j = 0; j < 4; j++) tim[4 * g + j] += tacc[j];
     if (g == 0 && tid == 0) {
         st->k = k;
         st->gamma = gamma;
@@ -918,7 +937,7 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
                                            ctx->device),
                 "hipDeviceGetAttribute") ||
         !hip_ok(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &per_cu, reinterpret_cast<const void *>(k_core_cg), kCoreThreads, 0),
+                         &per_cu, reinterpret_cast<const void *>(k_core_cg<false>), kCoreThreads, 0),
                 "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
         return nullptr;
     if (per_cu < 1) {
@@ -942,7 +961,7 @@ cicy_core *core_setup(cicy_ctx *ctx, int m, const std::vector<int64_t> &crp,
                                       H.sl_off[H.wsl[g]]);
     const int lds_cols = (int)std::min<long long>(most, kCoreLdsCols);
     if (!hip_ok(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &per_cu, reinterpret_cast<const void *>(k_core_cg), kCoreThreads,
+                         &per_cu, reinterpret_cast<const void *>(k_core_cg<false>), kCoreThreads,
                          (size_t)lds_cols * sizeof(int)),
                 "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
         return nullptr;
@@ -1263,7 +1282,7 @@ bool core_groups(cicy_ctx *ctx, int *cus) {
                                            ctx->device),
                 "hipDeviceGetAttribute") ||
         !hip_ok(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &per_cu, reinterpret_cast<const void *>(k_core_cg), kCoreThreads, 0),
+                         &per_cu, reinterpret_cast<const void *>(k_core_cg<false>), kCoreThreads, 0),
                 "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
         return false;
     if (per_cu < 1) {
@@ -1288,7 +1307,7 @@ bool core_lds_cols(cicy_ctx *ctx, const CoreHost &H, int *lds_cols) {
     *lds_cols = (int)std::min<long long>(most, kCoreLdsCols);
     int per_cu = 0;
     if (!hip_ok(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &per_cu, reinterpret_cast<const void *>(k_core_cg), kCoreThreads,
+                         &per_cu, reinterpret_cast<const void *>(k_core_cg<false>), kCoreThreads,
                          (size_t)*lds_cols * sizeof(int)),
                 "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
         return false;
@@ -1590,15 +1609,25 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
     ok = ok && hip_ok(ctx, hipEventCreate(&e0), "hipEventCreate") &&
          hip_ok(ctx, hipEventCreate(&e1), "hipEventCreate") &&
          hip_ok(ctx, hipEventRecord(e0, stream), "event");
+    const bool timed = getenv("CICY_CORE_TIMING") != nullptr;
+    unsigned long long *tim = nullptr;
+    if (timed) {
+        tim = static_cast<unsigned long long *>(ctx->pool.get(4 * sizeof(unsigned long long) *
+                                                              (size_t)z->G));
+        ok = ok && tim &&
+             hip_ok(ctx, hipMemsetAsync(tim, 0, 4 * sizeof(unsigned long long) * (size_t)z->G,
+                                        stream), "memset");
+    }
+    const void *kernel = timed ? reinterpret_cast<const void *>(k_core_cg<true>)
+                               : reinterpret_cast<const void *>(k_core_cg<false>);
     while (ok && !h.done) {
         int chunk = kCoreChunk;
-        void *args[] = {(void *)&a, (void *)&z->st, (void *)&chunk};
+        void *args[] = {(void *)&a, (void *)&z->st, (void *)&chunk, (void *)&tim};
         ok = hip_ok(ctx, hipMemsetAsync(z->ctl, 0, sizeof(GridCtl), stream), "memset") &&
              hip_ok(ctx, hipMemsetD32Async((hipDeviceptr_t)z->part, (int)kSlotEmptyWord,
                                            3 * kCoreSums * 2 * (size_t)z->G, stream),
                     "memset") &&
-             hip_ok(ctx, hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_core_cg),
-                                                    dim3(z->G), dim3(kCoreThreads), args,
+             hip_ok(ctx, hipLaunchCooperativeKernel(kernel, dim3(z->G), dim3(kCoreThreads), args,
                                                     (size_t)z->lds_cols * sizeof(int), stream),
                     "hipLaunchCooperativeKernel (core CG)") &&
              d2h_sync(ctx, &h, z->st, sizeof(h));
@@ -1613,6 +1642,29 @@ bool solve_core(cicy_ctx *ctx, cicy_core *z, const double *rhs, double tol, long
         (void)hipEventElapsedTime(&kms, e0, e1);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    if (timed && tim) {
+        // per-iteration time of each phase, mean and largest over the work-groups
+        std::vector<unsigned long long> t(4 * (size_t)z->G);
+        int khz = 0;
+        if (ok && d2h_sync(ctx, t.data(), tim, t.size() * sizeof(unsigned long long)) &&
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) ==
+                hipSuccess &&
+            khz > 0 && h.k > 0) {
+            const char *names[4] = {"phase u", "grid sum A", "phase w", "grid sum B"};
+            for (int j = 0; j < 4; j++) {
+                double mean = 0.0, most = 0.0;
+                for (int g = 0; g < z->G; g++) {
+                    const double us = (double)t[4 * (size_t)g + j] * 1e3 / khz / (double)h.k;
+                    mean += us / z->G;
+                    most = std::max(most, us);
+                }
+                fprintf(stderr, "[cicy core timing] %-10s mean %6.2f us  max %6.2f us per "
+                        "iteration (%lld iterations, %d work-groups)\n", names[j], mean, most,
+                        h.k, z->G);
+            }
+        }
+        ctx->pool.put(tim);
+    }
     if (ok) {
         hipLaunchKernelGGL(k_core_permute, dim3(pg), dim3(256), 0, stream, z->m, z->perm, z->x,
                            x_out, 1);
