@@ -278,6 +278,65 @@ def bench_backbone(args):
         flush=True)
 
 
+def bench_assembly(args):
+    """Device assembly of the configuration's largest x-percolating cluster (R_flag = 1,
+    family x, boundary lists of x) from the network held in HBM (cicy_dnet_assemble), timed
+    over K calls; then the same system carried on into the solver without leaving the device
+    (set-up + block-method solve), and the host assembler (cicy_net_assemble) timed once as
+    the CPU baseline."""
+    from cicy_geom_amd import dea, network
+    t0 = time.perf_counter()
+    net = network.generate(args.config)
+    k = max(net.percolating_clusters(0), key=lambda c: net.cluster(c)[0])
+    gen_s = time.perf_counter() - t0
+    ctx = dea.Context(0)
+    t = time.perf_counter()
+    dn = net.device(ctx)
+    copy_s = time.perf_counter() - t
+    for _ in range(args.warmup):
+        dn.assemble(k, 1, 0, None, 0).close()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        a = dn.assemble(k, 1, 0, None, 0)
+        a.close()
+    asm_ms = 1e3 * (time.perf_counter() - t) / args.steps
+    a = dn.assemble(k, 1, 0, None, 0)
+    n, nnz, nblocks = a.n, a.nnz, len(a.blocks)
+    a.close()
+    # network in HBM -> solution in HBM: assembly, system set-up, block-method solve
+    e2e = []
+    for i in range(args.warmup + args.steps):
+        t = time.perf_counter()
+        a = dn.assemble(k, 1, 0, None, 0)
+        sd = a.system()
+        sd.set_blocks(a.blocks)
+        st = sd.solve(dea.METHOD_CONDENSED_BLOCK, args.tol, 0, dea.REL_B)
+        sd.close()
+        a.close()
+        if i >= args.warmup:
+            e2e.append(time.perf_counter() - t)
+        if st.status != dea.CICY_OK:
+            raise RuntimeError(f"solve after device assembly: {st.as_dict()}")
+    e2e_ms = 1e3 * float(np.mean(e2e))
+    t = time.perf_counter()
+    net.assemble(k, 1, 0, direction=0)
+    host_s = time.perf_counter() - t
+    dn.close()
+    ctx.close()
+    print(json.dumps({
+        "metric": "DEA system assemblies/s (device, network in HBM)",
+        "value": 1e3 / asm_ms, "unit": "systems/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": asm_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"assembly of config {args.config}'s largest x-percolating "
+                               f"cluster, R_flag=1, N={n}, nnz={nnz}, {nblocks} plates"},
+        "network_to_solution_ms": e2e_ms, "network_to_solution_per_s": 1e3 / e2e_ms,
+        "network_copy_s": copy_s, "generate_s": gen_s,
+        "cpu_baseline": {"value": 1.0 / host_s, "unit": "systems/s", "cores": 1,
+                         "kind": "port", "sample": "one host assembly (cicy_net_assemble: "
+                         "triplet list, stable sort, fold) of the same cluster"}}), flush=True)
+
+
 def batch_realisation(k):
     """Systems of realisation k of the batch workload (configuration 4; seed 4_000_000 + k):
     U[2k, 40k] tubes at the density of the 5 um / 40k-tube RVE, the largest cluster
@@ -442,11 +501,12 @@ def main():
                     help="only run the SpMV kernels (both precisions) and a 1 GiB reference "
                          "copy, for hardware-counter runs")
     ap.add_argument("--workload", default="dea",
-                    choices=["dea", "contacts", "backbone", "netio", "batch"],
+                    choices=["dea", "contacts", "backbone", "netio", "batch", "assembly"],
                     help="dea: the conductance solve (the default line); contacts: the tube "
                          "contact search and cluster labels on synthetic wavy tubes; backbone: "
                          "the backbone scan and pruning of the configuration's largest "
-                         "cluster; netio: reading and writing the network files")
+                         "cluster; netio: reading and writing the network files; assembly: "
+                         "the device assembly of the configuration's largest cluster")
     ap.add_argument("--batch-realisations", type=int, default=256,
                     help="realisations of the batch workload (configuration 4)")
     ap.add_argument("--batch-pcg-realisations", type=int, default=8,
@@ -472,6 +532,9 @@ def main():
         return
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "assembly":
+        bench_assembly(args)
         return
     t0 = time.perf_counter()
     net = network.generate(args.config)

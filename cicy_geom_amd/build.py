@@ -23,7 +23,7 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["dea_solver.hip", "dea_setup.hip", "dea_mixed.hip", "dea_batched.hip", "dea_mpnn.hip",
                "dea_resident.hip", "dea_condensed.hip", "dea_core.hip", "dea_backbone.hip",
-               "dea_contacts.hip"]
+               "dea_contacts.hip", "dea_assembly.hip"]
 HIP_FLAGS = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function", f"-I{INCLUDE}", f"-I{CSRC}"]

@@ -132,6 +132,11 @@ struct cicy_net {
     std::vector<std::vector<int64_t>> cluster_gnps;    // plate ids, ascending
     std::vector<std::vector<int64_t>> cluster_mixed;   // tube-plate junction ids
     std::vector<std::vector<int64_t>> cluster_gjunc;   // plate-plate junction ids
+    // flat copies of the lists of lists, filled by cicy_net_arrays_get
+    struct Flat {
+        std::vector<int64_t> ptr[14], ent[14];
+        std::vector<double> thick;
+    } flat;
 };
 
 struct cicy_sss_holder {
@@ -1557,6 +1562,73 @@ cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int famil
 void cicy_sss_free(cicy_sss_view *v) {
     if (!v) return;
     delete static_cast<cicy_sss_holder *>(v->priv);
+}
+
+static_assert(sizeof(Junction) == sizeof(cicy_junction_rec) &&
+                  sizeof(MixedJunction) == sizeof(cicy_junction_rec) &&
+                  sizeof(GnpJunction) == sizeof(cicy_junction_rec),
+              "junction records share one layout");
+
+int cicy_net_arrays_get(cicy_net *net, cicy_net_arrays *out) {
+    if (!net || !out) return 0;
+    auto &f = net->flat;
+    int slot = 0;
+    // a list of lists as offsets + entries, kept in the network until the next call
+    // (`count` lists: missing ones are empty)
+    auto flatten = [&](const std::vector<std::vector<int64_t>> &ll, size_t count,
+                       const int64_t **ptr, const int64_t **ent) {
+        std::vector<int64_t> &p = f.ptr[slot], &e = f.ent[slot];
+        slot++;
+        p.assign(1, 0);
+        e.clear();
+        for (size_t i = 0; i < count; i++) {
+            if (i < ll.size()) e.insert(e.end(), ll[i].begin(), ll[i].end());
+            p.push_back((int64_t)e.size());
+        }
+        *ptr = p.data();
+        *ent = e.data();
+    };
+    auto faces = [&](const std::vector<int64_t> *six, const int64_t **ptr, const int64_t **ent) {
+        std::vector<std::vector<int64_t>> ll(six, six + 6);
+        flatten(ll, 6, ptr, ent);
+    };
+    cicy_net_arrays &a = *out;
+    a = cicy_net_arrays{};
+    a.n_points = (int64_t)net->xyz.size() / 3;
+    a.xyz = net->xyz.data();
+    a.n_tubes = (int64_t)net->radius.size();
+    a.radius = net->radius.data();
+    a.elem_ptr = net->elem_ptr.data();
+    a.elem_pts = net->elem_pts.data();
+    a.n_junctions = (int64_t)net->junctions.size();
+    a.junctions = reinterpret_cast<const cicy_junction_rec *>(net->junctions.data());
+    a.n_gnp_points = (int64_t)net->gxyz.size() / 3;
+    a.gxyz = net->gxyz.data();
+    a.n_gnps = (int64_t)net->gnps.size();
+    f.thick.resize(net->gnps.size());
+    for (size_t g = 0; g < net->gnps.size(); g++) f.thick[g] = net->gnps[g].t;
+    a.gnp_thickness = f.thick.data();
+    flatten(net->gstruct, net->gnps.size(), &a.gstruct_ptr, &a.gstruct);
+    flatten(net->gtri, net->gnps.size(), &a.gtri_ptr, &a.gtri);
+    a.n_mixed = (int64_t)net->mjunctions.size();
+    a.mixed = reinterpret_cast<const cicy_junction_rec *>(net->mjunctions.data());
+    a.n_gjunctions = (int64_t)net->gjunctions.size();
+    a.gjunctions = reinterpret_cast<const cicy_junction_rec *>(net->gjunctions.data());
+    faces(net->bpts, &a.bpts_ptr, &a.bpts);
+    faces(net->bcnt, &a.bcnt_ptr, &a.bcnt);
+    faces(net->gbpts, &a.gbpts_ptr, &a.gbpts);
+    faces(net->gbnd, &a.gbnd_ptr, &a.gbnd);
+    a.n_clusters = (int64_t)net->clusters.size();
+    flatten(net->clusters, net->clusters.size(), &a.cl_tubes_ptr, &a.cl_tubes);
+    flatten(net->cluster_junctions, net->clusters.size(), &a.cl_junc_ptr, &a.cl_junc);
+    flatten(net->cluster_gnps, net->clusters.size(), &a.cl_gnps_ptr, &a.cl_gnps);
+    flatten(net->cluster_mixed, net->clusters.size(), &a.cl_mixed_ptr, &a.cl_mixed);
+    flatten(net->cluster_gjunc, net->clusters.size(), &a.cl_gjunc_ptr, &a.cl_gjunc);
+    for (int d = 0; d < 3; d++) {
+        a.window_lo[d] = 0.0;
+        a.window_hi[d] = net->prm.box;
+    }
+    return 1;
 }
 
 }  // extern "C"

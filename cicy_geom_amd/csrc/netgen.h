@@ -9,6 +9,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "cicy_assembly.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -32,15 +34,7 @@ typedef struct cicy_net_params {
                            // points apart form one junction (Cutoff_dist::min_points)
 } cicy_net_params;
 
-typedef struct cicy_elec_params {
-    double applied_voltage;    // V_app
-    double resistivity_cnt;    // Ohm.um
-    int junction_exponential;  // 1: C1*d*exp(C2*d)/A; 0: constant junction_resistance
-    double junction_resistance;
-    double c1, c2;             // tunnelling constants
-    double resistivity_gnp_t;     // Ohm.um, across the plate (Electric_para)
-    double resistivity_gnp_surf;  // Ohm.um, in the plate
-} cicy_elec_params;
+// cicy_elec_params (the element laws' parameters) is declared in cicy_assembly.h.
 
 typedef struct cicy_net cicy_net;
 
@@ -131,6 +125,10 @@ cicy_sss_view *cicy_net_assemble(cicy_net *net, int64_t k, int r_flag, int famil
                                  const cicy_elec_params *ep, int direction, char *err,
                                  int64_t err_len);
 void cicy_sss_free(cicy_sss_view *v);
+
+// The network as flat arrays (cicy_assembly.h), for cicy_dnet_create.  The pointers stay
+// valid until the next call, a change of the network (cicy_net_backbone) or cicy_net_free.
+int cicy_net_arrays_get(cicy_net *net, cicy_net_arrays *out);
 
 // Backbone of cluster k (Backbone_Network::Determine_backbone_network,
 // Backbone_Network.cpp:10-61), from the voltages [nodes] of its unit-resistor solve (R_flag

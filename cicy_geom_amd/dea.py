@@ -49,6 +49,9 @@ EXPORTED_SYMBOLS = [
     "cicy_sys_create_device", "cicy_sys_set_rhs_device", "cicy_set_setup", "cicy_device_alloc",
     "cicy_device_free", "cicy_device_upload", "cicy_device_download",
 ]
+# Every symbol cicy_assembly.h declares (device assembly, same library).
+ASSEMBLY_SYMBOLS = ["cicy_dnet_create", "cicy_dnet_destroy", "cicy_dnet_assemble",
+                    "cicy_asm_get", "cicy_asm_free"]
 
 
 class SolveStats(ctypes.Structure):
@@ -61,6 +64,46 @@ class SolveStats(ctypes.Structure):
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_} | {
             "status_name": STATUS_NAMES.get(self.status, str(self.status))}
+
+
+class JunctionRec(ctypes.Structure):
+    """cicy_junction_rec: two points, the two particles they lie on, the gap (um)."""
+    _fields_ = [("p1", ctypes.c_int64), ("p2", ctypes.c_int64), ("e1", ctypes.c_int64),
+                ("e2", ctypes.c_int64), ("dist", ctypes.c_double)]
+
+
+def _struct_fields(spec):
+    out = []
+    for name in spec.split():
+        if name.startswith("n_"):
+            out.append((name, ctypes.c_int64))
+        else:
+            out.append((name, _VP))
+    return out
+
+
+class NetArrays(ctypes.Structure):
+    """cicy_net_arrays: a network in flat host arrays (cicy_assembly.h)."""
+    _fields_ = _struct_fields(
+        "n_points xyz n_tubes radius elem_ptr elem_pts n_junctions junctions n_gnp_points gxyz "
+        "n_gnps gnp_thickness gstruct_ptr gstruct gtri_ptr gtri n_mixed mixed n_gjunctions "
+        "gjunctions bpts_ptr bpts bcnt_ptr bcnt gbpts_ptr gbpts gbnd_ptr gbnd n_clusters "
+        "cl_tubes_ptr cl_tubes cl_junc_ptr cl_junc cl_gnps_ptr cl_gnps cl_mixed_ptr cl_mixed "
+        "cl_gjunc_ptr cl_gjunc") + [("window_lo", ctypes.c_double * 3),
+                                     ("window_hi", ctypes.c_double * 3)]
+
+
+class AsmView(ctypes.Structure):
+    """cicy_asm_view: sizes, host vectors and device arrays of one device assembly."""
+    _fields_ = [("nodes", ctypes.c_int64), ("reserved", ctypes.c_int64), ("n", ctypes.c_int64),
+                ("nnz", ctypes.c_int64), ("has_guess", ctypes.c_int),
+                ("direction", ctypes.c_int), ("vef", ctypes.c_double * 8),
+                ("d_row_ptr", _VP), ("d_col_ind", _VP), ("d_values", _VP), ("d_diag", _VP),
+                ("d_b", _VP), ("d_x0", _VP), ("d_r0", _VP), ("n_points", ctypes.c_int64),
+                ("n_gnp_points", ctypes.c_int64), ("d_lmm", _VP), ("d_lmm_gnp", _VP),
+                ("gnp_nodes", ctypes.c_int64), ("n_blocks", ctypes.c_int64), ("blocks", _VP),
+                ("ne", ctypes.c_int64 * 2), ("d_e_bnd", _VP * 2), ("d_e_int", _VP * 2),
+                ("d_e_re", _VP * 2)]
 
 
 _bound = None
@@ -123,6 +166,12 @@ def lib():
     L.cicy_device_upload.argtypes = [_VP, _VP, _VP, ctypes.c_int64]
     L.cicy_device_download.argtypes = [_VP, _VP, _VP, ctypes.c_int64]
     L.cicy_sys_destroy.argtypes = [_VP]
+    L.cicy_dnet_create.argtypes = [_VP, _VP, P(_VP)]
+    L.cicy_dnet_destroy.argtypes = [_VP]
+    L.cicy_dnet_assemble.argtypes = [_VP, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _VP,
+                                     ctypes.c_int, P(_VP)]
+    L.cicy_asm_get.argtypes = [_VP, P(AsmView)]
+    L.cicy_asm_free.argtypes = [_VP]
     L.cicy_sys_size.restype = ctypes.c_int64
     L.cicy_sys_size.argtypes = [_VP, ctypes.c_int]
     L.cicy_sys_set_rhs.argtypes = [_VP, _F64, _VP, _VP]
@@ -431,6 +480,11 @@ class Context:
         return System(self, None, None, None, None,
                       device=(n, nnz, d_row_ptr, d_col_ind, d_values, d_diagonal))
 
+    def device_network(self, arrays: NetArrays, keep=None) -> "DeviceNetwork":
+        """A network copied to the device once (cicy_dnet_create) from its flat arrays;
+        `keep` holds whatever owns the host arrays until the copy is done."""
+        return DeviceNetwork(self, arrays, keep)
+
     def batch(self, systems) -> "Batch":
         """systems: sequence of (row_ptr, col_ind, values, diagonal)."""
         return Batch(self, systems)
@@ -576,6 +630,108 @@ class System:
         if not self.ctx._L.cicy_sys_core(self._h, ctypes.byref(h)) or not h.value:
             return None
         return _Borrowed(self, h)
+
+
+class DeviceNetwork:
+    """A network in device memory (cicy_dnet): every cluster is assembled from it on the
+    device."""
+
+    def __init__(self, ctx: Context, arrays: NetArrays, keep=None):
+        self.ctx = ctx
+        h = _VP()
+        ctx._check(ctx._L.cicy_dnet_create(ctx._h, ctypes.addressof(arrays), ctypes.byref(h)),
+                   "cicy_dnet_create")
+        self._h = h
+        ctx._adopt(self)
+
+    def assemble(self, cluster: int, r_flag: int, family: int, elec=None,
+                 direction: int = -1) -> "DeviceAssembly":
+        """Cluster `cluster`'s system for (r_flag, family), with the boundary element lists
+        of `direction` (0..2; -1 none); elec: network.ElectricParams (default values)."""
+        if elec is None:
+            from .network import ElectricParams
+            elec = ElectricParams()
+        ep = elec._c()
+        h = _VP()
+        self.ctx._check(self.ctx._L.cicy_dnet_assemble(self._h, cluster, r_flag, family,
+                                                       ctypes.addressof(ep), direction,
+                                                       ctypes.byref(h)), "cicy_dnet_assemble")
+        return DeviceAssembly(self.ctx, h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx._L.cicy_dnet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if not _finalizing():
+            self.close()
+
+
+class DeviceAssembly:
+    """One cluster's system assembled on the device (cicy_asm): sizes as attributes,
+    arrays in device memory until close()."""
+
+    _ARRAYS = {"row_ptr": ("d_row_ptr", np.int64, "n1"), "col_ind": ("d_col_ind", np.int64, "nnz"),
+               "values": ("d_values", np.float64, "nnz"), "diag": ("d_diag", np.float64, "n"),
+               "b": ("d_b", np.float64, "n"), "x0": ("d_x0", np.float64, "n"),
+               "r0": ("d_r0", np.float64, "n"), "lmm": ("d_lmm", np.int64, "n_points"),
+               "lmm_gnp": ("d_lmm_gnp", np.int64, "n_gnp_points")}
+
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self._h = h
+        ctx._adopt(self)
+        v = AsmView()
+        ctx._check(ctx._L.cicy_asm_get(h, ctypes.byref(v)), "cicy_asm_get")
+        self.view = v
+        self.nodes, self.reserved, self.n, self.nnz = v.nodes, v.reserved, v.n, v.nnz
+        self.has_guess, self.direction = bool(v.has_guess), v.direction
+        self.vef = np.array(v.vef[:v.reserved])
+        self.gnp_nodes = v.gnp_nodes
+        nb = v.n_blocks
+        self.blocks = (np.ctypeslib.as_array(ctypes.cast(v.blocks, ctypes.POINTER(ctypes.c_int64)),
+                                             (2 * nb,)).reshape(nb, 2).copy()
+                       if nb else np.zeros((0, 2), np.int64))
+
+    def _download(self, ptr, dtype, count):
+        out = np.empty(count, dtype)
+        if count:
+            self.ctx._check(self.ctx._L.cicy_device_download(self.ctx._h, out.ctypes.data_as(_VP),
+                                                             ptr, out.nbytes),
+                            "cicy_device_download")
+        return out
+
+    def get(self, name: str) -> np.ndarray:
+        """One of row_ptr, col_ind, values, diag, b, x0, r0, lmm, lmm_gnp, copied to host."""
+        field, dtype, size = self._ARRAYS[name]
+        count = self.n + 1 if size == "n1" else getattr(self.view, size)
+        return self._download(getattr(self.view, field), dtype, count)
+
+    def boundary_elements(self):
+        """[(n_bnd, n_int, re) for b1, b2] as host arrays (empty without a direction)."""
+        v = self.view
+        return [(self._download(v.d_e_bnd[w], np.int64, v.ne[w]),
+                 self._download(v.d_e_int[w], np.int64, v.ne[w]),
+                 self._download(v.d_e_re[w], np.float64, v.ne[w])) for w in range(2)]
+
+    def system(self) -> "System":
+        """A solver system on the assembled arrays (no copy through the host), with r0, the
+        guess (when the reference uses one) and b set."""
+        v = self.view
+        s = self.ctx.system_device(v.n, v.nnz, v.d_row_ptr, v.d_col_ind, v.d_values, v.d_diag)
+        s.set_rhs_device(v.d_r0, v.d_x0 if v.has_guess else None, v.d_b)
+        s._asm = self
+        return s
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx._L.cicy_asm_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        if not _finalizing():
+            self.close()
 
 
 class _Borrowed(System):

@@ -102,6 +102,7 @@ def _bind():
                                       ctypes.c_int, ctypes.POINTER(_ElecParams),
                                       ctypes.c_int, ctypes.c_char_p, ctypes.c_int64]
     lib.cicy_sss_free.argtypes = [ctypes.POINTER(_SSSView)]
+    lib.cicy_net_arrays_get.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.cicy_net_backbone.argtypes = [ctypes.c_void_p, ctypes.c_int64, F64P, ctypes.c_int64,
                                       I64P, ctypes.c_int64, I64P, ctypes.c_int64, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, F64P,
@@ -415,6 +416,15 @@ class Network:
             raise RuntimeError(err.value.decode())
         return dict(zero_current=out[0], cnt_backbone=out[1], cnt_dead=out[2],
                     gnp_backbone=out[3], gnp_dead=out[4])
+
+    def device(self, ctx):
+        """This network copied to the device of `ctx` (dea.DeviceNetwork) for device
+        assembly; the copy does not follow later changes (backbone trimming)."""
+        from .dea import NetArrays
+        a = NetArrays()
+        if not self._lib.cicy_net_arrays_get(self._h, ctypes.addressof(a)):
+            raise RuntimeError("cicy_net_arrays_get failed")
+        return ctx.device_network(a, keep=self)
 
     def assemble(self, cluster: int, r_flag: int, family: int,
                  elec: ElectricParams | None = None, direction: int = -1,

@@ -26,11 +26,12 @@ def test_dea_header_symbols_exported():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert sorted(dea.EXPORTED_SYMBOLS) == names
+    assert sorted(dea.ASSEMBLY_SYMBOLS) == declared("cicy_assembly.h")
 
 
 def test_all_headers_exported():
-    libs = {"cicy_dea.h": "libcicy_dea.so", "cicy_netio.h": "libcicy_net.so",
-            "cicy_clusters.h": "libcicy_net.so"}
+    libs = {"cicy_dea.h": "libcicy_dea.so", "cicy_assembly.h": "libcicy_dea.so",
+            "cicy_netio.h": "libcicy_net.so", "cicy_clusters.h": "libcicy_net.so"}
     headers = [h for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h")]
     assert sorted(headers) == sorted(libs)  # every C header is checked against its library
     for h in headers:

@@ -241,3 +241,62 @@ def test_tube_only_workloads_unchanged():
     assert a.count("gnps") == 0 and a.count("gnp_points") == 0
     s = network.largest_percolating_system(a, 0)
     assert s.gnp_nodes == 0 and len(s.lmm_gnp) == 0
+
+
+def _flat(ptr, count, dtype=np.int64):
+    import ctypes
+    if count == 0:
+        return np.zeros(0, dtype)
+    ct = ctypes.c_int64 if dtype == np.int64 else ctypes.c_double
+    return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)), (count,)).copy()
+
+
+def test_flat_network_arrays_match_the_accessors(gnet):
+    """cicy_net_arrays_get (the device assembly's input) flattens every list of lists in
+    order: tubes' element points, plates' structure and triangulation, face lists, and each
+    cluster's tubes, plates and junction lists, as the per-item accessors return them."""
+    import ctypes
+    from cicy_geom_amd.dea import JunctionRec, NetArrays
+    a = NetArrays()
+    assert gnet._lib.cicy_net_arrays_get(gnet._h, ctypes.addressof(a))
+    assert a.n_points == gnet.count("points") and a.n_tubes == gnet.count("tubes")
+    assert a.n_gnps == gnet.count("gnps") and a.n_clusters == gnet.count("clusters")
+    assert np.array_equal(_flat(a.xyz, 3 * a.n_points, np.float64), gnet.xyz.reshape(-1))
+    ep = _flat(a.elem_ptr, a.n_tubes + 1)
+    pts = _flat(a.elem_pts, int(ep[-1]))
+    for t in range(0, a.n_tubes, 37):
+        assert np.array_equal(pts[ep[t]:ep[t + 1]], gnet.elements(t))
+    junc = np.ctypeslib.as_array(ctypes.cast(a.junctions, ctypes.POINTER(JunctionRec)),
+                                 (a.n_junctions,))
+    for j in range(0, a.n_junctions, 11):
+        p1, p2, n1, n2, d = gnet.junction(j)
+        assert (junc[j]["p1"], junc[j]["p2"], junc[j]["e1"], junc[j]["e2"], junc[j]["dist"]) == \
+            (p1, p2, n1, n2, d)
+    mixed = np.ctypeslib.as_array(ctypes.cast(a.mixed, ctypes.POINTER(JunctionRec)),
+                                  (a.n_mixed,))
+    for j in range(a.n_mixed):
+        assert tuple(mixed[j].tolist()) == tuple(gnet.mixed_junction(j))
+    gs_ptr = _flat(a.gstruct_ptr, a.n_gnps + 1)
+    gs = _flat(a.gstruct, int(gs_ptr[-1]))
+    gt_ptr = _flat(a.gtri_ptr, a.n_gnps + 1)
+    gt = _flat(a.gtri, int(gt_ptr[-1]))
+    thick = _flat(a.gnp_thickness, a.n_gnps, np.float64)
+    for g in range(a.n_gnps):
+        assert np.array_equal(gs[gs_ptr[g]:gs_ptr[g + 1]], gnet.gnp_structure(g))
+        assert np.array_equal(gt[gt_ptr[g]:gt_ptr[g + 1]],
+                              np.asarray(gnet.gnp_triangulation(g)).reshape(-1))
+        assert thick[g] == gnet.gnp(g)[3]
+    for name, fn in (("bpts", gnet.boundary_points), ("bcnt", gnet.boundary_tubes),
+                     ("gbpts", gnet.gnp_boundary_points), ("gbnd", gnet.boundary_gnps)):
+        fp = _flat(getattr(a, name + "_ptr"), 7)
+        fe = _flat(getattr(a, name), int(fp[-1]))
+        for b in range(6):
+            assert np.array_equal(fe[fp[b]:fp[b + 1]], fn(b)), (name, b)
+    for name, fn in (("cl_tubes", gnet.cluster_tubes), ("cl_junc", gnet.cluster_junctions),
+                     ("cl_gnps", gnet.cluster_gnps), ("cl_mixed", gnet.cluster_mixed_junctions),
+                     ("cl_gjunc", gnet.cluster_gnp_junctions)):
+        cp = _flat(getattr(a, name + "_ptr"), a.n_clusters + 1)
+        ce = _flat(getattr(a, name), int(cp[-1]))
+        for k in range(a.n_clusters):
+            assert np.array_equal(ce[cp[k]:cp[k + 1]], fn(k)), (name, k)
+    assert list(a.window_lo) == [0.0] * 3 and list(a.window_hi) == [gnet.params.box] * 3
