@@ -50,9 +50,16 @@ constexpr double kPi = 3.1415926535897932384626433832795;
 constexpr double kZero = 1.0e-10;
 constexpr int kAsmThreads = 256;
 constexpr long long kNone = LLONG_MAX;
+constexpr int kWave = 64;
+constexpr int kTubesPerBlock = kAsmThreads / kWave;  // one wave per tube
 
 inline unsigned grid_for(long long n) {
     return (unsigned)std::max<long long>(1, (n + kAsmThreads - 1) / kAsmThreads);
+}
+
+// work-groups for one wave per tube
+inline unsigned grid_tubes(long long T) {
+    return (unsigned)std::max<long long>(1, (T + kTubesPerBlock - 1) / kTubesPerBlock);
 }
 
 // the network's arrays on the device
@@ -152,9 +159,10 @@ __global__ void k_mark(int64_t *lmm, const int64_t *pts, long long count, int64_
 }
 
 // per cluster tube: new nodes (the end points only when no face reserved them) and
-// resistors (one for the three-point element whose ends share a reserved node); entry T = 0
+// resistors (one for the three-point element whose ends share a reserved node); entry T = 0.
+// ends[i]: bit 0 the first end is new, bit 1 the last end is new.
 __global__ void k_tube_count(NetDev d, const int64_t *ct, long long T, const int64_t *lmm,
-                             int64_t *nodes, int64_t *edges, AsmErr *err) {
+                             int64_t *nodes, int64_t *edges, unsigned char *ends, AsmErr *err) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i > T) return;
     if (i == T) {
@@ -169,63 +177,89 @@ __global__ void k_tube_count(NetDev d, const int64_t *ct, long long T, const int
         atomicMin(&err->bad_tube, i);
         nodes[i] = 0;
         edges[i] = 0;
+        ends[i] = 0;
         return;
     }
     const int64_t l0 = lmm[d.elem_pts[a]], l1 = lmm[d.elem_pts[b - 1]];
     nodes[i] = (ne - 2) + (l0 < 0 ? 1 : 0) + (l1 < 0 ? 1 : 0);
     edges[i] = (ne == 3 && l0 >= 0 && l0 == l1) ? 1 : ne - 1;
+    ends[i] = (unsigned char)((l0 < 0 ? 1 : 0) | (l1 < 0 ? 2 : 0));
 }
 
-// LM_matrix_for_cnts: the tube's points from its first node on, in element order
-__global__ void k_tube_number(NetDev d, const int64_t *ct, long long T, int64_t *lmm,
-                              const int64_t *base, int64_t first) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// LM_matrix_for_cnts: the tube's points from its first node on, in element order (one wave
+// per tube, a lane per element point)
+__global__ void __launch_bounds__(kAsmThreads)
+k_tube_number(NetDev d, const int64_t *ct, long long T, int64_t *lmm, const int64_t *base,
+              const unsigned char *ends, int64_t first) {
+    const long long i = (long long)blockIdx.x * kTubesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
     if (i >= T) return;
     const int64_t t = ct[i];
     const int64_t a = d.elem_ptr[t], b = d.elem_ptr[t + 1];
-    if (b - a <= 1) return;
-    int64_t g = first + base[i];
-    const int64_t pf = d.elem_pts[a];
-    if (lmm[pf] < 0) lmm[pf] = g++;
-    for (int64_t q = a + 1; q < b - 1; q++) lmm[d.elem_pts[q]] = g++;
-    const int64_t pl = d.elem_pts[b - 1];
-    if (lmm[pl] < 0) lmm[pl] = g++;
+    const int64_t ne = b - a;
+    if (ne <= 1) return;
+    const int64_t g0 = first + base[i];
+    const int e = ends[i];
+    const int64_t off = (e & 1) ? 1 : 0;
+    for (int64_t q = lane; q < ne; q += kWave) {
+        const int64_t p = d.elem_pts[a + q];
+        if (q == 0) {
+            if (e & 1) lmm[p] = g0;
+        } else if (q == ne - 1) {
+            if (e & 2) lmm[p] = g0 + off + (ne - 2);
+        } else {
+            lmm[p] = g0 + off + (q - 1);
+        }
+    }
 }
 
-// per cluster plate: claim its unnumbered points (-2) and count them, and count its
-// triangulation edges; entry G = 0
-__global__ void k_gnp_claim(NetDev d, const int64_t *cg, long long G, int64_t *lmm_g,
-                            int64_t *nodes, int64_t *tris) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > G) return;
+// per cluster plate (one work-group each): its points that no face reserved, and its
+// triangulation edges; entry G = 0 (written by the last work-group)
+__global__ void __launch_bounds__(kAsmThreads)
+k_gnp_count(NetDev d, const int64_t *cg, long long G, const int64_t *lmm_g, int64_t *nodes,
+            int64_t *tris) {
+    typedef hipcub::BlockReduce<int, kAsmThreads> Reduce;
+    __shared__ typename Reduce::TempStorage tmp;
+    const long long i = blockIdx.x;
     if (i == G) {
-        nodes[i] = 0;
-        tris[i] = 0;
+        if (threadIdx.x == 0) {
+            nodes[i] = 0;
+            tris[i] = 0;
+        }
         return;
     }
     const int64_t gp = cg[i];
-    int64_t c = 0;
-    for (int64_t q = d.gstruct_ptr[gp]; q < d.gstruct_ptr[gp + 1]; q++) {
-        const int64_t p = d.gstruct[q];
-        if (lmm_g[p] == -1) {
-            lmm_g[p] = -2;
-            c++;
-        }
+    int c = 0;
+    for (int64_t q = d.gstruct_ptr[gp] + threadIdx.x; q < d.gstruct_ptr[gp + 1]; q += blockDim.x)
+        c += lmm_g[d.gstruct[q]] < 0 ? 1 : 0;
+    c = Reduce(tmp).Sum(c);
+    if (threadIdx.x == 0) {
+        nodes[i] = c;
+        tris[i] = (d.gtri_ptr[gp + 1] - d.gtri_ptr[gp]) / 2;
     }
-    nodes[i] = c;
-    tris[i] = (d.gtri_ptr[gp + 1] - d.gtri_ptr[gp]) / 2;
 }
 
-// LM_matrix_for_gnps: the claimed points in structure order
-__global__ void k_gnp_number(NetDev d, const int64_t *cg, long long G, int64_t *lmm_g,
-                             const int64_t *base, int64_t first) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// LM_matrix_for_gnps: the plate's unreserved points numbered in structure order (a block
+// scan per chunk of points)
+__global__ void __launch_bounds__(kAsmThreads)
+k_gnp_number(NetDev d, const int64_t *cg, long long G, int64_t *lmm_g, const int64_t *base,
+             int64_t first) {
+    typedef hipcub::BlockScan<int, kAsmThreads> Scan;
+    __shared__ typename Scan::TempStorage tmp;
+    const long long i = blockIdx.x;
     if (i >= G) return;
     const int64_t gp = cg[i];
     int64_t g = first + base[i];
-    for (int64_t q = d.gstruct_ptr[gp]; q < d.gstruct_ptr[gp + 1]; q++) {
-        const int64_t p = d.gstruct[q];
-        if (lmm_g[p] == -2) lmm_g[p] = g++;
+    for (int64_t q0 = d.gstruct_ptr[gp]; q0 < d.gstruct_ptr[gp + 1]; q0 += blockDim.x) {
+        const int64_t q = q0 + threadIdx.x;
+        const bool in = q < d.gstruct_ptr[gp + 1];
+        const int64_t p = in ? d.gstruct[q] : 0;
+        const int f = (in && lmm_g[p] < 0) ? 1 : 0;
+        int off, total;
+        Scan(tmp).ExclusiveSum(f, off, total);
+        if (f) lmm_g[p] = g + off;
+        g += total;
+        __syncthreads();
     }
 }
 
@@ -280,20 +314,24 @@ __device__ inline void add_edge(const Recs &r, long long slot, int64_t n1, int64
     else put_rec(r, slot, hi, lo, -g, 0);
 }
 
-// Fill_2d_matrices_cnts (:489-553) with Three_points_in_element (:555-609)
-__global__ void k_edges_tubes(NetDev d, const int64_t *ct, long long T, const int64_t *lmm,
-                              const int64_t *base, int r_flag, cicy_elec_params ep, Recs r) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// Fill_2d_matrices_cnts (:489-553) with Three_points_in_element (:555-609): one wave per
+// tube, a lane per element (consecutive slots, so the records are written coalesced)
+__global__ void __launch_bounds__(kAsmThreads)
+k_edges_tubes(NetDev d, const int64_t *ct, long long T, const int64_t *lmm, const int64_t *base,
+              int r_flag, cicy_elec_params ep, Recs r) {
+    const long long i = (long long)blockIdx.x * kTubesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
     if (i >= T) return;
     const int64_t t = ct[i];
     const int64_t a = d.elem_ptr[t], b = d.elem_ptr[t + 1];
     const int64_t *e = d.elem_pts + a;
     const int64_t ne = b - a;
     const double rad = d.radius[t];
-    long long slot = base[i];
+    const long long slot0 = base[i];
     // the three-point element (its ends share a reserved node) took one slot in the count;
     // deciding from the slots keeps the writes inside them whatever the network holds
     if (ne == 3 && base[i + 1] - base[i] == 1) {
+        if (lane != 0) return;
         double g1 = 1.0, g2 = 1.0;
         if (r_flag == 1) {
             g1 = 1 / resistance_cnt(d.xyz, e[0], e[1], rad, ep.resistivity_cnt);
@@ -301,14 +339,15 @@ __global__ void k_edges_tubes(NetDev d, const int64_t *ct, long long T, const in
         }
         const int64_t n1 = lmm[e[0]], n2 = lmm[e[1]];
         const double s = g1 + g2;
-        r.dnode[2 * slot] = (int)n1;
-        r.dval[2 * slot] = s;
-        r.dnode[2 * slot + 1] = (int)n2;
-        r.dval[2 * slot + 1] = s;
-        put_rec(r, slot, n2, n1, -(g1 + g2), 0);
+        r.dnode[2 * slot0] = (int)n1;
+        r.dval[2 * slot0] = s;
+        r.dnode[2 * slot0 + 1] = (int)n2;
+        r.dval[2 * slot0 + 1] = s;
+        put_rec(r, slot0, n2, n1, -(g1 + g2), 0);
         return;
     }
-    for (int64_t q = 1; q < ne; q++, slot++) {
+    for (int64_t q = 1 + lane; q < ne; q += kWave) {
+        const long long slot = slot0 + q - 1;
         double re = (r_flag == 1) ? resistance_cnt(d.xyz, e[q - 1], e[q], rad, ep.resistivity_cnt)
                                   : 1.0;
         double g = (r_flag == 1) ? 1 / re : re;
@@ -360,19 +399,20 @@ __global__ void k_edges_mixed(NetDev d, const int64_t *cm, long long M, const in
 }
 
 // Fill_2d_matrices_gnp (:1021-1196): each plate's triangulation edges from the last to the
-// first; an end without a tube junction takes half the plate thickness as its radius
-__global__ void k_edges_gnp(NetDev d, const int64_t *cg, long long G, const int64_t *lmm_g,
-                            const int *owner, const int64_t *cm, const int64_t *base,
-                            long long slot0, int r_flag, cicy_elec_params ep, Recs r,
-                            AsmErr *err) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// first (one work-group per plate, edge j at slot base + ntri - 1 - j); an end without a tube
+// junction takes half the plate thickness as its radius
+__global__ void __launch_bounds__(kAsmThreads)
+k_edges_gnp(NetDev d, const int64_t *cg, long long G, const int64_t *lmm_g, const int *owner,
+            const int64_t *cm, const int64_t *base, long long slot0, int r_flag,
+            cicy_elec_params ep, Recs r, AsmErr *err) {
+    const long long i = blockIdx.x;
     if (i >= G) return;
     const int64_t gp = cg[i];
     const int64_t *tri = d.gtri + d.gtri_ptr[gp];
     const int64_t ntri = (d.gtri_ptr[gp + 1] - d.gtri_ptr[gp]) / 2;
     const double half_t = d.thick[gp] / 2;
-    long long slot = slot0 + base[i];
-    for (int64_t j = ntri - 1; j >= 0; j--, slot++) {
+    for (int64_t j = threadIdx.x; j < ntri; j += blockDim.x) {
+        const long long slot = slot0 + base[i] + (ntri - 1 - j);
         const int64_t v1 = tri[2 * j], v2 = tri[2 * j + 1];
         const int64_t n1 = lmm_g[v1], n2 = lmm_g[v2];
         if (n1 < 0 || n2 < 0) {
@@ -432,7 +472,7 @@ template <bool WRITE>
 __global__ void k_fold(const unsigned long long *skey, const int *sslot, const double *val,
                        const unsigned char *rule, long long S, const int *row_start, int cb,
                        int64_t rd, int64_t nodes, Vef vef, int64_t *cnt, const int64_t *row_ptr,
-                       int64_t *col_ind, double *values, int *erow, double *b) {
+                       int64_t *col_ind, double *values, int *erow, int *col32, double *b) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = nodes - rd;
     if (t > n) return;
@@ -463,7 +503,10 @@ __global__ void k_fold(const unsigned long long *skey, const int *sslot, const d
                 if (WRITE) {
                     col_ind[pos] = col - rd;
                     values[pos] = v;
-                    if (erow) erow[pos] = (int)t;
+                    if (erow) {
+                        erow[pos] = (int)t;
+                        col32[pos] = (int)(col - rd);
+                    }
                 }
                 pos++;
                 c++;
@@ -476,32 +519,38 @@ __global__ void k_fold(const unsigned long long *skey, const int *sslot, const d
     else cnt[t] = c;
 }
 
+// first index of each key below `limit` in a sorted int array
+__global__ void k_heads(const int *keys, long long count, long long limit, int *start) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int k = keys[i];
+    if (k < limit && (i == 0 || keys[i - 1] != k)) start[k] = (int)i;
+}
+
 // each free node's diagonal: its contributions in insertion order from 0
-__global__ void k_diag(const int *sdnode, const double *sdval, long long S2, int64_t rd,
-                       int64_t nodes, double *diag, AsmErr *err) {
+__global__ void k_diag(const int *sdnode, const double *sdval, const int *dstart, long long S2,
+                       int64_t rd, int64_t nodes, double *diag, AsmErr *err) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nodes - rd) return;
     const int node = (int)(rd + t);
-    long long lo = 0, hi = S2;
-    while (lo < hi) {
-        const long long mid = (lo + hi) >> 1;
-        if (sdnode[mid] < node) lo = mid + 1;
-        else hi = mid;
-    }
     double s = 0.0;
-    for (long long i = lo; i < S2 && sdnode[i] == node; i++) s += sdval[i];
+    const int lo = dstart[node];
+    if (lo >= 0)
+        for (long long i = lo; i < S2 && sdnode[i] == node; i++) s += sdval[i];
     diag[t] = s;
     if (fabs(s) < kZero) atomicMin(&err->floating, t);
 }
 
 // Initial_guess_for_cnt_nodes (:1434-1470): V_app / L * (max - coordinate) at each free tube
-// node
-__global__ void k_ramp(NetDev d, const int64_t *ct, long long T, const int64_t *lmm, int64_t rd,
-                       int axis, double vpc, double max_coord, double *x0) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// node (one wave per tube, a lane per element point)
+__global__ void __launch_bounds__(kAsmThreads)
+k_ramp(NetDev d, const int64_t *ct, long long T, const int64_t *lmm, int64_t rd, int axis,
+       double vpc, double max_coord, double *x0) {
+    const long long i = (long long)blockIdx.x * kTubesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
     if (i >= T) return;
     const int64_t t = ct[i];
-    for (int64_t q = d.elem_ptr[t]; q < d.elem_ptr[t + 1]; q++) {
+    for (int64_t q = d.elem_ptr[t] + lane; q < d.elem_ptr[t + 1]; q += kWave) {
         const int64_t pt = d.elem_pts[q];
         const int64_t nd = lmm[pt];
         if (nd >= 2) {
@@ -514,23 +563,19 @@ __global__ void k_ramp(NetDev d, const int64_t *ct, long long T, const int64_t *
 // Update_residual_vector (:1502-1535): r0 = b - A x0, each node's product summed as the
 // reference's row loop reaches it: diagonal, own row, then the later rows holding it
 __global__ void k_r0(const int64_t *row_ptr, const int64_t *col_ind, const double *values,
-                     const double *diag, const double *x0, const double *b,
-                     const int64_t *scol, const int *sent, const int *erow, long long nnz,
+                     const double *diag, const double *x0, const double *b, const int *scol,
+                     const int *sent, const int *cstart, const int *erow, long long nnz,
                      int64_t n, double *r0) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double ax = diag[i] * x0[i];
     for (int64_t j = row_ptr[i]; j < row_ptr[i + 1]; j++) ax = ax + values[j] * x0[col_ind[j]];
-    long long lo = 0, hi = nnz;
-    while (lo < hi) {
-        const long long mid = (lo + hi) >> 1;
-        if (scol[mid] < i) lo = mid + 1;
-        else hi = mid;
-    }
-    for (long long e = lo; e < nnz && scol[e] == i; e++) {
-        const int j = sent[e];
-        ax = ax + values[j] * x0[erow[j]];
-    }
+    const int lo = cstart[i];
+    if (lo >= 0)
+        for (long long e = lo; e < nnz && scol[e] == i; e++) {
+            const int j = sent[e];
+            ax = ax + values[j] * x0[erow[j]];
+        }
     r0[i] = b[i] - ax;
 }
 
@@ -554,86 +599,108 @@ __device__ bool in_cluster(const BndDev &bd, int64_t gp) {
     return lo < bd.G && bd.cg[lo] == gp;
 }
 
-// Currents_through_boundary_cnts (Electrical_analysis.cpp:499-545) and Currents_through_
-// boundary_gnps (:607-675): the elements that touch the face's reserved node, tubes first
-// (each tube's first then last element), then plates' triangulation edges.  WRITE = false
-// counts per list entry (entry nt + ng = 0), WRITE = true writes at the scanned positions.
+// Currents_through_boundary_cnts (Electrical_analysis.cpp:499-545): each face tube's first
+// then last element when its end node is 0 or 1 (one thread per tube).  WRITE = false
+// counts per tube (entry nt + ng = 0, written here), WRITE = true writes at the scanned
+// positions.
 template <bool WRITE>
-__global__ void k_bnd(NetDev d, BndDev bd, const int64_t *lmm, const int64_t *lmm_g,
-                      const int *owner, const int64_t *cm, cicy_elec_params ep, int64_t *cnt,
-                      const int64_t *pos0, int64_t *e_bnd, int64_t *e_int, double *e_re,
-                      AsmErr *err) {
+__global__ void k_bnd_tubes(NetDev d, BndDev bd, const int64_t *lmm, cicy_elec_params ep,
+                            int64_t *cnt, const int64_t *pos0, int64_t *e_bnd, int64_t *e_int,
+                            double *e_re) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long tot = bd.nt + bd.ng;
-    if (i > tot) return;
-    if (i == tot) {
-        if (!WRITE) cnt[tot] = 0;
+    if (i > bd.nt) return;
+    if (i == bd.nt) {
+        if (!WRITE) cnt[bd.nt + bd.ng] = 0;
         return;
     }
     int64_t c = 0;
     int64_t pos = WRITE ? pos0[i] : 0;
-    if (i < bd.nt) {
-        const int64_t t = bd.tubes[i];
-        const int64_t a = d.elem_ptr[t], b = d.elem_ptr[t + 1];
-        if (b - a >= 2) {
-            const int64_t *e = d.elem_pts + a;
-            const int64_t fr[2][2] = {{e[0], e[1]}, {e[b - a - 1], e[b - a - 2]}};
-            for (int side = 0; side < 2; side++) {
-                const int64_t p1 = fr[side][0], p2 = fr[side][1];
-                const int64_t n1 = lmm[p1];
-                if (n1 < 0 || n1 > 1) continue;
-                const int64_t n2 = lmm[p2];
-                if (n2 < 0) continue;
-                if (WRITE) {
-                    const double rad = d.radius[t];
-                    const double re = p1 > p2 ? resistance_cnt(d.xyz, p2, p1, rad, ep.resistivity_cnt)
-                                              : resistance_cnt(d.xyz, p1, p2, rad, ep.resistivity_cnt);
-                    e_bnd[pos] = n1;
-                    e_int[pos] = n2;
-                    e_re[pos] = re;
-                    pos++;
-                }
-                c++;
+    const int64_t t = bd.tubes[i];
+    const int64_t a = d.elem_ptr[t], b = d.elem_ptr[t + 1];
+    if (b - a >= 2) {
+        const int64_t *e = d.elem_pts + a;
+        const int64_t fr[2][2] = {{e[0], e[1]}, {e[b - a - 1], e[b - a - 2]}};
+        for (int side = 0; side < 2; side++) {
+            const int64_t p1 = fr[side][0], p2 = fr[side][1];
+            const int64_t n1 = lmm[p1];
+            if (n1 < 0 || n1 > 1) continue;
+            const int64_t n2 = lmm[p2];
+            if (n2 < 0) continue;
+            if (WRITE) {
+                const double rad = d.radius[t];
+                const double re = p1 > p2 ? resistance_cnt(d.xyz, p2, p1, rad, ep.resistivity_cnt)
+                                          : resistance_cnt(d.xyz, p1, p2, rad, ep.resistivity_cnt);
+                e_bnd[pos] = n1;
+                e_int[pos] = n2;
+                e_re[pos] = re;
+                pos++;
             }
+            c++;
         }
-    } else {
-        const long long ip = i - bd.nt;
-        const int64_t gp = bd.gnps[ip];
-        if (in_cluster(bd, gp)) {
-            const int64_t node = bd.w;
-            const int64_t *tri = d.gtri + d.gtri_ptr[gp];
-            const int64_t ntri = (d.gtri_ptr[gp + 1] - d.gtri_ptr[gp]) / 2;
-            const double half_t = d.thick[gp] / 2;
-            for (int64_t j = 0; j < ntri; j++) {
-                const int64_t v1 = tri[2 * j], v2 = tri[2 * j + 1];
-                const int64_t na = lmm_g[v1], nb = lmm_g[v2];
-                if (na > 1 && nb > 1) continue;
+    }
+    if (!WRITE) cnt[i] = c;
+}
+
+// Currents_through_boundary_gnps (Electrical_analysis.cpp:607-675): the triangulation edges
+// of the face's plates in the cluster that touch node 0 or 1, oriented from node w (one
+// work-group per face plate; positions inside a plate by a block scan per chunk of edges).
+// The count pass records the first error in the host's order.
+template <bool WRITE>
+__global__ void __launch_bounds__(kAsmThreads)
+k_bnd_plates(NetDev d, BndDev bd, const int64_t *lmm_g, const int *owner, const int64_t *cm,
+             cicy_elec_params ep, int64_t *cnt, const int64_t *pos0, int64_t *e_bnd,
+             int64_t *e_int, double *e_re, AsmErr *err) {
+    typedef hipcub::BlockScan<int, kAsmThreads> Scan;
+    __shared__ typename Scan::TempStorage tmp;
+    const long long ip = blockIdx.x;
+    if (ip >= bd.ng) return;
+    const int64_t gp = bd.gnps[ip];
+    int64_t pos = WRITE ? pos0[bd.nt + ip] : 0;
+    if (!in_cluster(bd, gp)) {
+        if (!WRITE && threadIdx.x == 0) cnt[bd.nt + ip] = 0;
+        return;
+    }
+    const int64_t node = bd.w;
+    const int64_t *tri = d.gtri + d.gtri_ptr[gp];
+    const int64_t ntri = (d.gtri_ptr[gp + 1] - d.gtri_ptr[gp]) / 2;
+    const double half_t = d.thick[gp] / 2;
+    for (int64_t j0 = 0; j0 < ntri; j0 += blockDim.x) {
+        const int64_t j = j0 + threadIdx.x;
+        int f = 0;
+        int64_t n1 = 0, n2 = 0;
+        double re = 0.0;
+        if (j < ntri) {
+            const int64_t v1 = tri[2 * j], v2 = tri[2 * j + 1];
+            const int64_t na = lmm_g[v1], nb = lmm_g[v2];
+            if (!(na > 1 && nb > 1)) {
                 const long long ekey = ((long long)bd.w << 61) | ((long long)ip << 30) |
                                        ((long long)j << 1);
                 if (na == nb) {
                     if (!WRITE) atomicMin(&err->bnd, ekey);
-                    break;
+                } else {
+                    n1 = (na == node) ? na : nb;
+                    n2 = (na == node) ? nb : na;
+                    const int o1 = owner[v1], o2 = owner[v2];
+                    const double rad1 = o1 >= 0 ? d.radius[d.mixed[cm[o1]].e1] : half_t;
+                    const double rad2 = o2 >= 0 ? d.radius[d.mixed[cm[o2]].e1] : half_t;
+                    if (resistance_gnp(d.gxyz + 3 * v1, d.gxyz + 3 * v2, rad1, rad2, ep, &re))
+                        f = 1;
+                    else if (!WRITE)
+                        atomicMin(&err->bnd, ekey | 1);
                 }
-                const int64_t n1 = (na == node) ? na : nb, n2 = (na == node) ? nb : na;
-                const int o1 = owner[v1], o2 = owner[v2];
-                const double rad1 = o1 >= 0 ? d.radius[d.mixed[cm[o1]].e1] : half_t;
-                const double rad2 = o2 >= 0 ? d.radius[d.mixed[cm[o2]].e1] : half_t;
-                double re;
-                if (!resistance_gnp(d.gxyz + 3 * v1, d.gxyz + 3 * v2, rad1, rad2, ep, &re)) {
-                    if (!WRITE) atomicMin(&err->bnd, ekey | 1);
-                    break;
-                }
-                if (WRITE) {
-                    e_bnd[pos] = n1;
-                    e_int[pos] = n2;
-                    e_re[pos] = re;
-                    pos++;
-                }
-                c++;
             }
         }
+        int off, total;
+        Scan(tmp).ExclusiveSum(f, off, total);
+        if (WRITE && f) {
+            e_bnd[pos + off] = n1;
+            e_int[pos + off] = n2;
+            e_re[pos + off] = re;
+        }
+        pos += total;
+        __syncthreads();
     }
-    if (!WRITE) cnt[i] = c;
+    if (!WRITE && threadIdx.x == 0) cnt[bd.nt + ip] = pos;
 }
 
 // up to 8 int64 entries of device arrays, gathered for one read-back
@@ -861,8 +928,8 @@ int cicy_dnet_create(cicy_ctx *ctx, const cicy_net_arrays *a, cicy_dnet **out) {
                 c.fail(m);
                 break;
             }
-    // a point belongs to one tube and a plate point to one plate (the numbering runs one
-    // thread per tube and per plate)
+    // a point belongs to one tube and a plate point to one plate, once (the numbering runs
+    // per tube and per plate in parallel)
     if (c.ok) {
         std::vector<unsigned char> seen((size_t)a->n_points, 0);
         for (int64_t q = 0; q < n_elem; q++) {
@@ -885,10 +952,10 @@ int cicy_dnet_create(cicy_ctx *ctx, const cicy_net_arrays *a, cicy_dnet **out) {
         for (int64_t g = 0; c.ok && g < a->n_gnps; g++)
             for (int64_t q = a->gstruct_ptr[g]; q < a->gstruct_ptr[g + 1]; q++) {
                 int64_t &o = plate[(size_t)a->gstruct[q]];
-                if (o >= 0 && o != g) {
+                if (o >= 0) {
                     char m[160];
-                    snprintf(m, sizeof(m), "plate point %lld is in the structure of two plates",
-                             (long long)a->gstruct[q]);
+                    snprintf(m, sizeof(m), "plate point %lld appears twice in the plates' "
+                             "structures", (long long)a->gstruct[q]);
                     c.fail(m);
                     break;
                 }
@@ -1060,12 +1127,14 @@ int cicy_dnet_assemble(cicy_dnet *dn, int64_t k, int r_flag, int family,
     // node and resistor counts, scans
     int64_t *tn = nullptr, *te = nullptr, *tnb = nullptr, *teb = nullptr, *gn = nullptr,
             *gt = nullptr, *gnb = nullptr, *gtb = nullptr;
-    if (!ps.get(tn, T + 1) || !ps.get(te, T + 1) || !ps.get(tnb, T + 1) || !ps.get(teb, T + 1) ||
+    unsigned char *ends = nullptr;
+    if (!ps.get(ends, T) || !ps.get(tn, T + 1) || !ps.get(te, T + 1) || !ps.get(tnb, T + 1) ||
+        !ps.get(teb, T + 1) ||
         !ps.get(gn, G + 1) || !ps.get(gt, G + 1) || !ps.get(gnb, G + 1) || !ps.get(gtb, G + 1))
         return fail();
     hipLaunchKernelGGL(k_tube_count, dim3(grid_for(T + 1)), dim3(kAsmThreads), 0, st, d, ct, T, lmm,
-                       tn, te, err);
-    hipLaunchKernelGGL(k_gnp_claim, dim3(grid_for(G + 1)), dim3(kAsmThreads), 0, st, d, cg, G,
+                       tn, te, ends, err);
+    hipLaunchKernelGGL(k_gnp_count, dim3((unsigned)(G + 1)), dim3(kAsmThreads), 0, st, d, cg, G,
                        lmm_g, gn, gt);
     if (!run.launched("k_tube_count") || !run.scan(tn, tnb, T + 1) || !run.scan(te, teb, T + 1) ||
         !run.scan(gn, gnb, G + 1) || !run.scan(gt, gtb, G + 1))
@@ -1092,10 +1161,11 @@ int cicy_dnet_assemble(cicy_dnet *dn, int64_t k, int r_flag, int family,
         return fail();
     }
     mark("numbering counts");
-    hipLaunchKernelGGL(k_tube_number, dim3(grid_for(T)), dim3(kAsmThreads), 0, st, d, ct, T, lmm,
-                       tnb, rd);
-    hipLaunchKernelGGL(k_gnp_number, dim3(grid_for(G)), dim3(kAsmThreads), 0, st, d, cg, G, lmm_g,
-                       gnb, first_gnp);
+    hipLaunchKernelGGL(k_tube_number, dim3(grid_tubes(T)), dim3(kAsmThreads), 0, st, d, ct, T,
+                       lmm, tnb, ends, rd);
+    if (G > 0)
+        hipLaunchKernelGGL(k_gnp_number, dim3((unsigned)G), dim3(kAsmThreads), 0, st, d, cg, G,
+                           lmm_g, gnb, first_gnp);
     int *owner = nullptr;
     if (!ps.get(owner, dn->n_gpts)) return fail();
     hipLaunchKernelGGL(k_fill32, dim3(grid_for(dn->n_gpts)), dim3(kAsmThreads), 0, st, owner,
@@ -1119,7 +1189,7 @@ int cicy_dnet_assemble(cicy_dnet *dn, int64_t k, int r_flag, int family,
         return fail();
     long long s0 = 0;
     if (T > 0)
-        hipLaunchKernelGGL(k_edges_tubes, dim3(grid_for(T)), dim3(kAsmThreads), 0, st, d, ct, T, lmm,
+        hipLaunchKernelGGL(k_edges_tubes, dim3(grid_tubes(T)), dim3(kAsmThreads), 0, st, d, ct, T, lmm,
                            teb, r_flag, ep, r);
     s0 += Et;
     if (J > 0)
@@ -1131,8 +1201,8 @@ int cicy_dnet_assemble(cicy_dnet *dn, int64_t k, int r_flag, int family,
                            lmm_g, s0, rd, r_flag, ep, r);
     s0 += M;
     if (G > 0)
-        hipLaunchKernelGGL(k_edges_gnp, dim3(grid_for(G)), dim3(kAsmThreads), 0, st, d, cg, G, lmm_g,
-                           owner, cm, gtb, s0, r_flag, ep, r, err);
+        hipLaunchKernelGGL(k_edges_gnp, dim3((unsigned)G), dim3(kAsmThreads), 0, st, d, cg, G,
+                           lmm_g, owner, cm, gtb, s0, r_flag, ep, r, err);
     s0 += Eg;
     if (GJ > 0)
         hipLaunchKernelGGL(k_edges_gjunc, dim3(grid_for(GJ)), dim3(kAsmThreads), 0, st, d, cgj, GJ,
@@ -1160,7 +1230,7 @@ int cicy_dnet_assemble(cicy_dnet *dn, int64_t k, int r_flag, int family,
                        row_start);
     hipLaunchKernelGGL(k_fold<false>, dim3(grid_for(n + 1)), dim3(kAsmThreads), 0, st, skey, sslot,
                        r.val, r.rule, S, row_start, cb, rd, nodes, vef, cnt, nullptr, nullptr,
-                       nullptr, nullptr, nullptr);
+                       nullptr, nullptr, nullptr, nullptr);
     if (!run.launched("k_fold") || !run.scan(cnt, row_ptr, n + 1) ||
         !run.pick(pickbuf, {{row_ptr, n}, {word(&err->gnp), 0}}, h))
         return fail();
@@ -1172,15 +1242,20 @@ int cicy_dnet_assemble(cicy_dnet *dn, int64_t k, int r_flag, int family,
     const long long nnz = h[0];
     int64_t *col_ind = nullptr;
     double *values = nullptr, *b = nullptr, *diag = nullptr, *x0 = nullptr, *r0 = nullptr;
-    int *erow = nullptr;
+    int *erow = nullptr, *col32 = nullptr, *dstart = nullptr;
     if (!run.out(col_ind, nnz) || !run.out(values, nnz) || !run.out(b, n) || !run.out(diag, n) ||
-        !run.out(x0, n) || !run.out(r0, n) || (has_guess && !ps.get(erow, nnz)))
+        !run.out(x0, n) || !run.out(r0, n) || !ps.get(dstart, nodes) ||
+        (has_guess && (!ps.get(erow, nnz) || !ps.get(col32, nnz))))
         return fail();
     hipLaunchKernelGGL(k_fold<true>, dim3(grid_for(n + 1)), dim3(kAsmThreads), 0, st, skey, sslot,
                        r.val, r.rule, S, row_start, cb, rd, nodes, vef, nullptr, row_ptr, col_ind,
-                       values, erow, b);
-    hipLaunchKernelGGL(k_diag, dim3(grid_for(n)), dim3(kAsmThreads), 0, st, sdnode, sdval, 2 * S, rd,
-                       nodes, diag, err);
+                       values, erow, col32, b);
+    hipLaunchKernelGGL(k_fill32, dim3(grid_for(nodes)), dim3(kAsmThreads), 0, st, dstart,
+                       (long long)nodes, -1);
+    hipLaunchKernelGGL(k_heads, dim3(grid_for(2 * S)), dim3(kAsmThreads), 0, st, sdnode, 2 * S,
+                       (long long)nodes, dstart);
+    hipLaunchKernelGGL(k_diag, dim3(grid_for(n)), dim3(kAsmThreads), 0, st, sdnode, sdval, dstart,
+                       2 * S, rd, nodes, diag, err);
     if (!run.launched("rows")) return fail();
     mark("rows and diagonal");
 
@@ -1192,16 +1267,21 @@ int cicy_dnet_assemble(cicy_dnet *dn, int64_t k, int r_flag, int family,
         const double l_win = dn->hi[axis] - dn->lo[axis], max_coord = dn->hi[axis];
         const double vpc = ep.applied_voltage / l_win;
         if (T > 0)
-            hipLaunchKernelGGL(k_ramp, dim3(grid_for(T)), dim3(kAsmThreads), 0, st, d, ct, T, lmm, rd,
+            hipLaunchKernelGGL(k_ramp, dim3(grid_tubes(T)), dim3(kAsmThreads), 0, st, d, ct, T, lmm, rd,
                                axis, vpc, max_coord, x0);
-        int *eiota = nullptr, *sent = nullptr;
-        int64_t *scol = nullptr;
-        if (!ps.get(eiota, nnz) || !ps.get(sent, nnz) || !ps.get(scol, nnz)) return fail();
-        hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz)), dim3(kAsmThreads), 0, st, eiota, nnz);
-        if (!run.launched("k_ramp") || !run.sort(col_ind, scol, eiota, sent, nnz, cb))
+        int *eiota = nullptr, *sent = nullptr, *scol = nullptr, *cstart = nullptr;
+        if (!ps.get(eiota, nnz) || !ps.get(sent, nnz) || !ps.get(scol, nnz) ||
+            !ps.get(cstart, n))
             return fail();
+        hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz)), dim3(kAsmThreads), 0, st, eiota, nnz);
+        hipLaunchKernelGGL(k_fill32, dim3(grid_for(n)), dim3(kAsmThreads), 0, st, cstart,
+                           (long long)n, -1);
+        if (!run.launched("k_ramp") || !run.sort(col32, scol, eiota, sent, nnz, cb))
+            return fail();
+        hipLaunchKernelGGL(k_heads, dim3(grid_for(nnz)), dim3(kAsmThreads), 0, st, scol, nnz,
+                           (long long)n, cstart);
         hipLaunchKernelGGL(k_r0, dim3(grid_for(n)), dim3(kAsmThreads), 0, st, row_ptr, col_ind,
-                           values, diag, x0, b, scol, sent, erow, nnz, n, r0);
+                           values, diag, x0, b, scol, sent, cstart, erow, nnz, n, r0);
     } else if (!hip_ok(ctx, hipMemcpyAsync(r0, b, (size_t)n * sizeof(double),
                                            hipMemcpyDeviceToDevice, st),
                        "D2D")) {
@@ -1229,9 +1309,13 @@ int cicy_dnet_assemble(cicy_dnet *dn, int64_t k, int r_flag, int family,
             q.w = w;
             bnd_tot[w] = q.nt + q.ng;
             if (!ps.get(bcnt[w], bnd_tot[w] + 1) || !ps.get(bpos[w], bnd_tot[w] + 1)) return fail();
-            hipLaunchKernelGGL(k_bnd<false>, dim3(grid_for(bnd_tot[w] + 1)), dim3(kAsmThreads), 0, st,
-                               d, q, lmm, lmm_g, owner, cm, ep, bcnt[w], nullptr, nullptr, nullptr,
-                               nullptr, err);
+            hipLaunchKernelGGL(k_bnd_tubes<false>, dim3(grid_for(q.nt + 1)), dim3(kAsmThreads),
+                               0, st, d, q, lmm, ep, bcnt[w], nullptr, nullptr, nullptr,
+                               nullptr);
+            if (q.ng > 0)
+                hipLaunchKernelGGL(k_bnd_plates<false>, dim3((unsigned)q.ng), dim3(kAsmThreads),
+                                   0, st, d, q, lmm_g, owner, cm, ep, bcnt[w], nullptr, nullptr,
+                                   nullptr, nullptr, err);
             if (!run.launched("k_bnd") || !run.scan(bcnt[w], bpos[w], bnd_tot[w] + 1))
                 return fail();
         }
@@ -1257,9 +1341,14 @@ int cicy_dnet_assemble(cicy_dnet *dn, int64_t k, int r_flag, int family,
             if (!run.out(v.d_e_bnd[w], v.ne[w]) || !run.out(v.d_e_int[w], v.ne[w]) ||
                 !run.out(v.d_e_re[w], v.ne[w]))
                 return fail();
-            hipLaunchKernelGGL(k_bnd<true>, dim3(grid_for(bnd_tot[w] + 1)), dim3(kAsmThreads), 0, st,
-                               d, bd[w], lmm, lmm_g, owner, cm, ep, nullptr, bpos[w], v.d_e_bnd[w],
-                               v.d_e_int[w], v.d_e_re[w], err);
+            hipLaunchKernelGGL(k_bnd_tubes<true>, dim3(grid_for(bd[w].nt + 1)),
+                               dim3(kAsmThreads), 0, st, d, bd[w], lmm, ep, nullptr, bpos[w],
+                               v.d_e_bnd[w], v.d_e_int[w], v.d_e_re[w]);
+            if (bd[w].ng > 0)
+                hipLaunchKernelGGL(k_bnd_plates<true>, dim3((unsigned)bd[w].ng),
+                                   dim3(kAsmThreads), 0, st, d, bd[w], lmm_g, owner, cm, ep,
+                                   nullptr, bpos[w], v.d_e_bnd[w], v.d_e_int[w], v.d_e_re[w],
+                                   err);
             if (!run.launched("k_bnd")) return fail();
         }
         v.direction = direction;
