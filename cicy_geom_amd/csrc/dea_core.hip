@@ -1161,7 +1161,8 @@ __global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ o
     }
 }
 
-// kept block b: its rows' new numbers, its constant, its FP32 remainder
+// kept block b: its rows' new numbers (STRUCT), or its constant and FP32 remainder
+template <bool STRUCT>
 __global__ __launch_bounds__(kLT) void k_block_keep(const int *__restrict__ kq0,
                                                     const int *__restrict__ knb,
                                                     const int *__restrict__ krow0,
@@ -1175,12 +1176,15 @@ __global__ __launch_bounds__(kLT) void k_block_keep(const int *__restrict__ kq0,
                                                     double *__restrict__ blk_c,
                                                     float *__restrict__ binv) {
     const int b = blockIdx.x, q0 = kq0[b], nb = knb[b], row0 = krow0[b];
-    const double c = cg[kg[b]];
-    for (int t = threadIdx.x; t < nb; t += kLT) {
-        in_blk[q0 + t] = 1;
-        iperm[q0 + t] = row0 + t;
-        perm[row0 + t] = q0 + t;
+    if (STRUCT) {
+        for (int t = threadIdx.x; t < nb; t += kLT) {
+            in_blk[q0 + t] = 1;
+            iperm[q0 + t] = row0 + t;
+            perm[row0 + t] = q0 + t;
+        }
+        return;
     }
+    const double c = cg[kg[b]];
     const double *src = inv + kgoff[b];
     float *dst = binv + kboff[b];
     for (int e = threadIdx.x; e < nb * nb; e += kLT) dst[e] = (float)(src[e] - c);
@@ -1393,6 +1397,10 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
         !ps.get(dgoff, (size_t)ng + 1) || !ps.get(dinv, (size_t)goff[(size_t)ng]) ||
         !ps.get(dcg, (size_t)ng))
         return nullptr;
+    // The inverses run on the second stream while this stream lays the core out on the
+    // assumption that every block is kept (a block whose Cholesky fails goes back to Jacobi,
+    // which changes the layout): the layout is rebuilt in the rare case that one failed.
+    const bool overlap = ng > 0 && ctx->stream2 != nullptr;
     if (ng > 0) {
         const size_t lds = ((size_t)nbmax * (nbmax | 1) + 2 * (size_t)nbmax) * sizeof(double);
         if (!hip_ok(ctx, hipMemcpyAsync(dgq0, gq0.data(), (size_t)ng * sizeof(int),
@@ -1407,174 +1415,203 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)lds), "hipFuncSetAttribute"))
             return nullptr;
-        // blocks of at most kSmallBlock rows need a quarter of the LDS of a 128-row block,
-        // so several share a CU: they run on the context's second stream beside the large
-        // ones (the groups are sorted by size, the small ones last in gord)
+        // blocks of at most kSmallBlock rows need a quarter of the LDS of a 128-row block, so
+        // several share a CU: they run on this stream, the large ones on the second (the
+        // groups are sorted by size, the small ones last in gord)
         int nbig = 0;
         while (nbig < ng && gnb[(size_t)gord[(size_t)nbig]] > kSmallBlock) nbig++;
         const int nsmall = ng - nbig;
         const size_t lds_small = ((size_t)kSmallBlock * (kSmallBlock | 1) + 2 * kSmallBlock) *
                                  sizeof(double);
-        bool launched = true;
-        if (nsmall > 0 && nbig > 0 && ctx->stream2) {
-            launched = hip_ok(ctx, hipEventRecord(ctx->ev_fork, st), "event") &&
-                       hip_ok(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0), "wait");
-            if (launched) {
-                hipLaunchKernelGGL(k_block_inverse, dim3(nsmall), dim3(kLT), lds_small,
-                                   ctx->stream2, dord + nbig, dgq0, dgnb, dgoff, frp, fcol, fval,
-                                   cdiag, dinv, dok, dcg);
-                hipLaunchKernelGGL(k_block_inverse, dim3(nbig), dim3(kLT), lds, st, dord, dgq0,
-                                   dgnb, dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
-                launched = hip_ok(ctx, hipGetLastError(), "k_block_inverse") &&
-                           hip_ok(ctx, hipEventRecord(ctx->ev_join, ctx->stream2), "event") &&
-                           hip_ok(ctx, hipStreamWaitEvent(st, ctx->ev_join, 0), "wait");
-            }
+        if (overlap) {
+            if (!hip_ok(ctx, hipEventRecord(ctx->ev_fork, st), "event") ||
+                !hip_ok(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0), "wait"))
+                return nullptr;
+            if (nbig > 0)
+                hipLaunchKernelGGL(k_block_inverse, dim3(nbig), dim3(kLT), lds, ctx->stream2, dord,
+                                   dgq0, dgnb, dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
+            if (nsmall > 0)
+                hipLaunchKernelGGL(k_block_inverse, dim3(nsmall), dim3(kLT), lds_small, st,
+                                   dord + nbig, dgq0, dgnb, dgoff, frp, fcol, fval, cdiag, dinv,
+                                   dok, dcg);
+            if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
+                !hip_ok(ctx, hipEventRecord(ctx->ev_join, ctx->stream2), "event"))
+                return nullptr;
+            std::fill(okh.begin(), okh.end(), 1);
         } else {
             hipLaunchKernelGGL(k_block_inverse, dim3(ng), dim3(kLT), lds, st, dord, dgq0, dgnb,
                                dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
+            if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
+                !d2h_sync(ctx, okh.data(), dok, (size_t)ng * sizeof(int)))
+                return nullptr;
         }
-        if (!launched) return nullptr;
-        if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
-            !d2h_sync(ctx, okh.data(), dok, (size_t)ng * sizeof(int)))
-            return nullptr;
     }
     clk.mark("device core: block inverses");
-    // kept blocks, numbered after the Jacobi rows in group order
-    CoreHost H;
-    H.m = m;
-    std::vector<int> kq0, knb, krow0, kg;
-    std::vector<long long> kgoff, kboff;
-    int blocked = 0;
-    for (int g = 0; g < ng; g++)
-        if (okh[(size_t)g]) blocked += gnb[(size_t)g];
-    H.mj = m - blocked;
-    {
-        int row0 = H.mj;
-        long long at = 0;
-        for (int g = 0; g < ng; g++) {
-            if (!okh[(size_t)g]) continue;
-            const int nb = gnb[(size_t)g];
-            kq0.push_back(gq0[(size_t)g]);
-            knb.push_back(nb);
-            krow0.push_back(row0);
-            kg.push_back(g);
-            kgoff.push_back(goff[(size_t)g]);
-            kboff.push_back(at);
-            H.blk_row0.push_back(row0);
-            H.blk_n.push_back(nb);
-            H.blk_off.push_back(at);
-            at += (long long)nb * nb;
-            row0 += nb;
+    // the layout for the blocks kept (okh[g] != 0); after_join: the inverses are still running
+    // on the second stream, joined before their values are copied
+    auto layout = [&](const std::vector<int> &okh, bool after_join) -> cicy_core * {
+        // kept blocks, numbered after the Jacobi rows in group order
+        CoreHost H;
+        H.m = m;
+        std::vector<int> kq0, knb, krow0, kg;
+        std::vector<long long> kgoff, kboff;
+        int blocked = 0;
+        for (int g = 0; g < ng; g++)
+            if (okh[(size_t)g]) blocked += gnb[(size_t)g];
+        H.mj = m - blocked;
+        {
+            int row0 = H.mj;
+            long long at = 0;
+            for (int g = 0; g < ng; g++) {
+                if (!okh[(size_t)g]) continue;
+                const int nb = gnb[(size_t)g];
+                kq0.push_back(gq0[(size_t)g]);
+                knb.push_back(nb);
+                krow0.push_back(row0);
+                kg.push_back(g);
+                kgoff.push_back(goff[(size_t)g]);
+                kboff.push_back(at);
+                H.blk_row0.push_back(row0);
+                H.blk_n.push_back(nb);
+                H.blk_off.push_back(at);
+                at += (long long)nb * nb;
+                row0 += nb;
+            }
         }
-    }
-    const int nbk = (int)kq0.size();
-    cicy_core *z = new cicy_core();
-    z->ctx = ctx;
-    z->m = m;
-    z->nb = nbk;
-    z->blocked_rows = blocked;
-    z->inv_entries = nbk > 0 ? kboff.back() + (long long)knb.back() * knb.back() : 0;
-    auto fail = [&]() -> cicy_core * {
-        core_free(z);
-        return nullptr;
-    };
-    int *in_blk, *iperm, *jflag, *jpos, *len2, *rp2, *col2, *dk[4];
-    long long *dkl[2];
-    double *val2;
-    if (!ps.get(in_blk, (size_t)m + 1) || !ps.get(iperm, (size_t)m + 1) ||
-        !ps.get(jflag, (size_t)m + 1) || !ps.get(jpos, (size_t)m + 1) ||
-        !ps.get(len2, (size_t)m + 1) || !ps.get(rp2, (size_t)m + 1) ||
-        !core_dev(ctx, z, &z->perm, (size_t)m) || !core_dev(ctx, z, &z->diag, (size_t)m) ||
-        !core_dev(ctx, z, &z->blk_c, (size_t)nbk) ||
-        !core_dev(ctx, z, &z->binv, (size_t)z->inv_entries) ||
-        !hip_ok(ctx, hipMemsetAsync(in_blk, 0, ((size_t)m + 1) * sizeof(int), st), "memset"))
-        return fail();
-    for (int k = 0; k < 4; k++)
-        if (!ps.get(dk[k], (size_t)nbk)) return fail();
-    for (int k = 0; k < 2; k++)
-        if (!ps.get(dkl[k], (size_t)nbk)) return fail();
-    if (nbk > 0) {
-        const std::vector<int> *iv[] = {&kq0, &knb, &krow0, &kg};
+        const int nbk = (int)kq0.size();
+        cicy_core *z = new cicy_core();
+        z->ctx = ctx;
+        z->m = m;
+        z->nb = nbk;
+        z->blocked_rows = blocked;
+        z->inv_entries = nbk > 0 ? kboff.back() + (long long)knb.back() * knb.back() : 0;
+        auto fail = [&]() -> cicy_core * {
+            core_free(z);
+            return nullptr;
+        };
+        int *in_blk, *iperm, *jflag, *jpos, *len2, *rp2, *col2, *dk[4];
+        long long *dkl[2];
+        double *val2;
+        if (!ps.get(in_blk, (size_t)m + 1) || !ps.get(iperm, (size_t)m + 1) ||
+            !ps.get(jflag, (size_t)m + 1) || !ps.get(jpos, (size_t)m + 1) ||
+            !ps.get(len2, (size_t)m + 1) || !ps.get(rp2, (size_t)m + 1) ||
+            !core_dev(ctx, z, &z->perm, (size_t)m) || !core_dev(ctx, z, &z->diag, (size_t)m) ||
+            !core_dev(ctx, z, &z->blk_c, (size_t)nbk) ||
+            !core_dev(ctx, z, &z->binv, (size_t)z->inv_entries) ||
+            !hip_ok(ctx, hipMemsetAsync(in_blk, 0, ((size_t)m + 1) * sizeof(int), st), "memset"))
+            return fail();
         for (int k = 0; k < 4; k++)
-            if (!hip_ok(ctx, hipMemcpyAsync(dk[k], iv[k]->data(), (size_t)nbk * sizeof(int),
+            if (!ps.get(dk[k], (size_t)nbk)) return fail();
+        for (int k = 0; k < 2; k++)
+            if (!ps.get(dkl[k], (size_t)nbk)) return fail();
+        if (nbk > 0) {
+            const std::vector<int> *iv[] = {&kq0, &knb, &krow0, &kg};
+            for (int k = 0; k < 4; k++)
+                if (!hip_ok(ctx, hipMemcpyAsync(dk[k], iv[k]->data(), (size_t)nbk * sizeof(int),
+                                                hipMemcpyHostToDevice, st), "H2D"))
+                    return fail();
+            if (!hip_ok(ctx, hipMemcpyAsync(dkl[0], kgoff.data(), (size_t)nbk * sizeof(long long),
+                                            hipMemcpyHostToDevice, st), "H2D") ||
+                !hip_ok(ctx, hipMemcpyAsync(dkl[1], kboff.data(), (size_t)nbk * sizeof(long long),
                                             hipMemcpyHostToDevice, st), "H2D"))
                 return fail();
-        if (!hip_ok(ctx, hipMemcpyAsync(dkl[0], kgoff.data(), (size_t)nbk * sizeof(long long),
-                                        hipMemcpyHostToDevice, st), "H2D") ||
-            !hip_ok(ctx, hipMemcpyAsync(dkl[1], kboff.data(), (size_t)nbk * sizeof(long long),
-                                        hipMemcpyHostToDevice, st), "H2D"))
+            hipLaunchKernelGGL(k_block_keep<true>, dim3(nbk), dim3(kLT), 0, st, dk[0], dk[1], dk[2],
+                               dkl[0], dkl[1], dk[3], dinv, dcg, in_blk, iperm, z->perm, z->blk_c,
+                               z->binv);
+        }
+        hipLaunchKernelGGL(k_jacobi_flags, dim3(core_grid(m + 1)), dim3(kLT), 0, st, m, in_blk, jflag);
+        size_t tmp_bytes = 0;
+        unsigned char *tmp = nullptr;
+        auto scan = [&](const int *in, int *out, int count) {
+            tmp_bytes = 0;
+            return hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, in, out, count, st),
+                          "scan") &&
+                   ps.get(tmp, tmp_bytes) &&
+                   hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, count, st),
+                          "scan");
+        };
+        if (!scan(jflag, jpos, m + 1)) return fail();
+        hipLaunchKernelGGL(k_jacobi_perm, dim3(core_grid(m)), dim3(kLT), 0, st, m, in_blk, jpos, iperm,
+                           z->perm);
+        hipLaunchKernelGGL(k_perm_len, dim3(core_grid(m + 1)), dim3(kLT), 0, st, m, z->perm, frp, len2);
+        if (!scan(len2, rp2, m + 1)) return fail();
+        H.rp.resize((size_t)m + 1);
+        if (!hip_ok(ctx, hipGetLastError(), "core renumbering") ||
+            !d2h_sync(ctx, H.rp.data(), rp2, ((size_t)m + 1) * sizeof(int)))
             return fail();
-        hipLaunchKernelGGL(k_block_keep, dim3(nbk), dim3(kLT), 0, st, dk[0], dk[1], dk[2], dkl[0],
-                           dkl[1], dk[3], dinv, dcg, in_blk, iperm, z->perm, z->blk_c, z->binv);
-    }
-    hipLaunchKernelGGL(k_jacobi_flags, dim3(core_grid(m + 1)), dim3(kLT), 0, st, m, in_blk, jflag);
-    size_t tmp_bytes = 0;
-    unsigned char *tmp = nullptr;
-    auto scan = [&](const int *in, int *out, int count) {
-        tmp_bytes = 0;
-        return hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, in, out, count, st),
-                      "scan") &&
-               ps.get(tmp, tmp_bytes) &&
-               hip_ok(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, count, st),
-                      "scan");
+        const int nnz2 = H.rp[(size_t)m];
+        if (!ps.get(col2, (size_t)nnz2) || !ps.get(val2, (size_t)nnz2)) return fail();
+        hipLaunchKernelGGL(k_permute_rows, dim3(core_grid(m)), dim3(kLT), 0, st, m, H.mj, z->perm,
+                           iperm, frp, fcol, fval, cdiag, rp2, col2, val2, z->diag);
+        clk.mark("device core: renumbered rows");
+        // partitions and slices on the host (from the row lengths), the ELL on the device
+        std::vector<int> sl_base, sl_end;
+        const size_t ell = core_partitions(H, cus, sl_base, sl_end);
+        if (clk.on) {
+            int longest = 0;
+            for (int r = 0; r < m; r++) longest = std::max(longest, H.rp[(size_t)r + 1] - H.rp[(size_t)r]);
+            fprintf(stderr, "[cicy setup]   core: %d rows, %d entries, %zu ELL slots (x%.2f), "
+                    "longest row %d, %d Jacobi rows, %zu blocks\n", m, nnz2, ell,
+                    nnz2 > 0 ? (double)ell / nnz2 : 0.0, longest, H.mj, H.blk_n.size());
+        }
+        z->G = H.G;
+        z->nnz2 = nnz2;
+        int lds_cols = 0;
+        if (!core_lds_cols(ctx, H, &lds_cols)) return fail();
+        z->lds_cols = lds_cols;
+        const int nsl = (int)sl_base.size();
+        int *dsb, *dse;
+        if (!ps.get(dsb, (size_t)nsl) || !ps.get(dse, (size_t)nsl) ||
+            !up_vec(ctx, z, &z->ju, H.ju) || !up_vec(ctx, z, &z->bu, H.bu) ||
+            !up_vec(ctx, z, &z->wr, H.wr) || !up_vec(ctx, z, &z->wsl, H.wsl) ||
+            !up_vec(ctx, z, &z->sl_off, H.sl_off) || !up_vec(ctx, z, &z->sl_len, H.sl_len) ||
+            !up_vec(ctx, z, &z->blk_row0, H.blk_row0) || !up_vec(ctx, z, &z->blk_n, H.blk_n) ||
+            !up_vec(ctx, z, &z->blk_off, H.blk_off) || !core_dev(ctx, z, &z->ell_col, ell) ||
+            !core_dev(ctx, z, &z->ell_val, ell) ||
+            (nsl > 0 && (!hip_ok(ctx, hipMemcpyAsync(dsb, sl_base.data(), (size_t)nsl * sizeof(int),
+                                                     hipMemcpyHostToDevice, st), "H2D") ||
+                         !hip_ok(ctx, hipMemcpyAsync(dse, sl_end.data(), (size_t)nsl * sizeof(int),
+                                                     hipMemcpyHostToDevice, st), "H2D"))))
+            return fail();
+        if (nsl > 0)
+            hipLaunchKernelGGL(k_fill_ell, dim3(core_grid((long long)nsl * kWave)), dim3(kLT), 0, st,
+                               nsl, dsb, dse, z->sl_off, z->sl_len, rp2, col2, val2, z->ell_col,
+                               z->ell_val);
+        // the inverses' values: after the second stream's inverses when they ran beside this
+        if (after_join && !hip_ok(ctx, hipStreamWaitEvent(st, ctx->ev_join, 0), "wait"))
+            return fail();
+        if (nbk > 0)
+            hipLaunchKernelGGL(k_block_keep<false>, dim3(nbk), dim3(kLT), 0, st, dk[0], dk[1], dk[2],
+                               dkl[0], dkl[1], dk[3], dinv, dcg, in_blk, iperm, z->perm, z->blk_c,
+                               z->binv);
+        const bool ok = hip_ok(ctx, hipGetLastError(), "k_fill_ell") &&
+                        zeros(ctx, z, &z->x, m) && zeros(ctx, z, &z->r, m) && zeros(ctx, z, &z->u, m) &&
+                        zeros(ctx, z, &z->w, m) && zeros(ctx, z, &z->p, m) && zeros(ctx, z, &z->s, m) &&
+                        core_alloc(ctx, (void **)&z->part,
+                                   3 * kCoreSums * sizeof(unsigned long long) * (size_t)H.G) &&
+                        core_alloc(ctx, (void **)&z->ctl, sizeof(GridCtl)) &&
+                        core_alloc(ctx, (void **)&z->st, sizeof(CoreState)) &&
+                        hip_ok(ctx, hipStreamSynchronize(st), "sync");
+        clk.mark("device core: slices + upload");
+        if (!ok) return fail();
+        return z;
     };
-    if (!scan(jflag, jpos, m + 1)) return fail();
-    hipLaunchKernelGGL(k_jacobi_perm, dim3(core_grid(m)), dim3(kLT), 0, st, m, in_blk, jpos, iperm,
-                       z->perm);
-    hipLaunchKernelGGL(k_perm_len, dim3(core_grid(m + 1)), dim3(kLT), 0, st, m, z->perm, frp, len2);
-    if (!scan(len2, rp2, m + 1)) return fail();
-    H.rp.resize((size_t)m + 1);
-    if (!hip_ok(ctx, hipGetLastError(), "core renumbering") ||
-        !d2h_sync(ctx, H.rp.data(), rp2, ((size_t)m + 1) * sizeof(int)))
-        return fail();
-    const int nnz2 = H.rp[(size_t)m];
-    if (!ps.get(col2, (size_t)nnz2) || !ps.get(val2, (size_t)nnz2)) return fail();
-    hipLaunchKernelGGL(k_permute_rows, dim3(core_grid(m)), dim3(kLT), 0, st, m, H.mj, z->perm,
-                       iperm, frp, fcol, fval, cdiag, rp2, col2, val2, z->diag);
-    clk.mark("device core: renumbered rows");
-    // partitions and slices on the host (from the row lengths), the ELL on the device
-    std::vector<int> sl_base, sl_end;
-    const size_t ell = core_partitions(H, cus, sl_base, sl_end);
-    if (clk.on) {
-        int longest = 0;
-        for (int r = 0; r < m; r++) longest = std::max(longest, H.rp[(size_t)r + 1] - H.rp[(size_t)r]);
-        fprintf(stderr, "[cicy setup]   core: %d rows, %d entries, %zu ELL slots (x%.2f), "
-                "longest row %d, %d Jacobi rows, %zu blocks\n", m, nnz2, ell,
-                nnz2 > 0 ? (double)ell / nnz2 : 0.0, longest, H.mj, H.blk_n.size());
+    cicy_core *z = layout(okh, overlap);
+    if (!z && overlap) {  // the second stream's inverses still use this call's temporaries
+        (void)hipStreamSynchronize(ctx->stream2);
+        return nullptr;
     }
-    z->G = H.G;
-    z->nnz2 = nnz2;
-    int lds_cols = 0;
-    if (!core_lds_cols(ctx, H, &lds_cols)) return fail();
-    z->lds_cols = lds_cols;
-    const int nsl = (int)sl_base.size();
-    int *dsb, *dse;
-    if (!ps.get(dsb, (size_t)nsl) || !ps.get(dse, (size_t)nsl) ||
-        !up_vec(ctx, z, &z->ju, H.ju) || !up_vec(ctx, z, &z->bu, H.bu) ||
-        !up_vec(ctx, z, &z->wr, H.wr) || !up_vec(ctx, z, &z->wsl, H.wsl) ||
-        !up_vec(ctx, z, &z->sl_off, H.sl_off) || !up_vec(ctx, z, &z->sl_len, H.sl_len) ||
-        !up_vec(ctx, z, &z->blk_row0, H.blk_row0) || !up_vec(ctx, z, &z->blk_n, H.blk_n) ||
-        !up_vec(ctx, z, &z->blk_off, H.blk_off) || !core_dev(ctx, z, &z->ell_col, ell) ||
-        !core_dev(ctx, z, &z->ell_val, ell) ||
-        (nsl > 0 && (!hip_ok(ctx, hipMemcpyAsync(dsb, sl_base.data(), (size_t)nsl * sizeof(int),
-                                                 hipMemcpyHostToDevice, st), "H2D") ||
-                     !hip_ok(ctx, hipMemcpyAsync(dse, sl_end.data(), (size_t)nsl * sizeof(int),
-                                                 hipMemcpyHostToDevice, st), "H2D"))))
-        return fail();
-    if (nsl > 0)
-        hipLaunchKernelGGL(k_fill_ell, dim3(core_grid((long long)nsl * kWave)), dim3(kLT), 0, st,
-                           nsl, dsb, dse, z->sl_off, z->sl_len, rp2, col2, val2, z->ell_col,
-                           z->ell_val);
-    const bool ok = hip_ok(ctx, hipGetLastError(), "k_fill_ell") &&
-                    zeros(ctx, z, &z->x, m) && zeros(ctx, z, &z->r, m) && zeros(ctx, z, &z->u, m) &&
-                    zeros(ctx, z, &z->w, m) && zeros(ctx, z, &z->p, m) && zeros(ctx, z, &z->s, m) &&
-                    core_alloc(ctx, (void **)&z->part,
-                               3 * kCoreSums * sizeof(unsigned long long) * (size_t)H.G) &&
-                    core_alloc(ctx, (void **)&z->ctl, sizeof(GridCtl)) &&
-                    core_alloc(ctx, (void **)&z->st, sizeof(CoreState)) &&
-                    hip_ok(ctx, hipStreamSynchronize(st), "sync");
-    clk.mark("device core: slices + upload");
-    if (!ok) return fail();
+    if (z && overlap) {
+        std::vector<int> okd((size_t)ng, 0);
+        if (!d2h_sync(ctx, okd.data(), dok, (size_t)ng * sizeof(int))) {
+            core_free(z);
+            return nullptr;
+        }
+        if (okd != okh) {  // a block went back to Jacobi: lay the core out again
+            core_free(z);
+            z = layout(okd, false);
+        }
+    }
     return z;
 }
 
