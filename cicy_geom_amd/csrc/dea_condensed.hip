@@ -627,7 +627,7 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = elim_d
         }
         // a round that would take fewer than 1 % of the nodes left is not taken: the rounds
         // stop (as on the device)
-        if (nsel == 0 || (long long)nsel * 100 < left) {
+        if (nsel == 0 || (long long)nsel * min_round_fraction() < left) {
             std::fill(in.begin(), in.end(), 0);
             break;
         }

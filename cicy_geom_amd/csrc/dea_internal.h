@@ -143,6 +143,16 @@ inline int elim_degree() {
     return kElimDegree;
 }
 
+// The series rounds stop at the first that would eliminate fewer than 1 / this many of the
+// nodes left (such a round costs a pass over the whole reduced system for little gain);
+// CICY_ROUND_FRACTION (2..1000) overrides it (a tuning knob; both forms read it)
+constexpr int kMinRoundFraction = 100;
+inline long long min_round_fraction() {
+    if (const char *e = getenv("CICY_ROUND_FRACTION"))
+        return std::max(2, std::min(1000, atoi(e)));
+    return kMinRoundFraction;
+}
+
 // Elimination priority of node k of the reduced system in the series rounds: a fixed
 // integer hash of k with k itself as tie-break (lower value = earlier).  The rounds take
 // the greedy independent set of their candidates in this order, which the device finds in

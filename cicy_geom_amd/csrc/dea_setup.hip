@@ -88,9 +88,6 @@ constexpr int kUndecided = 1, kIn = 2, kOut = 3;  // candidate states of a round
 constexpr int kMaxMisPasses = 4096;
 constexpr int kMisBatch = 16;  // independent-set passes launched per read-back
 constexpr int kPathChunkS = 8;  // path rows whose loads the factorisation issues together
-// the rounds stop at the first that would eliminate fewer than 1 / kMinRoundFraction of the
-// nodes left
-constexpr long long kMinRoundFraction = 100;
 
 inline int grid_of(long long n) {
     return (int)std::max<long long>(1, std::min<long long>((n + kST - 1) / kST, 1 << 16));
@@ -1263,7 +1260,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         nent = tot[1];
         // a round that would take fewer than 1 % of the nodes left is not worth its pass over
         // the whole system: the rounds stop (the host form stops by the same rule)
-        if (nsel == 0 || (long long)nsel * kMinRoundFraction < left) break;
+        if (nsel == 0 || (long long)nsel * min_round_fraction() < left) break;
         left -= nsel;
         if ((size_t)R0 + nsel > rcap - 1 || (size_t)E0 + nent > ecap - 1) {
             set_error(ctx, "series rounds: record arrays overflow");
