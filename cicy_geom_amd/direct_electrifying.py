@@ -331,6 +331,59 @@ def network_resistances(network: Network, electric_param: ElectricParams | None 
     return resistors
 
 
+def network_resistances_device(network: Network, electric_param: ElectricParams | None = None,
+                               tolerance: float = 1e-10,
+                               method: int = dea.METHOD_CONDENSED_BLOCK,
+                               ctx: dea.Context | None = None, with_details: bool = False):
+    """network_resistances with the network copied to the device once and every (cluster,
+    direction) system assembled there (cicy_dnet_assemble: the assembly half of
+    Compute_voltage_field, Direct_Electrifying.cpp:11-113) and handed to the solver without
+    a copy through the host; the plate groups of the block method come with the assembly.
+    The stopping rule (||r|| <= tol * ||r0|| from the ramp guess), the currents and the
+    combination are network_resistances'.  Returns what network_resistances returns."""
+    ep = electric_param or ElectricParams()
+    ctx = ctx or default_context()
+    per_dir = {0: [], 1: [], 2: []}
+    details = {0: [], 1: [], 2: []}
+    dn = network.device(ctx)
+    try:
+        for k in range(network.count("clusters")):
+            _, fam = network.cluster(k)
+            if fam < 0:
+                continue
+            dirs = vector_of_directions(fam)
+            if dirs is None:
+                raise ValueError(f"Invalid family: {fam}")
+            for d in dirs:
+                a = dn.assemble(k, 1, d, ep, d)
+                try:
+                    sysd = a.system()
+                    try:
+                        if method == dea.METHOD_CONDENSED_BLOCK and a.gnp_nodes > 0:
+                            sysd.set_blocks(a.blocks)
+                        st = sysd.solve(method, tolerance, 0, dea.REL_R0)
+                        if st.status not in (dea.CICY_OK, dea.CICY_MAXITER):
+                            raise RuntimeError(f"solve of cluster {k}, direction {d}: "
+                                               + st.as_dict()["status_name"])
+                        x, _ = sysd.get()
+                    finally:
+                        sysd.close()
+                    volt = np.concatenate([a.vef, x])
+                    i1, i2 = (ctx.boundary_current(*e, volt) for e in a.boundary_elements())
+                finally:
+                    a.close()
+                R = ep.applied_voltage / i1
+                per_dir[d].append(R)
+                details[d].append((k, R, i1, i2, st))
+    finally:
+        dn.close()
+    resistors, resistivities = calculate_resistances_and_resistivities(
+        network.params.box, ep, [per_dir[d] for d in range(3)])
+    if with_details:
+        return resistors, resistivities, details
+    return resistors
+
+
 def network_resistances_batched(network: Network,
                                 electric_param: ElectricParams | None = None,
                                 tolerance: float = 1e-10,
