@@ -91,6 +91,10 @@ constexpr double kCostRow = 2.0e-3, kCostEntry = 6.3e-5, kCostRound = 3.2;
 // Phase-w cost of a row, in units of one stored entry (a row is one lane's chain of
 // dependent loads; its entries are the slots of that chain)
 constexpr int kCostWRow = 64;
+// A Jacobi row's phase-w cost relative to a block row's: its gathers reach columns spread
+// over the core, a plate row's stay near its plate (configuration 3, CICY_CORE_TIMING:
+// 15.4 us for 1 125 Jacobi rows against 12.3 us for 1 104 block rows per work-group)
+constexpr double kCostWJacobi = 1.25;
 
 struct CoreState {
     long long k;         // iterations done in this solve
@@ -280,15 +284,21 @@ size_t core_partitions(CoreHost &H, int max_groups, std::vector<int> &sl_base,
             H.ju[g + 1] = (int)given;
         }
     }
-    // phase w: the rows cut into G runs of equal cost (kCostWRow per row + its entries)
+    // phase w: the rows cut into G runs of equal cost (kCostWRow per row + its entries,
+    // times kCostWJacobi for a Jacobi row)
     H.wr.assign((size_t)G + 1, m);
     H.wr[0] = 0;
     {
-        const double total = (double)kCostWRow * m + (double)H.rp[m];
+        auto wcost = [&](int r) {
+            const double c = (double)kCostWRow + (double)(H.rp[r + 1] - H.rp[r]);
+            return r < H.mj ? kCostWJacobi * c : c;
+        };
+        double total = 0.0;
+        for (int r = 0; r < m; r++) total += wcost(r);
         double acc = 0.0;
         int g = 1;
         for (int r = 0; r < m && g < G; r++) {
-            acc += kCostWRow + (H.rp[r + 1] - H.rp[r]);
+            acc += wcost(r);
             while (g < G && acc >= total * (double)g / G) H.wr[g++] = r + 1;
         }
     }
