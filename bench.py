@@ -309,7 +309,8 @@ def bench_assembly(args):
         t = time.perf_counter()
         a = dn.assemble(k, 1, 0, None, 0)
         sd = a.system()
-        sd.set_blocks(a.blocks)
+        if len(a.blocks):
+            sd.set_blocks(a.blocks)
         st = sd.solve(dea.METHOD_CONDENSED_BLOCK, args.tol, 0, dea.REL_B)
         sd.close()
         a.close()
@@ -337,18 +338,22 @@ def bench_assembly(args):
                          "triplet list, stable sort, fold) of the same cluster"}}), flush=True)
 
 
-def batch_realisation(k):
+def batch_realisation(k, keep_net=False):
     """Systems of realisation k of the batch workload (configuration 4; seed 4_000_000 + k):
     U[2k, 40k] tubes at the density of the 5 um / 40k-tube RVE, the largest cluster
-    percolating along each direction, R_flag = 1 (family := direction)."""
+    percolating along each direction, R_flag = 1 (family := direction).  With keep_net,
+    also the network and the (cluster, direction) of each system."""
     from cicy_geom_amd import network
-    out = []
+    out, jobs = [], []
     net = network.generate(4, realisation=k)
     for d in range(3):
         ks = net.percolating_clusters(d)
         if ks:
             c = max(ks, key=lambda q: net.cluster(q)[0])
             out.append(net.assemble(c, 1, d, direction=d))
+            jobs.append((c, d))
+    if keep_net:
+        return out, net, jobs
     net.close()
     return out
 
@@ -361,16 +366,40 @@ def bench_batch(args):
           read-back); generation is outside the timed region;
       (b) the first --batch-pcg-realisations of them in one batch (cicy_batch: the systems
           concatenated, per-system scalars and stopping tests in shared launches, FP64
-          Jacobi-PCG, the reference's algorithm), timed from the host arrays likewise.
+          Jacobi-PCG, the reference's algorithm), timed from the host arrays likewise;
+      (c) the same systems from each network held in HBM (copied once per realisation,
+          not timed): device assembly, set-up and solve (network_to_solution_device).
     value = systems / s of (a) over all R realisations."""
     from cicy_geom_amd import dea
     ctx = dea.Context(0)
     t_gen, wall, count, iters, worst, rows = 0.0, 0.0, 0, 0, 0.0, []
+    dwall, dcopy, dcount, dworst = 0.0, 0.0, 0, 0.0
     kept = []
     for k in range(args.batch_realisations):
         t = time.perf_counter()
-        systems = batch_realisation(k)
+        systems, net, jobs = batch_realisation(k, keep_net=True)
         t_gen += time.perf_counter() - t
+        # (c) the same systems from the network in HBM: device assembly, set-up and solve
+        t = time.perf_counter()
+        dn = net.device(ctx)
+        dcopy += time.perf_counter() - t
+        for c, d in jobs:
+            t = time.perf_counter()
+            a = dn.assemble(c, 1, d, None, d)
+            sysd = a.system()
+            if len(a.blocks):
+                sysd.set_blocks(a.blocks)
+            st = sysd.solve(dea.METHOD_CONDENSED_BLOCK, args.tol, MAX_ITER, dea.REL_B)
+            sysd.close()
+            a.close()
+            dwall += time.perf_counter() - t
+            dcount += 1
+            dworst = max(dworst, st.true_rel_res)
+            if st.status != dea.CICY_OK:
+                log(f"[bench] WARNING device-assembled system of realisation {k} did not "
+                    "converge")
+        dn.close()
+        net.close()
         if k < args.batch_pcg_realisations:
             kept.extend(systems)
         for s in systems:
@@ -393,7 +422,12 @@ def bench_batch(args):
     out = {"condensed_block_sequential": dict(
         systems=count, seconds=wall, solves_per_s=count / wall, core_iters=iters,
         worst_true_rel_res=worst, rows_min=min(rows), rows_max=max(rows),
-        rows_total=int(sum(rows)))}
+        rows_total=int(sum(rows))),
+        "network_to_solution_device": dict(
+            systems=dcount, seconds=dwall, solves_per_s=dcount / dwall if dwall else None,
+            network_copy_s=dcopy, worst_true_rel_res=dworst,
+            step="device assembly + set-up + block-method solve, network in HBM")}
+    log(f"[bench] batch network_to_solution_device: {out['network_to_solution_device']}")
     log(f"[bench] batch condensed_block: {out['condensed_block_sequential']}")
     if kept and not args.no_plain:
         t = time.perf_counter()

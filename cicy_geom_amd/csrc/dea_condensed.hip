@@ -733,7 +733,14 @@ void build_series_host(const CondHost &C, SeriesHost &Z, int max_degree = elim_d
     Z.cdiag.resize((size_t)Z.mc);
     for (int q = 0; q < Z.mc; q++) {
         const int i = Z.core_nodes[q];
-        Z.cdiag[q] = d[i];
+        // a node left without neighbours and with no usable pivot is what the rounds leave
+        // of a floating group of tubes (its rows sum to zero, so its diagonal comes out as
+        // rounding noise): it is decoupled, diagonal 1, and the group takes the voltage its
+        // zero right-hand side gives, one of the singular block's solutions
+        const bool isolated = arp[i] == arp[i + 1];
+        Z.cdiag[q] = (isolated && std::isfinite(d[i]) && !(d[i] > 1e-12 * std::fabs(C.sdiag[i])))
+                         ? 1.0
+                         : d[i];
         for (int t = arp[i]; t < arp[i + 1]; t++) {
             const int c = core_of[acol[t]];
             if (c < q) {
@@ -878,7 +885,7 @@ cicy_cond *condensed_setup(cicy_ctx *ctx, const BlockedHost &h, const BandHost &
              up(ctx, c->rv_off, rv_off) && up(ctx, c->rv_rec, rv_rec) && up(ctx, c->rv_a, rv_a) &&
              hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
         if (ok && Z.mc > 0) {
-            // a core row with a vanishing diagonal (a floating group of tubes) is refused by
+            // a core row with a vanishing diagonal that still has neighbours is refused by
             // the layout check: then there is simply no condensed system
             bool positive = true;
             for (double v : Z.cdiag)

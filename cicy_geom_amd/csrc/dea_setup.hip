@@ -804,15 +804,19 @@ __global__ __launch_bounds__(kST) void k_core_fill(
     int m, const int *__restrict__ alive, const int *__restrict__ core_of,
     const int *__restrict__ cpos, const int *__restrict__ arp, const int *__restrict__ acol,
     const double *__restrict__ aval, const double *__restrict__ d,
-    const int *__restrict__ anchors, int *__restrict__ core_nodes, int *__restrict__ frp,
-    int *__restrict__ fcol, double *__restrict__ fval, double *__restrict__ cdiag,
-    int *__restrict__ orig) {
+    const double *__restrict__ sd0, const int *__restrict__ anchors,
+    int *__restrict__ core_nodes, int *__restrict__ frp, int *__restrict__ fcol,
+    double *__restrict__ fval, double *__restrict__ cdiag, int *__restrict__ orig) {
     GRID_LOOP(k, m) {
         if (!alive[k]) continue;
         const int q = core_of[k], o = cpos[k];
         core_nodes[q] = (int)k;
         frp[q] = o;
-        cdiag[q] = d[k];
+        // what the rounds leave of a floating group of tubes: a node without neighbours and
+        // without a usable pivot, decoupled with diagonal 1 (as the host form)
+        const bool isolated = arp[k] == arp[k + 1];
+        const double dk = d[k];
+        cdiag[q] = (isolated && isfinite(dk) && !(dk > 1e-12 * fabs(sd0[k]))) ? 1.0 : dk;
         orig[q] = anchors[k];
         for (int t = arp[k]; t < arp[k + 1]; t++) {
             fcol[o + t - arp[k]] = core_of[acol[t]];
@@ -1343,10 +1347,10 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
             !dev_alloc(ctx, &c->d_cdiag, (size_t)mc) || !dev_alloc(ctx, &c->d_orig, (size_t)mc) ||
             !dev_alloc(ctx, &c->core_b, (size_t)mc) || !dev_alloc(ctx, &c->core_x, (size_t)mc))
             return fail();
-        LAUNCH(k_core_fill, m, m, alive, core_of, cpos2, arp, acol, aval, d, c->anchors,
+        LAUNCH(k_core_fill, m, m, alive, core_of, cpos2, arp, acol, aval, d, sdiag, c->anchors,
                c->core_nodes, c->d_frp, c->d_fcol, c->d_fval, c->d_cdiag, c->d_orig);
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, c->d_frp + mc, nnz2);
-        // a core row with a vanishing diagonal (a floating group of tubes): no condensed
+        // a core row with a vanishing diagonal that still has neighbours: no condensed
         // system, as on the host
         int bad = 0;
         if (!hip_ok(ctx, hipMemsetAsync(totbuf, 0, sizeof(int), st), "memset")) return fail();

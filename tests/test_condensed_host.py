@@ -341,3 +341,50 @@ def test_threaded_condensation_is_independent_of_the_thread_count(monkeypatch):
     x = dea.host_expand(s.row_ptr, s.col_ind, s.values, s.diag, s.b, xj)
     ax = oracle.spmv_sss(s.row_ptr, s.col_ind, s.values, s.diag, x)
     assert np.linalg.norm(s.b - ax) <= 1e-11 * np.linalg.norm(s.b)
+
+
+def floating_triangle_system(k=6, g=3.0):
+    """The definite chain system plus a floating triangle of tube nodes (rows n0, n0 + 1,
+    n0 + 2; conductance g on every side, no electrode): n0 and n0 + 2 are anchors (the side
+    between them is a junction), n0 + 1 a path row.  The path elimination leaves the two
+    anchors joined by 1.5 g, each with diagonal 1.5 g; the series round eliminates one, and
+    the other is left without neighbours and with diagonal 1.5 g - (1.5 g)^2 / 1.5 g = 0."""
+    rp0, col0, val0, diag0 = chain_system(k, g)
+    n0 = len(diag0)
+    rp, col, val = list(rp0), list(col0), list(val0)
+    rows = {n0: [], n0 + 1: [(n0, -g)], n0 + 2: [(n0, -g), (n0 + 1, -g)]}
+    for r in (n0, n0 + 1, n0 + 2):
+        for c, v in rows[r]:
+            col.append(c)
+            val.append(v)
+        rp.append(len(col))
+    diag = np.concatenate([diag0, [2 * g] * 3])
+    return ((rp0, col0, val0, diag0),
+            (np.array(rp, np.int64), np.array(col, np.int64), np.array(val), diag), n0)
+
+
+def test_floating_group_reduced_to_one_node_is_decoupled():
+    """A floating group that the series rounds reduce to a single node leaves that node
+    without neighbours and with a diagonal of rounding size (here exactly 0): the core gives
+    it diagonal 1 instead of refusing the system, the group takes the voltage of its zero
+    right-hand side (a solution of its singular, consistent block), and the definite part
+    is solved exactly."""
+    (rp0, col0, val0, diag0), (rp, col, val, diag), n0 = floating_triangle_system()
+    b = np.zeros(len(diag))
+    b[0] = 1.0
+    rows, crp, ccol, cval, cdiag, crhs, rounds = dea.host_core(rp, col, val, diag, b)
+    assert rounds >= 1
+    q = int(np.searchsorted(rows, n0 + 2)) if n0 + 2 in set(rows) else int(
+        np.searchsorted(rows, n0))
+    assert rows[q] in (n0, n0 + 2)
+    C = dense(crp, ccol, cval, cdiag)
+    assert cdiag[q] == 1.0
+    assert np.count_nonzero(C[q]) == 1 and np.count_nonzero(C[:, q]) == 1  # decoupled
+    assert np.all(cdiag > 0.0)
+    xc = np.linalg.solve(C, crhs)
+    x = dea.host_core_expand(rp, col, val, diag, b, xc)
+    A = dense(rp, col, val, diag)
+    assert np.linalg.norm(b - A @ x) <= 1e-13
+    assert np.allclose(x[n0:], x[n0], rtol=0, atol=1e-15)   # one voltage on the group
+    want = np.linalg.solve(dense(rp0, col0, val0, diag0), b[:n0])
+    assert np.allclose(x[:n0], want, rtol=1e-12)

@@ -201,3 +201,69 @@ def test_device_setup_config3_equals_host_setup(ctx):
     assert info_d == {k: v for k, v in info_h.items()}
     assert st_d.status == dea.CICY_OK and st_d.iters == st_h.iters
     assert np.array_equal(x_d, x_h)
+
+
+def test_floating_group_reduced_to_one_node_solves(ctx):
+    """A floating group that the series rounds reduce to one node (diagonal 0, no
+    neighbours) is decoupled with diagonal 1 by both set-ups, bit for bit, and the block
+    method solves the system (it used to refuse it)."""
+    from test_condensed_host import floating_triangle_system
+    _, (rp, col, val, diag), n0 = floating_triangle_system()
+    b = np.zeros(len(diag))
+    b[0] = 1.0
+    out = []
+    for where in ("host", "device"):
+        ctx.set_setup(where)
+        try:
+            arrs = [ctx.upload(np.ascontiguousarray(a, dt)) for a, dt in
+                    ((rp, np.int64), (col, np.int64), (val, np.float64), (diag, np.float64))]
+            sysd = ctx.system_device(len(diag), len(val), *arrs)
+            sysd.set_rhs(b, None, b)
+            st = sysd.solve(dea.METHOD_CONDENSED_BLOCK, 1e-12, 0, dea.REL_B)
+            x, _ = sysd.get()
+            out.append((st, x, sysd.size("core_rows")))
+            sysd.close()
+            for a in arrs:
+                a.close()
+        finally:
+            ctx.set_setup("device")
+    (st_h, x_h, core_h), (st_d, x_d, core_d) = out
+    assert st_h.status == dea.CICY_OK and st_d.status == dea.CICY_OK
+    assert core_h == core_d > 0
+    assert np.array_equal(x_h, x_d)
+    rows = np.repeat(np.arange(len(diag)), np.diff(rp))
+    ax = diag * x_d
+    np.add.at(ax, rows, val * x_d[col])
+    np.add.at(ax, col, val * x_d[rows])
+    assert np.linalg.norm(b - ax) <= 1e-12 * np.linalg.norm(b)
+
+
+def test_configuration4_floating_component_solves(ctx):
+    """Configuration 4, realisation 16 (seed 4 000 016): the x-direction system holds a
+    floating group of tubes that the rounds reduce to one node.  Assembled on the device and
+    solved by the block method it converges, and its resistance equals the plain FP64
+    PCG's on the host-assembled system within 1e-9."""
+    net = network.generate(4, realisation=16)
+    k = max(net.percolating_clusters(0), key=lambda c: net.cluster(c)[0])
+    s = net.assemble(k, 1, 0, direction=0)
+    dn = net.device(ctx)
+    a = dn.assemble(k, 1, 0, None, 0)
+    sd = a.system()
+    st = sd.solve(dea.METHOD_CONDENSED_BLOCK, 1e-12, 0, dea.REL_R0)
+    x_b, _ = sd.get()
+    sd.close()
+    cur_b = ctx.boundary_current(*a.boundary_elements()[0], np.concatenate([a.vef, x_b]))
+    a.close()
+    dn.close()
+    assert st.status == dea.CICY_OK, st.as_dict()
+    sh = ctx.system(s.row_ptr, s.col_ind, s.values, s.diag)
+    sh.set_rhs(s.r0, s.x0, s.b)
+    st_p = sh.solve(dea.METHOD_PCG_F64, 1e-12, 0, dea.REL_R0)
+    x_p, _ = sh.get()
+    sh.close()
+    cur_p = ctx.boundary_current(*s.boundary_elements[0], np.concatenate([s.vef, x_p]))
+    print(f"\nconfig 4 realisation 16: N={s.n}, block {st.iters} core iterations, plain "
+          f"{st_p.iters}; I {cur_b:.12e} / {cur_p:.12e}")
+    assert st_p.status == dea.CICY_OK
+    assert abs(cur_b - cur_p) <= 1e-9 * abs(cur_p)
+    net.close()
