@@ -669,6 +669,8 @@ __global__ __launch_bounds__(kST) void k_receivers(int m, const int *__restrict_
     }
 }
 
+constexpr int kMergeLists = 8;  // eliminated neighbours merged without staging their fill
+
 // The new row of each surviving node: its old entries without the eliminated neighbours,
 // merged with the fill of every eliminated neighbour (record order; equal columns summed
 // after the old value, in record order).  Written at ub offsets, true lengths in len.
@@ -693,6 +695,71 @@ __global__ __launch_bounds__(kST) void k_merge(
             }
             len[u] = arp[u + 1] - arp[u];
             continue;
+        }
+        // with at most kMergeLists eliminated neighbours: one merge of the old row (without
+        // them) and their records' rows (each ascending, the entry for u left out), equal
+        // columns summed old value first, then in record order -- the order the sort below
+        // gives, without staging the fill
+        {
+            int lp[kMergeLists], le[kMergeLists], hc[kMergeLists];
+            double la[kMergeLists], ld[kMergeLists];
+            int nl = 0;
+            bool fits = true;
+            for (int q = arp[u]; q < arp[u + 1]; q++) {
+                const int j = acol[q];
+                if (!sel[j]) continue;
+                if (nl == kMergeLists) {
+                    fits = false;
+                    break;
+                }
+                const int rec = rec_of[j];
+                lp[nl] = e_off[rec];
+                le[nl] = e_off[rec + 1];
+                la[nl] = aval[q];
+                ld[nl] = e_d[rec];
+                nl++;
+            }
+            if (fits) {
+#pragma unroll
+                for (int l = 0; l < kMergeLists; l++) {
+                    if (l >= nl) break;
+                    while (lp[l] < le[l] && e_nbr[lp[l]] == (int)u) lp[l]++;
+                    hc[l] = lp[l] < le[l] ? e_nbr[lp[l]] : INT_MAX;
+                }
+                const int o0 = o;
+                int q = arp[u];
+                const int qe = arp[u + 1];
+                while (q < qe && sel[acol[q]]) q++;
+                for (;;) {
+                    const int co = q < qe ? acol[q] : INT_MAX;
+                    int c = co;
+#pragma unroll
+                    for (int l = 0; l < kMergeLists; l++)
+                        if (l < nl) c = min(c, hc[l]);
+                    if (c == INT_MAX) break;
+                    double v = 0.0;
+                    bool have = false;
+                    if (co == c) {
+                        v = aval[q++];
+                        have = true;
+                        while (q < qe && sel[acol[q]]) q++;
+                    }
+#pragma unroll
+                    for (int l = 0; l < kMergeLists; l++) {
+                        if (l >= nl || hc[l] != c) continue;
+                        const double fv = -la[l] * e_a[lp[l]] / ld[l];
+                        v = have ? v + fv : fv;
+                        have = true;
+                        lp[l]++;
+                        while (lp[l] < le[l] && e_nbr[lp[l]] == (int)u) lp[l]++;
+                        hc[l] = lp[l] < le[l] ? e_nbr[lp[l]] : INT_MAX;
+                    }
+                    ocol[o] = c;
+                    oval[o++] = v;
+                }
+                len[u] = o - o0;
+                continue;
+            }
         }
         // the fill, in record order, then stably sorted by column
         const int f0 = fpos[u];
