@@ -246,10 +246,19 @@ __global__ __launch_bounds__(kST) void k_path_flags(int n, const int *__restrict
     }
 }
 
+// each path's first and last row: a path row whose next row does not continue the path
+// ends one; paths are disjoint runs in row order, so the path ending at row i is the last
+// one started at or before i
 __global__ __launch_bounds__(kST) void k_path_starts(int n, const int *__restrict__ pflag,
                                                      const int *__restrict__ pscan,
-                                                     int *__restrict__ ps) {
-    GRID_LOOP(i, n) if (pflag[i]) ps[pscan[i]] = (int)i;
+                                                     const int *__restrict__ anchor_of,
+                                                     const double *__restrict__ band,
+                                                     int *__restrict__ ps, int *__restrict__ pe) {
+    GRID_LOOP(i, n) {
+        if (pflag[i]) ps[pscan[i]] = (int)i;
+        if (anchor_of[i] < 0 && (i + 1 == n || anchor_of[i + 1] >= 0 || band[i + 1] == 0.0))
+            pe[pscan[i] + pflag[i] - 1] = (int)i;
+    }
 }
 
 __global__ __launch_bounds__(kST) void k_fill_factors(int n, double *__restrict__ lfac,
@@ -260,7 +269,7 @@ __global__ __launch_bounds__(kST) void k_fill_factors(int n, double *__restrict_
     }
 }
 
-// One thread per path: its end, neighbouring anchors, Thomas factors, and its Schur terms
+// One thread per path: its neighbouring anchors, Thomas factors, and its Schur terms
 // (termL on the left anchor's diagonal, termR on the right one's, fillv the coupling
 // between them).  err: lowest path with a non-positive pivot, (path << 2) | kind with kind
 // 1 on a path coupled to an anchor, 2 inside a floating path.
@@ -271,12 +280,9 @@ __global__ __launch_bounds__(kST) void k_paths(
     double *__restrict__ mfac, double *__restrict__ g, double *__restrict__ termL,
     double *__restrict__ termR, double *__restrict__ fillv, unsigned long long *__restrict__ err) {
     GRID_LOOP(p, np) {
-        const int s = ps[p];
-        int e = s;
-        while (e + 1 < n && anchor_of[e + 1] < 0 && band[e + 1] != 0.0) e++;
+        const int s = ps[p], e = pe[p];
         const int L = (s > 0 && band[s] != 0.0) ? s - 1 : -1;
         const int R = (e + 1 < n && band[e + 1] != 0.0) ? e + 1 : -1;
-        pe[p] = e;
         pl[p] = L >= 0 ? anchor_of[L] : -1;
         pr[p] = R >= 0 ? anchor_of[R] : -1;
         termL[p] = 0.0;
@@ -1198,7 +1204,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         !hip_ok(ctx, hipMemcpyAsync(c->band, s->band, (nv + 1) * sizeof(double),
                                     hipMemcpyDeviceToDevice, st), "copy band"))
         return fail();
-    LAUNCH(k_path_starts, n, n, pflag, pscan, c->ps);
+    LAUNCH(k_path_starts, n, n, pflag, pscan, c->anchor_of, s->band, c->ps, c->pe);
     LAUNCH(k_fill_factors, n, n, c->lfac, c->mfac);
     if (np > 0)
         LAUNCH(k_paths, np, np, n, c->ps, c->anchor_of, s->band, s->diag, c->pe, c->pl, c->pr,
