@@ -288,9 +288,20 @@ __global__ __launch_bounds__(kST) void k_paths(
         termL[p] = 0.0;
         termR[p] = 0.0;
         fillv[p] = 0.0;
-        // the Thomas factors, the loads of kPathChunkS rows issued before their arithmetic
+        // the Thomas factors, the loads of kPathChunkS rows issued before their arithmetic;
+        // the pivots checked and, for a path with an anchor on the left, g = T^-1 band[s] e_s
+        // forward-eliminated as they come
+        const double pivot_floor = 1e-12 * diag[e];
+        auto bad = [&](double v) { return !(v > pivot_floor) || !isfinite(v); };
         double mprev = diag[s];
         mfac[s] = mprev;
+        bool bad_inner = s < e && bad(mprev), bad_last = s == e && bad(mprev);
+        const bool left = L >= 0;
+        double gprev = 0.0;
+        if (left) {
+            gprev = band[s];
+            g[s] = gprev;
+        }
         for (int k0 = s + 1; k0 <= e; k0 += kPathChunkS) {
             double bb[kPathChunkS], dd[kPathChunkS];
 #pragma unroll
@@ -307,21 +318,20 @@ __global__ __launch_bounds__(kST) void k_paths(
                 mprev = dd[t] - l * bb[t];
                 lfac[k] = l;
                 mfac[k] = mprev;
+                if (k == e) bad_last = bad(mprev);
+                else bad_inner = bad_inner || bad(mprev);
+                if (left) {
+                    gprev = -l * gprev;
+                    g[k] = gprev;
+                }
             }
         }
-        const double pivot_floor = 1e-12 * diag[e];
-        bool singular = false;
-        for (int k = s; k <= e; k++)
-            if (!(mfac[k] > pivot_floor) || !isfinite(mfac[k])) singular = true;
-        if (singular) {
+        if (bad_inner || bad_last) {
             if (L >= 0 || R >= 0) {
                 atomicMin(err, ((unsigned long long)p << 2) | 1ull);
                 continue;
             }
-            bool inner_bad = false;
-            for (int k = s; k < e; k++)
-                if (!(mfac[k] > pivot_floor) || !isfinite(mfac[k])) inner_bad = true;
-            if (inner_bad) {
+            if (bad_inner) {
                 atomicMin(err, ((unsigned long long)p << 2) | 2ull);
                 continue;
             }
@@ -330,16 +340,30 @@ __global__ __launch_bounds__(kST) void k_paths(
             mfac[e] = 1.0;
             continue;
         }
-        if (L >= 0) {
-            g[s] = band[s];
-            for (int k = s + 1; k <= e; k++) g[k] = -lfac[k] * g[k - 1];
-            double u = g[e] / mfac[e];
+        if (left) {
+            // back-substitution of g, the loads of kPathChunkS rows issued together
+            double u = gprev / mprev;
             const double ue = u;
-            for (int k = e - 1; k >= s; k--) u = (g[k] - band[k + 1] * u) / mfac[k];
+            for (int k0 = e - 1; k0 >= s; k0 -= kPathChunkS) {
+                double gg[kPathChunkS], bb[kPathChunkS], mm[kPathChunkS];
+#pragma unroll
+                for (int t = 0; t < kPathChunkS; t++) {
+                    const int k = max(k0 - t, s);
+                    gg[t] = g[k];
+                    bb[t] = band[k + 1];
+                    mm[t] = mfac[k];
+                }
+#pragma unroll
+                for (int t = 0; t < kPathChunkS; t++) {
+                    const int k = k0 - t;
+                    if (k < s) break;
+                    u = (gg[t] - bb[t] * u) / mm[t];
+                }
+            }
             termL[p] = band[s] * u;
             if (R >= 0) fillv[p] = -band[e + 1] * ue;
         }
-        if (R >= 0) termR[p] = band[e + 1] * (band[e + 1] / mfac[e]);
+        if (R >= 0) termR[p] = band[e + 1] * (band[e + 1] / mprev);
     }
 }
 
