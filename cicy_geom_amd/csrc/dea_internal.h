@@ -153,6 +153,13 @@ inline long long min_round_fraction() {
     return kMinRoundFraction;
 }
 
+// Whether the series rounds find their independent set in one cooperative launch
+// (CICY_MIS_ASYNC=0 makes them use batches of passes instead; the set is the same).
+inline bool mis_async_enabled() {
+    const char *e = getenv("CICY_MIS_ASYNC");
+    return !(e && e[0] == '0');
+}
+
 // Elimination priority of node k of the reduced system in the series rounds: a fixed
 // integer hash of k with k itself as tie-break (lower value = earlier).  The rounds take
 // the greedy independent set of their candidates in this order, which the device finds in

@@ -23,6 +23,7 @@
 // #pragma clang fp contract(off) keeps the compiler from fusing what the host computes as
 // separate products and sums.
 #include "dea_internal.h"
+#include "dea_grid.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -581,6 +582,51 @@ __global__ __launch_bounds__(kST) void k_mis_pass(int m, const int *__restrict__
     if (__syncthreads_or(any_wait) && threadIdx.x == 0) *undecided = 1u;
 }
 
+// The round's independent set in one cooperative launch: each thread sweeps its
+// candidates until every one is decided, by k_mis_pass's rule.  Decisions are final and
+// depend only on earlier candidates' final decisions, so the set is the one the passes
+// give; states are read and written with device-coherent atomics, every work-group is
+// resident, and the undecided candidate of lowest priority can always be decided, so
+// every thread finishes.  A thread past max_sweeps stops and sets *gave_up (the host then
+// finishes the set with passes).
+__global__ __launch_bounds__(kST) void k_mis_async(int m, const int *__restrict__ arp,
+                                                   const int *__restrict__ acol, int *state,
+                                                   int *__restrict__ gave_up,
+                                                   unsigned max_sweeps) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long first = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (unsigned sweep = 0;; sweep++) {
+        bool any_wait = false;
+        for (long long k = first; k < m; k += stride) {
+            if (ld_coh_i(&state[k]) != kUndecided) continue;
+            const unsigned long long pk = elim_priority((int)k);
+            bool blocked = false, wait = false;
+            for (int t = arp[k]; t < arp[k + 1]; t++) {
+                const int j = acol[t];
+                const int sj = ld_coh_i(&state[j]);
+                if (sj == kIn) {
+                    blocked = true;
+                    break;
+                }
+                if (sj == kUndecided && elim_priority(j) < pk) wait = true;
+            }
+            if (blocked)
+                __hip_atomic_store(&state[k], kOut, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else if (!wait)
+                __hip_atomic_store(&state[k], kIn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                any_wait = true;
+        }
+        if (!any_wait) return;
+        if (sweep >= max_sweeps) {
+            __hip_atomic_store(gave_up, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+constexpr unsigned kMisMaxSweeps = 1u << 16;
+
 // four counts per node, prefix-summed together in one scan
 struct Cnt4 {
     int v[4];
@@ -876,6 +922,13 @@ struct LastArgs {
 __global__ __launch_bounds__(64) void k_last(LastArgs a, int k, int at, int *__restrict__ out) {
     if ((int)threadIdx.x < k) out[threadIdx.x] = a.p[threadIdx.x][at];
 }
+struct PickArgs {
+    const int *p[8];
+    int at[8];
+};
+__global__ __launch_bounds__(64) void k_pick_ints(PickArgs a, int k, int *__restrict__ out) {
+    if ((int)threadIdx.x < k) out[threadIdx.x] = a.p[threadIdx.x][a.at[threadIdx.x]];
+}
 
 // *bad = 1 when some v[i] is not positive and finite
 __global__ __launch_bounds__(kST) void k_nonpositive(int n, const double *__restrict__ v,
@@ -982,6 +1035,21 @@ struct Cub {
 // last element of a device int array (a scan's total), waited for
 bool last_int(cicy_ctx *ctx, const int *p, long long at, int *out) {
     return d2h_sync(ctx, out, p + at, sizeof(int));
+}
+
+// entries of device int arrays (each at its own index), with one wait; `buf` holds 8 ints
+bool pick_ints(cicy_ctx *ctx, int *buf, std::initializer_list<std::pair<const int *, int>> at,
+               int *out) {
+    PickArgs a{};
+    int k = 0;
+    for (const auto &x : at) {
+        a.p[k] = x.first;
+        a.at[k] = x.second;
+        k++;
+    }
+    hipLaunchKernelGGL(k_pick_ints, dim3(1), dim3(64), 0, ctx->stream, a, k, buf);
+    return hip_ok(ctx, hipGetLastError(), "k_pick_ints") &&
+           d2h_sync(ctx, out, buf, (size_t)k * sizeof(int));
 }
 
 // the entries at `at` of several device int arrays, with one wait; `buf` holds 8 ints
@@ -1294,8 +1362,23 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         !ps.get(tpos, mv + 1) || !ps.get(cpos, mv + 1) ||
         !ps.get(fpos, mv + 1) ||
         !ps.get(upos, mv + 1) || !ps.get(len, mv + 1) || !ps.get(d, mv + 1) ||
-        !ps.get(undecided, kMisBatch) || !ps.get(totbuf, 8))
+        !ps.get(undecided, kMisBatch + 1) || !ps.get(totbuf, 8))
         return fail();
+    int *gave_up = reinterpret_cast<int *>(undecided + kMisBatch);
+    // the cooperative independent-set grid: every work-group resident, at most one thread
+    // per node
+    int mis_grid = 0;
+    if (mis_async_enabled() && m > 0) {
+        int cus = 0, per_cu = 0;
+        if (!hip_ok(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                               ctx->device),
+                    "hipDeviceGetAttribute") ||
+            !hip_ok(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per_cu, reinterpret_cast<const void *>(k_mis_async), kST, 0),
+                    "hipOccupancyMaxActiveBlocksPerMultiprocessor"))
+            return fail();
+        mis_grid = (int)std::min<long long>((long long)cus * per_cu, (m + kST - 1) / kST);
+    }
     if (!lower_to_full(ctx, ps, m, nred, rrp, rcol, rval, arp, acol, aval) ||
         !hip_ok(ctx, hipMemcpyAsync(d, sdiag, mv * sizeof(double), hipMemcpyDeviceToDevice, st),
                 "copy") ||
@@ -1328,37 +1411,61 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         LAUNCH(k_cand, m, m, arp, d, sdiag, alive, state, max_degree);
         // passes in batches of kMisBatch launches, each counting the candidates left waiting
         // into its own word; a pass after one that left none returns at once
-        int passes = 0;
-        for (;;) {
-            unsigned int und[kMisBatch];
-            if (!hip_ok(ctx, hipMemsetAsync(undecided, 0, kMisBatch * sizeof(unsigned), st),
-                        "memset"))
-                return fail();
-            for (int q = 0; q < kMisBatch; q++)
-                LAUNCH(k_mis_pass, m, m, arp, acol, state,
-                       q > 0 ? undecided + q - 1 : (const unsigned *)nullptr, undecided + q);
-            if (!hip_ok(ctx, hipGetLastError(), "k_mis_pass") ||
-                !d2h_sync(ctx, und, undecided, sizeof(und)))
-                return fail();
-            bool settled = false;
-            for (int q = 0; q < kMisBatch && !settled; q++) {
-                passes++;
-                settled = und[q] == 0;
+        int passes = 0;  // batch passes run (none when the cooperative set settles)
+        auto mis_passes = [&]() -> bool {
+            for (;;) {
+                unsigned int und[kMisBatch];
+                if (!hip_ok(ctx, hipMemsetAsync(undecided, 0, kMisBatch * sizeof(unsigned), st),
+                            "memset"))
+                    return false;
+                for (int q = 0; q < kMisBatch; q++)
+                    LAUNCH(k_mis_pass, m, m, arp, acol, state,
+                           q > 0 ? undecided + q - 1 : (const unsigned *)nullptr, undecided + q);
+                if (!hip_ok(ctx, hipGetLastError(), "k_mis_pass") ||
+                    !d2h_sync(ctx, und, undecided, sizeof(und)))
+                    return false;
+                bool settled = false;
+                for (int q = 0; q < kMisBatch && !settled; q++) {
+                    passes++;
+                    settled = und[q] == 0;
+                }
+                if (settled) return true;
+                if (passes > kMaxMisPasses) {
+                    set_error(ctx, "series rounds: independent set not settled after %d passes",
+                              kMaxMisPasses);
+                    return false;
+                }
             }
-            if (settled) break;
-            if (passes > kMaxMisPasses) {
-                set_error(ctx, "series rounds: independent set not settled after %d passes",
-                          kMaxMisPasses);
+        };
+        // the selected nodes' positions and record offsets; tot[0] selected nodes, tot[1]
+        // record entries, tot[2] whether the cooperative set gave up
+        int tot[8];
+        auto select = [&](bool with_gave_up) -> bool {
+            LAUNCH(k_sel_flags, mv + 1, m, state, arp, sel, c4);
+            if (!cub.scan4(c4, c4s, m + 1)) return false;
+            LAUNCH(k_unpack4, mv + 1, m + 1, c4s, selpos, degoff, (int *)nullptr,
+                   (int *)nullptr);
+            if (with_gave_up)
+                return pick_ints(ctx, totbuf, {{selpos, m}, {degoff, m}, {gave_up, 0}}, tot);
+            tot[2] = 0;
+            return last_ints(ctx, totbuf, {selpos, degoff}, m, tot);
+        };
+        if (mis_grid > 0) {
+            // the whole set in one cooperative launch; should it give up, the passes finish it
+            unsigned sweeps = kMisMaxSweeps;
+            void *args[] = {(void *)&m, (void *)&arp, (void *)&acol, (void *)&state,
+                            (void *)&gave_up, (void *)&sweeps};
+            if (!hip_ok(ctx, hipMemsetAsync(gave_up, 0, sizeof(int), st), "memset") ||
+                !hip_ok(ctx, hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_mis_async),
+                                                        dim3(mis_grid), dim3(kST), args, 0, st),
+                        "hipLaunchCooperativeKernel (independent set)") ||
+                !select(true))
                 return fail();
-            }
+            if (tot[2] != 0 && (!mis_passes() || !select(false))) return fail();
+        } else if (!mis_passes() || !select(false)) {
+            return fail();
         }
-        LAUNCH(k_sel_flags, mv + 1, m, state, arp, sel, c4);
-        int nsel = 0, nent = 0, tot[8];
-        if (!cub.scan4(c4, c4s, m + 1)) return fail();
-        LAUNCH(k_unpack4, mv + 1, m + 1, c4s, selpos, degoff, (int *)nullptr, (int *)nullptr);
-        if (!last_ints(ctx, totbuf, {selpos, degoff}, m, tot)) return fail();
-        nsel = tot[0];
-        nent = tot[1];
+        const int nsel = tot[0], nent = tot[1];
         // a round that would take fewer than 1 % of the nodes left is not worth its pass over
         // the whole system: the rounds stop (the host form stops by the same rule)
         if (nsel == 0 || (long long)nsel * min_round_fraction() < left) break;
