@@ -135,6 +135,14 @@ __global__ __launch_bounds__(kST) void k_rows_check(int n, long long nnz,
     }
 }
 
+__global__ __launch_bounds__(64) void k_row_ends(int n, const int64_t *__restrict__ rp,
+                                                int64_t *__restrict__ out) {
+    if (threadIdx.x == 0) {
+        out[0] = rp[0];
+        out[1] = rp[n];
+    }
+}
+
 // band value (the last stored entry of column r-1) and the count of the row's other
 // entries; each of those is counted in its column's transposed count
 __global__ __launch_bounds__(kST) void k_band_rows(int n, const int64_t *__restrict__ rp,
@@ -212,6 +220,59 @@ __global__ __launch_bounds__(kST) void k_xblk(int nblocks, int n, int rb,
                                               const int *__restrict__ xrp,
                                               int *__restrict__ xblk) {
     GRID_LOOP(k, nblocks + 1) xblk[k] = xrp[std::min<long long>(n, k * (long long)rb)];
+}
+
+// The row blocks' launch order, as build_band orders them: the heavy blocks (more than
+// `limit` half-entries, four times the mean) first, by descending count with ties in block
+// order, then the other blocks in block order.  One work-group: each thread takes a run of
+// blocks, a block scan gives the runs' offsets; the light blocks go straight to their place
+// in `order`, the heavy ones to hv[0, H) in block order (H to *nheavy) for k_band_rank.
+constexpr int kOrderThreads = 1024;
+__global__ __launch_bounds__(kOrderThreads) void k_band_split(int nblocks,
+                                                              const int *__restrict__ xblk,
+                                                              double limit, int *__restrict__ hv,
+                                                              int *__restrict__ nheavy,
+                                                              int *__restrict__ order) {
+    using Scan = hipcub::BlockScan<unsigned long long, kOrderThreads>;
+    __shared__ typename Scan::TempStorage tmp;
+    const int per = (nblocks + kOrderThreads - 1) / kOrderThreads;
+    const int k0 = (int)std::min<long long>(nblocks, (long long)threadIdx.x * per);
+    const int k1 = (int)std::min<long long>(nblocks, (long long)k0 + per);
+    auto heavy = [&](int k) {
+        const int c = xblk[k + 1] - xblk[k];
+        return c > 0 && (double)c > limit;
+    };
+    unsigned long long mine = 0;  // heavy count << 32 | light count
+    for (int k = k0; k < k1; k++) mine += heavy(k) ? (1ull << 32) : 1ull;
+    unsigned long long off, all;
+    Scan(tmp).ExclusiveSum(mine, off, all);
+    const int H = (int)(all >> 32);
+    int h = (int)(off >> 32), l = H + (int)(off & 0xffffffffull);
+    for (int k = k0; k < k1; k++) {
+        if (heavy(k))
+            hv[h++] = k;
+        else
+            order[l++] = k;
+    }
+    if (threadIdx.x == 0) *nheavy = H;
+}
+
+// order[rank] for each heavy block: its rank among the heavy blocks by (descending count,
+// block index)
+__global__ __launch_bounds__(kST) void k_band_rank(int bound, const int *__restrict__ xblk,
+                                                   const int *__restrict__ hv,
+                                                   const int *__restrict__ nheavy,
+                                                   int *__restrict__ order) {
+    const int H = *nheavy;
+    GRID_LOOP(i, std::min(bound, H)) {
+        const int k = hv[i], c = xblk[k + 1] - xblk[k];
+        int rank = 0;
+        for (int j = 0; j < H; j++) {
+            const int kj = hv[j], cj = xblk[kj + 1] - xblk[kj];
+            rank += (cj > c || (cj == c && kj < k)) ? 1 : 0;
+        }
+        order[rank] = k;
+    }
 }
 
 // ---- 2. condensation ---------------------------------------------------------------------
@@ -654,8 +715,10 @@ __global__ __launch_bounds__(kST) void k_sel_flags(int m, const int *__restrict_
                                                    int *__restrict__ sel, Cnt4 *__restrict__ c4) {
     GRID_LOOP(k, m + 1) {
         const int s = k < m && state[k] == kIn;
+        const int deg = s ? arp[k + 1] - arp[k] : 0;
         sel[k] = s;
-        c4[k] = Cnt4{{s, s ? arp[k + 1] - arp[k] : 0, 0, 0}};
+        // selected, record entries, and a bound on the fill the node causes
+        c4[k] = Cnt4{{s, deg, deg * (deg - 1), 0}};
     }
 }
 
@@ -720,10 +783,13 @@ __global__ __launch_bounds__(kST) void k_receivers(int m, const int *__restrict_
                                                    const double *__restrict__ aval,
                                                    const int *__restrict__ sel,
                                                    const int *__restrict__ rec_of,
-                                                   const double *__restrict__ e_d, int RV0, int C0,
+                                                   const double *__restrict__ e_d,
+                                                   const int *__restrict__ rs,
                                                    int *__restrict__ rv_node, int *__restrict__ rv_off,
                                                    int *__restrict__ rv_rec, double *__restrict__ rv_a,
                                                    double *__restrict__ d) {
+    if (rs[0]) return;  // the round overflowed its arrays (k_round_guard)
+    const int RV0 = rs[1], C0 = rs[2];
     GRID_LOOP(u, m) {
         if (!tflag[u]) continue;
         const int t = RV0 + tpos[u];
@@ -745,6 +811,34 @@ __global__ __launch_bounds__(kST) void k_receivers(int m, const int *__restrict_
     }
 }
 
+// Round state on the device (rs): rs[0] an overflow flag, rs[1] and rs[2] the receivers
+// and contributions written so far, rs[3 + q] the first receiver of round q.  The round's
+// arrays are sized from bounds the host reads with the independent set; the exact totals
+// come from the targets' scan and stay on the device.
+constexpr int kMaxRounds = 1024;
+__global__ __launch_bounds__(64) void k_round_guard(int m, const int *__restrict__ tpos,
+                                                    const int *__restrict__ cpos,
+                                                    const int *__restrict__ fpos,
+                                                    const int *__restrict__ upos,
+                                                    int *__restrict__ rs, long long ecap,
+                                                    long long fill_cap, long long ub_cap) {
+    if (threadIdx.x == 0 &&
+        ((long long)rs[1] + tpos[m] > ecap || (long long)rs[2] + cpos[m] > ecap ||
+         fpos[m] > fill_cap || upos[m] > ub_cap))
+        rs[0] = 1;
+}
+__global__ __launch_bounds__(64) void k_round_end(int m, const int *__restrict__ tpos,
+                                                  const int *__restrict__ cpos,
+                                                  int *__restrict__ rs, int q) {
+    if (threadIdx.x == 0) {
+        if (!rs[0]) {
+            rs[1] += tpos[m];
+            rs[2] += cpos[m];
+        }
+        rs[3 + q + 1] = rs[1];
+    }
+}
+
 constexpr int kMergeLists = 8;  // eliminated neighbours merged without staging their fill
 
 // The new row of each surviving node: its old entries without the eliminated neighbours,
@@ -757,9 +851,10 @@ __global__ __launch_bounds__(kST) void k_merge(
     const int *__restrict__ rec_of, const int *__restrict__ e_off, const int *__restrict__ e_nbr,
     const double *__restrict__ e_a, const double *__restrict__ e_d, int *__restrict__ fcol,
     double *__restrict__ fval, int *__restrict__ ocol, double *__restrict__ oval,
-    int *__restrict__ len) {
+    int *__restrict__ len, const int *__restrict__ rs) {
+    const bool overflow = rs[0] != 0;  // nothing is written then (k_round_guard)
     GRID_LOOP(u, m + 1) {
-        if (u == m || !alive[u] || sel[u]) {
+        if (overflow || u == m || !alive[u] || sel[u]) {
             len[u] = 0;
             continue;
         }
@@ -896,7 +991,9 @@ __global__ __launch_bounds__(kST) void k_compact(int m, const int *__restrict__ 
                                                  const int *__restrict__ ocol,
                                                  const double *__restrict__ oval,
                                                  int *__restrict__ ncol,
-                                                 double *__restrict__ nval) {
+                                                 double *__restrict__ nval,
+                                                 const int *__restrict__ rs) {
+    if (rs[0]) return;  // the round overflowed its arrays (k_round_guard)
     GRID_LOOP(u, m) {
         const int b = upos[u], o = nrp[u], l = nrp[u + 1] - nrp[u];
         for (int t = 0; t < l; t++) {
@@ -911,6 +1008,9 @@ __global__ __launch_bounds__(kST) void k_kill(int m, const int *__restrict__ sel
     GRID_LOOP(k, m) if (sel[k]) alive[k] = 0;
 }
 
+__global__ __launch_bounds__(kST) void k_fill_i32(int *__restrict__ p, int n, int v) {
+    GRID_LOOP(i, n) p[i] = v;
+}
 __global__ __launch_bounds__(kST) void k_set_int(int *__restrict__ p, int v) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
@@ -1110,28 +1210,27 @@ bool band_from_device(cicy_sys *s, const int64_t *row_ptr, const int64_t *col_in
     const int n = s->n, nnz = s->nnz, rb = s->rb;
     PoolScope ps(ctx);
     Cub cub{ctx, ps};
-    // row_ptr[0] and row_ptr[n]
-    if (n > 0) {
-        int64_t ends[2] = {0, 0};
-        if (!hip_ok(ctx, hipMemcpyAsync(&ends[0], row_ptr, 8, hipMemcpyDeviceToHost, st), "D2H") ||
-            !hip_ok(ctx, hipMemcpyAsync(&ends[1], row_ptr + n, 8, hipMemcpyDeviceToHost, st),
-                    "D2H") ||
-            !hip_ok(ctx, hipStreamSynchronize(st), "sync"))
-            return false;
-        if (ends[0] != 0 || ends[1] != nnz) {
-            set_error(ctx, "row_ptr must start at 0 and end at nnz (row_ptr[n]=%lld, nnz=%lld)",
-                      (long long)ends[1], (long long)nnz);
-            return false;
-        }
-    }
+    // err[0]: the first bad row's key (k_rows_check, which checks every row's bounds before
+    // it reads the row); err[1], err[2]: row_ptr[0] and row_ptr[n]; one read-back for all
     unsigned long long *err;
-    if (!ps.get(err, 1) ||
+    if (!ps.get(err, 3) ||
         !hip_ok(ctx, hipMemsetAsync(err, 0xff, sizeof(unsigned long long), st), "memset"))
         return false;
-    if (n > 0) LAUNCH(k_rows_check, n, n, (long long)nnz, row_ptr, col_ind, diag, err);
-    unsigned long long key = ~0ull;
-    if (!hip_ok(ctx, hipGetLastError(), "k_rows_check") || !d2h_sync(ctx, &key, err, 8))
+    if (n > 0) {
+        hipLaunchKernelGGL(k_row_ends, dim3(1), dim3(64), 0, st, n, row_ptr,
+                           reinterpret_cast<int64_t *>(err + 1));
+        LAUNCH(k_rows_check, n, n, (long long)nnz, row_ptr, col_ind, diag, err);
+    }
+    unsigned long long got[3] = {~0ull, 0, 0};
+    if (!hip_ok(ctx, hipGetLastError(), "k_rows_check") ||
+        !d2h_sync(ctx, got, err, (n > 0 ? 3 : 1) * sizeof(unsigned long long)))
         return false;
+    const unsigned long long key = got[0];
+    if (n > 0 && ((int64_t)got[1] != 0 || (int64_t)got[2] != nnz)) {
+        set_error(ctx, "row_ptr must start at 0 and end at nnz (row_ptr[n]=%lld, nnz=%lld)",
+                  (long long)(int64_t)got[2], (long long)nnz);
+        return false;
+    }
     if (key != ~0ull) {
         const long long r = (long long)((key >> 31) >> 2);
         const int kind = (int)((key >> 31) & 3);
@@ -1185,33 +1284,22 @@ bool band_from_device(cicy_sys *s, const int64_t *row_ptr, const int64_t *col_in
     if (noff > 0)
         LAUNCH(k_band_transpose, noff, noff, skey, sidx, cstart, s->xrp, own, trow, tval, s->xnbr,
                s->xval);
-    // row blocks and their launch order (heaviest blocks first, as build_band orders them)
-    std::vector<int> xblk((size_t)s->nblocks + 1, 0);
+    // row blocks and their launch order (heaviest blocks first, as build_band orders them),
+    // found on the device
+    int *hv, *nheavy;
     if (!dev_alloc(ctx, &s->xblk, (size_t)s->nblocks + 1) ||
-        !dev_alloc(ctx, &s->xorder, std::max<size_t>((size_t)s->nblocks, 1)))
+        !dev_alloc(ctx, &s->xorder, std::max<size_t>((size_t)s->nblocks, 1)) ||
+        !ps.get(hv, std::max<size_t>((size_t)s->nblocks, 1)) || !ps.get(nheavy, 1))
         return false;
     LAUNCH(k_xblk, s->nblocks + 1, s->nblocks, n, rb, s->xrp, s->xblk);
-    if (!hip_ok(ctx, hipGetLastError(), "band layout") ||
-        !d2h_sync(ctx, xblk.data(), s->xblk, xblk.size() * sizeof(int)))
-        return false;
-    std::vector<int> order((size_t)s->nblocks);
-    const double mean = s->nblocks > 0 ? (double)nhalf / s->nblocks : 0.0;
-    std::vector<std::pair<int, int>> heavy;
-    for (int k = 0; k < s->nblocks; k++) {
-        const int c = xblk[(size_t)k + 1] - xblk[(size_t)k];
-        if (c > 4.0 * mean && c > 0) heavy.push_back({-c, k});
+    if (s->nblocks > 0) {
+        const double mean = (double)nhalf / s->nblocks;
+        hipLaunchKernelGGL(k_band_split, dim3(1), dim3(kOrderThreads), 0, st, s->nblocks, s->xblk,
+                           4.0 * mean, hv, nheavy, s->xorder);
+        LAUNCH(k_band_rank, s->nblocks, s->nblocks, s->xblk, hv, nheavy, s->xorder);
     }
-    std::sort(heavy.begin(), heavy.end());
-    std::vector<char> first((size_t)s->nblocks, 0);
-    int at = 0;
-    for (auto &e : heavy) {
-        order[(size_t)at++] = e.second;
-        first[(size_t)e.second] = 1;
-    }
-    for (int k = 0; k < s->nblocks; k++)
-        if (!first[(size_t)k]) order[(size_t)at++] = k;
-    return hip_ok(ctx, hipMemcpyAsync(s->xorder, order.data(), order.size() * sizeof(int),
-                                      hipMemcpyHostToDevice, st), "H2D") &&
+    // the caller's arrays are read only during the call: wait for the last read
+    return hip_ok(ctx, hipGetLastError(), "band layout") &&
            hip_ok(ctx, hipMemcpyAsync(s->diag, diag, nv * sizeof(double),
                                       hipMemcpyDeviceToDevice, st), "copy diagonal") &&
            hip_ok(ctx, hipStreamSynchronize(st), "sync");
@@ -1350,7 +1438,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
     int *arp, *acol, *brp, *bcol;
     double *aval, *bval;
     int *alive, *state, *sel, *selpos, *degoff, *rec_of, *tflag, *tpos, *cpos, *fpos, *upos, *len,
-        *totbuf;
+        *totbuf, *rs;
     Cnt4 *c4, *c4s;
     double *d;
     unsigned int *undecided;
@@ -1362,8 +1450,13 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         !ps.get(tpos, mv + 1) || !ps.get(cpos, mv + 1) ||
         !ps.get(fpos, mv + 1) ||
         !ps.get(upos, mv + 1) || !ps.get(len, mv + 1) || !ps.get(d, mv + 1) ||
-        !ps.get(undecided, kMisBatch + 1) || !ps.get(totbuf, 8))
+        !ps.get(undecided, kMisBatch + 1) || !ps.get(totbuf, 8) ||
+        !ps.get(rs, 3 + kMaxRounds + 1) ||
+        !hip_ok(ctx, hipMemsetAsync(rs, 0, (3 + kMaxRounds + 1) * sizeof(int), st), "memset") ||
+        !hip_ok(ctx, hipMemsetAsync(undecided, 0, (kMisBatch + 1) * sizeof(unsigned), st),
+                "memset"))
         return fail();
+    // set by k_mis_async when it gives up; cleared again before the passes finish the set
     int *gave_up = reinterpret_cast<int *>(undecided + kMisBatch);
     // the cooperative independent-set grid: every work-group resident, at most one thread
     // per node
@@ -1384,14 +1477,8 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
                 "copy") ||
         !hip_ok(ctx, hipMemsetAsync(alive, 0, (mv + 1) * sizeof(int), st), "memset"))
         return fail();
-    {
-        // every node starts alive: alive = 1 on [0, m)
-        std::vector<int> ones(mv, 1);
-        if (m > 0 && !hip_ok(ctx, hipMemcpyAsync(alive, ones.data(), mv * sizeof(int),
-                                                 hipMemcpyHostToDevice, st), "H2D"))
-            return fail();
-        if (!hip_ok(ctx, hipStreamSynchronize(st), "sync")) return fail();
-    }
+    // every node starts alive: alive = 1 on [0, m)
+    if (m > 0) LAUNCH(k_fill_i32, m, alive, m, 1);
     // record and receiver arrays: at most m records, D m record entries, D m contributions
     // (each entry of a record is one contribution), D m receivers (D the degree cap)
     const int max_degree = elim_degree();
@@ -1437,32 +1524,43 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
                 }
             }
         };
-        // the selected nodes' positions and record offsets; tot[0] selected nodes, tot[1]
-        // record entries, tot[2] whether the cooperative set gave up
+        // tot: 0 selected nodes, 1 their record entries, 2 a bound on the round's fill,
+        // 3 the adjacency's entries, 4 whether the cooperative set gave up, 5 whether the
+        // last round overflowed its arrays
         int tot[8];
-        auto select = [&](bool with_gave_up) -> bool {
+        auto select = [&]() -> bool {
             LAUNCH(k_sel_flags, mv + 1, m, state, arp, sel, c4);
             if (!cub.scan4(c4, c4s, m + 1)) return false;
             LAUNCH(k_unpack4, mv + 1, m + 1, c4s, selpos, degoff, (int *)nullptr,
                    (int *)nullptr);
-            if (with_gave_up)
-                return pick_ints(ctx, totbuf, {{selpos, m}, {degoff, m}, {gave_up, 0}}, tot);
-            tot[2] = 0;
-            return last_ints(ctx, totbuf, {selpos, degoff}, m, tot);
+            return pick_ints(ctx, totbuf,
+                             {{selpos, m},
+                              {degoff, m},
+                              {reinterpret_cast<const int *>(c4s), 4 * m + 2},
+                              {arp, m},
+                              {gave_up, 0},
+                              {rs, 0}},
+                             tot);
         };
         if (mis_grid > 0) {
             // the whole set in one cooperative launch; should it give up, the passes finish it
             unsigned sweeps = kMisMaxSweeps;
             void *args[] = {(void *)&m, (void *)&arp, (void *)&acol, (void *)&state,
                             (void *)&gave_up, (void *)&sweeps};
-            if (!hip_ok(ctx, hipMemsetAsync(gave_up, 0, sizeof(int), st), "memset") ||
-                !hip_ok(ctx, hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_mis_async),
+            if (!hip_ok(ctx, hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_mis_async),
                                                         dim3(mis_grid), dim3(kST), args, 0, st),
                         "hipLaunchCooperativeKernel (independent set)") ||
-                !select(true))
+                !select())
                 return fail();
-            if (tot[2] != 0 && (!mis_passes() || !select(false))) return fail();
-        } else if (!mis_passes() || !select(false)) {
+            if (tot[4] != 0 &&
+                (!hip_ok(ctx, hipMemsetAsync(gave_up, 0, sizeof(int), st), "memset") ||
+                 !mis_passes() || !select()))
+                return fail();
+        } else if (!mis_passes() || !select()) {
+            return fail();
+        }
+        if (tot[5] != 0) {
+            set_error(ctx, "series rounds: receiver or merge arrays overflow");
             return fail();
         }
         const int nsel = tot[0], nent = tot[1];
@@ -1470,61 +1568,73 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         // the whole system: the rounds stop (the host form stops by the same rule)
         if (nsel == 0 || (long long)nsel * min_round_fraction() < left) break;
         left -= nsel;
-        if ((size_t)R0 + nsel > rcap - 1 || (size_t)E0 + nent > ecap - 1) {
+        const int q = (int)c->round_off.size() - 1;
+        if ((size_t)R0 + nsel > rcap - 1 || (size_t)E0 + nent > ecap - 1 || q >= kMaxRounds) {
             set_error(ctx, "series rounds: record arrays overflow");
             return fail();
         }
+        // the fill of a receiver is at most deg - 1 per eliminated neighbour of degree deg,
+        // and a new row holds at most its old entries and its fill
+        const long long fill_cap = tot[2], ub_cap = (long long)tot[3] + tot[2];
         LAUNCH(k_records, m, m, sel, selpos, degoff, arp, acol, aval, d, R0, E0, c->e_node,
                c->e_off, c->e_d, c->e_nbr, c->e_a, rec_of);
         LAUNCH(k_targets, mv + 1, m, alive, sel, arp, acol, tflag, c4);
         if (!cub.scan4(c4, c4s, m + 1)) return fail();
         LAUNCH(k_unpack4, mv + 1, m + 1, c4s, tpos, cpos, fpos, upos);
-        if (!last_ints(ctx, totbuf, {tpos, cpos, fpos, upos}, m, tot)) return fail();
-        const int ntarg = tot[0], ncontrib = tot[1], nfill = tot[2], nub = tot[3];
-        if ((size_t)RV0 + ntarg > ecap - 1 || (size_t)C0 + ncontrib > ecap - 1) {
-            set_error(ctx, "series rounds: receiver arrays overflow");
-            return fail();
-        }
-        LAUNCH(k_receivers, m, m, tflag, tpos, cpos, arp, acol, aval, sel, rec_of, c->e_d, RV0, C0,
-               c->rv_node, c->rv_off, c->rv_rec, c->rv_a, d);
         int *fc, *oc;
         double *fvv, *ov;
-        if (!ps.get(fc, (size_t)nfill) || !ps.get(fvv, (size_t)nfill) || !ps.get(oc, (size_t)nub) ||
-            !ps.get(ov, (size_t)nub))
+        if (!ps.get(fc, (size_t)fill_cap) || !ps.get(fvv, (size_t)fill_cap) ||
+            !ps.get(oc, (size_t)ub_cap) || !ps.get(ov, (size_t)ub_cap))
             return fail();
+        hipLaunchKernelGGL(k_round_guard, dim3(1), dim3(64), 0, st, m, tpos, cpos, fpos, upos, rs,
+                           (long long)ecap - 1, fill_cap, ub_cap);
+        LAUNCH(k_receivers, m, m, tflag, tpos, cpos, arp, acol, aval, sel, rec_of, c->e_d, rs,
+               c->rv_node, c->rv_off, c->rv_rec, c->rv_a, d);
         // the end of the round's last record (the merge reads e_off[rec + 1])
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, c->e_off + R0 + nsel, E0 + nent);
         LAUNCH(k_merge, mv + 1, m, alive, sel, tflag, arp, acol, aval, upos, fpos, rec_of, c->e_off,
-               c->e_nbr, c->e_a, c->e_d, fc, fvv, oc, ov, len);
-        // the new rows hold at most nub entries (the merge's bound), so no read-back of their
-        // true total is needed
-        if (!cub.scan(len, brp, m + 1) || !ps.get(bcol, (size_t)std::max(nub, 1)) ||
-            !ps.get(bval, (size_t)std::max(nub, 1)))
+               c->e_nbr, c->e_a, c->e_d, fc, fvv, oc, ov, len, rs);
+        // the new rows hold at most ub_cap entries, so no read-back of their true total is
+        // needed
+        if (!cub.scan(len, brp, m + 1) || !ps.get(bcol, (size_t)std::max(ub_cap, 1ll)) ||
+            !ps.get(bval, (size_t)std::max(ub_cap, 1ll)))
             return fail();
-        LAUNCH(k_compact, m, m, upos, brp, oc, ov, bcol, bval);
+        LAUNCH(k_compact, m, m, upos, brp, oc, ov, bcol, bval, rs);
+        hipLaunchKernelGGL(k_round_end, dim3(1), dim3(64), 0, st, m, tpos, cpos, rs, q);
         LAUNCH(k_kill, m, m, sel, alive);
         // return the old adjacency and the merge buffers to the pool, keep the new one
-        for (void *q : {(void *)acol, (void *)aval, (void *)fc, (void *)fvv, (void *)oc, (void *)ov}) {
-            ctx->pool.put(q);
-            ps.got.erase(std::remove(ps.got.begin(), ps.got.end(), q), ps.got.end());
+        for (void *p : {(void *)acol, (void *)aval, (void *)fc, (void *)fvv, (void *)oc, (void *)ov}) {
+            ctx->pool.put(p);
+            ps.got.erase(std::remove(ps.got.begin(), ps.got.end(), p), ps.got.end());
         }
         std::swap(arp, brp);
         acol = bcol;
         aval = bval;
         R0 += nsel;
         E0 += nent;
-        RV0 += ntarg;
-        C0 += ncontrib;
         c->round_off.push_back(R0);
-        c->rv_round_off.push_back(RV0);
         if (!hip_ok(ctx, hipGetLastError(), "series round")) return fail();
         if (clk.on) {
-            (void)hipStreamSynchronize(st);
+            int ex[8];
+            if (!last_ints(ctx, totbuf, {tpos, fpos}, m, ex)) return fail();
             fprintf(stderr, "[cicy setup]   round %zu: %d eliminated, %d receivers, %d fill, "
-                    "%d independent-set passes\n", c->round_off.size() - 1, nsel, ntarg, nfill,
+                    "%d independent-set passes\n", c->round_off.size() - 1, nsel, ex[0], ex[1],
                     passes);
             clk.mark("device condense: round");
         }
+    }
+    // the receivers' round offsets and totals, with the overflow flag of the last round
+    const int rounds = (int)c->round_off.size() - 1;
+    {
+        std::vector<int> h((size_t)rounds + 4);
+        if (!d2h_sync(ctx, h.data(), rs, h.size() * sizeof(int))) return fail();
+        if (h[0] != 0) {
+            set_error(ctx, "series rounds: receiver or merge arrays overflow");
+            return fail();
+        }
+        RV0 = h[1];
+        C0 = h[2];
+        c->rv_round_off.assign(h.begin() + 3, h.end());
     }
     // closing offsets of the record and receiver lists
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, c->e_off + R0, E0);
@@ -1539,9 +1649,12 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
     int mc = 0, nnz2 = 0;
     if (m > 0) {
         LAUNCH(k_core_len, mv + 1, m, alive, arp, cflag, clen);
+        int ct[8];
         if (!cub.scan(cflag, core_of, m + 1) || !cub.scan(clen, cpos2, m + 1) ||
-            !last_int(ctx, core_of, m, &mc) || !last_int(ctx, cpos2, m, &nnz2))
+            !last_ints(ctx, totbuf, {core_of, cpos2}, m, ct))
             return fail();
+        mc = ct[0];
+        nnz2 = ct[1];
     }
     c->mc = mc;
     c->nnz_core = nnz2 / 2;

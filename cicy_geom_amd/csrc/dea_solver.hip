@@ -655,6 +655,31 @@ bool sum_squares(cicy_ctx *ctx, const double *v, size_t n, double *out) {
     return true;
 }
 
+// sums of squares of two vectors of n entries (v2 may be null: *out2 = 0), one read-back
+bool sum_squares2(cicy_ctx *ctx, const double *v1, const double *v2, size_t n, double *out1,
+                  double *out2) {
+    double *part = nullptr;
+    double h[2 * kSqGroups];
+    *out1 = *out2 = 0.0;
+    if (n == 0) return true;
+    PoolScope ps(ctx);
+    if (!ps.get(part, 2 * kSqGroups)) return false;
+    hipLaunchKernelGGL(k_sum_squares, dim3(kSqGroups), dim3(kThreads), 0, ctx->stream, n, v1, part);
+    if (v2)
+        hipLaunchKernelGGL(k_sum_squares, dim3(kSqGroups), dim3(kThreads), 0, ctx->stream, n, v2,
+                           part + kSqGroups);
+    if (!hip_ok(ctx, hipGetLastError(), "k_sum_squares") ||
+        !d2h_sync(ctx, h, part, (v2 ? 2 : 1) * kSqGroups * sizeof(double)))
+        return false;
+    double t1 = 0.0, t2 = 0.0;
+    for (int g = 0; g < kSqGroups; g++) t1 += h[g];
+    if (v2)
+        for (int g = 0; g < kSqGroups; g++) t2 += h[kSqGroups + g];
+    *out1 = t1;
+    *out2 = t2;
+    return true;
+}
+
 bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel, double *b2) {
     cicy_ctx *ctx = s->ctx;
     if (!s->has_b) {
@@ -1112,8 +1137,7 @@ int cicy_sys_set_rhs_device(cicy_sys *s, const double *d_r0, const double *d_x0,
         ok = ok && hip_ok(ctx, hipMemcpyAsync(s->b, d_b, nv * sizeof(double),
                                               hipMemcpyDeviceToDevice, st), "copy b");
     double r02 = 0.0, b2 = 0.0;
-    ok = ok && cicy::sum_squares(ctx, s->r0, nv, &r02) &&
-         (!d_b || cicy::sum_squares(ctx, s->b, nv, &b2));
+    ok = ok && cicy::sum_squares2(ctx, s->r0, d_b ? s->b : nullptr, nv, &r02, &b2);
     if (!ok) return 0;
     s->r0norm2 = r02;
     s->has_b = d_b != nullptr;
