@@ -237,6 +237,7 @@ struct cicy_ctx {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
     cicy::PcgState *host_state = nullptr;  // pinned, 2 slots
+    void *readback = nullptr;               // pinned, kReadbackBytes: small read-backs land here
     double *bc_ws = nullptr;   // boundary-current workspace (grows, kept until destroy)
     double *scratch = nullptr; // this device's copy of the allocation-free reduction words
     size_t bc_words = 0;

@@ -224,55 +224,73 @@ __global__ __launch_bounds__(kST) void k_xblk(int nblocks, int n, int rb,
 
 // The row blocks' launch order, as build_band orders them: the heavy blocks (more than
 // `limit` half-entries, four times the mean) first, by descending count with ties in block
-// order, then the other blocks in block order.  One work-group: each thread takes a run of
-// blocks, a block scan gives the runs' offsets; the light blocks go straight to their place
-// in `order`, the heavy ones to hv[0, H) in block order (H to *nheavy) for k_band_rank.
+// order, then the other blocks in block order.  One work-group walks the blocks in chunks of
+// its size, a block scan per chunk placing the light blocks straight into `order` and the
+// heavy ones, with their counts, into hv/hc[0, H) in block order (H to *nheavy) for
+// k_band_rank.
 constexpr int kOrderThreads = 1024;
 __global__ __launch_bounds__(kOrderThreads) void k_band_split(int nblocks,
                                                               const int *__restrict__ xblk,
                                                               double limit, int *__restrict__ hv,
+                                                              int *__restrict__ hc,
                                                               int *__restrict__ nheavy,
                                                               int *__restrict__ order) {
-    using Scan = hipcub::BlockScan<unsigned long long, kOrderThreads>;
+    using Scan = hipcub::BlockScan<int, kOrderThreads>;
     __shared__ typename Scan::TempStorage tmp;
-    const int per = (nblocks + kOrderThreads - 1) / kOrderThreads;
-    const int k0 = (int)std::min<long long>(nblocks, (long long)threadIdx.x * per);
-    const int k1 = (int)std::min<long long>(nblocks, (long long)k0 + per);
-    auto heavy = [&](int k) {
-        const int c = xblk[k + 1] - xblk[k];
-        return c > 0 && (double)c > limit;
-    };
-    unsigned long long mine = 0;  // heavy count << 32 | light count
-    for (int k = k0; k < k1; k++) mine += heavy(k) ? (1ull << 32) : 1ull;
-    unsigned long long off, all;
-    Scan(tmp).ExclusiveSum(mine, off, all);
-    const int H = (int)(all >> 32);
-    int h = (int)(off >> 32), l = H + (int)(off & 0xffffffffull);
-    for (int k = k0; k < k1; k++) {
-        if (heavy(k))
-            hv[h++] = k;
-        else
-            order[l++] = k;
+    int h0 = 0, l0 = 0;  // heavy and light blocks before this chunk
+    for (int base = 0; base < nblocks; base += kOrderThreads) {
+        const int k = base + (int)threadIdx.x;
+        int c = 0;
+        bool heavy = false;
+        if (k < nblocks) {
+            c = xblk[k + 1] - xblk[k];
+            heavy = c > 0 && (double)c > limit;
+        }
+        int off, nh;
+        Scan(tmp).ExclusiveSum(heavy ? 1 : 0, off, nh);
+        if (k < nblocks) {
+            if (heavy) {
+                hv[h0 + off] = k;
+                hc[h0 + off] = c;
+            } else {
+                // light blocks go after every heavy one: their final place is fixed once H is
+                // known, so keep their rank among the light blocks for now
+                order[l0 + ((int)threadIdx.x - off)] = k;
+            }
+        }
+        h0 += nh;
+        l0 += std::min(kOrderThreads, nblocks - base) - nh;
+        __syncthreads();  // tmp is reused by the next chunk's scan
     }
-    if (threadIdx.x == 0) *nheavy = H;
+    if (threadIdx.x == 0) *nheavy = h0;
 }
 
-// order[rank] for each heavy block: its rank among the heavy blocks by (descending count,
-// block index)
-__global__ __launch_bounds__(kST) void k_band_rank(int bound, const int *__restrict__ xblk,
-                                                   const int *__restrict__ hv,
+// Each heavy block's rank among the heavy blocks by (descending count, block index), one
+// wave per heavy block (its lanes read the heavy list in coalesced strides and the counts
+// are summed across the wave); order[0, H) takes the heavy blocks and the light ones
+// (listed at light[0, L) by k_band_split) move up by H.
+__global__ __launch_bounds__(kST) void k_band_rank(int nblocks, const int *__restrict__ hv,
+                                                   const int *__restrict__ hc,
                                                    const int *__restrict__ nheavy,
+                                                   const int *__restrict__ light,
                                                    int *__restrict__ order) {
     const int H = *nheavy;
-    GRID_LOOP(i, std::min(bound, H)) {
-        const int k = hv[i], c = xblk[k + 1] - xblk[k];
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const long long waves = (long long)gridDim.x * (kST / kWave);
+    for (long long i = ((long long)blockIdx.x * kST + threadIdx.x) / kWave; i < H; i += waves) {
+        const int k = hv[i], c = hc[i];
         int rank = 0;
-        for (int j = 0; j < H; j++) {
-            const int kj = hv[j], cj = xblk[kj + 1] - xblk[kj];
+        for (int j = lane; j < H; j += kWave) {
+            const int cj = hc[j], kj = hv[j];
             rank += (cj > c || (cj == c && kj < k)) ? 1 : 0;
         }
-        order[rank] = k;
+        for (int o = kWave / 2; o > 0; o >>= 1) rank += __shfl_xor(rank, o, kWave);
+        if (lane == 0) order[rank] = k;
     }
+    // the light blocks, in block order after the heavy ones
+    for (long long t = (long long)blockIdx.x * kST + threadIdx.x; t < (long long)nblocks - H;
+         t += (long long)gridDim.x * kST)
+        order[H + t] = light[t];
 }
 
 // ---- 2. condensation ---------------------------------------------------------------------
@@ -1026,8 +1044,10 @@ struct PickArgs {
     const int *p[8];
     int at[8];
 };
-__global__ __launch_bounds__(64) void k_pick_ints(PickArgs a, int k, int *__restrict__ out) {
+__global__ __launch_bounds__(64) void k_pick_ints(PickArgs a, int k, const int *__restrict__ extra,
+                                                 int nextra, int *__restrict__ out) {
     if ((int)threadIdx.x < k) out[threadIdx.x] = a.p[threadIdx.x][a.at[threadIdx.x]];
+    for (int t = (int)threadIdx.x; t < nextra; t += 64) out[8 + t] = extra[t];
 }
 
 // *bad = 1 when some v[i] is not positive and finite
@@ -1137,9 +1157,10 @@ bool last_int(cicy_ctx *ctx, const int *p, long long at, int *out) {
     return d2h_sync(ctx, out, p + at, sizeof(int));
 }
 
-// entries of device int arrays (each at its own index), with one wait; `buf` holds 8 ints
+// entries of device int arrays (each at its own index) and, optionally, the first nextra
+// entries of `extra` (into extra_out), with one wait; `buf` holds 8 + nextra ints
 bool pick_ints(cicy_ctx *ctx, int *buf, std::initializer_list<std::pair<const int *, int>> at,
-               int *out) {
+               int *out, const int *extra = nullptr, int nextra = 0, int *extra_out = nullptr) {
     PickArgs a{};
     int k = 0;
     for (const auto &x : at) {
@@ -1147,9 +1168,14 @@ bool pick_ints(cicy_ctx *ctx, int *buf, std::initializer_list<std::pair<const in
         a.at[k] = x.second;
         k++;
     }
-    hipLaunchKernelGGL(k_pick_ints, dim3(1), dim3(64), 0, ctx->stream, a, k, buf);
-    return hip_ok(ctx, hipGetLastError(), "k_pick_ints") &&
-           d2h_sync(ctx, out, buf, (size_t)k * sizeof(int));
+    hipLaunchKernelGGL(k_pick_ints, dim3(1), dim3(64), 0, ctx->stream, a, k, extra, nextra, buf);
+    std::vector<int> h((size_t)8 + nextra);
+    if (!hip_ok(ctx, hipGetLastError(), "k_pick_ints") ||
+        !d2h_sync(ctx, h.data(), buf, (nextra > 0 ? 8 + (size_t)nextra : (size_t)k) * sizeof(int)))
+        return false;
+    std::copy(h.begin(), h.begin() + k, out);
+    if (nextra > 0) std::copy(h.begin() + 8, h.end(), extra_out);
+    return true;
 }
 
 // the entries at `at` of several device int arrays, with one wait; `buf` holds 8 ints
@@ -1286,17 +1312,17 @@ bool band_from_device(cicy_sys *s, const int64_t *row_ptr, const int64_t *col_in
                s->xval);
     // row blocks and their launch order (heaviest blocks first, as build_band orders them),
     // found on the device
-    int *hv, *nheavy;
-    if (!dev_alloc(ctx, &s->xblk, (size_t)s->nblocks + 1) ||
-        !dev_alloc(ctx, &s->xorder, std::max<size_t>((size_t)s->nblocks, 1)) ||
-        !ps.get(hv, std::max<size_t>((size_t)s->nblocks, 1)) || !ps.get(nheavy, 1))
+    int *hv, *hc, *light, *nheavy;
+    const size_t nbv = std::max<size_t>((size_t)s->nblocks, 1);
+    if (!dev_alloc(ctx, &s->xblk, (size_t)s->nblocks + 1) || !dev_alloc(ctx, &s->xorder, nbv) ||
+        !ps.get(hv, nbv) || !ps.get(hc, nbv) || !ps.get(light, nbv) || !ps.get(nheavy, 1))
         return false;
     LAUNCH(k_xblk, s->nblocks + 1, s->nblocks, n, rb, s->xrp, s->xblk);
     if (s->nblocks > 0) {
         const double mean = (double)nhalf / s->nblocks;
         hipLaunchKernelGGL(k_band_split, dim3(1), dim3(kOrderThreads), 0, st, s->nblocks, s->xblk,
-                           4.0 * mean, hv, nheavy, s->xorder);
-        LAUNCH(k_band_rank, s->nblocks, s->nblocks, s->xblk, hv, nheavy, s->xorder);
+                           4.0 * mean, hv, hc, nheavy, light);
+        LAUNCH(k_band_rank, s->nblocks, s->nblocks, hv, hc, nheavy, light, s->xorder);
     }
     // the caller's arrays are read only during the call: wait for the last read
     return hip_ok(ctx, hipGetLastError(), "band layout") &&
@@ -1450,7 +1476,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         !ps.get(tpos, mv + 1) || !ps.get(cpos, mv + 1) ||
         !ps.get(fpos, mv + 1) ||
         !ps.get(upos, mv + 1) || !ps.get(len, mv + 1) || !ps.get(d, mv + 1) ||
-        !ps.get(undecided, kMisBatch + 1) || !ps.get(totbuf, 8) ||
+        !ps.get(undecided, kMisBatch + 1) || !ps.get(totbuf, 8 + 3 + kMaxRounds + 1) ||
         !ps.get(rs, 3 + kMaxRounds + 1) ||
         !hip_ok(ctx, hipMemsetAsync(rs, 0, (3 + kMaxRounds + 1) * sizeof(int), st), "memset") ||
         !hip_ok(ctx, hipMemsetAsync(undecided, 0, (kMisBatch + 1) * sizeof(unsigned), st),
@@ -1493,6 +1519,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
     c->round_off.assign(1, 0);
     c->rv_round_off.assign(1, 0);
     int R0 = 0, E0 = 0, RV0 = 0, C0 = 0;
+    std::vector<int> rs_host(3 + kMaxRounds + 1, 0);  // rs as of the last selection
     long long left = m;  // nodes not eliminated yet
     while (m > 0) {
         LAUNCH(k_cand, m, m, arp, d, sdiag, alive, state, max_degree);
@@ -1528,6 +1555,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         // 3 the adjacency's entries, 4 whether the cooperative set gave up, 5 whether the
         // last round overflowed its arrays
         int tot[8];
+        const int q = (int)c->round_off.size() - 1;  // this round's index
         auto select = [&]() -> bool {
             LAUNCH(k_sel_flags, mv + 1, m, state, arp, sel, c4);
             if (!cub.scan4(c4, c4s, m + 1)) return false;
@@ -1540,7 +1568,7 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
                               {arp, m},
                               {gave_up, 0},
                               {rs, 0}},
-                             tot);
+                             tot, rs, q + 4, rs_host.data());
         };
         if (mis_grid > 0) {
             // the whole set in one cooperative launch; should it give up, the passes finish it
@@ -1568,7 +1596,6 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
         // the whole system: the rounds stop (the host form stops by the same rule)
         if (nsel == 0 || (long long)nsel * min_round_fraction() < left) break;
         left -= nsel;
-        const int q = (int)c->round_off.size() - 1;
         if ((size_t)R0 + nsel > rcap - 1 || (size_t)E0 + nent > ecap - 1 || q >= kMaxRounds) {
             set_error(ctx, "series rounds: record arrays overflow");
             return fail();
@@ -1623,19 +1650,12 @@ cicy_cond *condensed_setup_device(cicy_sys *s, const char **why, bool *failed) {
             clk.mark("device condense: round");
         }
     }
-    // the receivers' round offsets and totals, with the overflow flag of the last round
+    // the receivers' round offsets and totals, read with the last round's selection (or, with
+    // no round at all, zeros)
     const int rounds = (int)c->round_off.size() - 1;
-    {
-        std::vector<int> h((size_t)rounds + 4);
-        if (!d2h_sync(ctx, h.data(), rs, h.size() * sizeof(int))) return fail();
-        if (h[0] != 0) {
-            set_error(ctx, "series rounds: receiver or merge arrays overflow");
-            return fail();
-        }
-        RV0 = h[1];
-        C0 = h[2];
-        c->rv_round_off.assign(h.begin() + 3, h.end());
-    }
+    RV0 = rs_host[1];
+    C0 = rs_host[2];
+    c->rv_round_off.assign(rs_host.begin() + 3, rs_host.begin() + 3 + rounds + 1);
     // closing offsets of the record and receiver lists
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, c->e_off + R0, E0);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, c->rv_off + RV0, C0);

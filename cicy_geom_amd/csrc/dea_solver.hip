@@ -252,7 +252,18 @@ bool hip_ok(cicy_ctx *ctx, hipError_t e, const char *what) {
     return false;
 }
 
+// Small read-backs go through the context's pinned buffer: a copy into pageable memory is
+// staged by the runtime and waits longer.
+constexpr size_t kReadbackBytes = 64 << 10;
 bool d2h_sync(cicy_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (ctx->readback && bytes <= kReadbackBytes) {
+        if (!hip_ok(ctx, hipMemcpyAsync(ctx->readback, src, bytes, hipMemcpyDeviceToHost,
+                                        ctx->stream), "D2H") ||
+            !hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync"))
+            return false;
+        memcpy(dst, ctx->readback, bytes);
+        return true;
+    }
     return hip_ok(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream),
                   "D2H") &&
            hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
@@ -801,7 +812,10 @@ int cicy_create(cicy_ctx **out, int device) {
         hipEventCreateWithFlags(&ctx->ev_poll[1], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&ctx->host_state, 2 * sizeof(PcgState), hipHostMallocDefault) !=
             hipSuccess ||
+        hipHostMalloc(&ctx->readback, cicy::kReadbackBytes, hipHostMallocDefault) != hipSuccess ||
         !(ctx->scratch = scratch_address(device))) {
+        if (ctx->host_state) (void)hipHostFree(ctx->host_state);
+        if (ctx->readback) (void)hipHostFree(ctx->readback);
         delete ctx;
         return 0;
     }
@@ -814,6 +828,7 @@ void cicy_destroy(cicy_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->host_state) (void)hipHostFree(ctx->host_state);
+    if (ctx->readback) (void)hipHostFree(ctx->readback);
     cicy::dev_release(ctx, ctx->bc_ws);
     ctx->pool.clear();
     (void)hipEventDestroy(ctx->ev_start);
@@ -974,8 +989,7 @@ bool sys_alloc_vectors(cicy_sys *s, size_t *total) {
          hip_ok(ctx, hipMemsetAsync(s->gcount, 0,
                                     std::max<size_t>((size_t)s->nblocks, 1) * sizeof(unsigned),
                                     ctx->stream), "memset") &&
-         hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), ctx->stream), "memset") &&
-         hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+         hip_ok(ctx, hipMemsetAsync(s->st, 0, sizeof(PcgState), ctx->stream), "memset");
     *total += tot;
     return ok;
 }
