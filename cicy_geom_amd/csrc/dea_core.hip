@@ -385,7 +385,8 @@ bool build_core_host(int m, const std::vector<int64_t> &crp, const std::vector<i
             if (e - q >= 2) groups.emplace_back(q, e);
         }
     }
-    std::sort(groups.begin(), groups.end());
+    // the caller's groups usually come in row order already
+    if (!std::is_sorted(groups.begin(), groups.end())) std::sort(groups.begin(), groups.end());
     for (size_t g = 1; g < groups.size(); g++)
         if (groups[g].first < groups[g - 1].second) {
             *why = "row groups overlap";
@@ -904,9 +905,7 @@ bool up_vec(cicy_ctx *ctx, cicy_core *z, T **dst, const std::vector<T> &src) {
     if (!core_alloc(ctx, (void **)dst, bytes)) return false;
     z->bytes += bytes;
     if (src.empty()) return true;
-    return hip_ok(ctx, hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T),
-                                      hipMemcpyHostToDevice, ctx->stream),
-                  "H2D (core)");
+    return h2d_async(ctx, *dst, src.data(), src.size() * sizeof(T), ctx->stream);
 }
 
 bool zeros(cicy_ctx *ctx, cicy_core *z, double **dst, size_t count) {
@@ -1361,8 +1360,7 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
         int64_t *dr;
         int *dq;
         if (!ps.get(dr, 2 * (size_t)nblocks) || !ps.get(dq, 2 * (size_t)nblocks) ||
-            !hip_ok(ctx, hipMemcpyAsync(dr, ranges, 2 * (size_t)nblocks * sizeof(int64_t),
-                                        hipMemcpyHostToDevice, st), "H2D"))
+            !h2d_async(ctx, dr, ranges, 2 * (size_t)nblocks * sizeof(int64_t), st))
             return nullptr;
         hipLaunchKernelGGL(k_group_bounds, dim3(core_grid(2 * nblocks)), dim3(kLT), 0, st,
                            (int)(2 * nblocks), dr, orig, m, dq);
@@ -1370,6 +1368,7 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
             !d2h_sync(ctx, qb.data(), dq, 2 * (size_t)nblocks * sizeof(int)))
             return nullptr;
     }
+    clk.mark("device core: group bounds");
     std::vector<std::pair<int, int>> groups;
     for (int64_t b = 0; b < nblocks; b++) {
         const int q0 = qb[(size_t)(2 * b)], q1 = qb[(size_t)(2 * b + 1)];
@@ -1378,7 +1377,8 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
             if (e - q >= 2) groups.emplace_back(q, e);
         }
     }
-    std::sort(groups.begin(), groups.end());
+    // the caller's groups usually come in row order already
+    if (!std::is_sorted(groups.begin(), groups.end())) std::sort(groups.begin(), groups.end());
     for (size_t g = 1; g < groups.size(); g++)
         if (groups[g].first < groups[g - 1].second) {
             set_error(ctx, "core blocks: row groups overlap");
@@ -1398,33 +1398,36 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
     int *dgq0, *dgnb, *dok, *dord;
     long long *dgoff;
     double *dinv, *dcg;
+    // groups by descending size, equal sizes in group order (a counting sort: sizes are at
+    // most nbmax)
     std::vector<int> okh((size_t)ng, 0), gord((size_t)ng);
-    for (int g = 0; g < ng; g++) gord[(size_t)g] = g;
-    std::stable_sort(gord.begin(), gord.end(),
-                     [&](int a, int b) { return gnb[(size_t)a] > gnb[(size_t)b]; });
+    {
+        std::vector<int> start((size_t)nbmax + 2, 0);
+        for (int g = 0; g < ng; g++) start[(size_t)(nbmax - gnb[(size_t)g]) + 1]++;
+        for (int v = 0; v <= nbmax; v++) start[(size_t)v + 1] += start[(size_t)v];
+        for (int g = 0; g < ng; g++) gord[(size_t)start[(size_t)(nbmax - gnb[(size_t)g])]++] = g;
+    }
     if (!ps.get(dgq0, (size_t)ng) || !ps.get(dgnb, (size_t)ng) || !ps.get(dok, (size_t)ng) ||
         !ps.get(dord, (size_t)ng) ||
         !ps.get(dgoff, (size_t)ng + 1) || !ps.get(dinv, (size_t)goff[(size_t)ng]) ||
         !ps.get(dcg, (size_t)ng))
         return nullptr;
+    clk.mark("device core: groups + buffers");
     // The inverses run on the second stream while this stream lays the core out on the
     // assumption that every block is kept (a block whose Cholesky fails goes back to Jacobi,
     // which changes the layout): the layout is rebuilt in the rare case that one failed.
     const bool overlap = ng > 0 && ctx->stream2 != nullptr;
     if (ng > 0) {
         const size_t lds = ((size_t)nbmax * (nbmax | 1) + 2 * (size_t)nbmax) * sizeof(double);
-        if (!hip_ok(ctx, hipMemcpyAsync(dgq0, gq0.data(), (size_t)ng * sizeof(int),
-                                        hipMemcpyHostToDevice, st), "H2D") ||
-            !hip_ok(ctx, hipMemcpyAsync(dgnb, gnb.data(), (size_t)ng * sizeof(int),
-                                        hipMemcpyHostToDevice, st), "H2D") ||
-            !hip_ok(ctx, hipMemcpyAsync(dord, gord.data(), (size_t)ng * sizeof(int),
-                                        hipMemcpyHostToDevice, st), "H2D") ||
-            !hip_ok(ctx, hipMemcpyAsync(dgoff, goff.data(), ((size_t)ng + 1) * sizeof(long long),
-                                        hipMemcpyHostToDevice, st), "H2D") ||
+        if (!h2d_async(ctx, dgq0, gq0.data(), (size_t)ng * sizeof(int), st) ||
+            !h2d_async(ctx, dgnb, gnb.data(), (size_t)ng * sizeof(int), st) ||
+            !h2d_async(ctx, dord, gord.data(), (size_t)ng * sizeof(int), st) ||
+            !h2d_async(ctx, dgoff, goff.data(), ((size_t)ng + 1) * sizeof(long long), st) ||
             !hip_ok(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_inverse),
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)lds), "hipFuncSetAttribute"))
             return nullptr;
+        clk.mark("device core: group uploads");
         // blocks of at most kSmallBlock rows need a quarter of the LDS of a 128-row block, so
         // several share a CU: they run on this stream, the large ones on the second (the
         // groups are sorted by size, the small ones last in gord)
@@ -1517,13 +1520,10 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
         if (nbk > 0) {
             const std::vector<int> *iv[] = {&kq0, &knb, &krow0, &kg};
             for (int k = 0; k < 4; k++)
-                if (!hip_ok(ctx, hipMemcpyAsync(dk[k], iv[k]->data(), (size_t)nbk * sizeof(int),
-                                                hipMemcpyHostToDevice, st), "H2D"))
+                if (!h2d_async(ctx, dk[k], iv[k]->data(), (size_t)nbk * sizeof(int), st))
                     return fail();
-            if (!hip_ok(ctx, hipMemcpyAsync(dkl[0], kgoff.data(), (size_t)nbk * sizeof(long long),
-                                            hipMemcpyHostToDevice, st), "H2D") ||
-                !hip_ok(ctx, hipMemcpyAsync(dkl[1], kboff.data(), (size_t)nbk * sizeof(long long),
-                                            hipMemcpyHostToDevice, st), "H2D"))
+            if (!h2d_async(ctx, dkl[0], kgoff.data(), (size_t)nbk * sizeof(long long), st) ||
+                !h2d_async(ctx, dkl[1], kboff.data(), (size_t)nbk * sizeof(long long), st))
                 return fail();
             hipLaunchKernelGGL(k_block_keep<true>, dim3(nbk), dim3(kLT), 0, st, dk[0], dk[1], dk[2],
                                dkl[0], dkl[1], dk[3], dinv, dcg, in_blk, iperm, z->perm, z->blk_c,
@@ -1578,10 +1578,8 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
             !up_vec(ctx, z, &z->blk_row0, H.blk_row0) || !up_vec(ctx, z, &z->blk_n, H.blk_n) ||
             !up_vec(ctx, z, &z->blk_off, H.blk_off) || !core_dev(ctx, z, &z->ell_col, ell) ||
             !core_dev(ctx, z, &z->ell_val, ell) ||
-            (nsl > 0 && (!hip_ok(ctx, hipMemcpyAsync(dsb, sl_base.data(), (size_t)nsl * sizeof(int),
-                                                     hipMemcpyHostToDevice, st), "H2D") ||
-                         !hip_ok(ctx, hipMemcpyAsync(dse, sl_end.data(), (size_t)nsl * sizeof(int),
-                                                     hipMemcpyHostToDevice, st), "H2D"))))
+            (nsl > 0 && (!h2d_async(ctx, dsb, sl_base.data(), (size_t)nsl * sizeof(int), st) ||
+                         !h2d_async(ctx, dse, sl_end.data(), (size_t)nsl * sizeof(int), st))))
             return fail();
         if (nsl > 0)
             hipLaunchKernelGGL(k_fill_ell, dim3(core_grid((long long)nsl * kWave)), dim3(kLT), 0, st,

@@ -238,6 +238,8 @@ struct cicy_ctx {
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
     cicy::PcgState *host_state = nullptr;  // pinned, 2 slots
     void *readback = nullptr;               // pinned, kReadbackBytes: small read-backs land here
+    char *upload = nullptr;                 // pinned, kUploadBytes: staging of small uploads
+    size_t upload_used = 0;                 // bytes of `upload` whose copies may be in flight
     double *bc_ws = nullptr;   // boundary-current workspace (grows, kept until destroy)
     double *scratch = nullptr; // this device's copy of the allocation-free reduction words
     size_t bc_words = 0;
@@ -306,6 +308,9 @@ bool run_chunks(cicy_sys *s, hipGraphExec_t exec, void (*direct)(cicy_sys *) = n
 // device -> host copy ordered after all work queued on the context's stream, then waited
 // for (the stream is non-blocking, so a plain hipMemcpy would not wait for it)
 bool d2h_sync(cicy_ctx *ctx, void *dst, const void *src, size_t bytes);
+// host -> device copy on stream st; on the context's stream a small one is staged in the
+// context's pinned upload buffer (src may be released on return either way)
+bool h2d_async(cicy_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t st);
 bool sys_true_residual(cicy_sys *s, const double *xdev, double *rel, double *b2 = nullptr);
 void sys_spmv64(cicy_sys *s, const double *xdev, double *ydev);  // y = A x on the stream
 bool solve_pcg64(cicy_sys *s, double tol, long long max_iter, int rel_to,

@@ -262,11 +262,31 @@ bool d2h_sync(cicy_ctx *ctx, void *dst, const void *src, size_t bytes) {
             !hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync"))
             return false;
         memcpy(dst, ctx->readback, bytes);
+        ctx->upload_used = 0;  // every staged upload on the stream has completed
         return true;
     }
-    return hip_ok(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream),
-                  "D2H") &&
-           hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (!hip_ok(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream),
+                "D2H") ||
+        !hip_ok(ctx, hipStreamSynchronize(ctx->stream), "sync"))
+        return false;
+    ctx->upload_used = 0;
+    return true;
+}
+
+// Uploads staged in pinned memory are asynchronous (a pageable source makes the runtime
+// stage it itself, which waits longer); the staging space is reused after the next wait on
+// the context's stream (d2h_sync), and an upload that does not fit goes directly.
+constexpr size_t kUploadBytes = 1 << 20;
+bool h2d_async(cicy_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return true;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (st == ctx->stream && ctx->upload && ctx->upload_used + need <= kUploadBytes) {
+        char *stage = ctx->upload + ctx->upload_used;
+        memcpy(stage, src, bytes);
+        ctx->upload_used += need;
+        return hip_ok(ctx, hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, st), "H2D");
+    }
+    return hip_ok(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), "H2D");
 }
 
 // Validates the reference's SSS arrays and builds the blocked layout.
@@ -813,9 +833,12 @@ int cicy_create(cicy_ctx **out, int device) {
         hipHostMalloc((void **)&ctx->host_state, 2 * sizeof(PcgState), hipHostMallocDefault) !=
             hipSuccess ||
         hipHostMalloc(&ctx->readback, cicy::kReadbackBytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->upload, cicy::kUploadBytes, hipHostMallocDefault) !=
+            hipSuccess ||
         !(ctx->scratch = scratch_address(device))) {
         if (ctx->host_state) (void)hipHostFree(ctx->host_state);
         if (ctx->readback) (void)hipHostFree(ctx->readback);
+        if (ctx->upload) (void)hipHostFree(ctx->upload);
         delete ctx;
         return 0;
     }
@@ -829,6 +852,7 @@ void cicy_destroy(cicy_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->host_state) (void)hipHostFree(ctx->host_state);
     if (ctx->readback) (void)hipHostFree(ctx->readback);
+    if (ctx->upload) (void)hipHostFree(ctx->upload);
     cicy::dev_release(ctx, ctx->bc_ws);
     ctx->pool.clear();
     (void)hipEventDestroy(ctx->ev_start);
