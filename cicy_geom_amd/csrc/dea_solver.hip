@@ -254,7 +254,7 @@ bool hip_ok(cicy_ctx *ctx, hipError_t e, const char *what) {
 
 // Small read-backs go through the context's pinned buffer: a copy into pageable memory is
 // staged by the runtime and waits longer.
-constexpr size_t kReadbackBytes = 64 << 10;
+constexpr size_t kReadbackBytes = 4 << 20;
 bool d2h_sync(cicy_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (ctx->readback && bytes <= kReadbackBytes) {
         if (!hip_ok(ctx, hipMemcpyAsync(ctx->readback, src, bytes, hipMemcpyDeviceToHost,
