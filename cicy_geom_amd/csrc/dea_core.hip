@@ -1063,7 +1063,9 @@ __device__ __forceinline__ void tri_index(int e, int *i, int *j) {
 // LDS: the block with an odd row stride (ld = n | 1, no bank conflicts on columns): L in
 // the lower triangle, W in the strict upper one (W[i][j] at [j][i]) and W's diagonal after
 // the block; the inverse's lower triangle then replaces L.
-__global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ order,
+constexpr int kBigInverseThreads = 1024;  // threads per block inverse of over 64 rows
+template <int T>
+__global__ __launch_bounds__(T) void k_block_inverse(const int *__restrict__ order,
                                                        const int *__restrict__ gq0,
                                                        const int *__restrict__ gnb,
                                                        const long long *__restrict__ goff,
@@ -1081,10 +1083,10 @@ __global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ o
     double *wd = A + (size_t)nb * ld;
     double *rs = wd + nb;
     double *out = inv + goff[g];
-    for (int e = tid; e < nb * ld; e += kLT) A[e] = 0.0;
+    for (int e = tid; e < nb * ld; e += T) A[e] = 0.0;
     if (tid == 0) fail = 0;
     __syncthreads();
-    for (int i = tid; i < nb; i += kLT) {
+    for (int i = tid; i < nb; i += T) {
         A[i * ld + i] = cdiag[q0 + i];
         for (int k = frp[q0 + i]; k < frp[q0 + i + 1]; k++) {
             const int c = fcol[k] - q0;
@@ -1102,10 +1104,10 @@ __global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ o
         __syncthreads();
         if (fail) break;
         const double l = A[k * ld + k];
-        for (int i = k + 1 + tid; i < nb; i += kLT) A[i * ld + k] = A[i * ld + k] / l;
+        for (int i = k + 1 + tid; i < nb; i += T) A[i * ld + k] = A[i * ld + k] / l;
         __syncthreads();
         const int r = nb - k - 1, cnt = r * (r + 1) / 2;
-        for (int e = tid; e < cnt; e += kLT) {
+        for (int e = tid; e < cnt; e += T) {
             int a, c;
             tri_index(e, &a, &c);
             const int i = k + 1 + a, j = k + 1 + c;
@@ -1121,18 +1123,18 @@ __global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ o
         return;
     }
     // W = L^-1: the strict upper triangle holds the accumulators, then W
-    for (int e = tid; e < nb * ld; e += kLT) {
+    for (int e = tid; e < nb * ld; e += T) {
         const int i = e / ld, j = e % ld;
         if (j > i && j < nb) A[e] = 0.0;
     }
     __syncthreads();
     for (int k = 0; k < nb; k++) {
         const double inv_d = 1.0 / A[k * ld + k];
-        for (int j = tid; j < k; j += kLT) A[j * ld + k] = A[j * ld + k] * inv_d;
+        for (int j = tid; j < k; j += T) A[j * ld + k] = A[j * ld + k] * inv_d;
         if (tid == 0) wd[k] = inv_d;
         __syncthreads();
         const int rows = nb - k - 1, cols = k + 1;
-        for (int e = tid; e < rows * cols; e += kLT) {
+        for (int e = tid; e < rows * cols; e += T) {
             const int i = k + 1 + e / cols, j = e % cols;
             const double wkj = j == k ? wd[k] : A[j * ld + k];
             A[j * ld + i] = A[j * ld + i] - A[i * ld + k] * wkj;
@@ -1141,7 +1143,7 @@ __global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ o
     }
     // A^-1 = W^T W into the lower triangle (L is no longer needed) and to global memory
     const int ntri = nb * (nb + 1) / 2;
-    for (int e = tid; e < ntri; e += kLT) {
+    for (int e = tid; e < ntri; e += T) {
         int i, j;
         tri_index(e, &i, &j);
         double acc = 0.0;
@@ -1155,7 +1157,7 @@ __global__ __launch_bounds__(kLT) void k_block_inverse(const int *__restrict__ o
         out[(size_t)j * nb + i] = acc;
     }
     __syncthreads();
-    for (int i = tid; i < nb; i += kLT) {
+    for (int i = tid; i < nb; i += T) {
         double t = 0.0;
         for (int j = 0; j < nb; j++) t = t + (j <= i ? A[i * ld + j] : A[j * ld + i]);
         rs[i] = t;
@@ -1423,9 +1425,13 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
             !h2d_async(ctx, dgnb, gnb.data(), (size_t)ng * sizeof(int), st) ||
             !h2d_async(ctx, dord, gord.data(), (size_t)ng * sizeof(int), st) ||
             !h2d_async(ctx, dgoff, goff.data(), ((size_t)ng + 1) * sizeof(long long), st) ||
-            !hip_ok(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_inverse),
+            !hip_ok(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_inverse<kLT>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)lds), "hipFuncSetAttribute"))
+                                             (int)lds), "hipFuncSetAttribute") ||
+            !hip_ok(ctx, hipFuncSetAttribute(
+                             reinterpret_cast<const void *>(k_block_inverse<kBigInverseThreads>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                    "hipFuncSetAttribute"))
             return nullptr;
         clk.mark("device core: group uploads");
         // blocks of at most kSmallBlock rows need a quarter of the LDS of a 128-row block, so
@@ -1440,11 +1446,14 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
             if (!hip_ok(ctx, hipEventRecord(ctx->ev_fork, st), "event") ||
                 !hip_ok(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0), "wait"))
                 return nullptr;
+            // a large block's inverse is a chain of barrier-separated steps: more threads per
+            // block shorten each step (every entry still gets the same operations)
             if (nbig > 0)
-                hipLaunchKernelGGL(k_block_inverse, dim3(nbig), dim3(kLT), lds, ctx->stream2, dord,
-                                   dgq0, dgnb, dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
+                hipLaunchKernelGGL(k_block_inverse<kBigInverseThreads>, dim3(nbig),
+                                   dim3(kBigInverseThreads), lds, ctx->stream2, dord, dgq0, dgnb,
+                                   dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
             if (nsmall > 0)
-                hipLaunchKernelGGL(k_block_inverse, dim3(nsmall), dim3(kLT), lds_small, st,
+                hipLaunchKernelGGL(k_block_inverse<kLT>, dim3(nsmall), dim3(kLT), lds_small, st,
                                    dord + nbig, dgq0, dgnb, dgoff, frp, fcol, fval, cdiag, dinv,
                                    dok, dcg);
             if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
@@ -1452,7 +1461,7 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
                 return nullptr;
             std::fill(okh.begin(), okh.end(), 1);
         } else {
-            hipLaunchKernelGGL(k_block_inverse, dim3(ng), dim3(kLT), lds, st, dord, dgq0, dgnb,
+            hipLaunchKernelGGL(k_block_inverse<kLT>, dim3(ng), dim3(kLT), lds, st, dord, dgq0, dgnb,
                                dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
             if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
                 !d2h_sync(ctx, okh.data(), dok, (size_t)ng * sizeof(int)))
