@@ -1435,7 +1435,7 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
             return nullptr;
         clk.mark("device core: group uploads");
         // blocks of at most kSmallBlock rows need a quarter of the LDS of a 128-row block, so
-        // several share a CU: they run on this stream, the large ones on the second (the
+        // several share a CU: they run in a launch of their own after the large ones (the
         // groups are sorted by size, the small ones last in gord)
         int nbig = 0;
         while (nbig < ng && gnb[(size_t)gord[(size_t)nbig]] > kSmallBlock) nbig++;
@@ -1452,10 +1452,12 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
                 hipLaunchKernelGGL(k_block_inverse<kBigInverseThreads>, dim3(nbig),
                                    dim3(kBigInverseThreads), lds, ctx->stream2, dord, dgq0, dgnb,
                                    dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
+            // the small blocks follow on the second stream too, so this stream goes straight to
+            // the layout
             if (nsmall > 0)
-                hipLaunchKernelGGL(k_block_inverse<kLT>, dim3(nsmall), dim3(kLT), lds_small, st,
-                                   dord + nbig, dgq0, dgnb, dgoff, frp, fcol, fval, cdiag, dinv,
-                                   dok, dcg);
+                hipLaunchKernelGGL(k_block_inverse<kLT>, dim3(nsmall), dim3(kLT), lds_small,
+                                   ctx->stream2, dord + nbig, dgq0, dgnb, dgoff, frp, fcol, fval,
+                                   cdiag, dinv, dok, dcg);
             if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
                 !hip_ok(ctx, hipEventRecord(ctx->ev_join, ctx->stream2), "event"))
                 return nullptr;
