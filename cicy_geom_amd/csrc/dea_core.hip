@@ -70,7 +70,6 @@ constexpr int kCoreThreads = 1024;
 constexpr int kCoreWaves = kCoreThreads / kWave;
 constexpr int kCorePairs = kCoreWaves / 2;  // blocks per round: two waves share one block
 constexpr int kCoreMaxBlock = 128;     // rows of a dense block (larger groups are split)
-constexpr int kSmallBlock = 64;        // blocks inverted beside the large ones (second stream)
 // the split size in effect: kCoreMaxBlock, or CICY_CORE_MAX_BLOCK (16..kCoreMaxBlock) when set
 // (a tuning knob; host and device layouts read the same value)
 inline int core_max_block() {
@@ -1063,7 +1062,7 @@ __device__ __forceinline__ void tri_index(int e, int *i, int *j) {
 // LDS: the block with an odd row stride (ld = n | 1, no bank conflicts on columns): L in
 // the lower triangle, W in the strict upper one (W[i][j] at [j][i]) and W's diagonal after
 // the block; the inverse's lower triangle then replaces L.
-constexpr int kBigInverseThreads = 1024;  // threads per block inverse of over 64 rows
+constexpr int kBigInverseThreads = 1024;  // threads per block inverse beside the layout
 template <int T>
 __global__ __launch_bounds__(T) void k_block_inverse(const int *__restrict__ order,
                                                        const int *__restrict__ gq0,
@@ -1434,30 +1433,17 @@ cicy_core *core_setup_device(cicy_ctx *ctx, int m, const int *frp, const int *fc
                     "hipFuncSetAttribute"))
             return nullptr;
         clk.mark("device core: group uploads");
-        // blocks of at most kSmallBlock rows need a quarter of the LDS of a 128-row block, so
-        // several share a CU: they run in a launch of their own after the large ones (the
-        // groups are sorted by size, the small ones last in gord)
-        int nbig = 0;
-        while (nbig < ng && gnb[(size_t)gord[(size_t)nbig]] > kSmallBlock) nbig++;
-        const int nsmall = ng - nbig;
-        const size_t lds_small = ((size_t)kSmallBlock * (kSmallBlock | 1) + 2 * kSmallBlock) *
-                                 sizeof(double);
         if (overlap) {
             if (!hip_ok(ctx, hipEventRecord(ctx->ev_fork, st), "event") ||
                 !hip_ok(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0), "wait"))
                 return nullptr;
-            // a large block's inverse is a chain of barrier-separated steps: more threads per
-            // block shorten each step (every entry still gets the same operations)
-            if (nbig > 0)
-                hipLaunchKernelGGL(k_block_inverse<kBigInverseThreads>, dim3(nbig),
-                                   dim3(kBigInverseThreads), lds, ctx->stream2, dord, dgq0, dgnb,
-                                   dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
-            // the small blocks follow on the second stream too, so this stream goes straight to
-            // the layout
-            if (nsmall > 0)
-                hipLaunchKernelGGL(k_block_inverse<kLT>, dim3(nsmall), dim3(kLT), lds_small,
-                                   ctx->stream2, dord + nbig, dgq0, dgnb, dgoff, frp, fcol, fval,
-                                   cdiag, dinv, dok, dcg);
+            // all the inverses on the second stream, largest first, so this stream goes
+            // straight to the layout; a block's inverse is a chain of barrier-separated steps,
+            // and more threads per block shorten each step (every entry still gets the same
+            // operations), while the small blocks fill the CUs the large ones leave
+            hipLaunchKernelGGL(k_block_inverse<kBigInverseThreads>, dim3(ng),
+                               dim3(kBigInverseThreads), lds, ctx->stream2, dord, dgq0, dgnb,
+                               dgoff, frp, fcol, fval, cdiag, dinv, dok, dcg);
             if (!hip_ok(ctx, hipGetLastError(), "k_block_inverse") ||
                 !hip_ok(ctx, hipEventRecord(ctx->ev_join, ctx->stream2), "event"))
                 return nullptr;
